@@ -1,0 +1,68 @@
+"""Shared test helpers: run a parity case through the oracle in the exact
+call sequence of oracle/ref_driver.cpp and compare against the fixtures."""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def load_golden(name: str):
+    with open(os.path.join(GOLDEN, f"{name}.json")) as f:
+        meta = json.load(f)
+    full = dict(np.load(os.path.join(GOLDEN, f"{name}.npz")))
+    return meta, full
+
+
+def run_oracle_case(case):
+    """Every array ref_driver.cpp dumps, computed by the oracle."""
+    from oracle.oracle import Oracle
+
+    o = Oracle(case.blocks, case.isym, case.nfringe, case.mexclude)
+    out = {}
+    try:
+        for s in range(len(case.moves) + 1):
+            if s > 0:
+                for bi, xb in enumerate(case.moves[s - 1]):
+                    if xb is not None:
+                        o.set_coords(bi, xb)
+            o.profile()
+            o.connect()
+            pre = "" if s == 0 else f"s{s}."
+            for k, v in o.results(case.qs, case.nvar, case.interptype).items():
+                out[pre + k] = v
+        out["weight_aborts"] = o.L.orc_weight_aborts()
+    finally:
+        o.close()
+    return out
+
+
+def compare_to_golden(name: str, got: dict, skip=()):
+    """Returns a list of mismatch descriptions (empty when bit-exact)."""
+    meta, full = load_golden(name)
+    bad = []
+    for k, m in meta.items():
+        if any(k.endswith(s) for s in skip):
+            continue
+        if k not in got:
+            bad.append(f"{k}: missing")
+            continue
+        a = np.ascontiguousarray(got[k]).astype(m["dtype"])
+        if a.size != m["size"]:
+            bad.append(f"{k}: size {a.size} != {m['size']}")
+            continue
+        if digest(a) != m["sha256"]:
+            if k in full:
+                d = np.nonzero(full[k] != a)[0]
+                bad.append(f"{k}: {d.size} of {a.size} differ, first at {d[:4]}")
+            else:
+                bad.append(f"{k}: digest differs")
+    return bad

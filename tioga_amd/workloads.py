@@ -1,0 +1,269 @@
+"""Seeded synthetic overset meshes for the connectivity workloads.
+
+Every generator returns :class:`Block` objects in TIOGA's registration
+layout (tioga.h:121-123): FP64 AoS coordinates, 1-based connectivity per cell
+type (codetypes.h:36 ``BASE 1``), 1-based wall / overset-boundary node lists.
+
+Workloads (sizes as stated; all coordinates FP64):
+
+* ``config1``  background 48^3 hex cube [-1,1]^3 (tag 2) + lat-long spherical
+  hex shell, r 0.25..0.6, 64 azimuth x 32 polar x 24 radial cells, radial
+  stretching 1.1 (tag 1).  Pole nodes are kept per azimuth (coincident
+  copies), which exercises the coincident-query dedupe.
+* ``config2``  background 256^3 hex cube with interior nodes jittered by
+  uniform +-10% of the spacing (tag 2) + a mixed-element shell (about 3.9M
+  cells: 40% tet, 30% prism, 5% pyramid, 25% hex by cell count) between an
+  ellipsoid wall with semi-axes (0.3, 0.2, 0.2) and a sphere of radius 0.6
+  (tag 1).  Poles are merged, so pole rows are prisms.
+* ``tiny``     the config-1 construction at a size the oracle finishes in a
+  fraction of a second (parity cases).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+
+@dataclass
+class Block:
+    tag: int
+    x: np.ndarray                      # (nnodes, 3) float64
+    types: List[Tuple[int, np.ndarray]]  # [(nv, conn (nc, nv) int32, 1-based)]
+    wbc: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    obc: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    nres: Optional[np.ndarray] = None
+    cres: Optional[np.ndarray] = None
+
+    @property
+    def nnodes(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def ncells(self) -> int:
+        return int(sum(c.shape[0] for _, c in self.types))
+
+    def normalized(self) -> "Block":
+        """Contiguous arrays of the exact dtypes the C ABI binds."""
+        return Block(
+            self.tag,
+            np.ascontiguousarray(self.x, dtype=np.float64),
+            [(int(nv), np.ascontiguousarray(c, dtype=np.int32)) for nv, c in self.types],
+            np.ascontiguousarray(self.wbc, dtype=np.int32),
+            np.ascontiguousarray(self.obc, dtype=np.int32),
+            None if self.nres is None else np.ascontiguousarray(self.nres, dtype=np.float64),
+            None if self.cres is None else np.ascontiguousarray(self.cres, dtype=np.float64),
+        )
+
+
+def signed_volume(x: np.ndarray, nv: int, conn: np.ndarray) -> np.ndarray:
+    """Cell volume by the divergence theorem, the face split of
+    cellVolume.f90:67-91 (vectorised; used only to check orientation)."""
+    faces = {
+        4: [(1, 2, 3), (1, 4, 2), (2, 4, 3), (1, 3, 4)],
+        5: [(1, 2, 3, 4), (1, 5, 2), (2, 5, 3), (4, 3, 5), (1, 4, 5)],
+        6: [(1, 2, 3), (1, 4, 5, 2), (2, 5, 6, 3), (1, 3, 6, 4), (4, 6, 5)],
+        8: [(1, 2, 3, 4), (1, 5, 6, 2), (2, 6, 7, 3), (3, 7, 8, 4), (1, 4, 8, 5), (5, 8, 7, 6)],
+    }[nv]
+    v = x[conn - 1]
+
+    def bp(a, b, c):
+        return np.einsum("ij,ij->i", a, np.cross(b, c))
+
+    vol = np.zeros(conn.shape[0])
+    for f in faces:
+        p = [v[:, k - 1] for k in f]
+        if len(f) == 3:
+            vol -= 0.5 * bp(p[0], p[1], p[2])
+        else:
+            vol -= 0.25 * (bp(p[0], p[1], p[2]) + bp(p[0], p[2], p[3]) + bp(p[0], p[1], p[3]) + bp(p[1], p[2], p[3]))
+    return vol / 3.0
+
+
+def cart_box(n: int, tag: int, lo: float = -1.0, hi: float = 1.0, jitter: float = 0.0,
+             seed: int = 2) -> Block:
+    """n^3 hex cells on [lo,hi]^3; node id 1 + i + (n+1)(j + (n+1)k); hex
+    vertices (i,j,k),(i+1,j,k),(i+1,j+1,k),(i,j+1,k) then the same at k+1
+    (the vertex order of poisson_utils.f90:77-88).  Interior nodes are moved
+    by an independent uniform +-jitter*h per coordinate."""
+    m = n + 1
+    g = lo + (hi - lo) * np.arange(m, dtype=np.float64) / n
+    k, j, i = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    x = np.empty((m * m * m, 3))
+    x[:, 0] = g[i.ravel()]
+    x[:, 1] = g[j.ravel()]
+    x[:, 2] = g[k.ravel()]
+    del k, j, i
+    if jitter > 0.0:
+        h = (hi - lo) / n
+        rng = np.random.default_rng(seed)
+        idx = np.arange(m)
+        inner = (idx > 0) & (idx < n)
+        mask = (inner[:, None, None] & inner[None, :, None] & inner[None, None, :]).ravel()
+        d = rng.uniform(-jitter * h, jitter * h, size=(int(mask.sum()), 3))
+        x[mask] += d
+    c = np.arange(n)
+    kk, jj, ii = np.meshgrid(c, c, c, indexing="ij")
+    base = (ii + m * (jj + m * kk)).ravel().astype(np.int64)
+    del kk, jj, ii
+    off = np.array([0, 1, 1 + m, m, m * m, 1 + m * m, 1 + m + m * m, m + m * m], np.int64)
+    conn = (base[:, None] + off[None, :] + 1).astype(np.int32)
+    return Block(tag, x, [(8, conn)])
+
+
+def _radii(r0: float, r1: float, nr: int, stretch: float) -> np.ndarray:
+    k = np.arange(nr + 1, dtype=np.float64)
+    if stretch == 1.0:
+        return r0 + (r1 - r0) * k / nr
+    dr0 = (r1 - r0) * (stretch - 1.0) / (stretch ** nr - 1.0)
+    return r0 + dr0 * (stretch ** k - 1.0) / (stretch - 1.0)
+
+
+def latlong_shell(nphi: int, ntheta: int, nr: int, r0: float, r1: float, stretch: float, tag: int,
+                  center=(0.0, 0.0, 0.0)) -> Block:
+    """Hex shell, phi_i = 2 pi i / nphi (seam merged), theta_j = pi j / ntheta,
+    r_k geometric from r0 to r1.  Pole nodes are kept per azimuth.  Node id
+    1 + i + nphi (j + (ntheta+1) k).  Cell vertex order runs (r, theta, phi),
+    a right-handed frame, so cell volumes are positive.  wbc = k=0 nodes,
+    obc = k=nr nodes."""
+    r = _radii(r0, r1, nr, stretch)
+    th = np.pi * np.arange(ntheta + 1) / ntheta
+    ph = 2.0 * np.pi * np.arange(nphi) / nphi
+    K, J, I = np.meshgrid(np.arange(nr + 1), np.arange(ntheta + 1), np.arange(nphi), indexing="ij")
+    rr, tt, pp = r[K.ravel()], th[J.ravel()], ph[I.ravel()]
+    x = np.stack([rr * np.sin(tt) * np.cos(pp), rr * np.sin(tt) * np.sin(pp), rr * np.cos(tt)], axis=1)
+    x += np.asarray(center, np.float64)[None, :]
+
+    def nid(i, j, k):
+        return 1 + (i % nphi) + nphi * (j + (ntheta + 1) * k)
+
+    K, J, I = np.meshgrid(np.arange(nr), np.arange(ntheta), np.arange(nphi), indexing="ij")
+    k, j, i = K.ravel(), J.ravel(), I.ravel()
+    conn = np.stack([nid(i, j, k), nid(i, j, k + 1), nid(i, j + 1, k + 1), nid(i, j + 1, k),
+                     nid(i + 1, j, k), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1), nid(i + 1, j + 1, k)],
+                    axis=1).astype(np.int32)
+    nlayer = nphi * (ntheta + 1)
+    wbc = (np.arange(nlayer) + 1).astype(np.int32)
+    obc = (np.arange(nlayer) + 1 + nr * nlayer).astype(np.int32)
+    return Block(tag, x, [(8, conn)], wbc, obc)
+
+
+# Kuhn split of a hex (local vertices 0..7 in TIOGA order) into 6 positive
+# tets sharing the 0-6 diagonal; the split is face-conforming between
+# neighbours of one structured grid.
+_KUHN = [(0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6)]
+# hex -> 2 prisms cut through the 0-2 / 4-6 diagonal plane
+_PRISMS = [(0, 1, 2, 4, 5, 6), (0, 2, 3, 4, 6, 7)]
+# hex -> 3 pyramids with apex at vertex 0 on the faces not containing it
+_PYRS = [(1, 5, 6, 2, 0), (2, 6, 7, 3, 0), (4, 7, 6, 5, 0)]
+
+
+def _orient(x: np.ndarray, nv: int, conn: np.ndarray) -> np.ndarray:
+    """Reorder cells with negative volume so every cell is positively oriented."""
+    vol = signed_volume(x, nv, conn)
+    neg = vol < 0
+    if neg.any():
+        c = conn.copy()
+        if nv == 4:
+            c[neg] = c[neg][:, [1, 0, 2, 3]]
+        elif nv == 5:
+            c[neg] = c[neg][:, [0, 3, 2, 1, 4]]
+        elif nv == 6:
+            c[neg] = c[neg][:, [3, 4, 5, 0, 1, 2]]
+        else:
+            c[neg] = c[neg][:, [4, 5, 6, 7, 0, 1, 2, 3]]
+        conn = c
+    return conn
+
+
+def mixed_shell(nphi: int, ntheta: int, layers=(15, 9, 1, 4), tag: int = 1,
+                wall_axes=(0.3, 0.2, 0.2), r_outer: float = 0.6, stretch: float = 1.08) -> Block:
+    """Mixed-element shell from an ellipsoid wall to a sphere.  Radial
+    layers, wall outward: hex, prism (2 per hex), pyramid (3 per hex), tet (6
+    per hex, Kuhn).  Poles are merged, so the two polar rows of every layer are
+    prisms.  With layers (15, 9, 1, 4) the cell counts are 25/30/5/40 % hex/
+    prism/pyramid/tet."""
+    lh, lp, ly, lt = layers
+    nr = lh + lp + ly + lt
+    t = _radii(0.0, 1.0, nr, stretch)
+    th = np.pi * np.arange(1, ntheta) / ntheta           # interior rings only
+    ph = 2.0 * np.pi * np.arange(nphi) / nphi
+    a, b, c = wall_axes
+    # ring nodes per layer: (ntheta-1)*nphi, plus 2 poles
+    TT, PP = np.meshgrid(th, ph, indexing="ij")
+    dirs = np.stack([np.sin(TT) * np.cos(PP), np.sin(TT) * np.sin(PP), np.cos(TT)], -1).reshape(-1, 3)
+    dirs = np.concatenate([dirs, [[0.0, 0.0, 1.0], [0.0, 0.0, -1.0]]], 0)
+    inner = dirs * np.array([a, b, c])[None, :]
+    outer = dirs * r_outer
+    nl = dirs.shape[0]
+    x = (inner[None, :, :] * (1.0 - t)[:, None, None] + outer[None, :, :] * t[:, None, None]).reshape(-1, 3)
+    north, south = nl - 2, nl - 1
+
+    def nid(i, j, k):
+        # j in 0..ntheta (0 = north pole, ntheta = south pole)
+        ring = np.where(j == 0, north, np.where(j == ntheta, south, (j - 1) * nphi + (i % nphi)))
+        return 1 + ring + nl * k
+
+    out = {4: [], 5: [], 6: [], 8: []}
+    for k in range(nr):
+        J, I = np.meshgrid(np.arange(ntheta), np.arange(nphi), indexing="ij")
+        j, i = J.ravel(), I.ravel()
+        kk = np.full_like(j, k)
+        hexv = np.stack([nid(i, j, kk), nid(i, j, kk + 1), nid(i, j + 1, kk + 1), nid(i, j + 1, kk),
+                         nid(i + 1, j, kk), nid(i + 1, j, kk + 1), nid(i + 1, j + 1, kk + 1),
+                         nid(i + 1, j + 1, kk)], 1)
+        pole = (j == 0) | (j == ntheta - 1)
+        # polar rows: the hex has a collapsed edge -> prism with distinct nodes
+        pn = hexv[pole & (j == 0)]
+        if pn.size:   # north row: hex vertices 0,4 and 1,5 coincide (j = 0)
+            out[6].append(pn[:, [0, 3, 7, 1, 2, 6]])
+        ps = hexv[pole & (j == ntheta - 1)]
+        if ps.size:   # south row: vertices 3 and 7 coincide (j+1 = ntheta)
+            out[6].append(ps[:, [0, 3, 4, 1, 2, 5]])
+        hv = hexv[~pole]
+        if k < lh:
+            out[8].append(hv)
+        elif k < lh + lp:
+            out[6].extend(hv[:, list(p)] for p in _PRISMS)
+        elif k < lh + lp + ly:
+            out[5].extend(hv[:, list(p)] for p in _PYRS)
+        else:
+            out[4].extend(hv[:, list(p)] for p in _KUHN)
+    types = []
+    for nv in (4, 5, 6, 8):
+        if out[nv]:
+            conn = np.concatenate(out[nv], 0).astype(np.int32)
+            types.append((nv, _orient(x, nv, conn)))
+    wbc = (np.arange(nl) + 1).astype(np.int32)
+    obc = (np.arange(nl) + 1 + nr * nl).astype(np.int32)
+    return Block(tag, x, types, wbc, obc)
+
+
+def fields(nnodes: int, nvar: int, seed: int) -> np.ndarray:
+    """nvar FP64 nodal fields, uniform[-1,1], row layout (node-major)."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1.0, 1.0, size=nnodes * nvar)
+
+
+def config1() -> List[Block]:
+    return [latlong_shell(64, 32, 24, 0.25, 0.6, 1.1, tag=1), cart_box(48, tag=2)]
+
+
+def tiny(nbox: int = 10, nphi: int = 16, ntheta: int = 8, nr: int = 6) -> List[Block]:
+    return [latlong_shell(nphi, ntheta, nr, 0.25, 0.6, 1.1, tag=1), cart_box(nbox, tag=2)]
+
+
+def config2(nbox: int = 256, nphi: int = 360, ntheta: int = 180) -> List[Block]:
+    return [mixed_shell(nphi, ntheta, tag=1), cart_box(nbox, tag=2, jitter=0.1, seed=2)]
+
+
+def rigid_motion(x: np.ndarray, angle_deg: float, axis, shift) -> np.ndarray:
+    """Rotate about the centroid by angle_deg around axis, then translate."""
+    axis = np.asarray(axis, np.float64)
+    axis = axis / np.linalg.norm(axis)
+    th = np.deg2rad(angle_deg)
+    kx = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    R = np.eye(3) + np.sin(th) * kx + (1 - np.cos(th)) * kx @ kx
+    c = x.mean(axis=0)
+    return (x - c) @ R.T + c + np.asarray(shift, np.float64)
