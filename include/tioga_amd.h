@@ -1,0 +1,136 @@
+/*
+ * tioga_amd.h -- C ABI of the MI355X overset-connectivity engine
+ * (libtioga_amd.so).
+ *
+ * The tioga_* entry points are the ones TIOGA's Fortran/C interface exports,
+ * with the same names, argument meaning (every argument by pointer, 1-based
+ * block ids and node lists, mesh tags >= 1) and call order
+ * (init -> register -> [set*] -> preprocess -> performconnectivity ->
+ * registersolution -> dataupdate -> [getdonor*] -> delete).  Each
+ * declaration cites the reference function it replaces in
+ * src/tiogaInterface.C.  The engine runs one rank's share of the work on the
+ * current HIP device; the communicator argument is accepted and ignored, and
+ * a job with more than one rank is rejected.
+ *
+ * The reference reports errors by printf and MPI_Abort; these entry points
+ * keep its void signatures, print the error to stderr and record it, and
+ * tioga_amd_status() returns it (0 = no error since the last reset).
+ *
+ * The tioga_amd_* entry points are extensions with no counterpart in the
+ * reference: status, inspection of internal arrays, and device timing.
+ */
+#ifndef TIOGA_AMD_H
+#define TIOGA_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* MPICH's MPI_Comm is an int; the handle is not used (one rank). */
+typedef int tioga_comm_t;
+
+/* tiogaInterface.C:36-59 */
+void tioga_init_f90_(int *scomm);
+/* tiogaInterface.C:61-82 */
+void tioga_init_(tioga_comm_t tcomm);
+/* tiogaInterface.C:84-106: per cell type, the variadic arguments are
+   (int *nv, int *nc, int *conn) with conn 1-based, nv*nc entries. */
+void tioga_registergrid_data_(int *btag, int *nnodes, double *xyz, int *ibl, int *nwbc, int *nobc,
+                              int *wbcnode, int *obcnode, int *ntypes, ...);
+/* tiogaInterface.C:108-130 (bid is 1-based, at most 100 blocks) */
+void tioga_registergrid_data_mb_(int *bid, int *btag, int *nnodes, double *xyz, int *ibl, int *nwbc,
+                                 int *nobc, int *wbcnode, int *obcnode, int *ntypes, ...);
+/* tiogaInterface.C:150-153 */
+void tioga_preprocess_grids_(void);
+/* tiogaInterface.C:155-158 */
+void tioga_performconnectivity_(void);
+/* tiogaInterface.C:170-173 */
+void tioga_registersolution_(int *bid, double *q);
+/* tiogaInterface.C:175-216: itype contains "row" (q[node*nvar+k]) or
+   "column" (q[k*nnodes+node]) */
+void tioga_dataupdate_mb_(int *nvar, char *itype);
+/* tiogaInterface.C:218-224 (registers q under tag 0, as the reference does) */
+void tioga_dataupdate_(double *q, int *nvar, char *itype);
+/* tiogaInterface.C:226-243 (Tecplot output is not produced; prints a note) */
+void tioga_writeoutputfiles_(int *nvar, char *itype);
+/* tiogaInterface.C:244-247 */
+void tioga_getdonorcount_(int *btag, int *dcount, int *fcount);
+/* tiogaInterface.C:248-251: receptors[4*dcount] = (rank, node, block, nw);
+   indices/frac hold nw+1 entries per receptor, the last = donor cell, 0.0 */
+void tioga_getdonorinfo_(int *btag, int *receptors, int *indices, double *frac, int *dcount);
+/* tiogaInterface.C:253-256 */
+void tioga_setsymmetry_(int *isym);
+/* tiogaInterface.C:258-261 (first registered block) */
+void tioga_setresolutions_(double *nres, double *cres);
+/* tiogaInterface.C:263-266 */
+void tioga_setresolutions_multi_(int *btag, double *nres, double *cres);
+/* tiogaInterface.C:268-271 (first registered block) */
+void tioga_setcelliblank_(int *iblank_cell);
+/* tiogaInterface.C:302-305 */
+void tioga_reduce_fringes_(void);
+/* tiogaInterface.C:307-310 */
+void tioga_setnfringe_(int *nfringe);
+/* tiogaInterface.C:312-315 */
+void tioga_setmexclude_(int *mexclude);
+/* tiogaInterface.C:317-328 */
+void tioga_delete_(void);
+
+/* Out of scope of this engine (AMR Cartesian connectivity, high-order and
+   p4est callbacks); they link, print that the feature is not available and
+   set the error status.  tiogaInterface.C:132-148, 160-168, 273-300. */
+void tioga_register_amr_global_data_(int *nf, int *qstride, double *qnodein, int *idata, double *rdata,
+                                     int *ngridsin, int *qnodesize);
+void tioga_register_amr_patch_count_(int *npatches);
+void tioga_register_amr_local_data_(int *ipatch, int *global_id, int *iblank, double *q);
+void tioga_performconnectivity_highorder_(void);
+void tioga_performconnectivity_amr_(void);
+void tioga_set_highorder_callback_(void (*f1)(int *, int *), void (*f2)(int *, int *, double *),
+                                   void (*f3)(int *, double *, int *, double *),
+                                   void (*f4)(int *, double *, int *, int *, double *, double *, int *),
+                                   void (*f5)(int *, int *, double *, int *, int *, double *));
+void tioga_set_p4est_(void);
+void tioga_set_amr_callback_(void (*f1)(int *, double *, int *, double *));
+void tioga_set_p4est_search_callback_(void (*f1)(double *xsearch, int *process_id, int *cell_id, int *npts),
+                                      void (*f2)(int *pid, int *iflag));
+
+/* ---------------- extensions (no counterpart in the reference) ---------------- */
+
+/* Error status of the global instance: 0 when no call has failed since the
+   last tioga_amd_clear_status(); the message is copied into msg (len bytes). */
+int tioga_amd_status(char *msg, int len);
+void tioga_amd_clear_status(void);
+
+/* Internal per-block arrays under the reference's member names ("nodeRes",
+   "cellRes", "obb", "xsearch", "res_search", "isearch", "tagsearch",
+   "donorId", "xtag", "elementList", "adtIntegers", "adtReals",
+   "adtExtents", "iblank_cell").  Returns the element count (-1: unknown
+   name or tag) and copies into out when out != NULL. */
+long tioga_amd_block_array(int btag, const char *name, void *out);
+
+/* Hole map of mesh tag mtag as {0,1} ints (1 = inside the hole); returns
+   nx*ny*nz, or 0 when the mesh has no wall. */
+long tioga_amd_hole_map(int mtag, int *nx, double *ext, int *sam);
+
+/* Device time (ms per pass, HIP events on the engine's stream) of nrep
+   re-runs of the donor-search kernels over the resident search structures of
+   the last performconnectivity; *nsearch = donor searches per pass. */
+double tioga_amd_time_search(int nrep, long long *nsearch);
+
+/* Device time (ms per pass) of nrep dataUpdate gather/scatter passes with q
+   resident in HBM (uploaded once from the registered arrays); *nreceptors =
+   interpolation records per pass. */
+double tioga_amd_time_dataupdate(int nvar, int interptype, int nrep, long long *nreceptors);
+
+/* Host wall times (ms) of the last performconnectivity phases:
+   out[0..6] = hole map, boxes, query gathering, search preparation,
+   search kernels (device), donor selection, total; out[7] = searches. */
+void tioga_amd_phase_times(double *out);
+
+/* Donor weights outside [-0.2, 1.2] seen (the reference aborts on one). */
+int tioga_amd_weight_aborts(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

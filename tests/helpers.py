@@ -66,3 +66,30 @@ def compare_to_golden(name: str, got: dict, skip=()):
             else:
                 bad.append(f"{k}: digest differs")
     return bad
+
+
+def run_engine_case(case):
+    """Every array ref_driver.cpp dumps, computed by the HIP engine through
+    its C ABI, in the driver's call sequence."""
+    from tioga_amd.tioga import Tioga
+
+    t = Tioga()
+    out = {}
+    try:
+        t.set_params(case.isym, case.nfringe, case.mexclude)
+        for b in case.blocks:
+            t.register_grid(b)
+        for s in range(len(case.moves) + 1):
+            if s > 0:
+                for bi, xb in enumerate(case.moves[s - 1]):
+                    if xb is not None:
+                        t.set_coords(bi, xb)
+            t.preprocess()
+            t.connect()
+            pre = "" if s == 0 else f"s{s}."
+            for k, v in t.results(case.qs, case.nvar, case.interptype).items():
+                out[pre + k] = v
+        out["weight_aborts"] = t.weight_aborts()
+    finally:
+        t.close()
+    return out

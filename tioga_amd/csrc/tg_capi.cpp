@@ -1,0 +1,373 @@
+// tg_capi.cpp -- the TIOGA::tioga class and the tioga_* C ABI over the
+// engine (include/tioga.h, include/tioga_amd.h).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <exception>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/tioga.h"
+#include "../../include/tioga_amd.h"
+#include "tg_engine.h"
+
+namespace {
+
+std::string g_err;
+int g_status = 0;
+
+void set_error(const char *where, const char *what)
+{
+  g_status = 1;
+  g_err = std::string(where) + ": " + what;
+  fprintf(stderr, "tioga_amd: %s\n", g_err.c_str());
+}
+
+template <class F>
+void guarded(const char *where, F &&f)
+{
+  try {
+    f();
+  } catch (const std::exception &e) {
+    set_error(where, e.what());
+  }
+}
+
+}  // namespace
+
+namespace TIOGA {
+
+tioga::tioga() : mexclude(3), nfringe(1), eng_(new tg::Engine) {}
+
+tioga::~tioga() { delete eng_; }
+
+void tioga::registerGridData(int btag, int nnodes, double *xyz, int *ibl, int nwbc, int nobc, int *wbcnode,
+                             int *obcnode, int ntypes, int *nv, int *nc, int **vconn, uint64_t *cell_gid,
+                             uint64_t *node_gid)
+{
+  eng_->register_grid(btag, nnodes, xyz, ibl, nwbc, nobc, wbcnode, obcnode, ntypes, nv, nc, vconn);
+  eng_->set_gids(btag, cell_gid, node_gid);
+}
+
+void tioga::registerSolution(int btag, double *q) { eng_->register_solution(btag, q); }
+
+void tioga::profile(void)
+{
+  eng_->mexclude = mexclude;
+  eng_->nfringe = nfringe;
+  eng_->profile();
+}
+
+void tioga::performConnectivity(void) { eng_->perform_connectivity(); }
+
+void tioga::dataUpdate(int nvar, int interptype, int at_points)
+{
+  if (at_points) throw std::runtime_error("tioga_amd: dataUpdate at_points=1 (high-order path) is not available");
+  eng_->data_update(nvar, interptype);
+}
+
+void tioga::getDonorCount(int btag, int *dcount, int *fcount) { eng_->get_donor_count(btag, dcount, fcount); }
+
+void tioga::getDonorInfo(int btag, int *receptors, int *indices, double *frac, int *dcount)
+{
+  eng_->get_donor_info(btag, receptors, indices, frac, dcount);
+}
+
+void tioga::getReceptorInfo(std::vector<int> &receptors) { eng_->get_receptor_info(receptors); }
+
+void tioga::setSymmetry(int syminput) { eng_->isym = syminput; }
+
+void tioga::setResolutions(double *nres, double *cres) { eng_->set_resolutions_first(nres, cres); }
+
+void tioga::setResolutions(int btag, double *nres, double *cres) { eng_->set_resolutions(btag, nres, cres); }
+
+void tioga::setMexclude(int *m) { mexclude = *m; }
+
+void tioga::setNfringe(int *n) { nfringe = *n; }
+
+void tioga::set_cell_iblank(int *iblank_cell) { eng_->set_cell_iblank_first(iblank_cell); }
+
+void tioga::set_cell_iblank(int btag, int *ib_cell) { eng_->set_cell_iblank(btag, ib_cell); }
+
+void tioga::set_uniform_hex_flag(int, int) {}
+
+void tioga::reduce_fringes(void)
+{
+  throw std::runtime_error("tioga_amd: reduce_fringes is not available in this engine");
+}
+
+}  // namespace TIOGA
+
+// ---------------------------------------------------------------------------
+// C ABI (tiogaInterface.C)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+TIOGA::tioga *g_tg = nullptr;
+constexpr int MAXBLOCKS = 100;  // globals.h:30
+struct InpData {
+  std::vector<int> nv, nc;
+  std::vector<int *> vconn;
+};
+InpData g_idata[MAXBLOCKS];
+
+TIOGA::tioga &tg_or_throw()
+{
+  if (!g_tg) throw std::runtime_error("tioga_init_ has not been called");
+  return *g_tg;
+}
+
+void register_va(int iblk, int *btag, int *nnodes, double *xyz, int *ibl, int *nwbc, int *nobc, int *wbcnode,
+                 int *obcnode, int *ntypes, va_list ap)
+{
+  if (iblk < 0 || iblk >= MAXBLOCKS) throw std::runtime_error("block id out of range 1..100");
+  InpData &d = g_idata[iblk];
+  d.nv.resize(*ntypes);
+  d.nc.resize(*ntypes);
+  d.vconn.resize(*ntypes);
+  for (int i = 0; i < *ntypes; i++) {
+    d.nv[i] = *(va_arg(ap, int *));
+    d.nc[i] = *(va_arg(ap, int *));
+    d.vconn[i] = va_arg(ap, int *);
+  }
+  tg_or_throw().registerGridData(*btag, *nnodes, xyz, ibl, *nwbc, *nobc, wbcnode, obcnode, *ntypes, d.nv.data(),
+                                 d.nc.data(), d.vconn.data());
+}
+
+int parse_itype(const char *itype)
+{
+  if (strstr(itype, "row")) return 0;
+  if (strstr(itype, "column")) return 1;
+  return -1;
+}
+
+void not_available(const char *what) { set_error(what, "not available in this engine (outside its scope)"); }
+
+}  // namespace
+
+extern "C" {
+
+void tioga_init_f90_(int *) { tioga_init_(0); }
+
+void tioga_init_(tioga_comm_t)
+{
+  guarded("tioga_init", [] {
+    delete g_tg;
+    g_tg = nullptr;
+    for (auto &d : g_idata) d = InpData();
+    g_tg = new TIOGA::tioga;
+  });
+}
+
+void tioga_registergrid_data_(int *btag, int *nnodes, double *xyz, int *ibl, int *nwbc, int *nobc, int *wbcnode,
+                              int *obcnode, int *ntypes, ...)
+{
+  va_list ap;
+  va_start(ap, ntypes);
+  guarded("tioga_registergrid_data", [&] {
+    register_va(0, btag, nnodes, xyz, ibl, nwbc, nobc, wbcnode, obcnode, ntypes, ap);
+  });
+  va_end(ap);
+}
+
+void tioga_registergrid_data_mb_(int *bid, int *btag, int *nnodes, double *xyz, int *ibl, int *nwbc, int *nobc,
+                                 int *wbcnode, int *obcnode, int *ntypes, ...)
+{
+  va_list ap;
+  va_start(ap, ntypes);
+  guarded("tioga_registergrid_data_mb", [&] {
+    register_va(*bid - 1, btag, nnodes, xyz, ibl, nwbc, nobc, wbcnode, obcnode, ntypes, ap);
+  });
+  va_end(ap);
+}
+
+void tioga_preprocess_grids_(void)
+{
+  guarded("tioga_preprocess_grids", [] { tg_or_throw().profile(); });
+}
+
+void tioga_performconnectivity_(void)
+{
+  guarded("tioga_performconnectivity", [] { tg_or_throw().performConnectivity(); });
+}
+
+void tioga_registersolution_(int *bid, double *q)
+{
+  guarded("tioga_registersolution", [&] { tg_or_throw().registerSolution(*bid, q); });
+}
+
+void tioga_dataupdate_mb_(int *nvar, char *itype)
+{
+  guarded("tioga_dataupdate_mb", [&] {
+    int t = parse_itype(itype);
+    if (t < 0) {
+      printf("#tiogaInterface.C:dataupdate_:unknown data orientation\n");
+      return;
+    }
+    tg_or_throw().dataUpdate(*nvar, t);
+  });
+}
+
+void tioga_dataupdate_(double *q, int *nvar, char *itype)
+{
+  guarded("tioga_dataupdate", [&] {
+    tg_or_throw().registerSolution(0, q);
+    tioga_dataupdate_mb_(nvar, itype);
+  });
+}
+
+void tioga_writeoutputfiles_(int *, char *itype)
+{
+  if (parse_itype(itype) < 0) {
+    printf("#tiogaInterface.C:dataupdate_:unknown data orientation\n");
+    return;
+  }
+  fprintf(stderr, "tioga_amd: tioga_writeoutputfiles_ writes no Tecplot files in this engine\n");
+}
+
+void tioga_getdonorcount_(int *btag, int *dcount, int *fcount)
+{
+  guarded("tioga_getdonorcount", [&] { tg_or_throw().getDonorCount(*btag, dcount, fcount); });
+}
+
+void tioga_getdonorinfo_(int *btag, int *receptors, int *indices, double *frac, int *dcount)
+{
+  guarded("tioga_getdonorinfo", [&] { tg_or_throw().getDonorInfo(*btag, receptors, indices, frac, dcount); });
+}
+
+void tioga_setsymmetry_(int *isym)
+{
+  guarded("tioga_setsymmetry", [&] { tg_or_throw().setSymmetry(*isym); });
+}
+
+void tioga_setresolutions_(double *nres, double *cres)
+{
+  guarded("tioga_setresolutions", [&] { tg_or_throw().setResolutions(nres, cres); });
+}
+
+void tioga_setresolutions_multi_(int *btag, double *nres, double *cres)
+{
+  guarded("tioga_setresolutions_multi", [&] { tg_or_throw().setResolutions(*btag, nres, cres); });
+}
+
+void tioga_setcelliblank_(int *iblank_cell)
+{
+  guarded("tioga_setcelliblank", [&] { tg_or_throw().set_cell_iblank(iblank_cell); });
+}
+
+void tioga_reduce_fringes_(void)
+{
+  guarded("tioga_reduce_fringes", [] { tg_or_throw().reduce_fringes(); });
+}
+
+void tioga_setnfringe_(int *nfringe)
+{
+  guarded("tioga_setnfringe", [&] { tg_or_throw().setNfringe(nfringe); });
+}
+
+void tioga_setmexclude_(int *mexclude)
+{
+  guarded("tioga_setmexclude", [&] { tg_or_throw().setMexclude(mexclude); });
+}
+
+void tioga_delete_(void)
+{
+  guarded("tioga_delete", [] {
+    delete g_tg;
+    g_tg = nullptr;
+    for (auto &d : g_idata) d = InpData();
+  });
+}
+
+void tioga_register_amr_global_data_(int *, int *, double *, int *, double *, int *, int *)
+{
+  not_available("tioga_register_amr_global_data");
+}
+void tioga_register_amr_patch_count_(int *) { not_available("tioga_register_amr_patch_count"); }
+void tioga_register_amr_local_data_(int *, int *, int *, double *) { not_available("tioga_register_amr_local_data"); }
+void tioga_performconnectivity_highorder_(void) { not_available("tioga_performconnectivity_highorder"); }
+void tioga_performconnectivity_amr_(void) { not_available("tioga_performconnectivity_amr"); }
+void tioga_set_highorder_callback_(void (*)(int *, int *), void (*)(int *, int *, double *),
+                                   void (*)(int *, double *, int *, double *),
+                                   void (*)(int *, double *, int *, int *, double *, double *, int *),
+                                   void (*)(int *, int *, double *, int *, int *, double *))
+{
+  not_available("tioga_set_highorder_callback");
+}
+void tioga_set_p4est_(void) { not_available("tioga_set_p4est"); }
+void tioga_set_amr_callback_(void (*)(int *, double *, int *, double *)) { not_available("tioga_set_amr_callback"); }
+void tioga_set_p4est_search_callback_(void (*)(double *, int *, int *, int *), void (*)(int *, int *))
+{
+  not_available("tioga_set_p4est_search_callback");
+}
+
+int tioga_amd_status(char *msg, int len)
+{
+  if (msg && len > 0) {
+    strncpy(msg, g_err.c_str(), (size_t)len - 1);
+    msg[len - 1] = 0;
+  }
+  return g_status;
+}
+
+void tioga_amd_clear_status(void)
+{
+  g_status = 0;
+  g_err.clear();
+}
+
+long tioga_amd_block_array(int btag, const char *name, void *out)
+{
+  long r = -1;
+  guarded("tioga_amd_block_array", [&] { r = tg_or_throw().engine()->block_array(btag, name, out); });
+  return r;
+}
+
+long tioga_amd_hole_map(int mtag, int *nx, double *ext, int *sam)
+{
+  long r = 0;
+  guarded("tioga_amd_hole_map", [&] { r = tg_or_throw().engine()->hole_map(mtag, nx, ext, sam); });
+  return r;
+}
+
+double tioga_amd_time_search(int nrep, long long *nsearch)
+{
+  double r = 0;
+  guarded("tioga_amd_time_search", [&] { r = tg_or_throw().engine()->time_search(nrep, nsearch); });
+  return r;
+}
+
+double tioga_amd_time_dataupdate(int nvar, int interptype, int nrep, long long *nreceptors)
+{
+  double r = 0;
+  guarded("tioga_amd_time_dataupdate",
+          [&] { r = tg_or_throw().engine()->time_data_update(nvar, interptype, nrep, nreceptors); });
+  return r;
+}
+
+void tioga_amd_phase_times(double *out)
+{
+  guarded("tioga_amd_phase_times", [&] {
+    const tg::PhaseTimes &t = tg_or_throw().engine()->last_times;
+    out[0] = t.holemap;
+    out[1] = t.boxes;
+    out[2] = t.searchdata;
+    out[3] = t.search_prep;
+    out[4] = t.search_kernel;
+    out[5] = t.donors;
+    out[6] = t.total;
+    out[7] = (double)t.nsearch;
+  });
+}
+
+int tioga_amd_weight_aborts(void)
+{
+  int r = 0;
+  guarded("tioga_amd_weight_aborts", [&] { r = tg_or_throw().engine()->weight_aborts(); });
+  return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,218 @@
+// tg_engine.h -- the MI355X connectivity engine behind the tioga C ABI and
+// the TIOGA::tioga class.
+//
+// One Engine holds one rank's share of an overset system: every registered
+// block's coordinates, connectivity, resolutions, search arrays, ADTs, hole
+// maps and donor tables live in HBM.  The user's host arrays are kept by
+// pointer exactly as the reference keeps them (MeshBlock.C:46-61): they are
+// read at preprocess (coordinates, connectivity, boundary lists, user
+// resolutions) and written at the end of performConnectivity (iblank,
+// iblank_cell) and dataUpdate (q).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "tg_common.h"
+
+namespace tg {
+
+void hip_check(hipError_t e, const char *what, const char *file, int line);
+#define TG_HIP(x) ::tg::hip_check((x), #x, __FILE__, __LINE__)
+
+// Owning device buffer that grows on demand.
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t cap = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  T *reserve(size_t n)
+  {
+    if (n == 0) n = 1;
+    if (n > cap) {
+      release();
+      TG_HIP(hipMalloc((void **)&p, n * sizeof(T)));
+      cap = n;
+    }
+    return p;
+  }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  operator T *() const { return p; }
+};
+
+constexpr int MAXTYPES = 8;
+
+// Cell types of a block: cells [cstart[t], cstart[t+1]) have nv[t] vertices.
+struct TypeInfo {
+  int ntypes;
+  int nv[MAXTYPES];
+  int cstart[MAXTYPES + 1];
+};
+
+// Hole map of one mesh tag as the kernels see it.
+struct HoleMapView {
+  const unsigned char *sam;  // nx*ny*nz, 1 = inside the hole
+  int nx[3];
+  int exist;
+  double ext[6];
+};
+
+// One interpolation record of a donor block (INTERPLIST, codetypes.h:122-131).
+struct InterpHost {
+  int cancel, nweights, rinfo[3];
+  int inode[9];
+  double w[9];
+};
+
+struct Block {
+  // ---- registration: the user's memory, kept by pointer ----
+  int tag = 0, nnodes = 0, ncells = 0;
+  TypeInfo ti{};
+  double *ux = nullptr;
+  int *uiblank = nullptr;
+  int nwbc = 0, nobc = 0;
+  int *uwbc = nullptr, *uobc = nullptr;
+  int *uconn[MAXTYPES] = {};
+  double *unres = nullptr, *ucres = nullptr;
+  int *uiblank_cell = nullptr;
+  std::vector<int> own_iblank_cell;
+  double *uq = nullptr;
+  uint64_t *ucell_gid = nullptr, *unode_gid = nullptr;
+  double resolutionScale = 1.0;
+  bool topo_uploaded = false;
+
+  // ---- device state ----
+  DBuf<double> x;          // 3V, AoS as registered
+  DBuf<int> conn;          // 8 per cell, 0-based, unused slots -1
+  DBuf<int> n2c_off, n2c;  // node -> incident cells, cell order (with multiplicity)
+  DBuf<unsigned char> wallnode, obcnode;
+  DBuf<int> iblank, iblank_cell;
+  DBuf<double> cellRes, nodeRes;
+  DBuf<unsigned> nodebin;  // 10^3 bin of each node in the block's OBB frame
+  DBuf<int> invmap;        // nodes ordered by (bin, descending node id)
+  OBB obb{};
+  double mapdx[3] = {0, 0, 0};
+
+  // search arrays (MeshBlock.h: xsearch, res_search, isearch, tagsearch, donorId, xtag)
+  int nsearch = 0;
+  DBuf<double> xsearch, res_search;
+  DBuf<int> isearch, tagsearch, donorId, xtag;
+  // ADT over the cells that overlap the query set's OBB
+  int nelem = 0, adt_depth = 0;
+  DBuf<int> elist, adtInt;
+  DBuf<double> ebox, adtReal;
+  double adtExt[6] = {0, 0, 0, 0, 0, 0};
+
+  // donor bookkeeping (receptor side: one entry per node)
+  DBuf<int> has_donor, head_arr, head_blk, head_remote;
+  DBuf<unsigned long long> rmax, head_key, tagmask;
+  // donor side: interpolation list
+  int ninterp = 0;
+  DBuf<int> interp_inode, interp_nw, interp_cancel, interp_rnode, interp_rblk, interp2donor;
+  DBuf<double> interp_w;
+  std::vector<InterpHost> interp_host;
+  bool interp_host_valid = false;
+};
+
+struct Pair {
+  int local = 0, ob = 0, iblk_remote = 0, tag_remote = 0, send_tag = 0, recv_tag = 0;
+  OBB obc{};
+};
+
+struct HoleMap {
+  int exist = 0, nx[3] = {0, 0, 0};
+  double ext[6] = {0, 0, 0, 0, 0, 0};
+  DBuf<unsigned char> sam;
+};
+
+// Per-phase device time of the last performConnectivity (ms, HIP events).
+struct PhaseTimes {
+  double holemap = 0, boxes = 0, searchdata = 0, search_prep = 0, search_kernel = 0, donors = 0,
+         total = 0;
+  long long nsearch = 0;
+};
+
+class Engine {
+ public:
+  Engine();
+  ~Engine();
+  Engine(const Engine &) = delete;
+  Engine &operator=(const Engine &) = delete;
+
+  // tioga.h:106-107 defaults
+  int isym = 3, nfringe = 1, mexclude = 3;
+
+  void register_grid(int btag, int nnodes, double *xyz, int *ibl, int nwbc, int nobc, int *wbc, int *obc,
+                     int ntypes, const int *nv, const int *nc, int **vconn);
+  void set_gids(int btag, uint64_t *cell_gid, uint64_t *node_gid);
+  void set_resolutions(int btag, double *nres, double *cres);
+  void set_resolutions_first(double *nres, double *cres);
+  void set_cell_iblank(int btag, int *ibc);
+  void set_cell_iblank_first(int *ibc);
+  void profile();
+  void perform_connectivity();
+  void register_solution(int btag, double *q);
+  void data_update(int nvar, int interptype);
+  void get_donor_count(int btag, int *dcount, int *fcount);
+  void get_donor_info(int btag, int *receptors, int *indices, double *frac, int *dcount);
+  void get_receptor_info(std::vector<int> &receptors);
+
+  // inspection of internal arrays (same names as the reference's members)
+  long block_array(int btag, const std::string &name, void *out);
+  long hole_map(int mtag, int nx[3], double ext[6], int *sam);
+
+  // Re-run the donor search kernels of every block over the resident ADTs
+  // and search arrays `nrep` times; returns the mean device ms per pass.
+  double time_search(int nrep, long long *nsearch);
+  // Re-run the dataUpdate gather/scatter on device-resident q `nrep` times
+  // (q uploaded once from the registered arrays); mean device ms per pass.
+  double time_data_update(int nvar, int interptype, int nrep, long long *nreceptors);
+
+  PhaseTimes last_times;
+  int weight_aborts() const { return weight_aborts_; }
+  hipStream_t stream() const { return s_; }
+
+ private:
+  Block *find(int btag, int *iblk = nullptr);
+  void upload_topology(Block &b);
+  void tag_boundary(Block &b);
+  void get_hole_map();
+  void exchange_boxes();
+  void exchange_search_data();
+  void search(Block &b);
+  void exchange_donors();
+  void cell_iblanks(Block &b);
+  void launch_search(Block &b);
+  void sync();
+
+  hipStream_t s_ = nullptr;
+  std::vector<Block *> blocks_;
+  std::map<int, int> tag_to_blk_;
+  std::vector<HoleMap> hm_;
+  DBuf<HoleMapView> hm_view_;
+  int nmesh_ = 0;
+  std::vector<Pair> pairs_;
+  int nsend_ = 0;
+  int weight_aborts_ = 0;
+  bool connected_ = false;
+  // scratch
+  DBuf<int> iscratch_, iscratch2_;
+  DBuf<unsigned> uscratch_, uscratch2_;
+  DBuf<unsigned long long> kscratch_;
+  DBuf<int> err_;
+  std::vector<DBuf<double> *> qdev_;
+};
+
+}  // namespace tg

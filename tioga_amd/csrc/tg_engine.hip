@@ -1,0 +1,2077 @@
+// tg_engine.hip -- device kernels and orchestration of the connectivity path.
+//
+// Phase order follows tioga::performConnectivity (tioga.C:112-160): hole
+// maps, block-pair boxes, query gathering, per-block donor search, donor
+// selection and iblanking, cell iblank.  Each kernel cites the reference
+// routine whose result it reproduces.  Order-dependent sequential steps of
+// the reference are either reformulated as order-free reductions with the
+// same result (documented at each kernel) or run on the host in tg_seq.cpp.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+
+#include "tg_engine.h"
+#include "tg_prims.h"
+#include "tg_seq.h"
+
+namespace tg {
+
+void hip_check(hipError_t e, const char *what, const char *file, int line)
+{
+  if (e == hipSuccess) return;
+  char buf[512];
+  snprintf(buf, sizeof buf, "tioga_amd: HIP error %s (%s) at %s:%d", hipGetErrorString(e), what, file, line);
+  fprintf(stderr, "%s\n", buf);
+  throw std::runtime_error(buf);
+}
+
+namespace {
+
+constexpr int BS = 256;
+inline unsigned nb(long long n, int bs = BS) { return (unsigned)((n + bs - 1) / bs); }
+
+__device__ __forceinline__ int type_of(const TypeInfo &ti, int c)
+{
+  int t = 0;
+  while (t < ti.ntypes - 1 && c >= ti.cstart[t + 1]) t++;
+  return t;
+}
+
+__device__ __forceinline__ void load_xyz(const double *x, int n, double p[3])
+{
+  p[0] = x[3 * (size_t)n];
+  p[1] = x[3 * (size_t)n + 1];
+  p[2] = x[3 * (size_t)n + 2];
+}
+
+// ---------------------------------------------------------------------------
+// topology
+// ---------------------------------------------------------------------------
+
+// 1-based per-type connectivity -> 0-based, 8 slots per cell
+__global__ void k_pack_conn(int nc, int nv, const int *__restrict__ src, int cell0, int *__restrict__ conn)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  for (int m = 0; m < 8; m++) conn[8 * (size_t)(cell0 + i) + m] = (m < nv) ? src[(size_t)nv * i + m] - BASE : -1;
+}
+
+// (node, cell) incidence pairs in cell-major order, and per-node counts
+__global__ void k_incidence(int ncells, int nnodes, const int *__restrict__ conn, unsigned *__restrict__ key,
+                            int *__restrict__ val, int *__restrict__ cnt)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  for (int m = 0; m < 8; m++) {
+    int v = conn[8 * (size_t)c + m];
+    size_t o = 8 * (size_t)c + m;
+    key[o] = v >= 0 ? (unsigned)v : (unsigned)nnodes;  // unused slots sort last
+    val[o] = c;
+    if (v >= 0) atomicAdd(&cnt[v], 1);
+  }
+}
+
+__global__ void k_flag_list(int n, const int *__restrict__ list, unsigned char *__restrict__ flag)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[list[i] - BASE] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// preprocess: MeshBlock::tagBoundary (MeshBlock.C:94-339)
+// ---------------------------------------------------------------------------
+
+__global__ void k_cell_res(int ncells, TypeInfo ti, const int *__restrict__ conn, const double *__restrict__ x,
+                           double scale, double *__restrict__ cellRes)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  int nv = ti.nv[type_of(ti, c)];
+  double xv[8][3];
+  for (int m = 0; m < nv; m++) load_xyz(x, conn[8 * (size_t)c + m], xv[m]);
+  double vol = cell_volume(xv, nv);
+  cellRes[c] = vol * scale;
+}
+
+// nodeRes = running sum of incident cell resolutions in cell order, divided by
+// the incidence count (MeshBlock.C:152-185).
+__global__ void k_node_res(int nnodes, const int *__restrict__ off, const int *__restrict__ n2c,
+                           const double *__restrict__ cellRes, double *__restrict__ nodeRes)
+{
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nnodes) return;
+  double s = 0.0;
+  int b = off[v], e = off[v + 1];
+  for (int k = b; k < e; k++) s += cellRes[n2c[k]];
+  if (e > b) s /= (e - b);
+  nodeRes[v] = s;
+}
+
+// bin of each node in the 10^3 map over the block's OBB (MeshBlock.C:171-226)
+__global__ void k_node_bin(int nnodes, const double *__restrict__ x, OBB o, double dx0, double dx1, double dx2,
+                           unsigned *__restrict__ key, int *__restrict__ val)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnodes) return;
+  int v = nnodes - 1 - i;  // descending node id, so a stable sort keeps it
+  double p[3], xd[3];
+  load_xyz(x, v, p);
+  double mdx[3] = {dx0, dx1, dx2};
+  int idx[3];
+  for (int j = 0; j < 3; j++) {
+    xd[j] = o.dxc[j];
+    for (int k = 0; k < 3; k++) xd[j] += (p[k] - o.xc[k]) * o.vec[j][k];
+    idx[j] = (int)(xd[j] / mdx[j]);
+    idx[j] = idx[j] < 0 ? 0 : (idx[j] > 9 ? 9 : idx[j]);
+  }
+  key[i] = (unsigned)(idx[2] * 100 + idx[1] * 10 + idx[0]);
+  val[i] = v;
+}
+
+__global__ void k_scatter_bin(int n, const unsigned *__restrict__ key, const int *__restrict__ val,
+                              unsigned *__restrict__ nodebin)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) nodebin[val[i]] = key[i];
+}
+
+// Cells touching an overset-boundary node: their nodes get nodeRes = BIGVALUE
+// and seed the mexclude layers (MeshBlock.C:228-262).
+__global__ void k_obc_cells(int ncells, TypeInfo ti, const int *__restrict__ conn,
+                            const unsigned char *__restrict__ obcnode, double *__restrict__ nodeRes,
+                            unsigned char *__restrict__ ex, unsigned char *__restrict__ ex1)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  int nv = ti.nv[type_of(ti, c)];
+  const int *cv = conn + 8 * (size_t)c;
+  int tag = 0;
+  for (int m = 0; m < nv; m++) tag |= obcnode[cv[m]];
+  if (!tag) return;
+  for (int m = 0; m < nv; m++) {
+    nodeRes[cv[m]] = BIGVALUE;
+    ex[cv[m]] = 1;
+    ex1[cv[m]] = 1;
+  }
+}
+
+// One mexclude layer (MeshBlock.C:264-335): cells touching a marked node get
+// cellRes = BIGVALUE and mark their other nodes for the next layer.
+__global__ void k_mexclude(int ncells, TypeInfo ti, const int *__restrict__ conn,
+                           const unsigned char *__restrict__ ex, unsigned char *__restrict__ ex1,
+                           double *__restrict__ cellRes)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  int nv = ti.nv[type_of(ti, c)];
+  const int *cv = conn + 8 * (size_t)c;
+  for (int m = 0; m < nv; m++)
+    if (ex[cv[m]] == 1) {
+      cellRes[c] = BIGVALUE;
+      break;
+    }
+  if (cellRes[c] == BIGVALUE)
+    for (int m = 0; m < nv; m++)
+      if (ex[cv[m]] != 1) ex1[cv[m]] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// hole map: holeMap.C:40-188, MeshBlock.C:666-772, utils.c:192-295
+// ---------------------------------------------------------------------------
+
+// Wall-touching cells paint their index box (floor of the offset / spacing,
+// clipped to the map) with 2 (MeshBlock.C:666-772).
+__global__ void k_mark_wall(int ncells, TypeInfo ti, const int *__restrict__ conn, const double *__restrict__ x,
+                            const unsigned char *__restrict__ wallnode, HoleMapView h, double ds0, double ds1,
+                            double ds2, unsigned char *__restrict__ sam)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  int nv = ti.nv[type_of(ti, c)];
+  const int *cv = conn + 8 * (size_t)c;
+  int w = 0;
+  for (int m = 0; m < nv; m++) w |= wallnode[cv[m]];
+  if (!w) return;
+  double ds[3] = {ds0, ds1, ds2};
+  int imin[3] = {BIGINT, BIGINT, BIGINT}, imax[3] = {-BIGINT, -BIGINT, -BIGINT};
+  for (int m = 0; m < nv; m++) {
+    double p[3];
+    load_xyz(x, cv[m], p);
+    for (int k = 0; k < 3; k++) {
+      int iv = (int)floor((p[k] - h.ext[k]) / ds[k]);
+      imin[k] = iv < imin[k] ? iv : imin[k];
+      imax[k] = iv > imax[k] ? iv : imax[k];
+    }
+  }
+  for (int j = 0; j < 3; j++) {
+    if (imin[j] < 0) imin[j] = 0;
+    if (imax[j] > h.nx[j] - 1) imax[j] = h.nx[j] - 1;
+  }
+  for (int kk = imin[2]; kk <= imax[2]; kk++)
+    for (int jj = imin[1]; jj <= imax[1]; jj++)
+      for (int ii = imin[0]; ii <= imax[0]; ii++)
+        sam[((size_t)kk * h.nx[1] + jj) * h.nx[0] + ii] = 2;
+}
+
+__global__ void k_hole_shell(int nx, int ny, int nz, unsigned char *__restrict__ sam)
+{
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  size_t n = (size_t)nx * ny * nz;
+  if (i >= n) return;
+  int a = (int)(i % nx), b = (int)((i / nx) % ny), c = (int)(i / ((size_t)nx * ny));
+  if (a == 0 || a == nx - 1 || b == 0 || b == ny - 1 || c == 0 || c == nz - 1) sam[i] = 1;
+}
+
+// Flood fill of the outside region (utils.c:228-290).  The reference repeats
+// in-place sweeps painting 0-cells that touch a 1-cell through the isym
+// stencil until nothing changes; its fixed point is the set of 0-cells joined
+// to the boundary shell through 0-cells, which any sweep order reaches.  Here
+// each thread owns one interior line along `axis` and propagates forwards and
+// backwards along it, also taking paint from the allowed cross neighbours.
+__global__ void k_line_fill(int nx, int ny, int nz, int axis, int allow, unsigned char *__restrict__ sam,
+                            int *__restrict__ changed)
+{
+  int n[3] = {nx, ny, nz};
+  size_t st[3] = {1, (size_t)nx, (size_t)nx * ny};
+  int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+  int m1 = n[a1] - 2, m2 = n[a2] - 2;
+  if (m1 <= 0 || m2 <= 0 || n[axis] < 3) return;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)m1 * m2) return;
+  int i1 = 1 + (int)(t % m1), i2 = 1 + (int)(t / m1);
+  size_t base = i1 * st[a1] + i2 * st[a2];
+  size_t sa = st[axis];
+  bool c1 = (allow >> a1) & 1, c2 = (allow >> a2) & 1;
+  int ch = 0;
+  auto paintable = [&](size_t m) -> bool {
+    if (c1 && (sam[m - st[a1]] == 1 || sam[m + st[a1]] == 1)) return true;
+    if (c2 && (sam[m - st[a2]] == 1 || sam[m + st[a2]] == 1)) return true;
+    return false;
+  };
+  unsigned char prev = sam[base];
+  for (int i = 1; i < n[axis] - 1; i++) {
+    size_t m = base + i * sa;
+    unsigned char v = sam[m];
+    if (v == 0 && (prev == 1 || paintable(m))) {
+      v = 1;
+      sam[m] = 1;
+      ch = 1;
+    }
+    prev = v;
+  }
+  prev = sam[base + (size_t)(n[axis] - 1) * sa];
+  for (int i = n[axis] - 2; i >= 1; i--) {
+    size_t m = base + i * sa;
+    unsigned char v = sam[m];
+    if (v == 0 && (prev == 1 || paintable(m))) {
+      v = 1;
+      sam[m] = 1;
+      ch = 1;
+    }
+    prev = v;
+  }
+  if (ch) *changed = 1;
+}
+
+__global__ void k_hole_final(size_t n, unsigned char *__restrict__ sam)
+{
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) sam[i] = (sam[i] == 0 || sam[i] == 2) ? 1 : 0;
+}
+
+// checkHoleMap (utils.c:170-185): truncation toward zero, range test on the int
+__device__ __forceinline__ int check_hole(const double p[3], const HoleMapView &h)
+{
+  int ix[3];
+  for (int i = 0; i < 3; i++) {
+    double dx = (h.ext[i + 3] - h.ext[i]) / h.nx[i];
+    ix[i] = (int)((p[i] - h.ext[i]) / dx);
+    if (ix[i] < 0 || ix[i] > h.nx[i] - 1) return 0;
+  }
+  return h.sam[((size_t)ix[2] * h.nx[1] + ix[1]) * h.nx[0] + ix[0]];
+}
+
+// ---------------------------------------------------------------------------
+// boxes: MeshBlock::getReducedOBB (MeshBlock.C:774-838)
+// ---------------------------------------------------------------------------
+
+// Bounds, in the block's own OBB frame, of its cells whose box in the
+// partner's OBB frame overlaps the partner OBB.  min/max are exact, so the
+// reduction order does not matter.
+__global__ void k_reduced_obb(int ncells, TypeInfo ti, const int *__restrict__ conn, const double *__restrict__ x,
+                              OBB own, OBB obc, unsigned long long *__restrict__ out)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double lo[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, hi[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
+  bool keep = false;
+  if (c < ncells) {
+    int nv = ti.nv[type_of(ti, c)];
+    const int *cv = conn + 8 * (size_t)c;
+    double bb[6] = {BIGVALUE, BIGVALUE, BIGVALUE, -BIGVALUE, -BIGVALUE, -BIGVALUE};
+    double p[3], xd[3];
+    for (int m = 0; m < nv; m++) {
+      load_xyz(x, cv[m], p);
+      to_frame(p, obc.xc, obc.vec, xd);
+      for (int j = 0; j < 3; j++) bb[j] = dmin(bb[j], xd[j]);
+      for (int j = 0; j < 3; j++) bb[j + 3] = dmax(bb[j + 3], xd[j]);
+    }
+    keep = true;
+    for (int j = 0; j < 3; j++)
+      if (bb[j] > obc.dxc[j]) keep = false;
+    for (int j = 0; j < 3; j++)
+      if (bb[j + 3] < -obc.dxc[j]) keep = false;
+    if (keep)
+      for (int m = 0; m < nv; m++) {
+        load_xyz(x, cv[m], p);
+        to_frame(p, own.xc, own.vec, xd);
+        for (int j = 0; j < 3; j++) lo[j] = dmin(lo[j], xd[j]);
+        for (int j = 0; j < 3; j++) hi[j] = dmax(hi[j], xd[j]);
+      }
+  }
+  // wave-level reduction, then one atomic per wave
+  for (int j = 0; j < 3; j++) {
+    unsigned long long kl = dkey(lo[j]), kh = dkey(hi[j]);
+    for (int o = 32; o > 0; o >>= 1) {
+      unsigned long long a = __shfl_xor(kl, o), b = __shfl_xor(kh, o);
+      kl = a < kl ? a : kl;
+      kh = b > kh ? b : kh;
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&out[j], kl);
+      atomicMax(&out[j + 3], kh);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// query gathering: MeshBlock::getQueryPoints2 (MeshBlock.C:953-1117)
+// ---------------------------------------------------------------------------
+
+// Position p of the block's bin order emits its node when the node's bin
+// passes the bin-level test and the node lies in the partner's reduced OBB.
+__global__ void k_query_flag(int nnodes, const int *__restrict__ invmap, const unsigned *__restrict__ nodebin,
+                             const unsigned char *__restrict__ binmask, const double *__restrict__ x, OBB obc,
+                             int *__restrict__ flag)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nnodes) return;
+  int v = invmap[p];
+  int f = 0;
+  if (binmask[nodebin[v]]) {
+    double q[3], xd[3];
+    load_xyz(x, v, q);
+    to_frame(q, obc.xc, obc.vec, xd);
+    f = (fabs(xd[0]) <= obc.dxc[0] && fabs(xd[1]) <= obc.dxc[1] && fabs(xd[2]) <= obc.dxc[2]);
+  }
+  flag[p] = f;
+}
+
+// exchangeSearchData.C:200-255: the receiver's search arrays for one packet
+__global__ void k_fill_search(int n, const int *__restrict__ sel, const int *__restrict__ invmap,
+                              const double *__restrict__ xs, const double *__restrict__ nodeRes, int off,
+                              int rblk, int rtag, double *__restrict__ xsearch, double *__restrict__ res_search,
+                              int *__restrict__ isearch, int *__restrict__ tagsearch)
+{
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int nd = invmap[sel[j]];
+  size_t o = off + (size_t)j;
+  isearch[3 * o] = 0;
+  isearch[3 * o + 1] = nd;
+  isearch[3 * o + 2] = rblk;
+  tagsearch[o] = rtag;
+  xsearch[3 * o] = xs[3 * (size_t)nd];
+  xsearch[3 * o + 1] = xs[3 * (size_t)nd + 1];
+  xsearch[3 * o + 2] = xs[3 * (size_t)nd + 2];
+  res_search[o] = nodeRes[nd];
+}
+
+// ---------------------------------------------------------------------------
+// search: MeshBlock::search (search.C:107-265)
+// ---------------------------------------------------------------------------
+
+// Cells whose box in the query-set OBB frame overlaps that OBB (search.C:123-167)
+__global__ void k_cell_filter(int ncells, TypeInfo ti, const int *__restrict__ conn, const double *__restrict__ x,
+                              OBB obq, int *__restrict__ flag)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  int nv = ti.nv[type_of(ti, c)];
+  const int *cv = conn + 8 * (size_t)c;
+  double lo[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, hi[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
+  for (int m = 0; m < nv; m++) {
+    double p[3], xd[3];
+    load_xyz(x, cv[m], p);
+    for (int j = 0; j < 3; j++) {
+      xd[j] = 0;
+      for (int k = 0; k < 3; k++) xd[j] += (p[k] - obq.xc[k]) * obq.vec[j][k];
+      lo[j] = dmin(lo[j], xd[j]);
+      hi[j] = dmax(hi[j], xd[j]);
+    }
+  }
+  double xc[3], dxc[3];
+  for (int j = 0; j < 3; j++) {
+    xc[j] = (hi[j] + lo[j]) * 0.5;
+    dxc[j] = (hi[j] - lo[j]) * 0.5;
+  }
+  // stored reversed: the element list is LIFO (descending cell index)
+  flag[ncells - 1 - c] = (fabs(xc[0]) <= (dxc[0] + obq.dxc[0]) && fabs(xc[1]) <= (dxc[1] + obq.dxc[1]) &&
+                          fabs(xc[2]) <= (dxc[2] + obq.dxc[2]));
+}
+
+// element list and element boxes (search.C:183-219)
+__global__ void k_elem_box(int nelem, int ncells, const int *__restrict__ sel, TypeInfo ti,
+                           const int *__restrict__ conn, const double *__restrict__ x, int *__restrict__ elist,
+                           double *__restrict__ ebox)
+{
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  int c = ncells - 1 - sel[e];
+  elist[e] = c;
+  int nv = ti.nv[type_of(ti, c)];
+  const int *cv = conn + 8 * (size_t)c;
+  double lo[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, hi[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
+  for (int m = 0; m < nv; m++) {
+    double p[3];
+    load_xyz(x, cv[m], p);
+    for (int j = 0; j < 3; j++) {
+      lo[j] = dmin(lo[j], p[j]);
+      hi[j] = dmax(hi[j], p[j]);
+    }
+  }
+  for (int j = 0; j < 3; j++) {
+    ebox[6 * (size_t)e + j] = lo[j];
+    ebox[6 * (size_t)e + 3 + j] = hi[j];
+  }
+}
+
+struct SearchView {
+  int nq;
+  const double *xs;
+  const int *xtag;
+  int *donorId;
+  // ADT
+  const int *adtInt;
+  const double *adtReal;
+  const double *ebox;
+  const int *elist;
+  double ext[6];
+  // donor block cells
+  TypeInfo ti;
+  const int *conn;
+  const double *x;
+  const double *cellRes;
+};
+
+__device__ __forceinline__ bool in_box(const double p[3], const double *b)
+{
+  return p[0] >= b[0] - TOL && p[1] >= b[1] - TOL && p[2] >= b[2] - TOL && p[0] <= b[3] + TOL &&
+         p[1] <= b[4] + TOL && p[2] <= b[5] + TOL;
+}
+
+// MeshBlock::checkContainment (checkContainment.C:27-96): weights of p in
+// the element's cell; accept when every weight lies in [-TOL, 1+TOL]; flag a
+// near-zero weight in a BIGVALUE cell, which keeps the search going.
+__device__ __forceinline__ void contain(const SearchView &S, int e, const double p[3], int &res, int &flag)
+{
+  int c = S.elist[e];
+  int nv = S.ti.nv[type_of(S.ti, c)];
+  const int *cv = S.conn + 8 * (size_t)c;
+  double xv[8][3], frac[8];
+  for (int m = 0; m < nv; m++) load_xyz(S.x, cv[m], xv[m]);
+  nodal_weights(xv, p, frac, nv);
+  res = c;
+  flag = 0;
+  bool big = (S.cellRes[c] == BIGVALUE);
+  for (int m = 0; m < nv; m++) {
+    if ((frac[m] + TOL) * (frac[m] - 1.0 - TOL) > 0) {
+      res = -1;
+      return;
+    }
+    if (fabs(frac[m]) < TOL && big) flag = 1;
+  }
+}
+
+constexpr int SEARCH_BS = 128;
+constexpr int STACK = 32;
+
+// ADT::searchADT / searchIntersections (searchADTrecursion.C:26-101) as an
+// explicit-stack preorder walk: a node's own element is tested first, then
+// its left and right subtrees whose boxes hold the point.  Every visited
+// element overwrites the result; the walk stops at the first clean hit.  The
+// per-thread stacks live in LDS, strided by the block size.
+__global__ void __launch_bounds__(SEARCH_BS) k_search(SearchView S)
+{
+  __shared__ int stk[STACK][SEARCH_BS];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S.nq) return;
+  if (S.xtag[i] != i) return;
+  double p[3];
+  load_xyz(S.xs, i, p);
+  int res = -1, flag = 0;
+  bool inroot = true;
+  for (int d = 0; d < 3; d++) inroot = inroot && (p[d] >= S.ext[2 * d] - TOL);
+  for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= S.ext[2 * d + 1] + TOL);
+  if (inroot) {
+    const int t = threadIdx.x;
+    int sp = 0;
+    stk[sp++][t] = 0;
+    while (sp > 0) {
+      int node = stk[--sp][t];
+      int4 a = reinterpret_cast<const int4 *>(S.adtInt)[node];
+      if (in_box(p, S.ebox + 6 * (size_t)a.x)) {
+        contain(S, a.x, p, res, flag);
+        if (res > -1 && flag == 0) break;
+      }
+      if (a.z > -1) {
+        int rn = S.adtInt[4 * (size_t)a.z + 3];
+        if (in_box(p, S.adtReal + 6 * (size_t)rn)) stk[sp++][t] = rn;
+      }
+      if (a.y > -1) {
+        int ln = S.adtInt[4 * (size_t)a.y + 3];
+        if (in_box(p, S.adtReal + 6 * (size_t)ln)) stk[sp++][t] = ln;
+      }
+    }
+  }
+  S.donorId[i] = res;
+}
+
+// coincident queries take their representative's donor (search.C:251-265)
+__global__ void k_copy_dup(int nq, const int *__restrict__ xtag, int *__restrict__ donorId)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  int r = i;
+  while (xtag[r] != r) r = xtag[r];
+  if (r != i) donorId[i] = donorId[r];
+}
+
+// ---------------------------------------------------------------------------
+// donor selection: exchangeDonors.C:25-301, bookKeeping.C:74-648
+// ---------------------------------------------------------------------------
+//
+// The reference inserts each donor record into a per-receptor linked list
+// sorted by donorRes, equal keys in arrival order (linklist.c:62-104).
+// Downstream only reads: whether the list is empty, the donor mesh tags, the
+// max receptorRes, and the head (the least donorRes, earliest arrival), so
+// these are formed here by order-free atomic reductions.  Arrival order is
+// (donor block, search index), encoded as soff[donor block] + i.
+
+struct DonorSrc {
+  int nsearch, arrival0, dblk, dtag;
+  const int *donorId, *isearch, *xtag;
+  const double *cellRes, *res_search;
+};
+
+struct RecvTables {  // per receptor block, indexed by block id
+  int *has[128];
+  unsigned long long *rmax[128], *hkey[128], *tmask[128];
+  int *harr[128], *hblk[128], *hrem[128];
+};
+
+__global__ void k_donor_pass1(DonorSrc D, int nwords, const RecvTables *__restrict__ R)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D.nsearch || D.donorId[i] < 0) return;
+  int rb = D.isearch[3 * i + 2], nd = D.isearch[3 * i + 1];
+  R->has[rb][nd] = 1;
+  atomicMax(&R->rmax[rb][nd], dkey(D.res_search[D.xtag[i]]));
+  atomicMin(&R->hkey[rb][nd], dkey(D.cellRes[D.donorId[i]]));
+  int t = D.dtag - BASE;
+  atomicOr(&R->tmask[rb][(size_t)nd * nwords + (t >> 6)], 1ull << (t & 63));
+}
+
+__global__ void k_donor_pass2(DonorSrc D, const RecvTables *__restrict__ R)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D.nsearch || D.donorId[i] < 0) return;
+  int rb = D.isearch[3 * i + 2], nd = D.isearch[3 * i + 1];
+  if (dkey(D.cellRes[D.donorId[i]]) == R->hkey[rb][nd]) atomicMin(&R->harr[rb][nd], D.arrival0 + i);
+}
+
+__global__ void k_donor_pass3(DonorSrc D, const RecvTables *__restrict__ R)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D.nsearch || D.donorId[i] < 0) return;
+  int rb = D.isearch[3 * i + 2], nd = D.isearch[3 * i + 1];
+  if (R->harr[rb][nd] == D.arrival0 + i) {
+    R->hblk[rb][nd] = D.dblk;
+    R->hrem[rb][nd] = i;
+  }
+}
+
+struct HoleSet {
+  int nmesh;
+  const HoleMapView *hm;
+};
+
+// MeshBlock::processDonors, first part (bookKeeping.C:159-238): iblank from
+// the hole maps of the other meshes (those that did not donate, when the node
+// has donors) and nodeRes raised to the largest receptor resolution.
+__global__ void k_process1(int nnodes, int mytag, const double *__restrict__ x, HoleSet H, int nwords,
+                           const int *__restrict__ has, const unsigned long long *__restrict__ rmax,
+                           const unsigned long long *__restrict__ tmask, int *__restrict__ iblank,
+                           double *__restrict__ nodeRes, unsigned char *__restrict__ mtag)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnodes) return;
+  int ib = 1;
+  double p[3];
+  load_xyz(x, i, p);
+  bool hd = has[i] != 0;
+  if (hd) nodeRes[i] = dmax(nodeRes[i], dunkey(rmax[i]));
+  for (int j = 0; j < H.nmesh; j++) {
+    if (j == mytag - BASE || !H.hm[j].exist) continue;
+    if (hd && ((tmask[(size_t)i * nwords + (j >> 6)] >> (j & 63)) & 1ull)) continue;
+    if (check_hole(p, H.hm[j])) {
+      ib = 0;
+      break;
+    }
+  }
+  iblank[i] = ib;
+  mtag[i] = (ib == 0);
+}
+
+// nfringe layers around hole nodes (bookKeeping.C:240-286)
+__global__ void k_fringe(int ncells, TypeInfo ti, const int *__restrict__ conn, const unsigned char *__restrict__ mt,
+                         unsigned char *__restrict__ mt1)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  int nv = ti.nv[type_of(ti, c)];
+  const int *cv = conn + 8 * (size_t)c;
+  int any = 0;
+  for (int m = 0; m < nv; m++) any |= (mt[cv[m]] == 1);
+  if (!any) return;
+  for (int m = 0; m < nv; m++)
+    if (mt[cv[m]] != 1) mt1[cv[m]] = 1;
+}
+
+// bookKeeping.C:288-340: fringe nodes become mandatory receptors; a node with
+// donors is a receptor when its best donor is finer than it.
+__global__ void k_process2(int nnodes, const unsigned char *__restrict__ mt1, const int *__restrict__ has,
+                           const unsigned long long *__restrict__ hkey, const int *__restrict__ hblk,
+                           const int *__restrict__ blk_tag, int *__restrict__ iblank, double *__restrict__ nodeRes,
+                           int *__restrict__ isrec)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnodes) return;
+  if (mt1[i] && iblank[i]) nodeRes[i] = BIGVALUE;
+  int r = 0;
+  if (has[i] && iblank[i] != 0 && dunkey(hkey[i]) < nodeRes[i]) {
+    iblank[i] = -blk_tag[hblk[i]];
+    r = 1;
+  }
+  isrec[i] = r;
+}
+
+// Records sent back to donors (bookKeeping.C:318-340), in receptor order.
+struct RecordList {
+  int n;
+  int *node, *dblk, *remote;
+  double *rres;
+};
+
+__global__ void k_emit_records(int n, const int *__restrict__ sel, const int *__restrict__ hblk,
+                               const int *__restrict__ hrem, const double *__restrict__ nodeRes, double scale,
+                               int off, RecordList L)
+{
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int i = sel[j];
+  L.node[off + j] = i;
+  L.dblk[off + j] = hblk[i];
+  L.remote[off + j] = hrem[i];
+  L.rres[off + j] = (scale > 1.0) ? -nodeRes[i] : nodeRes[i];
+}
+
+__global__ void k_flag_dblk(int n, const int *__restrict__ dblk, int d, int *__restrict__ flag)
+{
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) flag[j] = (dblk[j] == d);
+}
+
+struct InterpView {
+  int n;
+  const int *sel;  // record index for recid r
+  RecordList L;
+  // donor block
+  TypeInfo ti;
+  const int *conn;
+  const double *x;
+  const double *nodeRes;
+  const int *iblank;  // before any reset of this phase
+  const int *donorId, *isearch, *tagsearch;
+  const double *xsearch;
+  double scale;
+  int *tmin;
+  unsigned char *cancelv;
+  int *inode, *nw, *cancel, *rnode, *rblk, *i2d;
+  double *w;
+  int *err;
+};
+
+// Mandatory receptors (receptorRes == BIGVALUE) reset their donor cell's
+// negative-iblank vertices to 1 and put every vertex with iblank <= 0 and
+// finite nodeRes on the cancel list (bookKeeping.C:456-468).  A record
+// later in the donor's record order sees such a vertex as 1: tmin[v] is the
+// first record that resets v.
+__global__ void k_interp_mandatory(InterpView V)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= V.n) return;
+  int k = V.sel[r];
+  double rr = fabs(V.L.rres[k]);
+  if (!(rr == BIGVALUE && V.scale == 1.0)) return;
+  int c = V.donorId[V.L.remote[k]];
+  int nv = V.ti.nv[type_of(V.ti, c)];
+  const int *cv = V.conn + 8 * (size_t)c;
+  for (int m = 0; m < nv; m++) {
+    int v = cv[m];
+    if (V.iblank[v] <= 0 && V.nodeRes[v] != BIGVALUE) {
+      if (V.iblank[v] < 0) atomicMin(&V.tmin[v], r);
+      V.cancelv[v] = 1;
+    }
+  }
+}
+
+// MeshBlock::findInterpData (bookKeeping.C:385-535) for record r, seeing the
+// iblank values the reference sees at that point of its sequential walk.
+__global__ void k_interp(InterpView V)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= V.n) return;
+  int k = V.sel[r];
+  double rr2 = V.L.rres[k], rr = fabs(rr2);
+  int irec = V.L.remote[k];
+  int meshtagrecv = V.tagsearch[irec];
+  int c = V.donorId[irec];
+  int nv = V.ti.nv[type_of(V.ti, c)];
+  const int *cv = V.conn + 8 * (size_t)c;
+  double xv[8][3], xp[3], frac[8];
+  int accept = 1;
+  for (int m = 0; m < nv; m++) {
+    int v = cv[m];
+    int ib = (V.tmin[v] < r) ? 1 : V.iblank[v];
+    if (ib <= 0 && rr2 > 0.0) {
+      if (V.nodeRes[v] == BIGVALUE) accept = 0;
+      if (abs(ib) == meshtagrecv) accept = 0;
+    }
+    load_xyz(V.x, v, xv[m]);
+  }
+  load_xyz(V.xsearch, irec, xp);
+  nodal_weights(xv, xp, frac, nv);
+  V.i2d[irec] = r;
+  int bad = 0;
+  for (int m = 0; m < nv; m++) {
+    V.inode[9 * (size_t)r + m] = cv[m];
+    V.w[9 * (size_t)r + m] = frac[m];
+    if (frac[m] < -0.2 || frac[m] > 1.2) bad = 1;
+  }
+  V.inode[9 * (size_t)r + nv] = c;
+  V.w[9 * (size_t)r + nv] = 0.0;
+  V.nw[r] = nv;
+  V.cancel[r] = (accept == 0 && rr != BIGVALUE) ? 1 : 0;
+  V.rnode[r] = V.isearch[3 * irec + 1];
+  V.rblk[r] = V.isearch[3 * irec + 2];
+  if (bad) atomicAdd(V.err, 1);
+}
+
+__global__ void k_interp_reset(int nnodes, const int *__restrict__ tmin, int *__restrict__ iblank)
+{
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nnodes && tmin[v] != BIGINT) iblank[v] = 1;
+}
+
+struct CancelTables {
+  int *i2d[128];
+  int *cancel[128];
+};
+
+// cancel lists (bookKeeping.C:542-570): a vertex on a donor's cancel list
+// cancels its own donor's interpolation.
+__global__ void k_cancel(int nnodes, const unsigned char *__restrict__ cancelv, const int *__restrict__ has,
+                         const int *__restrict__ hblk, const int *__restrict__ hrem,
+                         const CancelTables *__restrict__ T)
+{
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nnodes || !cancelv[v] || !has[v]) return;
+  int d = hblk[v];
+  int iptr = T->i2d[d][hrem[v]];
+  if (iptr > -1) T->cancel[d][iptr] = 1;
+}
+
+// MeshBlock::resetCoincident (bookKeeping.C:572-597)
+__global__ void k_coinc1(int nq, const int *__restrict__ i2d, const int *__restrict__ xtag,
+                         const int *__restrict__ cancel, int *__restrict__ ireset)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  int p = i2d[i];
+  if (p > -1) atomicMin(&ireset[xtag[i]], cancel[p]);
+}
+
+__global__ void k_coinc2(int nq, const int *__restrict__ i2d, const int *__restrict__ xtag,
+                         const int *__restrict__ ireset, int *__restrict__ cancel)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  int p = i2d[i];
+  if (p > -1 && cancel[p] == 1) cancel[p] = ireset[xtag[i]];
+}
+
+__global__ void k_clear_neg(int n, int *__restrict__ iblank)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && iblank[i] < 0) iblank[i] = 1;
+}
+
+struct IblankTables {
+  int *ib[128];
+};
+
+__global__ void k_set_receptors(int n, const int *__restrict__ cancel, const int *__restrict__ rnode,
+                                const int *__restrict__ rblk, const IblankTables *__restrict__ T)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n && !cancel[r]) T->ib[rblk[r]][rnode[r]] = -1;
+}
+
+// MeshBlock::getCellIblanks (highOrder.C:88-150)
+__global__ void k_cell_iblank(int ncells, TypeInfo ti, const int *__restrict__ conn, const int *__restrict__ iblank,
+                              int *__restrict__ ibc)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  int nv = ti.nv[type_of(ti, c)];
+  const int *cv = conn + 8 * (size_t)c;
+  int out = 1, cnt = 0;
+  for (int m = 0; m < nv; m++) {
+    int v = iblank[cv[m]];
+    if (v == 0) {
+      out = 0;
+      break;
+    }
+    cnt += (v == -1);
+  }
+  if (out && cnt == nv) out = -1;
+  ibc[c] = out;
+}
+
+// ---------------------------------------------------------------------------
+// dataUpdate: MeshBlock::getInterpolatedSolution (dataUpdate.C:27-103) and
+// updateSolnData (dataUpdate.C:105-119)
+// ---------------------------------------------------------------------------
+
+__global__ void k_gather(int n, int nvar, int interptype, int nnodes, const int *__restrict__ inode,
+                         const double *__restrict__ w, const int *__restrict__ nw, const int *__restrict__ cancel,
+                         const double *__restrict__ q, double *__restrict__ out)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n || cancel[r]) return;
+  int m_n = nw[r];
+  for (int k = 0; k < nvar; k++) {
+    double acc = 0;
+    for (int m = 0; m < m_n; m++) {
+      int v = inode[9 * (size_t)r + m];
+      double qv = interptype == 0 ? q[(size_t)v * nvar + k] : q[(size_t)k * nnodes + v];
+      acc += qv * w[9 * (size_t)r + m];
+    }
+    out[(size_t)r * nvar + k] = acc;
+  }
+}
+
+struct QTables {
+  double *q[128];
+  int nnodes[128];
+};
+
+__global__ void k_scatter(int n, int nvar, int interptype, const int *__restrict__ cancel,
+                          const int *__restrict__ rnode, const int *__restrict__ rblk, const double *__restrict__ vals,
+                          const QTables *__restrict__ T)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n || cancel[r]) return;
+  int b = rblk[r], nd = rnode[r];
+  double *q = T->q[b];
+  for (int k = 0; k < nvar; k++) {
+    double v = vals[(size_t)r * nvar + k];
+    if (interptype == 0) q[(size_t)nd * nvar + k] = v;
+    else q[(size_t)k * T->nnodes[b] + nd] = v;
+  }
+}
+
+template <class T>
+__global__ void k_fill(size_t n, T *p, T v)
+{
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <class T>
+void fill(T *p, size_t n, T v, hipStream_t s)
+{
+  if (n) k_fill<T><<<nb((long long)n), BS, 0, s>>>(n, p, v);
+}
+
+// a small device-side copy of a host struct, valid until the next call
+template <class T>
+struct DevStruct {
+  DBuf<T> buf;
+  T *put(const T &h, hipStream_t s)
+  {
+    buf.reserve(1);
+    TG_HIP(hipMemcpyAsync(buf.p, &h, sizeof(T), hipMemcpyHostToDevice, s));
+    return buf.p;
+  }
+};
+
+struct Timer {
+  hipEvent_t a, b;
+  hipStream_t s;
+  explicit Timer(hipStream_t st) : s(st)
+  {
+    TG_HIP(hipEventCreate(&a));
+    TG_HIP(hipEventCreate(&b));
+    TG_HIP(hipEventRecord(a, s));
+  }
+  double stop()
+  {
+    float ms = 0;
+    TG_HIP(hipEventRecord(b, s));
+    TG_HIP(hipEventSynchronize(b));
+    TG_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms;
+  }
+};
+
+}  // namespace
+
+// ===========================================================================
+// Engine
+// ===========================================================================
+
+Engine::Engine()
+{
+  TG_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+  err_.reserve(1);
+}
+
+Engine::~Engine()
+{
+  for (Block *b : blocks_) delete b;
+  for (auto *q : qdev_) delete q;
+  if (s_) (void)hipStreamDestroy(s_);
+}
+
+void Engine::sync() { TG_HIP(hipStreamSynchronize(s_)); }
+
+Block *Engine::find(int btag, int *iblk)
+{
+  auto it = tag_to_blk_.find(btag);
+  if (it == tag_to_blk_.end()) return nullptr;
+  if (iblk) *iblk = it->second;
+  return blocks_[it->second];
+}
+
+// tioga::registerGridData (tioga.C:64-86)
+void Engine::register_grid(int btag, int nnodes, double *xyz, int *ibl, int nwbc, int nobc, int *wbc, int *obc,
+                           int ntypes, const int *nv, const int *nc, int **vconn)
+{
+  if (ntypes < 1 || ntypes > MAXTYPES) throw std::runtime_error("tioga_amd: ntypes out of range");
+  if (btag < 1) throw std::runtime_error("tioga_amd: mesh tags must be >= 1");
+  int iblk;
+  Block *b = find(btag, &iblk);
+  if (!b) {
+    if (blocks_.size() >= 128) throw std::runtime_error("tioga_amd: at most 128 blocks");
+    b = new Block;
+    blocks_.push_back(b);
+    tag_to_blk_[btag] = (int)blocks_.size() - 1;
+  }
+  b->tag = btag;
+  b->nnodes = nnodes;
+  b->ux = xyz;
+  b->uiblank = ibl;
+  b->nwbc = nwbc;
+  b->nobc = nobc;
+  b->uwbc = wbc;
+  b->uobc = obc;
+  b->ti.ntypes = ntypes;
+  b->ti.cstart[0] = 0;
+  for (int t = 0; t < ntypes; t++) {
+    if (nv[t] != 4 && nv[t] != 5 && nv[t] != 6 && nv[t] != 8)
+      throw std::runtime_error("tioga_amd: cells must have 4, 5, 6 or 8 vertices");
+    b->ti.nv[t] = nv[t];
+    b->ti.cstart[t + 1] = b->ti.cstart[t] + nc[t];
+    b->uconn[t] = vconn[t];
+  }
+  b->ncells = b->ti.cstart[ntypes];
+  b->topo_uploaded = false;
+  b->unres = b->ucres = nullptr;
+  connected_ = false;
+}
+
+void Engine::set_gids(int btag, uint64_t *cell_gid, uint64_t *node_gid)
+{
+  Block *b = find(btag);
+  if (!b) throw std::runtime_error("tioga_amd: gids for an unknown tag");
+  b->ucell_gid = cell_gid;
+  b->unode_gid = node_gid;
+}
+
+void Engine::set_resolutions_first(double *nres, double *cres)
+{
+  if (blocks_.empty()) throw std::runtime_error("tioga_amd: setResolutions before any registration");
+  blocks_[0]->unres = nres;
+  blocks_[0]->ucres = cres;
+}
+
+void Engine::set_cell_iblank_first(int *ibc)
+{
+  if (blocks_.empty()) throw std::runtime_error("tioga_amd: set_cell_iblank before any registration");
+  blocks_[0]->uiblank_cell = ibc;
+}
+
+void Engine::set_resolutions(int btag, double *nres, double *cres)
+{
+  Block *b = find(btag);
+  if (!b) throw std::runtime_error("tioga_amd: setResolutions for an unknown tag");
+  b->unres = nres;
+  b->ucres = cres;
+}
+
+void Engine::set_cell_iblank(int btag, int *ibc)
+{
+  Block *b = find(btag);
+  if (!b) throw std::runtime_error("tioga_amd: set_cell_iblank for an unknown tag");
+  b->uiblank_cell = ibc;
+}
+
+void Engine::register_solution(int btag, double *q)
+{
+  Block *b = find(btag);
+  if (!b) throw std::runtime_error("tioga_amd: registerSolution for an unknown tag");
+  b->uq = q;
+}
+
+// Node ids outside 1..nnodes would make every later kernel read out of
+// bounds; reject them before anything reaches the device.
+static void validate_ids(const int *ids, size_t n, int nnodes, const char *what)
+{
+  for (size_t i = 0; i < n; i++)
+    if (ids[i] < BASE || ids[i] > nnodes)
+      throw std::runtime_error(std::string("tioga_amd: ") + what + " holds node id " + std::to_string(ids[i]) +
+                               " outside 1.." + std::to_string(nnodes));
+}
+
+void Engine::upload_topology(Block &b)
+{
+  const int N = b.ncells, V = b.nnodes;
+  for (int t = 0; t < b.ti.ntypes; t++)
+    validate_ids(b.uconn[t], (size_t)(b.ti.cstart[t + 1] - b.ti.cstart[t]) * b.ti.nv[t], V, "connectivity");
+  b.conn.reserve(8 * (size_t)N);
+  DBuf<int> stage;
+  for (int t = 0; t < b.ti.ntypes; t++) {
+    int nc = b.ti.cstart[t + 1] - b.ti.cstart[t];
+    if (nc == 0) continue;
+    size_t n = (size_t)nc * b.ti.nv[t];
+    stage.reserve(n);
+    TG_HIP(hipMemcpyAsync(stage.p, b.uconn[t], n * sizeof(int), hipMemcpyHostToDevice, s_));
+    k_pack_conn<<<nb(nc), BS, 0, s_>>>(nc, b.ti.nv[t], stage.p, b.ti.cstart[t], b.conn.p);
+  }
+  // node -> cell incidence in cell order: stable sort of (node, cell) pairs
+  size_t np = 8 * (size_t)N;
+  DBuf<unsigned> k0, k1;
+  DBuf<int> v0, v1, cnt;
+  k0.reserve(np);
+  k1.reserve(np);
+  v0.reserve(np);
+  v1.reserve(np);
+  cnt.reserve(V + 1);
+  fill<int>(cnt.p, V + 1, 0, s_);
+  if (N) k_incidence<<<nb(N), BS, 0, s_>>>(N, V, b.conn.p, k0.p, v0.p, cnt.p);
+  int bits = 1;
+  while (bits < 32 && (1u << bits) <= (unsigned)V) bits++;
+  sort_pairs(k0.p, k1.p, v0.p, v1.p, (int)np, bits, s_);
+  b.n2c_off.reserve(V + 1);
+  exclusive_sum(cnt.p, b.n2c_off.p, V + 1, s_);
+  // the sorted pairs start with the valid ones (keys < V); keep them
+  b.n2c.reserve(np);
+  TG_HIP(hipMemcpyAsync(b.n2c.p, v1.p, np * sizeof(int), hipMemcpyDeviceToDevice, s_));
+  sync();
+  b.topo_uploaded = true;
+}
+
+// MeshBlock::preprocess (MeshBlock.C:74-92) + tagBoundary (MeshBlock.C:94-339)
+void Engine::tag_boundary(Block &b)
+{
+  const int N = b.ncells, V = b.nnodes;
+  b.cellRes.reserve(N);
+  b.nodeRes.reserve(V);
+  if (b.unres == nullptr && b.ucres == nullptr) {
+    if (N) k_cell_res<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, b.x.p, b.resolutionScale, b.cellRes.p);
+    if (V) k_node_res<<<nb(V), BS, 0, s_>>>(V, b.n2c_off.p, b.n2c.p, b.cellRes.p, b.nodeRes.p);
+  } else {
+    TG_HIP(hipMemcpyAsync(b.cellRes.p, b.ucres, sizeof(double) * N, hipMemcpyHostToDevice, s_));
+    TG_HIP(hipMemcpyAsync(b.nodeRes.p, b.unres, sizeof(double) * V, hipMemcpyHostToDevice, s_));
+  }
+  // 10^3 bins over the block OBB, nodes in (bin, descending id) order
+  for (int j = 0; j < 3; j++) b.mapdx[j] = 2 * b.obb.dxc[j] / 10;
+  uscratch_.reserve(V);
+  uscratch2_.reserve(V);
+  iscratch_.reserve(V);
+  b.invmap.reserve(V);
+  b.nodebin.reserve(V);
+  if (V) {
+    k_node_bin<<<nb(V), BS, 0, s_>>>(V, b.x.p, b.obb, b.mapdx[0], b.mapdx[1], b.mapdx[2], uscratch_.p,
+                                     iscratch_.p);
+    sort_pairs(uscratch_.p, uscratch2_.p, iscratch_.p, b.invmap.p, V, 10, s_);
+    k_scatter_bin<<<nb(V), BS, 0, s_>>>(V, uscratch2_.p, b.invmap.p, b.nodebin.p);
+  }
+  // overset boundary and mexclude layers
+  b.obcnode.reserve(V);
+  fill<unsigned char>(b.obcnode.p, V, 0, s_);
+  DBuf<int> lst;
+  if (b.nobc > 0) {
+    lst.reserve(b.nobc);
+    TG_HIP(hipMemcpyAsync(lst.p, b.uobc, sizeof(int) * b.nobc, hipMemcpyHostToDevice, s_));
+    k_flag_list<<<nb(b.nobc), BS, 0, s_>>>(b.nobc, lst.p, b.obcnode.p);
+  }
+  DBuf<unsigned char> ex, ex1;
+  ex.reserve(V);
+  ex1.reserve(V);
+  fill<unsigned char>(ex.p, V, 0, s_);
+  fill<unsigned char>(ex1.p, V, 0, s_);
+  if (N) k_obc_cells<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, b.obcnode.p, b.nodeRes.p, ex.p, ex1.p);
+  for (int it = 0; it < mexclude; it++) {
+    if (N) k_mexclude<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, ex.p, ex1.p, b.cellRes.p);
+    TG_HIP(hipMemcpyAsync(ex.p, ex1.p, V, hipMemcpyDeviceToDevice, s_));
+  }
+  // wall nodes, for the hole map
+  b.wallnode.reserve(V);
+  fill<unsigned char>(b.wallnode.p, V, 0, s_);
+  if (b.nwbc > 0) {
+    lst.reserve(b.nwbc);
+    TG_HIP(hipMemcpyAsync(lst.p, b.uwbc, sizeof(int) * b.nwbc, hipMemcpyHostToDevice, s_));
+    k_flag_list<<<nb(b.nwbc), BS, 0, s_>>>(b.nwbc, lst.p, b.wallnode.p);
+  }
+  sync();
+}
+
+// tioga::profile (tioga.C:95-110)
+void Engine::profile()
+{
+  for (Block *bp : blocks_) {
+    Block &b = *bp;
+    if (!b.topo_uploaded) upload_topology(b);
+    validate_ids(b.uwbc, b.nwbc, b.nnodes, "wall boundary list");
+    validate_ids(b.uobc, b.nobc, b.nnodes, "overset boundary list");
+    for (int i = 0; i < b.nnodes; i++) b.uiblank[i] = 1;  // MeshBlock.C:80
+    b.x.reserve(3 * (size_t)b.nnodes);
+    TG_HIP(hipMemcpyAsync(b.x.p, b.ux, sizeof(double) * 3 * b.nnodes, hipMemcpyHostToDevice, s_));
+    b.iblank.reserve(b.nnodes);
+    fill<int>(b.iblank.p, b.nnodes, 1, s_);
+    find_obb(b.ux, nullptr, b.nnodes, b.obb);
+    tag_boundary(b);
+  }
+}
+
+// tioga::getHoleMap (holeMap.C:40-188), single rank
+void Engine::get_hole_map()
+{
+  int maxtag = 1;
+  for (Block *b : blocks_) maxtag = std::max(maxtag, b->tag);
+  std::vector<HoleMap> old;
+  old.swap(hm_);
+  hm_ = std::vector<HoleMap>(maxtag);
+  // reuse the old maps' storage
+  for (size_t i = 0; i < old.size() && i < hm_.size(); i++) std::swap(hm_[i].sam.p, old[i].sam.p), std::swap(hm_[i].sam.cap, old[i].sam.cap);
+  std::vector<double> bmin(3 * maxtag, BIGVALUE), bmax(3 * maxtag, -BIGVALUE);
+  for (Block *bp : blocks_) {
+    Block &b = *bp;
+    HoleMap &h = hm_[b.tag - 1];
+    if (b.nwbc <= 0) {  // MeshBlock.C:640-647
+      h.exist = 0;
+      for (int j = 0; j < 3; j++) bmin[3 * (b.tag - 1) + j] = bmax[3 * (b.tag - 1) + j] = 0;
+      continue;
+    }
+    h.exist = 1;
+    double lo[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, hi[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
+    for (int i = 0; i < b.nwbc; i++) {
+      const double *p = b.ux + 3 * (size_t)(b.uwbc[i] - BASE);
+      for (int j = 0; j < 3; j++) {
+        lo[j] = dmin(lo[j], p[j]);
+        hi[j] = dmax(hi[j], p[j]);
+      }
+    }
+    for (int j = 0; j < 3; j++) {
+      bmin[3 * (b.tag - 1) + j] = lo[j];
+      bmax[3 * (b.tag - 1) + j] = hi[j];
+    }
+  }
+  std::vector<HoleMapView> views(maxtag);
+  for (int i = 0; i < maxtag; i++) {
+    HoleMap &h = hm_[i];
+    HoleMapView &hv = views[i];
+    hv.exist = h.exist;
+    hv.sam = nullptr;
+    if (!h.exist) continue;
+    double ds[3];
+    for (int j = 0; j < 3; j++) {
+      h.ext[j] = bmin[3 * i + j];
+      h.ext[j + 3] = bmax[3 * i + j];
+      ds[j] = h.ext[j + 3] - h.ext[j];
+    }
+    double dsmax = dmax(dmax(ds[0], ds[1]), ds[2]);
+    double dsbox = dsmax / HOLEMAPSIZE;
+    for (int j = 0; j < 3; j++) {
+      h.ext[j] -= (2 * dsbox);
+      h.ext[j + 3] += (2 * dsbox);
+      h.nx[j] = (int)floor(dmax((h.ext[j + 3] - h.ext[j]) / dsbox, 1));
+    }
+    size_t n = (size_t)h.nx[0] * h.nx[1] * h.nx[2];
+    h.sam.reserve(n);
+    fill<unsigned char>(h.sam.p, n, 0, s_);
+    hv.sam = h.sam.p;
+    memcpy(hv.nx, h.nx, sizeof hv.nx);
+    memcpy(hv.ext, h.ext, sizeof hv.ext);
+  }
+  for (Block *bp : blocks_) {
+    Block &b = *bp;
+    HoleMap &h = hm_[b.tag - 1];
+    if (!h.exist || b.ncells == 0) continue;
+    double ds[3];
+    for (int k = 0; k < 3; k++) ds[k] = (h.ext[k + 3] - h.ext[k]) / h.nx[k];
+    k_mark_wall<<<nb(b.ncells), BS, 0, s_>>>(b.ncells, b.ti, b.conn.p, b.x.p, b.wallnode.p, views[b.tag - 1],
+                                             ds[0], ds[1], ds[2], h.sam.p);
+  }
+  nmesh_ = maxtag;
+  // flood fill with the isym stencil: 3 excludes z, 1 excludes x, 2 excludes y
+  int allow = 7;
+  if (isym == 1) allow = 6;
+  else if (isym == 2) allow = 5;
+  else if (isym == 3) allow = 3;
+  int *changed = err_.p;  // reuse the one-int device word
+  for (int i = 0; i < maxtag; i++) {
+    HoleMap &h = hm_[i];
+    if (!h.exist) continue;
+    size_t n = (size_t)h.nx[0] * h.nx[1] * h.nx[2];
+    k_hole_shell<<<nb((long long)n), BS, 0, s_>>>(h.nx[0], h.nx[1], h.nx[2], h.sam.p);
+    for (int round = 0;; round++) {
+      int hc = 0;
+      TG_HIP(hipMemsetAsync(changed, 0, sizeof(int), s_));
+      for (int a = 0; a < 3; a++) {
+        if (!((allow >> a) & 1)) continue;
+        long long lines = (long long)(h.nx[(a + 1) % 3] - 2) * (h.nx[(a + 2) % 3] - 2);
+        if (lines > 0)
+          k_line_fill<<<nb(lines), BS, 0, s_>>>(h.nx[0], h.nx[1], h.nx[2], a, allow, h.sam.p, changed);
+      }
+      TG_HIP(hipMemcpyAsync(&hc, changed, sizeof(int), hipMemcpyDeviceToHost, s_));
+      sync();
+      if (!hc) break;
+    }
+    k_hole_final<<<nb((long long)n), BS, 0, s_>>>(n, h.sam.p);
+  }
+  hm_view_.reserve(maxtag);
+  TG_HIP(hipMemcpyAsync(hm_view_.p, views.data(), sizeof(HoleMapView) * maxtag, hipMemcpyHostToDevice, s_));
+  sync();
+}
+
+// tioga::exchangeBoxes (exchangeBoxes.C:34-272), single rank
+void Engine::exchange_boxes()
+{
+  const int nb_ = (int)blocks_.size();
+  int maxtag = -1;
+  for (Block *b : blocks_) maxtag = std::max(maxtag, std::abs(b->tag));
+  pairs_.clear();
+  for (int ob = 0; ob < nb_; ob++)
+    for (int ib = 0; ib < nb_; ib++) {
+      Block &A = *blocks_[ib], &B = *blocks_[ob];
+      if (std::abs(B.tag) == A.tag) continue;
+      if (obb_intersect(A.obb, B.obb) || obb_intersect(B.obb, A.obb)) {
+        Pair p;
+        p.local = ib;
+        p.ob = ob;
+        pairs_.push_back(p);
+      }
+    }
+  nsend_ = pairs_.empty() ? 0 : 1;
+  const int np = (int)pairs_.size();
+  kscratch_.reserve(6 * (size_t)std::max(np, 1));
+  std::vector<unsigned long long> init(6 * (size_t)std::max(np, 1));
+  for (int i = 0; i < np; i++)
+    for (int j = 0; j < 3; j++) {
+      init[6 * i + j] = dkey(BIGVALUE);
+      init[6 * i + 3 + j] = dkey(-BIGVALUE);
+    }
+  TG_HIP(hipMemcpyAsync(kscratch_.p, init.data(), sizeof(unsigned long long) * init.size(), hipMemcpyHostToDevice,
+                        s_));
+  for (int i = 0; i < np; i++) {
+    Pair &p = pairs_[i];
+    Block &A = *blocks_[p.local], &B = *blocks_[p.ob];
+    p.recv_tag = maxtag * (A.tag - 1) + B.tag - 1;
+    p.send_tag = maxtag * (B.tag - 1) + (A.tag - 1);
+    if (A.ncells)
+      k_reduced_obb<<<nb(A.ncells), BS, 0, s_>>>(A.ncells, A.ti, A.conn.p, A.x.p, A.obb, B.obb,
+                                                 kscratch_.p + 6 * (size_t)i);
+    memcpy(p.obc.vec, B.obb.vec, sizeof(double) * 9);
+  }
+  std::vector<unsigned long long> keys(6 * (size_t)std::max(np, 1));
+  TG_HIP(hipMemcpyAsync(keys.data(), kscratch_.p, sizeof(unsigned long long) * keys.size(), hipMemcpyDeviceToHost,
+                        s_));
+  sync();
+  std::vector<double> rd(6 * (size_t)std::max(np, 1));
+  for (int i = 0; i < np; i++) {  // MeshBlock.C:829-837
+    Block &A = *blocks_[pairs_[i].local];
+    double bb[6];
+    for (int j = 0; j < 6; j++) bb[j] = dunkey(keys[6 * i + j]);
+    for (int j = 0; j < 3; j++) {
+      rd[6 * i + j] = A.obb.xc[j];
+      for (int k = 0; k < 3; k++) rd[6 * i + j] += ((bb[k] + bb[k + 3]) * 0.5) * A.obb.vec[k][j];
+      rd[6 * i + 3 + j] = (bb[j + 3] - bb[j]) * 0.51;
+    }
+  }
+  // packet i lands on the pair whose receive key is its send key; std::map
+  // keeps the last pair inserted under a key (exchangeBoxes.C:206-243)
+  for (int i = 0; i < np; i++) {
+    int key = pairs_[i].send_tag, ii = -1;
+    for (int j = 0; j < np; j++)
+      if (pairs_[j].recv_tag == key) ii = j;
+    if (ii < 0) continue;
+    pairs_[ii].iblk_remote = pairs_[i].local;
+    pairs_[ii].tag_remote = blocks_[pairs_[i].local]->tag;
+    for (int j = 0; j < 3; j++) {
+      pairs_[ii].obc.xc[j] = rd[6 * i + j];
+      pairs_[ii].obc.dxc[j] = rd[6 * i + 3 + j];
+    }
+  }
+}
+
+// tioga::exchangeSearchData (exchangeSearchData.C:27-263), single rank
+void Engine::exchange_search_data()
+{
+  const int np = (int)pairs_.size();
+  for (Block *b : blocks_) b->nsearch = 0;
+  if (nsend_ == 0) return;
+  std::vector<int> cnt(np), dest(np, -1);
+  std::vector<DBuf<int> *> lists(np);
+  DBuf<unsigned char> binmask;
+  binmask.reserve(1000);
+  for (int i = 0; i < np; i++) {
+    Block &A = *blocks_[pairs_[i].local];
+    const OBB &obc = pairs_[i].obc;
+    // bin range and bin-level test (MeshBlock.C:973-1063)
+    double xv[8][3], xd[3], mdx[3], xmin[3], xmax[3];
+    int imin[3], imax[3];
+    obb_corners(obc.xc, obc.dxc, obc.vec, xv);
+    for (int j = 0; j < 3; j++) {
+      xmin[j] = BIGVALUE;
+      xmax[j] = -BIGVALUE;
+    }
+    for (int n = 0; n < 8; n++) {
+      to_frame(xv[n], A.obb.xc, A.obb.vec, xd);
+      for (int j = 0; j < 3; j++) {
+        xmin[j] = dmin(xmin[j], xd[j] + A.obb.dxc[j]);
+        xmax[j] = dmax(xmax[j], xd[j] + A.obb.dxc[j]);
+      }
+    }
+    for (int j = 0; j < 3; j++) {
+      double delta = 0.01 * (xmax[j] - xmin[j]);
+      xmin[j] -= delta;
+      xmax[j] += delta;
+      imin[j] = (int)dmax(xmin[j] / A.mapdx[j], 0);
+      imax[j] = (int)dmin(xmax[j] / A.mapdx[j], 10 - 1);
+      mdx[j] = 0.5 * A.mapdx[j];
+    }
+    obb_corners(obc.xc, mdx, A.obb.vec, xv);
+    for (int j = 0; j < 3; j++) {
+      xmin[j] = BIGVALUE;
+      xmax[j] = -BIGVALUE;
+    }
+    for (int m = 0; m < 8; m++) {
+      to_frame(xv[m], obc.xc, obc.vec, xd);
+      for (int j = 0; j < 3; j++) {
+        xmin[j] = dmin(xmin[j], xd[j]);
+        xmax[j] = dmax(xmax[j], xd[j]);
+      }
+    }
+    unsigned char mask[1000];
+    memset(mask, 0, sizeof mask);
+    for (int l = imin[2]; l <= imax[2]; l++)
+      for (int k = imin[1]; k <= imax[1]; k++)
+        for (int j = imin[0]; j <= imax[0]; j++) {
+          double xc[3];
+          xd[0] = -A.obb.dxc[0] + j * A.mapdx[0] + A.mapdx[0] * 0.5;
+          xd[1] = -A.obb.dxc[1] + k * A.mapdx[1] + A.mapdx[1] * 0.5;
+          xd[2] = -A.obb.dxc[2] + l * A.mapdx[2] + A.mapdx[2] * 0.5;
+          for (int n = 0; n < 3; n++) {
+            xc[n] = A.obb.xc[n];
+            for (int ij = 0; ij < 3; ij++) xc[n] += (xd[ij] * A.obb.vec[ij][n]);
+          }
+          to_frame(xc, obc.xc, obc.vec, xd);
+          bool out = false;
+          for (int ij = 0; ij < 3 && !out; ij++) out = (xmin[ij] + xd[ij] > obc.dxc[ij]);
+          if (out) continue;
+          for (int ij = 0; ij < 3 && !out; ij++) out = (xmax[ij] + xd[ij] < -obc.dxc[ij]);
+          if (out) continue;
+          mask[l * 100 + k * 10 + j] = 1;
+        }
+    TG_HIP(hipMemcpyAsync(binmask.p, mask, 1000, hipMemcpyHostToDevice, s_));
+    iscratch_.reserve(A.nnodes);
+    if (A.nnodes)
+      k_query_flag<<<nb(A.nnodes), BS, 0, s_>>>(A.nnodes, A.invmap.p, A.nodebin.p, binmask.p, A.x.p, obc,
+                                                iscratch_.p);
+    lists[i] = new DBuf<int>;
+    lists[i]->reserve(A.nnodes);
+    cnt[i] = select_flagged(iscratch_.p, A.nnodes, lists[i]->p, s_);
+  }
+  for (int i = 0; i < np; i++) {
+    int key = pairs_[i].send_tag, ii = -1;
+    for (int j = 0; j < np; j++)
+      if (pairs_[j].recv_tag == key) ii = j;
+    dest[i] = ii;
+    if (ii >= 0) blocks_[pairs_[ii].local]->nsearch += cnt[i];
+  }
+  for (Block *bp : blocks_) {
+    Block &b = *bp;
+    size_t n = std::max(b.nsearch, 1);
+    b.xsearch.reserve(3 * n);
+    b.res_search.reserve(n);
+    b.isearch.reserve(3 * n);
+    b.tagsearch.reserve(n);
+    b.donorId.reserve(n);
+    b.xtag.reserve(n);
+  }
+  std::vector<int> off(blocks_.size(), 0);
+  for (int i = 0; i < np; i++) {
+    int ii = dest[i];
+    if (ii < 0 || cnt[i] == 0) continue;
+    Block &A = *blocks_[pairs_[i].local];
+    int rb = pairs_[ii].local;
+    Block &R = *blocks_[rb];
+    k_fill_search<<<nb(cnt[i]), BS, 0, s_>>>(cnt[i], lists[i]->p, A.invmap.p, A.x.p, A.nodeRes.p, off[rb],
+                                             pairs_[ii].iblk_remote, pairs_[ii].tag_remote, R.xsearch.p,
+                                             R.res_search.p, R.isearch.p, R.tagsearch.p);
+    off[rb] += cnt[i];
+  }
+  sync();
+  for (auto *l : lists) delete l;
+}
+
+// MeshBlock::search (search.C:78-269), NODEGID off
+void Engine::search(Block &b)
+{
+  const int Q = b.nsearch;
+  b.nelem = 0;
+  if (Q == 0) return;
+  // host: query-set OBB and coincident-point dedupe (sequential order-fixing)
+  std::vector<double> xs(3 * (size_t)Q), rs(Q);
+  std::vector<int> ts(Q), xt(Q);
+  TG_HIP(hipMemcpyAsync(xs.data(), b.xsearch.p, sizeof(double) * 3 * Q, hipMemcpyDeviceToHost, s_));
+  TG_HIP(hipMemcpyAsync(rs.data(), b.res_search.p, sizeof(double) * Q, hipMemcpyDeviceToHost, s_));
+  TG_HIP(hipMemcpyAsync(ts.data(), b.tagsearch.p, sizeof(int) * Q, hipMemcpyDeviceToHost, s_));
+  sync();
+  OBB obq;
+  find_obb(xs.data(), nullptr, Q, obq);
+  // device: cell filter -> LIFO element list -> element boxes
+  const int N = b.ncells;
+  iscratch_.reserve(N);
+  iscratch2_.reserve(N);
+  if (N) k_cell_filter<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, b.x.p, obq, iscratch_.p);
+  int ne = select_flagged(iscratch_.p, N, iscratch2_.p, s_);
+  b.nelem = ne;
+  b.elist.reserve(ne);
+  b.ebox.reserve(6 * (size_t)ne);
+  b.adtInt.reserve(4 * (size_t)ne);
+  b.adtReal.reserve(6 * (size_t)ne);
+  if (ne) k_elem_box<<<nb(ne), BS, 0, s_>>>(ne, N, iscratch2_.p, b.ti, b.conn.p, b.x.p, b.elist.p, b.ebox.p);
+  std::vector<double> eb(6 * (size_t)ne), ar(6 * (size_t)ne);
+  std::vector<int> ai(4 * (size_t)ne);
+  TG_HIP(hipMemcpyAsync(eb.data(), b.ebox.p, sizeof(double) * eb.size(), hipMemcpyDeviceToHost, s_));
+  sync();
+  b.adt_depth = build_adt(ne, eb.data(), ai.data(), ar.data(), b.adtExt);
+  if (b.adt_depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
+  TG_HIP(hipMemcpyAsync(b.adtInt.p, ai.data(), sizeof(int) * ai.size(), hipMemcpyHostToDevice, s_));
+  TG_HIP(hipMemcpyAsync(b.adtReal.p, ar.data(), sizeof(double) * ar.size(), hipMemcpyHostToDevice, s_));
+  bool gid_mode = true;
+  for (Block *o : blocks_) gid_mode = gid_mode && (o->unode_gid != nullptr);
+  if (gid_mode) {
+    // search.C:48-74: first occurrence of each node global id
+    std::vector<int> is(3 * (size_t)Q);
+    TG_HIP(hipMemcpyAsync(is.data(), b.isearch.p, sizeof(int) * 3 * Q, hipMemcpyDeviceToHost, s_));
+    sync();
+    std::unordered_map<uint64_t, int> first;
+    first.reserve(2 * (size_t)Q);
+    for (int i = 0; i < Q; i++) {
+      uint64_t g = blocks_[is[3 * i + 2]]->unode_gid[is[3 * i + 1]];
+      auto it = first.find(g);
+      if (it != first.end()) {
+        xt[i] = it->second;
+        rs[it->second] = std::max(rs[it->second], rs[i]);
+      } else {
+        first[g] = i;
+        xt[i] = i;
+      }
+    }
+    for (int i = 0; i < Q; i++) rs[i] = rs[xt[i]];
+  } else {
+    unique_nodes_octree(xs.data(), ts.data(), rs.data(), xt.data(), Q);
+  }
+  TG_HIP(hipMemcpyAsync(b.xtag.p, xt.data(), sizeof(int) * Q, hipMemcpyHostToDevice, s_));
+  TG_HIP(hipMemcpyAsync(b.res_search.p, rs.data(), sizeof(double) * Q, hipMemcpyHostToDevice, s_));
+  sync();
+}
+
+void Engine::launch_search(Block &b)
+{
+  if (b.nsearch == 0) return;
+  SearchView S;
+  S.nq = b.nsearch;
+  S.xs = b.xsearch.p;
+  S.xtag = b.xtag.p;
+  S.donorId = b.donorId.p;
+  S.adtInt = b.adtInt.p;
+  S.adtReal = b.adtReal.p;
+  S.ebox = b.ebox.p;
+  S.elist = b.elist.p;
+  memcpy(S.ext, b.adtExt, sizeof S.ext);
+  S.ti = b.ti;
+  S.conn = b.conn.p;
+  S.x = b.x.p;
+  S.cellRes = b.cellRes.p;
+  if (b.nelem == 0) {
+    fill<int>(b.donorId.p, b.nsearch, -1, s_);
+    return;
+  }
+  k_search<<<nb(b.nsearch, SEARCH_BS), SEARCH_BS, 0, s_>>>(S);
+  k_copy_dup<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.xtag.p, b.donorId.p);
+}
+
+// tioga::exchangeDonors (exchangeDonors.C:25-301), single rank
+void Engine::exchange_donors()
+{
+  const int nbk = (int)blocks_.size();
+  if (nsend_ == 0 || nbk == 0) return;
+  const int nwords = (nmesh_ + 63) / 64;
+  RecvTables rt{};
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    const int V = b.nnodes;
+    b.has_donor.reserve(V);
+    b.rmax.reserve(V);
+    b.head_key.reserve(V);
+    b.head_arr.reserve(V);
+    b.head_blk.reserve(V);
+    b.head_remote.reserve(V);
+    b.tagmask.reserve((size_t)V * nwords);
+    fill<int>(b.has_donor.p, V, 0, s_);
+    fill<unsigned long long>(b.rmax.p, V, 0ull, s_);
+    fill<unsigned long long>(b.head_key.p, V, ~0ull, s_);
+    fill<int>(b.head_arr.p, V, BIGINT, s_);
+    fill<int>(b.head_blk.p, V, -1, s_);
+    fill<int>(b.head_remote.p, V, -1, s_);
+    fill<unsigned long long>(b.tagmask.p, (size_t)V * nwords, 0ull, s_);
+    rt.has[ib] = b.has_donor.p;
+    rt.rmax[ib] = b.rmax.p;
+    rt.hkey[ib] = b.head_key.p;
+    rt.tmask[ib] = b.tagmask.p;
+    rt.harr[ib] = b.head_arr.p;
+    rt.hblk[ib] = b.head_blk.p;
+    rt.hrem[ib] = b.head_remote.p;
+  }
+  DevStruct<RecvTables> drt;
+  RecvTables *prt = drt.put(rt, s_);
+  // comm1: donor records, arrival order = (donor block, search index)
+  std::vector<DonorSrc> src(nbk);
+  int arrival = 0;
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &D = *blocks_[ib];
+    src[ib] = DonorSrc{D.nsearch, arrival, ib, D.tag, D.donorId.p, D.isearch.p, D.xtag.p, D.cellRes.p,
+                       D.res_search.p};
+    arrival += D.nsearch;
+  }
+  for (int ib = 0; ib < nbk; ib++)
+    if (src[ib].nsearch) k_donor_pass1<<<nb(src[ib].nsearch), BS, 0, s_>>>(src[ib], nwords, prt);
+  for (int ib = 0; ib < nbk; ib++)
+    if (src[ib].nsearch) k_donor_pass2<<<nb(src[ib].nsearch), BS, 0, s_>>>(src[ib], prt);
+  for (int ib = 0; ib < nbk; ib++)
+    if (src[ib].nsearch) k_donor_pass3<<<nb(src[ib].nsearch), BS, 0, s_>>>(src[ib], prt);
+  // processDonors for every block; records in (receptor block, node) order
+  std::vector<int> blk_tag(nbk);
+  for (int ib = 0; ib < nbk; ib++) blk_tag[ib] = blocks_[ib]->tag;
+  DBuf<int> dtag;
+  dtag.reserve(nbk);
+  TG_HIP(hipMemcpyAsync(dtag.p, blk_tag.data(), sizeof(int) * nbk, hipMemcpyHostToDevice, s_));
+  HoleSet H{nmesh_, hm_view_.p};
+  int totrec = 0;
+  std::vector<DBuf<int> *> sel(nbk);
+  std::vector<int> nrec(nbk);
+  DBuf<unsigned char> mt, mt1;
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    const int V = b.nnodes, N = b.ncells;
+    mt.reserve(V);
+    mt1.reserve(V);
+    if (V)
+      k_process1<<<nb(V), BS, 0, s_>>>(V, b.tag, b.x.p, H, nwords, b.has_donor.p, b.rmax.p, b.tagmask.p,
+                                       b.iblank.p, b.nodeRes.p, mt.p);
+    TG_HIP(hipMemcpyAsync(mt1.p, mt.p, V, hipMemcpyDeviceToDevice, s_));
+    for (int it = 0; it < nfringe; it++) {
+      if (N) k_fringe<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, mt.p, mt1.p);
+      TG_HIP(hipMemcpyAsync(mt.p, mt1.p, V, hipMemcpyDeviceToDevice, s_));
+    }
+    iscratch_.reserve(V);
+    if (V)
+      k_process2<<<nb(V), BS, 0, s_>>>(V, mt1.p, b.has_donor.p, b.head_key.p, b.head_blk.p, dtag.p, b.iblank.p,
+                                       b.nodeRes.p, iscratch_.p);
+    sel[ib] = new DBuf<int>;
+    sel[ib]->reserve(V);
+    nrec[ib] = select_flagged(iscratch_.p, V, sel[ib]->p, s_);
+    totrec += nrec[ib];
+  }
+  RecordList L;
+  DBuf<int> lnode, ldblk, lrem;
+  DBuf<double> lrres;
+  L.n = totrec;
+  L.node = lnode.reserve(totrec);
+  L.dblk = ldblk.reserve(totrec);
+  L.remote = lrem.reserve(totrec);
+  L.rres = lrres.reserve(totrec);
+  for (int ib = 0, off = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    if (nrec[ib])
+      k_emit_records<<<nb(nrec[ib]), BS, 0, s_>>>(nrec[ib], sel[ib]->p, b.head_blk.p, b.head_remote.p,
+                                                  b.nodeRes.p, b.resolutionScale, off, L);
+    off += nrec[ib];
+    delete sel[ib];
+  }
+  // comm2: findInterpData per donor block, records in arrival order
+  DBuf<int> flag, dsel, tmin;
+  std::vector<DBuf<unsigned char> *> cancelv(nbk);
+  flag.reserve(totrec);
+  CancelTables ct{};
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &D = *blocks_[ib];
+    int nd = 0;
+    dsel.reserve(totrec);
+    if (totrec) {
+      k_flag_dblk<<<nb(totrec), BS, 0, s_>>>(totrec, L.dblk, ib, flag.p);
+      nd = select_flagged(flag.p, totrec, dsel.p, s_);
+    }
+    D.ninterp = nd;
+    D.interp_inode.reserve(9 * (size_t)nd);
+    D.interp_w.reserve(9 * (size_t)nd);
+    D.interp_nw.reserve(nd);
+    D.interp_cancel.reserve(nd);
+    D.interp_rnode.reserve(nd);
+    D.interp_rblk.reserve(nd);
+    D.interp2donor.reserve(std::max(D.nsearch, 1));
+    fill<int>(D.interp2donor.p, std::max(D.nsearch, 1), -1, s_);
+    tmin.reserve(D.nnodes);
+    fill<int>(tmin.p, D.nnodes, BIGINT, s_);
+    cancelv[ib] = new DBuf<unsigned char>;
+    cancelv[ib]->reserve(D.nnodes);
+    fill<unsigned char>(cancelv[ib]->p, D.nnodes, 0, s_);
+    ct.i2d[ib] = D.interp2donor.p;
+    ct.cancel[ib] = D.interp_cancel.p;
+    if (nd == 0) continue;
+    InterpView V;
+    V.n = nd;
+    V.sel = dsel.p;
+    V.L = L;
+    V.ti = D.ti;
+    V.conn = D.conn.p;
+    V.x = D.x.p;
+    V.nodeRes = D.nodeRes.p;
+    V.iblank = D.iblank.p;
+    V.donorId = D.donorId.p;
+    V.isearch = D.isearch.p;
+    V.tagsearch = D.tagsearch.p;
+    V.xsearch = D.xsearch.p;
+    V.scale = D.resolutionScale;
+    V.tmin = tmin.p;
+    V.cancelv = cancelv[ib]->p;
+    V.inode = D.interp_inode.p;
+    V.nw = D.interp_nw.p;
+    V.cancel = D.interp_cancel.p;
+    V.rnode = D.interp_rnode.p;
+    V.rblk = D.interp_rblk.p;
+    V.i2d = D.interp2donor.p;
+    V.w = D.interp_w.p;
+    V.err = err_.p;
+    TG_HIP(hipMemsetAsync(err_.p, 0, sizeof(int), s_));
+    k_interp_mandatory<<<nb(nd), BS, 0, s_>>>(V);
+    k_interp<<<nb(nd), BS, 0, s_>>>(V);
+    k_interp_reset<<<nb(D.nnodes), BS, 0, s_>>>(D.nnodes, tmin.p, D.iblank.p);
+    int nbad = 0;
+    TG_HIP(hipMemcpyAsync(&nbad, err_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
+    sync();
+    if (nbad) {
+      weight_aborts_ += nbad;
+      fprintf(stderr, "tioga_amd: %d donor weights outside [-0.2, 1.2] in block %d (the reference aborts here)\n",
+              nbad, D.tag);
+    }
+  }
+  // comm3: cancellations, then coincident resets
+  DevStruct<CancelTables> dct;
+  CancelTables *pct = dct.put(ct, s_);
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    if (b.nnodes)
+      k_cancel<<<nb(b.nnodes), BS, 0, s_>>>(b.nnodes, cancelv[ib]->p, b.has_donor.p, b.head_blk.p,
+                                            b.head_remote.p, pct);
+  }
+  DBuf<int> ireset;
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    if (b.nsearch == 0) continue;
+    ireset.reserve(b.nsearch);
+    fill<int>(ireset.p, b.nsearch, 1, s_);
+    k_coinc1<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.interp2donor.p, b.xtag.p, b.interp_cancel.p, ireset.p);
+    k_coinc2<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.interp2donor.p, b.xtag.p, ireset.p, b.interp_cancel.p);
+  }
+  // comm4: clear negative iblanks, then mark live receptors
+  IblankTables it{};
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    if (b.nnodes) k_clear_neg<<<nb(b.nnodes), BS, 0, s_>>>(b.nnodes, b.iblank.p);
+    it.ib[ib] = b.iblank.p;
+  }
+  DevStruct<IblankTables> dit;
+  IblankTables *pit = dit.put(it, s_);
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    if (b.ninterp)
+      k_set_receptors<<<nb(b.ninterp), BS, 0, s_>>>(b.ninterp, b.interp_cancel.p, b.interp_rnode.p,
+                                                    b.interp_rblk.p, pit);
+  }
+  sync();
+  for (auto *c : cancelv) delete c;
+}
+
+void Engine::cell_iblanks(Block &b)
+{
+  b.iblank_cell.reserve(b.ncells);
+  if (b.ncells) k_cell_iblank<<<nb(b.ncells), BS, 0, s_>>>(b.ncells, b.ti, b.conn.p, b.iblank.p, b.iblank_cell.p);
+}
+
+// tioga::performConnectivity (tioga.C:112-160)
+void Engine::perform_connectivity()
+{
+  PhaseTimes t;
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+  auto t0 = now();
+  get_hole_map();
+  auto t1 = now();
+  exchange_boxes();
+  auto t2 = now();
+  exchange_search_data();
+  auto t3 = now();
+  for (Block *b : blocks_) {
+    b->ninterp = 0;
+    b->interp_host_valid = false;
+    search(*b);
+  }
+  auto t4 = now();
+  {
+    Timer tm(s_);
+    for (Block *b : blocks_) launch_search(*b);
+    t.search_kernel = tm.stop();
+  }
+  auto t5 = now();
+  exchange_donors();
+  for (Block *b : blocks_) cell_iblanks(*b);
+  sync();
+  auto t6 = now();
+  // write back to the user's arrays
+  for (Block *bp : blocks_) {
+    Block &b = *bp;
+    TG_HIP(hipMemcpyAsync(b.uiblank, b.iblank.p, sizeof(int) * b.nnodes, hipMemcpyDeviceToHost, s_));
+    int *ibc = b.uiblank_cell;
+    if (!ibc) {
+      b.own_iblank_cell.resize(std::max(b.ncells, 1));
+      ibc = b.own_iblank_cell.data();
+    }
+    TG_HIP(hipMemcpyAsync(ibc, b.iblank_cell.p, sizeof(int) * b.ncells, hipMemcpyDeviceToHost, s_));
+    b.uq = nullptr;  // solutions are registered again after connectivity (tioga.C:152-155)
+  }
+  sync();
+  auto t7 = now();
+  t.holemap = ms(t0, t1);
+  t.boxes = ms(t1, t2);
+  t.searchdata = ms(t2, t3);
+  t.search_prep = ms(t3, t4);
+  t.donors = ms(t5, t6);
+  t.total = ms(t0, t7);
+  t.nsearch = 0;
+  for (Block *b : blocks_) t.nsearch += b->nsearch;
+  last_times = t;
+  connected_ = true;
+}
+
+// tioga::dataUpdate (tioga.C:336-524), at_points = 0
+void Engine::data_update(int nvar, int interptype)
+{
+  for (Block *b : blocks_)
+    if (b->uq == nullptr) return;
+  if (nsend_ == 0 || nvar <= 0) return;
+  const int nbk = (int)blocks_.size();
+  while ((int)qdev_.size() < nbk) qdev_.push_back(new DBuf<double>);
+  QTables qt{};
+  size_t tot = 0;
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    size_t n = (size_t)b.nnodes * nvar;
+    qdev_[ib]->reserve(n);
+    TG_HIP(hipMemcpyAsync(qdev_[ib]->p, b.uq, sizeof(double) * n, hipMemcpyHostToDevice, s_));
+    qt.q[ib] = qdev_[ib]->p;
+    qt.nnodes[ib] = b.nnodes;
+    tot += (size_t)b.ninterp * nvar;
+  }
+  DBuf<double> vals;
+  vals.reserve(tot);
+  for (int ib = 0, off = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    if (b.ninterp)
+      k_gather<<<nb(b.ninterp), BS, 0, s_>>>(b.ninterp, nvar, interptype, b.nnodes, b.interp_inode.p, b.interp_w.p,
+                                             b.interp_nw.p, b.interp_cancel.p, qdev_[ib]->p,
+                                             vals.p + (size_t)off * nvar);
+    off += b.ninterp;
+  }
+  DevStruct<QTables> dq;
+  QTables *pq = dq.put(qt, s_);
+  for (int ib = 0, off = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    if (b.ninterp)
+      k_scatter<<<nb(b.ninterp), BS, 0, s_>>>(b.ninterp, nvar, interptype, b.interp_cancel.p, b.interp_rnode.p,
+                                              b.interp_rblk.p, vals.p + (size_t)off * nvar, pq);
+    off += b.ninterp;
+  }
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    TG_HIP(hipMemcpyAsync(b.uq, qdev_[ib]->p, sizeof(double) * (size_t)b.nnodes * nvar, hipMemcpyDeviceToHost,
+                          s_));
+  }
+  sync();
+}
+
+static void fetch_interp(Block &b, hipStream_t s)
+{
+  if (b.interp_host_valid) return;
+  const int n = b.ninterp;
+  std::vector<int> inode(9 * (size_t)n), nw(n), cancel(n), rnode(n), rblk(n);
+  std::vector<double> w(9 * (size_t)n);
+  if (n) {
+    TG_HIP(hipMemcpyAsync(inode.data(), b.interp_inode.p, sizeof(int) * inode.size(), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipMemcpyAsync(w.data(), b.interp_w.p, sizeof(double) * w.size(), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipMemcpyAsync(nw.data(), b.interp_nw.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipMemcpyAsync(cancel.data(), b.interp_cancel.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipMemcpyAsync(rnode.data(), b.interp_rnode.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipMemcpyAsync(rblk.data(), b.interp_rblk.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+  }
+  b.interp_host.resize(n);
+  for (int r = 0; r < n; r++) {
+    InterpHost &h = b.interp_host[r];
+    h.cancel = cancel[r];
+    h.nweights = nw[r];
+    h.rinfo[0] = 0;
+    h.rinfo[1] = rnode[r];
+    h.rinfo[2] = rblk[r];
+    for (int m = 0; m < 9; m++) {
+      h.inode[m] = inode[9 * (size_t)r + m];
+      h.w[m] = w[9 * (size_t)r + m];
+    }
+  }
+  b.interp_host_valid = true;
+}
+
+// tioga::getDonorCount (tioga.C:533-549, dataUpdate.C:121-134)
+void Engine::get_donor_count(int btag, int *dcount, int *fcount)
+{
+  *dcount = *fcount = 0;
+  Block *b = find(btag);
+  if (!b || nsend_ == 0) return;
+  fetch_interp(*b, s_);
+  for (auto &h : b->interp_host)
+    if (!h.cancel) {
+      (*dcount)++;
+      *fcount += h.nweights + 1;
+    }
+}
+
+// tioga::getDonorInfo (tioga.C:551-570, dataUpdate.C:136-159); rank 0
+void Engine::get_donor_info(int btag, int *receptors, int *indices, double *frac, int *dcount)
+{
+  (void)dcount;
+  Block *b = find(btag);
+  if (!b || nsend_ == 0) return;
+  fetch_interp(*b, s_);
+  size_t j = 0, k = 0;
+  for (auto &h : b->interp_host) {
+    if (h.cancel) continue;
+    for (int m = 0; m < h.nweights + 1; m++) {
+      indices[j] = h.inode[m];
+      frac[j] = h.w[m];
+      j++;
+    }
+    receptors[k++] = h.rinfo[0];
+    receptors[k++] = h.rinfo[1];
+    receptors[k++] = h.rinfo[2];
+    receptors[k++] = h.nweights;
+  }
+}
+
+// tioga::getReceptorInfo (tioga.C:572-671, dataUpdate.C:161-178), single
+// rank: per live record (node, receptor mesh tag, donor cell global id as two
+// ints); the cell index stands in for the global id without cell_gid.
+void Engine::get_receptor_info(std::vector<int> &receptors)
+{
+  receptors.clear();
+  if (nsend_ == 0) return;
+  for (Block *bp : blocks_) {
+    Block &b = *bp;
+    fetch_interp(b, s_);
+    for (auto &h : b.interp_host) {
+      if (h.cancel) continue;
+      receptors.push_back(h.rinfo[1]);
+      receptors.push_back(blocks_[h.rinfo[2]]->tag);
+      int don = h.inode[h.nweights];
+      uint64_t g = b.ucell_gid ? b.ucell_gid[don] : (uint64_t)don;
+      int two[2];
+      memcpy(two, &g, sizeof g);
+      receptors.push_back(two[0]);
+      receptors.push_back(two[1]);
+    }
+  }
+}
+
+template <class T>
+static long copy_out(const DBuf<T> &d, size_t n, void *out, hipStream_t s)
+{
+  if (out && n) {
+    TG_HIP(hipMemcpyAsync(out, d.p, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+  }
+  return (long)n;
+}
+
+long Engine::block_array(int btag, const std::string &nm, void *out)
+{
+  Block *bp = find(btag);
+  if (!bp) return -1;
+  Block &b = *bp;
+  const size_t Q = b.nsearch;
+  if (nm == "nodeRes") return copy_out(b.nodeRes, b.nnodes, out, s_);
+  if (nm == "cellRes") return copy_out(b.cellRes, b.ncells, out, s_);
+  if (nm == "xsearch") return copy_out(b.xsearch, 3 * Q, out, s_);
+  if (nm == "res_search") return copy_out(b.res_search, Q, out, s_);
+  if (nm == "isearch") return copy_out(b.isearch, 3 * Q, out, s_);
+  if (nm == "tagsearch") return copy_out(b.tagsearch, Q, out, s_);
+  if (nm == "donorId") return copy_out(b.donorId, Q, out, s_);
+  if (nm == "xtag") return copy_out(b.xtag, Q, out, s_);
+  if (nm == "elementList") return copy_out(b.elist, Q ? b.nelem : 0, out, s_);
+  if (nm == "adtIntegers") return copy_out(b.adtInt, Q ? 4 * (size_t)b.nelem : 0, out, s_);
+  if (nm == "adtReals") return copy_out(b.adtReal, Q ? 6 * (size_t)b.nelem : 0, out, s_);
+  if (nm == "iblank_cell") return copy_out(b.iblank_cell, b.ncells, out, s_);
+  if (nm == "adtExtents") {
+    if (out && Q) memcpy(out, b.adtExt, sizeof b.adtExt);
+    return Q ? 6 : 0;
+  }
+  if (nm == "obb") {
+    if (out) {
+      double *o = (double *)out;
+      memcpy(o, b.obb.xc, 24);
+      memcpy(o + 3, b.obb.dxc, 24);
+      memcpy(o + 6, b.obb.vec, 72);
+    }
+    return 15;
+  }
+  return -1;
+}
+
+long Engine::hole_map(int mtag, int nx[3], double ext[6], int *sam)
+{
+  if (mtag < 1 || mtag > (int)hm_.size()) return 0;
+  HoleMap &h = hm_[mtag - 1];
+  if (!h.exist) return 0;
+  size_t n = (size_t)h.nx[0] * h.nx[1] * h.nx[2];
+  if (nx) memcpy(nx, h.nx, sizeof(int) * 3);
+  if (ext) memcpy(ext, h.ext, sizeof(double) * 6);
+  if (sam) {
+    std::vector<unsigned char> tmp(n);
+    TG_HIP(hipMemcpyAsync(tmp.data(), h.sam.p, n, hipMemcpyDeviceToHost, s_));
+    sync();
+    for (size_t i = 0; i < n; i++) sam[i] = tmp[i];
+  }
+  return (long)n;
+}
+
+double Engine::time_search(int nrep, long long *nsearch)
+{
+  long long q = 0;
+  for (Block *b : blocks_) q += b->nsearch;
+  if (nsearch) *nsearch = q;
+  if (!connected_ || nrep <= 0) return 0.0;
+  Timer tm(s_);
+  for (int r = 0; r < nrep; r++)
+    for (Block *b : blocks_) launch_search(*b);
+  return tm.stop() / nrep;
+}
+
+double Engine::time_data_update(int nvar, int interptype, int nrep, long long *nrecv)
+{
+  const int nbk = (int)blocks_.size();
+  long long nr = 0;
+  for (Block *b : blocks_) nr += b->ninterp;
+  if (nrecv) *nrecv = nr;
+  for (Block *b : blocks_)
+    if (b->uq == nullptr) return 0.0;
+  if (nsend_ == 0 || nvar <= 0 || nrep <= 0) return 0.0;
+  while ((int)qdev_.size() < nbk) qdev_.push_back(new DBuf<double>);
+  QTables qt{};
+  size_t tot = 0;
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    size_t n = (size_t)b.nnodes * nvar;
+    qdev_[ib]->reserve(n);
+    TG_HIP(hipMemcpyAsync(qdev_[ib]->p, b.uq, sizeof(double) * n, hipMemcpyHostToDevice, s_));
+    qt.q[ib] = qdev_[ib]->p;
+    qt.nnodes[ib] = b.nnodes;
+    tot += (size_t)b.ninterp * nvar;
+  }
+  DBuf<double> vals;
+  vals.reserve(tot);
+  DevStruct<QTables> dq;
+  QTables *pq = dq.put(qt, s_);
+  sync();
+  Timer tm(s_);
+  for (int rep = 0; rep < nrep; rep++) {
+    for (int ib = 0, off = 0; ib < nbk; ib++) {
+      Block &b = *blocks_[ib];
+      if (b.ninterp)
+        k_gather<<<nb(b.ninterp), BS, 0, s_>>>(b.ninterp, nvar, interptype, b.nnodes, b.interp_inode.p,
+                                               b.interp_w.p, b.interp_nw.p, b.interp_cancel.p, qdev_[ib]->p,
+                                               vals.p + (size_t)off * nvar);
+      off += b.ninterp;
+    }
+    for (int ib = 0, off = 0; ib < nbk; ib++) {
+      Block &b = *blocks_[ib];
+      if (b.ninterp)
+        k_scatter<<<nb(b.ninterp), BS, 0, s_>>>(b.ninterp, nvar, interptype, b.interp_cancel.p, b.interp_rnode.p,
+                                                b.interp_rblk.p, vals.p + (size_t)off * nvar, pq);
+      off += b.ninterp;
+    }
+  }
+  return tm.stop() / nrep;
+}
+
+}  // namespace tg

@@ -1,0 +1,250 @@
+"""Python handle on libtioga_amd.so through its C ABI (include/tioga_amd.h).
+
+:class:`Tioga` mirrors the reference's tiogaInterface call sequence
+(tiogaInterface.C:36-328): init, registergrid_data_mb, set*, preprocess_grids,
+performconnectivity, registersolution, dataupdate_mb, getdonorcount/info,
+delete.  Arrays are numpy arrays whose memory the library keeps by pointer,
+as the reference keeps the solver's arrays, so the handle holds references to
+them.  Every call checks the library's error status and raises on failure;
+there is no fallback path: without the built library or a GPU, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtioga_amd.so")
+
+_P = C.c_void_p
+_lib = None
+
+
+class TiogaError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the in-tree library (build it with tioga_amd.build first)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TiogaError(f"{LIB_PATH} is missing: run `python -m tioga_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.tioga_init_.argtypes = [C.c_int]
+        for name in ("tioga_preprocess_grids_", "tioga_performconnectivity_", "tioga_delete_",
+                     "tioga_reduce_fringes_"):
+            getattr(L, name).argtypes = []
+        L.tioga_registersolution_.argtypes = [_P, _P]
+        L.tioga_dataupdate_mb_.argtypes = [_P, C.c_char_p]
+        L.tioga_getdonorcount_.argtypes = [_P, _P, _P]
+        L.tioga_getdonorinfo_.argtypes = [_P, _P, _P, _P, _P]
+        L.tioga_setsymmetry_.argtypes = [_P]
+        L.tioga_setnfringe_.argtypes = [_P]
+        L.tioga_setmexclude_.argtypes = [_P]
+        L.tioga_setresolutions_multi_.argtypes = [_P, _P, _P]
+        L.tioga_setcelliblank_.argtypes = [_P]
+        L.tioga_amd_status.argtypes = [C.c_char_p, C.c_int]
+        L.tioga_amd_status.restype = C.c_int
+        L.tioga_amd_clear_status.argtypes = []
+        L.tioga_amd_block_array.argtypes = [C.c_int, C.c_char_p, _P]
+        L.tioga_amd_block_array.restype = C.c_long
+        L.tioga_amd_hole_map.argtypes = [C.c_int, _P, _P, _P]
+        L.tioga_amd_hole_map.restype = C.c_long
+        L.tioga_amd_time_search.argtypes = [C.c_int, _P]
+        L.tioga_amd_time_search.restype = C.c_double
+        L.tioga_amd_time_dataupdate.argtypes = [C.c_int, C.c_int, C.c_int, _P]
+        L.tioga_amd_time_dataupdate.restype = C.c_double
+        L.tioga_amd_phase_times.argtypes = [_P]
+        L.tioga_amd_weight_aborts.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def _ip(v: int):
+    return C.byref(C.c_int(v))
+
+
+_I32 = {"isearch", "tagsearch", "donorId", "xtag", "elementList", "adtIntegers", "iblank_cell"}
+PHASES = ("holemap", "boxes", "searchdata", "search_prep", "search_kernel", "donors", "total", "nsearch")
+
+
+class Tioga:
+    """One overset system on the current GPU (the C ABI keeps one global
+    instance, as the reference's `tg`)."""
+
+    def __init__(self):
+        self.L = load()
+        self.L.tioga_amd_clear_status()
+        self.L.tioga_init_(0)
+        self._check()
+        self._keep: List[object] = []
+        self.blocks = []
+        self.iblank: List[np.ndarray] = []
+        self.iblank_cell: List[np.ndarray] = []
+        self.q: Dict[int, np.ndarray] = {}
+
+    def _check(self):
+        buf = C.create_string_buffer(1024)
+        if self.L.tioga_amd_status(buf, 1024):
+            msg = buf.value.decode()
+            self.L.tioga_amd_clear_status()
+            raise TiogaError(msg)
+
+    def close(self):
+        if self.L is not None:
+            self.L.tioga_delete_()
+            self._check()
+            self.L = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- registration (tiogaInterface.C:108-130) ----
+    def register_grid(self, block, bid: Optional[int] = None) -> None:
+        b = block.normalized()
+        ib = np.ones(b.nnodes, np.int32)
+        args = []
+        for nv, c in b.types:
+            nvp, ncp = C.c_int(nv), C.c_int(c.shape[0])
+            self._keep += [nvp, ncp, c]
+            args += [C.byref(nvp), C.byref(ncp), _p(c)]
+        bid = len(self.blocks) + 1 if bid is None else bid
+        self._keep += [b, ib]
+        self.L.tioga_registergrid_data_mb_(_ip(bid), _ip(b.tag), _ip(b.nnodes), _p(b.x), _p(ib),
+                                           _ip(len(b.wbc)), _ip(len(b.obc)), _p(b.wbc), _p(b.obc),
+                                           _ip(len(b.types)), *args)
+        self._check()
+        if b.nres is not None:
+            self.L.tioga_setresolutions_multi_(_ip(b.tag), _p(b.nres), _p(b.cres))
+            self._check()
+        self.blocks.append(b)
+        self.iblank.append(ib)
+
+    def set_params(self, isym: int = 3, nfringe: int = 1, mexclude: int = 3):
+        self.L.tioga_setsymmetry_(_ip(isym))
+        self.L.tioga_setnfringe_(_ip(nfringe))
+        self.L.tioga_setmexclude_(_ip(mexclude))
+        self._check()
+
+    def set_coords(self, bi: int, x: np.ndarray) -> None:
+        """Move block bi in place (the library keeps the user's pointer)."""
+        self.blocks[bi].x[...] = x
+
+    def preprocess(self):
+        self.L.tioga_preprocess_grids_()
+        self._check()
+
+    def connect(self):
+        self.L.tioga_performconnectivity_()
+        self._check()
+
+    def register_solution(self, btag: int, q: np.ndarray):
+        self.q[btag] = q
+        self.L.tioga_registersolution_(_ip(btag), _p(q))
+        self._check()
+
+    def data_update(self, nvar: int, interptype: int = 0):
+        self.L.tioga_dataupdate_mb_(_ip(nvar), b"row" if interptype == 0 else b"column")
+        self._check()
+
+    def data_update_copies(self, qs: Sequence[np.ndarray], nvar: int, interptype: int = 0):
+        out = [np.ascontiguousarray(q, np.float64).copy() for q in qs]
+        for b, q in zip(self.blocks, out):
+            self.register_solution(b.tag, q)
+        self.data_update(nvar, interptype)
+        return out
+
+    def donor_info(self, btag: int):
+        d = np.zeros(1, np.int32)
+        f = np.zeros(1, np.int32)
+        self.L.tioga_getdonorcount_(_ip(btag), _p(d), _p(f))
+        self._check()
+        dc, fc = int(d[0]), int(f[0])
+        rec = np.zeros(4 * dc + 1, np.int32)
+        ind = np.zeros(fc + 1, np.int32)
+        frac = np.zeros(fc + 1, np.float64)
+        if dc > 0:
+            self.L.tioga_getdonorinfo_(_ip(btag), _p(rec), _p(ind), _p(frac), _p(d))
+            self._check()
+        return rec[:4 * dc], ind[:fc], frac[:fc]
+
+    # ---- extensions ----
+    def array(self, btag: int, name: str) -> np.ndarray:
+        n = self.L.tioga_amd_block_array(btag, name.encode(), None)
+        self._check()
+        out = np.zeros(max(n, 0), np.int32 if name in _I32 else np.float64)
+        if n > 0:
+            self.L.tioga_amd_block_array(btag, name.encode(), _p(out))
+            self._check()
+        return out
+
+    def hole_map(self, mtag: int):
+        nx = np.zeros(3, np.int32)
+        ext = np.zeros(6)
+        n = self.L.tioga_amd_hole_map(mtag, _p(nx), _p(ext), None)
+        self._check()
+        if n <= 0:
+            return None
+        sam = np.zeros(n, np.int32)
+        self.L.tioga_amd_hole_map(mtag, _p(nx), _p(ext), _p(sam))
+        self._check()
+        return nx, ext, sam
+
+    def time_search(self, nrep: int):
+        n = C.c_longlong(0)
+        ms = self.L.tioga_amd_time_search(nrep, C.byref(n))
+        self._check()
+        return ms, n.value
+
+    def time_data_update(self, nvar: int, interptype: int, nrep: int):
+        n = C.c_longlong(0)
+        ms = self.L.tioga_amd_time_dataupdate(nvar, interptype, nrep, C.byref(n))
+        self._check()
+        return ms, n.value
+
+    def phase_times(self) -> Dict[str, float]:
+        out = np.zeros(8)
+        self.L.tioga_amd_phase_times(_p(out))
+        self._check()
+        return dict(zip(PHASES, out.tolist()))
+
+    def weight_aborts(self) -> int:
+        return int(self.L.tioga_amd_weight_aborts())
+
+    def results(self, qs=None, nvar: int = 0, interptype: int = 0) -> Dict[str, np.ndarray]:
+        """Every array the reference driver dumps, under the same names."""
+        r: Dict[str, np.ndarray] = {}
+        for i, b in enumerate(self.blocks):
+            p = f"b{i}."
+            r[p + "iblank"] = self.iblank[i].copy()
+            r[p + "iblank_cell"] = self.array(b.tag, "iblank_cell")
+            for nm in ("nodeRes", "cellRes", "obb", "xsearch", "res_search", "isearch", "tagsearch",
+                       "donorId", "xtag", "elementList", "adtIntegers", "adtReals", "adtExtents"):
+                a = self.array(b.tag, nm)
+                if a.size or nm != "adtExtents":
+                    r[p + nm] = a
+            r[p + "nsearch"] = np.array([r[p + "donorId"].size], np.int32)
+            rec, ind, frac = self.donor_info(b.tag)
+            r[p + "dcount"] = np.array([rec.size // 4], np.int32)
+            r[p + "receptors"], r[p + "indices"], r[p + "frac"] = rec, ind, frac
+        mt = max(b.tag for b in self.blocks)
+        for m in range(1, mt + 1):
+            hm = self.hole_map(m)
+            r[f"hm{m}.existWall"] = np.array([0 if hm is None else 1], np.int32)
+            if hm is not None:
+                r[f"hm{m}.nx"], r[f"hm{m}.extents"], r[f"hm{m}.sam"] = hm
+        if qs is not None:
+            for i, q in enumerate(self.data_update_copies(qs, nvar, interptype)):
+                r[f"b{i}.q"] = q
+        return r
