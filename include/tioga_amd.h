@@ -121,6 +121,20 @@ double tioga_amd_time_search(int nrep, long long *nsearch);
    interpolation records per pass. */
 double tioga_amd_time_dataupdate(int nvar, int interptype, int nrep, long long *nreceptors);
 
+/* Query sets: nq points (xyz AoS, host memory) searched against block
+   btag, whose search structure is built the way MeshBlock::search builds it
+   (search.C:107-233: cells overlapping the query set's OBB, ADT) with no
+   coincident-point dedupe.  The block must have been preprocessed.
+   create returns a set id (>= 0, or -1 on error); search writes donor[nq]
+   (cell index or -1); time returns the device ms of one pass over all sets
+   (mean of nrep) and the total query count; clear frees every set. */
+int tioga_amd_query_set_create(int btag, const double *xq, long nq);
+void tioga_amd_query_set_search(int id, int *donor);
+double tioga_amd_time_query_sets(int nrep, long long *nq);
+void tioga_amd_query_set_clear(void);
+/* number of cells in a query set's search structure (-1: bad id) */
+int tioga_amd_query_set_nelem(int id);
+
 /* Host wall times (ms) of the last performconnectivity phases:
    out[0..6] = hole map, boxes, query gathering, search preparation,
    search kernels (device), donor selection, total; out[7] = searches. */

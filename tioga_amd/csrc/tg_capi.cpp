@@ -348,6 +348,37 @@ double tioga_amd_time_dataupdate(int nvar, int interptype, int nrep, long long *
   return r;
 }
 
+int tioga_amd_query_set_create(int btag, const double *xq, long nq)
+{
+  int r = -1;
+  guarded("tioga_amd_query_set_create", [&] { r = tg_or_throw().engine()->query_set_create(btag, xq, nq); });
+  return r;
+}
+
+void tioga_amd_query_set_search(int id, int *donor)
+{
+  guarded("tioga_amd_query_set_search", [&] { tg_or_throw().engine()->query_set_search(id, donor); });
+}
+
+double tioga_amd_time_query_sets(int nrep, long long *nq)
+{
+  double r = 0;
+  guarded("tioga_amd_time_query_sets", [&] { r = tg_or_throw().engine()->time_query_sets(nrep, nq); });
+  return r;
+}
+
+void tioga_amd_query_set_clear(void)
+{
+  guarded("tioga_amd_query_set_clear", [] { tg_or_throw().engine()->query_set_clear(); });
+}
+
+int tioga_amd_query_set_nelem(int id)
+{
+  int r = -1;
+  guarded("tioga_amd_query_set_nelem", [&] { r = tg_or_throw().engine()->query_set_nelem(id); });
+  return r;
+}
+
 void tioga_amd_phase_times(double *out)
 {
   guarded("tioga_amd_phase_times", [&] {

@@ -126,6 +126,19 @@ struct Block {
   bool interp_host_valid = false;
 };
 
+// A standalone donor search: nq points against one block, with the search
+// structure built the way MeshBlock::search builds it (cells overlapping the
+// query set's OBB, ADT) and no coincident-point dedupe.
+struct QuerySet {
+  int blk = 0, nq = 0;
+  DBuf<double> xq;
+  DBuf<int> donor;
+  int nelem = 0, adt_depth = 0;
+  DBuf<int> elist, adtInt;
+  DBuf<double> ebox, adtReal;
+  double ext[6] = {0, 0, 0, 0, 0, 0};
+};
+
 struct Pair {
   int local = 0, ob = 0, iblk_remote = 0, tag_remote = 0, send_tag = 0, recv_tag = 0;
   OBB obc{};
@@ -180,6 +193,14 @@ class Engine {
   // (q uploaded once from the registered arrays); mean device ms per pass.
   double time_data_update(int nvar, int interptype, int nrep, long long *nreceptors);
 
+  // Query sets (extension): create returns an id; search runs the kernel
+  // and copies donors out; time_query_sets times nrep passes over all sets.
+  int query_set_create(int btag, const double *xq, long nq);
+  void query_set_search(int id, int *donor);
+  double time_query_sets(int nrep, long long *nq);
+  void query_set_clear();
+  int query_set_nelem(int id) const { return (id >= 0 && id < (int)qsets_.size()) ? qsets_[id]->nelem : -1; }
+
   PhaseTimes last_times;
   int weight_aborts() const { return weight_aborts_; }
   hipStream_t stream() const { return s_; }
@@ -195,6 +216,10 @@ class Engine {
   void exchange_donors();
   void cell_iblanks(Block &b);
   void launch_search(Block &b);
+  void build_search_structure(Block &b, const double *xq_host, int nq, int &nelem, int &depth, DBuf<int> &elist,
+                              DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, double ext[6]);
+  void launch_query_set(QuerySet &q);
+  std::vector<QuerySet *> qsets_;
   void sync();
 
   hipStream_t s_ = nullptr;

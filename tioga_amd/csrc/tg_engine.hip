@@ -510,7 +510,7 @@ __global__ void __launch_bounds__(SEARCH_BS) k_search(SearchView S)
   __shared__ int stk[STACK][SEARCH_BS];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= S.nq) return;
-  if (S.xtag[i] != i) return;
+  if (S.xtag && S.xtag[i] != i) return;
   double p[3];
   load_xyz(S.xs, i, p);
   int res = -1, flag = 0;
@@ -966,6 +966,7 @@ Engine::Engine()
 
 Engine::~Engine()
 {
+  query_set_clear();
   for (Block *b : blocks_) delete b;
   for (auto *q : qdev_) delete q;
   if (s_) (void)hipStreamDestroy(s_);
@@ -1478,28 +1479,7 @@ void Engine::search(Block &b)
   TG_HIP(hipMemcpyAsync(rs.data(), b.res_search.p, sizeof(double) * Q, hipMemcpyDeviceToHost, s_));
   TG_HIP(hipMemcpyAsync(ts.data(), b.tagsearch.p, sizeof(int) * Q, hipMemcpyDeviceToHost, s_));
   sync();
-  OBB obq;
-  find_obb(xs.data(), nullptr, Q, obq);
-  // device: cell filter -> LIFO element list -> element boxes
-  const int N = b.ncells;
-  iscratch_.reserve(N);
-  iscratch2_.reserve(N);
-  if (N) k_cell_filter<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, b.x.p, obq, iscratch_.p);
-  int ne = select_flagged(iscratch_.p, N, iscratch2_.p, s_);
-  b.nelem = ne;
-  b.elist.reserve(ne);
-  b.ebox.reserve(6 * (size_t)ne);
-  b.adtInt.reserve(4 * (size_t)ne);
-  b.adtReal.reserve(6 * (size_t)ne);
-  if (ne) k_elem_box<<<nb(ne), BS, 0, s_>>>(ne, N, iscratch2_.p, b.ti, b.conn.p, b.x.p, b.elist.p, b.ebox.p);
-  std::vector<double> eb(6 * (size_t)ne), ar(6 * (size_t)ne);
-  std::vector<int> ai(4 * (size_t)ne);
-  TG_HIP(hipMemcpyAsync(eb.data(), b.ebox.p, sizeof(double) * eb.size(), hipMemcpyDeviceToHost, s_));
-  sync();
-  b.adt_depth = build_adt(ne, eb.data(), ai.data(), ar.data(), b.adtExt);
-  if (b.adt_depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
-  TG_HIP(hipMemcpyAsync(b.adtInt.p, ai.data(), sizeof(int) * ai.size(), hipMemcpyHostToDevice, s_));
-  TG_HIP(hipMemcpyAsync(b.adtReal.p, ar.data(), sizeof(double) * ar.size(), hipMemcpyHostToDevice, s_));
+  build_search_structure(b, xs.data(), Q, b.nelem, b.adt_depth, b.elist, b.ebox, b.adtInt, b.adtReal, b.adtExt);
   bool gid_mode = true;
   for (Block *o : blocks_) gid_mode = gid_mode && (o->unode_gid != nullptr);
   if (gid_mode) {
@@ -1527,6 +1507,106 @@ void Engine::search(Block &b)
   TG_HIP(hipMemcpyAsync(b.xtag.p, xt.data(), sizeof(int) * Q, hipMemcpyHostToDevice, s_));
   TG_HIP(hipMemcpyAsync(b.res_search.p, rs.data(), sizeof(double) * Q, hipMemcpyHostToDevice, s_));
   sync();
+}
+
+// search.C:107-233: query-set OBB -> cells whose OBB-frame box overlaps it,
+// as a LIFO list (descending cell index) -> element boxes -> ADT
+void Engine::build_search_structure(Block &b, const double *xq, int nq, int &nelem, int &depth,
+                                    DBuf<int> &elist, DBuf<double> &ebox, DBuf<int> &adtInt,
+                                    DBuf<double> &adtReal, double ext[6])
+{
+  OBB obq;
+  find_obb(xq, nullptr, nq, obq);
+  const int N = b.ncells;
+  iscratch_.reserve(N);
+  iscratch2_.reserve(N);
+  if (N) k_cell_filter<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, b.x.p, obq, iscratch_.p);
+  int ne = select_flagged(iscratch_.p, N, iscratch2_.p, s_);
+  nelem = ne;
+  elist.reserve(ne);
+  ebox.reserve(6 * (size_t)ne);
+  adtInt.reserve(4 * (size_t)ne);
+  adtReal.reserve(6 * (size_t)ne);
+  if (ne) k_elem_box<<<nb(ne), BS, 0, s_>>>(ne, N, iscratch2_.p, b.ti, b.conn.p, b.x.p, elist.p, ebox.p);
+  std::vector<double> eb(6 * (size_t)ne), ar(6 * (size_t)ne);
+  std::vector<int> ai(4 * (size_t)ne);
+  TG_HIP(hipMemcpyAsync(eb.data(), ebox.p, sizeof(double) * eb.size(), hipMemcpyDeviceToHost, s_));
+  sync();
+  depth = build_adt(ne, eb.data(), ai.data(), ar.data(), ext);
+  if (depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
+  TG_HIP(hipMemcpyAsync(adtInt.p, ai.data(), sizeof(int) * ai.size(), hipMemcpyHostToDevice, s_));
+  TG_HIP(hipMemcpyAsync(adtReal.p, ar.data(), sizeof(double) * ar.size(), hipMemcpyHostToDevice, s_));
+  sync();
+}
+
+int Engine::query_set_create(int btag, const double *xq, long nq)
+{
+  int ib;
+  Block *b = find(btag, &ib);
+  if (!b) throw std::runtime_error("tioga_amd: query set for an unknown tag");
+  if (!b->topo_uploaded || b->cellRes.p == nullptr) throw std::runtime_error("tioga_amd: preprocess first");
+  if (nq <= 0 || nq > 0x7fffffffL) throw std::runtime_error("tioga_amd: query count out of range");
+  QuerySet *q = new QuerySet;
+  q->blk = ib;
+  q->nq = (int)nq;
+  q->xq.reserve(3 * (size_t)nq);
+  q->donor.reserve(nq);
+  TG_HIP(hipMemcpyAsync(q->xq.p, xq, sizeof(double) * 3 * nq, hipMemcpyHostToDevice, s_));
+  build_search_structure(*b, xq, (int)nq, q->nelem, q->adt_depth, q->elist, q->ebox, q->adtInt, q->adtReal,
+                         q->ext);
+  qsets_.push_back(q);
+  return (int)qsets_.size() - 1;
+}
+
+void Engine::launch_query_set(QuerySet &q)
+{
+  Block &b = *blocks_[q.blk];
+  if (q.nelem == 0) {
+    fill<int>(q.donor.p, q.nq, -1, s_);
+    return;
+  }
+  SearchView S;
+  S.nq = q.nq;
+  S.xs = q.xq.p;
+  S.xtag = nullptr;
+  S.donorId = q.donor.p;
+  S.adtInt = q.adtInt.p;
+  S.adtReal = q.adtReal.p;
+  S.ebox = q.ebox.p;
+  S.elist = q.elist.p;
+  memcpy(S.ext, q.ext, sizeof S.ext);
+  S.ti = b.ti;
+  S.conn = b.conn.p;
+  S.x = b.x.p;
+  S.cellRes = b.cellRes.p;
+  k_search<<<nb(q.nq, SEARCH_BS), SEARCH_BS, 0, s_>>>(S);
+}
+
+void Engine::query_set_search(int id, int *donor)
+{
+  if (id < 0 || id >= (int)qsets_.size()) throw std::runtime_error("tioga_amd: bad query set id");
+  QuerySet &q = *qsets_[id];
+  launch_query_set(q);
+  TG_HIP(hipMemcpyAsync(donor, q.donor.p, sizeof(int) * q.nq, hipMemcpyDeviceToHost, s_));
+  sync();
+}
+
+double Engine::time_query_sets(int nrep, long long *nq)
+{
+  long long n = 0;
+  for (QuerySet *q : qsets_) n += q->nq;
+  if (nq) *nq = n;
+  if (nrep <= 0 || qsets_.empty()) return 0.0;
+  Timer tm(s_);
+  for (int r = 0; r < nrep; r++)
+    for (QuerySet *q : qsets_) launch_query_set(*q);
+  return tm.stop() / nrep;
+}
+
+void Engine::query_set_clear()
+{
+  for (QuerySet *q : qsets_) delete q;
+  qsets_.clear();
 }
 
 void Engine::launch_search(Block &b)

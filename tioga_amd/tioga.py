@@ -59,6 +59,14 @@ def load():
         L.tioga_amd_time_dataupdate.argtypes = [C.c_int, C.c_int, C.c_int, _P]
         L.tioga_amd_time_dataupdate.restype = C.c_double
         L.tioga_amd_phase_times.argtypes = [_P]
+        L.tioga_amd_query_set_create.argtypes = [C.c_int, _P, C.c_long]
+        L.tioga_amd_query_set_create.restype = C.c_int
+        L.tioga_amd_query_set_search.argtypes = [C.c_int, _P]
+        L.tioga_amd_time_query_sets.argtypes = [C.c_int, _P]
+        L.tioga_amd_time_query_sets.restype = C.c_double
+        L.tioga_amd_query_set_clear.argtypes = []
+        L.tioga_amd_query_set_nelem.argtypes = [C.c_int]
+        L.tioga_amd_query_set_nelem.restype = C.c_int
         L.tioga_amd_weight_aborts.restype = C.c_int
         _lib = L
     return _lib
@@ -210,6 +218,29 @@ class Tioga:
     def time_data_update(self, nvar: int, interptype: int, nrep: int):
         n = C.c_longlong(0)
         ms = self.L.tioga_amd_time_dataupdate(nvar, interptype, nrep, C.byref(n))
+        self._check()
+        return ms, n.value
+
+    def query_set(self, btag: int, xq: np.ndarray) -> int:
+        xq = np.ascontiguousarray(xq, np.float64).reshape(-1, 3)
+        i = self.L.tioga_amd_query_set_create(btag, _p(xq), xq.shape[0])
+        self._check()
+        return i
+
+    def query_search(self, qid: int, nq: int) -> np.ndarray:
+        out = np.zeros(nq, np.int32)
+        self.L.tioga_amd_query_set_search(qid, _p(out))
+        self._check()
+        return out
+
+    def query_nelem(self, qid: int) -> int:
+        n = self.L.tioga_amd_query_set_nelem(qid)
+        self._check()
+        return int(n)
+
+    def time_query_sets(self, nrep: int):
+        n = C.c_longlong(0)
+        ms = self.L.tioga_amd_time_query_sets(nrep, C.byref(n))
         self._check()
         return ms, n.value
 
