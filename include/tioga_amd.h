@@ -140,6 +140,18 @@ int tioga_amd_query_set_nelem(int id);
    search kernels (device), donor selection, total; out[7] = searches. */
 void tioga_amd_phase_times(double *out);
 
+/* Host-side sequential stages, callable without a device (they are what
+   the engine runs on the host; exposed so they can be checked on their own):
+   median_select of median.F90:59-210 on x[0..n) permuting ix (lists != 0
+   runs the list-based partition rounds the device build uses); the
+   principal-axis OBB of utils.c:25-165 (out = xc[3], dxc[3], vec[3][3]);
+   the ADT of ADT.C:34-135 (returns the depth); the coincident-point dedupe
+   of utils.c:620-737. */
+void tioga_amd_host_median(int *ix, double *x, int n, int lists);
+void tioga_amd_host_find_obb(const double *x, int n, double *out);
+int tioga_amd_host_build_adt(int nelem, const double *ebox, int *adtInt, double *adtReal, double *ext);
+void tioga_amd_host_unique_nodes(const double *x, const int *meshtag, double *rtag, int *xtag, int n);
+
 /* Donor weights outside [-0.2, 1.2] seen (the reference aborts on one). */
 int tioga_amd_weight_aborts(void);
 

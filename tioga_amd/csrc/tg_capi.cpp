@@ -12,6 +12,7 @@
 #include "../../include/tioga.h"
 #include "../../include/tioga_amd.h"
 #include "tg_engine.h"
+#include "tg_seq.h"
 
 namespace {
 
@@ -392,6 +393,31 @@ void tioga_amd_phase_times(double *out)
     out[6] = t.total;
     out[7] = (double)t.nsearch;
   });
+}
+
+void tioga_amd_host_median(int *ix, double *x, int n, int lists)
+{
+  if (lists) tg::median_select_lists(ix, x, n);
+  else tg::median_select(ix, x, n);
+}
+
+void tioga_amd_host_find_obb(const double *x, int n, double *out)
+{
+  tg::OBB o;
+  tg::find_obb(x, nullptr, n, o);
+  memcpy(out, o.xc, 24);
+  memcpy(out + 3, o.dxc, 24);
+  memcpy(out + 6, o.vec, 72);
+}
+
+int tioga_amd_host_build_adt(int nelem, const double *ebox, int *adtInt, double *adtReal, double *ext)
+{
+  return tg::build_adt(nelem, ebox, adtInt, adtReal, ext);
+}
+
+void tioga_amd_host_unique_nodes(const double *x, const int *meshtag, double *rtag, int *xtag, int n)
+{
+  tg::unique_nodes_octree(x, meshtag, rtag, xtag, n);
 }
 
 int tioga_amd_weight_aborts(void)

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <string.h>
 
 #define TG_HD __host__ __device__ __forceinline__
 
@@ -221,6 +222,132 @@ TG_HD double cell_volume(const double xv[8][3], int nvert)
     }
   }
   return vol / 3.0;
+}
+
+// Partial quicksort that isolates the median of X[1..n] and permutes IX
+// alongside (median.F90:59-210; 1-based like the Fortran).  Positions
+// (n+1)/2 and, for even n, n/2+1 hold their sorted values on return; the
+// exact permutation it leaves decides the ADT's shape, so every step is kept.
+TG_HD void median_select(int *ix0, double *x0, int n)
+{
+  int *IX = ix0 - 1;
+  double *X = x0 - 1;
+  auto swap = [&](int a, int b) {
+    double t = X[a];
+    X[a] = X[b];
+    X[b] = t;
+    int it = IX[a];
+    IX[a] = IX[b];
+    IX[b] = it;
+  };
+  const int half = n / 2, halfp1 = half + 1;
+  const bool odd = (n != 2 * half);
+  if (n < 3) {
+    if (n == 2 && X[2] < X[1]) swap(1, 2);
+    return;
+  }
+  int lo = 1, hi = n;
+  for (;;) {
+    int mid = (lo + hi) / 2;
+    double pv = X[mid], a = X[lo], b = X[hi];
+    if (b < a) {
+      double t = b;
+      b = a;
+      a = t;
+    }
+    if (pv > b) pv = b;
+    else if (pv < a) pv = a;
+    int i = lo, j = hi;
+    for (;;) {
+      while (!(X[i] >= pv)) i++;
+      while (!(X[j] <= pv)) j--;
+      if (i < j) {
+        swap(i, j);
+        i++;
+        j--;
+        if (i <= j) continue;
+      }
+      break;
+    }
+    if (!odd) {
+      if (j == half && i == halfp1) return;
+      if (j < half) lo = i;
+      if (i > halfp1) hi = j;
+      if (i == j) {
+        if (i == half) lo = half;
+        if (j == halfp1) hi = halfp1;
+      }
+    } else {
+      if (j < halfp1) lo = i;
+      if (i > halfp1) hi = j;
+      if (i == j && i == halfp1) return;
+    }
+    if (lo < hi - 1) continue;
+    break;
+  }
+  if (!odd) {
+    if (X[halfp1] < X[half]) swap(half, halfp1);
+    return;
+  }
+  if (X[lo] > X[hi]) swap(lo, hi);
+}
+
+// One round of median_select's partition loop stated over lists, the form
+// the device runs with a whole workgroup.  A = positions in [lo,hi] with
+// X >= pv (ascending), B = positions with X <= pv (descending).  The scan
+// pointers of the sequential loop meet the pairs (A[m], B[m]) in order and
+// swap them while A[m] < B[m]; after K swaps they stop at
+//   i = A[K] if A[K] < B[K-1] else B[K-1],  j = B[K] if B[K] > A[K-1] else A[K-1]
+// (i = A[0], j = B[0] when K = 0).  Given K and the four list entries this
+// returns the (i, j) the sequential loop ends with.
+TG_HD void hoare_stop(int K, int nA, int nB, int aK, int aK1, int bK, int bK1, int a0, int b0, int &i, int &j)
+{
+  if (K == 0) {
+    i = a0;
+    j = b0;
+    return;
+  }
+  i = (K < nA && aK < bK1) ? aK : bK1;
+  j = (K < nB && bK > aK1) ? bK : aK1;
+}
+
+// The loop-control step of median_select after a partition round ending at
+// (i, j): narrows [lo, hi]; returns 1 when the median is in place (the
+// function returns), 2 when the loop falls through to the final fix-up, 0
+// to partition again.
+TG_HD int median_control(int n, int i, int j, int &lo, int &hi)
+{
+  const int half = n / 2, halfp1 = half + 1;
+  const bool odd = (n != 2 * half);
+  if (!odd) {
+    if (j == half && i == halfp1) return 1;
+    if (j < half) lo = i;
+    if (i > halfp1) hi = j;
+    if (i == j) {
+      if (i == half) lo = half;
+      if (j == halfp1) hi = halfp1;
+    }
+  } else {
+    if (j < halfp1) lo = i;
+    if (i > halfp1) hi = j;
+    if (i == j && i == halfp1) return 1;
+  }
+  return (lo < hi - 1) ? 0 : 2;
+}
+
+// median_select's pivot: X[mid] clamped to [min, max] of X[lo], X[hi]
+TG_HD double median_pivot(const double *X1, int lo, int hi)
+{
+  int mid = (lo + hi) / 2;
+  double pv = X1[mid], a = X1[lo], b = X1[hi];
+  if (b < a) {
+    double t = b;
+    b = a;
+    a = t;
+  }
+  if (pv > b) pv = b;
+  else if (pv < a) pv = a;
+  return pv;
 }
 
 // Oriented bounding box: centre, half extents, rows of vec = axes

@@ -61,6 +61,16 @@ struct TypeInfo {
   int cstart[MAXTYPES + 1];
 };
 
+// One ADT node as the search kernel reads it: 128 bytes, node ids in the
+// ADT's preorder.  sbox bounds the node's subtree, ebox is the box of the
+// node's own element; left/right are child node ids (-1: none).
+struct __align__(16) AdtNode {
+  double sbox[6];
+  double ebox[6];
+  int cell, left, right, nv;
+  int pad[4];
+};
+
 // Hole map of one mesh tag as the kernels see it.
 struct HoleMapView {
   const unsigned char *sam;  // nx*ny*nz, 1 = inside the hole
@@ -113,6 +123,7 @@ struct Block {
   int nelem = 0, adt_depth = 0;
   DBuf<int> elist, adtInt;
   DBuf<double> ebox, adtReal;
+  DBuf<AdtNode> nodes;
   double adtExt[6] = {0, 0, 0, 0, 0, 0};
 
   // donor bookkeeping (receptor side: one entry per node)
@@ -136,6 +147,7 @@ struct QuerySet {
   int nelem = 0, adt_depth = 0;
   DBuf<int> elist, adtInt;
   DBuf<double> ebox, adtReal;
+  DBuf<AdtNode> nodes;
   double ext[6] = {0, 0, 0, 0, 0, 0};
 };
 
@@ -217,7 +229,8 @@ class Engine {
   void cell_iblanks(Block &b);
   void launch_search(Block &b);
   void build_search_structure(Block &b, const double *xq_host, int nq, int &nelem, int &depth, DBuf<int> &elist,
-                              DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, double ext[6]);
+                              DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, DBuf<AdtNode> &nodes,
+                              double ext[6]);
   void launch_query_set(QuerySet &q);
   std::vector<QuerySet *> qsets_;
   void sync();

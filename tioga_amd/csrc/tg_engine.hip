@@ -333,7 +333,9 @@ __global__ void k_reduced_obb(int ncells, TypeInfo ti, const int *__restrict__ c
         for (int j = 0; j < 3; j++) hi[j] = dmax(hi[j], xd[j]);
       }
   }
-  // wave-level reduction, then one atomic per wave
+  // wave-level, then block-level reduction, then one atomic per block
+  __shared__ unsigned long long red[6][BS / 64];
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int j = 0; j < 3; j++) {
     unsigned long long kl = dkey(lo[j]), kh = dkey(hi[j]);
     for (int o = 32; o > 0; o >>= 1) {
@@ -342,9 +344,17 @@ __global__ void k_reduced_obb(int ncells, TypeInfo ti, const int *__restrict__ c
       kh = b > kh ? b : kh;
     }
     if ((threadIdx.x & 63) == 0) {
-      atomicMin(&out[j], kl);
-      atomicMax(&out[j + 3], kh);
+      red[j][w] = kl;
+      red[j + 3][w] = kh;
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int j = threadIdx.x;
+    unsigned long long k = red[j][0];
+    for (int q = 1; q < nw; q++) k = (j < 3) ? (red[j][q] < k ? red[j][q] : k) : (red[j][q] > k ? red[j][q] : k);
+    if (j < 3) atomicMin(&out[j], k);
+    else atomicMax(&out[j], k);
   }
 }
 
@@ -450,19 +460,36 @@ __global__ void k_elem_box(int nelem, int ncells, const int *__restrict__ sel, T
   }
 }
 
+static_assert(sizeof(AdtNode) == 128, "ADT node record must be 128 bytes");
+
+// ADT arrays (adtIntegers, adtReals, element boxes) -> node records
+__global__ void k_adt_nodes(int nelem, TypeInfo ti, const int *__restrict__ adtInt,
+                            const double *__restrict__ adtReal, const double *__restrict__ ebox,
+                            const int *__restrict__ elist, AdtNode *__restrict__ nodes)
+{
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nelem) return;
+  int4 a = reinterpret_cast<const int4 *>(adtInt)[n];
+  AdtNode N;
+  for (int j = 0; j < 6; j++) {
+    N.sbox[j] = adtReal[6 * (size_t)n + j];
+    N.ebox[j] = ebox[6 * (size_t)a.x + j];
+  }
+  N.cell = elist[a.x];
+  N.nv = ti.nv[type_of(ti, N.cell)];
+  N.left = a.y > -1 ? adtInt[4 * (size_t)a.y + 3] : -1;
+  N.right = a.z > -1 ? adtInt[4 * (size_t)a.z + 3] : -1;
+  for (int j = 0; j < 4; j++) N.pad[j] = 0;
+  nodes[n] = N;
+}
+
 struct SearchView {
   int nq;
   const double *xs;
   const int *xtag;
   int *donorId;
-  // ADT
-  const int *adtInt;
-  const double *adtReal;
-  const double *ebox;
-  const int *elist;
+  const AdtNode *nodes;
   double ext[6];
-  // donor block cells
-  TypeInfo ti;
   const int *conn;
   const double *x;
   const double *cellRes;
@@ -474,26 +501,38 @@ __device__ __forceinline__ bool in_box(const double p[3], const double *b)
          p[1] <= b[4] + TOL && p[2] <= b[5] + TOL;
 }
 
-// MeshBlock::checkContainment (checkContainment.C:27-96): weights of p in
-// the element's cell; accept when every weight lies in [-TOL, 1+TOL]; flag a
-// near-zero weight in a BIGVALUE cell, which keeps the search going.
-__device__ __forceinline__ void contain(const SearchView &S, int e, const double p[3], int &res, int &flag)
+// MeshBlock::checkContainment (checkContainment.C:27-96) for a cell with NV
+// vertices: weights of p; accept when every weight lies in [-TOL, 1+TOL];
+// flag a near-zero weight in a BIGVALUE cell, which keeps the search going.
+// NV is a template constant so the vertex and weight arrays stay in registers.
+template <int NV>
+__device__ __forceinline__ void contain_t(const SearchView &S, int c, const double p[3], int &res, int &flag)
 {
-  int c = S.elist[e];
-  int nv = S.ti.nv[type_of(S.ti, c)];
   const int *cv = S.conn + 8 * (size_t)c;
   double xv[8][3], frac[8];
-  for (int m = 0; m < nv; m++) load_xyz(S.x, cv[m], xv[m]);
-  nodal_weights(xv, p, frac, nv);
+#pragma unroll
+  for (int m = 0; m < NV; m++) load_xyz(S.x, cv[m], xv[m]);
+  nodal_weights(xv, p, frac, NV);
   res = c;
   flag = 0;
   bool big = (S.cellRes[c] == BIGVALUE);
-  for (int m = 0; m < nv; m++) {
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
     if ((frac[m] + TOL) * (frac[m] - 1.0 - TOL) > 0) {
       res = -1;
       return;
     }
     if (fabs(frac[m]) < TOL && big) flag = 1;
+  }
+}
+
+__device__ __forceinline__ void contain(const SearchView &S, int c, int nv, const double p[3], int &res, int &flag)
+{
+  switch (nv) {
+    case 4: contain_t<4>(S, c, p, res, flag); break;
+    case 5: contain_t<5>(S, c, p, res, flag); break;
+    case 6: contain_t<6>(S, c, p, res, flag); break;
+    default: contain_t<8>(S, c, p, res, flag); break;
   }
 }
 
@@ -505,7 +544,10 @@ constexpr int STACK = 32;
 // its left and right subtrees whose boxes hold the point.  Every visited
 // element overwrites the result; the walk stops at the first clean hit.  The
 // per-thread stacks live in LDS, strided by the block size.
-__global__ void __launch_bounds__(SEARCH_BS) k_search(SearchView S)
+#ifndef TG_SEARCH_WAVES
+#define TG_SEARCH_WAVES 4
+#endif
+__global__ void __launch_bounds__(SEARCH_BS, TG_SEARCH_WAVES) k_search(SearchView S)
 {
   __shared__ int stk[STACK][SEARCH_BS];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -522,20 +564,14 @@ __global__ void __launch_bounds__(SEARCH_BS) k_search(SearchView S)
     int sp = 0;
     stk[sp++][t] = 0;
     while (sp > 0) {
-      int node = stk[--sp][t];
-      int4 a = reinterpret_cast<const int4 *>(S.adtInt)[node];
-      if (in_box(p, S.ebox + 6 * (size_t)a.x)) {
-        contain(S, a.x, p, res, flag);
+      const AdtNode &N = S.nodes[stk[--sp][t]];
+      if (in_box(p, N.ebox)) {
+        contain(S, N.cell, N.nv, p, res, flag);
         if (res > -1 && flag == 0) break;
       }
-      if (a.z > -1) {
-        int rn = S.adtInt[4 * (size_t)a.z + 3];
-        if (in_box(p, S.adtReal + 6 * (size_t)rn)) stk[sp++][t] = rn;
-      }
-      if (a.y > -1) {
-        int ln = S.adtInt[4 * (size_t)a.y + 3];
-        if (in_box(p, S.adtReal + 6 * (size_t)ln)) stk[sp++][t] = ln;
-      }
+      int r = N.right, l = N.left;
+      if (r > -1 && in_box(p, S.nodes[r].sbox)) stk[sp++][t] = r;
+      if (l > -1 && in_box(p, S.nodes[l].sbox)) stk[sp++][t] = l;
     }
   }
   S.donorId[i] = res;
@@ -768,13 +804,14 @@ __global__ void k_interp(InterpView V)
   nodal_weights(xv, xp, frac, nv);
   V.i2d[irec] = r;
   int bad = 0;
+  const size_t n = V.n;
   for (int m = 0; m < nv; m++) {
-    V.inode[9 * (size_t)r + m] = cv[m];
-    V.w[9 * (size_t)r + m] = frac[m];
+    V.inode[m * n + r] = cv[m];
+    V.w[m * n + r] = frac[m];
     if (frac[m] < -0.2 || frac[m] > 1.2) bad = 1;
   }
-  V.inode[9 * (size_t)r + nv] = c;
-  V.w[9 * (size_t)r + nv] = 0.0;
+  V.inode[nv * n + r] = c;
+  V.w[nv * n + r] = 0.0;
   V.nw[r] = nv;
   V.cancel[r] = (accept == 0 && rr != BIGVALUE) ? 1 : 0;
   V.rnode[r] = V.isearch[3 * irec + 1];
@@ -868,20 +905,32 @@ __global__ void k_cell_iblank(int ncells, TypeInfo ti, const int *__restrict__ c
 // updateSolnData (dataUpdate.C:105-119)
 // ---------------------------------------------------------------------------
 
+// Records are stored structure-of-arrays (slot m of record r at m*n + r), so
+// a wave's index and weight loads are coalesced; the sum over the weights
+// runs in slot order m = 0..nw-1 as the reference sums it.
 __global__ void k_gather(int n, int nvar, int interptype, int nnodes, const int *__restrict__ inode,
                          const double *__restrict__ w, const int *__restrict__ nw, const int *__restrict__ cancel,
                          const double *__restrict__ q, double *__restrict__ out)
 {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n || cancel[r]) return;
-  int m_n = nw[r];
+  const int m_n = nw[r];
+  int v[8];
+  double wt[8];
+#pragma unroll
+  for (int m = 0; m < 8; m++)
+    if (m < m_n) {
+      v[m] = inode[(size_t)m * n + r];
+      wt[m] = w[(size_t)m * n + r];
+    }
   for (int k = 0; k < nvar; k++) {
     double acc = 0;
-    for (int m = 0; m < m_n; m++) {
-      int v = inode[9 * (size_t)r + m];
-      double qv = interptype == 0 ? q[(size_t)v * nvar + k] : q[(size_t)k * nnodes + v];
-      acc += qv * w[9 * (size_t)r + m];
-    }
+#pragma unroll
+    for (int m = 0; m < 8; m++)
+      if (m < m_n) {
+        double qv = interptype == 0 ? q[(size_t)v[m] * nvar + k] : q[(size_t)k * nnodes + v[m]];
+        acc += qv * wt[m];
+      }
     out[(size_t)r * nvar + k] = acc;
   }
 }
@@ -1479,7 +1528,8 @@ void Engine::search(Block &b)
   TG_HIP(hipMemcpyAsync(rs.data(), b.res_search.p, sizeof(double) * Q, hipMemcpyDeviceToHost, s_));
   TG_HIP(hipMemcpyAsync(ts.data(), b.tagsearch.p, sizeof(int) * Q, hipMemcpyDeviceToHost, s_));
   sync();
-  build_search_structure(b, xs.data(), Q, b.nelem, b.adt_depth, b.elist, b.ebox, b.adtInt, b.adtReal, b.adtExt);
+  build_search_structure(b, xs.data(), Q, b.nelem, b.adt_depth, b.elist, b.ebox, b.adtInt, b.adtReal, b.nodes,
+                         b.adtExt);
   bool gid_mode = true;
   for (Block *o : blocks_) gid_mode = gid_mode && (o->unode_gid != nullptr);
   if (gid_mode) {
@@ -1513,7 +1563,7 @@ void Engine::search(Block &b)
 // as a LIFO list (descending cell index) -> element boxes -> ADT
 void Engine::build_search_structure(Block &b, const double *xq, int nq, int &nelem, int &depth,
                                     DBuf<int> &elist, DBuf<double> &ebox, DBuf<int> &adtInt,
-                                    DBuf<double> &adtReal, double ext[6])
+                                    DBuf<double> &adtReal, DBuf<AdtNode> &nodes, double ext[6])
 {
   OBB obq;
   find_obb(xq, nullptr, nq, obq);
@@ -1536,6 +1586,8 @@ void Engine::build_search_structure(Block &b, const double *xq, int nq, int &nel
   if (depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
   TG_HIP(hipMemcpyAsync(adtInt.p, ai.data(), sizeof(int) * ai.size(), hipMemcpyHostToDevice, s_));
   TG_HIP(hipMemcpyAsync(adtReal.p, ar.data(), sizeof(double) * ar.size(), hipMemcpyHostToDevice, s_));
+  nodes.reserve(ne);
+  if (ne) k_adt_nodes<<<nb(ne), BS, 0, s_>>>(ne, b.ti, adtInt.p, adtReal.p, ebox.p, elist.p, nodes.p);
   sync();
 }
 
@@ -1553,7 +1605,7 @@ int Engine::query_set_create(int btag, const double *xq, long nq)
   q->donor.reserve(nq);
   TG_HIP(hipMemcpyAsync(q->xq.p, xq, sizeof(double) * 3 * nq, hipMemcpyHostToDevice, s_));
   build_search_structure(*b, xq, (int)nq, q->nelem, q->adt_depth, q->elist, q->ebox, q->adtInt, q->adtReal,
-                         q->ext);
+                         q->nodes, q->ext);
   qsets_.push_back(q);
   return (int)qsets_.size() - 1;
 }
@@ -1570,12 +1622,8 @@ void Engine::launch_query_set(QuerySet &q)
   S.xs = q.xq.p;
   S.xtag = nullptr;
   S.donorId = q.donor.p;
-  S.adtInt = q.adtInt.p;
-  S.adtReal = q.adtReal.p;
-  S.ebox = q.ebox.p;
-  S.elist = q.elist.p;
+  S.nodes = q.nodes.p;
   memcpy(S.ext, q.ext, sizeof S.ext);
-  S.ti = b.ti;
   S.conn = b.conn.p;
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
@@ -1617,12 +1665,8 @@ void Engine::launch_search(Block &b)
   S.xs = b.xsearch.p;
   S.xtag = b.xtag.p;
   S.donorId = b.donorId.p;
-  S.adtInt = b.adtInt.p;
-  S.adtReal = b.adtReal.p;
-  S.ebox = b.ebox.p;
-  S.elist = b.elist.p;
+  S.nodes = b.nodes.p;
   memcpy(S.ext, b.adtExt, sizeof S.ext);
-  S.ti = b.ti;
   S.conn = b.conn.p;
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
@@ -1967,8 +2011,8 @@ static void fetch_interp(Block &b, hipStream_t s)
     h.rinfo[1] = rnode[r];
     h.rinfo[2] = rblk[r];
     for (int m = 0; m < 9; m++) {
-      h.inode[m] = inode[9 * (size_t)r + m];
-      h.w[m] = w[9 * (size_t)r + m];
+      h.inode[m] = m <= h.nweights ? inode[(size_t)m * n + r] : 0;
+      h.w[m] = m <= h.nweights ? w[(size_t)m * n + r] : 0.0;
     }
   }
   b.interp_host_valid = true;

@@ -4,6 +4,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <utility>
 #include <vector>
 
 namespace tg {
@@ -106,8 +107,9 @@ void find_obb(const double *x, const int *idx, int n, OBB &o)
         o.dxc[j] = (1e-3 > d) ? 1e-3 : d * 0.5;
       }
     } else {
+      // utils.c:71-77 measures every coordinate from the first point's x
       for (int i = 0; i < n; i++)
-        for (int j = 0; j < 3; j++) o.dxc[j] = dmax(1e-3, fabs(P(i)[j] - P(0)[j]));
+        for (int j = 0; j < 3; j++) o.dxc[j] = dmax(1e-3, fabs(P(i)[j] - P(0)[0]));
     }
     return;
   }
@@ -198,22 +200,13 @@ void obb_corners(const double xc[3], const double dxc[3], const double vec[3][3]
   }
 }
 
-namespace {
-
-// Partial quicksort that isolates the median of X[1..n] and permutes IX
-// alongside (median.F90:59-210; 1-based like the Fortran).  Positions
-// (n+1)/2 and, for even n, n/2+1 hold their sorted values on return.
-void median_select(int *ix0, double *x0, int n)
+void median_select_lists(int *ix0, double *x0, int n)
 {
   int *IX = ix0 - 1;
   double *X = x0 - 1;
   auto swap = [&](int a, int b) {
-    double t = X[a];
-    X[a] = X[b];
-    X[b] = t;
-    int it = IX[a];
-    IX[a] = IX[b];
-    IX[b] = it;
+    std::swap(X[a], X[b]);
+    std::swap(IX[a], IX[b]);
   };
   const int half = n / 2, halfp1 = half + 1;
   const bool odd = (n != 2 * half);
@@ -222,43 +215,24 @@ void median_select(int *ix0, double *x0, int n)
     return;
   }
   int lo = 1, hi = n;
+  std::vector<int> A, B;
   for (;;) {
-    int mid = (lo + hi) / 2;
-    double pv = X[mid], a = X[lo], b = X[hi];
-    if (b < a) {
-      double t = b;
-      b = a;
-      a = t;
-    }
-    if (pv > b) pv = b;
-    else if (pv < a) pv = a;
-    int i = lo, j = hi;
-    for (;;) {
-      while (!(X[i] >= pv)) i++;
-      while (!(X[j] <= pv)) j--;
-      if (i < j) {
-        swap(i, j);
-        i++;
-        j--;
-        if (i <= j) continue;
-      }
-      break;
-    }
-    if (!odd) {
-      if (j == half && i == halfp1) return;
-      if (j < half) lo = i;
-      if (i > halfp1) hi = j;
-      if (i == j) {
-        if (i == half) lo = half;
-        if (j == halfp1) hi = halfp1;
-      }
-    } else {
-      if (j < halfp1) lo = i;
-      if (i > halfp1) hi = j;
-      if (i == j && i == halfp1) return;
-    }
-    if (lo < hi - 1) continue;
-    break;
+    double pv = median_pivot(X, lo, hi);
+    A.clear();
+    B.clear();
+    for (int p = lo; p <= hi; p++)
+      if (X[p] >= pv) A.push_back(p);
+    for (int p = hi; p >= lo; p--)
+      if (X[p] <= pv) B.push_back(p);
+    int nA = (int)A.size(), nB = (int)B.size(), K = 0;
+    while (K < nA && K < nB && A[K] < B[K]) K++;
+    for (int m = 0; m < K; m++) swap(A[m], B[m]);
+    int i, j;
+    hoare_stop(K, nA, nB, K < nA ? A[K] : 0, K > 0 ? A[K - 1] : 0, K < nB ? B[K] : 0, K > 0 ? B[K - 1] : 0, A[0],
+               B[0], i, j);
+    int st = median_control(n, i, j, lo, hi);
+    if (st == 1) return;
+    if (st == 2) break;
   }
   if (!odd) {
     if (X[halfp1] < X[half]) swap(half, halfp1);
@@ -266,6 +240,8 @@ void median_select(int *ix0, double *x0, int n)
   }
   if (X[lo] > X[hi]) swap(lo, hi);
 }
+
+namespace {
 
 struct AdtBuilder {
   const double *box;

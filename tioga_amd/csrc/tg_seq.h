@@ -30,6 +30,10 @@ void obb_corners(const double xc[3], const double dxc[3], const double vec[3][3]
 // Returns the tree depth (levels).
 int build_adt(int nelem, const double *ebox, int *adtInt, double *adtReal, double ext[6]);
 
+// median_select restated with list-based partition rounds (hoare_stop,
+// median_control), the formulation the device uses; same result.
+void median_select_lists(int *ix, double *x, int n);
+
 // Coincident-point dedupe (utils.c:620-737): xtag[i] = representative of i,
 // rtag (resolutions) raised to the max over each coincident group.
 void unique_nodes_octree(const double *x, const int *meshtag, double *rtag, int *xtag, int n);
