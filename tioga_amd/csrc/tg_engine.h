@@ -151,6 +151,19 @@ struct QuerySet {
   double ext[6] = {0, 0, 0, 0, 0, 0};
 };
 
+// Reusable device scratch of the ADT build (tg_adt.hip).
+struct AdtScratch {
+  DBuf<int> ea, A, B, nsmall;
+  DBuf<double> X;
+  DBuf<int4> segA, segB, small;  // segment descriptors (6 ints each)
+};
+
+// Builds the ADT over n element boxes (device, 6 doubles each) exactly as
+// ADT::buildADT does: adtInt[4n] and adtReal[6n] on the device, ext[6] (the
+// padded root extents) on the host.  Returns the tree depth in levels.
+int build_adt_device(int n, const double *ebox, int *adtInt, double *adtReal, double ext[6], AdtScratch &w,
+                     hipStream_t s);
+
 struct Pair {
   int local = 0, ob = 0, iblk_remote = 0, tag_remote = 0, send_tag = 0, recv_tag = 0;
   OBB obc{};
@@ -250,6 +263,7 @@ class Engine {
   DBuf<unsigned> uscratch_, uscratch2_;
   DBuf<unsigned long long> kscratch_;
   DBuf<int> err_;
+  AdtScratch adtw_;
   std::vector<DBuf<double> *> qdev_;
 };
 

@@ -1578,14 +1578,8 @@ void Engine::build_search_structure(Block &b, const double *xq, int nq, int &nel
   adtInt.reserve(4 * (size_t)ne);
   adtReal.reserve(6 * (size_t)ne);
   if (ne) k_elem_box<<<nb(ne), BS, 0, s_>>>(ne, N, iscratch2_.p, b.ti, b.conn.p, b.x.p, elist.p, ebox.p);
-  std::vector<double> eb(6 * (size_t)ne), ar(6 * (size_t)ne);
-  std::vector<int> ai(4 * (size_t)ne);
-  TG_HIP(hipMemcpyAsync(eb.data(), ebox.p, sizeof(double) * eb.size(), hipMemcpyDeviceToHost, s_));
-  sync();
-  depth = build_adt(ne, eb.data(), ai.data(), ar.data(), ext);
+  depth = build_adt_device(ne, ebox.p, adtInt.p, adtReal.p, ext, adtw_, s_);
   if (depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
-  TG_HIP(hipMemcpyAsync(adtInt.p, ai.data(), sizeof(int) * ai.size(), hipMemcpyHostToDevice, s_));
-  TG_HIP(hipMemcpyAsync(adtReal.p, ar.data(), sizeof(double) * ar.size(), hipMemcpyHostToDevice, s_));
   nodes.reserve(ne);
   if (ne) k_adt_nodes<<<nb(ne), BS, 0, s_>>>(ne, b.ti, adtInt.p, adtReal.p, ebox.p, elist.p, nodes.p);
   sync();
