@@ -164,6 +164,17 @@ struct AdtScratch {
 int build_adt_device(int n, const double *ebox, int *adtInt, double *adtReal, double ext[6], AdtScratch &w,
                      hipStream_t s);
 
+// Reusable device scratch of the coincident-point dedupe (tg_dedupe.hip).
+struct DedupeScratch {
+  DBuf<int> ea, tmp, counts;
+  DBuf<int2> nodesA, nodesB, leaves;
+};
+
+// uniquenodes_octree (utils.c:620-737) on the device: itag[n] and rtag[n]
+// (device) receive the reference's representative and raised resolutions.
+void unique_nodes_device(int n, const double *x, const int *tag, double *rtag, int *itag, DedupeScratch &w,
+                         hipStream_t s);
+
 struct Pair {
   int local = 0, ob = 0, iblk_remote = 0, tag_remote = 0, send_tag = 0, recv_tag = 0;
   OBB obc{};
@@ -264,6 +275,7 @@ class Engine {
   DBuf<unsigned long long> kscratch_;
   DBuf<int> err_;
   AdtScratch adtw_;
+  DedupeScratch dedw_;
   std::vector<DBuf<double> *> qdev_;
 };
 

@@ -1521,12 +1521,10 @@ void Engine::search(Block &b)
   const int Q = b.nsearch;
   b.nelem = 0;
   if (Q == 0) return;
-  // host: query-set OBB and coincident-point dedupe (sequential order-fixing)
-  std::vector<double> xs(3 * (size_t)Q), rs(Q);
-  std::vector<int> ts(Q), xt(Q);
+  // query-set OBB: a running FP64 sum whose rounding order fixes its bits,
+  // so the host takes it; everything else stays on the device
+  std::vector<double> xs(3 * (size_t)Q);
   TG_HIP(hipMemcpyAsync(xs.data(), b.xsearch.p, sizeof(double) * 3 * Q, hipMemcpyDeviceToHost, s_));
-  TG_HIP(hipMemcpyAsync(rs.data(), b.res_search.p, sizeof(double) * Q, hipMemcpyDeviceToHost, s_));
-  TG_HIP(hipMemcpyAsync(ts.data(), b.tagsearch.p, sizeof(int) * Q, hipMemcpyDeviceToHost, s_));
   sync();
   build_search_structure(b, xs.data(), Q, b.nelem, b.adt_depth, b.elist, b.ebox, b.adtInt, b.adtReal, b.nodes,
                          b.adtExt);
@@ -1534,7 +1532,9 @@ void Engine::search(Block &b)
   for (Block *o : blocks_) gid_mode = gid_mode && (o->unode_gid != nullptr);
   if (gid_mode) {
     // search.C:48-74: first occurrence of each node global id
-    std::vector<int> is(3 * (size_t)Q);
+    std::vector<double> rs(Q);
+    std::vector<int> xt(Q), is(3 * (size_t)Q);
+    TG_HIP(hipMemcpyAsync(rs.data(), b.res_search.p, sizeof(double) * Q, hipMemcpyDeviceToHost, s_));
     TG_HIP(hipMemcpyAsync(is.data(), b.isearch.p, sizeof(int) * 3 * Q, hipMemcpyDeviceToHost, s_));
     sync();
     std::unordered_map<uint64_t, int> first;
@@ -1551,12 +1551,13 @@ void Engine::search(Block &b)
       }
     }
     for (int i = 0; i < Q; i++) rs[i] = rs[xt[i]];
+    TG_HIP(hipMemcpyAsync(b.xtag.p, xt.data(), sizeof(int) * Q, hipMemcpyHostToDevice, s_));
+    TG_HIP(hipMemcpyAsync(b.res_search.p, rs.data(), sizeof(double) * Q, hipMemcpyHostToDevice, s_));
+    sync();
   } else {
-    unique_nodes_octree(xs.data(), ts.data(), rs.data(), xt.data(), Q);
+    unique_nodes_device(Q, b.xsearch.p, b.tagsearch.p, b.res_search.p, b.xtag.p, dedw_, s_);
+    sync();
   }
-  TG_HIP(hipMemcpyAsync(b.xtag.p, xt.data(), sizeof(int) * Q, hipMemcpyHostToDevice, s_));
-  TG_HIP(hipMemcpyAsync(b.res_search.p, rs.data(), sizeof(double) * Q, hipMemcpyHostToDevice, s_));
-  sync();
 }
 
 // search.C:107-233: query-set OBB -> cells whose OBB-frame box overlaps it,
