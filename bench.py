@@ -169,7 +169,7 @@ def main() -> int:
         tg.time_query_sets(1)
     torch.cuda.synchronize()
     t0 = time.time()
-    kern_ms, nq = tg.time_query_sets(args.steps)
+    pass_ms, nq, kern_ms = tg.time_query_sets(args.steps)
     torch.cuda.synchronize()
     wall = time.time() - t0
     ms_per_step = 1000.0 * wall / args.steps
@@ -181,7 +181,8 @@ def main() -> int:
     achieved = algo / (kern_ms / 1000.0) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
-            "kernel": "k_search", "algorithmic_bytes_per_step": algo, "launches_per_step": len(qids)}
+            "kernel": "k_search", "algorithmic_bytes_per_step": algo, "launches_per_step": len(qids),
+            "avg_launch_ms": round(kern_ms / len(qids), 4)}
 
     cpu = None
     if not args.no_cpu:
@@ -210,6 +211,7 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
         "search_kernel_ms": round(kern_ms, 4),
+        "search_pass_ms_device": round(pass_ms, 4),
         "found": int(sum(int((d >= 0).sum()) for d in donors)),
         "connectivity_ms": round(1000 * t_conn, 2),
         "connectivity_phases_ms": {k: round(v, 3) for k, v in phases.items()},

@@ -131,13 +131,18 @@ double tioga_amd_time_dataupdate(int nvar, int interptype, int nrep, long long *
 int tioga_amd_query_set_create(int btag, const double *xq, long nq);
 void tioga_amd_query_set_search(int id, int *donor);
 double tioga_amd_time_query_sets(int nrep, long long *nq);
+/* device ms per pass of the search kernel launches alone in the last
+   tioga_amd_time_query_sets (HIP events around each launch) */
+double tioga_amd_last_search_kernel_ms(void);
 void tioga_amd_query_set_clear(void);
 /* number of cells in a query set's search structure (-1: bad id) */
 int tioga_amd_query_set_nelem(int id);
 
 /* Host wall times (ms) of the last performconnectivity phases:
    out[0..6] = hole map, boxes, query gathering, search preparation,
-   search kernels (device), donor selection, total; out[7] = searches. */
+   search kernels (device), donor selection, total; out[7] = searches;
+   out[8..11] = inside search preparation: query-set OBB (host), cell
+   filter and element boxes, ADT build, coincident-point dedupe. */
 void tioga_amd_phase_times(double *out);
 
 /* Host-side sequential stages, callable without a device (they are what

@@ -373,6 +373,13 @@ void tioga_amd_query_set_clear(void)
   guarded("tioga_amd_query_set_clear", [] { tg_or_throw().engine()->query_set_clear(); });
 }
 
+double tioga_amd_last_search_kernel_ms(void)
+{
+  double r = 0;
+  guarded("tioga_amd_last_search_kernel_ms", [&] { r = tg_or_throw().engine()->last_search_kernel_ms(); });
+  return r;
+}
+
 int tioga_amd_query_set_nelem(int id)
 {
   int r = -1;
@@ -392,6 +399,10 @@ void tioga_amd_phase_times(double *out)
     out[5] = t.donors;
     out[6] = t.total;
     out[7] = (double)t.nsearch;
+    out[8] = t.prep_obb;
+    out[9] = t.prep_filter;
+    out[10] = t.prep_adt;
+    out[11] = t.prep_dedupe;
   });
 }
 

@@ -191,6 +191,8 @@ struct PhaseTimes {
   double holemap = 0, boxes = 0, searchdata = 0, search_prep = 0, search_kernel = 0, donors = 0,
          total = 0;
   long long nsearch = 0;
+  // inside search_prep (host wall ms, summed over blocks)
+  double prep_obb = 0, prep_filter = 0, prep_adt = 0, prep_dedupe = 0;
 };
 
 class Engine {
@@ -235,6 +237,9 @@ class Engine {
   void query_set_search(int id, int *donor);
   double time_query_sets(int nrep, long long *nq);
   void query_set_clear();
+  // mean device ms per pass of the k_search launches alone in the last
+  // time_query_sets (the pass also orders the queries)
+  double last_search_kernel_ms() const { return last_search_kernel_ms_; }
   int query_set_nelem(int id) const { return (id >= 0 && id < (int)qsets_.size()) ? qsets_[id]->nelem : -1; }
 
   PhaseTimes last_times;
@@ -256,6 +261,12 @@ class Engine {
                               DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, DBuf<AdtNode> &nodes,
                               double ext[6]);
   void launch_query_set(QuerySet &q);
+  const int *order_queries(const double *xs, int nq, const double ext[6]);
+  std::vector<hipEvent_t> *kernel_events_ = nullptr;
+  size_t event_slot_ = 0;
+  double last_search_kernel_ms_ = 0;
+  DBuf<unsigned> mk0_, mk1_;
+  DBuf<int> mv0_, mv1_;
   std::vector<QuerySet *> qsets_;
   void sync();
 
@@ -276,6 +287,7 @@ class Engine {
   DBuf<int> err_;
   AdtScratch adtw_;
   DedupeScratch dedw_;
+  double prep_acc_[4] = {0, 0, 0, 0};
   std::vector<DBuf<double> *> qdev_;
 };
 

@@ -483,8 +483,39 @@ __global__ void k_adt_nodes(int nelem, TypeInfo ti, const int *__restrict__ adtI
   nodes[n] = N;
 }
 
+// 30-bit Morton code of each query in the frame of the search structure's
+// root extents, so a wave takes a compact cluster of points (coherent
+// traversal paths and cell data in cache); the search order does not change
+// any result, each query's walk is independent.
+__device__ __forceinline__ unsigned spread10(unsigned v)
+{
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__global__ void k_morton(int nq, const double *__restrict__ xs, double x0, double x1, double y0, double y1, double z0,
+                         double z1, unsigned *__restrict__ key, int *__restrict__ idx)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  double lo[3] = {x0, y0, z0}, hi[3] = {x1, y1, z1};
+  unsigned c[3];
+  for (int d = 0; d < 3; d++) {
+    double u = (xs[3 * (size_t)i + d] - lo[d]) / (hi[d] - lo[d]);
+    u = u < 0 ? 0 : (u > 0.999999 ? 0.999999 : u);
+    c[d] = (unsigned)(u * 1024.0);
+  }
+  key[i] = (spread10(c[0]) << 2) | (spread10(c[1]) << 1) | spread10(c[2]);
+  idx[i] = i;
+}
+
 struct SearchView {
   int nq;
+  const int *perm;  // processing order (query index of thread t), or null
   const double *xs;
   const int *xtag;
   int *donorId;
@@ -552,6 +583,7 @@ __global__ void __launch_bounds__(SEARCH_BS, TG_SEARCH_WAVES) k_search(SearchVie
   __shared__ int stk[STACK][SEARCH_BS];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= S.nq) return;
+  if (S.perm) i = S.perm[i];
   if (S.xtag && S.xtag[i] != i) return;
   double p[3];
   load_xyz(S.xs, i, p);
@@ -1555,8 +1587,10 @@ void Engine::search(Block &b)
     TG_HIP(hipMemcpyAsync(b.res_search.p, rs.data(), sizeof(double) * Q, hipMemcpyHostToDevice, s_));
     sync();
   } else {
+    auto t0 = std::chrono::steady_clock::now();
     unique_nodes_device(Q, b.xsearch.p, b.tagsearch.p, b.res_search.p, b.xtag.p, dedw_, s_);
     sync();
+    prep_acc_[3] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
 }
 
@@ -1566,8 +1600,12 @@ void Engine::build_search_structure(Block &b, const double *xq, int nq, int &nel
                                     DBuf<int> &elist, DBuf<double> &ebox, DBuf<int> &adtInt,
                                     DBuf<double> &adtReal, DBuf<AdtNode> &nodes, double ext[6])
 {
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto ms = [](auto a, auto c) { return std::chrono::duration<double, std::milli>(c - a).count(); };
+  auto t0 = now();
   OBB obq;
   find_obb(xq, nullptr, nq, obq);
+  auto t1 = now();
   const int N = b.ncells;
   iscratch_.reserve(N);
   iscratch2_.reserve(N);
@@ -1579,11 +1617,17 @@ void Engine::build_search_structure(Block &b, const double *xq, int nq, int &nel
   adtInt.reserve(4 * (size_t)ne);
   adtReal.reserve(6 * (size_t)ne);
   if (ne) k_elem_box<<<nb(ne), BS, 0, s_>>>(ne, N, iscratch2_.p, b.ti, b.conn.p, b.x.p, elist.p, ebox.p);
+  sync();
+  auto t2 = now();
   depth = build_adt_device(ne, ebox.p, adtInt.p, adtReal.p, ext, adtw_, s_);
   if (depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
   nodes.reserve(ne);
   if (ne) k_adt_nodes<<<nb(ne), BS, 0, s_>>>(ne, b.ti, adtInt.p, adtReal.p, ebox.p, elist.p, nodes.p);
   sync();
+  auto t3 = now();
+  prep_acc_[0] += ms(t0, t1);
+  prep_acc_[1] += ms(t1, t2);
+  prep_acc_[2] += ms(t2, t3);
 }
 
 int Engine::query_set_create(int btag, const double *xq, long nq)
@@ -1605,6 +1649,17 @@ int Engine::query_set_create(int btag, const double *xq, long nq)
   return (int)qsets_.size() - 1;
 }
 
+const int *Engine::order_queries(const double *xs, int nq, const double ext[6])
+{
+  mk0_.reserve(nq);
+  mk1_.reserve(nq);
+  mv0_.reserve(nq);
+  mv1_.reserve(nq);
+  k_morton<<<nb(nq), BS, 0, s_>>>(nq, xs, ext[0], ext[1], ext[2], ext[3], ext[4], ext[5], mk0_.p, mv0_.p);
+  sort_pairs(mk0_.p, mk1_.p, mv0_.p, mv1_.p, nq, 30, s_);
+  return mv1_.p;
+}
+
 void Engine::launch_query_set(QuerySet &q)
 {
   Block &b = *blocks_[q.blk];
@@ -1614,6 +1669,7 @@ void Engine::launch_query_set(QuerySet &q)
   }
   SearchView S;
   S.nq = q.nq;
+  S.perm = order_queries(q.xq.p, q.nq, q.ext);
   S.xs = q.xq.p;
   S.xtag = nullptr;
   S.donorId = q.donor.p;
@@ -1622,7 +1678,9 @@ void Engine::launch_query_set(QuerySet &q)
   S.conn = b.conn.p;
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
+  if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
   k_search<<<nb(q.nq, SEARCH_BS), SEARCH_BS, 0, s_>>>(S);
+  if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
 }
 
 void Engine::query_set_search(int id, int *donor)
@@ -1640,10 +1698,25 @@ double Engine::time_query_sets(int nrep, long long *nq)
   for (QuerySet *q : qsets_) n += q->nq;
   if (nq) *nq = n;
   if (nrep <= 0 || qsets_.empty()) return 0.0;
+  // events around every k_search launch give the kernel's own duration
+  std::vector<hipEvent_t> ev(2 * qsets_.size() * nrep);
+  for (auto &e : ev) TG_HIP(hipEventCreate(&e));
+  kernel_events_ = &ev;
   Timer tm(s_);
   for (int r = 0; r < nrep; r++)
     for (QuerySet *q : qsets_) launch_query_set(*q);
-  return tm.stop() / nrep;
+  double pass = tm.stop() / nrep;
+  kernel_events_ = nullptr;
+  double k = 0;
+  for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+    float m = 0;
+    TG_HIP(hipEventElapsedTime(&m, ev[i], ev[i + 1]));
+    k += m;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  last_search_kernel_ms_ = k / nrep;
+  event_slot_ = 0;
+  return pass;
 }
 
 void Engine::query_set_clear()
@@ -1655,8 +1728,13 @@ void Engine::query_set_clear()
 void Engine::launch_search(Block &b)
 {
   if (b.nsearch == 0) return;
+  if (b.nelem == 0) {
+    fill<int>(b.donorId.p, b.nsearch, -1, s_);
+    return;
+  }
   SearchView S;
   S.nq = b.nsearch;
+  S.perm = order_queries(b.xsearch.p, b.nsearch, b.adtExt);
   S.xs = b.xsearch.p;
   S.xtag = b.xtag.p;
   S.donorId = b.donorId.p;
@@ -1665,10 +1743,6 @@ void Engine::launch_search(Block &b)
   S.conn = b.conn.p;
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
-  if (b.nelem == 0) {
-    fill<int>(b.donorId.p, b.nsearch, -1, s_);
-    return;
-  }
   k_search<<<nb(b.nsearch, SEARCH_BS), SEARCH_BS, 0, s_>>>(S);
   k_copy_dup<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.xtag.p, b.donorId.p);
 }
@@ -1894,6 +1968,7 @@ void Engine::perform_connectivity()
   auto t2 = now();
   exchange_search_data();
   auto t3 = now();
+  for (double &a : prep_acc_) a = 0;
   for (Block *b : blocks_) {
     b->ninterp = 0;
     b->interp_host_valid = false;
@@ -1932,6 +2007,10 @@ void Engine::perform_connectivity()
   t.total = ms(t0, t7);
   t.nsearch = 0;
   for (Block *b : blocks_) t.nsearch += b->nsearch;
+  t.prep_obb = prep_acc_[0];
+  t.prep_filter = prep_acc_[1];
+  t.prep_adt = prep_acc_[2];
+  t.prep_dedupe = prep_acc_[3];
   last_times = t;
   connected_ = true;
 }

@@ -65,6 +65,8 @@ def load():
         L.tioga_amd_time_query_sets.argtypes = [C.c_int, _P]
         L.tioga_amd_time_query_sets.restype = C.c_double
         L.tioga_amd_query_set_clear.argtypes = []
+        L.tioga_amd_last_search_kernel_ms.argtypes = []
+        L.tioga_amd_last_search_kernel_ms.restype = C.c_double
         L.tioga_amd_query_set_nelem.argtypes = [C.c_int]
         L.tioga_amd_query_set_nelem.restype = C.c_int
         L.tioga_amd_weight_aborts.restype = C.c_int
@@ -81,7 +83,8 @@ def _ip(v: int):
 
 
 _I32 = {"isearch", "tagsearch", "donorId", "xtag", "elementList", "adtIntegers", "iblank_cell"}
-PHASES = ("holemap", "boxes", "searchdata", "search_prep", "search_kernel", "donors", "total", "nsearch")
+PHASES = ("holemap", "boxes", "searchdata", "search_prep", "search_kernel", "donors", "total", "nsearch",
+          "prep_obb", "prep_filter", "prep_adt", "prep_dedupe")
 
 
 class Tioga:
@@ -239,13 +242,14 @@ class Tioga:
         return int(n)
 
     def time_query_sets(self, nrep: int):
+        """(ms per pass, queries per pass, ms per pass in the search kernel alone)"""
         n = C.c_longlong(0)
         ms = self.L.tioga_amd_time_query_sets(nrep, C.byref(n))
         self._check()
-        return ms, n.value
+        return ms, n.value, self.L.tioga_amd_last_search_kernel_ms()
 
     def phase_times(self) -> Dict[str, float]:
-        out = np.zeros(8)
+        out = np.zeros(12)
         self.L.tioga_amd_phase_times(_p(out))
         self._check()
         return dict(zip(PHASES, out.tolist()))
