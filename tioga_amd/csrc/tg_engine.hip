@@ -568,19 +568,28 @@ __device__ __forceinline__ void contain(const SearchView &S, int c, int nv, cons
 }
 
 constexpr int SEARCH_BS = 128;
-constexpr int STACK = 32;
+constexpr int STACK = 28;
+constexpr int NCAND = 12;
 
-// ADT::searchADT / searchIntersections (searchADTrecursion.C:26-101) as an
-// explicit-stack preorder walk: a node's own element is tested first, then
-// its left and right subtrees whose boxes hold the point.  Every visited
-// element overwrites the result; the walk stops at the first clean hit.  The
-// per-thread stacks live in LDS, strided by the block size.
 #ifndef TG_SEARCH_WAVES
 #define TG_SEARCH_WAVES 4
 #endif
+// ADT::searchADT / searchIntersections (searchADTrecursion.C:26-101).  The
+// reference walks the tree in preorder (a node's own element, then its left
+// and right subtrees whose boxes hold the point), tests every element whose
+// box holds the point, lets each test overwrite the result and stops at the
+// first clean hit.  Box tests have no side effects, so the same walk without
+// the containment tests lists the candidate elements in the order the
+// reference would test them; testing them afterwards in that order, up to
+// the first clean hit, gives the reference's result.  Splitting the walk
+// from the tests keeps a wave's FP64 Newton solves in lockstep instead of
+// serialising them across the traversal loop.  Stacks and candidate lists
+// live in LDS, strided by the block size; a query with more than NCAND
+// candidates repeats the walk with the tests interleaved.
 __global__ void __launch_bounds__(SEARCH_BS, TG_SEARCH_WAVES) k_search(SearchView S)
 {
   __shared__ int stk[STACK][SEARCH_BS];
+  __shared__ int cand[NCAND][SEARCH_BS];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= S.nq) return;
   if (S.perm) i = S.perm[i];
@@ -593,17 +602,37 @@ __global__ void __launch_bounds__(SEARCH_BS, TG_SEARCH_WAVES) k_search(SearchVie
   for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= S.ext[2 * d + 1] + TOL);
   if (inroot) {
     const int t = threadIdx.x;
-    int sp = 0;
+    int sp = 0, nc = 0;
     stk[sp++][t] = 0;
     while (sp > 0) {
       const AdtNode &N = S.nodes[stk[--sp][t]];
       if (in_box(p, N.ebox)) {
-        contain(S, N.cell, N.nv, p, res, flag);
-        if (res > -1 && flag == 0) break;
+        if (nc < NCAND) cand[nc][t] = N.cell | (N.nv << 28);
+        nc++;
       }
       int r = N.right, l = N.left;
       if (r > -1 && in_box(p, S.nodes[r].sbox)) stk[sp++][t] = r;
       if (l > -1 && in_box(p, S.nodes[l].sbox)) stk[sp++][t] = l;
+    }
+    if (nc <= NCAND) {
+      for (int k = 0; k < nc; k++) {
+        int c = cand[k][t];
+        contain(S, c & 0x0fffffff, c >> 28, p, res, flag);
+        if (res > -1 && flag == 0) break;
+      }
+    } else {
+      sp = 0;
+      stk[sp++][t] = 0;
+      while (sp > 0) {
+        const AdtNode &N = S.nodes[stk[--sp][t]];
+        if (in_box(p, N.ebox)) {
+          contain(S, N.cell, N.nv, p, res, flag);
+          if (res > -1 && flag == 0) break;
+        }
+        int r = N.right, l = N.left;
+        if (r > -1 && in_box(p, S.nodes[r].sbox)) stk[sp++][t] = r;
+        if (l > -1 && in_box(p, S.nodes[l].sbox)) stk[sp++][t] = l;
+      }
     }
   }
   S.donorId[i] = res;
@@ -1095,6 +1124,7 @@ void Engine::register_grid(int btag, int nnodes, double *xyz, int *ibl, int nwbc
     b->uconn[t] = vconn[t];
   }
   b->ncells = b->ti.cstart[ntypes];
+  if (b->ncells >= (1 << 28)) throw std::runtime_error("tioga_amd: at most 2^28 - 1 cells per block");
   b->topo_uploaded = false;
   b->unres = b->ucres = nullptr;
   connected_ = false;
