@@ -61,14 +61,14 @@ struct TypeInfo {
   int cstart[MAXTYPES + 1];
 };
 
-// One ADT node as the search kernel reads it: 128 bytes, node ids in the
-// ADT's preorder.  sbox bounds the node's subtree, ebox is the box of the
-// node's own element; left/right are child node ids (-1: none).
+// One ADT node as the search kernel reads it: 160 bytes, node ids in the
+// ADT's preorder; left/right are child node ids (-1: none) and the children's
+// subtree boxes sit in the parent, so one record decides a whole visit.
 struct __align__(16) AdtNode {
-  double sbox[6];
-  double ebox[6];
-  int cell, left, right, nv;
-  int pad[4];
+  double ebox[6];  // box of the node's own element
+  double lbox[6];  // subtree box of the left child
+  double rbox[6];  // subtree box of the right child
+  int cell, nv, left, right;
 };
 
 // Hole map of one mesh tag as the kernels see it.
@@ -262,6 +262,7 @@ class Engine {
                               double ext[6]);
   void launch_query_set(QuerySet &q);
   const int *order_queries(const double *xs, int nq, const double ext[6]);
+  DBuf<int> ovf_, novf_;  // queries with more candidates than the fast path holds
   std::vector<hipEvent_t> *kernel_events_ = nullptr;
   size_t event_slot_ = 0;
   double last_search_kernel_ms_ = 0;

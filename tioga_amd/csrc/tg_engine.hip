@@ -460,7 +460,7 @@ __global__ void k_elem_box(int nelem, int ncells, const int *__restrict__ sel, T
   }
 }
 
-static_assert(sizeof(AdtNode) == 128, "ADT node record must be 128 bytes");
+static_assert(sizeof(AdtNode) == 160, "ADT node record must be 160 bytes");
 
 // ADT arrays (adtIntegers, adtReals, element boxes) -> node records
 __global__ void k_adt_nodes(int nelem, TypeInfo ti, const int *__restrict__ adtInt,
@@ -471,15 +471,15 @@ __global__ void k_adt_nodes(int nelem, TypeInfo ti, const int *__restrict__ adtI
   if (n >= nelem) return;
   int4 a = reinterpret_cast<const int4 *>(adtInt)[n];
   AdtNode N;
-  for (int j = 0; j < 6; j++) {
-    N.sbox[j] = adtReal[6 * (size_t)n + j];
-    N.ebox[j] = ebox[6 * (size_t)a.x + j];
-  }
   N.cell = elist[a.x];
   N.nv = ti.nv[type_of(ti, N.cell)];
   N.left = a.y > -1 ? adtInt[4 * (size_t)a.y + 3] : -1;
   N.right = a.z > -1 ? adtInt[4 * (size_t)a.z + 3] : -1;
-  for (int j = 0; j < 4; j++) N.pad[j] = 0;
+  for (int j = 0; j < 6; j++) {
+    N.ebox[j] = ebox[6 * (size_t)a.x + j];
+    N.lbox[j] = N.left > -1 ? adtReal[6 * (size_t)N.left + j] : 0.0;
+    N.rbox[j] = N.right > -1 ? adtReal[6 * (size_t)N.right + j] : 0.0;
+  }
   nodes[n] = N;
 }
 
@@ -532,6 +532,13 @@ __device__ __forceinline__ bool in_box(const double p[3], const double *b)
          p[1] <= b[4] + TOL && p[2] <= b[5] + TOL;
 }
 
+// the same test evaluated without short-circuit branches
+__device__ __forceinline__ bool in_box_all(const double p[3], const double *b)
+{
+  return (p[0] >= b[0] - TOL) & (p[1] >= b[1] - TOL) & (p[2] >= b[2] - TOL) & (p[0] <= b[3] + TOL) &
+         (p[1] <= b[4] + TOL) & (p[2] <= b[5] + TOL);
+}
+
 // MeshBlock::checkContainment (checkContainment.C:27-96) for a cell with NV
 // vertices: weights of p; accept when every weight lies in [-TOL, 1+TOL];
 // flag a near-zero weight in a BIGVALUE cell, which keeps the search going.
@@ -567,75 +574,126 @@ __device__ __forceinline__ void contain(const SearchView &S, int c, int nv, cons
   }
 }
 
-constexpr int SEARCH_BS = 128;
-constexpr int STACK = 28;
-constexpr int NCAND = 12;
+constexpr int WAVE = 64;
+constexpr int STACK = 32;
+constexpr int NCAND = 16;
 
-#ifndef TG_SEARCH_WAVES
-#define TG_SEARCH_WAVES 4
-#endif
 // ADT::searchADT / searchIntersections (searchADTrecursion.C:26-101).  The
 // reference walks the tree in preorder (a node's own element, then its left
 // and right subtrees whose boxes hold the point), tests every element whose
 // box holds the point, lets each test overwrite the result and stops at the
 // first clean hit.  Box tests have no side effects, so the same walk without
-// the containment tests lists the candidate elements in the order the
-// reference would test them; testing them afterwards in that order, up to
-// the first clean hit, gives the reference's result.  Splitting the walk
-// from the tests keeps a wave's FP64 Newton solves in lockstep instead of
-// serialising them across the traversal loop.  Stacks and candidate lists
-// live in LDS, strided by the block size; a query with more than NCAND
-// candidates repeats the walk with the tests interleaved.
-__global__ void __launch_bounds__(SEARCH_BS, TG_SEARCH_WAVES) k_search(SearchView S)
+// the tests lists the candidate elements in the order the reference tests
+// them; testing them afterwards in that order, up to the first clean hit,
+// gives the reference's result.
+//
+// The 64 queries of a wave (consecutive in Morton order, so spatially close)
+// walk the tree together: the wave visits a node when any lane's own walk
+// would, carrying the mask of those lanes, so every lane sees exactly its
+// own preorder walk while the node records are wave-uniform scalar loads.
+// Lanes then test their candidates in lockstep.  A lane with more than NCAND
+// candidates is queued for k_search_lane, which walks with the tests
+// interleaved.
+__global__ void __launch_bounds__(WAVE) k_search(SearchView S, const AdtNode *__restrict__ nodes,
+                                                 int *__restrict__ overflow, int *__restrict__ noverflow)
 {
-  __shared__ int stk[STACK][SEARCH_BS];
-  __shared__ int cand[NCAND][SEARCH_BS];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= S.nq) return;
-  if (S.perm) i = S.perm[i];
-  if (S.xtag && S.xtag[i] != i) return;
-  double p[3];
-  load_xyz(S.xs, i, p);
-  int res = -1, flag = 0;
-  bool inroot = true;
-  for (int d = 0; d < 3; d++) inroot = inroot && (p[d] >= S.ext[2 * d] - TOL);
-  for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= S.ext[2 * d + 1] + TOL);
-  if (inroot) {
-    const int t = threadIdx.x;
-    int sp = 0, nc = 0;
-    stk[sp++][t] = 0;
-    while (sp > 0) {
-      const AdtNode &N = S.nodes[stk[--sp][t]];
-      if (in_box(p, N.ebox)) {
-        if (nc < NCAND) cand[nc][t] = N.cell | (N.nv << 28);
-        nc++;
-      }
-      int r = N.right, l = N.left;
-      if (r > -1 && in_box(p, S.nodes[r].sbox)) stk[sp++][t] = r;
-      if (l > -1 && in_box(p, S.nodes[l].sbox)) stk[sp++][t] = l;
+  __shared__ int snode[STACK];
+  __shared__ unsigned long long smask[STACK];
+  __shared__ int cand[NCAND][WAVE];
+  const int lane = threadIdx.x;
+  const int t = blockIdx.x * WAVE + lane;
+  int i = -1;
+  if (t < S.nq) {
+    i = S.perm ? S.perm[t] : t;
+    if (S.xtag && S.xtag[i] != i) i = -1;
+  }
+  double p[3] = {0.0, 0.0, 0.0};
+  bool inroot = false;
+  if (i >= 0) {
+    load_xyz(S.xs, i, p);
+    inroot = true;
+    for (int d = 0; d < 3; d++) inroot = inroot && (p[d] >= S.ext[2 * d] - TOL);
+    for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= S.ext[2 * d + 1] + TOL);
+  }
+  int nc = 0;
+  const unsigned long long m0 = __ballot(inroot);
+  int sp = 0;
+  if (m0) {
+    if (lane == 0) {
+      snode[0] = 0;
+      smask[0] = m0;
     }
-    if (nc <= NCAND) {
-      for (int k = 0; k < nc; k++) {
-        int c = cand[k][t];
-        contain(S, c & 0x0fffffff, c >> 28, p, res, flag);
-        if (res > -1 && flag == 0) break;
+    sp = 1;
+  }
+  while (sp > 0) {
+    sp--;
+    const int node = __builtin_amdgcn_readfirstlane(snode[sp]);
+    const unsigned long long m = smask[sp];
+    // the whole record in one batch of scalar loads, then the tests
+    const AdtNode N = nodes[node];
+    const bool act = (m >> lane) & 1ull;
+    if (act & in_box_all(p, N.ebox)) {
+      if (nc < NCAND) cand[nc][lane] = N.cell | (N.nv << 28);
+      nc++;
+    }
+    const int r = N.right, l = N.left;
+    const unsigned long long mr = (r > -1) ? __ballot(act & in_box_all(p, N.rbox)) : 0ull;
+    const unsigned long long ml = (l > -1) ? __ballot(act & in_box_all(p, N.lbox)) : 0ull;
+    if (mr) {
+      if (lane == 0) {
+        snode[sp] = r;
+        smask[sp] = mr;
       }
-    } else {
-      sp = 0;
-      stk[sp++][t] = 0;
-      while (sp > 0) {
-        const AdtNode &N = S.nodes[stk[--sp][t]];
-        if (in_box(p, N.ebox)) {
-          contain(S, N.cell, N.nv, p, res, flag);
-          if (res > -1 && flag == 0) break;
-        }
-        int r = N.right, l = N.left;
-        if (r > -1 && in_box(p, S.nodes[r].sbox)) stk[sp++][t] = r;
-        if (l > -1 && in_box(p, S.nodes[l].sbox)) stk[sp++][t] = l;
+      sp++;
+    }
+    if (ml) {
+      if (lane == 0) {
+        snode[sp] = l;
+        smask[sp] = ml;
       }
+      sp++;
     }
   }
-  S.donorId[i] = res;
+  if (i < 0) return;
+  int res = -1, flag = 0;
+  if (nc <= NCAND) {
+    for (int k = 0; k < nc; k++) {
+      const int c = cand[k][lane];
+      contain(S, c & 0x0fffffff, c >> 28, p, res, flag);
+      if (res > -1 && flag == 0) break;
+    }
+    S.donorId[i] = res;
+  } else {
+    overflow[atomicAdd(noverflow, 1)] = i;
+  }
+}
+
+// The walk of one query with the tests interleaved, exactly as the
+// reference runs it; used for the queries with more than NCAND candidates.
+constexpr int LANE_BS = 64;
+__global__ void __launch_bounds__(LANE_BS) k_search_lane(SearchView S, const AdtNode *__restrict__ nodes,
+                                                         const int *__restrict__ list, const int *__restrict__ count)
+{
+  __shared__ int stk[STACK][LANE_BS];
+  const int n = *count;
+  const int tid = threadIdx.x;
+  for (int k = blockIdx.x * LANE_BS + tid; k < n; k += gridDim.x * LANE_BS) {
+    const int i = list[k];
+    double p[3];
+    load_xyz(S.xs, i, p);
+    int res = -1, flag = 0, sp = 0;
+    stk[sp++][tid] = 0;
+    while (sp > 0) {
+      const AdtNode &N = nodes[stk[--sp][tid]];
+      if (in_box(p, N.ebox)) {
+        contain(S, N.cell, N.nv, p, res, flag);
+        if (res > -1 && flag == 0) break;
+      }
+      if (N.right > -1 && in_box(p, N.rbox)) stk[sp++][tid] = N.right;
+      if (N.left > -1 && in_box(p, N.lbox)) stk[sp++][tid] = N.left;
+    }
+    S.donorId[i] = res;
+  }
 }
 
 // coincident queries take their representative's donor (search.C:251-265)
@@ -1027,6 +1085,17 @@ template <class T>
 void fill(T *p, size_t n, T v, hipStream_t s)
 {
   if (n) k_fill<T><<<nb((long long)n), BS, 0, s>>>(n, p, v);
+}
+
+// launches the donor search over S (fast wave-cooperative path, then the
+// queries that overflowed it)
+void run_search(const SearchView &S, const AdtNode *nodes, DBuf<int> &ovf, DBuf<int> &novf, hipStream_t s)
+{
+  ovf.reserve(S.nq);
+  novf.reserve(1);
+  fill<int>(novf.p, 1, 0, s);
+  k_search<<<nb(S.nq, WAVE), WAVE, 0, s>>>(S, nodes, ovf.p, novf.p);
+  k_search_lane<<<256, LANE_BS, 0, s>>>(S, nodes, ovf.p, novf.p);
 }
 
 // a small device-side copy of a host struct, valid until the next call
@@ -1709,7 +1778,7 @@ void Engine::launch_query_set(QuerySet &q)
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
-  k_search<<<nb(q.nq, SEARCH_BS), SEARCH_BS, 0, s_>>>(S);
+  run_search(S, q.nodes.p, ovf_, novf_, s_);
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
 }
 
@@ -1773,7 +1842,7 @@ void Engine::launch_search(Block &b)
   S.conn = b.conn.p;
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
-  k_search<<<nb(b.nsearch, SEARCH_BS), SEARCH_BS, 0, s_>>>(S);
+  run_search(S, b.nodes.p, ovf_, novf_, s_);
   k_copy_dup<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.xtag.p, b.donorId.p);
 }
 
