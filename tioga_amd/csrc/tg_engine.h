@@ -175,6 +175,12 @@ struct DedupeScratch {
 void unique_nodes_device(int n, const double *x, const int *tag, double *rtag, int *itag, DedupeScratch &w,
                          hipStream_t s);
 
+// Reusable device scratch of the donor search: per-query candidate lists
+// and the queue of queries with more candidates than a list holds.
+struct SearchScratch {
+  DBuf<int> cand, ncand, ovf, novf;
+};
+
 struct Pair {
   int local = 0, ob = 0, iblk_remote = 0, tag_remote = 0, send_tag = 0, recv_tag = 0;
   OBB obc{};
@@ -262,7 +268,7 @@ class Engine {
                               double ext[6]);
   void launch_query_set(QuerySet &q);
   const int *order_queries(const double *xs, int nq, const double ext[6]);
-  DBuf<int> ovf_, novf_;  // queries with more candidates than the fast path holds
+  SearchScratch sw_;
   std::vector<hipEvent_t> *kernel_events_ = nullptr;
   size_t event_slot_ = 0;
   double last_search_kernel_ms_ = 0;

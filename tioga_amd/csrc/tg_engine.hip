@@ -587,19 +587,18 @@ constexpr int NCAND = 16;
 // them; testing them afterwards in that order, up to the first clean hit,
 // gives the reference's result.
 //
-// The 64 queries of a wave (consecutive in Morton order, so spatially close)
-// walk the tree together: the wave visits a node when any lane's own walk
-// would, carrying the mask of those lanes, so every lane sees exactly its
-// own preorder walk while the node records are wave-uniform scalar loads.
-// Lanes then test their candidates in lockstep.  A lane with more than NCAND
-// candidates is queued for k_search_lane, which walks with the tests
-// interleaved.
-__global__ void __launch_bounds__(WAVE) k_search(SearchView S, const AdtNode *__restrict__ nodes,
-                                                 int *__restrict__ overflow, int *__restrict__ noverflow)
+// k_collect: the 64 queries of a wave (consecutive in Morton order, so
+// spatially close) walk the tree together: the wave visits a node when any
+// lane's own walk would, carrying the mask of those lanes, so every lane sees
+// exactly its own preorder walk while node records are wave-uniform scalar
+// loads.  It needs few registers, so many waves hide the load latency.  Each
+// lane writes its candidate list (cell | nv << 28), in walk order, to
+// cand[i * NCAND ...] and its length to ncand[i] (-1: not searched).
+__global__ void __launch_bounds__(WAVE, 8) k_collect(SearchView S, const AdtNode *__restrict__ nodes,
+                                                     int *__restrict__ cand, int *__restrict__ ncand)
 {
   __shared__ int snode[STACK];
   __shared__ unsigned long long smask[STACK];
-  __shared__ int cand[NCAND][WAVE];
   const int lane = threadIdx.x;
   const int t = blockIdx.x * WAVE + lane;
   int i = -1;
@@ -616,6 +615,7 @@ __global__ void __launch_bounds__(WAVE) k_search(SearchView S, const AdtNode *__
     for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= S.ext[2 * d + 1] + TOL);
   }
   int nc = 0;
+  int *mine = cand + (size_t)(i < 0 ? 0 : i) * NCAND;
   const unsigned long long m0 = __ballot(inroot);
   int sp = 0;
   if (m0) {
@@ -633,7 +633,7 @@ __global__ void __launch_bounds__(WAVE) k_search(SearchView S, const AdtNode *__
     const AdtNode N = nodes[node];
     const bool act = (m >> lane) & 1ull;
     if (act & in_box_all(p, N.ebox)) {
-      if (nc < NCAND) cand[nc][lane] = N.cell | (N.nv << 28);
+      if (nc < NCAND) mine[nc] = N.cell | (N.nv << 28);
       nc++;
     }
     const int r = N.right, l = N.left;
@@ -654,18 +654,36 @@ __global__ void __launch_bounds__(WAVE) k_search(SearchView S, const AdtNode *__
       sp++;
     }
   }
-  if (i < 0) return;
-  int res = -1, flag = 0;
-  if (nc <= NCAND) {
-    for (int k = 0; k < nc; k++) {
-      const int c = cand[k][lane];
-      contain(S, c & 0x0fffffff, c >> 28, p, res, flag);
-      if (res > -1 && flag == 0) break;
-    }
-    S.donorId[i] = res;
-  } else {
+  if (i >= 0) ncand[i] = inroot ? nc : 0;
+}
+
+// k_contain: each query tests its candidates in walk order up to the first
+// clean hit; queries are taken in Morton order so a wave's Newton solves run
+// on neighbouring cells.  A query with more than NCAND candidates is queued
+// for k_search_lane.
+__global__ void __launch_bounds__(WAVE) k_contain(SearchView S, const int *__restrict__ cand,
+                                                  const int *__restrict__ ncand, int *__restrict__ overflow,
+                                                  int *__restrict__ noverflow)
+{
+  const int t = blockIdx.x * WAVE + threadIdx.x;
+  if (t >= S.nq) return;
+  const int i = S.perm ? S.perm[t] : t;
+  if (S.xtag && S.xtag[i] != i) return;
+  const int nc = ncand[i];
+  if (nc > NCAND) {
     overflow[atomicAdd(noverflow, 1)] = i;
+    return;
   }
+  double p[3];
+  load_xyz(S.xs, i, p);
+  int res = -1, flag = 0;
+  const int *mine = cand + (size_t)i * NCAND;
+  for (int k = 0; k < nc; k++) {
+    const int c = mine[k];
+    contain(S, c & 0x0fffffff, c >> 28, p, res, flag);
+    if (res > -1 && flag == 0) break;
+  }
+  S.donorId[i] = res;
 }
 
 // The walk of one query with the tests interleaved, exactly as the
@@ -1089,13 +1107,16 @@ void fill(T *p, size_t n, T v, hipStream_t s)
 
 // launches the donor search over S (fast wave-cooperative path, then the
 // queries that overflowed it)
-void run_search(const SearchView &S, const AdtNode *nodes, DBuf<int> &ovf, DBuf<int> &novf, hipStream_t s)
+void run_search(const SearchView &S, const AdtNode *nodes, SearchScratch &w, hipStream_t s)
 {
-  ovf.reserve(S.nq);
-  novf.reserve(1);
-  fill<int>(novf.p, 1, 0, s);
-  k_search<<<nb(S.nq, WAVE), WAVE, 0, s>>>(S, nodes, ovf.p, novf.p);
-  k_search_lane<<<256, LANE_BS, 0, s>>>(S, nodes, ovf.p, novf.p);
+  w.ovf.reserve(S.nq);
+  w.novf.reserve(1);
+  w.cand.reserve((size_t)S.nq * NCAND);
+  w.ncand.reserve(S.nq);
+  fill<int>(w.novf.p, 1, 0, s);
+  k_collect<<<nb(S.nq, WAVE), WAVE, 0, s>>>(S, nodes, w.cand.p, w.ncand.p);
+  k_contain<<<nb(S.nq, WAVE), WAVE, 0, s>>>(S, w.cand.p, w.ncand.p, w.ovf.p, w.novf.p);
+  k_search_lane<<<256, LANE_BS, 0, s>>>(S, nodes, w.ovf.p, w.novf.p);
 }
 
 // a small device-side copy of a host struct, valid until the next call
@@ -1778,7 +1799,7 @@ void Engine::launch_query_set(QuerySet &q)
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
-  run_search(S, q.nodes.p, ovf_, novf_, s_);
+  run_search(S, q.nodes.p, sw_, s_);
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
 }
 
@@ -1842,7 +1863,7 @@ void Engine::launch_search(Block &b)
   S.conn = b.conn.p;
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
-  run_search(S, b.nodes.p, ovf_, novf_, s_);
+  run_search(S, b.nodes.p, sw_, s_);
   k_copy_dup<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.xtag.p, b.donorId.p);
 }
 
