@@ -178,7 +178,8 @@ void unique_nodes_device(int n, const double *x, const int *tag, double *rtag, i
 // Reusable device scratch of the donor search: per-query candidate lists
 // and the queue of queries with more candidates than a list holds.
 struct SearchScratch {
-  DBuf<int> cand, ncand, ovf, novf;
+  DBuf<int> cell, query, status, qoff, qcnt, ovf, counters;
+  size_t cap = 0;  // capacity of the flat candidate list
 };
 
 struct Pair {
@@ -269,6 +270,7 @@ class Engine {
   void launch_query_set(QuerySet &q);
   const int *order_queries(const double *xs, int nq, const double ext[6]);
   SearchScratch sw_;
+  bool timing_ = false;  // inside time_query_sets: candidate capacity already established
   std::vector<hipEvent_t> *kernel_events_ = nullptr;
   size_t event_slot_ = 0;
   double last_search_kernel_ms_ = 0;

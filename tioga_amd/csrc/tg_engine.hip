@@ -576,29 +576,47 @@ __device__ __forceinline__ void contain(const SearchView &S, int c, int nv, cons
 
 constexpr int WAVE = 64;
 constexpr int STACK = 32;
-constexpr int NCAND = 16;
+constexpr int KSTAGE = 32;  // candidates per lane staged in LDS during the walk
+
+// Candidate lists of one search: a flat list of (cell | nv << 28, query)
+// entries, grouped per query in walk order, and per query its offset and
+// count (count 0: nothing to test).
+struct CandOut {
+  int *cell, *query, *status;
+  int *qoff, *qcnt;
+  int *used;  // entries allocated so far
+  int cap;    // capacity of the flat list
+  int *ovf, *novf;  // queries with more than KSTAGE candidates
+};
 
 // ADT::searchADT / searchIntersections (searchADTrecursion.C:26-101).  The
 // reference walks the tree in preorder (a node's own element, then its left
 // and right subtrees whose boxes hold the point), tests every element whose
 // box holds the point, lets each test overwrite the result and stops at the
 // first clean hit.  Box tests have no side effects, so the same walk without
-// the tests lists the candidate elements in the order the reference tests
-// them; testing them afterwards in that order, up to the first clean hit,
-// gives the reference's result.
+// the tests lists the candidates in the order the reference tests them, and
+// the reference's result is a function of that list and the candidates'
+// test outcomes alone: the first clean hit if there is one, otherwise the
+// outcome of the last candidate (its cell when accepted with a near-zero
+// weight in a BIGVALUE cell, else -1).  So the search runs as
+//   k_collect   the walk, listing candidates in walk order,
+//   k_eval      every candidate's containment test, one thread each,
+//   k_select    the rule above, per query,
+// which keeps long candidate lists and slow Newton solves from serialising
+// inside one lane.
 //
 // k_collect: the 64 queries of a wave (consecutive in Morton order, so
 // spatially close) walk the tree together: the wave visits a node when any
 // lane's own walk would, carrying the mask of those lanes, so every lane sees
 // exactly its own preorder walk while node records are wave-uniform scalar
-// loads.  It needs few registers, so many waves hide the load latency.  Each
-// lane writes its candidate list (cell | nv << 28), in walk order, to
-// cand[i * NCAND ...] and its length to ncand[i] (-1: not searched).
-__global__ void __launch_bounds__(WAVE, 8) k_collect(SearchView S, const AdtNode *__restrict__ nodes,
-                                                     int *__restrict__ cand, int *__restrict__ ncand)
+// loads.  Candidates are staged in LDS and copied out to a chunk the wave
+// allocates at the end; a lane with more than KSTAGE candidates is queued for
+// k_collect_more, which lists all of them.
+__global__ void __launch_bounds__(WAVE, 8) k_collect(SearchView S, const AdtNode *__restrict__ nodes, CandOut O)
 {
   __shared__ int snode[STACK];
   __shared__ unsigned long long smask[STACK];
+  __shared__ int stage[KSTAGE][WAVE];
   const int lane = threadIdx.x;
   const int t = blockIdx.x * WAVE + lane;
   int i = -1;
@@ -615,7 +633,6 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchView S, const AdtNode
     for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= S.ext[2 * d + 1] + TOL);
   }
   int nc = 0;
-  int *mine = cand + (size_t)(i < 0 ? 0 : i) * NCAND;
   const unsigned long long m0 = __ballot(inroot);
   int sp = 0;
   if (m0) {
@@ -633,7 +650,7 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchView S, const AdtNode
     const AdtNode N = nodes[node];
     const bool act = (m >> lane) & 1ull;
     if (act & in_box_all(p, N.ebox)) {
-      if (nc < NCAND) mine[nc] = N.cell | (N.nv << 28);
+      if (nc < KSTAGE) stage[nc][lane] = N.cell | (N.nv << 28);
       nc++;
     }
     const int r = N.right, l = N.left;
@@ -654,64 +671,93 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchView S, const AdtNode
       sp++;
     }
   }
-  if (i >= 0) ncand[i] = inroot ? nc : 0;
-}
-
-// k_contain: each query tests its candidates in walk order up to the first
-// clean hit; queries are taken in Morton order so a wave's Newton solves run
-// on neighbouring cells.  A query with more than NCAND candidates is queued
-// for k_search_lane.
-__global__ void __launch_bounds__(WAVE) k_contain(SearchView S, const int *__restrict__ cand,
-                                                  const int *__restrict__ ncand, int *__restrict__ overflow,
-                                                  int *__restrict__ noverflow)
-{
-  const int t = blockIdx.x * WAVE + threadIdx.x;
-  if (t >= S.nq) return;
-  const int i = S.perm ? S.perm[t] : t;
-  if (S.xtag && S.xtag[i] != i) return;
-  const int nc = ncand[i];
-  if (nc > NCAND) {
-    overflow[atomicAdd(noverflow, 1)] = i;
+  // one chunk of the flat list per wave
+  const int k = (nc < KSTAGE) ? nc : 0;  // overflowing lanes are listed later
+  int pre = k;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int v = __shfl_up(pre, o);
+    if (lane >= o) pre += v;
+  }
+  const int total = __shfl(pre, 63);
+  pre -= k;
+  int base = 0;
+  if (lane == 0 && total > 0) base = atomicAdd(O.used, total);
+  base = __shfl(base, 0);
+  if (i < 0) return;
+  O.qcnt[i] = nc;
+  if (nc > KSTAGE) {
+    O.ovf[atomicAdd(O.novf, 1)] = i;
     return;
   }
-  double p[3];
-  load_xyz(S.xs, i, p);
-  int res = -1, flag = 0;
-  const int *mine = cand + (size_t)i * NCAND;
-  for (int k = 0; k < nc; k++) {
-    const int c = mine[k];
-    contain(S, c & 0x0fffffff, c >> 28, p, res, flag);
-    if (res > -1 && flag == 0) break;
+  const int off = base + pre;
+  O.qoff[i] = off;
+  if (off + k > O.cap) return;  // flat list too small: detected by the host, which grows it and reruns
+  for (int j = 0; j < k; j++) {
+    O.cell[off + j] = stage[j][lane];
+    O.query[off + j] = i;
   }
-  S.donorId[i] = res;
 }
 
-// The walk of one query with the tests interleaved, exactly as the
-// reference runs it; used for the queries with more than NCAND candidates.
-constexpr int LANE_BS = 64;
-__global__ void __launch_bounds__(LANE_BS) k_search_lane(SearchView S, const AdtNode *__restrict__ nodes,
-                                                         const int *__restrict__ list, const int *__restrict__ count)
+// the walk of one query, listing all its candidates (queries with more than
+// KSTAGE); a grid-stride loop over the queue
+__global__ void __launch_bounds__(64) k_collect_more(SearchView S, const AdtNode *__restrict__ nodes, CandOut O)
 {
-  __shared__ int stk[STACK][LANE_BS];
-  const int n = *count;
-  const int tid = threadIdx.x;
-  for (int k = blockIdx.x * LANE_BS + tid; k < n; k += gridDim.x * LANE_BS) {
-    const int i = list[k];
+  __shared__ int stk[STACK][64];
+  const int n = *O.novf, tid = threadIdx.x;
+  for (int q = blockIdx.x * 64 + tid; q < n; q += gridDim.x * 64) {
+    const int i = O.ovf[q];
+    const int cnt = O.qcnt[i];
+    const int off = atomicAdd(O.used, cnt);
+    O.qoff[i] = off;
+    if (off + cnt > O.cap) continue;
     double p[3];
     load_xyz(S.xs, i, p);
-    int res = -1, flag = 0, sp = 0;
+    int sp = 0, nc = 0;
     stk[sp++][tid] = 0;
     while (sp > 0) {
       const AdtNode &N = nodes[stk[--sp][tid]];
-      if (in_box(p, N.ebox)) {
-        contain(S, N.cell, N.nv, p, res, flag);
-        if (res > -1 && flag == 0) break;
+      if (in_box_all(p, N.ebox)) {
+        O.cell[off + nc] = N.cell | (N.nv << 28);
+        O.query[off + nc] = i;
+        nc++;
       }
-      if (N.right > -1 && in_box(p, N.rbox)) stk[sp++][tid] = N.right;
-      if (N.left > -1 && in_box(p, N.lbox)) stk[sp++][tid] = N.left;
+      if (N.right > -1 && in_box_all(p, N.rbox)) stk[sp++][tid] = N.right;
+      if (N.left > -1 && in_box_all(p, N.lbox)) stk[sp++][tid] = N.left;
     }
-    S.donorId[i] = res;
   }
+}
+
+// containment test of every listed candidate: 0 reject, 1 accepted with a
+// near-zero weight in a BIGVALUE cell, 2 clean hit
+__global__ void __launch_bounds__(256) k_eval(SearchView S, CandOut O)
+{
+  const int n = min(*O.used, O.cap);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const int c = O.cell[j];
+    double p[3];
+    load_xyz(S.xs, O.query[j], p);
+    int res = -1, flag = 0;
+    contain(S, c & 0x0fffffff, c >> 28, p, res, flag);
+    O.status[j] = res < 0 ? 0 : (flag ? 1 : 2);
+  }
+}
+
+// the reference's result from the candidates' outcomes, in walk order
+__global__ void k_select(SearchView S, CandOut O)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S.nq) return;
+  if (S.xtag && S.xtag[i] != i) return;
+  const int nc = O.qcnt[i], off = O.qoff[i];
+  int res = -1;
+  for (int k = 0; k < nc; k++) {
+    const int st = O.status[off + k];
+    const int cell = O.cell[off + k] & 0x0fffffff;
+    res = st ? cell : -1;
+    if (st == 2) break;
+  }
+  S.donorId[i] = res;
 }
 
 // coincident queries take their representative's donor (search.C:251-265)
@@ -1107,16 +1153,36 @@ void fill(T *p, size_t n, T v, hipStream_t s)
 
 // launches the donor search over S (fast wave-cooperative path, then the
 // queries that overflowed it)
-void run_search(const SearchView &S, const AdtNode *nodes, SearchScratch &w, hipStream_t s)
+void run_search(const SearchView &S, const AdtNode *nodes, SearchScratch &w, hipStream_t s, bool check)
 {
+  if (S.nq <= 0) return;
+  w.qoff.reserve(S.nq);
+  w.qcnt.reserve(S.nq);
   w.ovf.reserve(S.nq);
-  w.novf.reserve(1);
-  w.cand.reserve((size_t)S.nq * NCAND);
-  w.ncand.reserve(S.nq);
-  fill<int>(w.novf.p, 1, 0, s);
-  k_collect<<<nb(S.nq, WAVE), WAVE, 0, s>>>(S, nodes, w.cand.p, w.ncand.p);
-  k_contain<<<nb(S.nq, WAVE), WAVE, 0, s>>>(S, w.cand.p, w.ncand.p, w.ovf.p, w.novf.p);
-  k_search_lane<<<256, LANE_BS, 0, s>>>(S, nodes, w.ovf.p, w.novf.p);
+  w.counters.reserve(2);
+  if (w.cap == 0) w.cap = std::max<size_t>(4 * (size_t)S.nq, 1 << 20);
+  for (;;) {
+    w.cell.reserve(w.cap);
+    w.query.reserve(w.cap);
+    w.status.reserve(w.cap);
+    CandOut O{w.cell.p, w.query.p, w.status.p, w.qoff.p, w.qcnt.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
+              w.ovf.p, w.counters.p + 1};
+    fill<int>(w.counters.p, 2, 0, s);
+    k_collect<<<nb(S.nq, WAVE), WAVE, 0, s>>>(S, nodes, O);
+    k_collect_more<<<1024, 64, 0, s>>>(S, nodes, O);
+    if (check) {
+      int used = 0;
+      TG_HIP(hipMemcpyAsync(&used, w.counters.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      TG_HIP(hipStreamSynchronize(s));
+      if ((size_t)used > w.cap) {  // grow the flat list and walk again
+        w.cap = (size_t)used + used / 4;
+        continue;
+      }
+    }
+    k_eval<<<2048, 256, 0, s>>>(S, O);
+    k_select<<<nb(S.nq), BS, 0, s>>>(S, O);
+    return;
+  }
 }
 
 // a small device-side copy of a host struct, valid until the next call
@@ -1799,7 +1865,7 @@ void Engine::launch_query_set(QuerySet &q)
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
-  run_search(S, q.nodes.p, sw_, s_);
+  run_search(S, q.nodes.p, sw_, s_, !timing_);
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
 }
 
@@ -1821,11 +1887,16 @@ double Engine::time_query_sets(int nrep, long long *nq)
   // events around every k_search launch give the kernel's own duration
   std::vector<hipEvent_t> ev(2 * qsets_.size() * nrep);
   for (auto &e : ev) TG_HIP(hipEventCreate(&e));
+  // establish the candidate-list capacity with one checked pass per set
+  for (QuerySet *q : qsets_) launch_query_set(*q);
+  sync();
   kernel_events_ = &ev;
+  timing_ = true;
   Timer tm(s_);
   for (int r = 0; r < nrep; r++)
     for (QuerySet *q : qsets_) launch_query_set(*q);
   double pass = tm.stop() / nrep;
+  timing_ = false;
   kernel_events_ = nullptr;
   double k = 0;
   for (size_t i = 0; i + 1 < ev.size(); i += 2) {
@@ -1863,7 +1934,7 @@ void Engine::launch_search(Block &b)
   S.conn = b.conn.p;
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
-  run_search(S, b.nodes.p, sw_, s_);
+  run_search(S, b.nodes.p, sw_, s_, true);
   k_copy_dup<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.xtag.p, b.donorId.p);
 }
 
