@@ -672,7 +672,7 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchView S, const AdtNode
     }
   }
   // one chunk of the flat list per wave
-  const int k = (nc < KSTAGE) ? nc : 0;  // overflowing lanes are listed later
+  const int k = (nc <= KSTAGE) ? nc : 0;  // overflowing lanes are listed later
   int pre = k;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
