@@ -728,17 +728,64 @@ __global__ void __launch_bounds__(64) k_collect_more(SearchView S, const AdtNode
   }
 }
 
-// containment test of every listed candidate: 0 reject, 1 accepted with a
-// near-zero weight in a BIGVALUE cell, 2 clean hit
-__global__ void __launch_bounds__(256) k_eval(SearchView S, CandOut O)
+// Candidate tests binned by cell type (bucket 0..3 = 4, 5, 6, 8 vertices),
+// so each test kernel runs one type's code with that type's register budget;
+// the outcome goes to the candidate's own slot, so the binning order is free.
+__device__ __forceinline__ int type_bucket(int nv) { return nv == 4 ? 0 : (nv == 5 ? 1 : (nv == 6 ? 2 : 3)); }
+
+__global__ void __launch_bounds__(256) k_bin_count(CandOut O, int *__restrict__ bcount)
 {
+  __shared__ int loc[4];
+  if (threadIdx.x < 4) loc[threadIdx.x] = 0;
+  __syncthreads();
   const int n = min(*O.used, O.cap);
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    const int c = O.cell[j];
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    atomicAdd(&loc[type_bucket((unsigned)O.cell[j] >> 28)], 1);
+  __syncthreads();
+  if (threadIdx.x < 4 && loc[threadIdx.x]) atomicAdd(&bcount[threadIdx.x], loc[threadIdx.x]);
+}
+
+// bucket b's entries go to blist[bstart[b] ...]; bcount[4..7] are the fill cursors
+__global__ void __launch_bounds__(256) k_bin_fill(CandOut O, int *__restrict__ bcount, int *__restrict__ blist)
+{
+  __shared__ int loc[4], base[4];
+  if (threadIdx.x < 4) loc[threadIdx.x] = 0;
+  __syncthreads();
+  const int n = min(*O.used, O.cap);
+  const int stride = gridDim.x * blockDim.x;
+  const int start = blockIdx.x * blockDim.x + threadIdx.x;
+  // pass 1: this block's counts; pass 2: its slice of each bucket
+  for (int j = start; j < n; j += stride) atomicAdd(&loc[type_bucket((unsigned)O.cell[j] >> 28)], 1);
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    int bstart = 0;
+    for (int q = 0; q < (int)threadIdx.x; q++) bstart += bcount[q];
+    base[threadIdx.x] = bstart + (loc[threadIdx.x] ? atomicAdd(&bcount[4 + threadIdx.x], loc[threadIdx.x]) : 0);
+    loc[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  for (int j = start; j < n; j += stride) {
+    const int b = type_bucket((unsigned)O.cell[j] >> 28);
+    blist[base[b] + atomicAdd(&loc[b], 1)] = j;
+  }
+}
+
+// containment test of every listed candidate of one cell type: 0 reject,
+// 1 accepted with a near-zero weight in a BIGVALUE cell, 2 clean hit
+template <int NV, int WAVES>
+__global__ void __launch_bounds__(256, WAVES) k_eval(SearchView S, CandOut O, const int *__restrict__ bcount,
+                                                     const int *__restrict__ blist)
+{
+  constexpr int B = NV == 4 ? 0 : (NV == 5 ? 1 : (NV == 6 ? 2 : 3));
+  int b0 = 0;
+  for (int q = 0; q < B; q++) b0 += bcount[q];
+  const int n = bcount[B];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int j = blist[b0 + k];
     double p[3];
     load_xyz(S.xs, O.query[j], p);
     int res = -1, flag = 0;
-    contain(S, c & 0x0fffffff, c >> 28, p, res, flag);
+    contain_t<NV>(S, O.cell[j] & 0x0fffffff, p, res, flag);
     O.status[j] = res < 0 ? 0 : (flag ? 1 : 2);
   }
 }
@@ -1179,7 +1226,15 @@ void run_search(const SearchView &S, const AdtNode *nodes, SearchScratch &w, hip
         continue;
       }
     }
-    k_eval<<<2048, 256, 0, s>>>(S, O);
+    w.bins.reserve(8);
+    w.blist.reserve(w.cap);
+    fill<int>(w.bins.p, 8, 0, s);
+    k_bin_count<<<2048, 256, 0, s>>>(O, w.bins.p);
+    k_bin_fill<<<2048, 256, 0, s>>>(O, w.bins.p, w.blist.p);
+    k_eval<4, 8><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p);
+    k_eval<5, 4><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p);
+    k_eval<6, 4><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p);
+    k_eval<8, 4><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p);
     k_select<<<nb(S.nq), BS, 0, s>>>(S, O);
     return;
   }
