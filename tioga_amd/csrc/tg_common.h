@@ -66,43 +66,133 @@ TG_HD int solve3(double a[3][3], double b[3])
 //   f0 + f1 u + f2 v + f3 w + f4 uv + f5 vw + f6 wu + f7 uvw = 0
 // from (1/2,1/2,1/2), full step, absolute stop sqrt(|r|^2) <= 1e-14, at most
 // 500 iterations; failure yields (2,0,0) (math.c:83-146).
-TG_HD void newton3(const double f[8][3], double &uo, double &vo, double &wo)
+//
+// newton3_steps runs iterations [it, itstop) from the state (u, v, w); it
+// returns 1 when the iteration has ended (converged, or a singular Jacobian:
+// ok = 0) and 0 when it stopped at itstop, so a solve can be continued later
+// from the same state with identical arithmetic.
+TG_HD int newton3_steps(const double f[8][3], double &u, double &v, double &w, int &it, int itstop, int &ok)
 {
-  double u = 0.5, v = 0.5, w = 0.5;
-  int ok = 1, iter;
-  for (iter = 0; iter < 500; iter++) {
+  for (; it < itstop; it++) {
     double uv = u * v, vw = v * w, wu = w * u, uvw = u * v * w;
     double rhs[3], lhs[3][3];
     for (int j = 0; j < 3; j++)
       rhs[j] = f[0][j] + f[1][j] * u + f[2][j] * v + f[3][j] * w + f[4][j] * uv + f[5][j] * vw +
                f[6][j] * wu + f[7][j] * uvw;
     double nrm = rhs[0] * rhs[0] + rhs[1] * rhs[1] + rhs[2] * rhs[2];
-    if (sqrt(nrm) <= 1e-14) break;
+    if (sqrt(nrm) <= 1e-14) return 1;
     for (int j = 0; j < 3; j++) {
       lhs[j][0] = f[1][j] + f[4][j] * v + f[6][j] * w + f[7][j] * vw;
       lhs[j][1] = f[2][j] + f[5][j] * w + f[4][j] * u + f[7][j] * wu;
       lhs[j][2] = f[3][j] + f[6][j] * u + f[5][j] * v + f[7][j] * uv;
     }
     ok = solve3(lhs, rhs);
-    if (!ok) break;
+    if (!ok) return 1;
     u -= rhs[0];
     v -= rhs[1];
     w -= rhs[2];
   }
-  if (iter == 500 || !ok) {
+  return it >= 500;
+}
+
+constexpr int NEWTON_MAX = 500;
+
+// the outcome of a finished solve (math.c:127-138)
+TG_HD void newton3_finish(int it, int ok, double &u, double &v, double &w)
+{
+  if (it == NEWTON_MAX || !ok) {
     u = 2.0;
     v = w = 0.0;
   }
+}
+
+TG_HD void newton3(const double f[8][3], double &uo, double &vo, double &wo)
+{
+  double u = 0.5, v = 0.5, w = 0.5;
+  int it = 0, ok = 1;
+  newton3_steps(f, u, v, w, it, NEWTON_MAX, ok);
+  newton3_finish(it, ok, u, v, w);
   uo = u;
   vo = v;
   wo = w;
+}
+
+// Coefficients of the isoparametric map of a pyramid (5), prism (6) or hex
+// (8) about the point xp (math.c:196-281).
+TG_HD void iso_coeffs(const double xv[8][3], const double xp[3], double f[8][3], int nvert)
+{
+  if (nvert == 5) {
+    for (int j = 0; j < 3; j++) {
+      f[0][j] = xv[0][j] - xp[j];
+      f[1][j] = xv[1][j] - xv[0][j];
+      f[2][j] = xv[3][j] - xv[0][j];
+      f[3][j] = xv[4][j] - xv[0][j];
+      f[4][j] = xv[0][j] - xv[1][j] + xv[2][j] - xv[3][j];
+      f[5][j] = xv[0][j] - xv[3][j];
+      f[6][j] = xv[0][j] - xv[1][j];
+      f[7][j] = -xv[0][j] + xv[1][j] - xv[2][j] + xv[3][j];
+    }
+  } else if (nvert == 6) {
+    for (int j = 0; j < 3; j++) {
+      f[0][j] = xv[0][j] - xp[j];
+      f[1][j] = xv[1][j] - xv[0][j];
+      f[2][j] = xv[2][j] - xv[0][j];
+      f[3][j] = xv[3][j] - xv[0][j];
+      f[4][j] = 0;
+      f[5][j] = xv[0][j] - xv[2][j] - xv[3][j] + xv[5][j];
+      f[6][j] = xv[0][j] - xv[1][j] - xv[3][j] + xv[4][j];
+      f[7][j] = 0.;
+    }
+  } else {
+    for (int j = 0; j < 3; j++) {
+      f[0][j] = xv[0][j] - xp[j];
+      f[1][j] = xv[1][j] - xv[0][j];
+      f[2][j] = xv[3][j] - xv[0][j];
+      f[3][j] = xv[4][j] - xv[0][j];
+      f[4][j] = xv[0][j] - xv[1][j] + xv[2][j] - xv[3][j];
+      f[5][j] = xv[0][j] - xv[3][j] + xv[7][j] - xv[4][j];
+      f[6][j] = xv[0][j] - xv[1][j] + xv[5][j] - xv[4][j];
+      f[7][j] = -xv[0][j] + xv[1][j] - xv[2][j] + xv[3][j] + xv[4][j] - xv[5][j] + xv[6][j] - xv[7][j];
+    }
+  }
+}
+
+// Nodal weights from the parametric point of a pyramid, prism or hex
+// (math.c:213-221, 243-253, 276-287).
+TG_HD void iso_weights(double u, double v, double w, double frac[8], int nvert)
+{
+  if (nvert == 5) {
+    double omu = 1.0 - u, omv = 1.0 - v, omw = 1.0 - w;
+    frac[0] = omu * omv * omw;
+    frac[1] = u * omv * omw;
+    frac[2] = u * v * omw;
+    frac[3] = omu * v * omw;
+    frac[4] = w;
+  } else if (nvert == 6) {
+    double omuv = 1.0 - u - v, omw = 1.0 - w;
+    frac[0] = omuv * omw;
+    frac[1] = u * omw;
+    frac[2] = v * omw;
+    frac[3] = omuv * w;
+    frac[4] = u * w;
+    frac[5] = v * w;
+  } else {
+    double omu = 1.0 - u, omv = 1.0 - v, omw = 1.0 - w;
+    frac[0] = omu * omv * omw;
+    frac[1] = u * omv * omw;
+    frac[2] = u * v * omw;
+    frac[3] = omu * v * omw;
+    frac[4] = omu * omv * w;
+    frac[5] = u * omv * w;
+    frac[6] = u * v * w;
+    frac[7] = omu * v * w;
+  }
 }
 
 // Nodal interpolation weights of point xp in a tet (direct solve), pyramid,
 // prism or hex (Newton) cell (math.c:149-294).
 TG_HD void nodal_weights(const double xv[8][3], const double xp[3], double frac[8], int nvert)
 {
-  double f[8][3], u, v, w;
   if (nvert == 4) {
     double lhs[3][3], rhs[3];
     for (int k = 0; k < 3; k++) {
@@ -120,69 +210,11 @@ TG_HD void nodal_weights(const double xv[8][3], const double xp[3], double frac[
     }
     return;
   }
-  if (nvert == 5) {
-    for (int j = 0; j < 3; j++) {
-      f[0][j] = xv[0][j] - xp[j];
-      f[1][j] = xv[1][j] - xv[0][j];
-      f[2][j] = xv[3][j] - xv[0][j];
-      f[3][j] = xv[4][j] - xv[0][j];
-      f[4][j] = xv[0][j] - xv[1][j] + xv[2][j] - xv[3][j];
-      f[5][j] = xv[0][j] - xv[3][j];
-      f[6][j] = xv[0][j] - xv[1][j];
-      f[7][j] = -xv[0][j] + xv[1][j] - xv[2][j] + xv[3][j];
-    }
-    newton3(f, u, v, w);
-    double omu = 1.0 - u, omv = 1.0 - v, omw = 1.0 - w;
-    frac[0] = omu * omv * omw;
-    frac[1] = u * omv * omw;
-    frac[2] = u * v * omw;
-    frac[3] = omu * v * omw;
-    frac[4] = w;
-    return;
-  }
-  if (nvert == 6) {
-    for (int j = 0; j < 3; j++) {
-      f[0][j] = xv[0][j] - xp[j];
-      f[1][j] = xv[1][j] - xv[0][j];
-      f[2][j] = xv[2][j] - xv[0][j];
-      f[3][j] = xv[3][j] - xv[0][j];
-      f[4][j] = 0;
-      f[5][j] = xv[0][j] - xv[2][j] - xv[3][j] + xv[5][j];
-      f[6][j] = xv[0][j] - xv[1][j] - xv[3][j] + xv[4][j];
-      f[7][j] = 0.;
-    }
-    newton3(f, u, v, w);
-    double omuv = 1.0 - u - v, omw = 1.0 - w;
-    frac[0] = omuv * omw;
-    frac[1] = u * omw;
-    frac[2] = v * omw;
-    frac[3] = omuv * w;
-    frac[4] = u * w;
-    frac[5] = v * w;
-    return;
-  }
-  if (nvert == 8) {
-    for (int j = 0; j < 3; j++) {
-      f[0][j] = xv[0][j] - xp[j];
-      f[1][j] = xv[1][j] - xv[0][j];
-      f[2][j] = xv[3][j] - xv[0][j];
-      f[3][j] = xv[4][j] - xv[0][j];
-      f[4][j] = xv[0][j] - xv[1][j] + xv[2][j] - xv[3][j];
-      f[5][j] = xv[0][j] - xv[3][j] + xv[7][j] - xv[4][j];
-      f[6][j] = xv[0][j] - xv[1][j] + xv[5][j] - xv[4][j];
-      f[7][j] = -xv[0][j] + xv[1][j] - xv[2][j] + xv[3][j] + xv[4][j] - xv[5][j] + xv[6][j] - xv[7][j];
-    }
-    newton3(f, u, v, w);
-    double omu = 1.0 - u, omv = 1.0 - v, omw = 1.0 - w;
-    frac[0] = omu * omv * omw;
-    frac[1] = u * omv * omw;
-    frac[2] = u * v * omw;
-    frac[3] = omu * v * omw;
-    frac[4] = omu * omv * w;
-    frac[5] = u * omv * w;
-    frac[6] = u * v * w;
-    frac[7] = omu * v * w;
-  }
+  if (nvert != 5 && nvert != 6 && nvert != 8) return;
+  double f[8][3], u, v, w;
+  iso_coeffs(xv, xp, f, nvert);
+  newton3(f, u, v, w);
+  iso_weights(u, v, w, frac, nvert);
 }
 
 // a . (b x c), expanded in the order of cellVolume.f90:99-109

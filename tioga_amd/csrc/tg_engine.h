@@ -178,7 +178,8 @@ void unique_nodes_device(int n, const double *x, const int *tag, double *rtag, i
 // Reusable device scratch of the donor search: per-query candidate lists
 // and the queue of queries with more candidates than a list holds.
 struct SearchScratch {
-  DBuf<int> cell, query, status, qoff, qcnt, ovf, counters, bins, blist;
+  DBuf<int> cell, query, status, qoff, qcnt, ovf, counters, bins, blist, lj, lcount;
+  DBuf<double4> lst;
   size_t cap = 0;  // capacity of the flat candidate list
 };
 

@@ -770,11 +770,45 @@ __global__ void __launch_bounds__(256) k_bin_fill(CandOut O, int *__restrict__ b
   }
 }
 
-// containment test of every listed candidate of one cell type: 0 reject,
-// 1 accepted with a near-zero weight in a BIGVALUE cell, 2 clean hit
+// Outcome of a containment test from the weights (checkContainment.C:62-85):
+// 0 reject, 1 accepted with a near-zero weight in a BIGVALUE cell, 2 clean
+template <int NV>
+__device__ __forceinline__ int test_status(const double frac[8], bool big)
+{
+  int flag = 0;
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    if ((frac[m] + TOL) * (frac[m] - 1.0 - TOL) > 0) return 0;
+    if (fabs(frac[m]) < TOL && big) flag = 1;
+  }
+  return flag ? 1 : 2;
+}
+
+template <int NV>
+__device__ __forceinline__ void cell_vertices(const SearchView &S, int c, double xv[8][3])
+{
+  const int *cv = S.conn + 8 * (size_t)c;
+#pragma unroll
+  for (int m = 0; m < NV; m++) load_xyz(S.x, cv[m], xv[m]);
+}
+
+// Long Newton solves (a few per cent of candidates run to the 500-iteration
+// cap) are continued in a separate pass so they do not hold up the lanes of
+// the short ones: the first pass stops every solve after NEWTON_SPLIT
+// iterations and queues the unfinished ones with their state.
+constexpr int NEWTON_SPLIT = 32;
+
+struct LongQueue {
+  int *j;        // candidate slot, per type region [bstart, bstart + cap)
+  double4 *st;   // (u, v, w, iterations so far)
+  int *n;        // per type count
+  int cap;       // per type capacity
+};
+
+// containment test of every listed candidate of one cell type
 template <int NV, int WAVES>
 __global__ void __launch_bounds__(256, WAVES) k_eval(SearchView S, CandOut O, const int *__restrict__ bcount,
-                                                     const int *__restrict__ blist)
+                                                     const int *__restrict__ blist, LongQueue L)
 {
   constexpr int B = NV == 4 ? 0 : (NV == 5 ? 1 : (NV == 6 ? 2 : 3));
   int b0 = 0;
@@ -782,11 +816,52 @@ __global__ void __launch_bounds__(256, WAVES) k_eval(SearchView S, CandOut O, co
   const int n = bcount[B];
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const int j = blist[b0 + k];
-    double p[3];
+    const int c = O.cell[j] & 0x0fffffff;
+    double p[3], xv[8][3], frac[8];
     load_xyz(S.xs, O.query[j], p);
-    int res = -1, flag = 0;
-    contain_t<NV>(S, O.cell[j] & 0x0fffffff, p, res, flag);
-    O.status[j] = res < 0 ? 0 : (flag ? 1 : 2);
+    cell_vertices<NV>(S, c, xv);
+    if (NV == 4) {
+      nodal_weights(xv, p, frac, 4);
+    } else {
+      double f[8][3], u = 0.5, v = 0.5, w = 0.5;
+      int it = 0, ok = 1;
+      iso_coeffs(xv, p, f, NV);
+      if (!newton3_steps(f, u, v, w, it, NEWTON_SPLIT, ok)) {
+        const int q = atomicAdd(&L.n[B], 1);
+        if (q < L.cap) {
+          L.j[B * L.cap + q] = j;
+          L.st[B * L.cap + q] = make_double4(u, v, w, (double)it);
+          continue;
+        }
+        newton3_steps(f, u, v, w, it, NEWTON_MAX, ok);  // queue full: finish here
+      }
+      newton3_finish(it, ok, u, v, w);
+      iso_weights(u, v, w, frac, NV);
+    }
+    O.status[j] = test_status<NV>(frac, S.cellRes[c] == BIGVALUE);
+  }
+}
+
+// the queued solves, continued from their saved state to the end
+template <int NV>
+__global__ void __launch_bounds__(256, 4) k_eval_long(SearchView S, CandOut O, LongQueue L)
+{
+  constexpr int B = NV == 5 ? 1 : (NV == 6 ? 2 : 3);
+  const int n = min(L.n[B], L.cap);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int j = L.j[B * L.cap + k];
+    const double4 st = L.st[B * L.cap + k];
+    const int c = O.cell[j] & 0x0fffffff;
+    double p[3], xv[8][3], frac[8], f[8][3];
+    load_xyz(S.xs, O.query[j], p);
+    cell_vertices<NV>(S, c, xv);
+    iso_coeffs(xv, p, f, NV);
+    double u = st.x, v = st.y, w = st.z;
+    int it = (int)st.w, ok = 1;
+    newton3_steps(f, u, v, w, it, NEWTON_MAX, ok);
+    newton3_finish(it, ok, u, v, w);
+    iso_weights(u, v, w, frac, NV);
+    O.status[j] = test_status<NV>(frac, S.cellRes[c] == BIGVALUE);
   }
 }
 
@@ -1231,10 +1306,19 @@ void run_search(const SearchView &S, const AdtNode *nodes, SearchScratch &w, hip
     fill<int>(w.bins.p, 8, 0, s);
     k_bin_count<<<2048, 256, 0, s>>>(O, w.bins.p);
     k_bin_fill<<<2048, 256, 0, s>>>(O, w.bins.p, w.blist.p);
-    k_eval<4, 8><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p);
-    k_eval<5, 4><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p);
-    k_eval<6, 4><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p);
-    k_eval<8, 4><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p);
+    const int lcap = (int)std::min<size_t>(w.cap / 4 + 1024, 0x7fffffff / 4);
+    w.lj.reserve(4 * (size_t)lcap);
+    w.lst.reserve(4 * (size_t)lcap);
+    w.lcount.reserve(4);
+    fill<int>(w.lcount.p, 4, 0, s);
+    LongQueue L{w.lj.p, w.lst.p, w.lcount.p, lcap};
+    k_eval<4, 8><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p, L);
+    k_eval<5, 4><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p, L);
+    k_eval<6, 4><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p, L);
+    k_eval<8, 4><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p, L);
+    k_eval_long<5><<<1024, 256, 0, s>>>(S, O, L);
+    k_eval_long<6><<<1024, 256, 0, s>>>(S, O, L);
+    k_eval_long<8><<<1024, 256, 0, s>>>(S, O, L);
     k_select<<<nb(S.nq), BS, 0, s>>>(S, O);
     return;
   }
