@@ -1083,22 +1083,21 @@ __global__ void k_interp_mandatory(InterpView V)
 
 // MeshBlock::findInterpData (bookKeeping.C:385-535) for record r, seeing the
 // iblank values the reference sees at that point of its sequential walk.
-__global__ void k_interp(InterpView V)
+// The cell's vertex count is a template constant, so every weight and vertex
+// array index is static (no run-time indexed private arrays).
+template <int NV>
+__device__ __forceinline__ void interp_record(const InterpView &V, int r, int c, int irec, double rr2, double rr)
 {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= V.n) return;
-  int k = V.sel[r];
-  double rr2 = V.L.rres[k], rr = fabs(rr2);
-  int irec = V.L.remote[k];
-  int meshtagrecv = V.tagsearch[irec];
-  int c = V.donorId[irec];
-  int nv = V.ti.nv[type_of(V.ti, c)];
+  const int meshtagrecv = V.tagsearch[irec];
   const int *cv = V.conn + 8 * (size_t)c;
   double xv[8][3], xp[3], frac[8];
+  int vert[NV];
   int accept = 1;
-  for (int m = 0; m < nv; m++) {
-    int v = cv[m];
-    int ib = (V.tmin[v] < r) ? 1 : V.iblank[v];
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    const int v = cv[m];
+    vert[m] = v;
+    const int ib = (V.tmin[v] < r) ? 1 : V.iblank[v];
     if (ib <= 0 && rr2 > 0.0) {
       if (V.nodeRes[v] == BIGVALUE) accept = 0;
       if (abs(ib) == meshtagrecv) accept = 0;
@@ -1106,22 +1105,39 @@ __global__ void k_interp(InterpView V)
     load_xyz(V.x, v, xv[m]);
   }
   load_xyz(V.xsearch, irec, xp);
-  nodal_weights(xv, xp, frac, nv);
+  nodal_weights(xv, xp, frac, NV);
   V.i2d[irec] = r;
   int bad = 0;
   const size_t n = V.n;
-  for (int m = 0; m < nv; m++) {
-    V.inode[m * n + r] = cv[m];
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    V.inode[m * n + r] = vert[m];
     V.w[m * n + r] = frac[m];
     if (frac[m] < -0.2 || frac[m] > 1.2) bad = 1;
   }
-  V.inode[nv * n + r] = c;
-  V.w[nv * n + r] = 0.0;
-  V.nw[r] = nv;
+  V.inode[NV * n + r] = c;
+  V.w[NV * n + r] = 0.0;
+  V.nw[r] = NV;
   V.cancel[r] = (accept == 0 && rr != BIGVALUE) ? 1 : 0;
   V.rnode[r] = V.isearch[3 * irec + 1];
   V.rblk[r] = V.isearch[3 * irec + 2];
   if (bad) atomicAdd(V.err, 1);
+}
+
+__global__ void k_interp(InterpView V)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= V.n) return;
+  const int k = V.sel[r];
+  const double rr2 = V.L.rres[k], rr = fabs(rr2);
+  const int irec = V.L.remote[k];
+  const int c = V.donorId[irec];
+  switch (V.ti.nv[type_of(V.ti, c)]) {
+    case 4: interp_record<4>(V, r, c, irec, rr2, rr); break;
+    case 5: interp_record<5>(V, r, c, irec, rr2, rr); break;
+    case 6: interp_record<6>(V, r, c, irec, rr2, rr); break;
+    default: interp_record<8>(V, r, c, irec, rr2, rr); break;
+  }
 }
 
 __global__ void k_interp_reset(int nnodes, const int *__restrict__ tmin, int *__restrict__ iblank)
