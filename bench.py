@@ -82,6 +82,48 @@ def dataupdate_bytes(tg, blocks, nvar: int) -> float:
     return tot
 
 
+def amr_gather_leg(shell, nvar: int = 8, updates: int = 1000):
+    """The AMR gather workload: the mixed-element shell, every node a
+    receptor, against 4096 patches of 16^3 cells on 3 levels (8 FP64 fields
+    per patch); `updates` device-resident dataUpdate_AMR gathers with fixed
+    donor weights, timed with HIP events on the engine's stream."""
+    from tioga_amd import workloads as W
+    from tioga_amd.tioga import Tioga
+
+    t0 = time.time()
+    body = W.all_receptors(shell)
+    grid = W.config5_grid()
+    pq = grid.patch_fields(nvar, seed=5)
+    log(f"AMR workload built in {time.time() - t0:.1f} s: {grid.ngrids} patches, "
+        f"{sum(q.size for q in pq) * 8 / 2**30:.2f} GiB of patch fields")
+    tg = Tioga()
+    tg.register_grid(body)
+    tg.register_amr(grid, pq)
+    tg.preprocess()
+    t0 = time.time()
+    tg.connect_amr()
+    t_conn = time.time() - t0
+    q = np.random.default_rng(6).uniform(-1, 1, body.nnodes * nvar)
+    tg.register_solution(body.tag, q)
+    ms, nrec = tg.time_data_update_amr(nvar, 0, updates)
+    uniq = int(tg.amr_array(0, "rec_unique_cells")[0])
+    nrcp = int(tg.amr_array(body.tag, "receptorIdCart").size)
+    tg.close()
+    # per record: patch, node, block|stencil bits, base index (4 B each), six
+    # basis values (48 B) read and nvar results (8 B each) written; per
+    # distinct donor cell its nvar field values read once
+    algo = nrec * (16 + 48 + 8 * nvar) + 8 * nvar * uniq
+    gbps = algo / (ms / 1000) / 1e9 if ms > 0 else None
+    log(f"AMR gather: {nrec} records, {uniq} donor cells, {ms:.4f} ms per update")
+    return {"workload": "3 levels (ratio 2), 4096 patches of 16^3 cells, one ghost layer, pdegree 0, "
+                        f"{nvar} FP64 fields; receptors: every node of the mixed-element shell",
+            "receptors": nrcp, "records": int(nrec), "unique_donor_cells": uniq, "updates": updates,
+            "kernel": "k_amr_gather", "connectivity_amr_ms": round(1000 * t_conn, 2),
+            "ms_per_update": round(ms, 5), "algorithmic_bytes_per_update": algo,
+            "GBps": round(gbps, 2) if gbps else None,
+            "frac": round(gbps / HBM_PEAK_GBPS, 4) if gbps else None}
+
+
 def cpu_baseline(blocks, sets, budget_s: float):
     """The oracle's sequential donor search (tioga_oracle.c, restating
     searchADTrecursion.C/checkContainment.C) on a bounded random sample of
@@ -120,16 +162,28 @@ def main() -> int:
     ap.add_argument("--workload", default="config2")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU search for the baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-amr", action="store_true", help="skip the AMR gather leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one rank on one GPU; multi-GPU runs are not part of "
-                                   "this build (see DESIGN.md for how the path shards)", "n_gpus": args.gpus}))
+                                   "this build; the path shards by mesh partition, one rank per GPU", "n_gpus": args.gpus}))
         return 2
 
     import torch
 
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
+    if args.calibrate_copy:
+        n = args.calibrate_copy * 2**20 // 8
+        a = torch.rand(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": 8 * n, "copies": 5}), flush=True)
+        return 0
     from tioga_amd.tioga import Tioga
 
     t0 = time.time()
@@ -193,6 +247,8 @@ def main() -> int:
                          f"{tsamp:.1f} s"}
 
     bdu = dataupdate_bytes(tg, blocks, nvar)
+    tg.close()
+    amr = None if args.no_amr else amr_gather_leg(blocks[0])
     out = {
         "metric": "donor searches/s on 1 GPU",
         "value": round(value, 1),
@@ -220,9 +276,9 @@ def main() -> int:
         "dataupdate": {"ms": round(du_ms, 4), "receptors": int(nrec), "nvar": nvar,
                        "algorithmic_bytes": bdu,
                        "GBps": round(bdu / (du_ms / 1000) / 1e9, 2) if du_ms > 0 else None},
+        "amr_gather": amr,
     }
     print(json.dumps(out), flush=True)
-    tg.close()
     return 0
 
 

@@ -4,7 +4,7 @@
 // Public methods, argument meaning and defaults follow the reference class
 // (src/tioga.h:47-261).  Private state is an engine handle, so users
 // recompile against this header.  Methods whose feature is outside this
-// engine (AMR Cartesian connectivity, high-order and p4est callbacks) are not
+// engine (high-order and p4est callbacks, the AMR donor callback) are not
 // declared, so a build that needs them fails at compile time.
 #ifndef TIOGA_AMD_TIOGA_H
 #define TIOGA_AMD_TIOGA_H
@@ -63,6 +63,18 @@ class tioga {
   void set_uniform_hex_flag(int btag, int flag);               // tioga.h:195-201 (accepted; the
                                                                // general search is always used)
   void reduce_fringes(void);                                   // tioga.C:737-885
+
+  // AMR Cartesian background, gather direction (near-body receptors take
+  // their values from Cartesian patches of pdegree 0)
+  void register_amr_global_data(int nf, int qstride, double *qnodein, int *idata, double *rdata, int ngridsin,
+                                int qnodesize);                                  // tioga.C:692-701
+  void set_amr_patch_count(int npatchesin);                                      // tioga.C:703-708
+  void register_amr_local_data(int ipatch, int global_id, int *iblank, double *q);  // tioga.C:710-713
+  void performConnectivityAMR(void);                           // tioga.C:180-216
+  void dataUpdate_AMR(int nvar, int interptype);               // tioga.C:218-334
+  // the reference's iamrGlobal: set once performConnectivityAMR ran with
+  // patches; tioga_dataupdate_mb_ then routes to dataUpdate_AMR
+  int iamrGlobal() const;
 
   tg::Engine *engine() { return eng_; }
 

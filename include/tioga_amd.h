@@ -75,15 +75,30 @@ void tioga_setmexclude_(int *mexclude);
 /* tiogaInterface.C:317-328 */
 void tioga_delete_(void);
 
-/* Out of scope of this engine (AMR Cartesian connectivity, high-order and
-   p4est callbacks); they link, print that the feature is not available and
-   set the error status.  tiogaInterface.C:132-148, 160-168, 273-300. */
+/* AMR Cartesian background, gather direction: near-body receptors (nodes of
+   cells whose nodes all carry nodeRes == BIGVALUE and whose cell iblank is 1,
+   cartOps.C:61-160) take their values from Cartesian patches of pdegree 0
+   by linear interpolation between cell centres.  One rank: every grid has
+   proc_id 0 and a local id of a registered patch.
+   tiogaInterface.C:132-136 (11 ints: global id, level, proc, porder, local
+   id, ilo[3], ihi[3]; 6 doubles: xlo[3], dx[3] per grid) */
 void tioga_register_amr_global_data_(int *nf, int *qstride, double *qnodein, int *idata, double *rdata,
                                      int *ngridsin, int *qnodesize);
+/* tiogaInterface.C:140-143 */
 void tioga_register_amr_patch_count_(int *npatches);
+/* tiogaInterface.C:145-148; q holds the patch's fields, field n at
+   n*prod(dims+2*nf) (get_amr_index_xyz.c:27-45) */
 void tioga_register_amr_local_data_(int *ipatch, int *global_id, int *iblank, double *q);
-void tioga_performconnectivity_highorder_(void);
+/* tiogaInterface.C:165-168 (tioga.C:180-216 without the Cartesian-receptor
+   direction and without its closing MPI_Abort); writes the near-body iblank
+   (-1 at receptors that found a patch) and iblank_cell.  After it,
+   tioga_dataupdate_mb_ runs dataUpdate_AMR (tiogaInterface.C:194-204). */
 void tioga_performconnectivity_amr_(void);
+
+/* Out of scope of this engine (high-order and p4est callbacks, the AMR donor
+   callback of pdegree > 0 patches); they link, print that the feature is not
+   available and set the error status.  tiogaInterface.C:160-163, 273-300. */
+void tioga_performconnectivity_highorder_(void);
 void tioga_set_highorder_callback_(void (*f1)(int *, int *), void (*f2)(int *, int *, double *),
                                    void (*f3)(int *, double *, int *, double *),
                                    void (*f4)(int *, double *, int *, int *, double *, double *, int *),
@@ -120,6 +135,19 @@ double tioga_amd_time_search(int nrep, long long *nsearch);
    resident in HBM (uploaded once from the registered arrays); *nreceptors =
    interpolation records per pass. */
 double tioga_amd_time_dataupdate(int nvar, int interptype, int nrep, long long *nreceptors);
+
+/* Device time (ms per update) of nrep dataUpdate_AMR gathers with the
+   patch and block solutions resident in HBM; *nrecords = interpolation
+   records per update. */
+double tioga_amd_time_dataupdate_amr(int nvar, int interptype, int nrep, long long *nrecords);
+
+/* AMR state after tioga_performconnectivity_amr_: for btag >= 1 the block's
+   "receptorIdCart", "donorIdCart" (grid index or -1) and "rxyzCart"; for
+   btag 0 the patch interpolation records in list order: "rec_patch",
+   "rec_info" (rank, point, block), "rec_ijk" (8 stencil cells i,j,k),
+   "rec_w" (8 weights) and "rec_unique_cells" (1 int: distinct donor cells).
+   Returns the element count (-1: unknown name). */
+long tioga_amd_amr_array(int btag, const char *name, void *out);
 
 /* Query sets: nq points (xyz AoS, host memory) searched against block
    btag, whose search structure is built the way MeshBlock::search builds it

@@ -70,6 +70,15 @@ def lib():
         L.orc_unique_nodes_octree.argtypes = [_P, _P, _P, _P, C.c_int]
         L.orc_build_adt.argtypes = [C.c_int, _P, _P, _P, _P]
         L.orc_weight_aborts.restype = C.c_int
+        L.orc_amr_global_data.argtypes = [_P, C.c_int, C.c_int, _P, _P, _P, C.c_int, C.c_int]
+        L.orc_amr_patch_count.argtypes = [_P, C.c_int]
+        L.orc_amr_local_data.argtypes = [_P, C.c_int, C.c_int, _P, _P]
+        L.orc_amr_search.argtypes = [_P, _P, C.c_int, _P]
+        L.orc_perform_connectivity_amr.argtypes = [_P]
+        L.orc_perform_connectivity_amr.restype = C.c_int
+        L.orc_data_update_amr.argtypes = [_P, C.c_int, C.c_int]
+        L.orc_amr_array.argtypes = [_P, C.c_char_p, _P]
+        L.orc_amr_array.restype = C.c_long
         _lib = L
     return _lib
 
@@ -78,7 +87,8 @@ def ptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data_as(_P)
 
 
-_I32_NAMES = {"isearch", "tagsearch", "donorId", "xtag", "elementList", "adtIntegers", "iblank_cell"}
+_I32_NAMES = {"isearch", "tagsearch", "donorId", "xtag", "elementList", "adtIntegers", "iblank_cell",
+              "receptorIdCart", "donorIdCart"}
 
 
 class Oracle:
@@ -169,6 +179,64 @@ class Oracle:
         self.L.orc_hole_map(self.h, mtag, ptr(nx), ptr(ext), ptr(sam))
         return nx, ext, sam
 
+    # ---- AMR Cartesian background, gather direction ----
+    def register_amr(self, grid, patch_q: Sequence[np.ndarray]) -> None:
+        idata = np.ascontiguousarray(grid.idata, np.int32)
+        rdata = np.ascontiguousarray(grid.rdata, np.float64)
+        qnode = np.ascontiguousarray(grid.qnode, np.float64)
+        self._keep += [idata, rdata, qnode]
+        self.L.orc_amr_global_data(self.h, grid.nf, grid.qstride, ptr(qnode), ptr(idata), ptr(rdata), grid.ngrids,
+                                   qnode.size)
+        self.L.orc_amr_patch_count(self.h, len(patch_q))
+        order = grid.patch_order()
+        for lp, q in enumerate(patch_q):
+            g = int(order[lp])
+            ibl = np.ones(grid.d3nf(g), np.int32)
+            q = np.ascontiguousarray(q, np.float64)
+            self._keep += [ibl, q]
+            self.L.orc_amr_local_data(self.h, lp, g, ptr(ibl), ptr(q))
+
+    def connect_amr(self) -> int:
+        return int(self.L.orc_perform_connectivity_amr(self.h))
+
+    def amr_search(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        out = np.zeros(x.shape[0], np.int32)
+        self.L.orc_amr_search(self.h, ptr(x), x.shape[0], ptr(out))
+        return out
+
+    def amr_array(self, name: str) -> np.ndarray:
+        n = self.L.orc_amr_array(self.h, name.encode(), None)
+        out = np.zeros(max(n, 0), np.float64 if name == "rec_w" else np.int32)
+        if n > 0:
+            self.L.orc_amr_array(self.h, name.encode(), ptr(out))
+        return out
+
+    def data_update_amr(self, qs: Sequence[np.ndarray], nvar: int, interptype: int = 0) -> List[np.ndarray]:
+        out = [np.ascontiguousarray(q, dtype=np.float64).copy() for q in qs]
+        for b, q in zip(self.blocks, out):
+            self.L.orc_register_solution(self.h, b.tag, ptr(q))
+        self.L.orc_data_update_amr(self.h, nvar, interptype)
+        return out
+
+    def amr_results(self, qs=None, nvar: int = 0, interptype: int = 0) -> Dict[str, np.ndarray]:
+        """The arrays the reference driver dumps for the AMR gather path."""
+        r: Dict[str, np.ndarray] = {}
+        for i, b in enumerate(self.blocks):
+            p = f"b{i}."
+            r[p + "iblank"] = self.iblank[i].copy()
+            r[p + "iblank_cell"] = self.iblank_cell[i][:b.ncells].copy()
+            r[p + "nodeRes"] = self.array(b.tag, "nodeRes")
+            for nm in ("receptorIdCart", "donorIdCart", "rxyzCart"):
+                r[p + nm] = self.array(b.tag, nm)
+            r[p + "ncart"] = np.array([r[p + "receptorIdCart"].size], np.int32)
+        for nm in ("rec_patch", "rec_info", "rec_ijk", "rec_w"):
+            r["amr." + nm] = self.amr_array(nm)
+        if qs is not None:
+            for i, q in enumerate(self.data_update_amr(qs, nvar, interptype)):
+                r[f"b{i}.q"] = q
+        return r
+
     def results(self, qs: Optional[Sequence[np.ndarray]] = None, nvar: int = 0,
                 interptype: int = 0) -> Dict[str, np.ndarray]:
         """The arrays the reference driver dumps, under the same names."""
@@ -204,10 +272,13 @@ class Oracle:
 
 def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptype: int = 0,
                isym: int = 3, nfringe: int = 1, mexclude: int = 3,
-               moves: Sequence[Sequence[Optional[np.ndarray]]] = ()) -> None:
+               moves: Sequence[Sequence[Optional[np.ndarray]]] = (), amr=None) -> None:
+    """amr = (AmrGrid, patch fields per local patch) selects the AMR gather
+    mode of the driver (format version 2)."""
     with open(path, "wb") as f:
         w = lambda fmt, *v: f.write(struct.pack("<" + fmt, *v))
-        w("9i", 0x41474954, 1, len(blocks), isym, nfringe, mexclude, nvar, interptype, len(moves))
+        w("9i", 0x41474954, 1 if amr is None else 2, len(blocks), isym, nfringe, mexclude, nvar, interptype,
+          len(moves))
         for b, q in zip(blocks, qs):
             b = b.normalized()
             has = b.nres is not None
@@ -222,6 +293,17 @@ def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptyp
                 f.write(b.nres.tobytes())
                 f.write(b.cres.tobytes())
             f.write(np.ascontiguousarray(q, np.float64).tobytes())
+        if amr is not None:
+            grid, pq = amr
+            w("i", 1)
+            w("5i", grid.nf, grid.qstride, grid.ngrids, grid.qnode.size, len(pq))
+            f.write(np.ascontiguousarray(grid.qnode, np.float64).tobytes())
+            f.write(np.ascontiguousarray(grid.idata, np.int32).tobytes())
+            f.write(np.ascontiguousarray(grid.rdata, np.float64).tobytes())
+            order = grid.patch_order()
+            for lp, q in enumerate(pq):
+                w("i", int(order[lp]))
+                f.write(np.ascontiguousarray(q, np.float64).tobytes())
         for step in moves:
             for xb in step:
                 if xb is None:
@@ -250,10 +332,12 @@ def read_result(path: str) -> Dict[str, np.ndarray]:
     return out
 
 
-def run_reference(case_path: str, out_path: str, timeout: float = 600.0) -> Dict[str, np.ndarray]:
-    """Run the compiled reference (oracle/_ref/ref_driver) on one rank."""
+def run_reference(case_path: str, out_path: str, timeout: float = 600.0,
+                  cwd: Optional[str] = None) -> Dict[str, np.ndarray]:
+    """Run the compiled reference (oracle/_ref/ref_driver) on one rank
+    (in cwd: the reference writes a cartGrid.dat there for AMR cases)."""
     env = dict(os.environ)
     env.setdefault("HYDRA_LAUNCHER", "fork")
-    subprocess.run([REF_DRIVER, case_path, out_path], check=True, timeout=timeout, env=env,
+    subprocess.run([REF_DRIVER, case_path, out_path], check=True, timeout=timeout, env=env, cwd=cwd,
                    stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     return read_result(out_path)

@@ -19,6 +19,13 @@
 //              f64 q[nnodes*nvar]
 //   per step s >= 1 (moving meshes): per block int32 moved;
 //              if moved: f64 x[3*nnodes] (new coordinates, same topology)
+//   version 2 adds, after the blocks: int32 has_amr; if has_amr:
+//              int32 nf, qstride, ngrids, qnodesize, npatch;
+//              f64 qnode[qnodesize]; int32 idata[11*ngrids];
+//              f64 rdata[6*ngrids]; per patch: int32 global_id,
+//              f64 q[d3nf*nvar] (d3nf = prod(dims+2*nf) of that grid).
+//   With AMR patches the driver runs the gather direction of the AMR path
+//   (see run_amr) instead of performConnectivity; nsteps must be 0.
 // Result file: records {int32 namelen, name, int32 dtype(0=i32,1=f64),
 //              int64 count, payload}.  Step s > 0 prefixes names with "s<s>.".
 #include <cstdint>
@@ -131,6 +138,98 @@ void dump(tioga *tg, std::vector<Block> &blocks, Writer &w, double tconn)
   }
 }
 
+struct Amr {
+  int nf = 0, qstride = 0, ngrids = 0, qnodesize = 0, npatch = 0;
+  std::vector<double> qnode, rdata;
+  std::vector<int> idata, pgid;
+  std::vector<std::vector<double>> q;
+  std::vector<std::vector<int>> ibl;
+};
+
+// tioga::performConnectivityAMR (tioga.C:180-216) on one rank without the
+// Cartesian-receptor direction (getCartReceptors, the near-body search of
+// Cartesian points and the donor half of exchangeAMRDonors) and without its
+// closing MPI_Abort; then tioga::dataUpdate_AMR (tioga.C:218-334).  Every
+// step is a call into the compiled reference except the loop that fills the
+// patch interpolation lists, which is exchangeAMRDonors.C:120-192 written
+// for a single rank (packets to self, so arrival order = packing order).
+void run_amr(tioga *tg, std::vector<Block> &blocks, Amr &A, int nvar, int itype, Writer &w)
+{
+  tg->register_amr_global_data(A.nf, A.qstride, A.qnode.data(), A.idata.data(), A.rdata.data(), A.ngrids,
+                               A.qnodesize);
+  tg->set_amr_patch_count(A.npatch);
+  for (int p = 0; p < A.npatch; p++) tg->register_amr_local_data(p, A.pgid[p], A.ibl[p].data(), A.q[p].data());
+  tg->profile();
+  double t0 = MPI_Wtime();
+  tg->cg->preprocess();
+  for (int i = 0; i < tg->ncart; i++) tg->cb[i].preprocess(tg->cg);
+  for (int ib = 0; ib < tg->nblocks; ib++) {
+    MeshBlock *mb = tg->mblocks[ib].get();
+    mb->getUnresolvedMandatoryReceptors();
+    tg->cg->search(mb->rxyzCart, mb->donorIdCart, mb->ntotalPointsCart);
+  }
+  for (int i = 0; i < tg->ncart; i++) {
+    tg->cb[i].clearLists();
+    tg->cb[i].initializeLists();
+  }
+  for (int ib = 0; ib < tg->nblocks; ib++) {
+    MeshBlock *mb = tg->mblocks[ib].get();
+    for (int i = 0; i < mb->ntotalPointsCart; i++)
+      if (mb->donorIdCart[i] != -1) {
+        int gid = mb->donorIdCart[i];
+        tg->cb[tg->cg->local_id[gid]].insertInInterpList(0, i, ib, &mb->rxyzCart[3 * i]);
+      }
+  }
+  for (int ib = 0; ib < tg->nblocks; ib++) tg->mblocks[ib]->setCartIblanks();
+  for (int ib = 0; ib < tg->nblocks; ib++) tg->mblocks[ib]->getCellIblanks();
+  double tconn = MPI_Wtime() - t0;
+  w.f64("time_connectivity", &tconn, 1);
+  for (size_t b = 0; b < blocks.size(); b++) {
+    Block &B = blocks[b];
+    MeshBlock *mb = tg->mblocks[tg->tag_iblk_map.find(B.tag)->second].get();
+    std::string p = "b" + std::to_string(b) + ".";
+    w.i32(p + "iblank", B.iblank.data(), B.nnodes);
+    w.i32(p + "iblank_cell", B.iblank_cell.data(), B.ncells);
+    w.f64(p + "nodeRes", mb->nodeRes, B.nnodes);
+    int n = mb->ntotalPointsCart;
+    w.i32(p + "ncart", &n, 1);
+    w.i32(p + "receptorIdCart", mb->receptorIdCart, n);
+    w.i32(p + "donorIdCart", mb->donorIdCart, n);
+    w.f64(p + "rxyzCart", mb->rxyzCart, 3 * (int64_t)n);
+  }
+  std::vector<int> rpatch, rinfo, rijk;
+  std::vector<double> rw;
+  for (int i = 0; i < tg->ncart; i++)
+    for (INTERPLIST2 *l = tg->cb[i].interpList; l; l = l->next) {
+      rpatch.push_back(i);
+      rinfo.insert(rinfo.end(), l->receptorInfo, l->receptorInfo + 3);
+      rijk.insert(rijk.end(), l->inode, l->inode + 3 * l->nweights);
+      rw.insert(rw.end(), l->weights, l->weights + l->nweights);
+    }
+  w.i32("amr.rec_patch", rpatch.data(), (int64_t)rpatch.size());
+  w.i32("amr.rec_info", rinfo.data(), (int64_t)rinfo.size());
+  w.i32("amr.rec_ijk", rijk.data(), (int64_t)rijk.size());
+  w.f64("amr.rec_w", rw.data(), (int64_t)rw.size());
+  // dataUpdate_AMR needs the Cartesian communicator map that
+  // exchangeAMRDonors.C:58-60 sets: on one rank, send to and receive from self
+  int self = 0;
+  tg->pc_cart->setMap(1, 1, &self, &self);
+  // registerSolution stores into qblock, which only performConnectivity
+  // allocates (tioga.C:152-155); allocate it the same way
+  if (!tg->qblock) {
+    tg->qblock = (double **)malloc(sizeof(double *) * tg->nblocks);
+    for (int ib = 0; ib < tg->nblocks; ib++) tg->qblock[ib] = NULL;
+  }
+  std::vector<std::vector<double>> qs(blocks.size());
+  for (size_t b = 0; b < blocks.size(); b++) {
+    qs[b] = blocks[b].q;
+    tg->registerSolution(blocks[b].tag, qs[b].data());
+  }
+  tg->dataUpdate_AMR(nvar, itype);
+  for (size_t b = 0; b < blocks.size(); b++)
+    w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -142,7 +241,9 @@ int main(int argc, char **argv)
   MPI_Init(&argc, &argv);
   Reader r{fopen(argv[1], "rb")};
   if (!r.f) { perror(argv[1]); return 2; }
-  if (r.get<int32_t>() != 0x41474954 || r.get<int32_t>() != 1) { fprintf(stderr, "bad case\n"); return 2; }
+  if (r.get<int32_t>() != 0x41474954) { fprintf(stderr, "bad case\n"); return 2; }
+  const int version = r.get<int32_t>();
+  if (version != 1 && version != 2) { fprintf(stderr, "bad case version\n"); return 2; }
   int nblocks = r.get<int32_t>(), isym = r.get<int32_t>(), nfringe = r.get<int32_t>();
   int mexclude = r.get<int32_t>(), nvar = r.get<int32_t>(), itype = r.get<int32_t>();
   int nsteps = r.get<int32_t>();
@@ -166,6 +267,25 @@ int main(int argc, char **argv)
     B.iblank.assign(B.nnodes, 1);
     B.iblank_cell.assign(B.ncells > 0 ? B.ncells : 1, 1);
   }
+  Amr amr;
+  int has_amr = version >= 2 ? r.get<int32_t>() : 0;
+  if (has_amr) {
+    amr.nf = r.get<int32_t>(); amr.qstride = r.get<int32_t>(); amr.ngrids = r.get<int32_t>();
+    amr.qnodesize = r.get<int32_t>(); amr.npatch = r.get<int32_t>();
+    r.arr(amr.qnode, amr.qnodesize);
+    r.arr(amr.idata, 11 * (size_t)amr.ngrids);
+    r.arr(amr.rdata, 6 * (size_t)amr.ngrids);
+    amr.pgid.resize(amr.npatch);
+    amr.q.resize(amr.npatch);
+    amr.ibl.resize(amr.npatch);
+    for (int p = 0; p < amr.npatch; p++) {
+      int g = amr.pgid[p] = r.get<int32_t>();
+      size_t d3nf = 1;
+      for (int n = 0; n < 3; n++) d3nf *= (size_t)(amr.idata[11 * g + 8 + n] - amr.idata[11 * g + 5 + n] + 1 + 2 * amr.nf);
+      r.arr(amr.q[p], d3nf * nvar);
+      amr.ibl[p].assign(d3nf, 1);
+    }
+  }
   Writer w{fopen(argv[2], "wb"), ""};
   tioga *tg = new tioga;
   tg->setCommunicator(MPI_COMM_WORLD, 0, 1);
@@ -177,6 +297,10 @@ int main(int argc, char **argv)
                          B.obc.data(), B.ntypes, B.nv.data(), B.nc.data(), B.vconn.data());
     if (B.has_res) tg->setResolutions(B.tag, B.nres.data(), B.cres.data());
     tg->set_cell_iblank(B.tag, B.iblank_cell.data());
+  }
+  if (has_amr) {
+    run_amr(tg, blocks, amr, nvar, itype, w);
+    nsteps = -1;
   }
   for (int s = 0; s <= nsteps; s++) {
     if (s > 0) {

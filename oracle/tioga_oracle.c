@@ -10,10 +10,11 @@
  * cell lists, first-hit ADT traversal, in-place mutation during
  * findInterpData) the order is reproduced as written.
  *
- * Parity status of every function is recorded in DESIGN.md; the leaf
- * numerics and the whole path are pinned against the compiled reference in
- * oracle/_ref (tests/test_oracle_vs_ref.py) and against closed forms
- * (tests/test_oracle_closed_forms.py).
+ * Every function is pinned against the compiled reference in oracle/_ref:
+ * tests/test_oracle_golden.py compares the whole path with the golden
+ * fixtures that reference wrote (tests/golden/), and
+ * tests/test_oracle_closed_forms.py checks the leaf numerics against closed
+ * forms.
  *
  * Build flags: -O2 -ffp-contract=off, no fast-math.
  */
@@ -768,6 +769,10 @@ typedef struct {
   int *cancelList;
   int ncancel, cancelCap;
   double *q;
+  /* receptors of the AMR Cartesian background (cartOps.C:61-160) */
+  int ncart;
+  int *receptorIdCart, *donorIdCart;
+  double *rxyzCart;
 } OrcBlock;
 
 typedef struct { int existWall, nx[3]; int *sam; double ext[6]; } OrcHM;
@@ -778,6 +783,27 @@ typedef struct {
   OrcOBB obc;
 } OrcPair;
 
+/* One interpolation record of a Cartesian patch: INTERPLIST2 as filled by
+   CartBlock::insertInInterpList (CartBlock.C:196-268) for pdegree 0. */
+typedef struct {
+  int rinfo[3];
+  int ijk[24];
+  double w[8];
+} AmrRec;
+
+/* CartGrid (CartGrid.h:26-76) and the CartBlock array of one rank. */
+typedef struct {
+  int ngrids, nf, qstride, qnodesize, maxlevel;
+  int *gid, *level, *proc, *porder, *lid, *ilo, *ihi, *dims;
+  double *xlo, *dx, *qnode, xlosup[3], *dxlvl;
+  int npatch;
+  int *pgid, **pibl;
+  double **pq;
+  int *roff; /* npatch + 1 offsets of each patch's records in rec */
+  AmrRec *rec;
+  int nrec;
+} OrcCart;
+
 struct OrcTioga {
   int nblocks;
   OrcBlock *b[MAXBLK];
@@ -786,6 +812,8 @@ struct OrcTioga {
   OrcHM *hm;
   OrcPair *pairs;
   int npairs, nsend;
+  OrcCart *cart;
+  int amr;
 };
 
 static void cell_locate(const OrcBlock *b, int icell, int *n, int *i)
@@ -835,8 +863,18 @@ static void free_block(OrcBlock *b)
   free(b->donorId); free(b->xtag); free(b->ebox); free(b->elist); free(b->adtInt); free(b->adtReal);
   free_donor_lists(b);
   free(b->interp); free(b->interp2donor); free(b->cancelList);
+  free(b->receptorIdCart); free(b->donorIdCart); free(b->rxyzCart);
   if (b->own_iblank_cell) free(b->iblank_cell);
   free(b);
+}
+
+static void free_cart(OrcCart *c)
+{
+  if (!c) return;
+  free(c->gid); free(c->level); free(c->proc); free(c->porder); free(c->lid);
+  free(c->ilo); free(c->ihi); free(c->dims); free(c->xlo); free(c->dx); free(c->qnode);
+  free(c->dxlvl); free(c->pgid); free(c->pibl); free(c->pq); free(c->roff); free(c->rec);
+  free(c);
 }
 
 void orc_destroy(OrcTioga *t)
@@ -849,6 +887,7 @@ void orc_destroy(OrcTioga *t)
     free(t->hm);
   }
   free(t->pairs);
+  free_cart(t->cart);
   free(t);
 }
 
@@ -1967,6 +2006,9 @@ long orc_block_array(OrcTioga *t, int btag, const char *name, void *out)
   if (!strcmp(name, "adtReals")) COPY_OUT(b->adtReal, 6 * b->nelem, double);
   if (!strcmp(name, "adtExtents")) COPY_OUT(b->adtExt, b->nsearch > 0 ? 6 : 0, double);
   if (!strcmp(name, "iblank_cell")) COPY_OUT(b->iblank_cell, b->ncells, int);
+  if (!strcmp(name, "receptorIdCart")) COPY_OUT(b->receptorIdCart, b->ncart, int);
+  if (!strcmp(name, "donorIdCart")) COPY_OUT(b->donorIdCart, b->ncart, int);
+  if (!strcmp(name, "rxyzCart")) COPY_OUT(b->rxyzCart, 3 * b->ncart, double);
   if (!strcmp(name, "obb")) {
     if (out) {
       double *o = (double *)out;
@@ -2003,4 +2045,330 @@ double orc_search_points(OrcTioga *t, int btag, const double *xq, int nq, int *d
   for (i = 0; i < nq; i++) donor[i] = search_adt(b, &xq[3 * i]);
   clock_gettime(CLOCK_MONOTONIC, &t1);
   return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+/* ======================================================================
+ * AMR Cartesian background, gather direction: near-body receptors take
+ * their values from Cartesian patches (pdegree 0, linear interpolation
+ * between cell centres).
+ * ====================================================================== */
+
+/* CartGrid::registerData (CartGrid.C:22-70) */
+void orc_amr_global_data(OrcTioga *t, int nf, int qstride, const double *qnode, const int *idata,
+                         const double *rdata, int ngrids, int qnodesize)
+{
+  int i, n;
+  OrcCart *c = t->cart;
+  if (c) free_cart(c);
+  c = (OrcCart *)calloc(1, sizeof(OrcCart));
+  t->cart = c;
+  c->ngrids = ngrids;
+  c->nf = nf;
+  c->qstride = qstride;
+  c->qnodesize = qnodesize;
+  c->gid = (int *)malloc(sizeof(int) * (ngrids + 1));
+  c->level = (int *)malloc(sizeof(int) * (ngrids + 1));
+  c->proc = (int *)malloc(sizeof(int) * (ngrids + 1));
+  c->porder = (int *)malloc(sizeof(int) * (ngrids + 1));
+  c->lid = (int *)malloc(sizeof(int) * (ngrids + 1));
+  c->ilo = (int *)malloc(sizeof(int) * 3 * (ngrids + 1));
+  c->ihi = (int *)malloc(sizeof(int) * 3 * (ngrids + 1));
+  c->dims = (int *)malloc(sizeof(int) * 3 * (ngrids + 1));
+  c->xlo = (double *)malloc(sizeof(double) * 3 * (ngrids + 1));
+  c->dx = (double *)malloc(sizeof(double) * 3 * (ngrids + 1));
+  c->qnode = (double *)malloc(sizeof(double) * (qnodesize + 1));
+  for (i = 0; i < qnodesize; i++) c->qnode[i] = qnode[i];
+  for (i = 0; i < ngrids; i++) {
+    const int *id = idata + 11 * i;
+    const double *rd = rdata + 6 * i;
+    c->gid[i] = id[0];
+    c->level[i] = id[1];
+    c->proc[i] = id[2];
+    c->porder[i] = id[3];
+    c->lid[i] = id[4];
+    for (n = 0; n < 3; n++) {
+      c->ilo[3 * i + n] = id[5 + n];
+      c->ihi[3 * i + n] = id[8 + n];
+      c->dims[3 * i + n] = id[8 + n] - id[5 + n] + 1;
+      c->xlo[3 * i + n] = rd[n];
+      c->dx[3 * i + n] = rd[3 + n];
+    }
+  }
+}
+
+/* tioga::set_amr_patch_count (tioga.C:703-708) */
+void orc_amr_patch_count(OrcTioga *t, int npatch)
+{
+  OrcCart *c = t->cart;
+  if (!c) return;
+  free(c->pgid); free(c->pibl); free(c->pq);
+  c->npatch = npatch;
+  c->pgid = (int *)calloc(npatch + 1, sizeof(int));
+  c->pibl = (int **)calloc(npatch + 1, sizeof(int *));
+  c->pq = (double **)calloc(npatch + 1, sizeof(double *));
+}
+
+/* tioga::register_amr_local_data (tioga.C:710-713), CartBlock::registerData
+   (CartBlock.h:52-58) */
+void orc_amr_local_data(OrcTioga *t, int ipatch, int global_id, int *iblank, double *q)
+{
+  OrcCart *c = t->cart;
+  if (!c || ipatch < 0 || ipatch >= c->npatch) return;
+  c->pgid[ipatch] = global_id;
+  c->pibl[ipatch] = iblank;
+  c->pq[ipatch] = q;
+}
+
+/* CartGrid::preprocess (CartGrid.C:77-104) */
+static void cart_preprocess(OrcCart *c)
+{
+  int i, n;
+  c->xlosup[0] = c->xlosup[1] = c->xlosup[2] = BIGV;
+  c->maxlevel = -1;
+  for (i = 0; i < c->ngrids; i++) {
+    for (n = 0; n < 3; n++) c->xlosup[n] = dmin(c->xlosup[n], c->xlo[3 * i + n]);
+    if (c->level[i] > c->maxlevel) c->maxlevel = c->level[i];
+  }
+  c->maxlevel++;
+  free(c->dxlvl);
+  c->dxlvl = (double *)malloc(sizeof(double) * 3 * (c->maxlevel + 1));
+  for (i = 0; i < c->ngrids; i++)
+    for (n = 0; n < 3; n++) c->dxlvl[3 * c->level[i] + n] = c->dx[3 * i + n];
+}
+
+/* CartGrid::search (CartGrid.C:111-145): the first patch, in registration
+   order, of the finest level whose box holds the point within +-TOL. */
+static int cart_search_point(const OrcCart *c, const double *x)
+{
+  int l, j, n;
+  for (l = c->maxlevel - 1; l >= 0; l--)
+    for (j = 0; j < c->ngrids; j++) {
+      if (c->level[j] != l) continue;
+      int flag = 1;
+      for (n = 0; n < 3; n++) flag = flag && ((x[n] - c->xlo[3 * j + n]) > -TOLV);
+      for (n = 0; n < 3; n++)
+        flag = flag && ((x[n] - (c->xlo[3 * j + n] + c->dx[3 * j + n] * (c->dims[3 * j + n]))) < TOLV);
+      if (flag) return j;
+    }
+  return -1;
+}
+
+void orc_amr_search(OrcTioga *t, const double *x, int n, int *donor)
+{
+  int i;
+  if (!t->cart) return;
+  cart_preprocess(t->cart);
+  for (i = 0; i < n; i++) donor[i] = cart_search_point(t->cart, x + 3 * (size_t)i);
+}
+
+/* MeshBlock::getUnresolvedMandatoryReceptors (cartOps.C:61-160), node form:
+   every node of a cell whose nodes all carry nodeRes == BIGVALUE and whose
+   iblank_cell is 1, in node order. */
+static void unresolved_mandatory(OrcBlock *b)
+{
+  int n, i, m, k = 0;
+  char *picked = (char *)calloc(b->nnodes + 1, 1);
+  for (n = 0; n < b->ntypes; n++) {
+    int nvert = b->nv[n];
+    for (i = 0; i < b->nc[n]; i++) {
+      int fcount = 0, ibc = b->iblank_cell ? b->iblank_cell[k] : 1;
+      for (m = 0; m < nvert; m++)
+        if (b->nodeRes[b->vconn[n][nvert * i + m] - ORC_BASE] == BIGV) fcount++;
+      if (fcount == nvert && ibc == 1)
+        for (m = 0; m < nvert; m++) picked[b->vconn[n][nvert * i + m] - ORC_BASE] = 1;
+      k++;
+    }
+  }
+  free(b->receptorIdCart); free(b->donorIdCart); free(b->rxyzCart);
+  b->ncart = 0;
+  for (i = 0; i < b->nnodes; i++) b->ncart += picked[i];
+  b->receptorIdCart = (int *)malloc(sizeof(int) * (b->ncart + 1));
+  b->donorIdCart = (int *)malloc(sizeof(int) * (b->ncart + 1));
+  b->rxyzCart = (double *)malloc(sizeof(double) * 3 * (b->ncart + 1));
+  m = 0;
+  for (i = 0; i < b->nnodes; i++)
+    if (picked[i]) {
+      b->receptorIdCart[m] = i;
+      for (n = 0; n < 3; n++) b->rxyzCart[3 * m + n] = b->x[3 * i + n];
+      m++;
+    }
+  free(picked);
+}
+
+/* CartBlock::insertInInterpList (CartBlock.C:196-268) with
+   cart_interp::linear_interpolation (linCartInterp.C:49-109) for p = 1.
+   Returns 0, or -1 where the reference's assert (CartBlock.C:246) fails. */
+static int insert_interp(const OrcCart *c, int gidx, const double *x, AmrRec *r)
+{
+  int n, ix[3], dims[3], start[3], i, j, k, ind;
+  double xlo[3], dx[3], rst[3], rc[3], px[2], py[2], pz[2];
+  for (n = 0; n < 3; n++) {
+    xlo[n] = c->xlo[3 * gidx + n];
+    dx[n] = c->dx[3 * gidx + n];
+    dims[n] = c->ihi[3 * gidx + n] - c->ilo[3 * gidx + n] + 1;
+  }
+  for (n = 0; n < 3; n++) {
+    ix[n] = (int)((x[n] - xlo[n]) / dx[n]);
+    rst[n] = (x[n] - xlo[n] - ix[n] * dx[n]) / dx[n];
+    if (ix[n] == dims[n] && fabs(rst[n]) < TOLV) {
+      ix[n]--;
+      rst[n] = (x[n] - xlo[n] - ix[n] * dx[n]) / dx[n];
+    }
+    if (!(ix[n] >= 0 && ix[n] < dims[n])) return -1;
+  }
+  /* create_donor_stencil (linCartInterp.C:74-95) */
+  for (n = 0; n < 3; n++) start[n] = (rst[n] - 0.5 >= 0) ? ix[n] : ix[n] - 1;
+  ind = 0;
+  for (k = 0; k < 2; k++)
+    for (j = 0; j < 2; j++)
+      for (i = 0; i < 2; i++) {
+        int s[3] = {start[0] + i, start[1] + j, start[2] + k};
+        for (n = 0; n < 3; n++) {
+          int v = s[n] < dims[n] - 1 ? s[n] : dims[n] - 1;
+          r->ijk[ind++] = v > 0 ? v : 0;
+        }
+      }
+  /* compute_ref_coords (linCartInterp.C:63-72), p = 1 */
+  for (n = 0; n < 3; n++)
+    rc[n] = (rst[n] - 0.5 >= 0) ? ((rst[n] - 0.5) / 1 * 2 - 1) : ((rst[n] + 0.5 * (2 * 1 - 1)) / 1 * 2 - 1);
+  /* compute_1d_bases + compute_weights (linCartInterp.C:24-61) */
+  px[0] = (1 - rc[0]) / 2; px[1] = (1 + rc[0]) / 2;
+  py[0] = (1 - rc[1]) / 2; py[1] = (1 + rc[1]) / 2;
+  pz[0] = (1 - rc[2]) / 2; pz[1] = (1 + rc[2]) / 2;
+  ind = 0;
+  for (k = 0; k < 2; k++)
+    for (j = 0; j < 2; j++)
+      for (i = 0; i < 2; i++) r->w[ind++] = pz[k] * py[j] * px[i];
+  return 0;
+}
+
+/* MeshBlock::setCartIblanks (cartOps.C:32-58), node form */
+static void set_cart_iblanks(OrcBlock *b)
+{
+  int m;
+  for (m = 0; m < b->ncart; m++)
+    if (b->donorIdCart[m] != -1) b->iblank[b->receptorIdCart[m]] = -1;
+}
+
+/* tioga::performConnectivityAMR (tioga.C:180-216) on one rank, gather
+   direction only: CartGrid/CartBlock preprocess, the unresolved mandatory
+   receptors of every block, CartGrid::search, the patch interpolation lists
+   in the order exchangeAMRDonors.C:120-192 inserts them (block, point),
+   setCartIblanks (exchangeAMRDonors.C:271-272) and getCellIblanks.  The
+   Cartesian-receptor direction (getCartReceptors, donor lists on patches,
+   processDonors, cancellation) is not part of this path.
+   Returns 0; -1 for a patch with pdegree > 0 (the reference needs a
+   user callback there, CartBlock.C:167-168); -2 where the reference's
+   stencil assert fails. */
+int orc_perform_connectivity_amr(OrcTioga *t)
+{
+  int ib, i, p, ret = 0;
+  OrcCart *c = t->cart;
+  if (!c) return -3;
+  cart_preprocess(c);
+  for (p = 0; p < c->npatch; p++)
+    if (c->porder[c->pgid[p]] != 0) return -1;
+  for (ib = 0; ib < t->nblocks; ib++) {
+    OrcBlock *b = t->b[ib];
+    unresolved_mandatory(b);
+    for (i = 0; i < b->ncart; i++) b->donorIdCart[i] = cart_search_point(c, b->rxyzCart + 3 * (size_t)i);
+  }
+  /* patch lists: count, then fill in (block, point) order */
+  free(c->roff); free(c->rec);
+  c->roff = (int *)calloc(c->npatch + 1, sizeof(int));
+  c->nrec = 0;
+  for (ib = 0; ib < t->nblocks; ib++) {
+    OrcBlock *b = t->b[ib];
+    for (i = 0; i < b->ncart; i++)
+      if (b->donorIdCart[i] != -1) { c->roff[c->lid[b->donorIdCart[i]] + 1]++; c->nrec++; }
+  }
+  for (p = 0; p < c->npatch; p++) c->roff[p + 1] += c->roff[p];
+  c->rec = (AmrRec *)malloc(sizeof(AmrRec) * (c->nrec + 1));
+  {
+    int *fill = (int *)calloc(c->npatch + 1, sizeof(int));
+    for (ib = 0; ib < t->nblocks; ib++) {
+      OrcBlock *b = t->b[ib];
+      for (i = 0; i < b->ncart; i++) {
+        int g = b->donorIdCart[i];
+        if (g == -1) continue;
+        int lp = c->lid[g];
+        AmrRec *r = &c->rec[c->roff[lp] + fill[lp]++];
+        r->rinfo[0] = 0;
+        r->rinfo[1] = i;
+        r->rinfo[2] = ib;
+        if (insert_interp(c, c->pgid[lp], b->rxyzCart + 3 * (size_t)i, r)) ret = -2;
+      }
+    }
+    free(fill);
+  }
+  for (ib = 0; ib < t->nblocks; ib++) set_cart_iblanks(t->b[ib]);
+  for (ib = 0; ib < t->nblocks; ib++) cell_iblanks(t->b[ib]);
+  t->amr = 1;
+  return ret;
+}
+
+/* tioga::dataUpdate_AMR (tioga.C:218-334) on one rank: CartBlock::
+   getInterpolatedData (CartBlock.C:44-140, pdegree 0, get_amr_index_xyz.c:1-56)
+   for every patch, then MeshBlock::updateSolnData (dataUpdate.C:105-119) at
+   receptorIdCart of the receiving block. */
+void orc_data_update_amr(OrcTioga *t, int nvar, int interptype)
+{
+  int p, r, m, n;
+  OrcCart *c = t->cart;
+  double *qq = (double *)malloc(sizeof(double) * (nvar + 1));
+  if (!c || !c->rec) { free(qq); return; }
+  for (p = 0; p < c->npatch; p++) {
+    int g = c->pgid[p];
+    int nx = c->dims[3 * g], ny = c->dims[3 * g + 1], nz = c->dims[3 * g + 2], nf = c->nf;
+    int d3nf = (nx + 2 * nf) * (ny + 2 * nf) * (nz + 2 * nf);
+    const double *q = c->pq[p];
+    for (r = c->roff[p]; r < c->roff[p + 1]; r++) {
+      const AmrRec *a = &c->rec[r];
+      for (n = 0; n < nvar; n++) qq[n] = 0;
+      for (m = 0; m < 8; m++) {
+        int i = a->ijk[3 * m], j = a->ijk[3 * m + 1], k = a->ijk[3 * m + 2];
+        int idx = (ny + 2 * nf) * (nx + 2 * nf) * (k + nf) + (nx + 2 * nf) * (j + nf) + (i + nf);
+        for (n = 0; n < nvar; n++) qq[n] += q[idx + d3nf * n] * a->w[m];
+      }
+      OrcBlock *b = t->b[a->rinfo[2]];
+      int inode = b->receptorIdCart[a->rinfo[1]];
+      if (b->q == NULL) continue;
+      for (n = 0; n < nvar; n++) {
+        if (interptype == 0) b->q[inode * nvar + n] = qq[n];
+        else b->q[b->nnodes * n + inode] = qq[n];
+      }
+    }
+  }
+  free(qq);
+}
+
+/* Patch interpolation records in list order: "rec_patch" (local patch),
+   "rec_info" (3 ints: rank, point, block), "rec_ijk" (24 ints), "rec_w"
+   (8 doubles). */
+long orc_amr_array(OrcTioga *t, const char *name, void *out)
+{
+  OrcCart *c = t->cart;
+  long i, n;
+  int p;
+  if (!c || !c->rec) return 0;
+  n = c->nrec;
+  if (!strcmp(name, "rec_patch")) {
+    if (out)
+      for (p = 0; p < c->npatch; p++)
+        for (i = c->roff[p]; i < c->roff[p + 1]; i++) ((int *)out)[i] = p;
+    return n;
+  }
+  if (!strcmp(name, "rec_info")) {
+    if (out) for (i = 0; i < n; i++) memcpy((int *)out + 3 * i, c->rec[i].rinfo, 12);
+    return 3 * n;
+  }
+  if (!strcmp(name, "rec_ijk")) {
+    if (out) for (i = 0; i < n; i++) memcpy((int *)out + 24 * i, c->rec[i].ijk, 96);
+    return 24 * n;
+  }
+  if (!strcmp(name, "rec_w")) {
+    if (out) for (i = 0; i < n; i++) memcpy((double *)out + 8 * i, c->rec[i].w, 64);
+    return 8 * n;
+  }
+  return -1;
 }

@@ -66,7 +66,7 @@ void orc_get_donor_info(OrcTioga *t, int btag, int *receptors, int *indices, dou
    Returns the number of elements; copies into out when out != NULL.
    names: "nodeRes","cellRes","xsearch","res_search","isearch","tagsearch",
    "donorId","xtag","elementList","adtIntegers","adtReals","iblank_cell",
-   "obb" (15 doubles: xc,dxc,vec) */
+   "obb" (15 doubles: xc,dxc,vec), "receptorIdCart","donorIdCart","rxyzCart" */
 long orc_block_array(OrcTioga *t, int btag, const char *name, void *out);
 /* hole map of mesh tag `mtag`: returns nx0*nx1*nx2 (0 when no wall);
    fills nx[3], ext[6] and sam when non-NULL */
@@ -80,6 +80,19 @@ int orc_weight_aborts(void);
    against the ADT of block `btag` built over all of its cells, sequentially,
    with containment tests; returns elapsed seconds. */
 double orc_search_points(OrcTioga *t, int btag, const double *xq, int nq, int *donor);
+
+/* ---- AMR Cartesian background, gather direction (single rank) ---- */
+/* tioga::register_amr_global_data (tioga.C:692-701): 11 ints and 6 doubles
+   per patch, as CartGrid::registerData (CartGrid.C:22-70) reads them */
+void orc_amr_global_data(OrcTioga *t, int nf, int qstride, const double *qnode, const int *idata,
+                         const double *rdata, int ngrids, int qnodesize);
+void orc_amr_patch_count(OrcTioga *t, int npatch);
+void orc_amr_local_data(OrcTioga *t, int ipatch, int global_id, int *iblank, double *q);
+/* CartGrid::search on arbitrary points */
+void orc_amr_search(OrcTioga *t, const double *x, int n, int *donor);
+int orc_perform_connectivity_amr(OrcTioga *t);
+void orc_data_update_amr(OrcTioga *t, int nvar, int interptype);
+long orc_amr_array(OrcTioga *t, const char *name, void *out);
 
 #ifdef __cplusplus
 }

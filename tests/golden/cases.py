@@ -85,3 +85,56 @@ def build(name: str) -> Case:
 
 SMALL_CASES = ["tiny", "tiny_isym0", "tiny_isym1", "tiny_isym2", "mixed", "userres", "moving", "three"]
 ALL_CASES = SMALL_CASES + ["config1"]
+
+
+# ---------------------------------------------------------------------------
+# AMR Cartesian background, gather direction: near-body blocks receive from
+# Cartesian patches (tioga.C:180-334 on one rank, pdegree 0)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class AmrCase:
+    name: str
+    blocks: list
+    grid: object           # tioga_amd.workloads.AmrGrid
+    nvar: int
+    interptype: int = 0
+
+    @property
+    def qs(self) -> List[np.ndarray]:
+        return [W.fields(b.nnodes, self.nvar, seed=31 + i) for i, b in enumerate(self.blocks)]
+
+    @property
+    def patch_q(self) -> List[np.ndarray]:
+        return self.grid.patch_fields(self.nvar, seed=41)
+
+
+def build_amr(name: str) -> AmrCase:
+    if name == "amr_fringe":
+        # receptors: the outer fringe of a lat-long shell (default resolutions);
+        # three levels around the body, grid list and local ids permuted
+        grid = W.amr_hierarchy((-1.0, -1.0, -1.0), 0.25,
+                               [((0, 0, 0), (8, 8, 8)), ((2, 2, 2), (14, 14, 14)), ((8, 8, 8), (24, 24, 24))],
+                               patch=4, nf=1, shuffle_seed=1, local_seed=2)
+        return AmrCase(name, [W.latlong_shell(16, 8, 6, 0.25, 0.6, 1.1, tag=1)], grid, nvar=5)
+    if name == "amr_all":
+        # every node of a mixed-element shell a receptor, plus the fringe of a
+        # second body; two ghost layers, column layout; the patches stop at
+        # z = 0.4, so receptors above find no donor
+        grid = W.amr_hierarchy((-1.0, -1.0, -1.0), 0.2,
+                               [((0, 0, 0), (10, 10, 7)), ((4, 4, 4), (16, 16, 14)),
+                                ((12, 12, 12), (28, 28, 28))],
+                               patch=4, nf=2, shuffle_seed=3, local_seed=4)
+        a = W.all_receptors(W.mixed_shell(12, 8, layers=(2, 2, 1, 2), tag=1))
+        b = W.latlong_shell(12, 6, 4, 0.1, 0.3, 1.05, tag=2, center=(0.3, -0.35, 0.2))
+        return AmrCase(name, [a, b], grid, nvar=3, interptype=1)
+    if name == "amr_faces":
+        # receptors exactly on cell faces, patch faces and the refined region's
+        # boundary: ties between patches, clamped stencils, upper-face points
+        grid = W.amr_hierarchy((-1.0, -1.0, -1.0), 0.25, [((0, 0, 0), (8, 8, 8)), ((4, 4, 4), (12, 12, 12))],
+                               patch=4, nf=1, shuffle_seed=5, local_seed=6)
+        return AmrCase(name, [W.all_receptors(W.cart_box(8, tag=1, lo=-0.5, hi=0.5))], grid, nvar=2)
+    raise KeyError(name)
+
+
+AMR_CASES = ["amr_fringe", "amr_all", "amr_faces"]

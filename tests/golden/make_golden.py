@@ -2,7 +2,8 @@
 compiled from its own sources (oracle/Makefile, target ``ref``), run on one
 MPI rank by oracle/ref_driver.cpp.
 
-For each case of cases.py, ``<case>.npz`` holds every array that the driver
+For each case of cases.py (the AMR gather cases run the driver's AMR mode),
+``<case>.npz`` holds every array that the driver
 dumps whose size is at most ``FULL_LIMIT`` elements, and ``<case>.json``
 holds the SHA-256 digest, dtype and size of every array (the hole maps and
 config-1 arrays are pinned by digest only).  Run in a container where
@@ -37,12 +38,16 @@ def digest(a: np.ndarray) -> str:
 
 def main(names):
     for name in names:
-        c = cases.build(name)
         with tempfile.TemporaryDirectory() as td:
             cp, op = os.path.join(td, "case.bin"), os.path.join(td, "out.bin")
-            O.write_case(cp, c.blocks, c.qs, c.nvar, c.interptype, c.isym, c.nfringe, c.mexclude,
-                         c.moves)
-            res = O.run_reference(cp, op)
+            if name in cases.AMR_CASES:
+                c = cases.build_amr(name)
+                O.write_case(cp, c.blocks, c.qs, c.nvar, c.interptype, amr=(c.grid, c.patch_q))
+            else:
+                c = cases.build(name)
+                O.write_case(cp, c.blocks, c.qs, c.nvar, c.interptype, c.isym, c.nfringe, c.mexclude,
+                             c.moves)
+            res = O.run_reference(cp, op, cwd=td)
         meta = {}
         full = {}
         for k, a in sorted(res.items()):
@@ -58,4 +63,4 @@ def main(names):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or cases.ALL_CASES)
+    main(sys.argv[1:] or cases.ALL_CASES + cases.AMR_CASES)

@@ -93,3 +93,37 @@ def run_engine_case(case):
     finally:
         t.close()
     return out
+
+
+def run_oracle_amr_case(case):
+    """The arrays ref_driver.cpp dumps in AMR mode, computed by the oracle."""
+    from oracle.oracle import Oracle
+
+    o = Oracle(case.blocks)
+    try:
+        o.register_amr(case.grid, case.patch_q)
+        o.profile()
+        rc = o.connect_amr()
+        out = o.amr_results(case.qs, case.nvar, case.interptype)
+        out["status"] = rc
+    finally:
+        o.close()
+    return out
+
+
+def run_engine_amr_case(case):
+    """The arrays ref_driver.cpp dumps in AMR mode, computed by the HIP
+    engine through the C ABI."""
+    from tioga_amd.tioga import Tioga
+
+    t = Tioga()
+    try:
+        for b in case.blocks:
+            t.register_grid(b)
+        t.register_amr(case.grid, case.patch_q)
+        t.preprocess()
+        t.connect_amr()
+        out = t.amr_results(case.qs, case.nvar, case.interptype)
+    finally:
+        t.close()
+    return out

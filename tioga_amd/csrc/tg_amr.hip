@@ -1,0 +1,813 @@
+// tg_amr.hip -- AMR Cartesian background, gather direction: near-body
+// receptors take their values from Cartesian patches (pdegree 0, linear
+// interpolation between cell centres).
+//
+// Reference path, single rank:
+//   * tioga::register_amr_global_data / set_amr_patch_count /
+//     register_amr_local_data (tioga.C:692-713, CartGrid.C:22-70);
+//   * tioga::performConnectivityAMR (tioga.C:180-216): CartGrid::preprocess
+//     (CartGrid.C:77-104), MeshBlock::getUnresolvedMandatoryReceptors
+//     (cartOps.C:61-160, node form), CartGrid::search (CartGrid.C:111-145),
+//     the patch interpolation lists of exchangeAMRDonors.C:120-192
+//     (CartBlock::insertInInterpList, CartBlock.C:196-268, with
+//     cart_interp::linear_interpolation, linCartInterp.C:49-109),
+//     MeshBlock::setCartIblanks (cartOps.C:32-58) and getCellIblanks;
+//   * tioga::dataUpdate_AMR (tioga.C:218-334): CartBlock::getInterpolatedData
+//     (CartBlock.C:44-140, get_amr_index_xyz.c:1-56) and
+//     MeshBlock::updateSolnData (dataUpdate.C:105-119).
+// The Cartesian-receptor direction (getCartReceptors, donor lists on the
+// patches, processDonors, cancellation) is not part of this path; on one
+// rank it never cancels a near-body receptor, since getCancellationData
+// (CartBlock.C:505-531) only emits cancellations of the other direction.
+//
+// Device layout:
+//   * patch table (64 B per grid) and, per refinement level, a uniform bin
+//     grid whose bins list (ascending grid index) every patch of that level
+//     whose box, widened by more than TOL, touches the bin;
+//   * receptors of all blocks concatenated: node id, block, coordinates,
+//     donor grid;
+//   * interpolation records sorted by local patch (stable in (block, point)
+//     order, the order the reference's lists hold), structure of arrays:
+//     patch, node, block|stencil bits, stencil base index, and the six 1-D
+//     basis values from which the 8 weights are formed exactly as
+//     compute_weights forms them (64 B per record instead of 108 B);
+//   * patch solutions packed in one buffer, nvar fields of d3nf doubles each.
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+#include "tg_engine.h"
+#include "tg_prims.h"
+
+namespace tg {
+
+struct AmrGridDev {
+  double xlo[3], dx[3];
+  int dims[3], level;
+};
+static_assert(sizeof(AmrGridDev) == 64, "patch records are 64 bytes");
+
+struct AmrLevelBins {
+  double o[3], h[3];
+  int nb[3], start;
+};
+
+struct AmrQTables {
+  double *q[128];
+  int nnodes[128];
+};
+
+struct AmrState {
+  // CartGrid (host copy, CartGrid.h:26-76)
+  int nf = 0, qstride = 0, ngrids = 0, qnodesize = 0, maxlevel = 0;
+  std::vector<int> gid, level, proc, porder, lid, dims;
+  std::vector<double> xlo, dx, qnode;
+  bool have_grid = false;
+  // CartBlocks of this rank
+  int npatch = 0;
+  std::vector<int> pgid;
+  std::vector<int *> pibl;
+  std::vector<double *> pq;
+  bool connected = false;  // performConnectivityAMR done (iamrGlobal)
+  // device tables
+  DBuf<AmrGridDev> grids;
+  DBuf<AmrLevelBins> levels;
+  DBuf<int> boff, blist, plid, ppgid, pd3nf;
+  // receptors (all blocks, concatenated)
+  int nrcp = 0;
+  std::vector<int> rcp_off;  // per block
+  DBuf<int> rnode, rblk, rdonor;
+  DBuf<double> rxyz;
+  DBuf<unsigned char> picked;
+  // records
+  int nrec = 0;
+  DBuf<int> rec_lp, rec_node, rec_meta, rec_base, rec_rid;
+  DBuf<double> rec_phi;
+  std::vector<int> rec_count;  // per local patch
+  // packed patch solutions
+  DBuf<double> q;
+  DBuf<long long> qoff;
+  DBuf<int> err;
+  DBuf<AmrQTables> qt;
+};
+
+namespace {
+
+constexpr int ABS = 256;
+inline unsigned anb(long long n) { return (unsigned)((n + ABS - 1) / ABS); }
+
+__device__ __forceinline__ int type_of_cell(const TypeInfo &ti, int c)
+{
+  int t = 0;
+  while (t + 1 < ti.ntypes && c >= ti.cstart[t + 1]) t++;
+  return t;
+}
+
+// getUnresolvedMandatoryReceptors (cartOps.C:73-97), node form: every node of
+// a cell whose nodes all carry nodeRes == BIGVALUE and whose cell iblank is 1
+__global__ void k_amr_pick(int ncells, TypeInfo ti, const int *__restrict__ conn, const double *__restrict__ nodeRes,
+                           const int *__restrict__ ibc, unsigned char *__restrict__ picked)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int nv = ti.nv[type_of_cell(ti, c)];
+  const int *cv = conn + 8 * (size_t)c;
+  int fcount = 0;
+  for (int m = 0; m < nv; m++) fcount += (nodeRes[cv[m]] == BIGVALUE);
+  if (fcount == nv && ibc[c] == 1)
+    for (int m = 0; m < nv; m++) picked[cv[m]] = 1;
+}
+
+__global__ void k_amr_flags(int n, const unsigned char *__restrict__ picked, int *__restrict__ flags)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[i] = picked[i];
+}
+
+__global__ void k_amr_receptors(int n, const int *__restrict__ nodes, const double *__restrict__ x, int blk,
+                                int *__restrict__ rnode, int *__restrict__ rblk, double *__restrict__ rxyz)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int v = nodes[i];
+  rnode[i] = v;
+  rblk[i] = blk;
+  for (int j = 0; j < 3; j++) rxyz[3 * (size_t)i + j] = x[3 * (size_t)v + j];
+}
+
+// CartGrid::search (CartGrid.C:111-145): the first grid, in registration
+// order, of the finest level whose box holds the point within +-TOL.  The
+// bins of a level list every grid that can pass the test for a point in the
+// bin, in ascending order, so the first grid of the list that passes is the
+// first grid of the level that passes.
+__global__ void k_amr_search(int n, const double *__restrict__ rxyz, int maxlevel,
+                             const AmrLevelBins *__restrict__ lv, const int *__restrict__ boff,
+                             const int *__restrict__ blist, const AmrGridDev *__restrict__ grids,
+                             int *__restrict__ donor)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x0 = rxyz[3 * (size_t)i], x1 = rxyz[3 * (size_t)i + 1], x2 = rxyz[3 * (size_t)i + 2];
+  const double xp[3] = {x0, x1, x2};
+  int found = -1;
+  for (int l = maxlevel - 1; l >= 0 && found < 0; l--) {
+    const AmrLevelBins L = lv[l];
+    if (L.nb[0] == 0) continue;
+    int b[3];
+    for (int a = 0; a < 3; a++) {
+      double t = floor((xp[a] - L.o[a]) / L.h[a]);
+      b[a] = t < 0.0 ? 0 : (t > (double)(L.nb[a] - 1) ? L.nb[a] - 1 : (int)t);
+    }
+    const int bin = L.start + (b[2] * L.nb[1] + b[1]) * L.nb[0] + b[0];
+    for (int k = boff[bin]; k < boff[bin + 1]; k++) {
+      const int j = blist[k];
+      const AmrGridDev &G = grids[j];
+      bool flag = true;
+      for (int a = 0; a < 3; a++) flag = flag && ((xp[a] - G.xlo[a]) > -TOL);
+      for (int a = 0; a < 3; a++) flag = flag && ((xp[a] - (G.xlo[a] + G.dx[a] * (G.dims[a]))) < TOL);
+      if (flag) {
+        found = j;
+        break;
+      }
+    }
+  }
+  donor[i] = found;
+}
+
+__global__ void k_amr_keys(int n, const int *__restrict__ donor, const int *__restrict__ plid,
+                           unsigned *__restrict__ key, int *__restrict__ flags)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int g = donor[i];
+  flags[i] = g >= 0;
+  key[i] = g >= 0 ? (unsigned)plid[g] : 0u;
+}
+
+__global__ void k_amr_take(int n, const int *__restrict__ idx, const unsigned *__restrict__ all,
+                           unsigned *__restrict__ out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = all[idx[i]];
+}
+
+// MeshBlock::setCartIblanks (cartOps.C:49-57)
+__global__ void k_amr_iblank(int n, const int *__restrict__ donor, const int *__restrict__ rnode,
+                             const int *__restrict__ rblk, int blk, int *__restrict__ iblank)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (rblk[i] == blk && donor[i] != -1) iblank[rnode[i]] = -1;
+}
+
+// CartBlock::insertInInterpList (CartBlock.C:196-268) with
+// linear_interpolation (linCartInterp.C:49-109) for p = 1; one thread per
+// record, records in list order (rid = receptor of the record).
+__global__ void k_amr_records(int nrec, const int *__restrict__ rid, const unsigned *__restrict__ lp_sorted,
+                              const double *__restrict__ rxyz, const int *__restrict__ rnode,
+                              const int *__restrict__ rblk, const int *__restrict__ ppgid,
+                              const AmrGridDev *__restrict__ grids, int nf, int *__restrict__ rec_lp,
+                              int *__restrict__ rec_node, int *__restrict__ rec_meta, int *__restrict__ rec_base,
+                              double *__restrict__ rec_phi, int *__restrict__ err)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  const int i = rid[r];
+  const int lp = (int)lp_sorted[r];
+  const AmrGridDev G = grids[ppgid[lp]];
+  int ix[3], v0[3], bits = 0;
+  double ph[6];
+  for (int n = 0; n < 3; n++) {
+    const double x = rxyz[3 * (size_t)i + n];
+    ix[n] = (int)((x - G.xlo[n]) / G.dx[n]);
+    double rst = (x - G.xlo[n] - ix[n] * G.dx[n]) / G.dx[n];
+    if (ix[n] == G.dims[n] && fabs(rst) < TOL) {
+      ix[n]--;
+      rst = (x - G.xlo[n] - ix[n] * G.dx[n]) / G.dx[n];
+    }
+    if (!(ix[n] >= 0 && ix[n] < G.dims[n])) atomicExch(err, 1);  // CartBlock.C:246
+    // create_donor_stencil (linCartInterp.C:74-95)
+    const int start = (rst - 0.5 >= 0) ? ix[n] : ix[n] - 1;
+    const int a0 = max(0, min(start, G.dims[n] - 1)), a1 = max(0, min(start + 1, G.dims[n] - 1));
+    v0[n] = a0;
+    bits |= (a1 - a0) << n;
+    // compute_ref_coords (linCartInterp.C:63-72) and compute_1d_bases (:24-47), p = 1
+    const double rc = (rst - 0.5 >= 0) ? ((rst - 0.5) / 1 * 2 - 1) : ((rst + 0.5 * (2 * 1 - 1)) / 1 * 2 - 1);
+    ph[2 * n] = (1 - rc) / 2;
+    ph[2 * n + 1] = (1 + rc) / 2;
+  }
+  const int sx = G.dims[0] + 2 * nf, sy = G.dims[1] + 2 * nf;
+  rec_lp[r] = lp;
+  rec_node[r] = rnode[i];
+  rec_meta[r] = (rblk[i] << 3) | bits;
+  rec_base[r] = sy * sx * (v0[2] + nf) + sx * (v0[1] + nf) + (v0[0] + nf);  // get_amr_index_xyz.c:40-42
+  for (int m = 0; m < 6; m++) rec_phi[(size_t)m * nrec + r] = ph[m];
+}
+
+// CartBlock::getInterpolatedData (CartBlock.C:113-127) and updateSolnData
+// (dataUpdate.C:105-119) fused: every record writes one receptor node.
+__global__ void __launch_bounds__(ABS) k_amr_gather(int nrec, int nvar, int interptype, const int *__restrict__ rec_lp,
+                                                    const int *__restrict__ rec_node, const int *__restrict__ rec_meta,
+                                                    const int *__restrict__ rec_base, const double *__restrict__ rec_phi,
+                                                    const double *__restrict__ q, const long long *__restrict__ qoff,
+                                                    const int *__restrict__ pd3nf, const int *__restrict__ pstride,
+                                                    const AmrQTables *__restrict__ T)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  const int lp = rec_lp[r], meta = rec_meta[r], base = rec_base[r], node = rec_node[r];
+  const double px0 = rec_phi[r], px1 = rec_phi[(size_t)nrec + r];
+  const double py0 = rec_phi[2 * (size_t)nrec + r], py1 = rec_phi[3 * (size_t)nrec + r];
+  const double pz0 = rec_phi[4 * (size_t)nrec + r], pz1 = rec_phi[5 * (size_t)nrec + r];
+  const double *qp = q + qoff[lp];
+  const int d3 = pd3nf[lp];
+  const int sx = pstride[2 * lp], sxy = pstride[2 * lp + 1];
+  const int ox = meta & 1, oy = ((meta >> 1) & 1) * sx, oz = ((meta >> 2) & 1) * sxy;
+  // weights in stencil order (k, j, i), each phi_z[k]*phi_y[j]*phi_x[i]
+  // (linCartInterp.C:49-61)
+  double w[8];
+  int off[8];
+  {
+    const double pz[2] = {pz0, pz1}, py[2] = {py0, py1}, px[2] = {px0, px1};
+    const int dz[2] = {0, oz}, dy[2] = {0, oy}, dxo[2] = {0, ox};
+    int m = 0;
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int i = 0; i < 2; i++, m++) {
+          w[m] = pz[k] * py[j] * px[i];
+          off[m] = base + dz[k] + dy[j] + dxo[i];
+        }
+  }
+  double *out = T->q[meta >> 3];
+  const int nn = T->nnodes[meta >> 3];
+  for (int n = 0; n < nvar; n++) {
+    const double *qn = qp + (size_t)d3 * n;
+    double acc = 0;
+#pragma unroll
+    for (int m = 0; m < 8; m++) acc += qn[off[m]] * w[m];
+    if (interptype == 0) out[(size_t)node * nvar + n] = acc;
+    else out[(size_t)nn * n + node] = acc;
+  }
+}
+
+}  // namespace
+
+AmrState &Engine::amr()
+{
+  if (!amr_) amr_ = new AmrState;
+  return *amr_;
+}
+
+void Engine::amr_release()
+{
+  delete amr_;
+  amr_ = nullptr;
+}
+
+bool Engine::amr_mode() const { return amr_ && amr_->connected && amr_->npatch > 0; }
+
+// tioga::register_amr_global_data (tioga.C:692-701), CartGrid::registerData
+// (CartGrid.C:22-70): 11 ints and 6 doubles per grid
+void Engine::amr_global_data(int nf, int qstride, const double *qnode, const int *idata, const double *rdata,
+                             int ngrids, int qnodesize)
+{
+  if (ngrids < 0 || qnodesize < 0) throw std::runtime_error("tioga_amd: negative AMR grid or qnode count");
+  AmrState &A = amr();
+  A.nf = nf;
+  A.qstride = qstride;
+  A.ngrids = ngrids;
+  A.qnodesize = qnodesize;
+  A.qnode.assign(qnode, qnode + qnodesize);
+  A.gid.resize(ngrids);
+  A.level.resize(ngrids);
+  A.proc.resize(ngrids);
+  A.porder.resize(ngrids);
+  A.lid.resize(ngrids);
+  A.dims.resize(3 * (size_t)ngrids);
+  A.xlo.resize(3 * (size_t)ngrids);
+  A.dx.resize(3 * (size_t)ngrids);
+  for (int i = 0; i < ngrids; i++) {
+    const int *id = idata + 11 * (size_t)i;
+    const double *rd = rdata + 6 * (size_t)i;
+    A.gid[i] = id[0];
+    A.level[i] = id[1];
+    A.proc[i] = id[2];
+    A.porder[i] = id[3];
+    A.lid[i] = id[4];
+    for (int n = 0; n < 3; n++) {
+      A.dims[3 * i + n] = id[8 + n] - id[5 + n] + 1;
+      A.xlo[3 * i + n] = rd[n];
+      A.dx[3 * i + n] = rd[3 + n];
+    }
+  }
+  A.have_grid = true;
+  A.connected = false;
+}
+
+// tioga::set_amr_patch_count (tioga.C:703-708)
+void Engine::amr_patch_count(int npatch)
+{
+  if (npatch < 0) throw std::runtime_error("tioga_amd: negative AMR patch count");
+  AmrState &A = amr();
+  A.npatch = npatch;
+  A.pgid.assign(npatch, 0);
+  A.pibl.assign(npatch, nullptr);
+  A.pq.assign(npatch, nullptr);
+  A.connected = false;
+}
+
+// tioga::register_amr_local_data (tioga.C:710-713)
+void Engine::amr_local_data(int ipatch, int global_id, int *iblank, double *q)
+{
+  AmrState &A = amr();
+  if (ipatch < 0 || ipatch >= A.npatch) throw std::runtime_error("tioga_amd: AMR patch index out of range");
+  A.pgid[ipatch] = global_id;
+  A.pibl[ipatch] = iblank;
+  A.pq[ipatch] = q;
+}
+
+// tioga::performConnectivityAMR (tioga.C:180-216), gather direction
+void Engine::perform_connectivity_amr()
+{
+  AmrState &A = amr();
+  if (!A.have_grid) throw std::runtime_error("tioga_amd: AMR grid data not registered");
+  const int G = A.ngrids;
+  for (int p = 0; p < A.npatch; p++) {
+    const int g = A.pgid[p];
+    if (g < 0 || g >= G) throw std::runtime_error("tioga_amd: AMR patch global id out of range");
+    if (A.porder[g] != 0)  // CartBlock.C:167-168 needs a user callback for pdegree > 0
+      throw std::runtime_error("tioga_amd: AMR patches with pdegree > 0 are not available in this engine");
+  }
+  for (int g = 0; g < G; g++) {
+    if (A.proc[g] != 0)
+      throw std::runtime_error("tioga_amd: AMR grid owned by another rank; this engine runs one rank");
+    if (A.lid[g] < 0 || A.lid[g] >= A.npatch)
+      throw std::runtime_error("tioga_amd: AMR grid local id outside the registered patches");
+    for (int n = 0; n < 3; n++)
+      if (A.dims[3 * g + n] <= 0 || !(A.dx[3 * g + n] > 0))
+        throw std::runtime_error("tioga_amd: AMR grid with empty extent or non-positive spacing");
+  }
+  // CartGrid::preprocess (CartGrid.C:77-104)
+  A.maxlevel = -1;
+  for (int g = 0; g < G; g++) {
+    if (A.level[g] < 0) throw std::runtime_error("tioga_amd: negative AMR level");
+    A.maxlevel = std::max(A.maxlevel, A.level[g]);
+  }
+  A.maxlevel++;
+  // patch table and per-level bins
+  std::vector<AmrGridDev> gt(std::max(G, 1));
+  for (int g = 0; g < G; g++)
+    for (int n = 0; n < 3; n++) {
+      gt[g].xlo[n] = A.xlo[3 * g + n];
+      gt[g].dx[n] = A.dx[3 * g + n];
+      gt[g].dims[n] = A.dims[3 * g + n];
+      gt[g].level = A.level[g];
+    }
+  std::vector<AmrLevelBins> lv(std::max(A.maxlevel, 1));
+  std::vector<std::vector<int>> bins;
+  for (int l = 0; l < A.maxlevel; l++) {
+    AmrLevelBins &L = lv[l];
+    L = AmrLevelBins{};
+    double lo[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, hi[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
+    double h[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, mag = 1.0;
+    int cnt = 0;
+    for (int g = 0; g < G; g++) {
+      if (A.level[g] != l) continue;
+      cnt++;
+      for (int n = 0; n < 3; n++) {
+        const double a = gt[g].xlo[n], b = a + gt[g].dx[n] * gt[g].dims[n];
+        lo[n] = std::min(lo[n], a);
+        hi[n] = std::max(hi[n], b);
+        h[n] = std::min(h[n], b - a);
+        mag = std::max(mag, std::max(std::fabs(a), std::fabs(b)));
+      }
+    }
+    if (cnt == 0) continue;
+    // bins of about one patch; widen the grid by a margin far above the
+    // test's TOL and the rounding of the coordinates, so that every grid
+    // that can pass the containment test for a point lies in its bin
+    const double margin = 4 * TOL + 64 * 2.220446049250313e-16 * mag;
+    for (int n = 0; n < 3; n++) {
+      L.o[n] = lo[n] - 2 * margin;
+      double ext = hi[n] - lo[n] + 4 * margin;
+      L.h[n] = std::max(h[n], ext / 512);
+      L.nb[n] = std::max(1, (int)std::ceil(ext / L.h[n]));
+    }
+    L.start = (int)bins.size();
+    const size_t nbin = (size_t)L.nb[0] * L.nb[1] * L.nb[2];
+    if (bins.size() + nbin > (size_t)1 << 28) throw std::runtime_error("tioga_amd: AMR level bin grid too large");
+    bins.resize(bins.size() + nbin);
+    for (int g = 0; g < G; g++) {
+      if (A.level[g] != l) continue;
+      int b0[3], b1[3];
+      for (int n = 0; n < 3; n++) {
+        const double a = gt[g].xlo[n], b = a + gt[g].dx[n] * gt[g].dims[n];
+        b0[n] = std::max(0, std::min(L.nb[n] - 1, (int)std::floor((a - L.o[n] - margin) / L.h[n])));
+        b1[n] = std::max(0, std::min(L.nb[n] - 1, (int)std::floor((b - L.o[n] + margin) / L.h[n])));
+      }
+      for (int k = b0[2]; k <= b1[2]; k++)
+        for (int j = b0[1]; j <= b1[1]; j++)
+          for (int i = b0[0]; i <= b1[0]; i++)
+            bins[L.start + ((size_t)k * L.nb[1] + j) * L.nb[0] + i].push_back(g);
+    }
+  }
+  std::vector<int> boff(bins.size() + 1, 0), blist;
+  for (size_t b = 0; b < bins.size(); b++) {
+    boff[b + 1] = boff[b] + (int)bins[b].size();
+    blist.insert(blist.end(), bins[b].begin(), bins[b].end());
+  }
+  A.grids.reserve(gt.size());
+  A.levels.reserve(lv.size());
+  A.boff.reserve(boff.size());
+  A.blist.reserve(std::max<size_t>(blist.size(), 1));
+  A.plid.reserve(std::max(G, 1));
+  TG_HIP(hipMemcpyAsync(A.grids.p, gt.data(), sizeof(AmrGridDev) * gt.size(), hipMemcpyHostToDevice, s_));
+  TG_HIP(hipMemcpyAsync(A.levels.p, lv.data(), sizeof(AmrLevelBins) * lv.size(), hipMemcpyHostToDevice, s_));
+  TG_HIP(hipMemcpyAsync(A.boff.p, boff.data(), sizeof(int) * boff.size(), hipMemcpyHostToDevice, s_));
+  if (!blist.empty())
+    TG_HIP(hipMemcpyAsync(A.blist.p, blist.data(), sizeof(int) * blist.size(), hipMemcpyHostToDevice, s_));
+  if (G) TG_HIP(hipMemcpyAsync(A.plid.p, A.lid.data(), sizeof(int) * G, hipMemcpyHostToDevice, s_));
+  // per local patch: registered grid, d3nf and strides
+  std::vector<int> pd3nf(std::max(A.npatch, 1)), pstride(2 * std::max(A.npatch, 1));
+  for (int p = 0; p < A.npatch; p++) {
+    const int g = A.pgid[p];
+    const long long sx = A.dims[3 * g] + 2 * A.nf, sy = A.dims[3 * g + 1] + 2 * A.nf, sz = A.dims[3 * g + 2] + 2 * A.nf;
+    if (sx * sy * sz > (1ll << 30)) throw std::runtime_error("tioga_amd: AMR patch too large");
+    pd3nf[p] = (int)(sx * sy * sz);
+    pstride[2 * p] = (int)sx;
+    pstride[2 * p + 1] = (int)(sx * sy);
+  }
+  A.ppgid.reserve(std::max(A.npatch, 1));
+  A.pd3nf.reserve(3 * std::max(A.npatch, 1));
+  if (A.npatch) {
+    TG_HIP(hipMemcpyAsync(A.ppgid.p, A.pgid.data(), sizeof(int) * A.npatch, hipMemcpyHostToDevice, s_));
+    TG_HIP(hipMemcpyAsync(A.pd3nf.p, pd3nf.data(), sizeof(int) * A.npatch, hipMemcpyHostToDevice, s_));
+    TG_HIP(hipMemcpyAsync(A.pd3nf.p + A.npatch, pstride.data(), sizeof(int) * 2 * A.npatch, hipMemcpyHostToDevice,
+                          s_));
+  }
+
+  // receptors of every block (getUnresolvedMandatoryReceptors), written
+  // straight into the concatenated receptor arrays (sized for every node)
+  const int nbk = (int)blocks_.size();
+  A.rcp_off.assign(nbk + 1, 0);
+  size_t maxn = 1, totn = 1;
+  for (Block *b : blocks_) {
+    maxn = std::max(maxn, (size_t)b->nnodes);
+    totn += (size_t)b->nnodes;
+  }
+  if (totn > (size_t)1 << 30) throw std::runtime_error("tioga_amd: too many AMR receptor candidates");
+  A.picked.reserve(maxn);
+  iscratch_.reserve(maxn);
+  iscratch2_.reserve(maxn);
+  A.rnode.reserve(totn);
+  A.rblk.reserve(totn);
+  A.rdonor.reserve(totn);
+  A.rxyz.reserve(3 * totn);
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    if (!b.topo_uploaded || b.nodeRes.p == nullptr)
+      throw std::runtime_error("tioga_amd: AMR connectivity needs preprocess (tioga_preprocess_grids) first");
+    // the cell iblank the reference reads: the user's array, else the one
+    // the last connectivity computed; with neither every cell counts as 1
+    const int *ibc = b.uiblank_cell ? b.uiblank_cell : (b.own_iblank_cell.empty() ? nullptr : b.own_iblank_cell.data());
+    std::vector<int> ones;
+    b.iblank_cell.reserve(b.ncells);
+    if (b.ncells) {
+      if (!ibc) {
+        ones.assign(b.ncells, 1);
+        ibc = ones.data();
+      }
+      TG_HIP(hipMemcpyAsync(b.iblank_cell.p, ibc, sizeof(int) * b.ncells, hipMemcpyHostToDevice, s_));
+    }
+    int nc = 0;
+    if (b.nnodes) {
+      TG_HIP(hipMemsetAsync(A.picked.p, 0, b.nnodes, s_));
+      if (b.ncells)
+        k_amr_pick<<<anb(b.ncells), ABS, 0, s_>>>(b.ncells, b.ti, b.conn.p, b.nodeRes.p, b.iblank_cell.p, A.picked.p);
+      k_amr_flags<<<anb(b.nnodes), ABS, 0, s_>>>(b.nnodes, A.picked.p, iscratch_.p);
+      nc = select_flagged(iscratch_.p, b.nnodes, iscratch2_.p, s_);
+    }
+    const int o = A.rcp_off[ib];
+    A.rcp_off[ib + 1] = o + nc;
+    if (nc)
+      k_amr_receptors<<<anb(nc), ABS, 0, s_>>>(nc, iscratch2_.p, b.x.p, ib, A.rnode.p + o, A.rblk.p + o,
+                                                A.rxyz.p + 3 * (size_t)o);
+    sync();
+  }
+  const int R = A.nrcp = A.rcp_off[nbk];
+  // CartGrid::search
+  if (R)
+    k_amr_search<<<anb(R), ABS, 0, s_>>>(R, A.rxyz.p, A.maxlevel, A.levels.p, A.boff.p, A.blist.p, A.grids.p,
+                                          A.rdonor.p);
+  // records: receptors with a donor, stable-sorted by local patch
+  A.nrec = 0;
+  if (R) {
+    uscratch_.reserve(R);
+    uscratch2_.reserve(R);
+    iscratch_.reserve(R);
+    iscratch2_.reserve(R);
+    k_amr_keys<<<anb(R), ABS, 0, s_>>>(R, A.rdonor.p, A.plid.p, uscratch_.p, iscratch_.p);
+    const int nr = select_flagged(iscratch_.p, R, iscratch2_.p, s_);
+    A.nrec = nr;
+    if (nr) {
+      DBuf<unsigned> keys, keys2;
+      keys.reserve(nr);
+      keys2.reserve(nr);
+      k_amr_take<<<anb(nr), ABS, 0, s_>>>(nr, iscratch2_.p, uscratch_.p, keys.p);
+      int bits = 1;
+      while (bits < 32 && ((unsigned)std::max(A.npatch - 1, 0) >> bits)) bits++;
+      A.rec_rid.reserve(nr);
+      sort_pairs(keys.p, keys2.p, iscratch2_.p, A.rec_rid.p, nr, bits, s_);
+      A.rec_lp.reserve(nr);
+      A.rec_node.reserve(nr);
+      A.rec_meta.reserve(nr);
+      A.rec_base.reserve(nr);
+      A.rec_phi.reserve(6 * (size_t)nr);
+      A.err.reserve(1);
+      TG_HIP(hipMemsetAsync(A.err.p, 0, sizeof(int), s_));
+      k_amr_records<<<anb(nr), ABS, 0, s_>>>(nr, A.rec_rid.p, keys2.p, A.rxyz.p, A.rnode.p, A.rblk.p, A.ppgid.p,
+                                             A.grids.p, A.nf, A.rec_lp.p, A.rec_node.p, A.rec_meta.p, A.rec_base.p,
+                                             A.rec_phi.p, A.err.p);
+      int e = 0;
+      TG_HIP(hipMemcpyAsync(&e, A.err.p, sizeof(int), hipMemcpyDeviceToHost, s_));
+      std::vector<unsigned> lps(nr);
+      TG_HIP(hipMemcpyAsync(lps.data(), keys2.p, sizeof(unsigned) * nr, hipMemcpyDeviceToHost, s_));
+      sync();
+      if (e)
+        throw std::runtime_error(
+            "tioga_amd: an AMR receptor lies outside its donor patch's cell range (the reference asserts, "
+            "CartBlock.C:246)");
+      A.rec_count.assign(A.npatch, 0);
+      for (unsigned v : lps) A.rec_count[v]++;
+    }
+  }
+  if (A.nrec == 0) A.rec_count.assign(A.npatch, 0);
+  // setCartIblanks, getCellIblanks, write back
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    if (R) k_amr_iblank<<<anb(R), ABS, 0, s_>>>(R, A.rdonor.p, A.rnode.p, A.rblk.p, ib, b.iblank.p);
+    cell_iblanks(b);
+  }
+  for (Block *bp : blocks_) {
+    Block &b = *bp;
+    TG_HIP(hipMemcpyAsync(b.uiblank, b.iblank.p, sizeof(int) * b.nnodes, hipMemcpyDeviceToHost, s_));
+    int *ibc = b.uiblank_cell;
+    if (!ibc) {
+      b.own_iblank_cell.resize(std::max(b.ncells, 1));
+      ibc = b.own_iblank_cell.data();
+    }
+    TG_HIP(hipMemcpyAsync(ibc, b.iblank_cell.p, sizeof(int) * b.ncells, hipMemcpyDeviceToHost, s_));
+  }
+  sync();
+  A.connected = true;
+}
+
+void Engine::amr_upload_q(int nvar)
+{
+  AmrState &A = amr();
+  std::vector<long long> off(std::max(A.npatch, 1), 0);
+  long long tot = 0;
+  for (int p = 0; p < A.npatch; p++) {
+    off[p] = tot;
+    if (A.rec_count.size() == (size_t)A.npatch && A.rec_count[p] > 0) {
+      const int g = A.pgid[p];
+      const long long d3 = (long long)(A.dims[3 * g] + 2 * A.nf) * (A.dims[3 * g + 1] + 2 * A.nf) *
+                           (A.dims[3 * g + 2] + 2 * A.nf);
+      tot += d3 * nvar;
+    }
+  }
+  A.q.reserve((size_t)std::max(tot, 1ll));
+  A.qoff.reserve(off.size());
+  TG_HIP(hipMemcpyAsync(A.qoff.p, off.data(), sizeof(long long) * off.size(), hipMemcpyHostToDevice, s_));
+  for (int p = 0; p < A.npatch; p++) {
+    if (!(A.rec_count.size() == (size_t)A.npatch && A.rec_count[p] > 0)) continue;
+    if (!A.pq[p]) throw std::runtime_error("tioga_amd: AMR patch solution not registered");
+    const int g = A.pgid[p];
+    const long long d3 = (long long)(A.dims[3 * g] + 2 * A.nf) * (A.dims[3 * g + 1] + 2 * A.nf) *
+                         (A.dims[3 * g + 2] + 2 * A.nf);
+    TG_HIP(hipMemcpyAsync(A.q.p + off[p], A.pq[p], sizeof(double) * d3 * nvar, hipMemcpyHostToDevice, s_));
+  }
+}
+
+void Engine::amr_tables()
+{
+  AmrState &A = amr();
+  AmrQTables qt{};
+  for (size_t ib = 0; ib < blocks_.size(); ib++) {
+    qt.q[ib] = qdev_[ib]->p;
+    qt.nnodes[ib] = blocks_[ib]->nnodes;
+  }
+  A.qt.reserve(1);
+  TG_HIP(hipMemcpyAsync(A.qt.p, &qt, sizeof qt, hipMemcpyHostToDevice, s_));
+  sync();
+}
+
+void Engine::amr_gather(int nvar, int interptype)
+{
+  AmrState &A = amr();
+  if (A.nrec == 0) return;
+  k_amr_gather<<<anb(A.nrec), ABS, 0, s_>>>(A.nrec, nvar, interptype, A.rec_lp.p, A.rec_node.p, A.rec_meta.p,
+                                             A.rec_base.p, A.rec_phi.p, A.q.p, A.qoff.p, A.pd3nf.p,
+                                             A.pd3nf.p + A.npatch, A.qt.p);
+}
+
+// tioga::dataUpdate_AMR (tioga.C:218-334) on one rank
+void Engine::data_update_amr(int nvar, int interptype)
+{
+  for (Block *b : blocks_)
+    if (b->uq == nullptr) {
+      fprintf(stderr, "Solution data not set, cannot update \n");  // tioga.C:236
+      return;
+    }
+  if (!amr_mode() || nvar <= 0) return;
+  AmrState &A = amr();
+  const int nbk = (int)blocks_.size();
+  while ((int)qdev_.size() < nbk) qdev_.push_back(new DBuf<double>);
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    size_t n = (size_t)b.nnodes * nvar;
+    qdev_[ib]->reserve(n);
+    TG_HIP(hipMemcpyAsync(qdev_[ib]->p, b.uq, sizeof(double) * n, hipMemcpyHostToDevice, s_));
+  }
+  amr_upload_q(nvar);
+  amr_tables();
+  if (A.nrec) amr_gather(nvar, interptype);
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    TG_HIP(hipMemcpyAsync(b.uq, qdev_[ib]->p, sizeof(double) * (size_t)b.nnodes * nvar, hipMemcpyDeviceToHost, s_));
+  }
+  sync();
+}
+
+// Device-resident dataUpdate_AMR: q of the blocks and patches uploaded once,
+// then `nrep` gathers; returns the mean device ms per update.
+double Engine::time_data_update_amr(int nvar, int interptype, int nrep, long long *nrec)
+{
+  AmrState &A = amr();
+  if (nrec) *nrec = A.nrec;
+  if (!amr_mode() || nvar <= 0 || nrep <= 0) return 0.0;
+  for (Block *b : blocks_)
+    if (b->uq == nullptr) throw std::runtime_error("tioga_amd: solution data not registered");
+  const int nbk = (int)blocks_.size();
+  while ((int)qdev_.size() < nbk) qdev_.push_back(new DBuf<double>);
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    size_t n = (size_t)b.nnodes * nvar;
+    qdev_[ib]->reserve(n);
+    TG_HIP(hipMemcpyAsync(qdev_[ib]->p, b.uq, sizeof(double) * n, hipMemcpyHostToDevice, s_));
+  }
+  amr_upload_q(nvar);
+  amr_tables();
+  amr_gather(nvar, interptype);  // warm
+  sync();
+  hipEvent_t e0, e1;
+  TG_HIP(hipEventCreate(&e0));
+  TG_HIP(hipEventCreate(&e1));
+  TG_HIP(hipEventRecord(e0, s_));
+  for (int i = 0; i < nrep; i++) amr_gather(nvar, interptype);
+  TG_HIP(hipEventRecord(e1, s_));
+  TG_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  TG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  TG_HIP(hipEventDestroy(e0));
+  TG_HIP(hipEventDestroy(e1));
+  return ms / nrep;
+}
+
+// Inspection, under the reference's names where it has them:
+//   per block (btag >= 1): "receptorIdCart", "donorIdCart", "rxyzCart"
+//   records (btag 0), in list order: "rec_patch", "rec_info" (rank, point,
+//   block), "rec_ijk" (8 stencil cells, i j k each), "rec_w" (8 weights),
+//   "rec_unique_cells" (number of distinct (patch, cell) donors)
+long Engine::amr_array(int btag, const std::string &name, void *out)
+{
+  if (!amr_ || !amr_->connected) return 0;
+  AmrState &A = *amr_;
+  if (btag >= 1) {
+    int ib = -1;
+    Block *b = find(btag, &ib);
+    if (!b) return -1;
+    const int o = A.rcp_off[ib], n = A.rcp_off[ib + 1] - o;
+    if (name == "receptorIdCart" || name == "donorIdCart") {
+      if (out && n)
+        TG_HIP(hipMemcpy(out, (name == "receptorIdCart" ? A.rnode.p : A.rdonor.p) + o, sizeof(int) * n,
+                         hipMemcpyDeviceToHost));
+      return n;
+    }
+    if (name == "rxyzCart") {
+      if (out && n) TG_HIP(hipMemcpy(out, A.rxyz.p + 3 * (size_t)o, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+      return 3 * (long)n;
+    }
+    return -1;
+  }
+  const int n = A.nrec;
+  const long want = name == "rec_patch" ? n : name == "rec_info" ? 3l * n : name == "rec_ijk" ? 24l * n
+                  : name == "rec_w" ? 8l * n : name == "rec_unique_cells" ? 1 : -1;
+  if (want < 0 || !out || n == 0) return want < 0 ? -1 : (n == 0 && name != "rec_unique_cells" ? 0 : want);
+  std::vector<int> lp(n), meta(n), base(n), rid(n);
+  std::vector<double> phi(6 * (size_t)n);
+  TG_HIP(hipMemcpy(lp.data(), A.rec_lp.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(meta.data(), A.rec_meta.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(base.data(), A.rec_base.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(rid.data(), A.rec_rid.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(phi.data(), A.rec_phi.p, sizeof(double) * phi.size(), hipMemcpyDeviceToHost));
+  if (name == "rec_patch") {
+    std::copy(lp.begin(), lp.end(), (int *)out);
+    return n;
+  }
+  if (name == "rec_info") {
+    int *o = (int *)out;
+    for (int r = 0; r < n; r++) {
+      const int blk = meta[r] >> 3;
+      o[3 * r] = 0;
+      o[3 * r + 1] = rid[r] - A.rcp_off[blk];
+      o[3 * r + 2] = blk;
+    }
+    return 3l * n;
+  }
+  if (name == "rec_w") {
+    double *o = (double *)out;
+    for (int r = 0; r < n; r++) {
+      const double px[2] = {phi[r], phi[(size_t)n + r]}, py[2] = {phi[2 * (size_t)n + r], phi[3 * (size_t)n + r]},
+                   pz[2] = {phi[4 * (size_t)n + r], phi[5 * (size_t)n + r]};
+      int m = 0;
+      for (int k = 0; k < 2; k++)
+        for (int j = 0; j < 2; j++)
+          for (int i = 0; i < 2; i++) o[8 * (size_t)r + m++] = pz[k] * py[j] * px[i];
+    }
+    return 8l * n;
+  }
+  // stencil cells from the base index and the per-axis step bits
+  std::vector<long long> cells;
+  int *o = (int *)out;
+  for (int r = 0; r < n; r++) {
+    const int g = A.pgid[lp[r]];
+    const int sx = A.dims[3 * g] + 2 * A.nf, sy = A.dims[3 * g + 1] + 2 * A.nf;
+    const int i0 = base[r] % sx - A.nf, j0 = (base[r] / sx) % sy - A.nf, k0 = base[r] / (sx * sy) - A.nf;
+    const int bx = meta[r] & 1, by = (meta[r] >> 1) & 1, bz = (meta[r] >> 2) & 1;
+    int m = 0;
+    for (int k = 0; k < 2; k++)
+      for (int j = 0; j < 2; j++)
+        for (int i = 0; i < 2; i++, m++) {
+          const int ci = i0 + i * bx, cj = j0 + j * by, ck = k0 + k * bz;
+          if (name == "rec_ijk") {
+            o[24 * (size_t)r + 3 * m] = ci;
+            o[24 * (size_t)r + 3 * m + 1] = cj;
+            o[24 * (size_t)r + 3 * m + 2] = ck;
+          } else {
+            cells.push_back(((long long)lp[r] << 32) | (long long)((ck + A.nf) * sy * sx + (cj + A.nf) * sx + ci + A.nf));
+          }
+        }
+  }
+  if (name == "rec_unique_cells") {
+    std::sort(cells.begin(), cells.end());
+    *o = (int)(std::unique(cells.begin(), cells.end()) - cells.begin());
+    return 1;
+  }
+  return 24l * n;
+}
+
+}  // namespace tg

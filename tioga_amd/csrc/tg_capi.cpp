@@ -99,6 +99,25 @@ void tioga::reduce_fringes(void)
   throw std::runtime_error("tioga_amd: reduce_fringes is not available in this engine");
 }
 
+void tioga::register_amr_global_data(int nf, int qstride, double *qnodein, int *idata, double *rdata, int ngridsin,
+                                     int qnodesize)
+{
+  eng_->amr_global_data(nf, qstride, qnodein, idata, rdata, ngridsin, qnodesize);
+}
+
+void tioga::set_amr_patch_count(int npatchesin) { eng_->amr_patch_count(npatchesin); }
+
+void tioga::register_amr_local_data(int ipatch, int global_id, int *iblank, double *q)
+{
+  eng_->amr_local_data(ipatch, global_id, iblank, q);
+}
+
+void tioga::performConnectivityAMR(void) { eng_->perform_connectivity_amr(); }
+
+void tioga::dataUpdate_AMR(int nvar, int interptype) { eng_->data_update_amr(nvar, interptype); }
+
+int tioga::iamrGlobal() const { return eng_->amr_mode() ? 1 : 0; }
+
 }  // namespace TIOGA
 
 // ---------------------------------------------------------------------------
@@ -208,7 +227,11 @@ void tioga_dataupdate_mb_(int *nvar, char *itype)
       printf("#tiogaInterface.C:dataupdate_:unknown data orientation\n");
       return;
     }
-    tg_or_throw().dataUpdate(*nvar, t);
+    // tiogaInterface.C:194-204
+    if (tg_or_throw().iamrGlobal())
+      tg_or_throw().dataUpdate_AMR(*nvar, t);
+    else
+      tg_or_throw().dataUpdate(*nvar, t);
   });
 }
 
@@ -283,14 +306,31 @@ void tioga_delete_(void)
   });
 }
 
-void tioga_register_amr_global_data_(int *, int *, double *, int *, double *, int *, int *)
+void tioga_register_amr_global_data_(int *nf, int *qstride, double *qnodein, int *idata, double *rdata,
+                                     int *ngridsin, int *qnodesize)
 {
-  not_available("tioga_register_amr_global_data");
+  guarded("tioga_register_amr_global_data", [&] {
+    tg_or_throw().register_amr_global_data(*nf, *qstride, qnodein, idata, rdata, *ngridsin, *qnodesize);
+  });
 }
-void tioga_register_amr_patch_count_(int *) { not_available("tioga_register_amr_patch_count"); }
-void tioga_register_amr_local_data_(int *, int *, int *, double *) { not_available("tioga_register_amr_local_data"); }
+
+void tioga_register_amr_patch_count_(int *npatches)
+{
+  guarded("tioga_register_amr_patch_count", [&] { tg_or_throw().set_amr_patch_count(*npatches); });
+}
+
+void tioga_register_amr_local_data_(int *ipatch, int *global_id, int *iblank, double *q)
+{
+  guarded("tioga_register_amr_local_data",
+          [&] { tg_or_throw().register_amr_local_data(*ipatch, *global_id, iblank, q); });
+}
+
+void tioga_performconnectivity_amr_(void)
+{
+  guarded("tioga_performconnectivity_amr", [] { tg_or_throw().performConnectivityAMR(); });
+}
+
 void tioga_performconnectivity_highorder_(void) { not_available("tioga_performconnectivity_highorder"); }
-void tioga_performconnectivity_amr_(void) { not_available("tioga_performconnectivity_amr"); }
 void tioga_set_highorder_callback_(void (*)(int *, int *), void (*)(int *, int *, double *),
                                    void (*)(int *, double *, int *, double *),
                                    void (*)(int *, double *, int *, int *, double *, double *, int *),
@@ -346,6 +386,21 @@ double tioga_amd_time_dataupdate(int nvar, int interptype, int nrep, long long *
   double r = 0;
   guarded("tioga_amd_time_dataupdate",
           [&] { r = tg_or_throw().engine()->time_data_update(nvar, interptype, nrep, nreceptors); });
+  return r;
+}
+
+double tioga_amd_time_dataupdate_amr(int nvar, int interptype, int nrep, long long *nrecords)
+{
+  double r = 0;
+  guarded("tioga_amd_time_dataupdate_amr",
+          [&] { r = tg_or_throw().engine()->time_data_update_amr(nvar, interptype, nrep, nrecords); });
+  return r;
+}
+
+long tioga_amd_amr_array(int btag, const char *name, void *out)
+{
+  long r = -1;
+  guarded("tioga_amd_amr_array", [&] { r = tg_or_throw().engine()->amr_array(btag, name, out); });
   return r;
 }
 

@@ -183,6 +183,8 @@ struct SearchScratch {
   size_t cap = 0;  // capacity of the flat candidate list
 };
 
+struct AmrState;  // tg_amr.hip
+
 struct Pair {
   int local = 0, ob = 0, iblk_remote = 0, tag_remote = 0, send_tag = 0, recv_tag = 0;
   OBB obc{};
@@ -250,6 +252,23 @@ class Engine {
   double last_search_kernel_ms() const { return last_search_kernel_ms_; }
   int query_set_nelem(int id) const { return (id >= 0 && id < (int)qsets_.size()) ? qsets_[id]->nelem : -1; }
 
+  // AMR Cartesian background, gather direction (tg_amr.hip):
+  // tioga::register_amr_global_data / set_amr_patch_count /
+  // register_amr_local_data (tioga.C:692-713), performConnectivityAMR
+  // (tioga.C:180-216) and dataUpdate_AMR (tioga.C:218-334) on one rank.
+  void amr_global_data(int nf, int qstride, const double *qnode, const int *idata, const double *rdata, int ngrids,
+                       int qnodesize);
+  void amr_patch_count(int npatch);
+  void amr_local_data(int ipatch, int global_id, int *iblank, double *q);
+  void perform_connectivity_amr();
+  void data_update_amr(int nvar, int interptype);
+  // true once performConnectivityAMR ran with patches (the reference's
+  // iamrGlobal, which routes tioga_dataupdate_mb_ to dataUpdate_AMR)
+  bool amr_mode() const;
+  // device-resident dataUpdate_AMR, `nrep` gathers; mean device ms each
+  double time_data_update_amr(int nvar, int interptype, int nrep, long long *nrec);
+  long amr_array(int btag, const std::string &name, void *out);
+
   PhaseTimes last_times;
   int weight_aborts() const { return weight_aborts_; }
   hipStream_t stream() const { return s_; }
@@ -299,6 +318,13 @@ class Engine {
   DedupeScratch dedw_;
   double prep_acc_[4] = {0, 0, 0, 0};
   std::vector<DBuf<double> *> qdev_;
+  // AMR Cartesian background (tg_amr.hip)
+  AmrState *amr_ = nullptr;
+  AmrState &amr();
+  void amr_release();
+  void amr_upload_q(int nvar);
+  void amr_tables();
+  void amr_gather(int nvar, int interptype);
 };
 
 }  // namespace tg

@@ -1390,6 +1390,7 @@ Engine::~Engine()
   query_set_clear();
   for (Block *b : blocks_) delete b;
   for (auto *q : qdev_) delete q;
+  amr_release();
   if (s_) (void)hipStreamDestroy(s_);
 }
 

@@ -70,6 +70,14 @@ def load():
         L.tioga_amd_query_set_nelem.argtypes = [C.c_int]
         L.tioga_amd_query_set_nelem.restype = C.c_int
         L.tioga_amd_weight_aborts.restype = C.c_int
+        L.tioga_register_amr_global_data_.argtypes = [_P] * 7
+        L.tioga_register_amr_patch_count_.argtypes = [_P]
+        L.tioga_register_amr_local_data_.argtypes = [_P] * 4
+        L.tioga_performconnectivity_amr_.argtypes = []
+        L.tioga_amd_time_dataupdate_amr.argtypes = [C.c_int, C.c_int, C.c_int, _P]
+        L.tioga_amd_time_dataupdate_amr.restype = C.c_double
+        L.tioga_amd_amr_array.argtypes = [C.c_int, C.c_char_p, _P]
+        L.tioga_amd_amr_array.restype = C.c_long
         _lib = L
     return _lib
 
@@ -83,6 +91,7 @@ def _ip(v: int):
 
 
 _I32 = {"isearch", "tagsearch", "donorId", "xtag", "elementList", "adtIntegers", "iblank_cell"}
+_AMR_F64 = {"rxyzCart", "rec_w"}
 PHASES = ("holemap", "boxes", "searchdata", "search_prep", "search_kernel", "donors", "total", "nsearch",
           "prep_obb", "prep_filter", "prep_adt", "prep_dedupe")
 
@@ -189,6 +198,65 @@ class Tioga:
             self.L.tioga_getdonorinfo_(_ip(btag), _p(rec), _p(ind), _p(frac), _p(d))
             self._check()
         return rec[:4 * dc], ind[:fc], frac[:fc]
+
+    # ---- AMR Cartesian background (tiogaInterface.C:132-148, 165-168) ----
+    def register_amr(self, grid, patch_q: Sequence[np.ndarray]) -> None:
+        """Register the patch hierarchy; patch_q[local id] holds that patch's
+        fields (kept by pointer, as the reference keeps them)."""
+        idata = np.ascontiguousarray(grid.idata, np.int32)
+        rdata = np.ascontiguousarray(grid.rdata, np.float64)
+        qnode = np.ascontiguousarray(grid.qnode, np.float64)
+        self._keep += [idata, rdata, qnode]
+        self.L.tioga_register_amr_global_data_(_ip(grid.nf), _ip(grid.qstride), _p(qnode), _p(idata), _p(rdata),
+                                               _ip(grid.ngrids), _ip(qnode.size))
+        self._check()
+        self.L.tioga_register_amr_patch_count_(_ip(len(patch_q)))
+        self._check()
+        order = grid.patch_order()
+        for lp, q in enumerate(patch_q):
+            g = int(order[lp])
+            ibl = np.ones(grid.d3nf(g), np.int32)
+            q = np.ascontiguousarray(q, np.float64)
+            self._keep += [ibl, q]
+            self.L.tioga_register_amr_local_data_(_ip(lp), _ip(g), _p(ibl), _p(q))
+            self._check()
+
+    def connect_amr(self):
+        self.L.tioga_performconnectivity_amr_()
+        self._check()
+
+    def amr_array(self, btag: int, name: str) -> np.ndarray:
+        n = self.L.tioga_amd_amr_array(btag, name.encode(), None)
+        self._check()
+        out = np.zeros(max(n, 0), np.float64 if name in _AMR_F64 else np.int32)
+        if n > 0:
+            self.L.tioga_amd_amr_array(btag, name.encode(), _p(out))
+            self._check()
+        return out
+
+    def time_data_update_amr(self, nvar: int, interptype: int, nrep: int):
+        n = C.c_longlong(0)
+        ms = self.L.tioga_amd_time_dataupdate_amr(nvar, interptype, nrep, C.byref(n))
+        self._check()
+        return ms, n.value
+
+    def amr_results(self, qs=None, nvar: int = 0, interptype: int = 0) -> Dict[str, np.ndarray]:
+        """The arrays the reference driver dumps for the AMR gather path."""
+        r: Dict[str, np.ndarray] = {}
+        for i, b in enumerate(self.blocks):
+            p = f"b{i}."
+            r[p + "iblank"] = self.iblank[i].copy()
+            r[p + "iblank_cell"] = self.array(b.tag, "iblank_cell")
+            r[p + "nodeRes"] = self.array(b.tag, "nodeRes")
+            for nm in ("receptorIdCart", "donorIdCart", "rxyzCart"):
+                r[p + nm] = self.amr_array(b.tag, nm)
+            r[p + "ncart"] = np.array([r[p + "receptorIdCart"].size], np.int32)
+        for nm in ("rec_patch", "rec_info", "rec_ijk", "rec_w"):
+            r["amr." + nm] = self.amr_array(0, nm)
+        if qs is not None:
+            for i, q in enumerate(self.data_update_copies(qs, nvar, interptype)):
+                r[f"b{i}.q"] = q
+        return r
 
     # ---- extensions ----
     def array(self, btag: int, name: str) -> np.ndarray:

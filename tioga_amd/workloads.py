@@ -17,6 +17,15 @@ Workloads (sizes as stated; all coordinates FP64):
   (tag 1).  Poles are merged, so pole rows are prisms.
 * ``tiny``     the config-1 construction at a size the oracle finishes in a
   fraction of a second (parity cases).
+* ``config5``  AMR Cartesian background: 3 levels (refinement ratio 2) of
+  4096 patches of 16^3 cells with one ghost layer on [-1,1]^3 (level 0: the
+  whole cube at h = 1/64, 512 patches; level 1: [-1,1]x[-0.75,0.75]x
+  [-0.5,0.5] at h/2, 1536 patches; level 2: [-0.5,0.5]^2x[-0.25,0.25] at
+  h/4, 2048 patches), pdegree 0, 8 FP64 fields per patch.  Receptors are the
+  nodes of the mixed-element near-body shell of config2 (the construction
+  named for this workload; config1's shell is all-hex), every one of them
+  marked unresolved by a node resolution of BIGVALUE, so every node takes its
+  value from the patches.
 """
 from __future__ import annotations
 
@@ -267,3 +276,94 @@ def rigid_motion(x: np.ndarray, angle_deg: float, axis, shift) -> np.ndarray:
     R = np.eye(3) + np.sin(th) * kx + (1 - np.cos(th)) * kx @ kx
     c = x.mean(axis=0)
     return (x - c) @ R.T + c + np.asarray(shift, np.float64)
+
+
+BIGVALUE = 1.0e15  # codetypes.h:41
+
+
+@dataclass
+class AmrGrid:
+    """A Cartesian patch hierarchy in the layout tioga_register_amr_global_data_
+    reads (tiogaInterface.C:132-136, CartGrid.C:22-70): per grid 11 ints
+    (global id, level, proc, porder, local id, ilo[3], ihi[3]) and 6 doubles
+    (xlo[3], dx[3]).  One rank: proc 0, and patch ``local id`` is registered
+    with ``tioga_register_amr_local_data_(local id, global id, ...)``."""
+    nf: int
+    idata: np.ndarray   # (ngrids, 11) int32
+    rdata: np.ndarray   # (ngrids, 6) float64
+    qnode: np.ndarray = field(default_factory=lambda: np.zeros(1))
+    qstride: int = 1
+
+    @property
+    def ngrids(self) -> int:
+        return int(self.idata.shape[0])
+
+    def dims(self, g: int) -> np.ndarray:
+        return self.idata[g, 8:11] - self.idata[g, 5:8] + 1
+
+    def d3nf(self, g: int) -> int:
+        return int(np.prod(self.dims(g) + 2 * self.nf))
+
+    def patch_order(self) -> np.ndarray:
+        """Global id of each local patch (local ids 0..ngrids-1)."""
+        out = np.zeros(self.ngrids, np.int64)
+        out[self.idata[:, 4]] = np.arange(self.ngrids)
+        return out
+
+    def patch_fields(self, nvar: int, seed: int) -> List[np.ndarray]:
+        """Per local patch: nvar fields uniform[-1,1], field n at
+        n*d3nf (get_amr_index_xyz.c:27-45), ghosts included."""
+        rng = np.random.default_rng(seed)
+        return [rng.uniform(-1.0, 1.0, self.d3nf(int(g)) * nvar) for g in self.patch_order()]
+
+
+def amr_hierarchy(origin, h0: float, level_boxes, patch: int = 16, nf: int = 1,
+                  shuffle_seed: Optional[int] = None, local_seed: Optional[int] = None) -> AmrGrid:
+    """Patches of ``patch``^3 cells tiling one box per level.  Level l has
+    spacing h0/2^l; ``level_boxes[l]`` = (lo[3], hi[3]) in level-l cell indices
+    (hi exclusive).  Grids are listed level by level, (k, j, i) within a level,
+    unless ``shuffle_seed`` permutes the list; ``local_seed`` permutes the
+    local ids."""
+    rows, reals = [], []
+    for l, (lo, hi) in enumerate(level_boxes):
+        h = h0 / 2 ** l
+        lo, hi = np.asarray(lo), np.asarray(hi)
+        for k in range(lo[2], hi[2], patch):
+            for j in range(lo[1], hi[1], patch):
+                for i in range(lo[0], hi[0], patch):
+                    a = np.array([i, j, k])
+                    b = np.minimum(a + patch, hi) - 1
+                    rows.append([0, l, 0, 0, 0, *a, *b])
+                    reals.append([origin[0] + a[0] * h, origin[1] + a[1] * h, origin[2] + a[2] * h, h, h, h])
+    idata = np.array(rows, np.int32)
+    rdata = np.array(reals, np.float64)
+    if shuffle_seed is not None:
+        p = np.random.default_rng(shuffle_seed).permutation(len(rows))
+        idata, rdata = idata[p], rdata[p]
+    n = idata.shape[0]
+    idata[:, 0] = np.arange(n)
+    lid = np.arange(n) if local_seed is None else np.random.default_rng(local_seed).permutation(n)
+    idata[:, 4] = lid
+    return AmrGrid(nf, np.ascontiguousarray(idata), np.ascontiguousarray(rdata))
+
+
+def all_receptors(b: Block) -> Block:
+    """The block with every node marked unresolved (node resolution
+    BIGVALUE, cell resolution 1), so that every node becomes a receptor of
+    the Cartesian background (cartOps.C:73-97)."""
+    b.nres = np.full(b.nnodes, BIGVALUE)
+    b.cres = np.ones(b.ncells)
+    return b
+
+
+def config5_grid() -> AmrGrid:
+    """The patch hierarchy of the AMR gather workload."""
+    return amr_hierarchy((-1.0, -1.0, -1.0), 2.0 / 128,
+                         [((0, 0, 0), (128, 128, 128)),
+                          ((0, 32, 64), (256, 224, 192)),
+                          ((128, 128, 192), (384, 384, 320))], patch=16, nf=1)
+
+
+def config5():
+    """(near-body blocks, AmrGrid) of the AMR gather workload."""
+    return [all_receptors(mixed_shell(360, 180, tag=1))], config5_grid()
