@@ -16,9 +16,13 @@ query set's OBB, ADT).  A step is one pass of the search kernels over both
 query sets with all inputs resident in HBM.
 
 Printed: one JSON line with searches/s, the search kernel's HBM roofline
-fraction (algorithmic bytes per launch over its HIP-event duration), the CPU
-baseline (the oracle's sequential search on a bounded sample, 1 core), the end
-to end connectivity time and dataUpdate bandwidth.
+fraction (algorithmic bytes per launch over its HIP-event duration; `traffic`
+is the HBM bytes per launch that rocprofv3 counted, from the committed
+summary under profiles/), the CPU baseline (the oracle's sequential search on
+a bounded sample, 1 core), the end to end connectivity time, dataUpdate
+bandwidth, and the AMR gather leg (`amr_gather`: 4096 patches of 16^3 cells on
+3 levels, every node of the mixed-element shell a receptor, 1000
+device-resident dataUpdate_AMR gathers of 8 fields).
 """
 from __future__ import annotations
 
@@ -82,6 +86,20 @@ def dataupdate_bytes(tg, blocks, nvar: int) -> float:
     return tot
 
 
+def profiled_traffic(summary: str, key: str):
+    """HBM bytes per launch that rocprofv3 --pmc measured for this workload
+    (FETCH_SIZE and WRITE_SIZE, each calibrated on a plain device copy); the
+    committed summary under profiles/ names the runs.  None when absent."""
+    try:
+        with open(os.path.join(ROOT, "profiles", summary)) as f:
+            for line in f:
+                if line.startswith(key + ":"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def amr_gather_leg(shell, nvar: int = 8, updates: int = 1000):
     """The AMR gather workload: the mixed-element shell, every node a
     receptor, against 4096 patches of 16^3 cells on 3 levels (8 FP64 fields
@@ -121,7 +139,8 @@ def amr_gather_leg(shell, nvar: int = 8, updates: int = 1000):
             "kernel": "k_amr_gather", "connectivity_amr_ms": round(1000 * t_conn, 2),
             "ms_per_update": round(ms, 5), "algorithmic_bytes_per_update": algo,
             "GBps": round(gbps, 2) if gbps else None,
-            "frac": round(gbps / HBM_PEAK_GBPS, 4) if gbps else None}
+            "frac": round(gbps / HBM_PEAK_GBPS, 4) if gbps else None,
+            "traffic": profiled_traffic("amr_gather_config5.md", "traffic_bytes_per_update")}
 
 
 def cpu_baseline(blocks, sets, budget_s: float):
@@ -234,7 +253,9 @@ def main() -> int:
     algo = sum(search_bytes(byblk[btag], d, ne) for (btag, _), d, ne in zip(sets, donors, nelem_of))
     achieved = algo / (kern_ms / 1000.0) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBPS, 5),
+            "traffic": profiled_traffic("search_pipeline_config2.md", "traffic_bytes_per_launch")
+            if args.workload == "config2" else None,
             "kernel": "k_search", "algorithmic_bytes_per_step": algo, "launches_per_step": len(qids),
             "avg_launch_ms": round(kern_ms / len(qids), 4)}
 

@@ -191,6 +191,37 @@ __global__ void k_amr_take(int n, const int *__restrict__ idx, const unsigned *_
   if (i < n) out[i] = all[idx[i]];
 }
 
+__global__ void k_amr_iota(int n, int *__restrict__ a)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+
+__global__ void k_amr_key_of(int n, const int *__restrict__ perm, const int *__restrict__ src,
+                             unsigned *__restrict__ key)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) key[i] = (unsigned)src[perm[i]];
+}
+
+// records from list order into gather order (perm[i] = list index)
+__global__ void k_amr_permute(int n, const int *__restrict__ perm, const int *__restrict__ a0,
+                              const int *__restrict__ a1, const int *__restrict__ a2, const int *__restrict__ a3,
+                              const int *__restrict__ a4, const double *__restrict__ ph, int *__restrict__ b0,
+                              int *__restrict__ b1, int *__restrict__ b2, int *__restrict__ b3, int *__restrict__ b4,
+                              double *__restrict__ phb)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = perm[i];
+  b0[i] = a0[r];
+  b1[i] = a1[r];
+  b2[i] = a2[r];
+  b3[i] = a3[r];
+  b4[i] = a4[r];
+  for (int m = 0; m < 6; m++) phb[(size_t)m * n + i] = ph[(size_t)m * n + r];
+}
+
 // MeshBlock::setCartIblanks (cartOps.C:49-57)
 __global__ void k_amr_iblank(int n, const int *__restrict__ donor, const int *__restrict__ rnode,
                              const int *__restrict__ rblk, int blk, int *__restrict__ iblank)
@@ -582,6 +613,43 @@ void Engine::perform_connectivity_amr()
             "CartBlock.C:246)");
       A.rec_count.assign(A.npatch, 0);
       for (unsigned v : lps) A.rec_count[v]++;
+      // Gather order: by patch, then by stencil base index, so that the lanes
+      // of a wave read neighbouring cells of one patch.  Every record writes
+      // its own receptor, so the order leaves the results unchanged; the
+      // list order stays recoverable as (patch, receptor index).
+      int maxd3 = 1;
+      for (int p = 0; p < A.npatch; p++) maxd3 = std::max(maxd3, pd3nf[p]);
+      int bbits = 1;
+      while (bbits < 32 && ((unsigned)(maxd3 - 1) >> bbits)) bbits++;
+      DBuf<int> perm0, perm1;
+      perm0.reserve(nr);
+      perm1.reserve(nr);
+      k_amr_iota<<<anb(nr), ABS, 0, s_>>>(nr, perm0.p);
+      k_amr_key_of<<<anb(nr), ABS, 0, s_>>>(nr, perm0.p, A.rec_base.p, keys.p);
+      sort_pairs(keys.p, keys2.p, perm0.p, perm1.p, nr, bbits, s_);
+      k_amr_key_of<<<anb(nr), ABS, 0, s_>>>(nr, perm1.p, A.rec_lp.p, keys.p);
+      sort_pairs(keys.p, keys2.p, perm1.p, perm0.p, nr, bits, s_);
+      DBuf<int> b0, b1, b2, b3, b4;
+      DBuf<double> bph;
+      b0.reserve(nr);
+      b1.reserve(nr);
+      b2.reserve(nr);
+      b3.reserve(nr);
+      b4.reserve(nr);
+      bph.reserve(6 * (size_t)nr);
+      k_amr_permute<<<anb(nr), ABS, 0, s_>>>(nr, perm0.p, A.rec_lp.p, A.rec_node.p, A.rec_meta.p, A.rec_base.p,
+                                             A.rec_rid.p, A.rec_phi.p, b0.p, b1.p, b2.p, b3.p, b4.p, bph.p);
+      sync();
+      auto take = [](auto &dst, auto &src) {
+        std::swap(dst.p, src.p);
+        std::swap(dst.cap, src.cap);
+      };
+      take(A.rec_lp, b0);
+      take(A.rec_node, b1);
+      take(A.rec_meta, b2);
+      take(A.rec_base, b3);
+      take(A.rec_rid, b4);
+      take(A.rec_phi, bph);
     }
   }
   if (A.nrec == 0) A.rec_count.assign(A.npatch, 0);
@@ -747,13 +815,29 @@ long Engine::amr_array(int btag, const std::string &name, void *out)
   const long want = name == "rec_patch" ? n : name == "rec_info" ? 3l * n : name == "rec_ijk" ? 24l * n
                   : name == "rec_w" ? 8l * n : name == "rec_unique_cells" ? 1 : -1;
   if (want < 0 || !out || n == 0) return want < 0 ? -1 : (n == 0 && name != "rec_unique_cells" ? 0 : want);
+  std::vector<int> glp(n), gmeta(n), gbase(n), grid(n);
+  std::vector<double> gphi(6 * (size_t)n);
+  TG_HIP(hipMemcpy(glp.data(), A.rec_lp.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(gmeta.data(), A.rec_meta.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(gbase.data(), A.rec_base.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(grid.data(), A.rec_rid.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(gphi.data(), A.rec_phi.p, sizeof(double) * gphi.size(), hipMemcpyDeviceToHost));
+  // records are stored in gather order; the list order is (patch, receptor)
+  std::vector<int> ord(n);
+  for (int r = 0; r < n; r++) ord[r] = r;
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) {
+    return glp[a] != glp[b] ? glp[a] < glp[b] : grid[a] < grid[b];
+  });
   std::vector<int> lp(n), meta(n), base(n), rid(n);
   std::vector<double> phi(6 * (size_t)n);
-  TG_HIP(hipMemcpy(lp.data(), A.rec_lp.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-  TG_HIP(hipMemcpy(meta.data(), A.rec_meta.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-  TG_HIP(hipMemcpy(base.data(), A.rec_base.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-  TG_HIP(hipMemcpy(rid.data(), A.rec_rid.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-  TG_HIP(hipMemcpy(phi.data(), A.rec_phi.p, sizeof(double) * phi.size(), hipMemcpyDeviceToHost));
+  for (int r = 0; r < n; r++) {
+    const int g = ord[r];
+    lp[r] = glp[g];
+    meta[r] = gmeta[g];
+    base[r] = gbase[g];
+    rid[r] = grid[g];
+    for (int m = 0; m < 6; m++) phi[(size_t)m * n + r] = gphi[(size_t)m * n + g];
+  }
   if (name == "rec_patch") {
     std::copy(lp.begin(), lp.end(), (int *)out);
     return n;
