@@ -17,6 +17,13 @@ Workloads (sizes as stated; all coordinates FP64):
   (tag 1).  Poles are merged, so pole rows are prisms.
 * ``tiny``     the config-1 construction at a size the oracle finishes in a
   fraction of a second (parity cases).
+* ``config3``  batching: 64 hex/prism spherical shells (tags 1..64) whose
+  cell counts are log-uniform in [1e4, 1e6], wall radius uniform in
+  [0.02, 0.08], outer radius 2.5 x the wall radius, centres uniform in
+  [-0.8, 0.8]^3, uniformly random rotations (Shoemake's quaternion method),
+  seed 3; background 192^3 hex cube [-1,1]^3 (tag 65).  Each shell is the
+  mixed_shell construction with a spherical wall and only hex and prism
+  layers (its polar rows are prisms).
 * ``config5``  AMR Cartesian background: 3 levels (refinement ratio 2) of
   4096 patches of 16^3 cells with one ghost layer on [-1,1]^3 (level 0: the
   whole cube at h = 1/64, 512 patches; level 1: [-1,1]x[-0.75,0.75]x
@@ -367,3 +374,42 @@ def config5_grid() -> AmrGrid:
 def config5():
     """(near-body blocks, AmrGrid) of the AMR gather workload."""
     return [all_receptors(mixed_shell(360, 180, tag=1))], config5_grid()
+
+
+def shoemake_rotation(u) -> np.ndarray:
+    """Uniformly random rotation from three uniform[0,1) numbers (K. Shoemake,
+    "Uniform random rotations", Graphics Gems III, 1992)."""
+    u1, u2, u3 = u
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    w, x, y, z = (a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2),
+                  b * np.sin(2 * np.pi * u3), b * np.cos(2 * np.pi * u3))
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def sphere_shell(ncells: float, r_wall: float, tag: int, center, rot) -> Block:
+    """Hex/prism shell of about ncells cells around a sphere of radius r_wall
+    out to 2.5 r_wall: nphi = 2 ntheta azimuths, ntheta polar rows and
+    ntheta/2 radial layers (half hex, half prism), rotated by rot and moved
+    to center."""
+    nth = max(4, int(round(ncells ** (1.0 / 3.0))))
+    nr = max(2, nth // 2)
+    lh = nr // 2
+    b = mixed_shell(2 * nth, nth, layers=(lh, nr - lh, 0, 0), tag=tag, wall_axes=(r_wall, r_wall, r_wall),
+                    r_outer=2.5 * r_wall, stretch=1.05)
+    b.x = b.x @ np.asarray(rot).T + np.asarray(center, np.float64)[None, :]
+    return b
+
+
+def config3(nshell: int = 64, nbox: int = 192, seed: int = 3, lo: float = 1e4, hi: float = 1e6) -> List[Block]:
+    rng = np.random.default_rng(seed)
+    blocks = []
+    for t in range(1, nshell + 1):
+        n = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        r = float(rng.uniform(0.02, 0.08))
+        c = rng.uniform(-0.8, 0.8, 3)
+        rot = shoemake_rotation(rng.uniform(size=3))
+        blocks.append(sphere_shell(n, r, t, c, rot))
+    blocks.append(cart_box(nbox, tag=nshell + 1))
+    return blocks
