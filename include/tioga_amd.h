@@ -183,6 +183,11 @@ void tioga_amd_phase_times(double *out);
 void tioga_amd_host_median(int *ix, double *x, int n, int lists);
 void tioga_amd_host_find_obb(const double *x, int n, double *out);
 int tioga_amd_host_build_adt(int nelem, const double *ebox, int *adtInt, double *adtReal, double *ext);
+/* The device ADT builder on ntree trees at once (tree t: n[t] boxes; all
+   arrays concatenated in tree order, host memory; ext: 6 per tree); returns
+   0, or -1 with the error status set.  Needs a GPU. */
+int tioga_amd_device_build_adts(int ntree, const int *n, const double *ebox, int *adtInt, double *adtReal,
+                                double *ext);
 void tioga_amd_host_unique_nodes(const double *x, const int *meshtag, double *rtag, int *xtag, int n);
 
 /* Donor weights outside [-0.2, 1.2] seen (the reference aborts on one). */

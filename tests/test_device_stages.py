@@ -1,0 +1,63 @@
+"""Device stages against the oracle at sizes the golden cases do not reach.
+
+The ADT builder splits segments longer than 65536 elements across many
+workgroups and builds several trees in one level loop; these tests run it
+through the C ABI on trees of mixed sizes (long, medium, short, single,
+empty) whose boxes have heavy ties along every cut dimension and signed
+zeros, and compare every tree node for node with the oracle's restatement
+of ADT.C / buildADTrecursion.c / median.F90."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle.oracle import lib as oracle_lib, ptr
+
+pytestmark = pytest.mark.gpu
+
+
+def boxes(n, rng):
+    c = np.round(rng.uniform(-1, 1, (n, 3)) * 64) / 64  # coarse grid: many equal coordinates
+    c[rng.uniform(size=(n, 3)) < 0.02] = 0.0
+    h = np.round(rng.uniform(0.001, 0.05, (n, 3)) * 512) / 512
+    lo, hi = c - h, c + h
+    z = rng.uniform(size=(n, 3)) < 0.01
+    lo[z] = -0.0  # signed zeros tie with +0.0 under <
+    hi[rng.uniform(size=(n, 3)) < 0.01] = 0.0
+    hi = np.maximum(hi, lo)
+    return np.ascontiguousarray(np.concatenate([lo, hi], 1))
+
+
+def test_device_adt_batch_matches_oracle():
+    from tioga_amd.tioga import load
+
+    L = load()
+    L.tioga_amd_device_build_adts.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.tioga_amd_device_build_adts.restype = C.c_int
+    O = oracle_lib()
+    rng = np.random.default_rng(7)
+    sizes = [300000, 70000, 5000, 129, 128, 1, 0, 2, 66000]
+    ebs = [boxes(n, rng) for n in sizes]
+    for batch in ([0], list(range(len(sizes)))):
+        ns = np.array([sizes[t] for t in batch], np.int32)
+        eb = np.ascontiguousarray(np.concatenate([ebs[t] for t in batch]).reshape(-1)) if ns.sum() else np.zeros(6)
+        tot = int(ns.sum())
+        ai = np.zeros(4 * max(tot, 1), np.int32)
+        ar = np.zeros(6 * max(tot, 1))
+        ext = np.zeros(6 * len(batch))
+        print(f"[device adt] trees {ns.tolist()}", flush=True)
+        rc = L.tioga_amd_device_build_adts(len(batch), ptr(ns), ptr(eb), ptr(ai), ptr(ar), ptr(ext))
+        assert rc == 0
+        off = 0
+        for k, t in enumerate(batch):
+            n = sizes[t]
+            wi = np.zeros(4 * max(n, 1), np.int32)
+            wr = np.zeros(6 * max(n, 1))
+            we = np.zeros(6)
+            O.orc_build_adt(n, ptr(ebs[t]), ptr(wi), ptr(wr), ptr(we))
+            assert np.array_equal(ai[4 * off:4 * (off + n)], wi[:4 * n]), f"tree {t} adtIntegers"
+            assert np.array_equal(ar[6 * off:6 * (off + n)].view(np.int64), wr[:6 * n].view(np.int64)), \
+                f"tree {t} adtReals"
+            if n:
+                assert np.array_equal(ext[6 * k:6 * k + 6], we), f"tree {t} extents"
+            off += n

@@ -7,15 +7,25 @@
 // segment, and the left / right subtrees are the segment's two halves.  The
 // quicksort's exact swap order decides every later level, so it is kept.
 //
-// Here all segments of one level are processed in one launch.  A segment
-// longer than SMALL gets a workgroup that runs the quicksort's partition
-// rounds in list form (hoare_stop in tg_common.h): flags X >= pivot and
-// X <= pivot over the active range, workgroup scans turn them into the
-// ascending list A and descending list B, the first K pairs with A[m] < B[m]
-// are swapped in parallel, and one thread applies the loop control.
-// Segments of at most SMALL elements are finished by one thread each with
-// the sequential quicksort.  Node ids are the reference's preorder counter:
-// the left child of node me is me+1, the right child me+nleft.
+// The split sizes depend on the segment length alone (nleft-1 and nav-nleft
+// with nleft = (nav+1)/2), so the host lists every segment of every level
+// ahead of time, for any number of trees built together.  Per level:
+//   * long segments (more than BIGMIN elements) run the quicksort's
+//     partition rounds in list form across many workgroups: flags X >= pivot
+//     and X <= pivot, per-chunk counts, a scan, the ascending list A and
+//     descending list B, the K pairs with A[m] < B[m] swapped in parallel
+//     (K found by bisection, since A[m] - B[m] increases with m), and one
+//     thread applying the loop control (hoare_stop, median_control in
+//     tg_common.h); the host reads the loop state every few rounds;
+//   * medium segments get one workgroup each, which runs the same rounds;
+//   * segments of at most SMALL elements are finished, whole subtree, by one
+//     thread each with the sequential quicksort, after the last level.
+// The node box is the reference's sequential TIOGA_MIN / TIOGA_MAX fold,
+// whose result is the value at the last position of the minimal (maximal)
+// class under < (so -0.0 and +0.0 tie and the later one wins); the parallel
+// reductions pick that same element.  Node ids are the reference's preorder
+// counter: the left child of node me is me+1, the right child me+nleft.
+#include <algorithm>
 #include <stdexcept>
 #include <vector>
 
@@ -27,12 +37,23 @@ namespace {
 
 constexpr int BT = 512;        // threads per workgroup for long segments
 constexpr int PER = 8;         // consecutive positions per thread per chunk
+constexpr int CH = BT * PER;   // positions per chunk of a multi-workgroup round
 constexpr int SMALL = 128;     // segments at most this long: one thread
+constexpr int BIGMIN = 1 << 16;  // segments longer than this: many workgroups
+constexpr int ROUNDS = 8;      // multi-workgroup rounds between host checks
 
+// base: offset of the segment's tree in the concatenated element arrays;
+// start: the segment's first position within its tree
 struct Seg {
-  int start, nav, me, parent, side, level;
+  int base, start, nav, me, parent, side, level, tree;
 };
-static_assert(sizeof(Seg) <= 2 * sizeof(int4), "segment descriptors are stored in two int4 slots");
+static_assert(sizeof(Seg) == 2 * sizeof(int4), "segment descriptors are stored in two int4 slots");
+
+struct BigState {
+  double pv;
+  int lo, hi, st, nA, nB, K, rounds, pad;
+};
+static_assert(sizeof(BigState) <= 2 * sizeof(double4), "loop states are stored in two double4 slots");
 
 inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -187,71 +208,89 @@ __device__ void wg_median(int *IX, double *X, int n, int *A, int *B, WgState &sh
   __syncthreads();
 }
 
-// One level: a workgroup per segment of that level (2^level slots; empty
-// slots have nav = 0).  Long segments are split here; short ones are queued
-// for the one-thread-per-subtree kernel.
-__global__ void __launch_bounds__(BT) k_adt_level(const Seg *__restrict__ in, Seg *__restrict__ out,
-                                                  Seg *__restrict__ small, int *__restrict__ nsmall, int *ea,
-                                                  double *X, int *A, int *B, const double *__restrict__ ebox,
-                                                  int *__restrict__ adtInt, double *__restrict__ adtReal)
+// ---------------------------------------------------------------------------
+// node box: the reference's sequential fold, exactly
+
+// canonical class key: -0.0 and +0.0 compare equal under <, so they share a key
+__device__ __forceinline__ unsigned long long ckey(double v) { return dkey(v + 0.0); }
+
+struct KP {
+  unsigned long long k;
+  int p;
+};
+// min class first, then the later position; max class first, then later
+__device__ __forceinline__ KP kp_min(KP a, KP b) { return (a.k < b.k || (a.k == b.k && a.p > b.p)) ? a : b; }
+__device__ __forceinline__ KP kp_max(KP a, KP b) { return (a.k > b.k || (a.k == b.k && a.p > b.p)) ? a : b; }
+
+// box of the segment's elements ix[0..nav), by one workgroup; positions are
+// segment positions, the result is written to r[6] by thread 0
+__device__ void wg_box(const int *ix, int nav, const double *eb, double *r)
+{
+  __shared__ unsigned long long rk[6][BT / 64];
+  __shared__ int rp[6][BT / 64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  KP m[6];
+  for (int j = 0; j < 3; j++) {
+    m[j] = KP{~0ull, -1};
+    m[j + 3] = KP{0ull, -1};
+  }
+  for (int i = t; i < nav; i += BT) {
+    const double *b = eb + 6 * (size_t)ix[i];
+    for (int j = 0; j < 3; j++) {
+      m[j] = kp_min(m[j], KP{ckey(b[j]), i});
+      m[j + 3] = kp_max(m[j + 3], KP{ckey(b[j + 3]), i});
+    }
+  }
+  for (int j = 0; j < 6; j++) {
+#pragma unroll
+    for (int q = 32; q > 0; q >>= 1) {
+      KP o{__shfl_xor(m[j].k, q), __shfl_xor(m[j].p, q)};
+      m[j] = j < 3 ? kp_min(m[j], o) : kp_max(m[j], o);
+    }
+    if (lane == 0) {
+      rk[j][w] = m[j].k;
+      rp[j][w] = m[j].p;
+    }
+  }
+  __syncthreads();
+  if (t < 6) {
+    KP a{rk[t][0], rp[t][0]};
+    for (int q = 1; q < BT / 64; q++) a = t < 3 ? kp_min(a, KP{rk[t][q], rp[t][q]}) : kp_max(a, KP{rk[t][q], rp[t][q]});
+    r[t] = eb[6 * (size_t)ix[a.p] + t];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void write_node(const Seg &s, int el, const double *r, int *adtInt, double *adtReal)
+{
+  const size_t me = (size_t)s.base + s.me;
+  adtInt[4 * me] = el;
+  adtInt[4 * me + 1] = adtInt[4 * me + 2] = adtInt[4 * me + 3] = -1;
+  for (int j = 0; j < 6; j++) adtReal[6 * me + j] = r[j];
+  if (s.side > 0) adtInt[4 * ((size_t)s.base + s.parent) + s.side] = el;
+}
+
+// Medium segments: one workgroup each.
+__global__ void __launch_bounds__(BT) k_adt_level(const Seg *__restrict__ segs, int *ea, double *X, int *A, int *B,
+                                                  const double *__restrict__ ebox, int *__restrict__ adtInt,
+                                                  double *__restrict__ adtReal, int *err)
 {
   __shared__ int lds[2 * (BT / 64)];
   __shared__ WgState sh;
-  __shared__ double red[6][BT / 64];
+  __shared__ double r[6];
   const int t = threadIdx.x;
-  const Seg s = in[blockIdx.x];
-  Seg *o = out + 2 * (size_t)blockIdx.x;
-  if (s.nav <= SMALL) {
-    if (t == 0) {
-      if (s.nav > 0) small[atomicAdd(nsmall, 1)] = s;
-      o[0] = Seg{0, 0, 0, 0, 0, 0};
-      o[1] = Seg{0, 0, 0, 0, 0, 0};
-    }
-    return;
-  }
+  const Seg s = segs[blockIdx.x];
+  const size_t g = (size_t)s.base + s.start;
+  const double *eb = ebox + 6 * (size_t)s.base;
   const int dim = s.level % 6;
-  int *ix = ea + s.start;
-  double *x = X + s.start;
-  for (int i = t; i < s.nav; i += BT) x[i] = ebox[6 * (size_t)ix[i] + dim];
+  int *ix = ea + g;
+  double *x = X + g;
+  for (int i = t; i < s.nav; i += BT) x[i] = eb[6 * (size_t)ix[i] + dim];
   __threadfence_block();
   __syncthreads();
-  wg_median(ix, x, s.nav, A + s.start, B + s.start, sh, lds, nsmall + 1);
-  // the node's box bounds the whole segment (buildADTrecursion.c:72-88)
-  double lo[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, hi[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
-  for (int i = t; i < s.nav; i += BT) {
-    const double *b = ebox + 6 * (size_t)ix[i];
-    for (int j = 0; j < 3; j++) {
-      lo[j] = dmin(lo[j], b[j]);
-      hi[j] = dmax(hi[j], b[j + 3]);
-    }
-  }
-  for (int j = 0; j < 3; j++)
-#pragma unroll
-    for (int q = 32; q > 0; q >>= 1) {
-      lo[j] = dmin(lo[j], __shfl_xor(lo[j], q));
-      hi[j] = dmax(hi[j], __shfl_xor(hi[j], q));
-    }
-  if ((t & 63) == 0)
-    for (int j = 0; j < 3; j++) {
-      red[j][t >> 6] = lo[j];
-      red[j + 3][t >> 6] = hi[j];
-    }
-  __syncthreads();
-  if (t == 0) {
-    double r[6];
-    for (int j = 0; j < 6; j++) {
-      r[j] = red[j][0];
-      for (int q = 1; q < BT / 64; q++) r[j] = j < 3 ? dmin(r[j], red[j][q]) : dmax(r[j], red[j][q]);
-    }
-    const int nleft = (s.nav + 1) / 2, me = s.me;
-    const int el = ix[nleft - 1];
-    adtInt[4 * (size_t)me] = el;
-    adtInt[4 * (size_t)me + 1] = adtInt[4 * (size_t)me + 2] = adtInt[4 * (size_t)me + 3] = -1;
-    for (int j = 0; j < 6; j++) adtReal[6 * (size_t)me + j] = r[j];
-    if (s.side > 0) adtInt[4 * (size_t)s.parent + s.side] = el;
-    o[0] = Seg{s.start, nleft - 1, me + 1, me, 1, s.level + 1};
-    o[1] = Seg{s.start + nleft, s.nav - nleft, me + nleft, me, 2, s.level + 1};
-  }
+  wg_median(ix, x, s.nav, A + g, B + g, sh, lds, err);
+  wg_box(ix, s.nav, eb, r);
+  if (t == 0) write_node(s, ix[(s.nav + 1) / 2 - 1], r, adtInt, adtReal);
 }
 
 // Whole subtrees of short segments, one thread each, with the sequential
@@ -261,114 +300,565 @@ __global__ void k_adt_small(int n, const Seg *__restrict__ segs, int *ea, double
 {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  Seg stk[24];
+  Seg stk[12];  // depth of a subtree of at most SMALL elements, plus one
   int sp = 0;
   if (segs[k].nav > 0) stk[sp++] = segs[k];
   while (sp > 0) {
     Seg s = stk[--sp];
-    int *ix = ea + s.start;
-    const int me = s.me;
+    int *ix = ea + s.base + s.start;
+    const double *eb = ebox + 6 * (size_t)s.base;
     if (s.nav == 1) {
-      adtInt[4 * (size_t)me] = ix[0];
-      adtInt[4 * (size_t)me + 1] = adtInt[4 * (size_t)me + 2] = adtInt[4 * (size_t)me + 3] = -1;
-      for (int j = 0; j < 6; j++) adtReal[6 * (size_t)me + j] = ebox[6 * (size_t)ix[0] + j];
-      if (s.side > 0) adtInt[4 * (size_t)s.parent + s.side] = ix[0];
+      double r[6];
+      for (int j = 0; j < 6; j++) r[j] = eb[6 * (size_t)ix[0] + j];
+      write_node(s, ix[0], r, adtInt, adtReal);
       continue;
     }
-    double *x = X + s.start;
+    double *x = X + s.base + s.start;
     const int dim = s.level % 6;
-    for (int i = 0; i < s.nav; i++) x[i] = ebox[6 * (size_t)ix[i] + dim];
+    for (int i = 0; i < s.nav; i++) x[i] = eb[6 * (size_t)ix[i] + dim];
     median_select(ix, x, s.nav);
     const int nleft = (s.nav + 1) / 2;
-    adtInt[4 * (size_t)me] = ix[nleft - 1];
-    adtInt[4 * (size_t)me + 1] = adtInt[4 * (size_t)me + 2] = adtInt[4 * (size_t)me + 3] = -1;
     double r[6] = {BIGVALUE, BIGVALUE, BIGVALUE, -BIGVALUE, -BIGVALUE, -BIGVALUE};
     for (int i = 0; i < s.nav; i++) {
-      const double *b = ebox + 6 * (size_t)ix[i];
+      const double *b = eb + 6 * (size_t)ix[i];
       for (int j = 0; j < 3; j++) {
         r[j] = dmin(r[j], b[j]);
         r[j + 3] = dmax(r[j + 3], b[j + 3]);
       }
     }
-    for (int j = 0; j < 6; j++) adtReal[6 * (size_t)me + j] = r[j];
-    if (s.side > 0) adtInt[4 * (size_t)s.parent + s.side] = ix[nleft - 1];
-    if (s.nav - nleft > 0) stk[sp++] = Seg{s.start + nleft, s.nav - nleft, me + nleft, me, 2, s.level + 1};
-    if (nleft > 1) stk[sp++] = Seg{s.start, nleft - 1, me + 1, me, 1, s.level + 1};
+    write_node(s, ix[nleft - 1], r, adtInt, adtReal);
+    if (s.nav - nleft > 0)
+      stk[sp++] = Seg{s.base, s.start + nleft, s.nav - nleft, s.me + nleft, s.me, 2, s.level + 1, s.tree};
+    if (nleft > 1) stk[sp++] = Seg{s.base, s.start, nleft - 1, s.me + 1, s.me, 1, s.level + 1, s.tree};
   }
 }
 
-__global__ void k_iota(int n, int *p)
+// ---------------------------------------------------------------------------
+// long segments: partition rounds across workgroups.  A chunk table lists
+// (segment, chunk) pairs; positions are 1-based within the segment.
+
+__global__ void __launch_bounds__(BT) k_big_load(const int2 *__restrict__ tab, const Seg *__restrict__ segs,
+                                                 const int *__restrict__ ea, double *__restrict__ X,
+                                                 const double *__restrict__ ebox)
+{
+  const int2 tc = tab[blockIdx.x];
+  const Seg s = segs[tc.x];
+  const size_t g = (size_t)s.base + s.start;
+  const double *eb = ebox + 6 * (size_t)s.base;
+  const int dim = s.level % 6;
+  const int o0 = tc.y * CH, o1 = min(o0 + CH, s.nav);
+  for (int o = o0 + threadIdx.x; o < o1; o += BT) X[g + o] = eb[6 * (size_t)ea[g + o] + dim];
+}
+
+__global__ void k_big_init(int nbig, const Seg *__restrict__ segs, const double *__restrict__ X,
+                           BigState *__restrict__ st)
+{
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbig) return;
+  const Seg s = segs[b];
+  BigState S{};
+  S.lo = 1;
+  S.hi = s.nav;
+  S.st = 0;
+  S.rounds = 0;
+  S.pv = median_pivot(X + (size_t)s.base + s.start - 1, 1, s.nav);
+  st[b] = S;
+}
+
+// R1: per chunk, the number of A and B flags
+__global__ void __launch_bounds__(BT) k_big_count(const int2 *__restrict__ tab, const Seg *__restrict__ segs,
+                                                  const BigState *__restrict__ st, const int *__restrict__ choff,
+                                                  const double *__restrict__ X, int *__restrict__ cntA,
+                                                  int *__restrict__ cntB)
+{
+  __shared__ int lds[2 * (BT / 64)];
+  const int2 tc = tab[blockIdx.x];
+  const BigState S = st[tc.x];
+  const int k = choff[tc.x] + tc.y;
+  if (S.st != 0) {
+    if (threadIdx.x == 0) cntA[k] = cntB[k] = 0;
+    return;
+  }
+  const Seg s = segs[tc.x];
+  const double *X1 = X + (size_t)s.base + s.start - 1;
+  const int len = S.hi - S.lo + 1;
+  const int o0 = tc.y * CH, o1 = min(o0 + CH, len);
+  int ca = 0, cb = 0;
+  for (int o = o0 + threadIdx.x; o < o1; o += BT) {
+    ca += X1[S.lo + o] >= S.pv;
+    cb += X1[S.hi - o] <= S.pv;
+  }
+  int ea, eb, ta, tb;
+  wg_scan2(ca, cb, ea, eb, ta, tb, lds);
+  if (threadIdx.x == 0) {
+    cntA[k] = ta;
+    cntB[k] = tb;
+  }
+}
+
+// R2: exclusive scan of the chunk counts of each segment (one workgroup each)
+__global__ void __launch_bounds__(BT) k_big_scan(const int *__restrict__ choff, const int *__restrict__ nch,
+                                                 BigState *__restrict__ st, int *cntA, int *cntB)
+{
+  __shared__ int lds[2 * (BT / 64)];
+  const int b = blockIdx.x;
+  if (st[b].st != 0) return;
+  int *a = cntA + choff[b], *c = cntB + choff[b];
+  const int n = nch[b];
+  int runA = 0, runB = 0;
+  for (int base = 0; base < n; base += BT) {
+    const int i = base + threadIdx.x;
+    const int va = i < n ? a[i] : 0, vb = i < n ? c[i] : 0;
+    int ea, eb, ta, tb;
+    wg_scan2(va, vb, ea, eb, ta, tb, lds);
+    if (i < n) {
+      a[i] = runA + ea;
+      c[i] = runB + eb;
+    }
+    runA += ta;
+    runB += tb;
+  }
+  if (threadIdx.x == 0) {
+    st[b].nA = runA;
+    st[b].nB = runB;
+  }
+}
+
+// R3: the lists A (ascending) and B (descending), chunk by chunk
+__global__ void __launch_bounds__(BT) k_big_fill(const int2 *__restrict__ tab, const Seg *__restrict__ segs,
+                                                 const BigState *__restrict__ st, const int *__restrict__ choff,
+                                                 const double *__restrict__ X, const int *__restrict__ cntA,
+                                                 const int *__restrict__ cntB, int *__restrict__ A, int *__restrict__ B)
+{
+  __shared__ int lds[2 * (BT / 64)];
+  const int2 tc = tab[blockIdx.x];
+  const BigState S = st[tc.x];
+  if (S.st != 0) return;
+  const int k = choff[tc.x] + tc.y;
+  const Seg s = segs[tc.x];
+  const size_t g = (size_t)s.base + s.start;
+  const double *X1 = X + g - 1;
+  const int len = S.hi - S.lo + 1;
+  const int o0 = tc.y * CH;
+  if (o0 >= len) return;
+  // thread t owns PER consecutive offsets of the chunk
+  int ca = 0, cb = 0;
+  unsigned fa = 0, fb = 0;
+#pragma unroll
+  for (int e = 0; e < PER; e++) {
+    const int o = o0 + threadIdx.x * PER + e;
+    if (o < len) {
+      if (X1[S.lo + o] >= S.pv) {
+        fa |= 1u << e;
+        ca++;
+      }
+      if (X1[S.hi - o] <= S.pv) {
+        fb |= 1u << e;
+        cb++;
+      }
+    }
+  }
+  int ra, rb, ta, tb;
+  wg_scan2(ca, cb, ra, rb, ta, tb, lds);
+  ra += cntA[k];
+  rb += cntB[k];
+  int *Ag = A + g, *Bg = B + g;
+#pragma unroll
+  for (int e = 0; e < PER; e++) {
+    const int o = o0 + threadIdx.x * PER + e;
+    if (fa & (1u << e)) Ag[ra++] = S.lo + o;
+    if (fb & (1u << e)) Bg[rb++] = S.hi - o;
+  }
+}
+
+// first m in [0, mab) with A[m] >= B[m] (A rises, B falls), else mab
+__device__ __forceinline__ int big_K(const int *Ag, const int *Bg, int mab)
+{
+  int lo = 0, hi = mab;
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (Ag[m] < Bg[m]) lo = m + 1;
+    else hi = m;
+  }
+  return lo;
+}
+
+// R4: swap the K pairs (A[m], B[m]), m < K
+__global__ void __launch_bounds__(BT) k_big_swap(const int2 *__restrict__ tab, const Seg *__restrict__ segs,
+                                                 BigState *__restrict__ st, int *ea, double *X,
+                                                 const int *__restrict__ A, const int *__restrict__ B)
+{
+  const int2 tc = tab[blockIdx.x];
+  const BigState S = st[tc.x];
+  if (S.st != 0) return;
+  const Seg s = segs[tc.x];
+  const size_t g = (size_t)s.base + s.start;
+  const int *Ag = A + g, *Bg = B + g;
+  const int K = big_K(Ag, Bg, min(S.nA, S.nB));
+  if (tc.y == 0 && threadIdx.x == 0) st[tc.x].K = K;
+  double *X1 = X + g - 1;
+  int *I1 = ea + g - 1;
+  const int m1 = min((tc.y + 1) * CH, K);
+  for (int m = tc.y * CH + threadIdx.x; m < m1; m += BT) {
+    const int a = Ag[m], b = Bg[m];
+    const double tx = X1[a];
+    X1[a] = X1[b];
+    X1[b] = tx;
+    const int ti = I1[a];
+    I1[a] = I1[b];
+    I1[b] = ti;
+  }
+}
+
+// R5: loop control and the next pivot, one thread per segment
+__global__ void k_big_ctrl(int nbig, const Seg *__restrict__ segs, BigState *__restrict__ st, int *ea, double *X,
+                           const int *__restrict__ A, const int *__restrict__ B, int count, int *__restrict__ active,
+                           int *__restrict__ err)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbig) return;
+  BigState S = st[b];
+  if (S.st != 0) return;
+  const Seg s = segs[b];
+  const size_t g = (size_t)s.base + s.start;
+  const int *Ag = A + g, *Bg = B + g;
+  double *X1 = X + g - 1;
+  int *I1 = ea + g - 1;
+  const int K = S.K, nA = S.nA, nB = S.nB;
+  int i, j;
+  hoare_stop(K, nA, nB, K < nA ? Ag[K] : 0, K > 0 ? Ag[K - 1] : 0, K < nB ? Bg[K] : 0, K > 0 ? Bg[K - 1] : 0, Ag[0],
+             Bg[0], i, j);
+  int lo = S.lo, hi = S.hi;
+  S.st = median_control(s.nav, i, j, lo, hi);
+  S.lo = lo;
+  S.hi = hi;
+  auto swap = [&](int a, int c) {
+    const double tx = X1[a];
+    X1[a] = X1[c];
+    X1[c] = tx;
+    const int ti = I1[a];
+    I1[a] = I1[c];
+    I1[c] = ti;
+  };
+  if (S.st == 2) {  // median.F90's final fix-up
+    const int half = s.nav / 2, halfp1 = half + 1;
+    if (s.nav == 2 * half) {
+      if (X1[halfp1] < X1[half]) swap(half, halfp1);
+    } else if (X1[lo] > X1[hi]) {
+      swap(lo, hi);
+    }
+  } else if (S.st == 0) {
+    if (++S.rounds > s.nav + 2) {  // every round narrows [lo, hi]; never expected
+      atomicAdd(err, 1);
+      S.st = 3;
+    } else {
+      S.pv = median_pivot(X1, lo, hi);
+      if (count) atomicAdd(active, 1);
+    }
+  }
+  st[b] = S;
+}
+
+// node boxes of long segments: the minimal / maximal class per bound, then
+// the last position in that class
+__global__ void k_big_box_init(int nbig, unsigned long long *__restrict__ bk, int *__restrict__ bp)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 6 * nbig) return;
+  bk[i] = (i % 6) < 3 ? ~0ull : 0ull;
+  bp[i] = -1;
+}
+
+__global__ void __launch_bounds__(BT) k_big_box_key(const int2 *__restrict__ tab, const Seg *__restrict__ segs,
+                                                    const int *__restrict__ ea, const double *__restrict__ ebox,
+                                                    unsigned long long *__restrict__ bk)
+{
+  const int2 tc = tab[blockIdx.x];
+  const Seg s = segs[tc.x];
+  const size_t g = (size_t)s.base + s.start;
+  const double *eb = ebox + 6 * (size_t)s.base;
+  const int o0 = tc.y * CH, o1 = min(o0 + CH, s.nav);
+  unsigned long long m[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
+  for (int o = o0 + threadIdx.x; o < o1; o += BT) {
+    const double *b = eb + 6 * (size_t)ea[g + o];
+    for (int j = 0; j < 3; j++) {
+      m[j] = min(m[j], ckey(b[j]));
+      m[j + 3] = max(m[j + 3], ckey(b[j + 3]));
+    }
+  }
+  for (int j = 0; j < 6; j++) {
+#pragma unroll
+    for (int q = 32; q > 0; q >>= 1) {
+      const unsigned long long o = __shfl_xor(m[j], q);
+      m[j] = j < 3 ? min(m[j], o) : max(m[j], o);
+    }
+  }
+  if ((threadIdx.x & 63) == 0)
+    for (int j = 0; j < 6; j++) {
+      if (j < 3) atomicMin(&bk[6 * tc.x + j], m[j]);
+      else atomicMax(&bk[6 * tc.x + j], m[j]);
+    }
+}
+
+__global__ void __launch_bounds__(BT) k_big_box_pos(const int2 *__restrict__ tab, const Seg *__restrict__ segs,
+                                                    const int *__restrict__ ea, const double *__restrict__ ebox,
+                                                    const unsigned long long *__restrict__ bk, int *__restrict__ bp)
+{
+  const int2 tc = tab[blockIdx.x];
+  const Seg s = segs[tc.x];
+  const size_t g = (size_t)s.base + s.start;
+  const double *eb = ebox + 6 * (size_t)s.base;
+  unsigned long long key[6];
+  for (int j = 0; j < 6; j++) key[j] = bk[6 * tc.x + j];
+  const int o0 = tc.y * CH, o1 = min(o0 + CH, s.nav);
+  int p[6] = {-1, -1, -1, -1, -1, -1};
+  for (int o = o0 + threadIdx.x; o < o1; o += BT) {
+    const double *b = eb + 6 * (size_t)ea[g + o];
+    for (int j = 0; j < 6; j++)
+      if (ckey(b[j]) == key[j]) p[j] = o;
+  }
+  for (int j = 0; j < 6; j++) {
+#pragma unroll
+    for (int q = 32; q > 0; q >>= 1) p[j] = max(p[j], __shfl_xor(p[j], q));
+  }
+  if ((threadIdx.x & 63) == 0)
+    for (int j = 0; j < 6; j++)
+      if (p[j] >= 0) atomicMax(&bp[6 * tc.x + j], p[j]);
+}
+
+__global__ void k_big_node(int nbig, const Seg *__restrict__ segs, const int *__restrict__ ea,
+                           const double *__restrict__ ebox, const int *__restrict__ bp, int *__restrict__ adtInt,
+                           double *__restrict__ adtReal)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbig) return;
+  const Seg s = segs[b];
+  const size_t g = (size_t)s.base + s.start;
+  const double *eb = ebox + 6 * (size_t)s.base;
+  double r[6];
+  for (int j = 0; j < 6; j++) r[j] = eb[6 * (size_t)ea[g + bp[6 * b + j]] + j];
+  write_node(s, ea[g + (s.nav + 1) / 2 - 1], r, adtInt, adtReal);
+}
+
+__global__ void k_iota_trees(int n, const int *__restrict__ base, int ntree, int *__restrict__ p)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = i;
+  if (i >= n) return;
+  int lo = 0, hi = ntree - 1;  // last tree with base <= i
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (base[m] <= i) lo = m;
+    else hi = m - 1;
+  }
+  p[i] = i - base[lo];
 }
 
 // slot 3 of row e: the node that holds element e (ADT.C:120-124)
-__global__ void k_adt_slot3(int n, int *adtInt)
+__global__ void k_adt_slot3(int n, const int *__restrict__ base, int ntree, int *adtInt)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) adtInt[4 * (size_t)adtInt[4 * (size_t)i] + 3] = i;
+  if (i >= n) return;
+  int lo = 0, hi = ntree - 1;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (base[m] <= i) lo = m;
+    else hi = m - 1;
+  }
+  const int b = base[lo];
+  adtInt[4 * ((size_t)b + adtInt[4 * (size_t)i]) + 3] = i - b;
 }
 
 }  // namespace
 
-int build_adt_device(int n, const double *ebox, int *adtInt, double *adtReal, double ext[6], AdtScratch &w,
-                     hipStream_t s)
+void build_adt_batch(AdtTree *trees, int ntree, AdtScratch &w, hipStream_t s)
 {
-  int depth = 0;
-  for (int m = n; m > 0; m /= 2) depth++;
-  if (n <= 0) {
-    for (int i = 0; i < 3; i++) {  // ADT.C:66-86 over no elements
+  std::vector<int> base(ntree + 1, 0);
+  for (int t = 0; t < ntree; t++) {
+    AdtTree &T = trees[t];
+    T.depth = 0;
+    for (int m = T.n; m > 0; m /= 2) T.depth++;
+    if (T.n < 0) throw std::runtime_error("tioga_amd: negative ADT size");
+    base[t + 1] = base[t] + T.n;
+    if (base[t + 1] < base[t]) throw std::runtime_error("tioga_amd: ADT batch too large");
+  }
+  const int N = base[ntree];
+  auto empty_ext = [](double ext[6]) {  // ADT.C:66-86 over no elements
+    for (int i = 0; i < 3; i++) {
       double lo = BIGVALUE, hi = -BIGVALUE, d = 0.01 * (hi - lo);
       ext[2 * i] = lo - d;
       ext[2 * i + 1] = hi + d;
     }
-    return 0;
+  };
+  if (N == 0) {
+    for (int t = 0; t < ntree; t++) empty_ext(trees[t].ext);
+    return;
   }
-  w.ea.reserve(n);
-  w.X.reserve(n);
-  w.A.reserve(n);
-  w.B.reserve(n);
-  int nlev = 0;
-  for (int m = n; m > SMALL; m /= 2) nlev++;
-  const size_t last = (size_t)1 << nlev;
-  w.segA.reserve(4 * last);
-  w.segB.reserve(4 * last);
-  w.small.reserve(4 * last + 2);
-  w.nsmall.reserve(2);
-  k_iota<<<nblocks(n, 256), 256, 0, s>>>(n, w.ea.p);
-  Seg root{0, n, 0, 0, 0, 0};
-  TG_HIP(hipMemcpyAsync(w.segA.p, &root, sizeof(Seg), hipMemcpyHostToDevice, s));
-  TG_HIP(hipMemsetAsync(w.nsmall.p, 0, 2 * sizeof(int), s));
-  Seg *in = (Seg *)w.segA.p, *out = (Seg *)w.segB.p;
-  size_t count = 1;
-  for (int l = 0; l < nlev; l++) {
-    k_adt_level<<<(unsigned)count, BT, 0, s>>>(in, out, (Seg *)w.small.p, w.nsmall.p, w.ea.p, w.X.p, w.A.p, w.B.p,
-                                              ebox, adtInt, adtReal);
-    std::swap(in, out);
-    count *= 2;
+  // one tree builds in its own arrays; several are concatenated
+  const bool single = (ntree == 1);
+  const double *ebox;
+  int *adtInt;
+  double *adtReal;
+  if (single) {
+    ebox = trees[0].ebox;
+    adtInt = trees[0].adtInt;
+    adtReal = trees[0].adtReal;
+  } else {
+    w.ebox.reserve(6 * (size_t)N);
+    w.adtInt.reserve(4 * (size_t)N);
+    w.adtReal.reserve(6 * (size_t)N);
+    for (int t = 0; t < ntree; t++)
+      if (trees[t].n)
+        TG_HIP(hipMemcpyAsync(w.ebox.p + 6 * (size_t)base[t], trees[t].ebox, sizeof(double) * 6 * trees[t].n,
+                              hipMemcpyDeviceToDevice, s));
+    ebox = w.ebox.p;
+    adtInt = w.adtInt.p;
+    adtReal = w.adtReal.p;
   }
-  int cnt[2] = {0, 0};
-  TG_HIP(hipMemcpyAsync(cnt, w.nsmall.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  // every segment of every level, from the sizes alone
+  std::vector<std::vector<Seg>> lev;
+  std::vector<Seg> small, cur, nxt;
+  for (int t = 0; t < ntree; t++)
+    if (trees[t].n > 0) cur.push_back(Seg{base[t], 0, trees[t].n, 0, 0, 0, 0, t});
+  while (!cur.empty()) {
+    std::vector<Seg> here;
+    nxt.clear();
+    for (const Seg &q : cur) {
+      if (q.nav <= SMALL) {
+        small.push_back(q);
+        continue;
+      }
+      here.push_back(q);
+      const int nleft = (q.nav + 1) / 2;
+      if (nleft > 1) nxt.push_back(Seg{q.base, q.start, nleft - 1, q.me + 1, q.me, 1, q.level + 1, q.tree});
+      if (q.nav - nleft > 0) nxt.push_back(Seg{q.base, q.start + nleft, q.nav - nleft, q.me + nleft, q.me, 2, q.level + 1, q.tree});
+    }
+    if (!here.empty()) lev.push_back(here);
+    cur.swap(nxt);
+  }
+  w.ea.reserve(N);
+  w.X.reserve(N);
+  w.A.reserve(N);
+  w.B.reserve(N);
+  w.err.reserve(2);
+  w.tb.reserve(ntree + 1);
+  TG_HIP(hipMemcpyAsync(w.tb.p, base.data(), sizeof(int) * (ntree + 1), hipMemcpyHostToDevice, s));
+  TG_HIP(hipMemsetAsync(w.err.p, 0, 2 * sizeof(int), s));
+  k_iota_trees<<<nblocks(N, 256), 256, 0, s>>>(N, w.tb.p, ntree, w.ea.p);
+  // upload the level lists: per level the medium segments, then the long ones
+  std::vector<Seg> all;
+  std::vector<size_t> med0, nmed, big0, nbigv;
+  for (auto &L : lev) {
+    med0.push_back(all.size());
+    for (const Seg &q : L)
+      if (q.nav <= BIGMIN) all.push_back(q);
+    nmed.push_back(all.size() - med0.back());
+    big0.push_back(all.size());
+    for (const Seg &q : L)
+      if (q.nav > BIGMIN) all.push_back(q);
+    nbigv.push_back(all.size() - big0.back());
+  }
+  const size_t nsmall0 = all.size();
+  all.insert(all.end(), small.begin(), small.end());
+  w.segs.reserve(2 * std::max<size_t>(all.size(), 1));
+  TG_HIP(hipMemcpyAsync(w.segs.p, all.data(), sizeof(Seg) * all.size(), hipMemcpyHostToDevice, s));
+  const Seg *segs = (const Seg *)w.segs.p;
+  for (size_t l = 0; l < lev.size(); l++) {
+    if (nmed[l])
+      k_adt_level<<<(unsigned)nmed[l], BT, 0, s>>>(segs + med0[l], w.ea.p, w.X.p, w.A.p, w.B.p, ebox, adtInt, adtReal,
+                                                  w.err.p);
+    const int nbig = (int)nbigv[l];
+    if (!nbig) continue;
+    const Seg *bs = segs + big0[l];
+    // chunk table of the long segments
+    std::vector<int2> tab;
+    std::vector<int> choff(nbig), nch(nbig);
+    for (int b = 0; b < nbig; b++) {
+      const Seg &q = all[big0[l] + b];
+      choff[b] = (int)tab.size();
+      nch[b] = (q.nav + CH - 1) / CH;
+      for (int c = 0; c < nch[b]; c++) tab.push_back(int2{b, c});
+    }
+    const int ntab = (int)tab.size();
+    w.tab.reserve(ntab);
+    w.choff.reserve(2 * nbig);
+    w.cntA.reserve(ntab);
+    w.cntB.reserve(ntab);
+    w.state.reserve(2 * (size_t)nbig);
+    w.bk.reserve(6 * (size_t)nbig);
+    w.bp.reserve(6 * (size_t)nbig);
+    TG_HIP(hipMemcpyAsync(w.tab.p, tab.data(), sizeof(int2) * ntab, hipMemcpyHostToDevice, s));
+    TG_HIP(hipMemcpyAsync(w.choff.p, choff.data(), sizeof(int) * nbig, hipMemcpyHostToDevice, s));
+    TG_HIP(hipMemcpyAsync(w.choff.p + nbig, nch.data(), sizeof(int) * nbig, hipMemcpyHostToDevice, s));
+    BigState *st = (BigState *)w.state.p;
+    k_big_load<<<ntab, BT, 0, s>>>(w.tab.p, bs, w.ea.p, w.X.p, ebox);
+    k_big_init<<<nblocks(nbig, 64), 64, 0, s>>>(nbig, bs, w.X.p, st);
+    for (;;) {
+      TG_HIP(hipMemsetAsync(w.err.p + 1, 0, sizeof(int), s));
+      for (int r = 0; r < ROUNDS; r++) {
+        k_big_count<<<ntab, BT, 0, s>>>(w.tab.p, bs, st, w.choff.p, w.X.p, w.cntA.p, w.cntB.p);
+        k_big_scan<<<nbig, BT, 0, s>>>(w.choff.p, w.choff.p + nbig, st, w.cntA.p, w.cntB.p);
+        k_big_fill<<<ntab, BT, 0, s>>>(w.tab.p, bs, st, w.choff.p, w.X.p, w.cntA.p, w.cntB.p, w.A.p, w.B.p);
+        k_big_swap<<<ntab, BT, 0, s>>>(w.tab.p, bs, st, w.ea.p, w.X.p, w.A.p, w.B.p);
+        k_big_ctrl<<<nblocks(nbig, 64), 64, 0, s>>>(nbig, bs, st, w.ea.p, w.X.p, w.A.p, w.B.p, r == ROUNDS - 1,
+                                                    w.err.p + 1, w.err.p);
+      }
+      int act = 0;
+      TG_HIP(hipMemcpyAsync(&act, w.err.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+      TG_HIP(hipStreamSynchronize(s));
+      if (act == 0) break;
+    }
+    k_big_box_init<<<nblocks(6 * nbig, 64), 64, 0, s>>>(nbig, w.bk.p, w.bp.p);
+    k_big_box_key<<<ntab, BT, 0, s>>>(w.tab.p, bs, w.ea.p, ebox, w.bk.p);
+    k_big_box_pos<<<ntab, BT, 0, s>>>(w.tab.p, bs, w.ea.p, ebox, w.bk.p, w.bp.p);
+    k_big_node<<<nblocks(nbig, 64), 64, 0, s>>>(nbig, bs, w.ea.p, ebox, w.bp.p, adtInt, adtReal);
+  }
+  if (!small.empty())
+    k_adt_small<<<nblocks((long long)small.size(), 64), 64, 0, s>>>((int)small.size(), segs + nsmall0, w.ea.p, w.X.p,
+                                                                   ebox, adtInt, adtReal);
+  k_adt_slot3<<<nblocks(N, 256), 256, 0, s>>>(N, w.tb.p, ntree, adtInt);
+  int err = 0;
+  TG_HIP(hipMemcpyAsync(&err, w.err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (!single)
+    for (int t = 0; t < ntree; t++)
+      if (trees[t].n) {
+        TG_HIP(hipMemcpyAsync(trees[t].adtInt, adtInt + 4 * (size_t)base[t], sizeof(int) * 4 * trees[t].n,
+                              hipMemcpyDeviceToDevice, s));
+        TG_HIP(hipMemcpyAsync(trees[t].adtReal, adtReal + 6 * (size_t)base[t], sizeof(double) * 6 * trees[t].n,
+                              hipMemcpyDeviceToDevice, s));
+      }
+  std::vector<double> roots(6 * (size_t)ntree);
+  for (int t = 0; t < ntree; t++)
+    if (trees[t].n)
+      TG_HIP(hipMemcpyAsync(roots.data() + 6 * t, trees[t].adtReal, sizeof(double) * 6, hipMemcpyDeviceToHost, s));
   TG_HIP(hipStreamSynchronize(s));
-  if (cnt[1]) throw std::runtime_error("tioga_amd: ADT build: partition loop exceeded its bound");
-  const int nsmall = cnt[0];
-  if (nsmall > 0)
-    k_adt_small<<<nblocks(nsmall, 64), 64, 0, s>>>(nsmall, (const Seg *)w.small.p, w.ea.p, w.X.p, ebox, adtInt,
-                                                 adtReal);
-  k_adt_small<<<nblocks((long long)count, 64), 64, 0, s>>>((int)count, in, w.ea.p, w.X.p, ebox, adtInt, adtReal);
-  k_adt_slot3<<<nblocks(n, 256), 256, 0, s>>>(n, adtInt);
-  double root_box[6];
-  TG_HIP(hipMemcpyAsync(root_box, adtReal, sizeof root_box, hipMemcpyDeviceToHost, s));
-  TG_HIP(hipStreamSynchronize(s));
-  for (int i = 0; i < 3; i++) {  // root bounds padded by 1 % (ADT.C:66-86)
-    ext[2 * i] = root_box[i];
-    ext[2 * i + 1] = root_box[i + 3];
-    double d = 0.01 * (ext[2 * i + 1] - ext[2 * i]);
-    ext[2 * i] -= d;
-    ext[2 * i + 1] += d;
+  if (err) throw std::runtime_error("tioga_amd: ADT build: partition loop exceeded its bound");
+  for (int t = 0; t < ntree; t++) {
+    double *ext = trees[t].ext;
+    if (trees[t].n == 0) {
+      empty_ext(ext);
+      continue;
+    }
+    for (int i = 0; i < 3; i++) {  // root bounds padded by 1 % (ADT.C:66-86)
+      ext[2 * i] = roots[6 * t + i];
+      ext[2 * i + 1] = roots[6 * t + i + 3];
+      double d = 0.01 * (ext[2 * i + 1] - ext[2 * i]);
+      ext[2 * i] -= d;
+      ext[2 * i + 1] += d;
+    }
   }
-  return depth;
+}
+
+int build_adt_device(int n, const double *ebox, int *adtInt, double *adtReal, double ext[6], AdtScratch &w,
+                     hipStream_t s)
+{
+  AdtTree t;
+  t.n = n;
+  t.ebox = ebox;
+  t.adtInt = adtInt;
+  t.adtReal = adtReal;
+  build_adt_batch(&t, 1, w, s);
+  for (int i = 0; i < 6; i++) ext[i] = t.ext[i];
+  return t.depth;
 }
 
 }  // namespace tg

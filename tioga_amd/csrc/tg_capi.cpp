@@ -481,6 +481,49 @@ int tioga_amd_host_build_adt(int nelem, const double *ebox, int *adtInt, double 
   return tg::build_adt(nelem, ebox, adtInt, adtReal, ext);
 }
 
+int tioga_amd_device_build_adts(int ntree, const int *n, const double *ebox, int *adtInt, double *adtReal,
+                                double *ext)
+{
+  int rc = -1;
+  guarded("tioga_amd_device_build_adts", [&] {
+    if (ntree <= 0) throw std::runtime_error("no trees");
+    hipStream_t s;
+    TG_HIP(hipStreamCreate(&s));
+    tg::AdtScratch w;
+    std::vector<tg::AdtTree> trees(ntree);
+    std::vector<tg::DBuf<double>> eb(ntree), ar(ntree);
+    std::vector<tg::DBuf<int>> ai(ntree);
+    size_t off = 0;
+    for (int t = 0; t < ntree; t++) {
+      if (n[t] < 0) throw std::runtime_error("negative tree size");
+      trees[t].n = n[t];
+      eb[t].reserve(6 * (size_t)n[t]);
+      ai[t].reserve(4 * (size_t)n[t]);
+      ar[t].reserve(6 * (size_t)n[t]);
+      if (n[t])
+        TG_HIP(hipMemcpyAsync(eb[t].p, ebox + 6 * off, sizeof(double) * 6 * n[t], hipMemcpyHostToDevice, s));
+      trees[t].ebox = eb[t].p;
+      trees[t].adtInt = ai[t].p;
+      trees[t].adtReal = ar[t].p;
+      off += n[t];
+    }
+    tg::build_adt_batch(trees.data(), ntree, w, s);
+    off = 0;
+    for (int t = 0; t < ntree; t++) {
+      if (n[t]) {
+        TG_HIP(hipMemcpyAsync(adtInt + 4 * off, ai[t].p, sizeof(int) * 4 * n[t], hipMemcpyDeviceToHost, s));
+        TG_HIP(hipMemcpyAsync(adtReal + 6 * off, ar[t].p, sizeof(double) * 6 * n[t], hipMemcpyDeviceToHost, s));
+      }
+      for (int i = 0; i < 6; i++) ext[6 * t + i] = trees[t].ext[i];
+      off += n[t];
+    }
+    TG_HIP(hipStreamSynchronize(s));
+    TG_HIP(hipStreamDestroy(s));
+    rc = 0;
+  });
+  return rc;
+}
+
 void tioga_amd_host_unique_nodes(const double *x, const int *meshtag, double *rtag, int *xtag, int n)
 {
   tg::unique_nodes_octree(x, meshtag, rtag, xtag, n);

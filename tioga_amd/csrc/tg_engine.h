@@ -153,14 +153,32 @@ struct QuerySet {
 
 // Reusable device scratch of the ADT build (tg_adt.hip).
 struct AdtScratch {
-  DBuf<int> ea, A, B, nsmall;
-  DBuf<double> X;
-  DBuf<int4> segA, segB, small;  // segment descriptors (6 ints each)
+  DBuf<int> ea, A, B, err, tb, choff, cntA, cntB, adtInt;
+  DBuf<double> X, ebox, adtReal;
+  DBuf<int4> segs;  // segment descriptors (8 ints each)
+  DBuf<int2> tab;
+  DBuf<double4> state;
+  DBuf<unsigned long long> bk;
+  DBuf<int> bp;
 };
 
-// Builds the ADT over n element boxes (device, 6 doubles each) exactly as
-// ADT::buildADT does: adtInt[4n] and adtReal[6n] on the device, ext[6] (the
-// padded root extents) on the host.  Returns the tree depth in levels.
+// One ADT of a batch: n element boxes (device, 6 doubles each) in; the
+// reference's adtInt[4n] / adtReal[6n] (device) and the padded root extents
+// ext[6] and depth (host) out.
+struct AdtTree {
+  int n = 0;
+  const double *ebox = nullptr;
+  int *adtInt = nullptr;
+  double *adtReal = nullptr;
+  double ext[6] = {0, 0, 0, 0, 0, 0};
+  int depth = 0;
+};
+
+// Builds every tree of the batch exactly as ADT::buildADT does, level by
+// level for all trees together.
+void build_adt_batch(AdtTree *trees, int ntree, AdtScratch &w, hipStream_t s);
+
+// One tree; returns its depth in levels.
 int build_adt_device(int n, const double *ebox, int *adtInt, double *adtReal, double ext[6], AdtScratch &w,
                      hipStream_t s);
 
