@@ -189,6 +189,12 @@ int tioga_amd_host_build_adt(int nelem, const double *ebox, int *adtInt, double 
 int tioga_amd_device_build_adts(int ntree, const int *n, const double *ebox, int *adtInt, double *adtReal,
                                 double *ext);
 void tioga_amd_host_unique_nodes(const double *x, const int *meshtag, double *rtag, int *xtag, int n);
+/* The device coincident-point dedupe on nset point sets at once (set k: n[k]
+   points; arrays concatenated in set order, host memory; rtag raised in
+   place, xtag = representative within the set); returns 0, or -1 with the
+   error status set.  Needs a GPU. */
+int tioga_amd_device_unique_nodes(int nset, const int *n, const double *x, const int *meshtag, double *rtag,
+                                  int *xtag);
 
 /* Donor weights outside [-0.2, 1.2] seen (the reference aborts on one). */
 int tioga_amd_weight_aborts(void);

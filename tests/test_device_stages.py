@@ -1,11 +1,14 @@
 """Device stages against the oracle at sizes the golden cases do not reach.
 
 The ADT builder splits segments longer than 65536 elements across many
-workgroups and builds several trees in one level loop; these tests run it
+workgroups and builds several trees in one level loop; the test runs it
 through the C ABI on trees of mixed sizes (long, medium, short, single,
 empty) whose boxes have heavy ties along every cut dimension and signed
-zeros, and compare every tree node for node with the oracle's restatement
-of ADT.C / buildADTrecursion.c / median.F90."""
+zeros, and compares every tree node for node with the oracle's restatement
+of ADT.C / buildADTrecursion.c / median.F90.  The coincident-point dedupe
+likewise splits octree nodes of more than 65536 points across workgroups
+and takes several point sets at once; its representatives and raised
+resolutions are compared with the oracle's uniquenodes_octree."""
 import ctypes as C
 
 import numpy as np
@@ -60,4 +63,51 @@ def test_device_adt_batch_matches_oracle():
                 f"tree {t} adtReals"
             if n:
                 assert np.array_equal(ext[6 * k:6 * k + 6], we), f"tree {t} extents"
+            off += n
+
+
+def points(n, rng, ndup=0.3):
+    """Points on a coarse grid (ties at the octree's mid planes) with groups
+    of up to 4 coincident copies, some of them under another mesh tag."""
+    m = max(1, int(n * (1 - ndup)))
+    base = np.round(rng.uniform(-1, 1, (m, 3)) * 256) / 256
+    pick = np.concatenate([np.arange(m), rng.integers(0, m, n - m)])
+    rng.shuffle(pick)
+    x = np.ascontiguousarray(base[pick])
+    tag = rng.integers(1, 3, n).astype(np.int32)
+    rtag = rng.uniform(0, 1, n)
+    return x, tag, rtag
+
+
+def test_device_dedupe_batch_matches_oracle():
+    from tioga_amd.tioga import load
+
+    L = load()
+    L.tioga_amd_device_unique_nodes.argtypes = [C.c_int] + [C.c_void_p] * 5
+    L.tioga_amd_device_unique_nodes.restype = C.c_int
+    O = oracle_lib()
+    rng = np.random.default_rng(11)
+    sizes = [400000, 70000, 25, 5, 0, 21, 120000]
+    sets = [points(n, rng) if n else (np.zeros((0, 3)), np.zeros(0, np.int32), np.zeros(0)) for n in sizes]
+    # one set of 500 identical points: a node that cannot be split
+    sets.append((np.ascontiguousarray(np.full((500, 3), 0.25)), np.ones(500, np.int32), rng.uniform(0, 1, 500)))
+    sizes.append(500)
+    for batch in ([0], list(range(len(sizes)))):
+        ns = np.array([sizes[k] for k in batch], np.int32)
+        x = np.ascontiguousarray(np.concatenate([sets[k][0] for k in batch]).reshape(-1))
+        tag = np.ascontiguousarray(np.concatenate([sets[k][1] for k in batch]))
+        rtag = np.ascontiguousarray(np.concatenate([sets[k][2] for k in batch]))
+        xtag = np.zeros(max(int(ns.sum()), 1), np.int32)
+        print(f"[device dedupe] sets {ns.tolist()}", flush=True)
+        assert L.tioga_amd_device_unique_nodes(len(batch), ptr(ns), ptr(x), ptr(tag), ptr(rtag), ptr(xtag)) == 0
+        off = 0
+        for k in batch:
+            n = sizes[k]
+            xs, ts, rs = sets[k]
+            wr = rs.copy()
+            wi = np.zeros(max(n, 1), np.int32)
+            if n:
+                O.orc_unique_nodes_octree(ptr(np.ascontiguousarray(xs)), ptr(ts), ptr(wr), ptr(wi), n)
+            assert np.array_equal(xtag[off:off + n], wi[:n]), f"set {k} xtag"
+            assert np.array_equal(rtag[off:off + n], wr[:n]), f"set {k} rtag"
             off += n

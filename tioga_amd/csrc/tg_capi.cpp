@@ -524,6 +524,49 @@ int tioga_amd_device_build_adts(int ntree, const int *n, const double *ebox, int
   return rc;
 }
 
+int tioga_amd_device_unique_nodes(int nset, const int *n, const double *x, const int *meshtag, double *rtag,
+                                  int *xtag)
+{
+  int rc = -1;
+  guarded("tioga_amd_device_unique_nodes", [&] {
+    if (nset <= 0) throw std::runtime_error("no point sets");
+    hipStream_t s;
+    TG_HIP(hipStreamCreate(&s));
+    tg::DedupeScratch w;
+    std::vector<tg::DedupeSet> sets(nset);
+    std::vector<tg::DBuf<double>> dx(nset), dr(nset);
+    std::vector<tg::DBuf<int>> dt(nset), di(nset);
+    size_t off = 0;
+    for (int k = 0; k < nset; k++) {
+      if (n[k] < 0) throw std::runtime_error("negative point count");
+      dx[k].reserve(3 * (size_t)n[k]);
+      dr[k].reserve(n[k]);
+      dt[k].reserve(n[k]);
+      di[k].reserve(n[k]);
+      if (n[k]) {
+        TG_HIP(hipMemcpyAsync(dx[k].p, x + 3 * off, sizeof(double) * 3 * n[k], hipMemcpyHostToDevice, s));
+        TG_HIP(hipMemcpyAsync(dr[k].p, rtag + off, sizeof(double) * n[k], hipMemcpyHostToDevice, s));
+        TG_HIP(hipMemcpyAsync(dt[k].p, meshtag + off, sizeof(int) * n[k], hipMemcpyHostToDevice, s));
+      }
+      sets[k] = tg::DedupeSet{n[k], dx[k].p, dt[k].p, dr[k].p, di[k].p};
+      off += n[k];
+    }
+    tg::unique_nodes_batch(sets.data(), nset, w, s);
+    off = 0;
+    for (int k = 0; k < nset; k++) {
+      if (n[k]) {
+        TG_HIP(hipMemcpyAsync(rtag + off, dr[k].p, sizeof(double) * n[k], hipMemcpyDeviceToHost, s));
+        TG_HIP(hipMemcpyAsync(xtag + off, di[k].p, sizeof(int) * n[k], hipMemcpyDeviceToHost, s));
+      }
+      off += n[k];
+    }
+    TG_HIP(hipStreamSynchronize(s));
+    TG_HIP(hipStreamDestroy(s));
+    rc = 0;
+  });
+  return rc;
+}
+
 void tioga_amd_host_unique_nodes(const double *x, const int *meshtag, double *rtag, int *xtag, int n)
 {
   tg::unique_nodes_octree(x, meshtag, rtag, xtag, n);

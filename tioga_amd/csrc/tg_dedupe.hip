@@ -13,8 +13,16 @@
 //
 // Here each level of the octree is one launch with a workgroup per node:
 // bounding box, octant counts, then a stable reversed scatter of the node's
-// points into its children.  Leaves are queued and finished one per thread.
+// points into its children.  Nodes of more than BIG points are split across
+// many workgroups instead (chunked box, counts, per-octant scans, scatter).
+// Several point sets (the query sets of several blocks) can be deduplicated
+// together, one root each.  Leaves are queued and finished one per thread.
+//
+// The bounding box enters only through mid = (max+min)/2 and
+// dx = (max-min)/2 + TOL, where -0.0 and +0.0 give the same octants, so the
+// box may be reduced in any order.
 #include <stdexcept>
+#include <vector>
 
 #include "tg_engine.h"
 
@@ -24,6 +32,8 @@ namespace {
 
 constexpr int BT = 256;
 constexpr int LEAF = 20;  // utils.c:633: split above 20 points
+constexpr int BIG = 1 << 16;  // nodes above this: many workgroups
+constexpr int CHB = 16 * BT;  // points per chunk of a big node
 
 struct Node {
   int start, len;
@@ -160,49 +170,316 @@ __global__ void k_oct_leaf(int n, const Node *__restrict__ leaves, const int *__
   }
 }
 
-__global__ void k_iota2(int n, int *a, int *b)
+__global__ void k_iota_sets(int n, const int *__restrict__ base, int nset, int *__restrict__ ea,
+                            int *__restrict__ itag)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = b[i] = i;
+  if (i >= n) return;
+  int lo = 0, hi = nset - 1;  // last set with base <= i
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (base[m] <= i) lo = m;
+    else hi = m - 1;
+  }
+  ea[i] = i;
+  itag[i] = i - base[lo];
+}
+
+// ---- nodes of more than BIG points: chunk table entries (node, chunk) ----
+
+struct BigBox {
+  double mid[3], dx[3];
+  int cnt[8], cft[9], one, pad[2];
+};
+
+__global__ void k_ob_init(int nbig, unsigned long long *__restrict__ bk)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 6 * nbig) bk[i] = (i % 6) < 3 ? ~0ull : 0ull;
+}
+
+__global__ void __launch_bounds__(BT) k_ob_box(const int2 *__restrict__ tab, const Node *__restrict__ nodes,
+                                               const int *__restrict__ ea, const double *__restrict__ x,
+                                               unsigned long long *__restrict__ bk)
+{
+  const int2 tc = tab[blockIdx.x];
+  const Node nd = nodes[tc.x];
+  const int o0 = tc.y * CHB, o1 = min(o0 + CHB, nd.len);
+  unsigned long long m[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
+  for (int o = o0 + threadIdx.x; o < o1; o += BT) {
+    const double *p = x + 3 * (size_t)ea[nd.start + o];
+    for (int j = 0; j < 3; j++) {
+      const unsigned long long k = dkey(p[j] + 0.0);
+      m[j] = min(m[j], k);
+      m[j + 3] = max(m[j + 3], k);
+    }
+  }
+  for (int j = 0; j < 6; j++) {
+#pragma unroll
+    for (int q = 32; q > 0; q >>= 1) {
+      const unsigned long long o = __shfl_xor(m[j], q);
+      m[j] = j < 3 ? min(m[j], o) : max(m[j], o);
+    }
+  }
+  if ((threadIdx.x & 63) == 0)
+    for (int j = 0; j < 6; j++) {
+      if (j < 3) atomicMin(&bk[6 * tc.x + j], m[j]);
+      else atomicMax(&bk[6 * tc.x + j], m[j]);
+    }
+}
+
+__global__ void k_ob_mid(int nbig, const unsigned long long *__restrict__ bk, BigBox *__restrict__ bb)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbig) return;
+  for (int j = 0; j < 3; j++) {  // utils.c:646-649
+    const double lo = dunkey(bk[6 * b + j]), hi = dunkey(bk[6 * b + j + 3]);
+    bb[b].mid[j] = (hi + lo) * 0.5;
+    bb[b].dx[j] = (hi - lo) * 0.5 + TOL;
+  }
+}
+
+__global__ void __launch_bounds__(BT) k_ob_count(const int2 *__restrict__ tab, const Node *__restrict__ nodes,
+                                                 const int *__restrict__ choff, const int *__restrict__ ea,
+                                                 const double *__restrict__ x, const BigBox *__restrict__ bb,
+                                                 int *__restrict__ cc)
+{
+  __shared__ int c[8];
+  const int2 tc = tab[blockIdx.x];
+  const Node nd = nodes[tc.x];
+  if (threadIdx.x < 8) c[threadIdx.x] = 0;
+  __syncthreads();
+  const int o0 = tc.y * CHB, o1 = min(o0 + CHB, nd.len);
+  int my[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int o = o0 + threadIdx.x; o < o1; o += BT) my[octant(x + 3 * (size_t)ea[nd.start + o], bb[tc.x].mid, bb[tc.x].dx)]++;
+  for (int q = 0; q < 8; q++)
+    if (my[q]) atomicAdd(&c[q], my[q]);
+  __syncthreads();
+  if (threadIdx.x < 8) cc[8 * (choff[tc.x] + tc.y) + threadIdx.x] = c[threadIdx.x];
+}
+
+// per big node: per-octant exclusive scan over its chunks, octant totals,
+// and whether one octant holds every point
+__global__ void k_ob_scan(int nbig, const Node *__restrict__ nodes, const int *__restrict__ choff,
+                          const int *__restrict__ nch, int *__restrict__ cc, BigBox *__restrict__ bb)
+{
+  const int b = blockIdx.x;
+  const int q = threadIdx.x;  // octant
+  if (q < 8) {
+    int run = 0;
+    int *c = cc + 8 * (size_t)choff[b];
+    for (int k = 0; k < nch[b]; k++) {
+      const int v = c[8 * k + q];
+      c[8 * k + q] = run;
+      run += v;
+    }
+    bb[b].cnt[q] = run;
+  }
+  __syncthreads();
+  if (q == 0) {
+    int one = 0;
+    bb[b].cft[0] = 0;
+    for (int o = 0; o < 8; o++) {
+      bb[b].cft[o + 1] = bb[b].cft[o] + bb[b].cnt[o];
+      one |= (bb[b].cnt[o] == nodes[b].len);
+    }
+    bb[b].one = one;
+  }
+}
+
+// stable scatter, each octant in reverse arrival order (utils.c:666-679)
+__global__ void __launch_bounds__(BT) k_ob_scatter(const int2 *__restrict__ tab, const Node *__restrict__ nodes,
+                                                   const int *__restrict__ choff, const int *__restrict__ ea,
+                                                   int *__restrict__ tmp, const double *__restrict__ x,
+                                                   const BigBox *__restrict__ bb, const int *__restrict__ cc)
+{
+  __shared__ int wcnt[BT / 64][8];
+  const int2 tc = tab[blockIdx.x];
+  const Node nd = nodes[tc.x];
+  const BigBox &B = bb[tc.x];
+  if (B.one) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = BT / 64;
+  int run[8];
+  for (int q = 0; q < 8; q++) run[q] = cc[8 * (choff[tc.x] + tc.y) + q];
+  const unsigned long long lt = (1ull << lane) - 1;
+  const int o0 = tc.y * CHB, o1 = min(o0 + CHB, nd.len);
+  for (int base = o0; base < o1; base += BT) {
+    const int i = base + t;
+    int o = -1;
+    if (i < o1) o = octant(x + 3 * (size_t)ea[nd.start + i], B.mid, B.dx);
+    int rank_in_wave = 0;
+    for (int q = 0; q < 8; q++) {
+      unsigned long long m = __ballot(o == q);
+      if (lane == 0) wcnt[w][q] = __popcll(m);
+      if (o == q) rank_in_wave = __popcll(m & lt);
+    }
+    __syncthreads();
+    if (o >= 0) {
+      int r = run[o] + rank_in_wave;
+      for (int q = 0; q < w; q++) r += wcnt[q][o];
+      tmp[nd.start + B.cft[o] + (B.cnt[o] - 1 - r)] = ea[nd.start + i];
+    }
+    for (int q = 0; q < 8; q++)
+      for (int v = 0; v < nw; v++) run[q] += wcnt[v][q];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(BT) k_ob_copy(const int2 *__restrict__ tab, const Node *__restrict__ nodes,
+                                                int *__restrict__ ea, const int *__restrict__ tmp,
+                                                const BigBox *__restrict__ bb)
+{
+  const int2 tc = tab[blockIdx.x];
+  const Node nd = nodes[tc.x];
+  if (bb[tc.x].one) return;
+  const int o0 = tc.y * CHB, o1 = min(o0 + CHB, nd.len);
+  for (int o = o0 + threadIdx.x; o < o1; o += BT) ea[nd.start + o] = tmp[nd.start + o];
+}
+
+__global__ void k_ob_emit(int nbig, const Node *__restrict__ nodes, const BigBox *__restrict__ bb,
+                          Node *__restrict__ nout, int *__restrict__ cnt, Node *__restrict__ bout,
+                          Node *__restrict__ leaves)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbig) return;
+  const Node nd = nodes[b];
+  const BigBox &B = bb[b];
+  if (B.one) {  // every point in one octant: the node is compared as a whole
+    leaves[atomicAdd(&cnt[1], 1)] = nd;
+    return;
+  }
+  for (int q = 0; q < 8; q++) {
+    if (B.cnt[q] == 0) continue;
+    const Node c{nd.start + B.cft[q], B.cnt[q]};
+    if (c.len > BIG) bout[atomicAdd(&cnt[2], 1)] = c;
+    else if (c.len > LEAF) nout[atomicAdd(&cnt[0], 1)] = c;
+    else leaves[atomicAdd(&cnt[1], 1)] = c;
+  }
 }
 
 }  // namespace
 
-void unique_nodes_device(int n, const double *x, const int *tag, double *rtag, int *itag, DedupeScratch &w,
-                         hipStream_t s)
+void unique_nodes_batch(DedupeSet *sets, int nset, DedupeScratch &w, hipStream_t s)
 {
-  if (n <= 0) return;
+  std::vector<int> base(nset + 1, 0);
+  for (int k = 0; k < nset; k++) {
+    if (sets[k].n < 0) throw std::runtime_error("tioga_amd: negative point count");
+    base[k + 1] = base[k] + sets[k].n;
+    if (base[k + 1] < base[k]) throw std::runtime_error("tioga_amd: dedupe batch too large");
+  }
+  const int n = base[nset];
+  if (n == 0) return;
+  const bool single = (nset == 1);
+  const double *x;
+  const int *tag;
+  double *rtag;
+  int *itag;
+  if (single) {
+    x = sets[0].x;
+    tag = sets[0].tag;
+    rtag = sets[0].rtag;
+    itag = sets[0].itag;
+  } else {
+    w.x.reserve(3 * (size_t)n);
+    w.tag.reserve(n);
+    w.rtag.reserve(n);
+    w.itag.reserve(n);
+    for (int k = 0; k < nset; k++)
+      if (sets[k].n) {
+        TG_HIP(hipMemcpyAsync(w.x.p + 3 * (size_t)base[k], sets[k].x, sizeof(double) * 3 * sets[k].n,
+                              hipMemcpyDeviceToDevice, s));
+        TG_HIP(hipMemcpyAsync(w.tag.p + base[k], sets[k].tag, sizeof(int) * sets[k].n, hipMemcpyDeviceToDevice, s));
+        TG_HIP(hipMemcpyAsync(w.rtag.p + base[k], sets[k].rtag, sizeof(double) * sets[k].n,
+                              hipMemcpyDeviceToDevice, s));
+      }
+    x = w.x.p;
+    tag = w.tag.p;
+    rtag = w.rtag.p;
+    itag = w.itag.p;
+  }
   w.ea.reserve(n);
   w.tmp.reserve(n);
-  // node lists: each level's nodes partition the points still being split
-  w.nodesA.reserve(n / LEAF + 2);
-  w.nodesB.reserve(n / LEAF + 2);
+  w.nodesA.reserve(n / LEAF + nset + 2);
+  w.nodesB.reserve(n / LEAF + nset + 2);
+  w.bigA.reserve(n / BIG + nset + 2);
+  w.bigB.reserve(n / BIG + nset + 2);
   w.leaves.reserve(n + 1);
-  w.counts.reserve(2);
-  k_iota2<<<nblocks(n, 256), 256, 0, s>>>(n, w.ea.p, itag);
-  int cnt[2] = {0, 0};  // [0] next-level nodes, [1] leaves
-  Node root{0, n};
-  if (n > LEAF) {
-    TG_HIP(hipMemcpyAsync(w.nodesA.p, &root, sizeof root, hipMemcpyHostToDevice, s));
-    TG_HIP(hipMemsetAsync(w.counts.p, 0, 2 * sizeof(int), s));
-    Node *in = (Node *)w.nodesA.p, *out = (Node *)w.nodesB.p;
-    int nin = 1;
-    while (nin > 0) {
-      TG_HIP(hipMemsetAsync(w.counts.p, 0, sizeof(int), s));
-      k_oct_level<<<nin, BT, 0, s>>>(nin, in, out, w.counts.p, (Node *)w.leaves.p, w.counts.p + 1, w.ea.p, w.tmp.p,
-                                     x);
-      TG_HIP(hipMemcpyAsync(cnt, w.counts.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  w.counts.reserve(4);
+  w.sb.reserve(nset + 1);
+  TG_HIP(hipMemcpyAsync(w.sb.p, base.data(), sizeof(int) * (nset + 1), hipMemcpyHostToDevice, s));
+  k_iota_sets<<<nblocks(n, 256), 256, 0, s>>>(n, w.sb.p, nset, w.ea.p, itag);
+  // roots
+  std::vector<Node> rn, rb, rl;
+  for (int k = 0; k < nset; k++) {
+    const Node r{base[k], sets[k].n};
+    if (r.len > BIG) rb.push_back(r);
+    else if (r.len > LEAF) rn.push_back(r);
+    else if (r.len > 0) rl.push_back(r);
+  }
+  if (!rn.empty())
+    TG_HIP(hipMemcpyAsync(w.nodesA.p, rn.data(), sizeof(Node) * rn.size(), hipMemcpyHostToDevice, s));
+  if (!rb.empty()) TG_HIP(hipMemcpyAsync(w.bigA.p, rb.data(), sizeof(Node) * rb.size(), hipMemcpyHostToDevice, s));
+  if (!rl.empty()) TG_HIP(hipMemcpyAsync(w.leaves.p, rl.data(), sizeof(Node) * rl.size(), hipMemcpyHostToDevice, s));
+  int cnt[3] = {(int)rn.size(), (int)rl.size(), (int)rb.size()};  // normal, leaves, big
+  Node *nin = (Node *)w.nodesA.p, *nout = (Node *)w.nodesB.p;
+  Node *bin = (Node *)w.bigA.p, *bout = (Node *)w.bigB.p;
+  while (cnt[0] + cnt[2] > 0) {
+    const int nnorm = cnt[0], nbig = cnt[2];
+    const int hc[3] = {0, cnt[1], 0};
+    TG_HIP(hipMemcpyAsync(w.counts.p, hc, sizeof hc, hipMemcpyHostToDevice, s));
+    if (nnorm)
+      k_oct_level<<<nnorm, BT, 0, s>>>(nnorm, nin, nout, w.counts.p, (Node *)w.leaves.p, w.counts.p + 1, w.ea.p,
+                                       w.tmp.p, x);
+    if (nbig) {
+      std::vector<Node> bn(nbig);
+      TG_HIP(hipMemcpyAsync(bn.data(), bin, sizeof(Node) * nbig, hipMemcpyDeviceToHost, s));
       TG_HIP(hipStreamSynchronize(s));
-      nin = cnt[0];
-      std::swap(in, out);
+      std::vector<int2> tab;
+      std::vector<int> choff(nbig), nch(nbig);
+      for (int b = 0; b < nbig; b++) {
+        choff[b] = (int)tab.size();
+        nch[b] = (bn[b].len + CHB - 1) / CHB;
+        for (int c = 0; c < nch[b]; c++) tab.push_back(int2{b, c});
+      }
+      const int ntab = (int)tab.size();
+      w.tab.reserve(ntab);
+      w.choff.reserve(2 * nbig);
+      w.cc.reserve(8 * (size_t)ntab);
+      w.bk.reserve(6 * (size_t)nbig);
+      w.bb.reserve((sizeof(BigBox) * nbig + sizeof(double4) - 1) / sizeof(double4));
+      TG_HIP(hipMemcpyAsync(w.tab.p, tab.data(), sizeof(int2) * ntab, hipMemcpyHostToDevice, s));
+      TG_HIP(hipMemcpyAsync(w.choff.p, choff.data(), sizeof(int) * nbig, hipMemcpyHostToDevice, s));
+      TG_HIP(hipMemcpyAsync(w.choff.p + nbig, nch.data(), sizeof(int) * nbig, hipMemcpyHostToDevice, s));
+      BigBox *bb = (BigBox *)w.bb.p;
+      k_ob_init<<<nblocks(6 * nbig, 64), 64, 0, s>>>(nbig, w.bk.p);
+      k_ob_box<<<ntab, BT, 0, s>>>(w.tab.p, bin, w.ea.p, x, w.bk.p);
+      k_ob_mid<<<nblocks(nbig, 64), 64, 0, s>>>(nbig, w.bk.p, bb);
+      k_ob_count<<<ntab, BT, 0, s>>>(w.tab.p, bin, w.choff.p, w.ea.p, x, bb, w.cc.p);
+      k_ob_scan<<<nbig, 64, 0, s>>>(nbig, bin, w.choff.p, w.choff.p + nbig, w.cc.p, bb);
+      k_ob_scatter<<<ntab, BT, 0, s>>>(w.tab.p, bin, w.choff.p, w.ea.p, w.tmp.p, x, bb, w.cc.p);
+      k_ob_copy<<<ntab, BT, 0, s>>>(w.tab.p, bin, w.ea.p, w.tmp.p, bb);
+      k_ob_emit<<<nblocks(nbig, 64), 64, 0, s>>>(nbig, bin, bb, nout, w.counts.p, bout, (Node *)w.leaves.p);
     }
-  } else {
-    TG_HIP(hipMemcpyAsync(w.leaves.p, &root, sizeof root, hipMemcpyHostToDevice, s));
+    TG_HIP(hipMemcpyAsync(cnt, w.counts.p, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     TG_HIP(hipStreamSynchronize(s));
-    cnt[1] = 1;
+    std::swap(nin, nout);
+    std::swap(bin, bout);
   }
   if (cnt[1] > 0)
     k_oct_leaf<<<nblocks(cnt[1], 64), 64, 0, s>>>(cnt[1], (const Node *)w.leaves.p, w.ea.p, x, tag, rtag, itag);
+  if (!single)
+    for (int k = 0; k < nset; k++)
+      if (sets[k].n) {
+        TG_HIP(hipMemcpyAsync(sets[k].rtag, rtag + base[k], sizeof(double) * sets[k].n, hipMemcpyDeviceToDevice, s));
+        TG_HIP(hipMemcpyAsync(sets[k].itag, itag + base[k], sizeof(int) * sets[k].n, hipMemcpyDeviceToDevice, s));
+      }
+}
+
+void unique_nodes_device(int n, const double *x, const int *tag, double *rtag, int *itag, DedupeScratch &w,
+                         hipStream_t s)
+{
+  DedupeSet d{n, x, tag, rtag, itag};
+  unique_nodes_batch(&d, 1, w, s);
 }
 
 }  // namespace tg

@@ -184,12 +184,27 @@ int build_adt_device(int n, const double *ebox, int *adtInt, double *adtReal, do
 
 // Reusable device scratch of the coincident-point dedupe (tg_dedupe.hip).
 struct DedupeScratch {
-  DBuf<int> ea, tmp, counts;
-  DBuf<int2> nodesA, nodesB, leaves;
+  DBuf<int> ea, tmp, counts, sb, choff, cc, tag, itag;
+  DBuf<int2> nodesA, nodesB, bigA, bigB, leaves, tab;
+  DBuf<double> x, rtag;
+  DBuf<unsigned long long> bk;
+  DBuf<double4> bb;
 };
 
-// uniquenodes_octree (utils.c:620-737) on the device: itag[n] and rtag[n]
-// (device) receive the reference's representative and raised resolutions.
+// One point set of a dedupe batch (device arrays): n points x[3n] with mesh
+// tags; rtag[n] is raised in place, itag[n] receives the representatives.
+struct DedupeSet {
+  int n;
+  const double *x;
+  const int *tag;
+  double *rtag;
+  int *itag;
+};
+
+// uniquenodes_octree (utils.c:620-737) on the device for every set of the
+// batch (one octree root each): the reference's representatives and raised
+// resolutions.
+void unique_nodes_batch(DedupeSet *sets, int nset, DedupeScratch &w, hipStream_t s);
 void unique_nodes_device(int n, const double *x, const int *tag, double *rtag, int *itag, DedupeScratch &w,
                          hipStream_t s);
 
