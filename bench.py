@@ -191,26 +191,43 @@ def batching_bench(args) -> int:
     t0 = time.time()
     tg.preprocess()
     log(f"preprocess {time.time() - t0:.2f} s")
-    for _ in range(args.warmup):
-        tg.connect()
-    torch.cuda.synchronize()
-    times, phases = [], []
-    for _ in range(args.steps):
-        t0 = time.time()
-        tg.connect()
-        times.append(time.time() - t0)
-        phases.append(tg.phase_times())
-        log(f"connectivity {1000 * times[-1]:.1f} ms")
-    nsearch = phases[-1]["nsearch"]
-    ms = 1000 * float(np.median(times))
-    out = {"metric": "donor searches/s of performconnectivity (64 shells + background)",
+    def run(batched: bool):
+        tg.set_batching(batched)
+        for _ in range(args.warmup):
+            tg.connect()
+        torch.cuda.synchronize()
+        times, phases = [], []
+        for _ in range(args.steps):
+            t0 = time.time()
+            tg.connect()
+            times.append(time.time() - t0)
+            phases.append(tg.phase_times())
+            log(f"connectivity ({'batched' if batched else 'block by block'}) {1000 * times[-1]:.1f} ms")
+        ib = [a.copy() for a in tg.iblank]
+        rec = [tg.donor_info(b.tag) for b in blocks]
+        return times, phases, ib, rec
+
+    t_plain, ph_plain, ib_plain, rec_plain = run(False)
+    t_b, ph_b, ib_b, rec_b = run(True)
+    same = all(np.array_equal(a, b) for a, b in zip(ib_plain, ib_b)) and all(
+        all(np.array_equal(x, y) for x, y in zip(r1, r2)) for r1, r2 in zip(rec_plain, rec_b))
+    nsearch = ph_b[-1]["nsearch"]
+    ms = 1000 * float(np.median(t_b))
+    ms_plain = 1000 * float(np.median(t_plain))
+    out = {"metric": "donor searches/s of performconnectivity (64 shells + background), batched form",
            "value": round(nsearch / (ms / 1000), 1), "unit": "searches/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "config3: 64 hex/prism shells (1e4..1e6 cells, log-uniform) + 192^3 background",
                       "blocks": len(blocks), "searches_per_step": int(nsearch)},
-           "p50_ms": round(ms, 3), "p99_ms": round(1000 * float(np.percentile(times, 99)), 3),
-           "phases_ms": {k: round(float(np.median([p[k] for p in phases])), 3) for k in phases[-1]}}
+           "p50_ms": round(ms, 3), "p99_ms": round(1000 * float(np.percentile(t_b, 99)), 3),
+           "phases_ms": {k: round(float(np.median([p[k] for p in ph_b])), 3) for k in ph_b[-1]},
+           "per_block_form": {"ms_per_step": round(ms_plain, 3),
+                              "value": round(nsearch / (ms_plain / 1000), 1),
+                              "phases_ms": {k: round(float(np.median([p[k] for p in ph_plain])), 3)
+                                            for k in ph_plain[-1]}},
+           "speedup_vs_per_block": round(ms_plain / ms, 3),
+           "identical_iblank_and_donors": bool(same)}
     tg.close()
     print(json.dumps(out), flush=True)
     return 0

@@ -136,6 +136,12 @@ double tioga_amd_time_search(int nrep, long long *nsearch);
    interpolation records per pass. */
 double tioga_amd_time_dataupdate(int nvar, int interptype, int nrep, long long *nreceptors);
 
+/* Batched form (on != 0, the default): the stages of performconnectivity
+   run for all blocks together (one ADT build, one coincident-point dedupe
+   for every block's query set, ...); 0 runs them block by block.  The
+   results are identical either way. */
+void tioga_amd_set_batching(int on);
+
 /* Device time (ms per update) of nrep dataUpdate_AMR gathers with the
    patch and block solutions resident in HBM; *nrecords = interpolation
    records per update. */

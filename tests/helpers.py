@@ -68,14 +68,15 @@ def compare_to_golden(name: str, got: dict, skip=()):
     return bad
 
 
-def run_engine_case(case):
+def run_engine_case(case, batched: bool = True):
     """Every array ref_driver.cpp dumps, computed by the HIP engine through
-    its C ABI, in the driver's call sequence."""
+    its C ABI, in the driver's call sequence (batched or block-by-block form)."""
     from tioga_amd.tioga import Tioga
 
     t = Tioga()
     out = {}
     try:
+        t.set_batching(batched)
         t.set_params(case.isym, case.nfringe, case.mexclude)
         for b in case.blocks:
             t.register_grid(b)

@@ -389,6 +389,11 @@ double tioga_amd_time_dataupdate(int nvar, int interptype, int nrep, long long *
   return r;
 }
 
+void tioga_amd_set_batching(int on)
+{
+  guarded("tioga_amd_set_batching", [&] { tg_or_throw().engine()->batched = (on != 0); });
+}
+
 double tioga_amd_time_dataupdate_amr(int nvar, int interptype, int nrep, long long *nrecords)
 {
   double r = 0;

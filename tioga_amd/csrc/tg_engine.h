@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "tg_common.h"
+#include "tg_prims.h"
 
 namespace tg {
 
@@ -112,6 +113,7 @@ struct Block {
   DBuf<double> cellRes, nodeRes;
   DBuf<unsigned> nodebin;  // 10^3 bin of each node in the block's OBB frame
   DBuf<int> invmap;        // nodes ordered by (bin, descending node id)
+  std::vector<int> binoff; // first position of each bin in that order (1001)
   OBB obb{};
   double mapdx[3] = {0, 0, 0};
 
@@ -247,6 +249,10 @@ class Engine {
 
   // tioga.h:106-107 defaults
   int isym = 3, nfringe = 1, mexclude = 3;
+  // Batched form: the stages of performConnectivity run for all blocks
+  // together (one ADT build, one dedupe, ... for every block); false runs
+  // them block by block.  Results are identical either way.
+  bool batched = true;
 
   void register_grid(int btag, int nnodes, double *xyz, int *ibl, int nwbc, int nobc, int *wbc, int *obc,
                      int ntypes, const int *nv, const int *nc, int **vconn);
@@ -314,9 +320,14 @@ class Engine {
   void exchange_boxes();
   void exchange_search_data();
   void search(Block &b);
+  void search_batched();
+  void dedupe_gid(Block &b);
+  double *pinned_ = nullptr;  // page-locked staging of the query points
+  size_t pinned_cap_ = 0;
   void exchange_donors();
   void cell_iblanks(Block &b);
   void launch_search(Block &b);
+
   void build_search_structure(Block &b, const double *xq_host, int nq, int &nelem, int &depth, DBuf<int> &elist,
                               DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, DBuf<AdtNode> &nodes,
                               double ext[6]);
@@ -351,6 +362,7 @@ class Engine {
   DedupeScratch dedw_;
   double prep_acc_[4] = {0, 0, 0, 0};
   std::vector<DBuf<double> *> qdev_;
+  DBuf<int> qlist_;  // query positions of every pair (exchange_search_data)
   // AMR Cartesian background (tg_amr.hip)
   AmrState *amr_ = nullptr;
   AmrState &amr();

@@ -10,7 +10,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <thread>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -284,6 +286,174 @@ __global__ void k_hole_final(size_t n, unsigned char *__restrict__ sam)
   if (i < n) sam[i] = (sam[i] == 0 || sam[i] == 2) ? 1 : 0;
 }
 
+// Flood fill of one 2-D slice in LDS.  With isym 3 (2) [1] the stencil has
+// no z (y) [x] neighbours, so every slice normal to that axis fills on its
+// own; the interior slices are filled here, the two outer ones belong to
+// the boundary shell.  The workgroup loads the slice, repeats row and
+// column sweeps (the k_line_fill rule) until nothing changes, and stores it.
+constexpr int SLICE_MAX = 48 * 1024;
+
+__global__ void __launch_bounds__(256) k_fill_slices(int axis, int nmaps, unsigned char *const *__restrict__ sams,
+                                                     const int *__restrict__ nxs)
+{
+  __shared__ unsigned char sl[SLICE_MAX];
+  __shared__ int chg;
+  const int m = blockIdx.y;
+  const int n[3] = {nxs[3 * m], nxs[3 * m + 1], nxs[3 * m + 2]};
+  const int k = blockIdx.x + 1;  // interior slice index along axis
+  if (k >= n[axis] - 1) return;
+  unsigned char *sam = sams[m];
+  const size_t st[3] = {1, (size_t)n[0], (size_t)n[0] * n[1]};
+  const int ua = axis == 0 ? 1 : 0, va = axis == 2 ? 1 : 2;  // in-slice axes, u fastest in memory
+  const int nu = n[ua], nv = n[va];
+  const int t = threadIdx.x;
+  for (int i = t; i < nu * nv; i += blockDim.x) {
+    const int u = i % nu, v = i / nu;
+    sl[i] = sam[k * st[axis] + u * st[ua] + v * st[va]];
+  }
+  __syncthreads();
+  for (;;) {
+    if (t == 0) chg = 0;
+    __syncthreads();
+    int ch = 0;
+    // rows: along u, interior rows v = 1..nv-2, cross neighbours v +- 1
+    for (int v = 1 + t; v < nv - 1; v += blockDim.x) {
+      unsigned char prev = sl[v * nu];
+      for (int u = 1; u < nu - 1; u++) {
+        const int i = v * nu + u;
+        unsigned char c = sl[i];
+        if (c == 0 && (prev == 1 || sl[i - nu] == 1 || sl[i + nu] == 1)) {
+          c = 1;
+          sl[i] = 1;
+          ch = 1;
+        }
+        prev = c;
+      }
+      prev = sl[v * nu + nu - 1];
+      for (int u = nu - 2; u >= 1; u--) {
+        const int i = v * nu + u;
+        unsigned char c = sl[i];
+        if (c == 0 && (prev == 1 || sl[i - nu] == 1 || sl[i + nu] == 1)) {
+          c = 1;
+          sl[i] = 1;
+          ch = 1;
+        }
+        prev = c;
+      }
+    }
+    __syncthreads();
+    // columns: along v, interior columns u = 1..nu-2, cross neighbours u +- 1
+    for (int u = 1 + t; u < nu - 1; u += blockDim.x) {
+      unsigned char prev = sl[u];
+      for (int v = 1; v < nv - 1; v++) {
+        const int i = v * nu + u;
+        unsigned char c = sl[i];
+        if (c == 0 && (prev == 1 || sl[i - 1] == 1 || sl[i + 1] == 1)) {
+          c = 1;
+          sl[i] = 1;
+          ch = 1;
+        }
+        prev = c;
+      }
+      prev = sl[(nv - 1) * nu + u];
+      for (int v = nv - 2; v >= 1; v--) {
+        const int i = v * nu + u;
+        unsigned char c = sl[i];
+        if (c == 0 && (prev == 1 || sl[i - 1] == 1 || sl[i + 1] == 1)) {
+          c = 1;
+          sl[i] = 1;
+          ch = 1;
+        }
+        prev = c;
+      }
+    }
+    if (ch) atomicOr(&chg, 1);
+    __syncthreads();
+    const int any = chg;
+    __syncthreads();
+    if (!any) break;
+  }
+  for (int i = t; i < nu * nv; i += blockDim.x) {
+    const int u = i % nu, v = i / nu;
+    sam[k * st[axis] + u * st[ua] + v * st[va]] = sl[i];
+  }
+}
+
+// All hole maps of a connectivity at once: the same steps, one launch each,
+// a two-dimensional grid of (cells or interior lines of one map, map).
+struct MapSet {
+  int nmap;
+  unsigned char *sam[128];
+  int nx[128][3];
+};
+
+// step 0: clear; step 1: boundary shell to 1; step 2: final values
+__global__ void k_maps_cells(const MapSet *__restrict__ M, int step)
+{
+  const int m = blockIdx.y;
+  const int nx = M->nx[m][0], ny = M->nx[m][1], nz = M->nx[m][2];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nx * ny * nz) return;
+  unsigned char *sam = M->sam[m];
+  if (step == 0) {
+    sam[k] = 0;
+  } else if (step == 1) {
+    const int a = k % nx, b = (k / nx) % ny, c = k / (nx * ny);
+    if (a == 0 || a == nx - 1 || b == 0 || b == ny - 1 || c == 0 || c == nz - 1) sam[k] = 1;
+  } else {
+    sam[k] = (sam[k] == 0 || sam[k] == 2) ? 1 : 0;
+  }
+}
+
+// k_line_fill over the interior lines along `axis` of every active map
+__global__ void k_maps_line_fill(const MapSet *__restrict__ M, int axis, int allow, const int *__restrict__ active,
+                                 int *__restrict__ changed)
+{
+  const int mp = blockIdx.y;
+  if (!active[mp]) return;
+  const int n[3] = {M->nx[mp][0], M->nx[mp][1], M->nx[mp][2]};
+  const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+  const int m1 = n[a1] - 2, m2 = n[a2] - 2;
+  if (m1 <= 0 || m2 <= 0 || n[axis] < 3) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m1 * m2) return;
+  unsigned char *sam = M->sam[mp];
+  const int st[3] = {1, n[0], n[0] * n[1]};
+  const int i1 = 1 + t % m1, i2 = 1 + t / m1;
+  const int base = i1 * st[a1] + i2 * st[a2];
+  const int sa = st[axis];
+  const bool c1 = (allow >> a1) & 1, c2 = (allow >> a2) & 1;
+  int ch = 0;
+  auto paintable = [&](int m) -> bool {
+    if (c1 && (sam[m - st[a1]] == 1 || sam[m + st[a1]] == 1)) return true;
+    if (c2 && (sam[m - st[a2]] == 1 || sam[m + st[a2]] == 1)) return true;
+    return false;
+  };
+  unsigned char prev = sam[base];
+  for (int i = 1; i < n[axis] - 1; i++) {
+    const int m = base + i * sa;
+    unsigned char v = sam[m];
+    if (v == 0 && (prev == 1 || paintable(m))) {
+      v = 1;
+      sam[m] = 1;
+      ch = 1;
+    }
+    prev = v;
+  }
+  prev = sam[base + (n[axis] - 1) * sa];
+  for (int i = n[axis] - 2; i >= 1; i--) {
+    const int m = base + i * sa;
+    unsigned char v = sam[m];
+    if (v == 0 && (prev == 1 || paintable(m))) {
+      v = 1;
+      sam[m] = 1;
+      ch = 1;
+    }
+    prev = v;
+  }
+  if (ch) changed[mp] = 1;
+}
+
 // checkHoleMap (utils.c:170-185): truncation toward zero, range test on the int
 __device__ __forceinline__ int check_hole(const double p[3], const HoleMapView &h)
 {
@@ -358,27 +528,170 @@ __global__ void k_reduced_obb(int ncells, TypeInfo ti, const int *__restrict__ c
   }
 }
 
+// The same for every pair of one local block in one pass over its cells:
+// each cell's vertices are loaded once and tested against every partner
+// OBB; a wave with a kept cell for a pair reduces and adds its box.
+struct BoxPairs {
+  int np;
+  int pair[128];
+  OBB obc[128];
+};
+
+__global__ void __launch_bounds__(BS) k_reduced_obb_multi(int ncells, TypeInfo ti, const int *__restrict__ conn,
+                                                          const double *__restrict__ x, OBB own,
+                                                          const BoxPairs *__restrict__ P,
+                                                          unsigned long long *__restrict__ out)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = c < ncells;
+  int nv = 0;
+  const int *cv = conn + 8 * (size_t)(valid ? c : 0);
+  double lo[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, hi[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
+  double cen[3] = {0, 0, 0}, rad = 0;
+  if (valid) {
+    nv = ti.nv[type_of(ti, c)];
+    for (int m = 0; m < nv; m++) {
+      double p[3], xd[3];
+      load_xyz(x, cv[m], p);
+      to_frame(p, own.xc, own.vec, xd);
+      for (int j = 0; j < 3; j++) {
+        lo[j] = dmin(lo[j], xd[j]);
+        hi[j] = dmax(hi[j], xd[j]);
+        cen[j] += p[j];
+      }
+    }
+    for (int j = 0; j < 3; j++) cen[j] /= nv;
+    for (int m = 0; m < nv; m++) {
+      double p[3];
+      load_xyz(x, cv[m], p);
+      const double d = sqrt((p[0] - cen[0]) * (p[0] - cen[0]) + (p[1] - cen[1]) * (p[1] - cen[1]) +
+                            (p[2] - cen[2]) * (p[2] - cen[2]));
+      rad = dmax(rad, d);
+    }
+    // a margin far above the rounding of the frame transforms
+    rad = rad * (1 + 1e-9) + 1e-12 * (1 + fabs(cen[0]) + fabs(cen[1]) + fabs(cen[2]));
+  }
+  const int np = P->np;
+  for (int k = 0; k < np; k++) {
+    const OBB &obc = P->obc[k];
+    bool keep = false;
+    if (valid) {
+      // the cell's bounding sphere misses the partner box: the vertex test
+      // below can not keep it either
+      double cd[3];
+      to_frame(cen, obc.xc, obc.vec, cd);
+      bool near = true;
+      for (int j = 0; j < 3; j++) near = near && (fabs(cd[j]) <= obc.dxc[j] + rad * (1 + 1e-9) + 1e-12);
+      if (near) {
+        double bb[6] = {BIGVALUE, BIGVALUE, BIGVALUE, -BIGVALUE, -BIGVALUE, -BIGVALUE};
+        for (int m = 0; m < nv; m++) {
+          double p[3], xd[3];
+          load_xyz(x, cv[m], p);
+          to_frame(p, obc.xc, obc.vec, xd);
+          for (int j = 0; j < 3; j++) bb[j] = dmin(bb[j], xd[j]);
+          for (int j = 0; j < 3; j++) bb[j + 3] = dmax(bb[j + 3], xd[j]);
+        }
+        keep = true;
+        for (int j = 0; j < 3; j++)
+          if (bb[j] > obc.dxc[j]) keep = false;
+        for (int j = 0; j < 3; j++)
+          if (bb[j + 3] < -obc.dxc[j]) keep = false;
+      }
+    }
+    if (__ballot(keep) == 0) continue;
+    for (int j = 0; j < 3; j++) {
+      unsigned long long kl = keep ? dkey(lo[j]) : dkey(BIGVALUE), kh = keep ? dkey(hi[j]) : dkey(-BIGVALUE);
+      for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long a = __shfl_xor(kl, o), b = __shfl_xor(kh, o);
+        kl = a < kl ? a : kl;
+        kh = b > kh ? b : kh;
+      }
+      if ((threadIdx.x & 63) == 0) {
+        atomicMin(&out[6 * (size_t)P->pair[k] + j], kl);
+        atomicMax(&out[6 * (size_t)P->pair[k] + 3 + j], kh);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // query gathering: MeshBlock::getQueryPoints2 (MeshBlock.C:953-1117)
 // ---------------------------------------------------------------------------
 
-// Position p of the block's bin order emits its node when the node's bin
-// passes the bin-level test and the node lies in the partner's reduced OBB.
-__global__ void k_query_flag(int nnodes, const int *__restrict__ invmap, const unsigned *__restrict__ nodebin,
-                             const unsigned char *__restrict__ binmask, const double *__restrict__ x, OBB obc,
-                             int *__restrict__ flag)
+// binoff[k] = first position of bin k in the (bin, node) order, written at
+// the positions where the bin changes (bins with no node keep V and are
+// fixed up on the host)
+__global__ void k_bin_starts(int n, const unsigned *__restrict__ sorted_bin, int *__restrict__ binoff)
 {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nnodes) return;
-  int v = invmap[p];
-  int f = 0;
-  if (binmask[nodebin[v]]) {
-    double q[3], xd[3];
-    load_xyz(x, v, q);
-    to_frame(q, obc.xc, obc.vec, xd);
-    f = (fabs(xd[0]) <= obc.dxc[0] && fabs(xd[1]) <= obc.dxc[1] && fabs(xd[2]) <= obc.dxc[2]);
+  if (p >= n) return;
+  const unsigned k = sorted_bin[p];
+  if (p == 0 || sorted_bin[p - 1] != k) binoff[k] = p;
+}
+
+// Candidate c of a pair's bin ranges (positions of the sender's bin order,
+// bins in (l, k, j) order): flag when the node lies in the partner's reduced
+// OBB (MeshBlock.C:1064-1081).  cand0[i] is pair i's first candidate; rng
+// holds per range (first candidate, first position), per pair in bin order.
+struct QueryPairs {
+  const int *cand0;      // npairs + 1
+  const int2 *rng;       // ranges of all pairs
+  const int *rng0;       // npairs + 1: first range of each pair
+  const OBB *obc;        // npairs: the partner's reduced OBB
+  const int *blk;        // npairs: the sending block
+  const int *invmap[128];
+  const double *x[128];
+};
+
+__global__ void k_query_cand(int ncand, int npairs, const QueryPairs *__restrict__ Q, int *__restrict__ flag,
+                             int *__restrict__ pos)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncand) return;
+  int lo = 0, hi = npairs - 1;  // pair: last with cand0 <= c
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (Q->cand0[m] <= c) lo = m;
+    else hi = m - 1;
   }
-  flag[p] = f;
+  const int i = lo;
+  int a = Q->rng0[i], z = Q->rng0[i + 1] - 1;  // range: last with first candidate <= c
+  while (a < z) {
+    const int m = (a + z + 1) >> 1;
+    if (Q->rng[m].x <= c) a = m;
+    else z = m - 1;
+  }
+  const int p = Q->rng[a].y + (c - Q->rng[a].x);
+  const int b = Q->blk[i];
+  const int v = Q->invmap[b][p];
+  double q[3], xd[3];
+  load_xyz(Q->x[b], v, q);
+  const OBB &o = Q->obc[i];
+  to_frame(q, o.xc, o.vec, xd);
+  flag[c] = (fabs(xd[0]) <= o.dxc[0] && fabs(xd[1]) <= o.dxc[1] && fabs(xd[2]) <= o.dxc[2]);
+  pos[c] = p;
+}
+
+// positions of the selected candidates, and per pair the number selected
+// (the selection is ascending, so each pair's are one run)
+__global__ void k_take_pos(int n, const int *__restrict__ sel, const int *__restrict__ pos, int *__restrict__ out)
+{
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = pos[sel[j]];
+}
+
+__global__ void k_pair_runs(int npairs, int nsel, const int *__restrict__ sel, const int *__restrict__ cand0,
+                            int *__restrict__ first)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > npairs) return;
+  int lo = 0, hi = nsel;  // first selected index >= cand0[i]
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (sel[m] < cand0[i]) lo = m + 1;
+    else hi = m;
+  }
+  first[i] = lo;
 }
 
 // exchangeSearchData.C:200-255: the receiver's search arrays for one packet
@@ -873,6 +1186,12 @@ __global__ void k_select(SearchView S, CandOut O)
   if (S.xtag && S.xtag[i] != i) return;
   const int nc = O.qcnt[i], off = O.qoff[i];
   int res = -1;
+  // a list that did not fit was not stored; the host sees the overflow in
+  // the used count and searches again with a larger list
+  if ((long long)off + nc > O.cap) {
+    S.donorId[i] = -1;
+    return;
+  }
   for (int k = 0; k < nc; k++) {
     const int st = O.status[off + k];
     const int cell = O.cell[off + k] & 0x0fffffff;
@@ -1391,6 +1710,7 @@ Engine::~Engine()
   for (Block *b : blocks_) delete b;
   for (auto *q : qdev_) delete q;
   amr_release();
+  if (pinned_) (void)hipHostFree(pinned_);
   if (s_) (void)hipStreamDestroy(s_);
 }
 
@@ -1559,6 +1879,17 @@ void Engine::tag_boundary(Block &b)
     sort_pairs(uscratch_.p, uscratch2_.p, iscratch_.p, b.invmap.p, V, 10, s_);
     k_scatter_bin<<<nb(V), BS, 0, s_>>>(V, uscratch2_.p, b.invmap.p, b.nodebin.p);
   }
+  // first position of each bin in that order (binoff[1000] = V)
+  {
+    DBuf<int> bo;
+    bo.reserve(1001);
+    fill<int>(bo.p, 1001, V, s_);
+    if (V) k_bin_starts<<<nb(V), BS, 0, s_>>>(V, uscratch2_.p, bo.p);
+    b.binoff.assign(1001, V);
+    TG_HIP(hipMemcpyAsync(b.binoff.data(), bo.p, sizeof(int) * 1001, hipMemcpyDeviceToHost, s_));
+    sync();
+    for (int k = 999; k >= 0; k--) b.binoff[k] = std::min(b.binoff[k], b.binoff[k + 1]);
+  }
   // overset boundary and mexclude layers
   b.obcnode.reserve(V);
   fill<unsigned char>(b.obcnode.p, V, 0, s_);
@@ -1592,6 +1923,17 @@ void Engine::tag_boundary(Block &b)
 // tioga::profile (tioga.C:95-110)
 void Engine::profile()
 {
+  // page-locked staging for the query points of the batched search: every
+  // node queried once is the usual upper end
+  size_t allpts = 0;
+  for (Block *bp : blocks_) allpts += 3 * (size_t)bp->nnodes;
+  if (batched && allpts > pinned_cap_) {
+    if (pinned_) TG_HIP(hipHostFree(pinned_));
+    pinned_ = nullptr;
+    pinned_cap_ = 0;
+    TG_HIP(hipHostMalloc((void **)&pinned_, sizeof(double) * allpts, hipHostMallocDefault));
+    pinned_cap_ = allpts;
+  }
   for (Block *bp : blocks_) {
     Block &b = *bp;
     if (!b.topo_uploaded) upload_topology(b);
@@ -1662,10 +2004,48 @@ void Engine::get_hole_map()
     }
     size_t n = (size_t)h.nx[0] * h.nx[1] * h.nx[2];
     h.sam.reserve(n);
-    fill<unsigned char>(h.sam.p, n, 0, s_);
+    if (!batched) fill<unsigned char>(h.sam.p, n, 0, s_);
     hv.sam = h.sam.p;
     memcpy(hv.nx, h.nx, sizeof hv.nx);
     memcpy(hv.ext, h.ext, sizeof hv.ext);
+  }
+  // the maps of every wall mesh together (batched form)
+  std::vector<int> live;
+  for (int i = 0; i < maxtag; i++)
+    if (hm_[i].exist) live.push_back(i);
+  if (live.size() > 128) throw std::runtime_error("tioga_amd: more than 128 wall meshes");
+  MapSet ms{};
+  DevStruct<MapSet> dms;
+  MapSet *pms = nullptr;
+  int allow = 7;  // isym stencil: 3 excludes z, 1 excludes x, 2 excludes y
+  if (isym == 1) allow = 6;
+  else if (isym == 2) allow = 5;
+  else if (isym == 3) allow = 3;
+  // with a 2-D stencil every slice normal to the excluded axis fills on its
+  // own, in LDS when it fits
+  const int saxis = isym == 3 ? 2 : (isym == 1 ? 0 : (isym == 2 ? 1 : -1));
+  bool slices = saxis >= 0;
+  int maxcells = 0, maxlines[3] = {0, 0, 0}, maxslices = 0;
+  if (!live.empty()) {
+    ms.nmap = (int)live.size();
+    for (int k = 0; k < ms.nmap; k++) {
+      HoleMap &h = hm_[live[k]];
+      ms.sam[k] = h.sam.p;
+      for (int j = 0; j < 3; j++) ms.nx[k][j] = h.nx[j];
+      const long long cells = (long long)h.nx[0] * h.nx[1] * h.nx[2];
+      if (cells > 0x7fffffffLL) throw std::runtime_error("tioga_amd: hole map too large");
+      maxcells = std::max(maxcells, (int)cells);
+      for (int a = 0; a < 3; a++) {
+        const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
+        if (h.nx[a1] > 2 && h.nx[a2] > 2) maxlines[a] = std::max(maxlines[a], (h.nx[a1] - 2) * (h.nx[a2] - 2));
+      }
+      if (saxis >= 0) {
+        slices = slices && (long long)h.nx[(saxis + 1) % 3] * h.nx[(saxis + 2) % 3] <= SLICE_MAX;
+        maxslices = std::max(maxslices, h.nx[saxis] - 2);
+      }
+    }
+    pms = dms.put(ms, s_);
+    if (batched) k_maps_cells<<<dim3(nb(maxcells), ms.nmap), BS, 0, s_>>>(pms, 0);
   }
   for (Block *bp : blocks_) {
     Block &b = *bp;
@@ -1677,18 +2057,45 @@ void Engine::get_hole_map()
                                              ds[0], ds[1], ds[2], h.sam.p);
   }
   nmesh_ = maxtag;
-  // flood fill with the isym stencil: 3 excludes z, 1 excludes x, 2 excludes y
-  int allow = 7;
-  if (isym == 1) allow = 6;
-  else if (isym == 2) allow = 5;
-  else if (isym == 3) allow = 3;
+  if (batched && !live.empty()) {
+    const int nm = ms.nmap;
+    k_maps_cells<<<dim3(nb(maxcells), nm), BS, 0, s_>>>(pms, 1);
+    DBuf<int> act, chg;
+    act.reserve(nm);
+    chg.reserve(nm);
+    std::vector<int> hact(nm, 1), hchg(nm);
+    if (slices && maxslices > 0)
+      k_fill_slices<<<dim3(maxslices, nm), 256, 0, s_>>>(saxis, nm, &pms->sam[0], &pms->nx[0][0]);
+    for (; !slices;) {
+      TG_HIP(hipMemcpyAsync(act.p, hact.data(), sizeof(int) * nm, hipMemcpyHostToDevice, s_));
+      fill<int>(chg.p, nm, 0, s_);
+      for (int a = 0; a < 3; a++)
+        if (((allow >> a) & 1) && maxlines[a] > 0)
+          k_maps_line_fill<<<dim3(nb(maxlines[a]), nm), BS, 0, s_>>>(pms, a, allow, act.p, chg.p);
+      TG_HIP(hipMemcpyAsync(hchg.data(), chg.p, sizeof(int) * nm, hipMemcpyDeviceToHost, s_));
+      sync();
+      bool any = false;
+      for (int k = 0; k < nm; k++) {
+        hact[k] = hact[k] && hchg[k];
+        any = any || hact[k];
+      }
+      if (!any) break;
+    }
+    k_maps_cells<<<dim3(nb(maxcells), nm), BS, 0, s_>>>(pms, 2);
+  }
   int *changed = err_.p;  // reuse the one-int device word
-  for (int i = 0; i < maxtag; i++) {
+  for (int i = 0; i < maxtag && !batched; i++) {
     HoleMap &h = hm_[i];
     if (!h.exist) continue;
     size_t n = (size_t)h.nx[0] * h.nx[1] * h.nx[2];
     k_hole_shell<<<nb((long long)n), BS, 0, s_>>>(h.nx[0], h.nx[1], h.nx[2], h.sam.p);
-    for (int round = 0;; round++) {
+    if (slices) {
+      int k = 0;
+      while (live[k] != i) k++;
+      if (h.nx[saxis] > 2)
+        k_fill_slices<<<dim3(h.nx[saxis] - 2, 1), 256, 0, s_>>>(saxis, 1, &pms->sam[k], &pms->nx[k][0]);
+    }
+    for (int round = 0; !slices; round++) {
       int hc = 0;
       TG_HIP(hipMemsetAsync(changed, 0, sizeof(int), s_));
       for (int a = 0; a < 3; a++) {
@@ -1737,15 +2144,37 @@ void Engine::exchange_boxes()
     }
   TG_HIP(hipMemcpyAsync(kscratch_.p, init.data(), sizeof(unsigned long long) * init.size(), hipMemcpyHostToDevice,
                         s_));
+  std::vector<int> npart(nb_, 0);  // pairs per local block
+  for (const Pair &p : pairs_) npart[p.local]++;
   for (int i = 0; i < np; i++) {
     Pair &p = pairs_[i];
     Block &A = *blocks_[p.local], &B = *blocks_[p.ob];
     p.recv_tag = maxtag * (A.tag - 1) + B.tag - 1;
     p.send_tag = maxtag * (B.tag - 1) + (A.tag - 1);
-    if (A.ncells)
+    if (A.ncells && !(batched && npart[p.local] > 1))
       k_reduced_obb<<<nb(A.ncells), BS, 0, s_>>>(A.ncells, A.ti, A.conn.p, A.x.p, A.obb, B.obb,
                                                  kscratch_.p + 6 * (size_t)i);
     memcpy(p.obc.vec, B.obb.vec, sizeof(double) * 9);
+  }
+  if (batched) {  // one pass over each block's cells for all of its pairs
+    std::vector<DevStruct<BoxPairs> *> tabs;
+    for (int ib = 0; ib < nb_; ib++) {
+      Block &A = *blocks_[ib];
+      BoxPairs bp{};
+      for (int i = 0; i < np; i++)
+        if (pairs_[i].local == ib) {
+          if (bp.np == 128) throw std::runtime_error("tioga_amd: more than 128 partners of one block");
+          bp.pair[bp.np] = i;
+          bp.obc[bp.np] = blocks_[pairs_[i].ob]->obb;
+          bp.np++;
+        }
+      if (bp.np < 2 || A.ncells == 0) continue;
+      tabs.push_back(new DevStruct<BoxPairs>);
+      k_reduced_obb_multi<<<nb(A.ncells), BS, 0, s_>>>(A.ncells, A.ti, A.conn.p, A.x.p, A.obb,
+                                                       tabs.back()->put(bp, s_), kscratch_.p);
+    }
+    sync();
+    for (auto *t : tabs) delete t;
   }
   std::vector<unsigned long long> keys(6 * (size_t)std::max(np, 1));
   TG_HIP(hipMemcpyAsync(keys.data(), kscratch_.p, sizeof(unsigned long long) * keys.size(), hipMemcpyDeviceToHost,
@@ -1784,10 +2213,9 @@ void Engine::exchange_search_data()
   const int np = (int)pairs_.size();
   for (Block *b : blocks_) b->nsearch = 0;
   if (nsend_ == 0) return;
-  std::vector<int> cnt(np), dest(np, -1);
-  std::vector<DBuf<int> *> lists(np);
-  DBuf<unsigned char> binmask;
-  binmask.reserve(1000);
+  std::vector<int> cnt(np), dest(np, -1), cand0(np + 1), rng0(np + 1), lst0(np);
+  std::vector<int2> rng;
+  int ncand = 0;
   for (int i = 0; i < np; i++) {
     Block &A = *blocks_[pairs_[i].local];
     const OBB &obc = pairs_[i].obc;
@@ -1847,14 +2275,78 @@ void Engine::exchange_search_data()
           if (out) continue;
           mask[l * 100 + k * 10 + j] = 1;
         }
-    TG_HIP(hipMemcpyAsync(binmask.p, mask, 1000, hipMemcpyHostToDevice, s_));
-    iscratch_.reserve(A.nnodes);
-    if (A.nnodes)
-      k_query_flag<<<nb(A.nnodes), BS, 0, s_>>>(A.nnodes, A.invmap.p, A.nodebin.p, binmask.p, A.x.p, obc,
-                                                iscratch_.p);
-    lists[i] = new DBuf<int>;
-    lists[i]->reserve(A.nnodes);
-    cnt[i] = select_flagged(iscratch_.p, A.nnodes, lists[i]->p, s_);
+    // candidates: the positions of the masked bins, bins in (l, k, j) order,
+    // which is the order the positions already have
+    rng0[i] = (int)rng.size();
+    cand0[i] = ncand;
+    for (int kb = 0; kb < 1000; kb++) {
+      if (!mask[kb]) continue;
+      const int p0 = A.binoff[kb], p1 = A.binoff[kb + 1];
+      if (p1 <= p0) continue;
+      rng.push_back(int2{ncand, p0});
+      if (ncand > 0x7fffffff - (p1 - p0)) throw std::runtime_error("tioga_amd: too many query candidates");
+      ncand += p1 - p0;
+    }
+  }
+  rng0[np] = (int)rng.size();
+  cand0[np] = ncand;
+  // one flag kernel over the candidates of every pair, one compaction
+  {
+    QueryPairs qp{};
+    DBuf<int> dcand0, drng0, first;
+    DBuf<int2> drng;
+    dcand0.reserve(np + 1);
+    drng0.reserve(np + 1);
+    first.reserve(np + 1);
+    drng.reserve(std::max<size_t>(rng.size(), 1));
+    TG_HIP(hipMemcpyAsync(dcand0.p, cand0.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice, s_));
+    TG_HIP(hipMemcpyAsync(drng0.p, rng0.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice, s_));
+    if (!rng.empty())
+      TG_HIP(hipMemcpyAsync(drng.p, rng.data(), sizeof(int2) * rng.size(), hipMemcpyHostToDevice, s_));
+    qp.cand0 = dcand0.p;
+    qp.rng = drng.p;
+    qp.rng0 = drng0.p;
+    for (int b = 0; b < (int)blocks_.size(); b++) {
+      qp.invmap[b] = blocks_[b]->invmap.p;
+      qp.x[b] = blocks_[b]->x.p;
+    }
+    std::vector<OBB> hobc(np);
+    std::vector<int> hblk(np);
+    for (int i = 0; i < np; i++) {
+      hobc[i] = pairs_[i].obc;
+      hblk[i] = pairs_[i].local;
+    }
+    DBuf<OBB> dobc;
+    DBuf<int> dblk;
+    dobc.reserve(np);
+    dblk.reserve(np);
+    TG_HIP(hipMemcpyAsync(dobc.p, hobc.data(), sizeof(OBB) * np, hipMemcpyHostToDevice, s_));
+    TG_HIP(hipMemcpyAsync(dblk.p, hblk.data(), sizeof(int) * np, hipMemcpyHostToDevice, s_));
+    qp.obc = dobc.p;
+    qp.blk = dblk.p;
+    DevStruct<QueryPairs> dq;
+    QueryPairs *pq = dq.put(qp, s_);
+    DBuf<int> flag, pos, sel;
+    flag.reserve(ncand);
+    pos.reserve(ncand);
+    sel.reserve(ncand);
+    qlist_.reserve(ncand);
+    int nsel = 0;
+    if (ncand) {
+      k_query_cand<<<nb(ncand), BS, 0, s_>>>(ncand, np, pq, flag.p, pos.p);
+      nsel = select_flagged(flag.p, ncand, sel.p, s_);
+    }
+    std::vector<int> hf(np + 1, 0);
+    if (nsel) {
+      k_take_pos<<<nb(nsel), BS, 0, s_>>>(nsel, sel.p, pos.p, qlist_.p);
+      k_pair_runs<<<nb(np + 1), BS, 0, s_>>>(np, nsel, sel.p, dcand0.p, first.p);
+      TG_HIP(hipMemcpyAsync(hf.data(), first.p, sizeof(int) * (np + 1), hipMemcpyDeviceToHost, s_));
+      sync();
+    }
+    for (int i = 0; i < np; i++) {
+      lst0[i] = hf[i];
+      cnt[i] = nsel ? hf[i + 1] - hf[i] : 0;
+    }
   }
   for (int i = 0; i < np; i++) {
     int key = pairs_[i].send_tag, ii = -1;
@@ -1880,13 +2372,12 @@ void Engine::exchange_search_data()
     Block &A = *blocks_[pairs_[i].local];
     int rb = pairs_[ii].local;
     Block &R = *blocks_[rb];
-    k_fill_search<<<nb(cnt[i]), BS, 0, s_>>>(cnt[i], lists[i]->p, A.invmap.p, A.x.p, A.nodeRes.p, off[rb],
+    k_fill_search<<<nb(cnt[i]), BS, 0, s_>>>(cnt[i], qlist_.p + lst0[i], A.invmap.p, A.x.p, A.nodeRes.p, off[rb],
                                              pairs_[ii].iblk_remote, pairs_[ii].tag_remote, R.xsearch.p,
                                              R.res_search.p, R.isearch.p, R.tagsearch.p);
     off[rb] += cnt[i];
   }
   sync();
-  for (auto *l : lists) delete l;
 }
 
 // MeshBlock::search (search.C:78-269), NODEGID off
@@ -1905,35 +2396,155 @@ void Engine::search(Block &b)
   bool gid_mode = true;
   for (Block *o : blocks_) gid_mode = gid_mode && (o->unode_gid != nullptr);
   if (gid_mode) {
-    // search.C:48-74: first occurrence of each node global id
-    std::vector<double> rs(Q);
-    std::vector<int> xt(Q), is(3 * (size_t)Q);
-    TG_HIP(hipMemcpyAsync(rs.data(), b.res_search.p, sizeof(double) * Q, hipMemcpyDeviceToHost, s_));
-    TG_HIP(hipMemcpyAsync(is.data(), b.isearch.p, sizeof(int) * 3 * Q, hipMemcpyDeviceToHost, s_));
-    sync();
-    std::unordered_map<uint64_t, int> first;
-    first.reserve(2 * (size_t)Q);
-    for (int i = 0; i < Q; i++) {
-      uint64_t g = blocks_[is[3 * i + 2]]->unode_gid[is[3 * i + 1]];
-      auto it = first.find(g);
-      if (it != first.end()) {
-        xt[i] = it->second;
-        rs[it->second] = std::max(rs[it->second], rs[i]);
-      } else {
-        first[g] = i;
-        xt[i] = i;
-      }
-    }
-    for (int i = 0; i < Q; i++) rs[i] = rs[xt[i]];
-    TG_HIP(hipMemcpyAsync(b.xtag.p, xt.data(), sizeof(int) * Q, hipMemcpyHostToDevice, s_));
-    TG_HIP(hipMemcpyAsync(b.res_search.p, rs.data(), sizeof(double) * Q, hipMemcpyHostToDevice, s_));
-    sync();
+    dedupe_gid(b);
   } else {
     auto t0 = std::chrono::steady_clock::now();
     unique_nodes_device(Q, b.xsearch.p, b.tagsearch.p, b.res_search.p, b.xtag.p, dedw_, s_);
     sync();
     prep_acc_[3] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
+}
+
+// MeshBlock::search (search.C:78-269) for every block at once: the query-set
+// OBBs on the host (in parallel over blocks), the cell filter per block, one
+// ADT build for all blocks' trees and one coincident-point dedupe for all
+// query sets.  Each block's arrays come out as search() makes them.
+void Engine::search_batched()
+{
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto ms = [](auto a, auto c) { return std::chrono::duration<double, std::milli>(c - a).count(); };
+  const int nbk = (int)blocks_.size();
+  std::vector<int> act;
+  size_t tot = 0;
+  for (int ib = 0; ib < nbk; ib++) {
+    blocks_[ib]->nelem = 0;
+    if (blocks_[ib]->nsearch > 0) {
+      act.push_back(ib);
+      tot += 3 * (size_t)blocks_[ib]->nsearch;
+    }
+  }
+  if (act.empty()) return;
+  auto t0 = now();
+  // query-set OBBs: running FP64 sums whose order fixes their bits, one
+  // host thread per block
+  if (tot > pinned_cap_) {
+    if (pinned_) TG_HIP(hipHostFree(pinned_));
+    pinned_ = nullptr;
+    pinned_cap_ = 0;
+    const size_t want = tot + tot / 2;
+    TG_HIP(hipHostMalloc((void **)&pinned_, sizeof(double) * want, hipHostMallocDefault));
+    pinned_cap_ = want;
+  }
+  std::vector<size_t> off(nbk, 0);
+  {
+    size_t o = 0;
+    for (int ib : act) {
+      off[ib] = o;
+      TG_HIP(hipMemcpyAsync(pinned_ + o, blocks_[ib]->xsearch.p, sizeof(double) * 3 * blocks_[ib]->nsearch,
+                            hipMemcpyDeviceToHost, s_));
+      o += 3 * (size_t)blocks_[ib]->nsearch;
+    }
+  }
+  sync();
+  // the OBBs on host threads while the device deduplicates the query sets
+  std::vector<OBB> obq(nbk);
+  std::vector<int> order = act;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return blocks_[a]->nsearch > blocks_[b]->nsearch; });
+  std::atomic<int> next{0};
+  auto work = [&] {
+    for (int k; (k = next.fetch_add(1)) < (int)order.size();) {
+      const int ib = order[k];
+      find_obb(pinned_ + off[ib], nullptr, blocks_[ib]->nsearch, obq[ib]);
+    }
+  };
+  const int nt = std::max(1, std::min((int)order.size(), (int)std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++) th.emplace_back(work);
+  auto td0 = now();
+  bool gid_mode = true;
+  for (Block *o : blocks_) gid_mode = gid_mode && (o->unode_gid != nullptr);
+  if (gid_mode) {
+    for (int ib : act) dedupe_gid(*blocks_[ib]);
+  } else {
+    std::vector<DedupeSet> sets;
+    for (int ib : act) {
+      Block &b = *blocks_[ib];
+      sets.push_back(DedupeSet{b.nsearch, b.xsearch.p, b.tagsearch.p, b.res_search.p, b.xtag.p});
+    }
+    unique_nodes_batch(sets.data(), (int)sets.size(), dedw_, s_);
+    sync();
+  }
+  auto td1 = now();
+  for (auto &t : th) t.join();
+  auto t1 = now();
+  // cells overlapping each query OBB, as a LIFO list, and their boxes
+  std::vector<AdtTree> trees(act.size());
+  for (size_t a = 0; a < act.size(); a++) {
+    Block &b = *blocks_[act[a]];
+    const int N = b.ncells;
+    iscratch_.reserve(N);
+    iscratch2_.reserve(N);
+    if (N) k_cell_filter<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, b.x.p, obq[act[a]], iscratch_.p);
+    const int ne = select_flagged(iscratch_.p, N, iscratch2_.p, s_);
+    b.nelem = ne;
+    b.elist.reserve(ne);
+    b.ebox.reserve(6 * (size_t)ne);
+    b.adtInt.reserve(4 * (size_t)ne);
+    b.adtReal.reserve(6 * (size_t)ne);
+    if (ne) k_elem_box<<<nb(ne), BS, 0, s_>>>(ne, N, iscratch2_.p, b.ti, b.conn.p, b.x.p, b.elist.p, b.ebox.p);
+    trees[a].n = ne;
+    trees[a].ebox = b.ebox.p;
+    trees[a].adtInt = b.adtInt.p;
+    trees[a].adtReal = b.adtReal.p;
+  }
+  sync();
+  auto t2 = now();
+  build_adt_batch(trees.data(), (int)trees.size(), adtw_, s_);
+  for (size_t a = 0; a < act.size(); a++) {
+    Block &b = *blocks_[act[a]];
+    b.adt_depth = trees[a].depth;
+    memcpy(b.adtExt, trees[a].ext, sizeof b.adtExt);
+    if (b.adt_depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
+    b.nodes.reserve(b.nelem);
+    if (b.nelem)
+      k_adt_nodes<<<nb(b.nelem), BS, 0, s_>>>(b.nelem, b.ti, b.adtInt.p, b.adtReal.p, b.ebox.p, b.elist.p,
+                                              b.nodes.p);
+  }
+  sync();
+  auto t3 = now();
+  // obb: staging and the host threads beyond the overlapped dedupe
+  prep_acc_[0] += ms(t0, td0) + ms(td1, t1);
+  prep_acc_[1] += ms(t1, t2);
+  prep_acc_[2] += ms(t2, t3);
+  prep_acc_[3] += ms(td0, td1);
+}
+
+// search.C:48-74: first occurrence of each node global id
+void Engine::dedupe_gid(Block &b)
+{
+  const int Q = b.nsearch;
+  std::vector<double> rs(Q);
+  std::vector<int> xt(Q), is(3 * (size_t)Q);
+  TG_HIP(hipMemcpyAsync(rs.data(), b.res_search.p, sizeof(double) * Q, hipMemcpyDeviceToHost, s_));
+  TG_HIP(hipMemcpyAsync(is.data(), b.isearch.p, sizeof(int) * 3 * Q, hipMemcpyDeviceToHost, s_));
+  sync();
+  std::unordered_map<uint64_t, int> first;
+  first.reserve(2 * (size_t)Q);
+  for (int i = 0; i < Q; i++) {
+    uint64_t g = blocks_[is[3 * i + 2]]->unode_gid[is[3 * i + 1]];
+    auto it = first.find(g);
+    if (it != first.end()) {
+      xt[i] = it->second;
+      rs[it->second] = std::max(rs[it->second], rs[i]);
+    } else {
+      first[g] = i;
+      xt[i] = i;
+    }
+  }
+  for (int i = 0; i < Q; i++) rs[i] = rs[xt[i]];
+  TG_HIP(hipMemcpyAsync(b.xtag.p, xt.data(), sizeof(int) * Q, hipMemcpyHostToDevice, s_));
+  TG_HIP(hipMemcpyAsync(b.res_search.p, rs.data(), sizeof(double) * Q, hipMemcpyHostToDevice, s_));
+  sync();
 }
 
 // search.C:107-233: query-set OBB -> cells whose OBB-frame box overlaps it,
@@ -2319,7 +2930,11 @@ void Engine::perform_connectivity()
   for (Block *b : blocks_) {
     b->ninterp = 0;
     b->interp_host_valid = false;
-    search(*b);
+  }
+  if (batched) {
+    search_batched();
+  } else {
+    for (Block *b : blocks_) search(*b);
   }
   auto t4 = now();
   {

@@ -15,28 +15,36 @@ void check(hipError_t e, const char *what)
   if (e != hipSuccess) throw std::runtime_error(std::string("tioga_amd: ") + what + ": " + hipGetErrorString(e));
 }
 
-// Scratch for the library calls; grows on demand and is kept.
-struct Temp {
-  void *p = nullptr;
-  size_t n = 0;
-  void *get(size_t want)
-  {
-    if (want > n) {
-      if (p) check(hipFree(p), "hipFree");
-      check(hipMalloc(&p, want), "hipMalloc(temp)");
-      n = want;
-    }
-    return p;
-  }
-  ~Temp()
-  {
-    if (p) (void)hipFree(p);
-  }
-};
-thread_local Temp g_temp;
+thread_local PrimTemp g_temp;
 thread_local int *g_count = nullptr;
 
 }  // namespace
+
+void *PrimTemp::get(size_t want)
+{
+  if (want > n) {
+    if (p) check(hipFree(p), "hipFree");
+    p = nullptr;
+    n = 0;
+    check(hipMalloc(&p, want), "hipMalloc(temp)");
+    n = want;
+  }
+  return p;
+}
+
+PrimTemp::~PrimTemp()
+{
+  if (p) (void)hipFree(p);
+}
+
+size_t sort_pairs_bytes(int n, int bits)
+{
+  size_t bytes = 0;
+  check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const unsigned *)nullptr, (unsigned *)nullptr,
+                                           (const int *)nullptr, (int *)nullptr, n, 0, bits),
+        "sort(size)");
+  return bytes;
+}
 
 int select_flagged(const int *flags, int n, int *out, hipStream_t s)
 {
@@ -53,14 +61,14 @@ int select_flagged(const int *flags, int n, int *out, hipStream_t s)
 }
 
 void sort_pairs(const unsigned *keys_in, unsigned *keys_out, const int *vals_in, int *vals_out, int n,
-                int bits, hipStream_t s)
+                int bits, hipStream_t s, PrimTemp *temp)
 {
   if (n <= 0) return;
   size_t bytes = 0;
   check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in, keys_out, vals_in, vals_out, n, 0, bits, s),
         "sort(size)");
-  check(hipcub::DeviceRadixSort::SortPairs(g_temp.get(bytes), bytes, keys_in, keys_out, vals_in, vals_out, n, 0,
-                                           bits, s),
+  PrimTemp &t = temp ? *temp : g_temp;
+  check(hipcub::DeviceRadixSort::SortPairs(t.get(bytes), bytes, keys_in, keys_out, vals_in, vals_out, n, 0, bits, s),
         "sort");
 }
 

@@ -76,6 +76,7 @@ def load():
         L.tioga_performconnectivity_amr_.argtypes = []
         L.tioga_amd_time_dataupdate_amr.argtypes = [C.c_int, C.c_int, C.c_int, _P]
         L.tioga_amd_time_dataupdate_amr.restype = C.c_double
+        L.tioga_amd_set_batching.argtypes = [C.c_int]
         L.tioga_amd_amr_array.argtypes = [C.c_int, C.c_char_p, _P]
         L.tioga_amd_amr_array.restype = C.c_long
         _lib = L
@@ -155,6 +156,11 @@ class Tioga:
         self.L.tioga_setsymmetry_(_ip(isym))
         self.L.tioga_setnfringe_(_ip(nfringe))
         self.L.tioga_setmexclude_(_ip(mexclude))
+        self._check()
+
+    def set_batching(self, on: bool) -> None:
+        """Batched form (all blocks per stage) or block by block."""
+        self.L.tioga_amd_set_batching(1 if on else 0)
         self._check()
 
     def set_coords(self, bi: int, x: np.ndarray) -> None:
