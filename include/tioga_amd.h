@@ -136,6 +136,23 @@ double tioga_amd_time_search(int nrep, long long *nsearch);
    interpolation records per pass. */
 double tioga_amd_time_dataupdate(int nvar, int interptype, int nrep, long long *nreceptors);
 
+/* Serving: requests that move one registered block rigidly and ask for its
+   connectivity with the background block alone (the two-block
+   performconnectivity on the moved coordinates, x' = R (x - c) + c + t per
+   node), served by nslots concurrent slots that share the background's
+   resident device data; requests start in arrival order while their device
+   memory estimates fit budget_gb.  Open after preprocess; the registered
+   blocks must not change while serving.  submit returns a request id (-1 on
+   error); wait blocks until that request is done and fills the moved
+   block's iblank (nshell entries), the background nodes whose iblank is not
+   1 (count in *nbg, up to cap entries copied), times[4] = submit, start and
+   done (seconds, one steady clock) and connectivity ms, and *nsearch. */
+int tioga_amd_serve_open(int bg_tag, int nslots, double budget_gb);
+long tioga_amd_serve_submit(int shell_tag, const double *rot9, const double *center3, const double *shift3);
+int tioga_amd_serve_wait(long id, int *shell_iblank, int nshell, int *nbg, int *bg_nodes, int *bg_vals, int cap,
+                         double *times, long long *nsearch);
+void tioga_amd_serve_close(void);
+
 /* Batched form (on != 0, the default): the stages of performconnectivity
    run for all blocks together (one ADT build, one coincident-point dedupe
    for every block's query set, ...); 0 runs them block by block.  The

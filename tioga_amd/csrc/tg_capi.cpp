@@ -13,6 +13,7 @@
 #include "../../include/tioga_amd.h"
 #include "tg_engine.h"
 #include "tg_seq.h"
+#include "tg_serve.h"
 
 namespace {
 
@@ -127,6 +128,7 @@ int tioga::iamrGlobal() const { return eng_->amr_mode() ? 1 : 0; }
 namespace {
 
 TIOGA::tioga *g_tg = nullptr;
+tg::Server *g_srv = nullptr;  // serving over g_tg's blocks (tioga_amd_serve_*)
 constexpr int MAXBLOCKS = 100;  // globals.h:30
 struct InpData {
   std::vector<int> nv, nc;
@@ -175,6 +177,8 @@ void tioga_init_f90_(int *) { tioga_init_(0); }
 void tioga_init_(tioga_comm_t)
 {
   guarded("tioga_init", [] {
+    delete g_srv;
+    g_srv = nullptr;
     delete g_tg;
     g_tg = nullptr;
     for (auto &d : g_idata) d = InpData();
@@ -300,6 +304,8 @@ void tioga_setmexclude_(int *mexclude)
 void tioga_delete_(void)
 {
   guarded("tioga_delete", [] {
+    delete g_srv;
+    g_srv = nullptr;
     delete g_tg;
     g_tg = nullptr;
     for (auto &d : g_idata) d = InpData();
@@ -387,6 +393,65 @@ double tioga_amd_time_dataupdate(int nvar, int interptype, int nrep, long long *
   guarded("tioga_amd_time_dataupdate",
           [&] { r = tg_or_throw().engine()->time_data_update(nvar, interptype, nrep, nreceptors); });
   return r;
+}
+
+// ---- serving (tg_serve.h) ----
+
+int tioga_amd_serve_open(int bg_tag, int nslots, double budget_gb)
+{
+  int rc = -1;
+  guarded("tioga_amd_serve_open", [&] {
+    delete g_srv;
+    g_srv = nullptr;
+    g_srv = new tg::Server(*tg_or_throw().engine(), bg_tag, nslots, budget_gb * 1e9);
+    rc = 0;
+  });
+  return rc;
+}
+
+long tioga_amd_serve_submit(int shell_tag, const double *rot, const double *center, const double *shift)
+{
+  long id = -1;
+  guarded("tioga_amd_serve_submit", [&] {
+    if (!g_srv) throw std::runtime_error("serving is not open");
+    id = g_srv->submit(shell_tag, rot, center, shift);
+  });
+  return id;
+}
+
+int tioga_amd_serve_wait(long id, int *shell_iblank, int nshell, int *nbg, int *bg_nodes, int *bg_vals, int cap,
+                         double *times, long long *nsearch)
+{
+  int rc = -1;
+  guarded("tioga_amd_serve_wait", [&] {
+    if (!g_srv) throw std::runtime_error("serving is not open");
+    tg::ServeResult r;
+    g_srv->wait(id, r);
+    if (r.status != 0) throw std::runtime_error("request failed: " + r.err);
+    if ((int)r.shell_iblank.size() != nshell) throw std::runtime_error("shell iblank size mismatch");
+    memcpy(shell_iblank, r.shell_iblank.data(), sizeof(int) * nshell);
+    *nbg = (int)r.bg_nodes.size();
+    const int k = std::min(cap, *nbg);
+    if (k > 0) {
+      memcpy(bg_nodes, r.bg_nodes.data(), sizeof(int) * k);
+      memcpy(bg_vals, r.bg_vals.data(), sizeof(int) * k);
+    }
+    times[0] = r.t_submit;
+    times[1] = r.t_start;
+    times[2] = r.t_done;
+    times[3] = r.connectivity_ms;
+    *nsearch = r.nsearch;
+    rc = 0;
+  });
+  return rc;
+}
+
+void tioga_amd_serve_close(void)
+{
+  guarded("tioga_amd_serve_close", [] {
+    delete g_srv;
+    g_srv = nullptr;
+  });
 }
 
 void tioga_amd_set_batching(int on)

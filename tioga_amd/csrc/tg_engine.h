@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include <map>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,7 @@ template <class T>
 struct DBuf {
   T *p = nullptr;
   size_t cap = 0;
+  bool own = true;  // false: a view of another engine's array (read only)
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
@@ -38,17 +40,26 @@ struct DBuf {
   {
     if (n == 0) n = 1;
     if (n > cap) {
+      if (!own) throw std::runtime_error("tioga_amd: a shared device array can not grow");
       release();
       TG_HIP(hipMalloc((void **)&p, n * sizeof(T)));
       cap = n;
     }
     return p;
   }
+  void borrow(const DBuf &src)
+  {
+    release();
+    p = src.p;
+    cap = src.cap;
+    own = false;
+  }
   void release()
   {
-    if (p) (void)hipFree(p);
+    if (p && own) (void)hipFree(p);
     p = nullptr;
     cap = 0;
+    own = true;
   }
   operator T *() const { return p; }
 };
@@ -103,6 +114,11 @@ struct Block {
   uint64_t *ucell_gid = nullptr, *unode_gid = nullptr;
   double resolutionScale = 1.0;
   bool topo_uploaded = false;
+  // 0: owns everything; 1: topology (conn, incidence, wall / overset node
+  // flags) is another engine's; 2: topology and geometry (coordinates,
+  // resolutions, bins, OBB) are another engine's.  Shared arrays are read only.
+  int share = 0;
+  bool writeback = true;  // write iblank / iblank_cell back to the user's arrays
 
   // ---- device state ----
   DBuf<double> x;          // 3V, AoS as registered
@@ -253,6 +269,26 @@ class Engine {
   // together (one ADT build, one dedupe, ... for every block); false runs
   // them block by block.  Results are identical either way.
   bool batched = true;
+  // page-locked staging for every registered node at preprocess (false:
+  // grown on demand by the search)
+  bool stage_all = true;
+
+  int block_count() const { return (int)blocks_.size(); }
+  int block_nnodes(int btag) const
+  {
+    auto it = tag_to_blk_.find(btag);
+    return it == tag_to_blk_.end() ? -1 : blocks_[it->second]->nnodes;
+  }
+  int block_ncells(int btag) const
+  {
+    auto it = tag_to_blk_.find(btag);
+    return it == tag_to_blk_.end() ? -1 : blocks_[it->second]->ncells;
+  }
+  const double *block_coords(int btag) const
+  {
+    auto it = tag_to_blk_.find(btag);
+    return it == tag_to_blk_.end() ? nullptr : blocks_[it->second]->ux;
+  }
 
   void register_grid(int btag, int nnodes, double *xyz, int *ibl, int nwbc, int nobc, int *wbc, int *obc,
                      int ntypes, const int *nv, const int *nc, int **vconn);
@@ -271,6 +307,16 @@ class Engine {
 
   // inspection of internal arrays (same names as the reference's members)
   long block_array(int btag, const std::string &name, void *out);
+
+  // Serving: block `as_tag` of this engine becomes a view of block
+  // `src_tag` of `src` (share 1: topology; 2: topology and geometry, see
+  // Block::share); ux / uiblank are this engine's host coordinate and iblank
+  // arrays (ignored for share 2); writeback false keeps iblank on the
+  // device.  slot >= 0 reuses that block slot (its device buffers too).
+  void adopt_block(const Engine &src, int src_tag, int as_tag, int share, double *ux, int *uiblank, bool writeback,
+                   int slot = -1);
+  // nodes of block btag whose iblank is not 1, with their values
+  void nonunit_iblank(int btag, std::vector<int> &nodes, std::vector<int> &vals);
   long hole_map(int mtag, int nx[3], double ext[6], int *sam);
 
   // Re-run the donor search kernels of every block over the resident ADTs

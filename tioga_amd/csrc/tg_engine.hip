@@ -1890,11 +1890,12 @@ void Engine::tag_boundary(Block &b)
     sync();
     for (int k = 999; k >= 0; k--) b.binoff[k] = std::min(b.binoff[k], b.binoff[k + 1]);
   }
-  // overset boundary and mexclude layers
-  b.obcnode.reserve(V);
-  fill<unsigned char>(b.obcnode.p, V, 0, s_);
+  // overset boundary and mexclude layers (the node flags depend on the
+  // topology alone, so a block sharing it has them already)
   DBuf<int> lst;
-  if (b.nobc > 0) {
+  if (b.share == 0) b.obcnode.reserve(V);
+  if (b.share == 0) fill<unsigned char>(b.obcnode.p, V, 0, s_);
+  if (b.share == 0 && b.nobc > 0) {
     lst.reserve(b.nobc);
     TG_HIP(hipMemcpyAsync(lst.p, b.uobc, sizeof(int) * b.nobc, hipMemcpyHostToDevice, s_));
     k_flag_list<<<nb(b.nobc), BS, 0, s_>>>(b.nobc, lst.p, b.obcnode.p);
@@ -1910,9 +1911,9 @@ void Engine::tag_boundary(Block &b)
     TG_HIP(hipMemcpyAsync(ex.p, ex1.p, V, hipMemcpyDeviceToDevice, s_));
   }
   // wall nodes, for the hole map
-  b.wallnode.reserve(V);
-  fill<unsigned char>(b.wallnode.p, V, 0, s_);
-  if (b.nwbc > 0) {
+  if (b.share == 0) b.wallnode.reserve(V);
+  if (b.share == 0) fill<unsigned char>(b.wallnode.p, V, 0, s_);
+  if (b.share == 0 && b.nwbc > 0) {
     lst.reserve(b.nwbc);
     TG_HIP(hipMemcpyAsync(lst.p, b.uwbc, sizeof(int) * b.nwbc, hipMemcpyHostToDevice, s_));
     k_flag_list<<<nb(b.nwbc), BS, 0, s_>>>(b.nwbc, lst.p, b.wallnode.p);
@@ -1927,7 +1928,7 @@ void Engine::profile()
   // node queried once is the usual upper end
   size_t allpts = 0;
   for (Block *bp : blocks_) allpts += 3 * (size_t)bp->nnodes;
-  if (batched && allpts > pinned_cap_) {
+  if (batched && stage_all && allpts > pinned_cap_) {
     if (pinned_) TG_HIP(hipHostFree(pinned_));
     pinned_ = nullptr;
     pinned_cap_ = 0;
@@ -1936,10 +1937,20 @@ void Engine::profile()
   }
   for (Block *bp : blocks_) {
     Block &b = *bp;
+    if (b.share == 2) {  // another engine's geometry: only this engine's iblank
+      b.iblank.reserve(b.nnodes);
+      fill<int>(b.iblank.p, b.nnodes, 1, s_);
+      if (b.writeback && b.uiblank)
+        for (int i = 0; i < b.nnodes; i++) b.uiblank[i] = 1;
+      continue;
+    }
     if (!b.topo_uploaded) upload_topology(b);
-    validate_ids(b.uwbc, b.nwbc, b.nnodes, "wall boundary list");
-    validate_ids(b.uobc, b.nobc, b.nnodes, "overset boundary list");
-    for (int i = 0; i < b.nnodes; i++) b.uiblank[i] = 1;  // MeshBlock.C:80
+    if (b.share == 0) {
+      validate_ids(b.uwbc, b.nwbc, b.nnodes, "wall boundary list");
+      validate_ids(b.uobc, b.nobc, b.nnodes, "overset boundary list");
+    }
+    if (b.writeback && b.uiblank)
+      for (int i = 0; i < b.nnodes; i++) b.uiblank[i] = 1;  // MeshBlock.C:80
     b.x.reserve(3 * (size_t)b.nnodes);
     TG_HIP(hipMemcpyAsync(b.x.p, b.ux, sizeof(double) * 3 * b.nnodes, hipMemcpyHostToDevice, s_));
     b.iblank.reserve(b.nnodes);
@@ -1947,6 +1958,110 @@ void Engine::profile()
     find_obb(b.ux, nullptr, b.nnodes, b.obb);
     tag_boundary(b);
   }
+}
+
+void Engine::adopt_block(const Engine &src, int src_tag, int as_tag, int share, double *ux, int *uiblank,
+                         bool writeback, int slot)
+{
+  auto it = src.tag_to_blk_.find(src_tag);
+  if (it == src.tag_to_blk_.end()) throw std::runtime_error("tioga_amd: adopt of an unknown tag");
+  const Block &S = *src.blocks_[it->second];
+  if (!S.topo_uploaded || S.nodeRes.p == nullptr) throw std::runtime_error("tioga_amd: adopt before preprocess");
+  if (share != 1 && share != 2) throw std::runtime_error("tioga_amd: share must be 1 or 2");
+  Block *b;
+  if (slot >= 0 && slot < (int)blocks_.size()) {
+    b = blocks_[slot];
+    tag_to_blk_.erase(b->tag);
+  } else {
+    if (blocks_.size() >= 128) throw std::runtime_error("tioga_amd: at most 128 blocks");
+    b = new Block;
+    blocks_.push_back(b);
+    slot = (int)blocks_.size() - 1;
+  }
+  if (tag_to_blk_.count(as_tag)) throw std::runtime_error("tioga_amd: adopt under a tag in use");
+  tag_to_blk_[as_tag] = slot;
+  b->tag = as_tag;
+  b->nnodes = S.nnodes;
+  b->ncells = S.ncells;
+  b->ti = S.ti;
+  b->nwbc = S.nwbc;
+  b->nobc = S.nobc;
+  b->uwbc = S.uwbc;
+  b->uobc = S.uobc;
+  for (int t = 0; t < MAXTYPES; t++) b->uconn[t] = S.uconn[t];
+  b->unres = S.unres;
+  b->ucres = S.ucres;
+  b->ucell_gid = S.ucell_gid;
+  b->unode_gid = S.unode_gid;
+  b->resolutionScale = S.resolutionScale;
+  b->uiblank_cell = nullptr;
+  b->uq = nullptr;
+  b->share = share;
+  b->writeback = writeback;
+  b->conn.borrow(S.conn);
+  b->n2c_off.borrow(S.n2c_off);
+  b->n2c.borrow(S.n2c);
+  b->wallnode.borrow(S.wallnode);
+  b->obcnode.borrow(S.obcnode);
+  b->topo_uploaded = true;
+  if (share == 2) {
+    b->ux = S.ux;
+    b->uiblank = nullptr;
+    b->x.borrow(S.x);
+    b->cellRes.borrow(S.cellRes);
+    b->nodeRes.borrow(S.nodeRes);
+    b->nodebin.borrow(S.nodebin);
+    b->invmap.borrow(S.invmap);
+    b->binoff = S.binoff;
+    b->obb = S.obb;
+    memcpy(b->mapdx, S.mapdx, sizeof b->mapdx);
+  } else {
+    b->ux = ux;
+    b->uiblank = uiblank;
+    // buffers this engine owns again if they were views
+    if (!b->x.own) b->x.release();
+    if (!b->cellRes.own) b->cellRes.release();
+    if (!b->nodeRes.own) b->nodeRes.release();
+    if (!b->nodebin.own) b->nodebin.release();
+    if (!b->invmap.own) b->invmap.release();
+  }
+  b->ninterp = 0;
+  b->interp_host_valid = false;
+  connected_ = false;
+}
+
+__global__ void k_nonunit(int n, const int *__restrict__ ib, int *__restrict__ flag)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = ib[i] != 1;
+}
+
+__global__ void k_gather_int(int n, const int *__restrict__ idx, const int *__restrict__ src, int *__restrict__ out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = src[idx[i]];
+}
+
+void Engine::nonunit_iblank(int btag, std::vector<int> &nodes, std::vector<int> &vals)
+{
+  Block *b = find(btag);
+  if (!b) throw std::runtime_error("tioga_amd: unknown tag");
+  const int V = b->nnodes;
+  nodes.clear();
+  vals.clear();
+  if (V == 0 || b->iblank.p == nullptr) return;
+  iscratch_.reserve(V);
+  iscratch2_.reserve(V);
+  k_nonunit<<<nb(V), BS, 0, s_>>>(V, b->iblank.p, iscratch_.p);
+  const int n = select_flagged(iscratch_.p, V, iscratch2_.p, s_);
+  nodes.resize(n);
+  vals.resize(n);
+  if (n) {
+    k_gather_int<<<nb(n), BS, 0, s_>>>(n, iscratch2_.p, b->iblank.p, iscratch_.p);
+    TG_HIP(hipMemcpyAsync(nodes.data(), iscratch2_.p, sizeof(int) * n, hipMemcpyDeviceToHost, s_));
+    TG_HIP(hipMemcpyAsync(vals.data(), iscratch_.p, sizeof(int) * n, hipMemcpyDeviceToHost, s_));
+  }
+  sync();
 }
 
 // tioga::getHoleMap (holeMap.C:40-188), single rank
@@ -2950,6 +3065,8 @@ void Engine::perform_connectivity()
   // write back to the user's arrays
   for (Block *bp : blocks_) {
     Block &b = *bp;
+    b.uq = nullptr;  // solutions are registered again after connectivity (tioga.C:152-155)
+    if (!b.writeback) continue;
     TG_HIP(hipMemcpyAsync(b.uiblank, b.iblank.p, sizeof(int) * b.nnodes, hipMemcpyDeviceToHost, s_));
     int *ibc = b.uiblank_cell;
     if (!ibc) {
@@ -2957,7 +3074,6 @@ void Engine::perform_connectivity()
       ibc = b.own_iblank_cell.data();
     }
     TG_HIP(hipMemcpyAsync(ibc, b.iblank_cell.p, sizeof(int) * b.ncells, hipMemcpyDeviceToHost, s_));
-    b.uq = nullptr;  // solutions are registered again after connectivity (tioga.C:152-155)
   }
   sync();
   auto t7 = now();

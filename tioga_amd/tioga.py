@@ -77,6 +77,13 @@ def load():
         L.tioga_amd_time_dataupdate_amr.argtypes = [C.c_int, C.c_int, C.c_int, _P]
         L.tioga_amd_time_dataupdate_amr.restype = C.c_double
         L.tioga_amd_set_batching.argtypes = [C.c_int]
+        L.tioga_amd_serve_open.argtypes = [C.c_int, C.c_int, C.c_double]
+        L.tioga_amd_serve_open.restype = C.c_int
+        L.tioga_amd_serve_submit.argtypes = [C.c_int, _P, _P, _P]
+        L.tioga_amd_serve_submit.restype = C.c_long
+        L.tioga_amd_serve_wait.argtypes = [C.c_long, _P, C.c_int, _P, _P, _P, C.c_int, _P, _P]
+        L.tioga_amd_serve_wait.restype = C.c_int
+        L.tioga_amd_serve_close.argtypes = []
         L.tioga_amd_amr_array.argtypes = [C.c_int, C.c_char_p, _P]
         L.tioga_amd_amr_array.restype = C.c_long
         _lib = L
@@ -263,6 +270,37 @@ class Tioga:
             for i, q in enumerate(self.data_update_copies(qs, nvar, interptype)):
                 r[f"b{i}.q"] = q
         return r
+
+    # ---- serving: moving blocks over the resident background ----
+    def serve_open(self, bg_tag: int, nslots: int = 4, budget_gb: float = 64.0) -> None:
+        self.L.tioga_amd_serve_open(bg_tag, nslots, budget_gb)
+        self._check()
+
+    def serve_submit(self, shell_tag: int, rot, center, shift) -> int:
+        r = np.ascontiguousarray(rot, np.float64).reshape(9)
+        c = np.ascontiguousarray(center, np.float64).reshape(3)
+        t = np.ascontiguousarray(shift, np.float64).reshape(3)
+        rid = self.L.tioga_amd_serve_submit(shell_tag, _p(r), _p(c), _p(t))
+        self._check()
+        return int(rid)
+
+    def serve_wait(self, rid: int, nshell: int, cap: int = 1 << 22) -> Dict[str, object]:
+        ib = np.zeros(max(nshell, 1), np.int32)
+        nbg = np.zeros(1, np.int32)
+        bn = np.zeros(cap, np.int32)
+        bv = np.zeros(cap, np.int32)
+        tm = np.zeros(4)
+        ns = C.c_longlong(0)
+        self.L.tioga_amd_serve_wait(rid, _p(ib), nshell, _p(nbg), _p(bn), _p(bv), cap, _p(tm), C.byref(ns))
+        self._check()
+        k = min(int(nbg[0]), cap)
+        return {"shell_iblank": ib[:nshell], "bg_nodes": bn[:k].copy(), "bg_vals": bv[:k].copy(),
+                "t_submit": tm[0], "t_start": tm[1], "t_done": tm[2], "connectivity_ms": tm[3],
+                "nsearch": ns.value}
+
+    def serve_close(self) -> None:
+        self.L.tioga_amd_serve_close()
+        self._check()
 
     # ---- extensions ----
     def array(self, btag: int, name: str) -> np.ndarray:

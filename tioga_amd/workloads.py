@@ -24,6 +24,13 @@ Workloads (sizes as stated; all coordinates FP64):
   seed 3; background 192^3 hex cube [-1,1]^3 (tag 65).  Each shell is the
   mixed_shell construction with a spherical wall and only hex and prism
   layers (its polar rows are prisms).
+* ``config4``  serving: the 256^3 jittered background of config2 (tag 17)
+  and 16 hex/prism spherical shells (tags 1..16) of log-uniform size in
+  [1e4, 2e6] cells, wall radius uniform in [0.02, 0.08], centres uniform in
+  [-0.6, 0.6]^3, seed 4; requests move one shell rigidly (rotation by an
+  angle uniform in [0, 5] degrees about a uniformly random axis, about the
+  shell's centroid, then a shift of uniformly random direction and length
+  uniform in [0, 0.02]) and arrive as a Poisson stream.
 * ``config5``  AMR Cartesian background: 3 levels (refinement ratio 2) of
   4096 patches of 16^3 cells with one ghost layer on [-1,1]^3 (level 0: the
   whole cube at h = 1/64, 512 patches; level 1: [-1,1]x[-0.75,0.75]x
@@ -413,3 +420,52 @@ def config3(nshell: int = 64, nbox: int = 192, seed: int = 3, lo: float = 1e4, h
         blocks.append(sphere_shell(n, r, t, c, rot))
     blocks.append(cart_box(nbox, tag=nshell + 1))
     return blocks
+
+
+def rigid_apply(x: np.ndarray, rot, center, shift) -> np.ndarray:
+    """x' = R (x - c) + c + t per node, in the operation order the serving
+    loop uses (each row of R applied left to right, then + c, then + t)."""
+    R = np.asarray(rot, np.float64).reshape(3, 3)
+    c = np.asarray(center, np.float64)
+    t = np.asarray(shift, np.float64)
+    d = [x[:, k] - c[k] for k in range(3)]
+    out = np.empty_like(x)
+    for j in range(3):
+        out[:, j] = R[j, 0] * d[0] + R[j, 1] * d[1] + R[j, 2] * d[2] + c[j] + t[j]
+    return out
+
+
+def axis_angle(axis, angle_deg) -> np.ndarray:
+    """Rotation matrix for angle_deg about axis (Rodrigues)."""
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    th = np.deg2rad(angle_deg)
+    k = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(th) * k + (1 - np.cos(th)) * k @ k
+
+
+def config4_shells(nshell: int = 16, seed: int = 4, lo: float = 1e4, hi: float = 2e6) -> List[Block]:
+    rng = np.random.default_rng(seed)
+    out = []
+    for t in range(1, nshell + 1):
+        n = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        r = float(rng.uniform(0.02, 0.08))
+        c = rng.uniform(-0.6, 0.6, 3)
+        out.append(sphere_shell(n, r, t, c, shoemake_rotation(rng.uniform(size=3))))
+    return out
+
+
+def motion_requests(n: int, nshell: int, seed: int):
+    """n requests: (shell tag, rotation matrix, shift), rotation uniform in
+    [0, 5] degrees about a uniformly random axis, shift of uniformly random
+    direction and length uniform in [0, 0.02]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        tag = int(rng.integers(1, nshell + 1))
+        ax = rng.normal(size=3)
+        R = axis_angle(ax, rng.uniform(0, 5))
+        d = rng.normal(size=3)
+        t = d / np.linalg.norm(d) * rng.uniform(0, 0.02)
+        out.append((tag, R, t))
+    return out
