@@ -319,9 +319,12 @@ def serving_bench(args) -> int:
     plain_sps = sum(p[1] for p in plain) / (sum(p[0] for p in plain) / 1000)
     log(f"plain form: {plain_ms:.1f} ms per request, {plain_sps:.3e} searches/s")
     best = max(per_rate, key=lambda r: r["searches_per_s"])
+    # the headline latency: the highest rate the slots keep up with
+    stable = [r for r in per_rate if r["throughput_req_s"] >= 0.95 * r["rate_req_s"]] or per_rate[:1]
+    head = max(stable, key=lambda r: r["rate_req_s"])
     out = {"metric": "p99 connectivity latency under Poisson load (serving), donor searches/s",
-           "value": best["p99_ms"], "unit": "ms", "n_gpus": 1, "steps": sum(r["requests"] for r in per_rate),
-           "warmup": len(shells), "ms_per_step": best["p50_ms"], "higher_is_better": False, "scaling": "weak",
+           "value": head["p99_ms"], "unit": "ms", "at_rate_req_s": head["rate_req_s"], "n_gpus": 1, "steps": sum(r["requests"] for r in per_rate),
+           "warmup": len(shells), "ms_per_step": head["p50_ms"], "higher_is_better": False, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "config4: 16 hex/prism shells (1e4..2e6 cells) moving rigidly over a resident "
                                   "256^3 jittered hex background", "slots": args.slots,
