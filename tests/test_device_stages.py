@@ -39,7 +39,7 @@ def test_device_adt_batch_matches_oracle():
     L.tioga_amd_device_build_adts.restype = C.c_int
     O = oracle_lib()
     rng = np.random.default_rng(7)
-    sizes = [300000, 70000, 5000, 129, 128, 1, 0, 2, 66000]
+    sizes = [300000, 70000, 5000, 129, 128, 1, 0, 2, 66000, 512, 513, 2048, 2049, 17, 16, 9000]
     ebs = [boxes(n, rng) for n in sizes]
     for batch in ([0], list(range(len(sizes)))):
         ns = np.array([sizes[t] for t in batch], np.int32)

@@ -18,8 +18,9 @@
 //     thread applying the loop control (hoare_stop, median_control in
 //     tg_common.h); the host reads the loop state every few rounds;
 //   * medium segments get one workgroup each, which runs the same rounds;
-//   * segments of at most SMALL elements are finished, whole subtree, by one
-//     thread each with the sequential quicksort, after the last level.
+//   * segments of at most WAVEMAX elements get one wave each;
+//   * segments of at most SUB elements are finished, whole subtree, by one
+//     wave each in LDS after the last level (k_adt_sub).
 // The node box is the reference's sequential TIOGA_MIN / TIOGA_MAX fold,
 // whose result is the value at the last position of the minimal (maximal)
 // class under < (so -0.0 and +0.0 tie and the later one wins); the parallel
@@ -38,7 +39,7 @@ namespace {
 constexpr int BT = 512;        // threads per workgroup for long segments
 constexpr int PER = 8;         // consecutive positions per thread per chunk
 constexpr int CH = BT * PER;   // positions per chunk of a multi-workgroup round
-constexpr int SMALL = 128;     // segments at most this long: one thread
+constexpr int WAVEMAX = 2048;  // segments at most this long (and above SUB): one wave
 constexpr int BIGMIN = 1 << 16;  // segments longer than this: many workgroups
 constexpr int ROUNDS = 8;      // multi-workgroup rounds between host checks
 
@@ -293,43 +294,289 @@ __global__ void __launch_bounds__(BT) k_adt_level(const Seg *__restrict__ segs, 
   if (t == 0) write_node(s, ix[(s.nav + 1) / 2 - 1], r, adtInt, adtReal);
 }
 
-// Whole subtrees of short segments, one thread each, with the sequential
-// quicksort and an explicit stack (buildADTrecursion.c:24-116).
-__global__ void k_adt_small(int n, const Seg *__restrict__ segs, int *ea, double *X,
-                            const double *__restrict__ ebox, int *__restrict__ adtInt, double *__restrict__ adtReal)
+// ---------------------------------------------------------------------------
+// Wave-level forms: one wave per segment, no workgroup barriers.  Lanes of a
+// wave exchange data through memory (LDS or global), so every hand-over is
+// fenced at workgroup scope around a wave barrier.
+
+__device__ __forceinline__ void wave_sync()
 {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  Seg stk[12];  // depth of a subtree of at most SMALL elements, plus one
-  int sp = 0;
-  if (segs[k].nav > 0) stk[sp++] = segs[k];
-  while (sp > 0) {
-    Seg s = stk[--sp];
-    int *ix = ea + s.base + s.start;
-    const double *eb = ebox + 6 * (size_t)s.base;
-    if (s.nav == 1) {
-      double r[6];
-      for (int j = 0; j < 6; j++) r[j] = eb[6 * (size_t)ix[0] + j];
-      write_node(s, ix[0], r, adtInt, adtReal);
-      continue;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// median_select on X[0..n) / IX[0..n) by one wave; A and B are n-int lists.
+// The rounds of wg_median with ballots in place of workgroup scans: lane l of
+// a 64-position chunk holds the chunk's l-th position from each end, so the
+// ballot prefix keeps A ascending and B descending.  Every lane computes the
+// loop control from the same values.
+__device__ void wave_median(int *IX, double *X, int n, int *A, int *B, int *err)
+{
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = (1ull << lane) - 1;
+  auto swap = [&](int a, int b) {  // 1-based positions
+    double tx = X[a - 1];
+    X[a - 1] = X[b - 1];
+    X[b - 1] = tx;
+    int ti = IX[a - 1];
+    IX[a - 1] = IX[b - 1];
+    IX[b - 1] = ti;
+  };
+  if (n < 3) {
+    if (lane == 0 && n == 2 && X[1] < X[0]) swap(1, 2);
+    wave_sync();
+    return;
+  }
+  int lo = 1, hi = n, st = 0;
+  // every round narrows [lo, hi], so n + 2 rounds bound the loop; the cap
+  // only guards against a broken invariant (reported, never expected)
+  for (int round = 0;; round++) {
+    if (round > n + 2) {
+      if (lane == 0) atomicAdd(err, 1);
+      break;
     }
-    double *x = X + s.base + s.start;
+    const double pv = median_pivot(X - 1, lo, hi);
+    const int len = hi - lo + 1;
+    int nA = 0, nB = 0;
+    for (int base = 0; base < len; base += 64) {
+      const int off = base + lane;
+      const bool fa = off < len && X[lo + off - 1] >= pv;
+      const bool fb = off < len && X[hi - off - 1] <= pv;
+      const unsigned long long ma = __ballot(fa), mb = __ballot(fb);
+      if (fa) A[nA + __popcll(ma & lt)] = lo + off;
+      if (fb) B[nB + __popcll(mb & lt)] = hi - off;
+      nA += __popcll(ma);
+      nB += __popcll(mb);
+    }
+    wave_sync();
+    // A[m] - B[m] increases with m: K is the number of leading pairs with
+    // A[m] < B[m]
+    const int mab = nA < nB ? nA : nB;
+    int K = 0;
+    for (int base = 0; base < mab; base += 64) {
+      const int m = base + lane;
+      const int c = __popcll(__ballot(m < mab && A[m] < B[m]));
+      K += c;
+      if (c < 64) break;
+    }
+    for (int m = lane; m < K; m += 64) swap(A[m], B[m]);
+    wave_sync();
+    int i, j;
+    hoare_stop(K, nA, nB, K < nA ? A[K] : 0, K > 0 ? A[K - 1] : 0, K < nB ? B[K] : 0, K > 0 ? B[K - 1] : 0, A[0], B[0],
+               i, j);
+    st = median_control(n, i, j, lo, hi);
+    wave_sync();  // the lists are rewritten by the next round
+    if (st != 0) break;
+  }
+  if (lane == 0 && st == 2) {
+    const int half = n / 2, halfp1 = half + 1;
+    if (n == 2 * half) {
+      if (X[halfp1 - 1] < X[half - 1]) swap(half, halfp1);
+    } else if (X[lo - 1] > X[hi - 1]) {
+      swap(lo, hi);
+    }
+  }
+  wave_sync();
+}
+
+// node box of the segment's elements ix[0..nav) by one wave (the fold rule
+// of wg_box); every lane returns the six bounds
+__device__ void wave_box(const int *ix, int nav, const double *eb, double *r)
+{
+  const int lane = threadIdx.x & 63;
+  KP m[6];
+  for (int j = 0; j < 3; j++) {
+    m[j] = KP{~0ull, -1};
+    m[j + 3] = KP{0ull, -1};
+  }
+  for (int i = lane; i < nav; i += 64) {
+    const double *b = eb + 6 * (size_t)ix[i];
+    for (int j = 0; j < 3; j++) {
+      m[j] = kp_min(m[j], KP{ckey(b[j]), i});
+      m[j + 3] = kp_max(m[j + 3], KP{ckey(b[j + 3]), i});
+    }
+  }
+  for (int j = 0; j < 6; j++) {
+#pragma unroll
+    for (int q = 32; q > 0; q >>= 1) {
+      KP o{__shfl_xor(m[j].k, q), __shfl_xor(m[j].p, q)};
+      m[j] = j < 3 ? kp_min(m[j], o) : kp_max(m[j], o);
+    }
+    r[j] = eb[6 * (size_t)ix[m[j].p] + j];
+  }
+}
+
+constexpr int WAVES = 4;  // waves per workgroup of the wave-level kernels
+
+// Segments of SUB < nav <= WAVEMAX elements, one level: a wave each, in
+// global memory.
+__global__ void __launch_bounds__(64 * WAVES) k_adt_wave(int n, const Seg *__restrict__ segs, int *ea, double *X,
+                                                         int *A, int *B, const double *__restrict__ ebox,
+                                                         int *__restrict__ adtInt, double *__restrict__ adtReal,
+                                                         int *err)
+{
+  const int k = blockIdx.x * WAVES + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (k >= n) return;  // whole waves leave; no workgroup barrier follows
+  const Seg s = segs[k];
+  const size_t g = (size_t)s.base + s.start;
+  const double *eb = ebox + 6 * (size_t)s.base;
+  const int dim = s.level % 6;
+  int *ix = ea + g;
+  double *x = X + g;
+  for (int i = lane; i < s.nav; i += 64) x[i] = eb[6 * (size_t)ix[i] + dim];
+  wave_sync();
+  wave_median(ix, x, s.nav, A + g, B + g, err);
+  double r[6];
+  wave_box(ix, s.nav, eb, r);
+  if (lane == 0) write_node(s, ix[(s.nav + 1) / 2 - 1], r, adtInt, adtReal);
+}
+
+// Whole subtrees of segments of at most SUB elements, one wave each, with the
+// segment's element ids and keys in LDS (buildADTrecursion.c:24-116).  The
+// wave takes the subtree's segments longer than LANE_MAX in preorder with
+// wave_median; their children of at most LANE_MAX elements (8 or more each,
+// so at most SUB / 8 = 64 of them) are then finished by one lane each with
+// the sequential quicksort.  Segments inside a subtree are packed into 64
+// bits relative to its root (start, nav, node, parent + 1: 10 bits each;
+// side: 2; level below the root: 5).
+constexpr int SUB = 512;
+constexpr int LANE_MAX = 16;
+constexpr int WSTK = 16;  // wave stack: pending right halves of segments > LANE_MAX, at most log2(SUB) deep
+constexpr int LSTK = 8;   // lane stack: at most log2(LANE_MAX) + 1 entries
+static_assert(SUB < 1024 && SUB / 8 <= 64, "subtree fields are 10 bits; at most one lane segment per lane");
+static_assert(SUB >= 64 * LSTK, "lane stacks live in the A and B lists");
+
+struct LSeg {
+  int start, nav, me, parent, side, dl;
+};
+__device__ __forceinline__ unsigned long long lpack(const LSeg &q)
+{
+  return (unsigned long long)q.start | ((unsigned long long)q.nav << 10) | ((unsigned long long)q.me << 20) |
+         ((unsigned long long)(q.parent + 1) << 30) | ((unsigned long long)q.side << 40) |
+         ((unsigned long long)q.dl << 42);
+}
+__device__ __forceinline__ LSeg lunpack(unsigned long long v)
+{
+  return LSeg{(int)(v & 1023), (int)((v >> 10) & 1023), (int)((v >> 20) & 1023), (int)((v >> 30) & 1023) - 1,
+              (int)((v >> 40) & 3), (int)((v >> 42) & 31)};
+}
+// the segment q of the subtree under root as a tree segment
+__device__ __forceinline__ Seg lglobal(const Seg &root, const LSeg &q)
+{
+  Seg s = root;
+  s.start = root.start + q.start;
+  s.nav = q.nav;
+  s.me = root.me + q.me;
+  if (q.parent >= 0) {
+    s.parent = root.me + q.parent;
+    s.side = q.side;
+  }
+  s.level = root.level + q.dl;
+  return s;
+}
+
+__global__ void __launch_bounds__(64 * WAVES) k_adt_sub(int n, const Seg *__restrict__ segs,
+                                                        const int *__restrict__ ea, const double *__restrict__ ebox,
+                                                        int *__restrict__ adtInt, double *__restrict__ adtReal,
+                                                        int *err)
+{
+  __shared__ double sX[WAVES][SUB];
+  __shared__ int sIX[WAVES][SUB];
+  __shared__ unsigned long long sAB[WAVES][SUB];  // A and B lists (SUB ints each); afterwards the lane stacks
+  __shared__ unsigned long long sW[WAVES][WSTK], sL[WAVES][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int k = blockIdx.x * WAVES + w;
+  if (k >= n) return;  // whole waves leave; no workgroup barrier follows
+  const Seg root = segs[k];
+  const double *eb = ebox + 6 * (size_t)root.base;
+  double *X = sX[w];
+  int *IX = sIX[w], *A = (int *)sAB[w], *B = A + SUB;
+  const int *src = ea + (size_t)root.base + root.start;
+  for (int i = lane; i < root.nav; i += 64) IX[i] = src[i];
+  int nw = 0, nl = 0;  // wave stack depth and lane list length (uniform)
+  const LSeg r0{0, root.nav, 0, -1, 0, 0};
+  if (root.nav > LANE_MAX) {
+    if (lane == 0) sW[w][0] = lpack(r0);
+    nw = 1;
+  } else {
+    if (lane == 0) sL[w][0] = lpack(r0);
+    nl = 1;
+  }
+  wave_sync();
+  while (nw > 0) {
+    const LSeg q = lunpack(sW[w][--nw]);
+    const Seg s = lglobal(root, q);
+    int *ix = IX + q.start;
+    double *x = X + q.start;
     const int dim = s.level % 6;
-    for (int i = 0; i < s.nav; i++) x[i] = eb[6 * (size_t)ix[i] + dim];
-    median_select(ix, x, s.nav);
-    const int nleft = (s.nav + 1) / 2;
-    double r[6] = {BIGVALUE, BIGVALUE, BIGVALUE, -BIGVALUE, -BIGVALUE, -BIGVALUE};
-    for (int i = 0; i < s.nav; i++) {
-      const double *b = eb + 6 * (size_t)ix[i];
-      for (int j = 0; j < 3; j++) {
-        r[j] = dmin(r[j], b[j]);
-        r[j + 3] = dmax(r[j + 3], b[j + 3]);
-      }
+    for (int i = lane; i < q.nav; i += 64) x[i] = eb[6 * (size_t)ix[i] + dim];
+    wave_sync();
+    wave_median(ix, x, q.nav, A, B, err);
+    double r[6];
+    wave_box(ix, q.nav, eb, r);
+    const int nleft = (q.nav + 1) / 2;
+    if (lane == 0) write_node(s, ix[nleft - 1], r, adtInt, adtReal);
+    // children: nleft - 1 and nav - nleft elements, both at least 8
+    const LSeg cl{q.start, nleft - 1, q.me + 1, q.me, 1, q.dl + 1};
+    const LSeg cr{q.start + nleft, q.nav - nleft, q.me + nleft, q.me, 2, q.dl + 1};
+    const int dw = (cl.nav > LANE_MAX) + (cr.nav > LANE_MAX);
+    if (nw + dw > WSTK || nl + 2 - dw > 64) {  // bounded by the sizes; reported, never expected
+      if (lane == 0) atomicAdd(err, 1);
+      return;
     }
-    write_node(s, ix[nleft - 1], r, adtInt, adtReal);
-    if (s.nav - nleft > 0)
-      stk[sp++] = Seg{s.base, s.start + nleft, s.nav - nleft, s.me + nleft, s.me, 2, s.level + 1, s.tree};
-    if (nleft > 1) stk[sp++] = Seg{s.base, s.start, nleft - 1, s.me + 1, s.me, 1, s.level + 1, s.tree};
+    if (cr.nav > LANE_MAX) {
+      if (lane == 0) sW[w][nw] = lpack(cr);
+      nw++;
+    } else {
+      if (lane == 0) sL[w][nl] = lpack(cr);
+      nl++;
+    }
+    if (cl.nav > LANE_MAX) {
+      if (lane == 0) sW[w][nw] = lpack(cl);
+      nw++;
+    } else {
+      if (lane == 0) sL[w][nl] = lpack(cl);
+      nl++;
+    }
+    wave_sync();
+  }
+  // lane phase: lane l finishes list entry l with the sequential quicksort
+  if (lane < nl) {
+    unsigned long long *stk = sAB[w] + lane * LSTK;
+    int sp = 0;
+    stk[sp++] = sL[w][lane];
+    while (sp > 0) {
+      const LSeg q = lunpack(stk[--sp]);
+      const Seg s = lglobal(root, q);
+      int *ix = IX + q.start;
+      if (q.nav == 1) {
+        double r[6];
+        for (int j = 0; j < 6; j++) r[j] = eb[6 * (size_t)ix[0] + j];
+        write_node(s, ix[0], r, adtInt, adtReal);
+        continue;
+      }
+      double *x = X + q.start;
+      const int dim = s.level % 6;
+      for (int i = 0; i < q.nav; i++) x[i] = eb[6 * (size_t)ix[i] + dim];
+      median_select(ix, x, q.nav);
+      const int nleft = (q.nav + 1) / 2;
+      double r[6] = {BIGVALUE, BIGVALUE, BIGVALUE, -BIGVALUE, -BIGVALUE, -BIGVALUE};
+      for (int i = 0; i < q.nav; i++) {
+        const double *b = eb + 6 * (size_t)ix[i];
+        for (int j = 0; j < 3; j++) {
+          r[j] = dmin(r[j], b[j]);
+          r[j + 3] = dmax(r[j + 3], b[j + 3]);
+        }
+      }
+      write_node(s, ix[nleft - 1], r, adtInt, adtReal);
+      if (sp + 2 > LSTK) {  // bounded by the sizes; reported, never expected
+        atomicAdd(err, 1);
+        break;
+      }
+      if (q.nav - nleft > 0) stk[sp++] = lpack(LSeg{q.start + nleft, q.nav - nleft, q.me + nleft, q.me, 2, q.dl + 1});
+      if (nleft > 1) stk[sp++] = lpack(LSeg{q.start, nleft - 1, q.me + 1, q.me, 1, q.dl + 1});
+    }
   }
 }
 
@@ -723,7 +970,7 @@ void build_adt_batch(AdtTree *trees, int ntree, AdtScratch &w, hipStream_t s)
     std::vector<Seg> here;
     nxt.clear();
     for (const Seg &q : cur) {
-      if (q.nav <= SMALL) {
+      if (q.nav <= SUB) {
         small.push_back(q);
         continue;
       }
@@ -744,13 +991,18 @@ void build_adt_batch(AdtTree *trees, int ntree, AdtScratch &w, hipStream_t s)
   TG_HIP(hipMemcpyAsync(w.tb.p, base.data(), sizeof(int) * (ntree + 1), hipMemcpyHostToDevice, s));
   TG_HIP(hipMemsetAsync(w.err.p, 0, 2 * sizeof(int), s));
   k_iota_trees<<<nblocks(N, 256), 256, 0, s>>>(N, w.tb.p, ntree, w.ea.p);
-  // upload the level lists: per level the medium segments, then the long ones
+  // upload the level lists: per level the wave segments, the workgroup
+  // segments, then the long ones
   std::vector<Seg> all;
-  std::vector<size_t> med0, nmed, big0, nbigv;
+  std::vector<size_t> wav0, nwav, med0, nmed, big0, nbigv;
   for (auto &L : lev) {
+    wav0.push_back(all.size());
+    for (const Seg &q : L)
+      if (q.nav <= WAVEMAX) all.push_back(q);
+    nwav.push_back(all.size() - wav0.back());
     med0.push_back(all.size());
     for (const Seg &q : L)
-      if (q.nav <= BIGMIN) all.push_back(q);
+      if (q.nav > WAVEMAX && q.nav <= BIGMIN) all.push_back(q);
     nmed.push_back(all.size() - med0.back());
     big0.push_back(all.size());
     for (const Seg &q : L)
@@ -763,6 +1015,9 @@ void build_adt_batch(AdtTree *trees, int ntree, AdtScratch &w, hipStream_t s)
   TG_HIP(hipMemcpyAsync(w.segs.p, all.data(), sizeof(Seg) * all.size(), hipMemcpyHostToDevice, s));
   const Seg *segs = (const Seg *)w.segs.p;
   for (size_t l = 0; l < lev.size(); l++) {
+    if (nwav[l])
+      k_adt_wave<<<nblocks((long long)nwav[l], WAVES), 64 * WAVES, 0, s>>>(
+          (int)nwav[l], segs + wav0[l], w.ea.p, w.X.p, w.A.p, w.B.p, ebox, adtInt, adtReal, w.err.p);
     if (nmed[l])
       k_adt_level<<<(unsigned)nmed[l], BT, 0, s>>>(segs + med0[l], w.ea.p, w.X.p, w.A.p, w.B.p, ebox, adtInt, adtReal,
                                                   w.err.p);
@@ -813,8 +1068,8 @@ void build_adt_batch(AdtTree *trees, int ntree, AdtScratch &w, hipStream_t s)
     k_big_node<<<nblocks(nbig, 64), 64, 0, s>>>(nbig, bs, w.ea.p, ebox, w.bp.p, adtInt, adtReal);
   }
   if (!small.empty())
-    k_adt_small<<<nblocks((long long)small.size(), 64), 64, 0, s>>>((int)small.size(), segs + nsmall0, w.ea.p, w.X.p,
-                                                                   ebox, adtInt, adtReal);
+    k_adt_sub<<<nblocks((long long)small.size(), WAVES), 64 * WAVES, 0, s>>>((int)small.size(), segs + nsmall0,
+                                                                               w.ea.p, ebox, adtInt, adtReal, w.err.p);
   k_adt_slot3<<<nblocks(N, 256), 256, 0, s>>>(N, w.tb.p, ntree, adtInt);
   int err = 0;
   TG_HIP(hipMemcpyAsync(&err, w.err.p, sizeof(int), hipMemcpyDeviceToHost, s));
