@@ -92,6 +92,16 @@ def test_device_dedupe_batch_matches_oracle():
     # one set of 500 identical points: a node that cannot be split
     sets.append((np.ascontiguousarray(np.full((500, 3), 0.25)), np.ones(500, np.int32), rng.uniform(0, 1, 500)))
     sizes.append(500)
+    # sets at the subtree threshold, and a graded one (geometric spacing
+    # towards a wall, duplicated) whose octree is deep and lopsided
+    for m in (1024, 1025, 3000):
+        sets.append(points(m, rng))
+        sizes.append(m)
+    g = 0.5 * 0.8 ** np.arange(450)
+    xg = np.stack([g, 0.3 * g, np.full_like(g, 0.1)], 1)
+    xg = np.ascontiguousarray(np.concatenate([xg, xg[::3]]))
+    sets.append((xg, np.ones(xg.shape[0], np.int32), rng.uniform(0, 1, xg.shape[0])))
+    sizes.append(xg.shape[0])
     for batch in ([0], list(range(len(sizes)))):
         ns = np.array([sizes[k] for k in batch], np.int32)
         x = np.ascontiguousarray(np.concatenate([sets[k][0] for k in batch]).reshape(-1))

@@ -15,6 +15,8 @@
 // bounding box, octant counts, then a stable reversed scatter of the node's
 // points into its children.  Nodes of more than BIG points are split across
 // many workgroups instead (chunked box, counts, per-octant scans, scatter).
+// Nodes of at most SUBD points are set aside and finished, whole subtree, by
+// one wave each with the points staged in LDS (k_oct_sub).
 // Several point sets (the query sets of several blocks) can be deduplicated
 // together, one root each.  Leaves are queued and finished one per thread.
 //
@@ -34,6 +36,7 @@ constexpr int BT = 256;
 constexpr int LEAF = 20;  // utils.c:633: split above 20 points
 constexpr int BIG = 1 << 16;  // nodes above this: many workgroups
 constexpr int CHB = 16 * BT;  // points per chunk of a big node
+constexpr int SUBD = 1024;  // nodes of at most this many points: whole subtree by one wave
 
 struct Node {
   int start, len;
@@ -52,7 +55,8 @@ __device__ __forceinline__ int octant(const double *p, const double mid[3], cons
 
 __global__ void __launch_bounds__(BT) k_oct_level(int nnode, const Node *__restrict__ in, Node *__restrict__ out,
                                                   int *__restrict__ nout, Node *__restrict__ leaves,
-                                                  int *__restrict__ nleaves, int *ea, int *tmp,
+                                                  int *__restrict__ nleaves, Node *__restrict__ subs,
+                                                  int *__restrict__ nsubs, int *ea, int *tmp,
                                                   const double *__restrict__ x)
 {
   __shared__ double red[6][BT / 64];
@@ -137,9 +141,138 @@ __global__ void __launch_bounds__(BT) k_oct_level(int nnode, const Node *__restr
   for (int i = t; i < nd.len; i += BT) ix[i] = tmp[nd.start + i];
   if (t < 8 && cnt[t] > 0) {
     Node c{nd.start + cft[t], cnt[t]};
-    if (c.len > LEAF) out[atomicAdd(nout, 1)] = c;
+    if (c.len > SUBD) out[atomicAdd(nout, 1)] = c;
+    else if (c.len > LEAF) subs[atomicAdd(nsubs, 1)] = c;
     else leaves[atomicAdd(nleaves, 1)] = c;
   }
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Whole subtrees of nodes of at most SUBD points, one wave each.  The points'
+// coordinates (by local id) and the node order (local ids) live in LDS; a
+// node's box, octant counts and stable reversed scatter are wave ballots, as
+// in k_oct_level.  Non-leaf children are pushed largest first, so the
+// smallest is taken next: a pending entry is at least as large as the child
+// being split below it, and the stack stays below 7 log8(SUBD / LEAF) + 8
+// entries.  Leaves go to the global leaf list; the final node order is
+// written back to ea for them.
+constexpr int DW = 2;     // waves per workgroup of k_oct_sub
+constexpr int DSTK = 32;  // stack entries: start | len << 16
+static_assert(SUBD <= 65535, "local ids are 16 bits");
+
+__global__ void __launch_bounds__(64 * DW) k_oct_sub(int n, const Node *__restrict__ roots, int *ea,
+                                                     const double *__restrict__ x, Node *__restrict__ leaves,
+                                                     int *__restrict__ nleaves, int *__restrict__ err)
+{
+  __shared__ double sx[DW][3][SUBD];
+  __shared__ int sg[DW][SUBD];
+  __shared__ unsigned short sP[DW][SUBD], sT[DW][SUBD];
+  __shared__ unsigned sS[DW][DSTK];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int k = blockIdx.x * DW + w;
+  if (k >= n) return;  // whole waves leave; no workgroup barrier follows
+  const Node root = roots[k];
+  const unsigned long long lt = (1ull << lane) - 1;
+  unsigned short *P = sP[w], *T = sT[w];
+  for (int i = lane; i < root.len; i += 64) {
+    const int g = ea[root.start + i];
+    sg[w][i] = g;
+    for (int j = 0; j < 3; j++) sx[w][j][i] = x[3 * (size_t)g + j];
+    P[i] = (unsigned short)i;
+  }
+  int sp = 0;
+  if (lane == 0) sS[w][0] = (unsigned)root.len << 16;
+  sp = 1;
+  wave_sync();
+  while (sp > 0) {
+    const unsigned e = sS[w][--sp];
+    const int cs = (int)(e & 0xffff), cl = (int)(e >> 16);
+    // bounding box (utils.c:635-645)
+    double lo[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, hi[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
+    for (int i = lane; i < cl; i += 64) {
+      const int l = P[cs + i];
+      for (int j = 0; j < 3; j++) {
+        lo[j] = dmin(lo[j], sx[w][j][l]);
+        hi[j] = dmax(hi[j], sx[w][j][l]);
+      }
+    }
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int q = 32; q > 0; q >>= 1) {
+        lo[j] = dmin(lo[j], __shfl_xor(lo[j], q));
+        hi[j] = dmax(hi[j], __shfl_xor(hi[j], q));
+      }
+    double mid[3], dx[3];
+    for (int j = 0; j < 3; j++) {
+      mid[j] = (hi[j] + lo[j]) * 0.5;
+      dx[j] = (hi[j] - lo[j]) * 0.5 + TOL;
+    }
+    auto oct_of = [&](int i) {
+      if (i >= cl) return -1;
+      const int l = P[cs + i];
+      const double p[3] = {sx[w][0][l], sx[w][1][l], sx[w][2][l]};
+      return octant(p, mid, dx);
+    };
+    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int base = 0; base < cl; base += 64) {
+      const int o = oct_of(base + lane);
+      for (int q = 0; q < 8; q++) cnt[q] += __popcll(__ballot(o == q));
+    }
+    bool one = false;
+    for (int q = 0; q < 8; q++) one = one || (cnt[q] == cl);
+    if (one) {  // every point in one octant: the node is compared as a whole
+      if (lane == 0) leaves[atomicAdd(nleaves, 1)] = Node{root.start + cs, cl};
+      continue;
+    }
+    int cft[9];
+    cft[0] = 0;
+    for (int q = 0; q < 8; q++) cft[q + 1] = cft[q] + cnt[q];
+    // stable scatter, each octant in reverse arrival order (utils.c:666-679)
+    int run[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int base = 0; base < cl; base += 64) {
+      const int i = base + lane;
+      const int o = oct_of(i);
+      int rank = 0;
+      for (int q = 0; q < 8; q++) {
+        const unsigned long long m = __ballot(o == q);
+        if (o == q) rank = run[q] + __popcll(m & lt);
+        run[q] += __popcll(m);
+      }
+      if (o >= 0) T[cs + cft[o] + (cnt[o] - 1 - rank)] = P[cs + i];
+    }
+    wave_sync();
+    for (int i = lane; i < cl; i += 64) P[cs + i] = T[cs + i];
+    // children: leaves out, the others pushed largest first
+    int ord[8], nn = 0;
+    for (int q = 0; q < 8; q++) {
+      if (cnt[q] == 0) continue;
+      if (cnt[q] <= LEAF) {
+        if (lane == 0) leaves[atomicAdd(nleaves, 1)] = Node{root.start + cs + cft[q], cnt[q]};
+        continue;
+      }
+      int m = nn++;
+      while (m > 0 && cnt[ord[m - 1]] < cnt[q]) {  // descending size, stable
+        ord[m] = ord[m - 1];
+        m--;
+      }
+      ord[m] = q;
+    }
+    if (sp + nn > DSTK) {  // bounded by the sizes; reported, never expected
+      if (lane == 0) atomicAdd(err, 1);
+      return;
+    }
+    if (lane == 0)
+      for (int m = 0; m < nn; m++) sS[w][sp + m] = (unsigned)(cs + cft[ord[m]]) | ((unsigned)cnt[ord[m]] << 16);
+    sp += nn;
+    wave_sync();
+  }
+  for (int i = lane; i < root.len; i += 64) ea[root.start + i] = sg[w][P[i]];
 }
 
 // all pairs of a leaf in list order (utils.c:691-716)
@@ -338,7 +471,7 @@ __global__ void __launch_bounds__(BT) k_ob_copy(const int2 *__restrict__ tab, co
 
 __global__ void k_ob_emit(int nbig, const Node *__restrict__ nodes, const BigBox *__restrict__ bb,
                           Node *__restrict__ nout, int *__restrict__ cnt, Node *__restrict__ bout,
-                          Node *__restrict__ leaves)
+                          Node *__restrict__ leaves, Node *__restrict__ subs)
 {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbig) return;
@@ -352,7 +485,8 @@ __global__ void k_ob_emit(int nbig, const Node *__restrict__ nodes, const BigBox
     if (B.cnt[q] == 0) continue;
     const Node c{nd.start + B.cft[q], B.cnt[q]};
     if (c.len > BIG) bout[atomicAdd(&cnt[2], 1)] = c;
-    else if (c.len > LEAF) nout[atomicAdd(&cnt[0], 1)] = c;
+    else if (c.len > SUBD) nout[atomicAdd(&cnt[0], 1)] = c;
+    else if (c.len > LEAF) subs[atomicAdd(&cnt[3], 1)] = c;
     else leaves[atomicAdd(&cnt[1], 1)] = c;
   }
 }
@@ -404,32 +538,35 @@ void unique_nodes_batch(DedupeSet *sets, int nset, DedupeScratch &w, hipStream_t
   w.bigA.reserve(n / BIG + nset + 2);
   w.bigB.reserve(n / BIG + nset + 2);
   w.leaves.reserve(n + 1);
-  w.counts.reserve(4);
+  w.subs.reserve(n / LEAF + nset + 2);
+  w.counts.reserve(8);
   w.sb.reserve(nset + 1);
   TG_HIP(hipMemcpyAsync(w.sb.p, base.data(), sizeof(int) * (nset + 1), hipMemcpyHostToDevice, s));
   k_iota_sets<<<nblocks(n, 256), 256, 0, s>>>(n, w.sb.p, nset, w.ea.p, itag);
   // roots
-  std::vector<Node> rn, rb, rl;
+  std::vector<Node> rn, rb, rl, rs;
   for (int k = 0; k < nset; k++) {
     const Node r{base[k], sets[k].n};
     if (r.len > BIG) rb.push_back(r);
-    else if (r.len > LEAF) rn.push_back(r);
+    else if (r.len > SUBD) rn.push_back(r);
+    else if (r.len > LEAF) rs.push_back(r);
     else if (r.len > 0) rl.push_back(r);
   }
+  if (!rs.empty()) TG_HIP(hipMemcpyAsync(w.subs.p, rs.data(), sizeof(Node) * rs.size(), hipMemcpyHostToDevice, s));
   if (!rn.empty())
     TG_HIP(hipMemcpyAsync(w.nodesA.p, rn.data(), sizeof(Node) * rn.size(), hipMemcpyHostToDevice, s));
   if (!rb.empty()) TG_HIP(hipMemcpyAsync(w.bigA.p, rb.data(), sizeof(Node) * rb.size(), hipMemcpyHostToDevice, s));
   if (!rl.empty()) TG_HIP(hipMemcpyAsync(w.leaves.p, rl.data(), sizeof(Node) * rl.size(), hipMemcpyHostToDevice, s));
-  int cnt[3] = {(int)rn.size(), (int)rl.size(), (int)rb.size()};  // normal, leaves, big
+  int cnt[4] = {(int)rn.size(), (int)rl.size(), (int)rb.size(), (int)rs.size()};  // normal, leaves, big, subtrees
   Node *nin = (Node *)w.nodesA.p, *nout = (Node *)w.nodesB.p;
   Node *bin = (Node *)w.bigA.p, *bout = (Node *)w.bigB.p;
   while (cnt[0] + cnt[2] > 0) {
     const int nnorm = cnt[0], nbig = cnt[2];
-    const int hc[3] = {0, cnt[1], 0};
+    const int hc[4] = {0, cnt[1], 0, cnt[3]};
     TG_HIP(hipMemcpyAsync(w.counts.p, hc, sizeof hc, hipMemcpyHostToDevice, s));
     if (nnorm)
-      k_oct_level<<<nnorm, BT, 0, s>>>(nnorm, nin, nout, w.counts.p, (Node *)w.leaves.p, w.counts.p + 1, w.ea.p,
-                                       w.tmp.p, x);
+      k_oct_level<<<nnorm, BT, 0, s>>>(nnorm, nin, nout, w.counts.p, (Node *)w.leaves.p, w.counts.p + 1,
+                                       (Node *)w.subs.p, w.counts.p + 3, w.ea.p, w.tmp.p, x);
     if (nbig) {
       std::vector<Node> bn(nbig);
       TG_HIP(hipMemcpyAsync(bn.data(), bin, sizeof(Node) * nbig, hipMemcpyDeviceToHost, s));
@@ -458,12 +595,25 @@ void unique_nodes_batch(DedupeSet *sets, int nset, DedupeScratch &w, hipStream_t
       k_ob_scan<<<nbig, 64, 0, s>>>(nbig, bin, w.choff.p, w.choff.p + nbig, w.cc.p, bb);
       k_ob_scatter<<<ntab, BT, 0, s>>>(w.tab.p, bin, w.choff.p, w.ea.p, w.tmp.p, x, bb, w.cc.p);
       k_ob_copy<<<ntab, BT, 0, s>>>(w.tab.p, bin, w.ea.p, w.tmp.p, bb);
-      k_ob_emit<<<nblocks(nbig, 64), 64, 0, s>>>(nbig, bin, bb, nout, w.counts.p, bout, (Node *)w.leaves.p);
+      k_ob_emit<<<nblocks(nbig, 64), 64, 0, s>>>(nbig, bin, bb, nout, w.counts.p, bout, (Node *)w.leaves.p,
+                                                 (Node *)w.subs.p);
     }
-    TG_HIP(hipMemcpyAsync(cnt, w.counts.p, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipMemcpyAsync(cnt, w.counts.p, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
     TG_HIP(hipStreamSynchronize(s));
     std::swap(nin, nout);
     std::swap(bin, bout);
+  }
+  if (cnt[3] > 0) {  // subtrees of at most SUBD points
+    const int hc[2] = {cnt[1], 0};
+    TG_HIP(hipMemcpyAsync(w.counts.p + 1, hc, sizeof(int), hipMemcpyHostToDevice, s));
+    TG_HIP(hipMemcpyAsync(w.counts.p + 4, hc + 1, sizeof(int), hipMemcpyHostToDevice, s));
+    k_oct_sub<<<nblocks(cnt[3], DW), 64 * DW, 0, s>>>(cnt[3], (const Node *)w.subs.p, w.ea.p, x, (Node *)w.leaves.p,
+                                                      w.counts.p + 1, w.counts.p + 4);
+    int back[4];
+    TG_HIP(hipMemcpyAsync(back, w.counts.p + 1, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    if (back[3]) throw std::runtime_error("tioga_amd: octree dedupe: subtree stack exceeded its bound");
+    cnt[1] = back[0];
   }
   if (cnt[1] > 0)
     k_oct_leaf<<<nblocks(cnt[1], 64), 64, 0, s>>>(cnt[1], (const Node *)w.leaves.p, w.ea.p, x, tag, rtag, itag);

@@ -203,7 +203,7 @@ int build_adt_device(int n, const double *ebox, int *adtInt, double *adtReal, do
 // Reusable device scratch of the coincident-point dedupe (tg_dedupe.hip).
 struct DedupeScratch {
   DBuf<int> ea, tmp, counts, sb, choff, cc, tag, itag;
-  DBuf<int2> nodesA, nodesB, bigA, bigB, leaves, tab;
+  DBuf<int2> nodesA, nodesB, bigA, bigB, leaves, tab, subs;
   DBuf<double> x, rtag;
   DBuf<unsigned long long> bk;
   DBuf<double4> bb;
@@ -229,7 +229,8 @@ void unique_nodes_device(int n, const double *x, const int *tag, double *rtag, i
 // Reusable device scratch of the donor search: per-query candidate lists
 // and the queue of queries with more candidates than a list holds.
 struct SearchScratch {
-  DBuf<int> cell, query, status, qoff, qcnt, ovf, counters, bins, blist, lj, lcount;
+  DBuf<int> cell, query, status, qoff, qcnt, ovf, counters, bins, blist, lj, lcount, qb, wb, qsid;
+  DBuf<unsigned char> sv;  // the table of searches of one launch
   DBuf<double4> lst;
   size_t cap = 0;  // capacity of the flat candidate list
 };
@@ -373,6 +374,7 @@ class Engine {
   void exchange_donors();
   void cell_iblanks(Block &b);
   void launch_search(Block &b);
+  void launch_searches();
 
   void build_search_structure(Block &b, const double *xq_host, int nq, int &nelem, int &depth, DBuf<int> &elist,
                               DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, DBuf<AdtNode> &nodes,

@@ -839,6 +839,59 @@ struct SearchView {
   const double *cellRes;
 };
 
+// Several searches (one per block, each with its own tree and query set)
+// run by one launch of each kernel.  Queries are numbered globally: search s
+// owns [qbase[s], qbase[s+1]); in k_collect its queries take the waves
+// [wbase[s], wbase[s+1]), so every wave belongs to one search.  qsid maps a
+// global query to its search.
+struct SearchSet {
+  const SearchView *sv;
+  const int *qbase, *wbase;
+  int *qsid;
+  int ns, nqt;
+};
+
+// Pointers read from the table are generic; the casts tell the compiler
+// that they address global memory (global rather than flat loads), and that
+// the table and the tree records are not written while a search runs, so
+// wave-uniform reads of them are scalar loads.
+template <class T>
+__device__ __forceinline__ T *as_global(T *p)
+{
+  return (T *)(__attribute__((address_space(1))) T *)p;
+}
+template <class T>
+using const_ptr = const __attribute__((address_space(4))) T *;
+
+__device__ __forceinline__ SearchView view_of(const SearchSet &T, int sid)
+{
+  const_ptr<SearchView> v = (const_ptr<SearchView>)T.sv + sid;
+  SearchView S;
+  S.nq = v->nq;
+  S.perm = as_global(v->perm);
+  S.xs = as_global(v->xs);
+  S.xtag = as_global(v->xtag);
+  S.donorId = as_global(v->donorId);
+  S.nodes = as_global(v->nodes);
+  for (int d = 0; d < 6; d++) S.ext[d] = v->ext[d];
+  S.conn = as_global(v->conn);
+  S.x = as_global(v->x);
+  S.cellRes = as_global(v->cellRes);
+  return S;
+}
+
+// the search owning wave wv (last s with wbase[s] <= wv)
+__device__ __forceinline__ int wave_search(const SearchSet &T, int wv)
+{
+  int lo = 0, hi = T.ns - 1;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (T.wbase[m] <= wv) lo = m;
+    else hi = m - 1;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ bool in_box(const double p[3], const double *b)
 {
   return p[0] >= b[0] - TOL && p[1] >= b[1] - TOL && p[2] >= b[2] - TOL && p[0] <= b[3] + TOL &&
@@ -925,16 +978,21 @@ struct CandOut {
 // loads.  Candidates are staged in LDS and copied out to a chunk the wave
 // allocates at the end; a lane with more than KSTAGE candidates is queued for
 // k_collect_more, which lists all of them.
-__global__ void __launch_bounds__(WAVE, 8) k_collect(SearchView S, const AdtNode *__restrict__ nodes, CandOut O)
+__global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
 {
   __shared__ int snode[STACK];
   __shared__ unsigned long long smask[STACK];
   __shared__ int stage[KSTAGE][WAVE];
   const int lane = threadIdx.x;
-  const int t = blockIdx.x * WAVE + lane;
+  const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, blockIdx.x));
+  const SearchView S = view_of(T, sid);
+  const const_ptr<AdtNode> nodes = (const_ptr<AdtNode>)S.nodes;
+  const int qb = T.qbase[sid];
+  const int t = (blockIdx.x - T.wbase[sid]) * WAVE + lane;
   int i = -1;
   if (t < S.nq) {
     i = S.perm ? S.perm[t] : t;
+    if (T.ns > 1) T.qsid[qb + i] = sid;
     if (S.xtag && S.xtag[i] != i) i = -1;
   }
   double p[3] = {0.0, 0.0, 0.0};
@@ -960,7 +1018,19 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchView S, const AdtNode
     const int node = __builtin_amdgcn_readfirstlane(snode[sp]);
     const unsigned long long m = smask[sp];
     // the whole record in one batch of scalar loads, then the tests
-    const AdtNode N = nodes[node];
+    AdtNode N;
+    {
+      const_ptr<AdtNode> r = nodes + node;
+      for (int d = 0; d < 6; d++) {
+        N.ebox[d] = r->ebox[d];
+        N.lbox[d] = r->lbox[d];
+        N.rbox[d] = r->rbox[d];
+      }
+      N.cell = r->cell;
+      N.nv = r->nv;
+      N.left = r->left;
+      N.right = r->right;
+    }
     const bool act = (m >> lane) & 1ull;
     if (act & in_box_all(p, N.ebox)) {
       if (nc < KSTAGE) stage[nc][lane] = N.cell | (N.nv << 28);
@@ -998,31 +1068,36 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchView S, const AdtNode
   if (lane == 0 && total > 0) base = atomicAdd(O.used, total);
   base = __shfl(base, 0);
   if (i < 0) return;
-  O.qcnt[i] = nc;
+  const int g = qb + i;
+  O.qcnt[g] = nc;
   if (nc > KSTAGE) {
-    O.ovf[atomicAdd(O.novf, 1)] = i;
+    O.ovf[atomicAdd(O.novf, 1)] = g;
     return;
   }
   const int off = base + pre;
-  O.qoff[i] = off;
+  O.qoff[g] = off;
   if (off + k > O.cap) return;  // flat list too small: detected by the host, which grows it and reruns
   for (int j = 0; j < k; j++) {
     O.cell[off + j] = stage[j][lane];
-    O.query[off + j] = i;
+    O.query[off + j] = g;
   }
 }
 
 // the walk of one query, listing all its candidates (queries with more than
 // KSTAGE); a grid-stride loop over the queue
-__global__ void __launch_bounds__(64) k_collect_more(SearchView S, const AdtNode *__restrict__ nodes, CandOut O)
+__global__ void __launch_bounds__(64) k_collect_more(SearchSet T, CandOut O)
 {
   __shared__ int stk[STACK][64];
   const int n = *O.novf, tid = threadIdx.x;
   for (int q = blockIdx.x * 64 + tid; q < n; q += gridDim.x * 64) {
-    const int i = O.ovf[q];
-    const int cnt = O.qcnt[i];
+    const int g = O.ovf[q];
+    const int sid = T.ns == 1 ? 0 : T.qsid[g];
+    const SearchView S = view_of(T, sid);
+    const AdtNode *nodes = S.nodes;
+    const int i = g - T.qbase[sid];
+    const int cnt = O.qcnt[g];
     const int off = atomicAdd(O.used, cnt);
-    O.qoff[i] = off;
+    O.qoff[g] = off;
     if (off + cnt > O.cap) continue;
     double p[3];
     load_xyz(S.xs, i, p);
@@ -1032,7 +1107,7 @@ __global__ void __launch_bounds__(64) k_collect_more(SearchView S, const AdtNode
       const AdtNode &N = nodes[stk[--sp][tid]];
       if (in_box_all(p, N.ebox)) {
         O.cell[off + nc] = N.cell | (N.nv << 28);
-        O.query[off + nc] = i;
+        O.query[off + nc] = g;
         nc++;
       }
       if (N.right > -1 && in_box_all(p, N.rbox)) stk[sp++][tid] = N.right;
@@ -1120,7 +1195,7 @@ struct LongQueue {
 
 // containment test of every listed candidate of one cell type
 template <int NV, int WAVES>
-__global__ void __launch_bounds__(256, WAVES) k_eval(SearchView S, CandOut O, const int *__restrict__ bcount,
+__global__ void __launch_bounds__(256, WAVES) k_eval(SearchSet T, CandOut O, const int *__restrict__ bcount,
                                                      const int *__restrict__ blist, LongQueue L)
 {
   constexpr int B = NV == 4 ? 0 : (NV == 5 ? 1 : (NV == 6 ? 2 : 3));
@@ -1130,8 +1205,10 @@ __global__ void __launch_bounds__(256, WAVES) k_eval(SearchView S, CandOut O, co
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const int j = blist[b0 + k];
     const int c = O.cell[j] & 0x0fffffff;
+    const int g = O.query[j], sid = T.ns == 1 ? 0 : T.qsid[g];
+    const SearchView S = view_of(T, sid);
     double p[3], xv[8][3], frac[8];
-    load_xyz(S.xs, O.query[j], p);
+    load_xyz(S.xs, g - T.qbase[sid], p);
     cell_vertices<NV>(S, c, xv);
     if (NV == 4) {
       nodal_weights(xv, p, frac, 4);
@@ -1157,7 +1234,7 @@ __global__ void __launch_bounds__(256, WAVES) k_eval(SearchView S, CandOut O, co
 
 // the queued solves, continued from their saved state to the end
 template <int NV>
-__global__ void __launch_bounds__(256, 4) k_eval_long(SearchView S, CandOut O, LongQueue L)
+__global__ void __launch_bounds__(256, 4) k_eval_long(SearchSet T, CandOut O, LongQueue L)
 {
   constexpr int B = NV == 5 ? 1 : (NV == 6 ? 2 : 3);
   const int n = min(L.n[B], L.cap);
@@ -1165,8 +1242,10 @@ __global__ void __launch_bounds__(256, 4) k_eval_long(SearchView S, CandOut O, L
     const int j = L.j[B * L.cap + k];
     const double4 st = L.st[B * L.cap + k];
     const int c = O.cell[j] & 0x0fffffff;
+    const int g = O.query[j], sid = T.ns == 1 ? 0 : T.qsid[g];
+    const SearchView S = view_of(T, sid);
     double p[3], xv[8][3], frac[8], f[8][3];
-    load_xyz(S.xs, O.query[j], p);
+    load_xyz(S.xs, g - T.qbase[sid], p);
     cell_vertices<NV>(S, c, xv);
     iso_coeffs(xv, p, f, NV);
     double u = st.x, v = st.y, w = st.z;
@@ -1179,12 +1258,15 @@ __global__ void __launch_bounds__(256, 4) k_eval_long(SearchView S, CandOut O, L
 }
 
 // the reference's result from the candidates' outcomes, in walk order
-__global__ void k_select(SearchView S, CandOut O)
+__global__ void k_select(SearchSet T, CandOut O)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= S.nq) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= T.nqt) return;
+  const int sid = T.ns == 1 ? 0 : T.qsid[g];
+  const SearchView S = view_of(T, sid);
+  const int i = g - T.qbase[sid];
   if (S.xtag && S.xtag[i] != i) return;
-  const int nc = O.qcnt[i], off = O.qoff[i];
+  const int nc = O.qcnt[g], off = O.qoff[g];
   int res = -1;
   // a list that did not fit was not stored; the host sees the overflow in
   // the used count and searches again with a larger list
@@ -1608,16 +1690,34 @@ void fill(T *p, size_t n, T v, hipStream_t s)
   if (n) k_fill<T><<<nb((long long)n), BS, 0, s>>>(n, p, v);
 }
 
-// launches the donor search over S (fast wave-cooperative path, then the
-// queries that overflowed it)
-void run_search(const SearchView &S, const AdtNode *nodes, SearchScratch &w, hipStream_t s, bool check)
+// launches the donor searches S[0..ns) together (fast wave-cooperative
+// path, then the queries that overflowed it); with check, the host reads
+// the candidate count after the walk and walks again with a larger list
+// when it did not fit, so no later kernel reads a list that overflowed
+void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bool check)
 {
-  if (S.nq <= 0) return;
-  w.qoff.reserve(S.nq);
-  w.qcnt.reserve(S.nq);
-  w.ovf.reserve(S.nq);
+  std::vector<int> qb(ns + 1, 0), wb(ns + 1, 0);
+  for (int k = 0; k < ns; k++) {
+    if (S[k].nq < 0) throw std::runtime_error("tioga_amd: negative query count");
+    qb[k + 1] = qb[k] + S[k].nq;
+    wb[k + 1] = wb[k] + (S[k].nq + WAVE - 1) / WAVE;
+    if (qb[k + 1] < qb[k]) throw std::runtime_error("tioga_amd: too many queries in one search launch");
+  }
+  const int nqt = qb[ns], nwave = wb[ns];
+  if (nqt <= 0) return;
+  w.sv.reserve(sizeof(SearchView) * ns);
+  w.qb.reserve(ns + 1);
+  w.wb.reserve(ns + 1);
+  w.qsid.reserve(nqt);
+  TG_HIP(hipMemcpyAsync(w.sv.p, S, sizeof(SearchView) * ns, hipMemcpyHostToDevice, s));
+  TG_HIP(hipMemcpyAsync(w.qb.p, qb.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
+  TG_HIP(hipMemcpyAsync(w.wb.p, wb.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
+  const SearchSet T{(const SearchView *)w.sv.p, w.qb.p, w.wb.p, w.qsid.p, ns, nqt};
+  w.qoff.reserve(nqt);
+  w.qcnt.reserve(nqt);
+  w.ovf.reserve(nqt);
   w.counters.reserve(2);
-  if (w.cap == 0) w.cap = std::max<size_t>(4 * (size_t)S.nq, 1 << 20);
+  if (w.cap == 0) w.cap = std::max<size_t>(4 * (size_t)nqt, 1 << 20);
   for (;;) {
     w.cell.reserve(w.cap);
     w.query.reserve(w.cap);
@@ -1625,8 +1725,8 @@ void run_search(const SearchView &S, const AdtNode *nodes, SearchScratch &w, hip
     CandOut O{w.cell.p, w.query.p, w.status.p, w.qoff.p, w.qcnt.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
               w.ovf.p, w.counters.p + 1};
     fill<int>(w.counters.p, 2, 0, s);
-    k_collect<<<nb(S.nq, WAVE), WAVE, 0, s>>>(S, nodes, O);
-    k_collect_more<<<1024, 64, 0, s>>>(S, nodes, O);
+    k_collect<<<nwave, WAVE, 0, s>>>(T, O);
+    k_collect_more<<<1024, 64, 0, s>>>(T, O);
     if (check) {
       int used = 0;
       TG_HIP(hipMemcpyAsync(&used, w.counters.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1647,14 +1747,14 @@ void run_search(const SearchView &S, const AdtNode *nodes, SearchScratch &w, hip
     w.lcount.reserve(4);
     fill<int>(w.lcount.p, 4, 0, s);
     LongQueue L{w.lj.p, w.lst.p, w.lcount.p, lcap};
-    k_eval<4, 8><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p, L);
-    k_eval<5, 4><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p, L);
-    k_eval<6, 4><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p, L);
-    k_eval<8, 4><<<2048, 256, 0, s>>>(S, O, w.bins.p, w.blist.p, L);
-    k_eval_long<5><<<1024, 256, 0, s>>>(S, O, L);
-    k_eval_long<6><<<1024, 256, 0, s>>>(S, O, L);
-    k_eval_long<8><<<1024, 256, 0, s>>>(S, O, L);
-    k_select<<<nb(S.nq), BS, 0, s>>>(S, O);
+    k_eval<4, 8><<<2048, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    k_eval<5, 4><<<2048, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    k_eval<6, 4><<<2048, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    k_eval<8, 4><<<2048, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    k_eval_long<5><<<1024, 256, 0, s>>>(T, O, L);
+    k_eval_long<6><<<1024, 256, 0, s>>>(T, O, L);
+    k_eval_long<8><<<1024, 256, 0, s>>>(T, O, L);
+    k_select<<<nb(nqt), BS, 0, s>>>(T, O);
     return;
   }
 }
@@ -2747,7 +2847,7 @@ void Engine::launch_query_set(QuerySet &q)
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
-  run_search(S, q.nodes.p, sw_, s_, !timing_);
+  run_search(&S, 1, sw_, s_, !timing_);
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
 }
 
@@ -2816,8 +2916,55 @@ void Engine::launch_search(Block &b)
   S.conn = b.conn.p;
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
-  run_search(S, b.nodes.p, sw_, s_, true);
+  run_search(&S, 1, sw_, s_, true);
   k_copy_dup<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.xtag.p, b.donorId.p);
+}
+
+// every block's donor search in one launch of each search kernel: each
+// block's queries in Morton order within its own range of one buffer
+void Engine::launch_searches()
+{
+  std::vector<Block *> act;
+  size_t tot = 0;
+  for (Block *b : blocks_) {
+    if (b->nsearch == 0) continue;
+    if (b->nelem == 0) {
+      fill<int>(b->donorId.p, b->nsearch, -1, s_);
+      continue;
+    }
+    act.push_back(b);
+    tot += b->nsearch;
+  }
+  if (act.empty()) return;
+  if (tot > 0x7fffffff) throw std::runtime_error("tioga_amd: too many queries in one search launch");
+  mk0_.reserve(tot);
+  mk1_.reserve(tot);
+  mv0_.reserve(tot);
+  mv1_.reserve(tot);
+  std::vector<SearchView> sv;
+  size_t off = 0;
+  for (Block *bp : act) {
+    Block &b = *bp;
+    const int nq = b.nsearch;
+    const double *e = b.adtExt;
+    k_morton<<<nb(nq), BS, 0, s_>>>(nq, b.xsearch.p, e[0], e[1], e[2], e[3], e[4], e[5], mk0_.p + off, mv0_.p + off);
+    sort_pairs(mk0_.p + off, mk1_.p + off, mv0_.p + off, mv1_.p + off, nq, 30, s_);
+    SearchView S;
+    S.nq = nq;
+    S.perm = mv1_.p + off;
+    S.xs = b.xsearch.p;
+    S.xtag = b.xtag.p;
+    S.donorId = b.donorId.p;
+    S.nodes = b.nodes.p;
+    memcpy(S.ext, b.adtExt, sizeof S.ext);
+    S.conn = b.conn.p;
+    S.x = b.x.p;
+    S.cellRes = b.cellRes.p;
+    sv.push_back(S);
+    off += nq;
+  }
+  run_search(sv.data(), (int)sv.size(), sw_, s_, true);
+  for (Block *b : act) k_copy_dup<<<nb(b->nsearch), BS, 0, s_>>>(b->nsearch, b->xtag.p, b->donorId.p);
 }
 
 // tioga::exchangeDonors (exchangeDonors.C:25-301), single rank
@@ -3054,7 +3201,11 @@ void Engine::perform_connectivity()
   auto t4 = now();
   {
     Timer tm(s_);
-    for (Block *b : blocks_) launch_search(*b);
+    if (batched) {
+      launch_searches();
+    } else {
+      for (Block *b : blocks_) launch_search(*b);
+    }
     t.search_kernel = tm.stop();
   }
   auto t5 = now();
@@ -3298,8 +3449,13 @@ double Engine::time_search(int nrep, long long *nsearch)
   if (nsearch) *nsearch = q;
   if (!connected_ || nrep <= 0) return 0.0;
   Timer tm(s_);
-  for (int r = 0; r < nrep; r++)
-    for (Block *b : blocks_) launch_search(*b);
+  for (int r = 0; r < nrep; r++) {
+    if (batched) {
+      launch_searches();
+    } else {
+      for (Block *b : blocks_) launch_search(*b);
+    }
+  }
   return tm.stop() / nrep;
 }
 
