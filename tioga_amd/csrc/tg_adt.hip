@@ -16,7 +16,8 @@
 //     descending list B, the K pairs with A[m] < B[m] swapped in parallel
 //     (K found by bisection, since A[m] - B[m] increases with m), and one
 //     thread applying the loop control (hoare_stop, median_control in
-//     tg_common.h); the host reads the loop state every few rounds;
+//     tg_common.h); once the active ranges are short, one workgroup per
+//     segment finishes the rounds from the saved loop state;
 //   * medium segments get one workgroup each, which runs the same rounds;
 //   * segments of at most WAVEMAX elements get one wave each;
 //   * segments of at most SUB elements are finished, whole subtree, by one
@@ -41,7 +42,7 @@ constexpr int PER = 8;         // consecutive positions per thread per chunk
 constexpr int CH = BT * PER;   // positions per chunk of a multi-workgroup round
 constexpr int WAVEMAX = 2048;  // segments at most this long (and above SUB): one wave
 constexpr int BIGMIN = 1 << 16;  // segments longer than this: many workgroups
-constexpr int ROUNDS = 8;      // multi-workgroup rounds between host checks
+constexpr int FINISH = 16384;  // active range a single workgroup finishes a long segment's rounds from
 
 // base: offset of the segment's tree in the concatenated element arrays;
 // start: the segment's first position within its tree
@@ -110,6 +111,9 @@ struct WgState {
   double pv;
 };
 
+__device__ void wg_median_rounds(int *IX, double *X, int n, int *A, int *B, WgState &sh, int *lds, int *err,
+                                 int round0);
+
 // median_select on X[0..n) / IX[0..n) by a whole workgroup; A and B are
 // n-int scratch lists.  Same permutation as the sequential function.
 __device__ void wg_median(int *IX, double *X, int n, int *A, int *B, WgState &sh, int *lds, int *err)
@@ -135,9 +139,26 @@ __device__ void wg_median(int *IX, double *X, int n, int *A, int *B, WgState &sh
     sh.st = 0;
   }
   __syncthreads();
+  wg_median_rounds(IX, X, n, A, B, sh, lds, err, 0);
+}
+
+// the partition rounds of wg_median from the state in sh (lo, hi; st == 0),
+// round0 rounds having been made already, to the end and the fix-up
+__device__ void wg_median_rounds(int *IX, double *X, int n, int *A, int *B, WgState &sh, int *lds, int *err,
+                                 int round0)
+{
+  const int t = threadIdx.x;
+  auto swap = [&](int a, int b) {  // 1-based positions
+    double tx = X[a - 1];
+    X[a - 1] = X[b - 1];
+    X[b - 1] = tx;
+    int ti = IX[a - 1];
+    IX[a - 1] = IX[b - 1];
+    IX[b - 1] = ti;
+  };
   // every round narrows [lo, hi], so n + 2 rounds bound the loop; the cap
   // only guards against a broken invariant (reported, never expected)
-  for (int round = 0;; round++) {
+  for (int round = round0;; round++) {
     if (round > n + 2) {
       if (t == 0) atomicAdd(err, 1);
       break;
@@ -806,6 +827,26 @@ __global__ void k_big_ctrl(int nbig, const Seg *__restrict__ segs, BigState *__r
   st[b] = S;
 }
 
+// the rounds a long segment still needs after the multi-workgroup ones, by
+// one workgroup from the saved loop state (the active range is short by then)
+__global__ void __launch_bounds__(BT) k_big_finish(const Seg *__restrict__ segs, const BigState *__restrict__ st,
+                                                   int *ea, double *X, int *A, int *B, int *err)
+{
+  __shared__ int lds[2 * (BT / 64)];
+  __shared__ WgState sh;
+  const BigState S = st[blockIdx.x];
+  if (S.st != 0) return;  // finished (uniform over the workgroup)
+  const Seg s = segs[blockIdx.x];
+  const size_t g = (size_t)s.base + s.start;
+  if (threadIdx.x == 0) {
+    sh.lo = S.lo;
+    sh.hi = S.hi;
+    sh.st = 0;
+  }
+  __syncthreads();
+  wg_median_rounds(ea + g, X + g, s.nav, A + g, B + g, sh, lds, err, S.rounds);
+}
+
 // node boxes of long segments: the minimal / maximal class per bound, then
 // the last position in that class
 __global__ void k_big_box_init(int nbig, unsigned long long *__restrict__ bk, int *__restrict__ bp)
@@ -1047,21 +1088,21 @@ void build_adt_batch(AdtTree *trees, int ntree, AdtScratch &w, hipStream_t s)
     BigState *st = (BigState *)w.state.p;
     k_big_load<<<ntab, BT, 0, s>>>(w.tab.p, bs, w.ea.p, w.X.p, ebox);
     k_big_init<<<nblocks(nbig, 64), 64, 0, s>>>(nbig, bs, w.X.p, st);
-    for (;;) {
-      TG_HIP(hipMemsetAsync(w.err.p + 1, 0, sizeof(int), s));
-      for (int r = 0; r < ROUNDS; r++) {
-        k_big_count<<<ntab, BT, 0, s>>>(w.tab.p, bs, st, w.choff.p, w.X.p, w.cntA.p, w.cntB.p);
-        k_big_scan<<<nbig, BT, 0, s>>>(w.choff.p, w.choff.p + nbig, st, w.cntA.p, w.cntB.p);
-        k_big_fill<<<ntab, BT, 0, s>>>(w.tab.p, bs, st, w.choff.p, w.X.p, w.cntA.p, w.cntB.p, w.A.p, w.B.p);
-        k_big_swap<<<ntab, BT, 0, s>>>(w.tab.p, bs, st, w.ea.p, w.X.p, w.A.p, w.B.p);
-        k_big_ctrl<<<nblocks(nbig, 64), 64, 0, s>>>(nbig, bs, st, w.ea.p, w.X.p, w.A.p, w.B.p, r == ROUNDS - 1,
-                                                    w.err.p + 1, w.err.p);
-      }
-      int act = 0;
-      TG_HIP(hipMemcpyAsync(&act, w.err.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-      TG_HIP(hipStreamSynchronize(s));
-      if (act == 0) break;
+    // multi-workgroup rounds while the active ranges are long (each round
+    // about halves them), then one workgroup per segment for the rest
+    int maxnav = 0;
+    for (int b = 0; b < nbig; b++) maxnav = std::max(maxnav, all[big0[l] + b].nav);
+    int rounds = 2;
+    while (rounds < 16 && (maxnav >> rounds) > FINISH) rounds++;
+    for (int r = 0; r < rounds; r++) {
+      k_big_count<<<ntab, BT, 0, s>>>(w.tab.p, bs, st, w.choff.p, w.X.p, w.cntA.p, w.cntB.p);
+      k_big_scan<<<nbig, BT, 0, s>>>(w.choff.p, w.choff.p + nbig, st, w.cntA.p, w.cntB.p);
+      k_big_fill<<<ntab, BT, 0, s>>>(w.tab.p, bs, st, w.choff.p, w.X.p, w.cntA.p, w.cntB.p, w.A.p, w.B.p);
+      k_big_swap<<<ntab, BT, 0, s>>>(w.tab.p, bs, st, w.ea.p, w.X.p, w.A.p, w.B.p);
+      k_big_ctrl<<<nblocks(nbig, 64), 64, 0, s>>>(nbig, bs, st, w.ea.p, w.X.p, w.A.p, w.B.p, 0, w.err.p + 1,
+                                                  w.err.p);
     }
+    k_big_finish<<<nbig, BT, 0, s>>>(bs, st, w.ea.p, w.X.p, w.A.p, w.B.p, w.err.p);
     k_big_box_init<<<nblocks(6 * nbig, 64), 64, 0, s>>>(nbig, w.bk.p, w.bp.p);
     k_big_box_key<<<ntab, BT, 0, s>>>(w.tab.p, bs, w.ea.p, ebox, w.bk.p);
     k_big_box_pos<<<ntab, BT, 0, s>>>(w.tab.p, bs, w.ea.p, ebox, w.bk.p, w.bp.p);

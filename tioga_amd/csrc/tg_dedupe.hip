@@ -160,20 +160,20 @@ __device__ __forceinline__ void wave_sync()
 // in k_oct_level.  Non-leaf children are pushed largest first, so the
 // smallest is taken next: a pending entry is at least as large as the child
 // being split below it, and the stack stays below 7 log8(SUBD / LEAF) + 8
-// entries.  Leaves go to the global leaf list; the final node order is
-// written back to ea for them.
+// entries.  The subtree's leaves of two or more points are listed in LDS and
+// compared at the end, one lane per leaf, as k_oct_leaf compares them.
 constexpr int DW = 2;     // waves per workgroup of k_oct_sub
 constexpr int DSTK = 32;  // stack entries: start | len << 16
 static_assert(SUBD <= 65535, "local ids are 16 bits");
 
-__global__ void __launch_bounds__(64 * DW) k_oct_sub(int n, const Node *__restrict__ roots, int *ea,
-                                                     const double *__restrict__ x, Node *__restrict__ leaves,
-                                                     int *__restrict__ nleaves, int *__restrict__ err)
+__global__ void __launch_bounds__(64 * DW) k_oct_sub(int n, const Node *__restrict__ roots, const int *__restrict__ ea,
+                                                     const double *__restrict__ x, const int *__restrict__ tag,
+                                                     double *rtag, int *itag, int *__restrict__ err)
 {
   __shared__ double sx[DW][3][SUBD];
   __shared__ int sg[DW][SUBD];
   __shared__ unsigned short sP[DW][SUBD], sT[DW][SUBD];
-  __shared__ unsigned sS[DW][DSTK];
+  __shared__ unsigned sS[DW][DSTK], sLeaf[DW][SUBD / 2];  // leaves: start | len << 16, two or more points
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int k = blockIdx.x * DW + w;
   if (k >= n) return;  // whole waves leave; no workgroup barrier follows
@@ -186,7 +186,16 @@ __global__ void __launch_bounds__(64 * DW) k_oct_sub(int n, const Node *__restri
     for (int j = 0; j < 3; j++) sx[w][j][i] = x[3 * (size_t)g + j];
     P[i] = (unsigned short)i;
   }
-  int sp = 0;
+  int sp = 0, nleaf = 0;
+  auto leaf = [&](int st, int len) {  // uniform; at most SUBD / 2 leaves of two or more points
+    if (len < 2) return;
+    if (nleaf >= SUBD / 2) {  // bounded by the sizes; reported, never expected
+      if (lane == 0) atomicAdd(err, 1);
+      return;
+    }
+    if (lane == 0) sLeaf[w][nleaf] = (unsigned)st | ((unsigned)len << 16);
+    nleaf++;
+  };
   if (lane == 0) sS[w][0] = (unsigned)root.len << 16;
   sp = 1;
   wave_sync();
@@ -227,7 +236,7 @@ __global__ void __launch_bounds__(64 * DW) k_oct_sub(int n, const Node *__restri
     bool one = false;
     for (int q = 0; q < 8; q++) one = one || (cnt[q] == cl);
     if (one) {  // every point in one octant: the node is compared as a whole
-      if (lane == 0) leaves[atomicAdd(nleaves, 1)] = Node{root.start + cs, cl};
+      leaf(cs, cl);
       continue;
     }
     int cft[9];
@@ -253,7 +262,7 @@ __global__ void __launch_bounds__(64 * DW) k_oct_sub(int n, const Node *__restri
     for (int q = 0; q < 8; q++) {
       if (cnt[q] == 0) continue;
       if (cnt[q] <= LEAF) {
-        if (lane == 0) leaves[atomicAdd(nleaves, 1)] = Node{root.start + cs + cft[q], cnt[q]};
+        leaf(cs + cft[q], cnt[q]);
         continue;
       }
       int m = nn++;
@@ -272,7 +281,30 @@ __global__ void __launch_bounds__(64 * DW) k_oct_sub(int n, const Node *__restri
     sp += nn;
     wave_sync();
   }
-  for (int i = lane; i < root.len; i += 64) ea[root.start + i] = sg[w][P[i]];
+  wave_sync();
+  // all pairs of each leaf in list order (utils.c:691-716); leaves touch
+  // disjoint points
+  for (int k = lane; k < nleaf; k += 64) {
+    const unsigned e = sLeaf[w][k];
+    const int st = (int)(e & 0xffff), len = (int)(e >> 16);
+    for (int a = 0; a < len; a++) {
+      const int la = P[st + a], p1 = sg[w][la];
+      const double a0 = sx[w][0][la], a1 = sx[w][1][la], a2 = sx[w][2][la];
+      const int t1 = tag[p1];
+      for (int b = a + 1; b < len; b++) {
+        const int lb = P[st + b], p2 = sg[w][lb];
+        if (fabs(a0 - sx[w][0][lb]) + fabs(a1 - sx[w][1][lb]) + fabs(a2 - sx[w][2][lb]) < TOL && t1 == tag[p2]) {
+          if (p1 > p2) {
+            rtag[p2] = dmax(rtag[p1], rtag[p2]);
+            itag[p1] = itag[p2];
+          } else {
+            rtag[p1] = dmax(rtag[p1], rtag[p2]);
+            itag[p2] = itag[p1];
+          }
+        }
+      }
+    }
+  }
 }
 
 // all pairs of a leaf in list order (utils.c:691-716)
@@ -603,17 +635,15 @@ void unique_nodes_batch(DedupeSet *sets, int nset, DedupeScratch &w, hipStream_t
     std::swap(nin, nout);
     std::swap(bin, bout);
   }
-  if (cnt[3] > 0) {  // subtrees of at most SUBD points
-    const int hc[2] = {cnt[1], 0};
-    TG_HIP(hipMemcpyAsync(w.counts.p + 1, hc, sizeof(int), hipMemcpyHostToDevice, s));
-    TG_HIP(hipMemcpyAsync(w.counts.p + 4, hc + 1, sizeof(int), hipMemcpyHostToDevice, s));
-    k_oct_sub<<<nblocks(cnt[3], DW), 64 * DW, 0, s>>>(cnt[3], (const Node *)w.subs.p, w.ea.p, x, (Node *)w.leaves.p,
-                                                      w.counts.p + 1, w.counts.p + 4);
-    int back[4];
-    TG_HIP(hipMemcpyAsync(back, w.counts.p + 1, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+  if (cnt[3] > 0) {  // subtrees of at most SUBD points, their leaves included
+    const int zero = 0;
+    TG_HIP(hipMemcpyAsync(w.counts.p + 4, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+    k_oct_sub<<<nblocks(cnt[3], DW), 64 * DW, 0, s>>>(cnt[3], (const Node *)w.subs.p, w.ea.p, x, tag, rtag, itag,
+                                                      w.counts.p + 4);
+    int bad = 0;
+    TG_HIP(hipMemcpyAsync(&bad, w.counts.p + 4, sizeof(int), hipMemcpyDeviceToHost, s));
     TG_HIP(hipStreamSynchronize(s));
-    if (back[3]) throw std::runtime_error("tioga_amd: octree dedupe: subtree stack exceeded its bound");
-    cnt[1] = back[0];
+    if (bad) throw std::runtime_error("tioga_amd: octree dedupe: subtree stack exceeded its bound");
   }
   if (cnt[1] > 0)
     k_oct_leaf<<<nblocks(cnt[1], 64), 64, 0, s>>>(cnt[1], (const Node *)w.leaves.p, w.ea.p, x, tag, rtag, itag);
