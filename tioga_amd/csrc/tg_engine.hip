@@ -942,6 +942,7 @@ __device__ __forceinline__ void contain(const SearchView &S, int c, int nv, cons
 
 constexpr int WAVE = 64;
 constexpr int STACK = 32;
+static_assert(STACK <= WAVE, "k_collect keeps its stack one entry per lane");
 constexpr int KSTAGE = 32;  // candidates per lane staged in LDS during the walk
 
 // Candidate lists of one search: a flat list of (cell | nv << 28, query)
@@ -980,8 +981,6 @@ struct CandOut {
 // k_collect_more, which lists all of them.
 __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
 {
-  __shared__ int snode[STACK];
-  __shared__ unsigned long long smask[STACK];
   __shared__ int stage[KSTAGE][WAVE];
   const int lane = threadIdx.x;
   const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, blockIdx.x));
@@ -1004,19 +1003,16 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
     for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= S.ext[2 * d + 1] + TOL);
   }
   int nc = 0;
-  const unsigned long long m0 = __ballot(inroot);
+  // pending right subtrees: entry k (node, lane mask) held by lane k, so a
+  // push is a select and a pop a readlane; the left child, when any lane
+  // enters it, is visited next without the stack (preorder)
+  int vnode = 0;
+  unsigned vmlo = 0, vmhi = 0;
   int sp = 0;
-  if (m0) {
-    if (lane == 0) {
-      snode[0] = 0;
-      smask[0] = m0;
-    }
-    sp = 1;
-  }
-  while (sp > 0) {
-    sp--;
-    const int node = __builtin_amdgcn_readfirstlane(snode[sp]);
-    const unsigned long long m = smask[sp];
+  int node = 0;
+  unsigned long long m = __ballot(inroot);
+  bool more = m != 0ull;
+  while (more) {
     // the whole record in one batch of scalar loads, then the tests
     AdtNode N;
     {
@@ -1040,18 +1036,23 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
     const unsigned long long mr = (r > -1) ? __ballot(act & in_box_all(p, N.rbox)) : 0ull;
     const unsigned long long ml = (l > -1) ? __ballot(act & in_box_all(p, N.lbox)) : 0ull;
     if (mr) {
-      if (lane == 0) {
-        snode[sp] = r;
-        smask[sp] = mr;
+      if (lane == sp) {
+        vnode = r;
+        vmlo = (unsigned)mr;
+        vmhi = (unsigned)(mr >> 32);
       }
       sp++;
     }
     if (ml) {
-      if (lane == 0) {
-        snode[sp] = l;
-        smask[sp] = ml;
-      }
-      sp++;
+      node = l;
+      m = ml;
+    } else if (sp > 0) {
+      sp--;
+      node = __builtin_amdgcn_readlane(vnode, sp);
+      m = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)vmhi, sp) << 32) |
+          (unsigned)__builtin_amdgcn_readlane((int)vmlo, sp);
+    } else {
+      more = false;
     }
   }
   // one chunk of the flat list per wave
