@@ -75,7 +75,8 @@ struct TypeInfo {
 
 // One ADT node as the search kernel reads it: 160 bytes, node ids in the
 // ADT's preorder; left/right are child node ids (-1: none) and the children's
-// subtree boxes sit in the parent, so one record decides a whole visit.
+// subtree boxes sit in the parent, so one record decides a whole visit.  The
+// boxes are stored padded by the search tolerance (lower - TOL, upper + TOL).
 struct __align__(16) AdtNode {
   double ebox[6];  // box of the node's own element
   double lbox[6];  // subtree box of the left child

@@ -788,10 +788,13 @@ __global__ void k_adt_nodes(int nelem, TypeInfo ti, const int *__restrict__ adtI
   N.nv = ti.nv[type_of(ti, N.cell)];
   N.left = a.y > -1 ? adtInt[4 * (size_t)a.y + 3] : -1;
   N.right = a.z > -1 ? adtInt[4 * (size_t)a.z + 3] : -1;
+  // bounds padded by TOL here, in the same FP64 operation the box test of
+  // the walk makes (lower - TOL, upper + TOL)
   for (int j = 0; j < 6; j++) {
-    N.ebox[j] = ebox[6 * (size_t)a.x + j];
-    N.lbox[j] = N.left > -1 ? adtReal[6 * (size_t)N.left + j] : 0.0;
-    N.rbox[j] = N.right > -1 ? adtReal[6 * (size_t)N.right + j] : 0.0;
+    const double pad = j < 3 ? -TOL : TOL;
+    N.ebox[j] = ebox[6 * (size_t)a.x + j] + pad;
+    N.lbox[j] = N.left > -1 ? adtReal[6 * (size_t)N.left + j] + pad : 0.0;
+    N.rbox[j] = N.right > -1 ? adtReal[6 * (size_t)N.right + j] + pad : 0.0;
   }
   nodes[n] = N;
 }
@@ -898,11 +901,11 @@ __device__ __forceinline__ bool in_box(const double p[3], const double *b)
          p[1] <= b[4] + TOL && p[2] <= b[5] + TOL;
 }
 
-// the same test evaluated without short-circuit branches
-__device__ __forceinline__ bool in_box_all(const double p[3], const double *b)
+// the same test against bounds already padded by TOL (tree node records),
+// evaluated without short-circuit branches
+__device__ __forceinline__ bool in_padded(const double p[3], const double *b)
 {
-  return (p[0] >= b[0] - TOL) & (p[1] >= b[1] - TOL) & (p[2] >= b[2] - TOL) & (p[0] <= b[3] + TOL) &
-         (p[1] <= b[4] + TOL) & (p[2] <= b[5] + TOL);
+  return (p[0] >= b[0]) & (p[1] >= b[1]) & (p[2] >= b[2]) & (p[0] <= b[3]) & (p[1] <= b[4]) & (p[2] <= b[5]);
 }
 
 // MeshBlock::checkContainment (checkContainment.C:27-96) for a cell with NV
@@ -1028,13 +1031,13 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
       N.right = r->right;
     }
     const bool act = (m >> lane) & 1ull;
-    if (act & in_box_all(p, N.ebox)) {
+    if (act & in_padded(p, N.ebox)) {
       if (nc < KSTAGE) stage[nc][lane] = N.cell | (N.nv << 28);
       nc++;
     }
     const int r = N.right, l = N.left;
-    const unsigned long long mr = (r > -1) ? __ballot(act & in_box_all(p, N.rbox)) : 0ull;
-    const unsigned long long ml = (l > -1) ? __ballot(act & in_box_all(p, N.lbox)) : 0ull;
+    const unsigned long long mr = (r > -1) ? __ballot(act & in_padded(p, N.rbox)) : 0ull;
+    const unsigned long long ml = (l > -1) ? __ballot(act & in_padded(p, N.lbox)) : 0ull;
     if (mr) {
       if (lane == sp) {
         vnode = r;
@@ -1106,13 +1109,13 @@ __global__ void __launch_bounds__(64) k_collect_more(SearchSet T, CandOut O)
     stk[sp++][tid] = 0;
     while (sp > 0) {
       const AdtNode &N = nodes[stk[--sp][tid]];
-      if (in_box_all(p, N.ebox)) {
+      if (in_padded(p, N.ebox)) {
         O.cell[off + nc] = N.cell | (N.nv << 28);
         O.query[off + nc] = g;
         nc++;
       }
-      if (N.right > -1 && in_box_all(p, N.rbox)) stk[sp++][tid] = N.right;
-      if (N.left > -1 && in_box_all(p, N.lbox)) stk[sp++][tid] = N.left;
+      if (N.right > -1 && in_padded(p, N.rbox)) stk[sp++][tid] = N.right;
+      if (N.left > -1 && in_padded(p, N.lbox)) stk[sp++][tid] = N.left;
     }
   }
 }
