@@ -47,6 +47,7 @@ def lib():
         L.orc_set_params.argtypes = [_P, C.c_int, C.c_int, C.c_int]
         L.orc_profile.argtypes = [_P]
         L.orc_perform_connectivity.argtypes = [_P]
+        L.orc_reduce_fringes.argtypes = [_P]
         L.orc_register_solution.argtypes = [_P, C.c_int, _P]
         L.orc_data_update.argtypes = [_P, C.c_int, C.c_int]
         L.orc_get_donor_count.argtypes = [_P, C.c_int, _P, _P]
@@ -141,6 +142,9 @@ class Oracle:
 
     def connect(self):
         self.L.orc_perform_connectivity(self.h)
+
+    def reduce_fringes(self):
+        self.L.orc_reduce_fringes(self.h)
 
     def data_update(self, qs: Sequence[np.ndarray], nvar: int, interptype: int = 0) -> List[np.ndarray]:
         out = [np.ascontiguousarray(q, dtype=np.float64).copy() for q in qs]
@@ -272,12 +276,14 @@ class Oracle:
 
 def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptype: int = 0,
                isym: int = 3, nfringe: int = 1, mexclude: int = 3,
-               moves: Sequence[Sequence[Optional[np.ndarray]]] = (), amr=None) -> None:
+               moves: Sequence[Sequence[Optional[np.ndarray]]] = (), amr=None, reduce: bool = False) -> None:
     """amr = (AmrGrid, patch fields per local patch) selects the AMR gather
-    mode of the driver (format version 2)."""
+    mode of the driver (format version 2); reduce runs reduce_fringes after
+    each step's connectivity (format version 3)."""
+    version = 3 if reduce else (1 if amr is None else 2)
     with open(path, "wb") as f:
         w = lambda fmt, *v: f.write(struct.pack("<" + fmt, *v))
-        w("9i", 0x41474954, 1 if amr is None else 2, len(blocks), isym, nfringe, mexclude, nvar, interptype,
+        w("9i", 0x41474954, version, len(blocks), isym, nfringe, mexclude, nvar, interptype,
           len(moves))
         for b, q in zip(blocks, qs):
             b = b.normalized()
@@ -293,6 +299,8 @@ def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptyp
                 f.write(b.nres.tobytes())
                 f.write(b.cres.tobytes())
             f.write(np.ascontiguousarray(q, np.float64).tobytes())
+        if version >= 2 and amr is None:
+            w("i", 0)
         if amr is not None:
             grid, pq = amr
             w("i", 1)
@@ -304,6 +312,8 @@ def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptyp
             for lp, q in enumerate(pq):
                 w("i", int(order[lp]))
                 f.write(np.ascontiguousarray(q, np.float64).tobytes())
+        if version >= 3:
+            w("i", int(reduce))
         for step in moves:
             for xb in step:
                 if xb is None:

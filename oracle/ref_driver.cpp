@@ -26,6 +26,10 @@
 //              f64 q[d3nf*nvar] (d3nf = prod(dims+2*nf) of that grid).
 //   With AMR patches the driver runs the gather direction of the AMR path
 //   (see run_amr) instead of performConnectivity; nsteps must be 0.
+//   version 3 adds, after has_amr: int32 reduce; if set, every step goes on
+//   after its dataUpdate with tioga::reduce_fringes (tioga.C:737-885), the
+//   full dump again and a dataUpdate on a fresh copy of q, all under the
+//   name prefix "rf.".
 // Result file: records {int32 namelen, name, int32 dtype(0=i32,1=f64),
 //              int64 count, payload}.  Step s > 0 prefixes names with "s<s>.".
 #include <cstdint>
@@ -243,7 +247,7 @@ int main(int argc, char **argv)
   if (!r.f) { perror(argv[1]); return 2; }
   if (r.get<int32_t>() != 0x41474954) { fprintf(stderr, "bad case\n"); return 2; }
   const int version = r.get<int32_t>();
-  if (version != 1 && version != 2) { fprintf(stderr, "bad case version\n"); return 2; }
+  if (version < 1 || version > 3) { fprintf(stderr, "bad case version\n"); return 2; }
   int nblocks = r.get<int32_t>(), isym = r.get<int32_t>(), nfringe = r.get<int32_t>();
   int mexclude = r.get<int32_t>(), nvar = r.get<int32_t>(), itype = r.get<int32_t>();
   int nsteps = r.get<int32_t>();
@@ -286,6 +290,7 @@ int main(int argc, char **argv)
       amr.ibl[p].assign(d3nf, 1);
     }
   }
+  const int reduce = version >= 3 ? r.get<int32_t>() : 0;
   Writer w{fopen(argv[2], "wb"), ""};
   tioga *tg = new tioga;
   tg->setCommunicator(MPI_COMM_WORLD, 0, 1);
@@ -328,6 +333,20 @@ int main(int argc, char **argv)
     tg->dataUpdate(nvar, itype);
     for (size_t b = 0; b < blocks.size(); b++)
       w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
+    if (reduce) {
+      const std::string p0 = w.prefix;
+      tg->reduce_fringes();
+      w.prefix = p0 + "rf.";
+      dump(tg, blocks, w, 0.0);
+      for (size_t b = 0; b < blocks.size(); b++) {
+        qs[b] = blocks[b].q;
+        tg->registerSolution(blocks[b].tag, qs[b].data());
+      }
+      tg->dataUpdate(nvar, itype);
+      for (size_t b = 0; b < blocks.size(); b++)
+        w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
+      w.prefix = p0;
+    }
   }
   fclose(w.f);
   fclose(r.f);

@@ -1855,6 +1855,82 @@ static void exchange_donors(OrcTioga *t)
   free(nint);
 }
 
+/* MeshBlock::reduce_fringes (bookKeeping.C:650-760): iblank_reduced keeps the
+   field nodes and, over nfringe + 1 sweeps, the receptors of cells that are
+   partly field or receptor; every sweep reads the previous sweep's values
+   (ibltmp) and only turns 0 into the node's negative iblank.  The receptors
+   it drops (iblank < 0, iblank_reduced == 0) form the new cancel list, in
+   node order.  Returns iblank_reduced (caller frees). */
+static int *block_reduce_fringes(OrcBlock *b)
+{
+  int i, n, m, iter;
+  const int nn = b->nnodes > 0 ? b->nnodes : 1;
+  int *ibr = (int *)malloc(sizeof(int) * nn), *tmp = (int *)malloc(sizeof(int) * nn);
+  for (i = 0; i < b->nnodes; i++) ibr[i] = b->iblank[i] > 0 ? b->iblank[i] : 0;
+  for (i = 0; i < b->nnodes; i++) tmp[i] = ibr[i];
+  for (iter = 0; iter < b->nfringe + 1; iter++) {
+    for (n = 0; n < b->ntypes; n++) {
+      const int nvert = b->nv[n];
+      for (i = 0; i < b->nc[n]; i++) {
+        int inode[8], ncount = 0;
+        for (m = 0; m < nvert; m++) {
+          inode[m] = b->vconn[n][nvert * i + m] - ORC_BASE;
+          ncount += (ibr[inode[m]] == 1 || ibr[inode[m]] < 0);
+        }
+        if (ncount > 0 && ncount < nvert)
+          for (m = 0; m < nvert; m++)
+            if (tmp[inode[m]] == 0 && b->iblank[inode[m]] < 0) tmp[inode[m]] = b->iblank[inode[m]];
+      }
+    }
+    for (i = 0; i < b->nnodes; i++) ibr[i] = tmp[i];
+  }
+  free(tmp);
+  b->ncancel = 0;
+  for (i = 0; i < b->nnodes; i++)
+    if (b->iblank[i] < 0 && ibr[i] == 0) push_cancel(b, i);
+  return ibr;
+}
+
+/* tioga::reduce_fringes (tioga.C:737-885) on one rank: the blocks' fringe
+   reduction, then comm 3 (getCancellationData / cancelDonor,
+   bookKeeping.C:542-570, with the head of each dropped receptor's donor
+   list), resetCoincident, and comm 4 (clearIblanks with iblank_reduced,
+   bookKeeping.C:615-625, then setIblanks for every live record).  Cell
+   iblanks are not recomputed, as in the reference. */
+void orc_reduce_fringes(OrcTioga *t)
+{
+  int ib, i;
+  int **ibr = (int **)calloc(t->nblocks > 0 ? t->nblocks : 1, sizeof(int *));
+  for (ib = 0; ib < t->nblocks; ib++) ibr[ib] = block_reduce_fringes(t->b[ib]);
+  for (ib = 0; ib < t->nblocks; ib++) {
+    OrcBlock *b = t->b[ib];
+    for (i = 0; i < b->ncancel; i++) {
+      DRec *h = b->donorList ? b->donorList[b->cancelList[i]] : NULL;
+      if (h == NULL) continue;
+      int db = 0;
+      OrcBlock *D = find_block(t, h->meshtag, &db);
+      if (!D || !D->interp2donor) continue;
+      int iptr = D->interp2donor[h->remoteid];
+      if (iptr > -1) D->interp[iptr].cancel = 1;
+    }
+  }
+  for (ib = 0; ib < t->nblocks; ib++) reset_coincident(t->b[ib]);
+  for (ib = 0; ib < t->nblocks; ib++) {
+    OrcBlock *b = t->b[ib];
+    for (i = 0; i < b->nnodes; i++)
+      if (b->iblank[i] < 0) b->iblank[i] = 1;
+    for (i = 0; i < b->nnodes; i++)
+      if (ibr[ib][i] == 0) b->iblank[i] = 0;
+  }
+  for (ib = 0; ib < t->nblocks; ib++) {
+    OrcBlock *b = t->b[ib];
+    for (i = 0; i < b->ninterp; i++)
+      if (!b->interp[i].cancel) t->b[b->interp[i].rinfo[2]]->iblank[b->interp[i].rinfo[1]] = -1;
+  }
+  for (ib = 0; ib < t->nblocks; ib++) free(ibr[ib]);
+  free(ibr);
+}
+
 /* highOrder.C:88-150 */
 static void cell_iblanks(OrcBlock *b)
 {

@@ -57,6 +57,7 @@ void orc_set_cell_iblank(OrcTioga *t, int btag, int *iblank_cell);
 void orc_set_params(OrcTioga *t, int isym, int nfringe, int mexclude);
 void orc_profile(OrcTioga *t);
 void orc_perform_connectivity(OrcTioga *t);
+void orc_reduce_fringes(OrcTioga *t);
 void orc_register_solution(OrcTioga *t, int btag, double *q);
 void orc_data_update(OrcTioga *t, int nvar, int interptype);
 void orc_get_donor_count(OrcTioga *t, int btag, int *dcount, int *fcount);

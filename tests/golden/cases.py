@@ -26,6 +26,7 @@ class Case:
     nfringe: int = 1
     mexclude: int = 3
     moves: List[List[Optional[np.ndarray]]] = field(default_factory=list)
+    reduce: bool = False  # tioga::reduce_fringes after each connectivity
 
     @property
     def qs(self) -> List[np.ndarray]:
@@ -59,6 +60,24 @@ def three_blocks():
     return [a, b, W.cart_box(12, tag=3)]
 
 
+def nested_boxes(shell: bool = False):
+    """A coarse box with a finer jittered box inside it (outer faces of the
+    fine box overset boundaries): every coarse node well inside the fine box
+    finds a finer donor, so the coarse mesh holds a thick receptor region
+    that reduce_fringes thins; optionally a walled shell inside both."""
+    coarse = W.cart_box(10, tag=1)
+    n = 20
+    fine = W.cart_box(n, tag=2, lo=-0.61, hi=0.59, jitter=0.05, seed=4)
+    m = n + 1
+    k, j, i = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    face = ((i == 0) | (i == n) | (j == 0) | (j == n) | (k == 0) | (k == n)).ravel()
+    fine.obc = (np.nonzero(face)[0] + 1).astype(np.int32)
+    out = [coarse, fine]
+    if shell:
+        out.append(W.latlong_shell(12, 8, 5, 0.12, 0.3, 1.1, tag=3, center=(0.02, -0.03, 0.01)))
+    return out
+
+
 def build(name: str) -> Case:
     if name == "tiny":
         return Case(name, W.tiny(), nvar=5)
@@ -80,11 +99,33 @@ def build(name: str) -> Case:
         return Case(name, three_blocks(), nvar=3)
     if name == "config1":
         return Case(name, W.config1(), nvar=5)
+    # reduce_fringes (tioga.C:737-885) after the connectivity: fringe depths
+    # nfringe + 1 = 2, 3 and 4 sweeps, mixed cell types, three blocks, and a
+    # moving body reduced at every step
+    if name == "tiny_rf":
+        return Case(name, W.tiny(), nvar=3, reduce=True)
+    if name == "isym0_rf":
+        return Case(name, W.tiny(), nvar=2, interptype=1, isym=0, nfringe=2, mexclude=1, reduce=True)
+    if name == "mixed_rf":
+        return Case(name, [W.mixed_shell(12, 8, layers=(2, 2, 1, 2)),
+                           W.cart_box(12, tag=2, jitter=0.1, seed=7)], nvar=2, nfringe=3, reduce=True)
+    if name == "three_rf":
+        return Case(name, three_blocks(), nvar=2, nfringe=2, reduce=True)
+    if name == "moving_rf":
+        blocks = W.tiny()
+        return Case(name, blocks, nvar=2, moves=_moves(blocks, 2, seed=21), reduce=True)
+    # overset-boundary bands (mexclude) thicker than the nfringe + 1 sweeps,
+    # so the reduction drops receptors and cancels their donors
+    if name == "nested_rf":
+        return Case(name, nested_boxes(), nvar=2, nfringe=0, mexclude=5, reduce=True)
+    if name == "nested3_rf":
+        return Case(name, nested_boxes(shell=True), nvar=3, nfringe=0, mexclude=4, reduce=True)
     raise KeyError(name)
 
 
 SMALL_CASES = ["tiny", "tiny_isym0", "tiny_isym1", "tiny_isym2", "mixed", "userres", "moving", "three"]
-ALL_CASES = SMALL_CASES + ["config1"]
+REDUCE_CASES = ["tiny_rf", "isym0_rf", "mixed_rf", "three_rf", "moving_rf", "nested_rf", "nested3_rf"]
+ALL_CASES = SMALL_CASES + ["config1"] + REDUCE_CASES
 
 
 # ---------------------------------------------------------------------------

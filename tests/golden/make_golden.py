@@ -46,7 +46,7 @@ def main(names):
             else:
                 c = cases.build(name)
                 O.write_case(cp, c.blocks, c.qs, c.nvar, c.interptype, c.isym, c.nfringe, c.mexclude,
-                             c.moves)
+                             c.moves, reduce=c.reduce)
             res = O.run_reference(cp, op, cwd=td)
         meta = {}
         full = {}

@@ -39,6 +39,10 @@ def run_oracle_case(case):
             pre = "" if s == 0 else f"s{s}."
             for k, v in o.results(case.qs, case.nvar, case.interptype).items():
                 out[pre + k] = v
+            if getattr(case, "reduce", False):
+                o.reduce_fringes()
+                for k, v in o.results(case.qs, case.nvar, case.interptype).items():
+                    out[pre + "rf." + k] = v
         out["weight_aborts"] = o.L.orc_weight_aborts()
     finally:
         o.close()
@@ -90,6 +94,10 @@ def run_engine_case(case, batched: bool = True):
             pre = "" if s == 0 else f"s{s}."
             for k, v in t.results(case.qs, case.nvar, case.interptype).items():
                 out[pre + k] = v
+            if getattr(case, "reduce", False):
+                t.reduce_fringes()
+                for k, v in t.results(case.qs, case.nvar, case.interptype).items():
+                    out[pre + "rf." + k] = v
         out["weight_aborts"] = t.weight_aborts()
     finally:
         t.close()

@@ -95,10 +95,7 @@ void tioga::set_cell_iblank(int btag, int *ib_cell) { eng_->set_cell_iblank(btag
 
 void tioga::set_uniform_hex_flag(int, int) {}
 
-void tioga::reduce_fringes(void)
-{
-  throw std::runtime_error("tioga_amd: reduce_fringes is not available in this engine");
-}
+void tioga::reduce_fringes(void) { eng_->reduce_fringes(); }
 
 void tioga::register_amr_global_data(int nf, int qstride, double *qnodein, int *idata, double *rdata, int ngridsin,
                                      int qnodesize)

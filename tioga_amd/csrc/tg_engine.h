@@ -301,6 +301,7 @@ class Engine {
   void set_cell_iblank_first(int *ibc);
   void profile();
   void perform_connectivity();
+  void reduce_fringes();
   void register_solution(int btag, double *q);
   void data_update(int nvar, int interptype);
   void get_donor_count(int btag, int *dcount, int *fcount);

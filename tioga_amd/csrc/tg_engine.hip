@@ -1605,6 +1605,56 @@ __global__ void k_set_receptors(int n, const int *__restrict__ cancel, const int
   if (r < n && !cancel[r]) T->ib[rblk[r]][rnode[r]] = -1;
 }
 
+// MeshBlock::reduce_fringes (bookKeeping.C:650-760).  iblank_reduced starts
+// as the field nodes; each of the nfringe + 1 sweeps reads the previous
+// sweep's values and, in every cell that is partly field or kept receptor,
+// turns the 0 of a receptor node into its iblank.  A sweep only moves values
+// from 0 to the node's own iblank, so the cells can run in any order.
+__global__ void k_rf_init(int n, const int *__restrict__ iblank, int *__restrict__ ibr, int *__restrict__ tmp)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int v = iblank[i] > 0 ? iblank[i] : 0;
+  ibr[i] = v;
+  tmp[i] = v;
+}
+
+__global__ void k_rf_sweep(int ncells, TypeInfo ti, const int *__restrict__ conn, const int *__restrict__ ibr,
+                           const int *__restrict__ iblank, int *tmp)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int nv = ti.nv[type_of(ti, c)];
+  const int *cv = conn + 8 * (size_t)c;
+  int ncount = 0;
+  for (int m = 0; m < nv; m++) {
+    const int r = ibr[cv[m]];
+    ncount += (r == 1 || r < 0);
+  }
+  if (ncount > 0 && ncount < nv)
+    for (int m = 0; m < nv; m++) {
+      const int v = cv[m];
+      if (tmp[v] == 0 && iblank[v] < 0) tmp[v] = iblank[v];
+    }
+}
+
+// the receptors the reduction drops: the new cancel list
+__global__ void k_rf_cancel(int n, const int *__restrict__ iblank, const int *__restrict__ ibr,
+                            unsigned char *__restrict__ cancelv)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) cancelv[i] = (iblank[i] < 0 && ibr[i] == 0);
+}
+
+// MeshBlock::clearIblanks with iblank_reduced (bookKeeping.C:615-625)
+__global__ void k_rf_clear(int n, const int *__restrict__ ibr, int *__restrict__ iblank)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (iblank[i] < 0) iblank[i] = 1;
+  if (ibr[i] == 0) iblank[i] = 0;
+}
+
 // MeshBlock::getCellIblanks (highOrder.C:88-150)
 __global__ void k_cell_iblank(int ncells, TypeInfo ti, const int *__restrict__ conn, const int *__restrict__ iblank,
                               int *__restrict__ ibc)
@@ -3246,6 +3296,82 @@ void Engine::perform_connectivity()
   t.prep_dedupe = prep_acc_[3];
   last_times = t;
   connected_ = true;
+}
+
+// tioga::reduce_fringes (tioga.C:737-885), one rank: each block's fringe
+// reduction, then comm 3 (the donor record behind the head of every dropped
+// receptor's donor list is cancelled, bookKeeping.C:542-570), resetCoincident,
+// and comm 4 (clearIblanks with iblank_reduced, then every live record marks
+// its receptor).  Cell iblanks are left as the connectivity computed them, as
+// the reference leaves them.
+void Engine::reduce_fringes()
+{
+  if (!connected_) throw std::runtime_error("tioga_amd: reduce_fringes needs performconnectivity first");
+  const int nbk = (int)blocks_.size();
+  std::vector<DBuf<int> *> ibr(nbk);
+  std::vector<DBuf<unsigned char> *> cancelv(nbk);
+  DBuf<int> tmp;
+  CancelTables ct{};
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    const int V = b.nnodes, N = b.ncells;
+    ibr[ib] = new DBuf<int>;
+    cancelv[ib] = new DBuf<unsigned char>;
+    ibr[ib]->reserve(std::max(V, 1));
+    cancelv[ib]->reserve(std::max(V, 1));
+    tmp.reserve(std::max(V, 1));
+    ct.i2d[ib] = b.interp2donor.p;
+    ct.cancel[ib] = b.interp_cancel.p;
+    if (V == 0) continue;
+    k_rf_init<<<nb(V), BS, 0, s_>>>(V, b.iblank.p, ibr[ib]->p, tmp.p);
+    for (int it = 0; it < nfringe + 1; it++) {
+      if (N) k_rf_sweep<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, ibr[ib]->p, b.iblank.p, tmp.p);
+      TG_HIP(hipMemcpyAsync(ibr[ib]->p, tmp.p, sizeof(int) * V, hipMemcpyDeviceToDevice, s_));
+    }
+    k_rf_cancel<<<nb(V), BS, 0, s_>>>(V, b.iblank.p, ibr[ib]->p, cancelv[ib]->p);
+  }
+  DevStruct<CancelTables> dct;
+  CancelTables *pct = dct.put(ct, s_);
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    if (b.nnodes && b.has_donor.p)
+      k_cancel<<<nb(b.nnodes), BS, 0, s_>>>(b.nnodes, cancelv[ib]->p, b.has_donor.p, b.head_blk.p,
+                                            b.head_remote.p, pct);
+  }
+  DBuf<int> ireset;
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    if (b.nsearch == 0 || b.interp2donor.p == nullptr) continue;
+    ireset.reserve(b.nsearch);
+    fill<int>(ireset.p, b.nsearch, 1, s_);
+    k_coinc1<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.interp2donor.p, b.xtag.p, b.interp_cancel.p, ireset.p);
+    k_coinc2<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.interp2donor.p, b.xtag.p, ireset.p, b.interp_cancel.p);
+  }
+  IblankTables it{};
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    if (b.nnodes) k_rf_clear<<<nb(b.nnodes), BS, 0, s_>>>(b.nnodes, ibr[ib]->p, b.iblank.p);
+    it.ib[ib] = b.iblank.p;
+  }
+  DevStruct<IblankTables> dit;
+  IblankTables *pit = dit.put(it, s_);
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    if (b.ninterp)
+      k_set_receptors<<<nb(b.ninterp), BS, 0, s_>>>(b.ninterp, b.interp_cancel.p, b.interp_rnode.p,
+                                                    b.interp_rblk.p, pit);
+    b.interp_host_valid = false;
+  }
+  for (Block *bp : blocks_) {
+    Block &b = *bp;
+    if (b.writeback && b.nnodes)
+      TG_HIP(hipMemcpyAsync(b.uiblank, b.iblank.p, sizeof(int) * b.nnodes, hipMemcpyDeviceToHost, s_));
+  }
+  sync();
+  for (int ib = 0; ib < nbk; ib++) {
+    delete ibr[ib];
+    delete cancelv[ib];
+  }
 }
 
 // tioga::dataUpdate (tioga.C:336-524), at_points = 0

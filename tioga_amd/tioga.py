@@ -182,6 +182,13 @@ class Tioga:
         self.L.tioga_performconnectivity_()
         self._check()
 
+    def reduce_fringes(self):
+        """tioga_reduce_fringes_ (tiogaInterface.C:302-305): thin the fringes
+        of the last connectivity to nfringe + 1 layers; updates the
+        registered iblank arrays."""
+        self.L.tioga_reduce_fringes_()
+        self._check()
+
     def register_solution(self, btag: int, q: np.ndarray):
         self.q[btag] = q
         self.L.tioga_registersolution_(_ip(btag), _p(q))
