@@ -447,6 +447,17 @@ def main() -> int:
                          f"{tsamp:.1f} s"}
 
     bdu = dataupdate_bytes(tg, blocks, nvar)
+    # tioga::reduce_fringes on this connectivity (last: it changes the
+    # interpolation records), iblank written back to the host arrays included
+    nrec_before = int(sum(int((ib == -1).sum()) for ib in tg.iblank))
+    torch.cuda.synchronize()
+    t0 = time.time()
+    tg.reduce_fringes()
+    t_rf = time.time() - t0
+    rf = {"ms": round(1000 * t_rf, 3), "receptors_before": nrec_before,
+          "receptors_after": int(sum(int((ib == -1).sum()) for ib in tg.iblank)),
+          "holes_after": int(sum(int((ib == 0).sum()) for ib in tg.iblank)), "nfringe": 1}
+    log(f"reduce_fringes {rf}")
     tg.close()
     amr = None if args.no_amr else amr_gather_leg(blocks[0])
     out = {
@@ -476,6 +487,7 @@ def main() -> int:
         "dataupdate": {"ms": round(du_ms, 4), "receptors": int(nrec), "nvar": nvar,
                        "algorithmic_bytes": bdu,
                        "GBps": round(bdu / (du_ms / 1000) / 1e9, 2) if du_ms > 0 else None},
+        "reduce_fringes": rf,
         "amr_gather": amr,
     }
     print(json.dumps(out), flush=True)

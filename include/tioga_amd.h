@@ -205,6 +205,11 @@ void tioga_amd_phase_times(double *out);
    of utils.c:620-737. */
 void tioga_amd_host_median(int *ix, double *x, int n, int lists);
 void tioga_amd_host_find_obb(const double *x, int n, double *out);
+/* the same OBB from component arrays, its running sums one chain per thread
+   and its extents fold over nthreads chunks (the form used for large query
+   sets); bit-identical to tioga_amd_host_find_obb */
+void tioga_amd_host_find_obb_soa(const double *x, const double *y, const double *z, int n, int nthreads,
+                                 double *out);
 int tioga_amd_host_build_adt(int nelem, const double *ebox, int *adtInt, double *adtReal, double *ext);
 /* The device ADT builder on ntree trees at once (tree t: n[t] boxes; all
    arrays concatenated in tree order, host memory; ext: 6 per tree); returns

@@ -543,6 +543,17 @@ void tioga_amd_host_find_obb(const double *x, int n, double *out)
   memcpy(out + 6, o.vec, 72);
 }
 
+void tioga_amd_host_find_obb_soa(const double *x, const double *y, const double *z, int n, int nthreads,
+                                 double *out)
+{
+  tg::OBB o;
+  const double *const c[3] = {x, y, z};
+  tg::find_obb_soa(c, n, o, nthreads);
+  memcpy(out, o.xc, 24);
+  memcpy(out + 3, o.dxc, 24);
+  memcpy(out + 6, o.vec, 72);
+}
+
 int tioga_amd_host_build_adt(int nelem, const double *ebox, int *adtInt, double *adtReal, double *ext)
 {
   return tg::build_adt(nelem, ebox, adtInt, adtReal, ext);

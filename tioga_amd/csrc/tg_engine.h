@@ -373,6 +373,7 @@ class Engine {
   void dedupe_gid(Block &b);
   double *pinned_ = nullptr;  // page-locked staging of the query points
   size_t pinned_cap_ = 0;
+  DBuf<double> soa_;          // a large query set as component arrays
   void exchange_donors();
   void cell_iblanks(Block &b);
   void launch_search(Block &b);

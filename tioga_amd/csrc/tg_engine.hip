@@ -799,6 +799,14 @@ __global__ void k_adt_nodes(int nelem, TypeInfo ti, const int *__restrict__ adtI
   nodes[n] = N;
 }
 
+// query points as three component arrays: out[j*n + i] = xs[3i + j]
+__global__ void k_aos_to_soa(int n, const double *__restrict__ xs, double *__restrict__ out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int j = 0; j < 3; j++) out[(size_t)j * n + i] = xs[3 * (size_t)i + j];
+}
+
 // 30-bit Morton code of each query in the frame of the search structure's
 // root extents, so a wave takes a compact cluster of points (coherent
 // traversal paths and cell data in cache); the search order does not change
@@ -1423,23 +1431,57 @@ struct RecordList {
   double *rres;
 };
 
-__global__ void k_emit_records(int n, const int *__restrict__ sel, const int *__restrict__ hblk,
-                               const int *__restrict__ hrem, const double *__restrict__ nodeRes, double scale,
-                               int off, RecordList L)
+// Per receptor block, the head-donor arrays the records are emitted from;
+// nodes of block b are numbered noff[b] .. noff[b+1]-1 in one flag array.
+struct EmitTables {
+  int nbk;
+  const int *noff;
+  const int *hblk[128], *hrem[128];
+  const double *nodeRes[128];
+  double scale[128];
+};
+
+// the records of every receptor block at once, in (receptor block, node)
+// order: sel lists the flagged global node numbers in ascending order
+__global__ void k_emit_records_all(int n, const int *__restrict__ sel, const EmitTables *__restrict__ T,
+                                   RecordList L)
 {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  int i = sel[j];
-  L.node[off + j] = i;
-  L.dblk[off + j] = hblk[i];
-  L.remote[off + j] = hrem[i];
-  L.rres[off + j] = (scale > 1.0) ? -nodeRes[i] : nodeRes[i];
+  const int g = sel[j];
+  int lo = 0, hi = T->nbk - 1;  // last block with noff <= g
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (T->noff[m] <= g) lo = m;
+    else hi = m - 1;
+  }
+  const int i = g - T->noff[lo];
+  L.node[j] = i;
+  L.dblk[j] = T->hblk[lo][i];
+  L.remote[j] = T->hrem[lo][i];
+  L.rres[j] = (T->scale[lo] > 1.0) ? -T->nodeRes[lo][i] : T->nodeRes[lo][i];
 }
 
-__global__ void k_flag_dblk(int n, const int *__restrict__ dblk, int d, int *__restrict__ flag)
+__global__ void k_dblk_keys(int n, const int *__restrict__ dblk, unsigned *__restrict__ key, int *__restrict__ idx)
 {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) flag[j] = (dblk[j] == d);
+  if (j >= n) return;
+  key[j] = (unsigned)dblk[j];
+  idx[j] = j;
+}
+
+// first position of each donor block's records in the sorted keys
+__global__ void k_dblk_starts(int nbk, int n, const unsigned *__restrict__ sorted, int *__restrict__ start)
+{
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nbk) return;
+  int lo = 0, hi = n;  // first j with sorted[j] >= b
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (sorted[m] < (unsigned)b) lo = m + 1;
+    else hi = m;
+  }
+  start[b] = lo;
 }
 
 struct InterpView {
@@ -2704,14 +2746,26 @@ void Engine::search_batched()
     TG_HIP(hipHostMalloc((void **)&pinned_, sizeof(double) * want, hipHostMallocDefault));
     pinned_cap_ = want;
   }
+  // large sets come over as three component arrays (transposed on the
+  // device), so their OBB sums can run one chain per thread
+  constexpr int OBB_SOA_MIN = 1 << 20;
   std::vector<size_t> off(nbk, 0);
+  std::vector<char> soa(nbk, 0);
   {
     size_t o = 0;
     for (int ib : act) {
+      Block &b = *blocks_[ib];
+      const int n = b.nsearch;
       off[ib] = o;
-      TG_HIP(hipMemcpyAsync(pinned_ + o, blocks_[ib]->xsearch.p, sizeof(double) * 3 * blocks_[ib]->nsearch,
-                            hipMemcpyDeviceToHost, s_));
-      o += 3 * (size_t)blocks_[ib]->nsearch;
+      if (n >= OBB_SOA_MIN) {
+        soa[ib] = 1;
+        soa_.reserve(3 * (size_t)n);
+        k_aos_to_soa<<<nb(n), BS, 0, s_>>>(n, b.xsearch.p, soa_.p);
+        TG_HIP(hipMemcpyAsync(pinned_ + o, soa_.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s_));
+      } else {
+        TG_HIP(hipMemcpyAsync(pinned_ + o, b.xsearch.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s_));
+      }
+      o += 3 * (size_t)n;
     }
   }
   sync();
@@ -2720,13 +2774,20 @@ void Engine::search_batched()
   std::vector<int> order = act;
   std::sort(order.begin(), order.end(), [&](int a, int b) { return blocks_[a]->nsearch > blocks_[b]->nsearch; });
   std::atomic<int> next{0};
+  const int hw = std::max(1, (int)std::thread::hardware_concurrency());
   auto work = [&] {
     for (int k; (k = next.fetch_add(1)) < (int)order.size();) {
       const int ib = order[k];
-      find_obb(pinned_ + off[ib], nullptr, blocks_[ib]->nsearch, obq[ib]);
+      const int n = blocks_[ib]->nsearch;
+      if (soa[ib]) {
+        const double *const c[3] = {pinned_ + off[ib], pinned_ + off[ib] + n, pinned_ + off[ib] + 2 * (size_t)n};
+        find_obb_soa(c, n, obq[ib], hw);
+      } else {
+        find_obb(pinned_ + off[ib], nullptr, n, obq[ib]);
+      }
     }
   };
-  const int nt = std::max(1, std::min((int)order.size(), (int)std::thread::hardware_concurrency()));
+  const int nt = std::max(1, std::min((int)order.size(), hw));
   std::vector<std::thread> th;
   for (int t = 0; t < nt; t++) th.emplace_back(work);
   auto td0 = now();
@@ -3077,13 +3138,28 @@ void Engine::exchange_donors()
   dtag.reserve(nbk);
   TG_HIP(hipMemcpyAsync(dtag.p, blk_tag.data(), sizeof(int) * nbk, hipMemcpyHostToDevice, s_));
   HoleSet H{nmesh_, hm_view_.p};
-  int totrec = 0;
-  std::vector<DBuf<int> *> sel(nbk);
-  std::vector<int> nrec(nbk);
+  std::vector<int> noff(nbk + 1, 0);
+  for (int ib = 0; ib < nbk; ib++) {
+    noff[ib + 1] = noff[ib] + blocks_[ib]->nnodes;
+    if (noff[ib + 1] < noff[ib]) throw std::runtime_error("tioga_amd: too many nodes for one donor exchange");
+  }
+  const int totV = noff[nbk];
+  DBuf<int> pflag, psel, dnoff;
+  pflag.reserve(std::max(totV, 1));
+  psel.reserve(std::max(totV, 1));
+  dnoff.reserve(nbk + 1);
+  TG_HIP(hipMemcpyAsync(dnoff.p, noff.data(), sizeof(int) * (nbk + 1), hipMemcpyHostToDevice, s_));
+  EmitTables et{};
+  et.nbk = nbk;
+  et.noff = dnoff.p;
   DBuf<unsigned char> mt, mt1;
   for (int ib = 0; ib < nbk; ib++) {
     Block &b = *blocks_[ib];
     const int V = b.nnodes, N = b.ncells;
+    et.hblk[ib] = b.head_blk.p;
+    et.hrem[ib] = b.head_remote.p;
+    et.nodeRes[ib] = b.nodeRes.p;
+    et.scale[ib] = b.resolutionScale;
     mt.reserve(V);
     mt1.reserve(V);
     if (V)
@@ -3094,15 +3170,12 @@ void Engine::exchange_donors()
       if (N) k_fringe<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, mt.p, mt1.p);
       TG_HIP(hipMemcpyAsync(mt.p, mt1.p, V, hipMemcpyDeviceToDevice, s_));
     }
-    iscratch_.reserve(V);
     if (V)
       k_process2<<<nb(V), BS, 0, s_>>>(V, mt1.p, b.has_donor.p, b.head_key.p, b.head_blk.p, dtag.p, b.iblank.p,
-                                       b.nodeRes.p, iscratch_.p);
-    sel[ib] = new DBuf<int>;
-    sel[ib]->reserve(V);
-    nrec[ib] = select_flagged(iscratch_.p, V, sel[ib]->p, s_);
-    totrec += nrec[ib];
+                                       b.nodeRes.p, pflag.p + noff[ib]);
   }
+  // the records of all receptor blocks, in (receptor block, node) order
+  const int totrec = totV ? select_flagged(pflag.p, totV, psel.p, s_) : 0;
   RecordList L;
   DBuf<int> lnode, ldblk, lrem;
   DBuf<double> lrres;
@@ -3111,27 +3184,40 @@ void Engine::exchange_donors()
   L.dblk = ldblk.reserve(totrec);
   L.remote = lrem.reserve(totrec);
   L.rres = lrres.reserve(totrec);
-  for (int ib = 0, off = 0; ib < nbk; ib++) {
-    Block &b = *blocks_[ib];
-    if (nrec[ib])
-      k_emit_records<<<nb(nrec[ib]), BS, 0, s_>>>(nrec[ib], sel[ib]->p, b.head_blk.p, b.head_remote.p,
-                                                  b.nodeRes.p, b.resolutionScale, off, L);
-    off += nrec[ib];
-    delete sel[ib];
+  DevStruct<EmitTables> det;
+  const EmitTables *pet = det.put(et, s_);
+  if (totrec) k_emit_records_all<<<nb(totrec), BS, 0, s_>>>(totrec, psel.p, pet, L);
+  // comm2: findInterpData per donor block, records in arrival order: a
+  // stable sort by donor block keeps each donor's records in list order
+  DBuf<int> dsel, tmin, dstart, didx;
+  DBuf<unsigned> dkey0, dkey1;
+  dsel.reserve(std::max(totrec, 1));
+  didx.reserve(std::max(totrec, 1));
+  dkey0.reserve(std::max(totrec, 1));
+  dkey1.reserve(std::max(totrec, 1));
+  dstart.reserve(nbk + 1);
+  std::vector<int> ndv(nbk, 0), doff(nbk + 1, 0);
+  if (totrec) {
+    k_dblk_keys<<<nb(totrec), BS, 0, s_>>>(totrec, L.dblk, dkey0.p, didx.p);
+    int bits = 1;
+    while (bits < 31 && (1 << bits) < nbk) bits++;
+    sort_pairs(dkey0.p, dkey1.p, didx.p, dsel.p, totrec, bits, s_);
+    k_dblk_starts<<<nb(nbk + 1), BS, 0, s_>>>(nbk, totrec, dkey1.p, dstart.p);
+    TG_HIP(hipMemcpyAsync(doff.data(), dstart.p, sizeof(int) * (nbk + 1), hipMemcpyDeviceToHost, s_));
+    sync();
   }
-  // comm2: findInterpData per donor block, records in arrival order
-  DBuf<int> flag, dsel, tmin;
+  for (int ib = 0; ib < nbk; ib++) ndv[ib] = doff[ib + 1] - doff[ib];
+  size_t maxV = 1;
+  for (Block *b : blocks_) maxV = std::max<size_t>(maxV, b->nnodes);
+  tmin.reserve(maxV);
+  DBuf<int> errs;
+  errs.reserve(nbk);
+  fill<int>(errs.p, nbk, 0, s_);
   std::vector<DBuf<unsigned char> *> cancelv(nbk);
-  flag.reserve(totrec);
   CancelTables ct{};
   for (int ib = 0; ib < nbk; ib++) {
     Block &D = *blocks_[ib];
-    int nd = 0;
-    dsel.reserve(totrec);
-    if (totrec) {
-      k_flag_dblk<<<nb(totrec), BS, 0, s_>>>(totrec, L.dblk, ib, flag.p);
-      nd = select_flagged(flag.p, totrec, dsel.p, s_);
-    }
+    const int nd = ndv[ib];
     D.ninterp = nd;
     D.interp_inode.reserve(9 * (size_t)nd);
     D.interp_w.reserve(9 * (size_t)nd);
@@ -3151,7 +3237,7 @@ void Engine::exchange_donors()
     if (nd == 0) continue;
     InterpView V;
     V.n = nd;
-    V.sel = dsel.p;
+    V.sel = dsel.p + doff[ib];
     V.L = L;
     V.ti = D.ti;
     V.conn = D.conn.p;
@@ -3172,19 +3258,22 @@ void Engine::exchange_donors()
     V.rblk = D.interp_rblk.p;
     V.i2d = D.interp2donor.p;
     V.w = D.interp_w.p;
-    V.err = err_.p;
-    TG_HIP(hipMemsetAsync(err_.p, 0, sizeof(int), s_));
+    V.err = errs.p + ib;
     k_interp_mandatory<<<nb(nd), BS, 0, s_>>>(V);
     k_interp<<<nb(nd), BS, 0, s_>>>(V);
     k_interp_reset<<<nb(D.nnodes), BS, 0, s_>>>(D.nnodes, tmin.p, D.iblank.p);
-    int nbad = 0;
-    TG_HIP(hipMemcpyAsync(&nbad, err_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
+  }
+  {
+    std::vector<int> nbad(nbk, 0);
+    TG_HIP(hipMemcpyAsync(nbad.data(), errs.p, sizeof(int) * nbk, hipMemcpyDeviceToHost, s_));
     sync();
-    if (nbad) {
-      weight_aborts_ += nbad;
-      fprintf(stderr, "tioga_amd: %d donor weights outside [-0.2, 1.2] in block %d (the reference aborts here)\n",
-              nbad, D.tag);
-    }
+    for (int ib = 0; ib < nbk; ib++)
+      if (nbad[ib]) {
+        weight_aborts_ += nbad[ib];
+        fprintf(stderr,
+                "tioga_amd: %d donor weights outside [-0.2, 1.2] in block %d (the reference aborts here)\n",
+                nbad[ib], blocks_[ib]->tag);
+      }
   }
   // comm3: cancellations, then coincident resets
   DevStruct<CancelTables> dct;

@@ -16,6 +16,8 @@ namespace tg {
 // kaiser.f:29-180).  x is AoS xyz; when idx is non-null the points are
 // x[3*idx[i]..], in that order.
 void find_obb(const double *x, const int *idx, int n, OBB &o);
+// the same OBB from component arrays c[0..2][n], on threads
+void find_obb_soa(const double *const c[3], int n, OBB &o, int nthreads);
 
 // 15-axis separating-axis test with eps 1e-12 (utils.c:297-384), including
 // the reference's |vA[i][j]|*D[j] projection of the centre offset.
