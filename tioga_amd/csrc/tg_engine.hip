@@ -542,75 +542,117 @@ __global__ void __launch_bounds__(BS) k_reduced_obb_multi(int ncells, TypeInfo t
                                                           const BoxPairs *__restrict__ P,
                                                           unsigned long long *__restrict__ out)
 {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = c < ncells;
-  int nv = 0;
-  const int *cv = conn + 8 * (size_t)(valid ? c : 0);
-  double lo[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, hi[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
-  double cen[3] = {0, 0, 0}, rad = 0;
-  if (valid) {
-    nv = ti.nv[type_of(ti, c)];
-    for (int m = 0; m < nv; m++) {
-      double p[3], xd[3];
-      load_xyz(x, cv[m], p);
-      to_frame(p, own.xc, own.vec, xd);
-      for (int j = 0; j < 3; j++) {
-        lo[j] = dmin(lo[j], xd[j]);
-        hi[j] = dmax(hi[j], xd[j]);
-        cen[j] += p[j];
-      }
-    }
-    for (int j = 0; j < 3; j++) cen[j] /= nv;
-    for (int m = 0; m < nv; m++) {
-      double p[3];
-      load_xyz(x, cv[m], p);
-      const double d = sqrt((p[0] - cen[0]) * (p[0] - cen[0]) + (p[1] - cen[1]) * (p[1] - cen[1]) +
-                            (p[2] - cen[2]) * (p[2] - cen[2]));
-      rad = dmax(rad, d);
-    }
-    // a margin far above the rounding of the frame transforms
-    rad = rad * (1 + 1e-9) + 1e-12 * (1 + fabs(cen[0]) + fabs(cen[1]) + fabs(cen[2]));
-  }
+  // per-workgroup bounds of every pair, folded over a grid-stride loop, so
+  // the global atomics come once per workgroup rather than once per wave
+  __shared__ unsigned long long skl[128 * 3], skh[128 * 3];
   const int np = P->np;
-  for (int k = 0; k < np; k++) {
-    const OBB &obc = P->obc[k];
-    bool keep = false;
+  for (int q = threadIdx.x; q < 3 * np; q += blockDim.x) {
+    skl[q] = dkey(BIGVALUE);
+    skh[q] = dkey(-BIGVALUE);
+  }
+  __syncthreads();
+  for (int base = blockIdx.x * blockDim.x; base < ncells; base += gridDim.x * blockDim.x) {
+    const int c = base + threadIdx.x;
+    const bool valid = c < ncells;
+    int nv = 0;
+    const int *cv = conn + 8 * (size_t)(valid ? c : 0);
+    double lo[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, hi[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
+    double cen[3] = {0, 0, 0}, rad = 0;
     if (valid) {
-      // the cell's bounding sphere misses the partner box: the vertex test
-      // below can not keep it either
-      double cd[3];
-      to_frame(cen, obc.xc, obc.vec, cd);
-      bool near = true;
-      for (int j = 0; j < 3; j++) near = near && (fabs(cd[j]) <= obc.dxc[j] + rad * (1 + 1e-9) + 1e-12);
-      if (near) {
-        double bb[6] = {BIGVALUE, BIGVALUE, BIGVALUE, -BIGVALUE, -BIGVALUE, -BIGVALUE};
-        for (int m = 0; m < nv; m++) {
-          double p[3], xd[3];
-          load_xyz(x, cv[m], p);
-          to_frame(p, obc.xc, obc.vec, xd);
-          for (int j = 0; j < 3; j++) bb[j] = dmin(bb[j], xd[j]);
-          for (int j = 0; j < 3; j++) bb[j + 3] = dmax(bb[j + 3], xd[j]);
+      nv = ti.nv[type_of(ti, c)];
+      for (int m = 0; m < nv; m++) {
+        double p[3], xd[3];
+        load_xyz(x, cv[m], p);
+        to_frame(p, own.xc, own.vec, xd);
+        for (int j = 0; j < 3; j++) {
+          lo[j] = dmin(lo[j], xd[j]);
+          hi[j] = dmax(hi[j], xd[j]);
+          cen[j] += p[j];
         }
-        keep = true;
-        for (int j = 0; j < 3; j++)
-          if (bb[j] > obc.dxc[j]) keep = false;
-        for (int j = 0; j < 3; j++)
-          if (bb[j + 3] < -obc.dxc[j]) keep = false;
       }
+      for (int j = 0; j < 3; j++) cen[j] /= nv;
+      for (int m = 0; m < nv; m++) {
+        double p[3];
+        load_xyz(x, cv[m], p);
+        const double d = sqrt((p[0] - cen[0]) * (p[0] - cen[0]) + (p[1] - cen[1]) * (p[1] - cen[1]) +
+                              (p[2] - cen[2]) * (p[2] - cen[2]));
+        rad = dmax(rad, d);
+      }
+      // a margin far above the rounding of the frame transforms
+      rad = rad * (1 + 1e-9) + 1e-12 * (1 + fabs(cen[0]) + fabs(cen[1]) + fabs(cen[2]));
     }
-    if (__ballot(keep) == 0) continue;
+    // the wave's sphere: around the box of its cells' spheres, so it holds
+    // every one of them; a partner box it misses is missed by every cell
+    double wlo[3], whi[3];
     for (int j = 0; j < 3; j++) {
-      unsigned long long kl = keep ? dkey(lo[j]) : dkey(BIGVALUE), kh = keep ? dkey(hi[j]) : dkey(-BIGVALUE);
+      double a = valid ? cen[j] - rad : BIGVALUE, b = valid ? cen[j] + rad : -BIGVALUE;
+#pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long a = __shfl_xor(kl, o), b = __shfl_xor(kh, o);
-        kl = a < kl ? a : kl;
-        kh = b > kh ? b : kh;
+        a = dmin(a, __shfl_xor(a, o));
+        b = dmax(b, __shfl_xor(b, o));
       }
-      if ((threadIdx.x & 63) == 0) {
-        atomicMin(&out[6 * (size_t)P->pair[k] + j], kl);
-        atomicMax(&out[6 * (size_t)P->pair[k] + 3 + j], kh);
+      wlo[j] = a;
+      whi[j] = b;
+    }
+    double wc[3], wr = 0;
+    for (int j = 0; j < 3; j++) {
+      wc[j] = 0.5 * (wlo[j] + whi[j]);
+      wr += (whi[j] - wlo[j]) * (whi[j] - wlo[j]);
+    }
+    wr = 0.5 * sqrt(wr) * (1 + 1e-9) + 1e-12 * (1 + fabs(wc[0]) + fabs(wc[1]) + fabs(wc[2]));
+    for (int k = 0; k < np; k++) {
+      const OBB &obc = P->obc[k];
+      {
+        double cd[3];
+        to_frame(wc, obc.xc, obc.vec, cd);
+        bool near = true;
+        for (int j = 0; j < 3; j++) near = near && (fabs(cd[j]) <= obc.dxc[j] + wr * (1 + 1e-9) + 1e-12);
+        if (!__ballot(near)) continue;  // the same value in every lane
+      }
+      bool keep = false;
+      if (valid) {
+        // the cell's bounding sphere misses the partner box: the vertex test
+        // below can not keep it either
+        double cd[3];
+        to_frame(cen, obc.xc, obc.vec, cd);
+        bool near = true;
+        for (int j = 0; j < 3; j++) near = near && (fabs(cd[j]) <= obc.dxc[j] + rad * (1 + 1e-9) + 1e-12);
+        if (near) {
+          double bb[6] = {BIGVALUE, BIGVALUE, BIGVALUE, -BIGVALUE, -BIGVALUE, -BIGVALUE};
+          for (int m = 0; m < nv; m++) {
+            double p[3], xd[3];
+            load_xyz(x, cv[m], p);
+            to_frame(p, obc.xc, obc.vec, xd);
+            for (int j = 0; j < 3; j++) bb[j] = dmin(bb[j], xd[j]);
+            for (int j = 0; j < 3; j++) bb[j + 3] = dmax(bb[j + 3], xd[j]);
+          }
+          keep = true;
+          for (int j = 0; j < 3; j++)
+            if (bb[j] > obc.dxc[j]) keep = false;
+          for (int j = 0; j < 3; j++)
+            if (bb[j + 3] < -obc.dxc[j]) keep = false;
+        }
+      }
+      if (__ballot(keep) == 0) continue;
+      for (int j = 0; j < 3; j++) {
+        unsigned long long kl = keep ? dkey(lo[j]) : dkey(BIGVALUE), kh = keep ? dkey(hi[j]) : dkey(-BIGVALUE);
+        for (int o = 32; o > 0; o >>= 1) {
+          unsigned long long a = __shfl_xor(kl, o), b = __shfl_xor(kh, o);
+          kl = a < kl ? a : kl;
+          kh = b > kh ? b : kh;
+        }
+        if ((threadIdx.x & 63) == 0) {
+          atomicMin(&skl[3 * k + j], kl);
+          atomicMax(&skh[3 * k + j], kh);
+        }
       }
     }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < 3 * np; q += blockDim.x) {
+    const int k = q / 3, j = q % 3;
+    if (skl[q] != dkey(BIGVALUE)) atomicMin(&out[6 * (size_t)P->pair[k] + j], skl[q]);
+    if (skh[q] != dkey(-BIGVALUE)) atomicMax(&out[6 * (size_t)P->pair[k] + 3 + j], skh[q]);
   }
 }
 
@@ -2481,8 +2523,8 @@ void Engine::exchange_boxes()
         }
       if (bp.np < 2 || A.ncells == 0) continue;
       tabs.push_back(new DevStruct<BoxPairs>);
-      k_reduced_obb_multi<<<nb(A.ncells), BS, 0, s_>>>(A.ncells, A.ti, A.conn.p, A.x.p, A.obb,
-                                                       tabs.back()->put(bp, s_), kscratch_.p);
+      k_reduced_obb_multi<<<std::min(nb(A.ncells), 1024u), BS, 0, s_>>>(A.ncells, A.ti, A.conn.p, A.x.p, A.obb,
+                                                                        tabs.back()->put(bp, s_), kscratch_.p);
     }
     sync();
     for (auto *t : tabs) delete t;
