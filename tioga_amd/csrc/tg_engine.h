@@ -126,6 +126,8 @@ struct Block {
   DBuf<int> conn;          // 8 per cell, 0-based, unused slots -1
   DBuf<int> n2c_off, n2c;  // node -> incident cells, cell order (with multiplicity)
   DBuf<unsigned char> wallnode, obcnode;
+  DBuf<int> wallcells;  // cells with a wall node (hole-map marking)
+  int nwallcells = 0;
   DBuf<int> iblank, iblank_cell;
   DBuf<double> cellRes, nodeRes;
   DBuf<unsigned> nodebin;  // 10^3 bin of each node in the block's OBB frame
@@ -374,6 +376,8 @@ class Engine {
   double *pinned_ = nullptr;  // page-locked staging of the query points
   size_t pinned_cap_ = 0;
   DBuf<double> soa_;          // a large query set as component arrays
+  DBuf<double> wallds_;       // hole-map cell sizes, per mesh tag
+  DBuf<unsigned char *> wallsams_;  // hole maps, per mesh tag
   void exchange_donors();
   void cell_iblanks(Block &b);
   void launch_search(Block &b);

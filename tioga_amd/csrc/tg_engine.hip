@@ -188,32 +188,66 @@ __global__ void k_mexclude(int ncells, TypeInfo ti, const int *__restrict__ conn
 
 // Wall-touching cells paint their index box (floor of the offset / spacing,
 // clipped to the map) with 2 (MeshBlock.C:666-772).
-__global__ void k_mark_wall(int ncells, TypeInfo ti, const int *__restrict__ conn, const double *__restrict__ x,
-                            const unsigned char *__restrict__ wallnode, HoleMapView h, double ds0, double ds1,
-                            double ds2, unsigned char *__restrict__ sam)
+// cells with a wall node among their vertices
+__global__ void k_wall_cells(int ncells, TypeInfo ti, const int *__restrict__ conn,
+                             const unsigned char *__restrict__ wallnode, int *__restrict__ flag)
 {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
-  int nv = ti.nv[type_of(ti, c)];
+  const int nv = ti.nv[type_of(ti, c)];
   const int *cv = conn + 8 * (size_t)c;
   int w = 0;
   for (int m = 0; m < nv; m++) w |= wallnode[cv[m]];
-  if (!w) return;
-  double ds[3] = {ds0, ds1, ds2};
+  flag[c] = w;
+}
+
+// Every walled block's wall cells in one launch: cell k of the concatenated
+// lists belongs to block b with off[b] <= k < off[b+1].
+struct WallSet {
+  int nb;
+  int off[129];
+  const int *cells[128];
+  const int *conn[128];
+  const double *x[128];
+  TypeInfo ti[128];
+  int map[128];  // hole map (mesh tag - 1) of the block
+};
+
+// MeshBlock::markWallBoundary (MeshBlock.C:637-700): the map cells covered
+// by the index box of each wall cell's vertices
+__global__ void k_mark_wall(const WallSet *__restrict__ W, const HoleMapView *__restrict__ views,
+                            const double *__restrict__ dsv, unsigned char *const *__restrict__ sams)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= W->off[W->nb]) return;
+  int lo = 0, hi = W->nb - 1;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (W->off[m] <= k) lo = m;
+    else hi = m - 1;
+  }
+  const int c = W->cells[lo][k - W->off[lo]];
+  const TypeInfo &ti = W->ti[lo];
+  const int nv = ti.nv[type_of(ti, c)];
+  const int *cv = W->conn[lo] + 8 * (size_t)c;
+  const int mp = W->map[lo];
+  const HoleMapView &h = views[mp];
+  const double ds[3] = {dsv[3 * mp], dsv[3 * mp + 1], dsv[3 * mp + 2]};
   int imin[3] = {BIGINT, BIGINT, BIGINT}, imax[3] = {-BIGINT, -BIGINT, -BIGINT};
   for (int m = 0; m < nv; m++) {
     double p[3];
-    load_xyz(x, cv[m], p);
-    for (int k = 0; k < 3; k++) {
-      int iv = (int)floor((p[k] - h.ext[k]) / ds[k]);
-      imin[k] = iv < imin[k] ? iv : imin[k];
-      imax[k] = iv > imax[k] ? iv : imax[k];
+    load_xyz(W->x[lo], cv[m], p);
+    for (int q = 0; q < 3; q++) {
+      int iv = (int)floor((p[q] - h.ext[q]) / ds[q]);
+      imin[q] = iv < imin[q] ? iv : imin[q];
+      imax[q] = iv > imax[q] ? iv : imax[q];
     }
   }
   for (int j = 0; j < 3; j++) {
     if (imin[j] < 0) imin[j] = 0;
     if (imax[j] > h.nx[j] - 1) imax[j] = h.nx[j] - 1;
   }
+  unsigned char *sam = sams[mp];
   for (int kk = imin[2]; kk <= imax[2]; kk++)
     for (int jj = imin[1]; jj <= imax[1]; jj++)
       for (int ii = imin[0]; ii <= imax[0]; ii++)
@@ -2156,6 +2190,16 @@ void Engine::tag_boundary(Block &b)
     TG_HIP(hipMemcpyAsync(lst.p, b.uwbc, sizeof(int) * b.nwbc, hipMemcpyHostToDevice, s_));
     k_flag_list<<<nb(b.nwbc), BS, 0, s_>>>(b.nwbc, lst.p, b.wallnode.p);
   }
+  // the cells touching a wall node, the only ones that mark the hole map
+  if (b.share == 0) {
+    b.nwallcells = 0;
+    if (b.nwbc > 0 && N > 0) {
+      iscratch_.reserve(N);
+      k_wall_cells<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, b.wallnode.p, iscratch_.p);
+      b.wallcells.reserve(N);
+      b.nwallcells = select_flagged(iscratch_.p, N, b.wallcells.p, s_);
+    }
+  }
   sync();
 }
 
@@ -2240,6 +2284,8 @@ void Engine::adopt_block(const Engine &src, int src_tag, int as_tag, int share, 
   b->n2c_off.borrow(S.n2c_off);
   b->n2c.borrow(S.n2c);
   b->wallnode.borrow(S.wallnode);
+  b->wallcells.borrow(S.wallcells);
+  b->nwallcells = S.nwallcells;
   b->obcnode.borrow(S.obcnode);
   b->topo_uploaded = true;
   if (share == 2) {
@@ -2400,14 +2446,40 @@ void Engine::get_hole_map()
     pms = dms.put(ms, s_);
     if (batched) k_maps_cells<<<dim3(nb(maxcells), ms.nmap), BS, 0, s_>>>(pms, 0);
   }
-  for (Block *bp : blocks_) {
-    Block &b = *bp;
-    HoleMap &h = hm_[b.tag - 1];
-    if (!h.exist || b.ncells == 0) continue;
-    double ds[3];
-    for (int k = 0; k < 3; k++) ds[k] = (h.ext[k + 3] - h.ext[k]) / h.nx[k];
-    k_mark_wall<<<nb(b.ncells), BS, 0, s_>>>(b.ncells, b.ti, b.conn.p, b.x.p, b.wallnode.p, views[b.tag - 1],
-                                             ds[0], ds[1], ds[2], h.sam.p);
+  {
+    WallSet W{};
+    std::vector<double> dsv(3 * (size_t)std::max(maxtag, 1), 1.0);
+    std::vector<unsigned char *> sams(std::max(maxtag, 1), nullptr);
+    for (int i = 0; i < maxtag; i++) {
+      HoleMap &h = hm_[i];
+      if (!h.exist) continue;
+      for (int k = 0; k < 3; k++) dsv[3 * i + k] = (h.ext[k + 3] - h.ext[k]) / h.nx[k];
+      sams[i] = h.sam.p;
+    }
+    for (Block *bp : blocks_) {
+      Block &b = *bp;
+      HoleMap &h = hm_[b.tag - 1];
+      if (!h.exist || b.nwallcells == 0) continue;
+      const int q = W.nb++;
+      W.off[q + 1] = W.off[q] + b.nwallcells;
+      W.cells[q] = b.wallcells.p;
+      W.conn[q] = b.conn.p;
+      W.x[q] = b.x.p;
+      W.ti[q] = b.ti;
+      W.map[q] = b.tag - 1;
+    }
+    if (W.nb > 0) {
+      DevStruct<WallSet> dw;
+      hm_view_.reserve(maxtag);
+      TG_HIP(hipMemcpyAsync(hm_view_.p, views.data(), sizeof(HoleMapView) * maxtag, hipMemcpyHostToDevice, s_));
+      wallds_.reserve(dsv.size());
+      wallsams_.reserve(sams.size());
+      TG_HIP(hipMemcpyAsync(wallds_.p, dsv.data(), sizeof(double) * dsv.size(), hipMemcpyHostToDevice, s_));
+      TG_HIP(hipMemcpyAsync(wallsams_.p, sams.data(), sizeof(unsigned char *) * sams.size(),
+                            hipMemcpyHostToDevice, s_));
+      k_mark_wall<<<nb(W.off[W.nb]), BS, 0, s_>>>(dw.put(W, s_), hm_view_.p, wallds_.p, wallsams_.p);
+      sync();  // the tables above are host memory until the copies have run
+    }
   }
   nmesh_ = maxtag;
   if (batched && !live.empty()) {
