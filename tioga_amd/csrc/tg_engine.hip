@@ -491,6 +491,12 @@ __global__ void k_maps_line_fill(const MapSet *__restrict__ M, int axis, int all
 // checkHoleMap (utils.c:170-185): truncation toward zero, range test on the int
 __device__ __forceinline__ int check_hole(const double p[3], const HoleMapView &h)
 {
+  // a point a whole map span outside the map's extents has |index| >= nx
+  // along that axis, whatever the rounding: out, without the divisions
+  for (int i = 0; i < 3; i++) {
+    const double span = h.ext[i + 3] - h.ext[i];
+    if (p[i] < h.ext[i] - span || p[i] > h.ext[i + 3] + span) return 0;
+  }
   int ix[3];
   for (int i = 0; i < 3; i++) {
     double dx = (h.ext[i + 3] - h.ext[i]) / h.nx[i];
