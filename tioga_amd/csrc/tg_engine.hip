@@ -830,13 +830,13 @@ __global__ void k_cell_filter(int ncells, TypeInfo ti, const int *__restrict__ c
 }
 
 // element list and element boxes (search.C:183-219)
-__global__ void k_elem_box(int nelem, int ncells, const int *__restrict__ sel, TypeInfo ti,
+__global__ void k_elem_box(int nelem, int ncells, const int *__restrict__ sel, int sel0, TypeInfo ti,
                            const int *__restrict__ conn, const double *__restrict__ x, int *__restrict__ elist,
                            double *__restrict__ ebox)
 {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nelem) return;
-  int c = ncells - 1 - sel[e];
+  int c = ncells - 1 - (sel[e] - sel0);
   elist[e] = c;
   int nv = ti.nv[type_of(ti, c)];
   const int *cv = conn + 8 * (size_t)c;
@@ -853,6 +853,23 @@ __global__ void k_elem_box(int nelem, int ncells, const int *__restrict__ sel, T
     ebox[6 * (size_t)e + j] = lo[j];
     ebox[6 * (size_t)e + 3 + j] = hi[j];
   }
+}
+
+// start[s] = first position j of the ascending list sorted[0..n) with
+// sorted[j] >= bound[s], s = 0..nseg
+__global__ void k_seg_starts(int nseg, int n, const int *__restrict__ sorted, const int *__restrict__ bound,
+                             int *__restrict__ start)
+{
+  int sg = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sg > nseg) return;
+  const int v = bound[sg];
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (sorted[m] < v) lo = m + 1;
+    else hi = m;
+  }
+  start[sg] = lo;
 }
 
 static_assert(sizeof(AdtNode) == 160, "ADT node record must be 160 bytes");
@@ -2927,21 +2944,45 @@ void Engine::search_batched()
   auto td1 = now();
   for (auto &t : th) t.join();
   auto t1 = now();
-  // cells overlapping each query OBB, as a LIFO list, and their boxes
+  // cells overlapping each query OBB, as a LIFO list, and their boxes: the
+  // flags of all blocks in one array (block a's cells from coff[a], each
+  // block's reversed), one compaction, and each block's range of it
   std::vector<AdtTree> trees(act.size());
-  for (size_t a = 0; a < act.size(); a++) {
+  const int na = (int)act.size();
+  std::vector<int> coff(na + 1, 0), eoff(na + 1, 0);
+  for (int a = 0; a < na; a++) {
+    coff[a + 1] = coff[a] + blocks_[act[a]]->ncells;
+    if (coff[a + 1] < coff[a]) throw std::runtime_error("tioga_amd: too many cells for one batched search");
+  }
+  const int totN = coff[na];
+  iscratch_.reserve(std::max(totN, 1));
+  iscratch2_.reserve(std::max(totN, 1));
+  for (int a = 0; a < na; a++) {
     Block &b = *blocks_[act[a]];
-    const int N = b.ncells;
-    iscratch_.reserve(N);
-    iscratch2_.reserve(N);
-    if (N) k_cell_filter<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, b.x.p, obq[act[a]], iscratch_.p);
-    const int ne = select_flagged(iscratch_.p, N, iscratch2_.p, s_);
+    if (b.ncells)
+      k_cell_filter<<<nb(b.ncells), BS, 0, s_>>>(b.ncells, b.ti, b.conn.p, b.x.p, obq[act[a]], iscratch_.p + coff[a]);
+  }
+  const int totE = totN ? select_flagged(iscratch_.p, totN, iscratch2_.p, s_) : 0;
+  if (totE) {
+    DBuf<int> dcoff, dstart;
+    dcoff.reserve(na + 1);
+    dstart.reserve(na + 1);
+    TG_HIP(hipMemcpyAsync(dcoff.p, coff.data(), sizeof(int) * (na + 1), hipMemcpyHostToDevice, s_));
+    k_seg_starts<<<nb(na + 1), BS, 0, s_>>>(na, totE, iscratch2_.p, dcoff.p, dstart.p);
+    TG_HIP(hipMemcpyAsync(eoff.data(), dstart.p, sizeof(int) * (na + 1), hipMemcpyDeviceToHost, s_));
+    sync();
+  }
+  for (int a = 0; a < na; a++) {
+    Block &b = *blocks_[act[a]];
+    const int N = b.ncells, ne = eoff[a + 1] - eoff[a];
     b.nelem = ne;
     b.elist.reserve(ne);
     b.ebox.reserve(6 * (size_t)ne);
     b.adtInt.reserve(4 * (size_t)ne);
     b.adtReal.reserve(6 * (size_t)ne);
-    if (ne) k_elem_box<<<nb(ne), BS, 0, s_>>>(ne, N, iscratch2_.p, b.ti, b.conn.p, b.x.p, b.elist.p, b.ebox.p);
+    if (ne)
+      k_elem_box<<<nb(ne), BS, 0, s_>>>(ne, N, iscratch2_.p + eoff[a], coff[a], b.ti, b.conn.p, b.x.p, b.elist.p,
+                                        b.ebox.p);
     trees[a].n = ne;
     trees[a].ebox = b.ebox.p;
     trees[a].adtInt = b.adtInt.p;
@@ -3019,7 +3060,7 @@ void Engine::build_search_structure(Block &b, const double *xq, int nq, int &nel
   ebox.reserve(6 * (size_t)ne);
   adtInt.reserve(4 * (size_t)ne);
   adtReal.reserve(6 * (size_t)ne);
-  if (ne) k_elem_box<<<nb(ne), BS, 0, s_>>>(ne, N, iscratch2_.p, b.ti, b.conn.p, b.x.p, elist.p, ebox.p);
+  if (ne) k_elem_box<<<nb(ne), BS, 0, s_>>>(ne, N, iscratch2_.p, 0, b.ti, b.conn.p, b.x.p, elist.p, ebox.p);
   sync();
   auto t2 = now();
   depth = build_adt_device(ne, ebox.p, adtInt.p, adtReal.p, ext, adtw_, s_);
