@@ -192,8 +192,9 @@ int tioga_amd_query_set_nelem(int id);
 /* Host wall times (ms) of the last performconnectivity phases:
    out[0..6] = hole map, boxes, query gathering, search preparation,
    search kernels (device), donor selection, total; out[7] = searches;
-   out[8..11] = inside search preparation: query-set OBB (host), cell
-   filter and element boxes, ADT build, coincident-point dedupe. */
+   out[8..11] = inside search preparation: query-set OBB, cell filter and
+   element boxes, ADT build, coincident-point dedupe; out[12] = segments of
+   the OBB running sums stepped through term by term. */
 void tioga_amd_phase_times(double *out);
 
 /* Host-side sequential stages, callable without a device (they are what
@@ -205,11 +206,6 @@ void tioga_amd_phase_times(double *out);
    of utils.c:620-737. */
 void tioga_amd_host_median(int *ix, double *x, int n, int lists);
 void tioga_amd_host_find_obb(const double *x, int n, double *out);
-/* the same OBB from component arrays, its running sums one chain per thread
-   and its extents fold over nthreads chunks (the form used for large query
-   sets); bit-identical to tioga_amd_host_find_obb */
-void tioga_amd_host_find_obb_soa(const double *x, const double *y, const double *z, int n, int nthreads,
-                                 double *out);
 int tioga_amd_host_build_adt(int nelem, const double *ebox, int *adtInt, double *adtReal, double *ext);
 /* The device ADT builder on ntree trees at once (tree t: n[t] boxes; all
    arrays concatenated in tree order, host memory; ext: 6 per tree); returns
@@ -223,6 +219,13 @@ void tioga_amd_host_unique_nodes(const double *x, const int *meshtag, double *rt
    error status set.  Needs a GPU. */
 int tioga_amd_device_unique_nodes(int nset, const int *n, const double *x, const int *meshtag, double *rtag,
                                   int *xtag);
+/* The device OBB of findOBB (utils.c:25-161) for nset point sets at once
+   (set k: n[k] > 0 points, AoS, concatenated in set order, host memory;
+   out: 15 doubles per set as tioga_amd_host_find_obb writes them); the bits
+   of tioga_amd_host_find_obb.  stepped (may be NULL) receives the number of
+   512-point segments whose running sums were stepped through term by term.
+   Returns 0, or -1 with the error status set.  Needs a GPU. */
+int tioga_amd_device_find_obbs(int nset, const int *n, const double *x, double *out, long long *stepped);
 
 /* Donor weights outside [-0.2, 1.2] seen (the reference aborts on one). */
 int tioga_amd_weight_aborts(void);

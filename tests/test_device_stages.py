@@ -121,3 +121,63 @@ def test_device_dedupe_batch_matches_oracle():
             assert np.array_equal(xtag[off:off + n], wi[:n]), f"set {k} xtag"
             assert np.array_equal(rtag[off:off + n], wr[:n]), f"set {k} rtag"
             off += n
+
+
+def obb_sets(rng):
+    """Point sets for the device OBB: sizes around the special branches and
+    the 512-point segments, sums that stay in one binade (offset clusters),
+    sums that keep returning to zero (centred data: every segment stepped
+    term by term), dyadic coordinates (quantisation ties), signed zeros and
+    a plane (extents tied at +-0), large offsets, tiny and subnormal scales,
+    and a set of millions of points (many segments, long prefix scans).
+    (Sets so small that the covariance underflows give all-NaN boxes whose
+    NaN sign bits are the compiler's choice, so none is included.)"""
+    sets = [rng.normal(size=(n, 3)) * [3.0, 1.0, 0.2] for n in (1, 2, 3, 4, 5, 511, 512, 513, 1025)]
+    c = rng.normal(size=(300000, 3))
+    sets.append(np.array([0.3, -0.5, 0.7]) + 0.05 * c / np.linalg.norm(c, axis=1)[:, None])
+    sets.append(rng.uniform(-1, 1, (20000, 3)))
+    g = np.round(rng.uniform(-1, 1, (200000, 3)) * 8) / 8
+    g[rng.uniform(size=g.shape) < 0.05] = -0.0
+    sets.append(g)
+    flat = np.zeros((70000, 3))
+    flat[:, :2] = rng.uniform(-1, 1, (70000, 2))
+    sets.append(flat)
+    sets.append(1e6 + 1e-3 * rng.normal(size=(100000, 3)))
+    sets.append(1e-60 * (0.5 + rng.uniform(size=(50000, 3))))
+    # a run of subnormal and tiny points before O(1) ones: running sums below
+    # the smallest binade the segment step takes
+    t = np.concatenate([1e-310 * rng.uniform(size=(1500, 3)), 1e-300 * rng.uniform(size=(1500, 3)),
+                        rng.normal(size=(3000, 3))])
+    sets.append(t)
+    i = np.arange(40)
+    sets.append(np.stack(np.meshgrid(i, i, i, indexing="ij"), -1)[..., ::-1].reshape(-1, 3) * (2.0 / 39) - 1.0)
+    sets.append(np.array([-0.2, 0.4, 0.1]) + 0.08 * rng.uniform(-1, 1, (2500000, 3)))
+    return [np.ascontiguousarray(s, dtype=np.float64) for s in sets]
+
+
+def test_device_obbs_match_oracle():
+    """The device OBB batch (segmented exact running sums, order-carrying
+    extents) gives the oracle's findOBB bits for every set."""
+    from tioga_amd.tioga import load
+
+    L = load()
+    L.tioga_amd_device_find_obbs.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.tioga_amd_device_find_obbs.restype = C.c_int
+    O = oracle_lib()
+    sets = obb_sets(np.random.default_rng(12))
+    want = []
+    for x in sets:
+        xc, dxc, vec = np.zeros(3), np.zeros(3), np.zeros(9)
+        O.orc_find_obb(ptr(x), ptr(xc), ptr(dxc), ptr(vec), x.shape[0])
+        want.append(np.concatenate([xc, dxc, vec]))
+    for batch in ([len(sets) - 1], [10], list(range(len(sets)))):
+        ns = np.array([sets[k].shape[0] for k in batch], np.int32)
+        x = np.ascontiguousarray(np.concatenate([sets[k] for k in batch]).reshape(-1))
+        out = np.zeros(15 * len(batch))
+        stepped = np.zeros(1, np.int64)
+        print(f"[device obb] sets {ns.tolist()}", flush=True)
+        assert L.tioga_amd_device_find_obbs(len(batch), ptr(ns), ptr(x), ptr(out), ptr(stepped)) == 0
+        print(f"[device obb] segments stepped term by term: {int(stepped[0])}", flush=True)
+        for a, k in enumerate(batch):
+            got = out[15 * a:15 * a + 15]
+            assert np.array_equal(got.view(np.int64), want[k].view(np.int64)), (k, sets[k].shape[0], got, want[k])

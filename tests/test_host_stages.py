@@ -25,7 +25,6 @@ def lib():
     L = C.CDLL(LIB)
     L.tioga_amd_host_median.argtypes = [P, P, C.c_int, C.c_int]
     L.tioga_amd_host_find_obb.argtypes = [P, C.c_int, P]
-    L.tioga_amd_host_find_obb_soa.argtypes = [P, P, P, C.c_int, C.c_int, P]
     L.tioga_amd_host_build_adt.argtypes = [C.c_int, P, P, P, P]
     L.tioga_amd_host_build_adt.restype = C.c_int
     L.tioga_amd_host_unique_nodes.argtypes = [P, P, P, P, C.c_int]
@@ -63,33 +62,6 @@ def test_find_obb_matches_oracle(lib):
         xc, dxc, vec = np.zeros(3), np.zeros(3), np.zeros(9)
         ol.orc_find_obb(ptr(x), ptr(xc), ptr(dxc), ptr(vec), n)
         assert np.array_equal(a, np.concatenate([xc, dxc, vec])), n
-
-
-def test_find_obb_components_on_threads_matches_oracle(lib):
-    """The threaded form over component arrays (large query sets): same bits
-    as the sequential fold, also where the extents tie at +-0 and at chunk
-    boundaries."""
-    rng = np.random.default_rng(8)
-    ol = O.lib()
-    cases = [rng.normal(size=(n, 3)) * [3.0, 1.0, 0.2] for n in (1, 2, 3, 4, 100, 300000)]
-    g = np.round(rng.uniform(-1, 1, (200000, 3)) * 8) / 8  # many equal coordinates
-    g[rng.uniform(size=g.shape) < 0.05] = -0.0
-    cases.append(g)
-    flat = np.zeros((70000, 3))  # a plane: zero extent along one OBB axis
-    flat[:, :2] = rng.uniform(-1, 1, (70000, 2))
-    cases.append(flat)
-    for x in cases:
-        x = np.ascontiguousarray(x)
-        n = x.shape[0]
-        want = np.zeros(15)
-        xc, dxc, vec = np.zeros(3), np.zeros(3), np.zeros(9)
-        ol.orc_find_obb(ptr(x), ptr(xc), ptr(dxc), ptr(vec), n)
-        want = np.concatenate([xc, dxc, vec])
-        comp = [np.ascontiguousarray(x[:, j]) for j in range(3)]
-        for nt in (1, 3, 8):
-            a = np.zeros(15)
-            lib.tioga_amd_host_find_obb_soa(ptr(comp[0]), ptr(comp[1]), ptr(comp[2]), n, nt, ptr(a))
-            assert np.array_equal(a.view(np.int64), want.view(np.int64)), (n, nt)
 
 
 def test_build_adt_matches_oracle(lib):

@@ -525,6 +525,7 @@ void tioga_amd_phase_times(double *out)
     out[9] = t.prep_filter;
     out[10] = t.prep_adt;
     out[11] = t.prep_dedupe;
+    out[12] = (double)t.obb_stepped;
   });
 }
 
@@ -538,17 +539,6 @@ void tioga_amd_host_find_obb(const double *x, int n, double *out)
 {
   tg::OBB o;
   tg::find_obb(x, nullptr, n, o);
-  memcpy(out, o.xc, 24);
-  memcpy(out + 3, o.dxc, 24);
-  memcpy(out + 6, o.vec, 72);
-}
-
-void tioga_amd_host_find_obb_soa(const double *x, const double *y, const double *z, int n, int nthreads,
-                                 double *out)
-{
-  tg::OBB o;
-  const double *const c[3] = {x, y, z};
-  tg::find_obb_soa(c, n, o, nthreads);
   memcpy(out, o.xc, 24);
   memcpy(out + 3, o.dxc, 24);
   memcpy(out + 6, o.vec, 72);
@@ -640,6 +630,38 @@ int tioga_amd_device_unique_nodes(int nset, const int *n, const double *x, const
     }
     TG_HIP(hipStreamSynchronize(s));
     TG_HIP(hipStreamDestroy(s));
+    rc = 0;
+  });
+  return rc;
+}
+
+int tioga_amd_device_find_obbs(int nset, const int *n, const double *x, double *out, long long *stepped)
+{
+  int rc = -1;
+  guarded("tioga_amd_device_find_obbs", [&] {
+    if (nset <= 0) throw std::runtime_error("no point sets");
+    hipStream_t s;
+    TG_HIP(hipStreamCreate(&s));
+    tg::ObbScratch w;
+    std::vector<tg::ObbSet> sets(nset);
+    std::vector<tg::DBuf<double>> dx(nset);
+    std::vector<tg::OBB> o(nset);
+    size_t off = 0;
+    for (int k = 0; k < nset; k++) {
+      if (n[k] <= 0) throw std::runtime_error("empty point set");
+      dx[k].reserve(3 * (size_t)n[k]);
+      TG_HIP(hipMemcpyAsync(dx[k].p, x + 3 * off, sizeof(double) * 3 * n[k], hipMemcpyHostToDevice, s));
+      sets[k] = tg::ObbSet{n[k], dx[k].p};
+      off += n[k];
+    }
+    tg::find_obbs_device(sets.data(), nset, o.data(), w, s, stepped);
+    TG_HIP(hipStreamSynchronize(s));
+    TG_HIP(hipStreamDestroy(s));
+    for (int k = 0; k < nset; k++) {
+      memcpy(out + 15 * k, o[k].xc, 24);
+      memcpy(out + 15 * k + 3, o[k].dxc, 24);
+      memcpy(out + 15 * k + 6, o[k].vec, 72);
+    }
     rc = 0;
   });
   return rc;

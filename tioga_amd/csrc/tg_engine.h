@@ -229,6 +229,32 @@ void unique_nodes_batch(DedupeSet *sets, int nset, DedupeScratch &w, hipStream_t
 void unique_nodes_device(int n, const double *x, const int *tag, double *rtag, int *itag, DedupeScratch &w,
                          hipStream_t s);
 
+// One point set of an OBB batch: n points, AoS xyz, in device memory.
+struct ObbSet {
+  int n;
+  const double *x;
+};
+
+// The device table of an OBB batch (tg_obb.hip): segments [seg0, seg1) of
+// the chain sums and workgroups [blk0, blk1) of the extents pass.
+struct ObbSetDev {
+  const double *x;
+  int n, seg0, seg1, blk0, blk1;
+};
+
+// Reusable device scratch of the OBB batch.
+struct ObbScratch {
+  DBuf<ObbSetDev> sets;
+  DBuf<double> apx, qP, qN, xc, aa, vec, lohi, part;
+  DBuf<int> ep, qbad, nstep;
+};
+
+// findOBB (utils.c:25-161) for every set of the batch, with the bits of the
+// sequential host computation (find_obb); *stepped (if given) receives the
+// number of segments whose terms had to be added one by one.
+void find_obbs_device(const ObbSet *sets, int nset, OBB *out, ObbScratch &w, hipStream_t s,
+                      long long *stepped = nullptr);
+
 // Reusable device scratch of the donor search: per-query candidate lists
 // and the queue of queries with more candidates than a list holds.
 struct SearchScratch {
@@ -258,6 +284,7 @@ struct PhaseTimes {
   long long nsearch = 0;
   // inside search_prep (host wall ms, summed over blocks)
   double prep_obb = 0, prep_filter = 0, prep_adt = 0, prep_dedupe = 0;
+  long long obb_stepped = 0;  // OBB sum segments stepped term by term
 };
 
 class Engine {
@@ -273,10 +300,6 @@ class Engine {
   // together (one ADT build, one dedupe, ... for every block); false runs
   // them block by block.  Results are identical either way.
   bool batched = true;
-  // page-locked staging for every registered node at preprocess (false:
-  // grown on demand by the search)
-  bool stage_all = true;
-
   int block_count() const { return (int)blocks_.size(); }
   int block_nnodes(int btag) const
   {
@@ -373,9 +396,7 @@ class Engine {
   void search(Block &b);
   void search_batched();
   void dedupe_gid(Block &b);
-  double *pinned_ = nullptr;  // page-locked staging of the query points
-  size_t pinned_cap_ = 0;
-  DBuf<double> soa_;          // a large query set as component arrays
+  ObbScratch obbw_;           // query-set OBB batches
   DBuf<double> wallds_;       // hole-map cell sizes, per mesh tag
   DBuf<unsigned char *> wallsams_;  // hole maps, per mesh tag
   void exchange_donors();
@@ -383,7 +404,7 @@ class Engine {
   void launch_search(Block &b);
   void launch_searches();
 
-  void build_search_structure(Block &b, const double *xq_host, int nq, int &nelem, int &depth, DBuf<int> &elist,
+  void build_search_structure(Block &b, const OBB &obq, int &nelem, int &depth, DBuf<int> &elist,
                               DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, DBuf<AdtNode> &nodes,
                               double ext[6]);
   void launch_query_set(QuerySet &q);
@@ -416,6 +437,7 @@ class Engine {
   AdtScratch adtw_;
   DedupeScratch dedw_;
   double prep_acc_[4] = {0, 0, 0, 0};
+  long long obb_stepped_ = 0;
   std::vector<DBuf<double> *> qdev_;
   DBuf<int> qlist_;  // query positions of every pair (exchange_search_data)
   // AMR Cartesian background (tg_amr.hip)

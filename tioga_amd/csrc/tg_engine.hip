@@ -10,9 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
-#include <thread>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -896,14 +894,6 @@ __global__ void k_adt_nodes(int nelem, TypeInfo ti, const int *__restrict__ adtI
     N.rbox[j] = N.right > -1 ? adtReal[6 * (size_t)N.right + j] + pad : 0.0;
   }
   nodes[n] = N;
-}
-
-// query points as three component arrays: out[j*n + i] = xs[3i + j]
-__global__ void k_aos_to_soa(int n, const double *__restrict__ xs, double *__restrict__ out)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  for (int j = 0; j < 3; j++) out[(size_t)j * n + i] = xs[3 * (size_t)i + j];
 }
 
 // 30-bit Morton code of each query in the frame of the search structure's
@@ -2005,7 +1995,6 @@ Engine::~Engine()
   for (Block *b : blocks_) delete b;
   for (auto *q : qdev_) delete q;
   amr_release();
-  if (pinned_) (void)hipHostFree(pinned_);
   if (s_) (void)hipStreamDestroy(s_);
 }
 
@@ -2229,17 +2218,6 @@ void Engine::tag_boundary(Block &b)
 // tioga::profile (tioga.C:95-110)
 void Engine::profile()
 {
-  // page-locked staging for the query points of the batched search: every
-  // node queried once is the usual upper end
-  size_t allpts = 0;
-  for (Block *bp : blocks_) allpts += 3 * (size_t)bp->nnodes;
-  if (batched && stage_all && allpts > pinned_cap_) {
-    if (pinned_) TG_HIP(hipHostFree(pinned_));
-    pinned_ = nullptr;
-    pinned_cap_ = 0;
-    TG_HIP(hipHostMalloc((void **)&pinned_, sizeof(double) * allpts, hipHostMallocDefault));
-    pinned_cap_ = allpts;
-  }
   for (Block *bp : blocks_) {
     Block &b = *bp;
     if (b.share == 2) {  // another engine's geometry: only this engine's iblank
@@ -2834,22 +2812,23 @@ void Engine::search(Block &b)
   const int Q = b.nsearch;
   b.nelem = 0;
   if (Q == 0) return;
-  // query-set OBB: a running FP64 sum whose rounding order fixes its bits,
-  // so the host takes it; everything else stays on the device
-  std::vector<double> xs(3 * (size_t)Q);
-  TG_HIP(hipMemcpyAsync(xs.data(), b.xsearch.p, sizeof(double) * 3 * Q, hipMemcpyDeviceToHost, s_));
-  sync();
-  build_search_structure(b, xs.data(), Q, b.nelem, b.adt_depth, b.elist, b.ebox, b.adtInt, b.adtReal, b.nodes,
-                         b.adtExt);
+  auto t0 = std::chrono::steady_clock::now();
+  OBB obq;
+  const ObbSet os{Q, b.xsearch.p};
+  long long st = 0;
+  find_obbs_device(&os, 1, &obq, obbw_, s_, &st);
+  obb_stepped_ += st;
+  prep_acc_[0] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  build_search_structure(b, obq, b.nelem, b.adt_depth, b.elist, b.ebox, b.adtInt, b.adtReal, b.nodes, b.adtExt);
   bool gid_mode = true;
   for (Block *o : blocks_) gid_mode = gid_mode && (o->unode_gid != nullptr);
   if (gid_mode) {
     dedupe_gid(b);
   } else {
-    auto t0 = std::chrono::steady_clock::now();
+    auto td = std::chrono::steady_clock::now();
     unique_nodes_device(Q, b.xsearch.p, b.tagsearch.p, b.res_search.p, b.xtag.p, dedw_, s_);
     sync();
-    prep_acc_[3] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    prep_acc_[3] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - td).count();
   }
 }
 
@@ -2863,70 +2842,24 @@ void Engine::search_batched()
   auto ms = [](auto a, auto c) { return std::chrono::duration<double, std::milli>(c - a).count(); };
   const int nbk = (int)blocks_.size();
   std::vector<int> act;
-  size_t tot = 0;
   for (int ib = 0; ib < nbk; ib++) {
     blocks_[ib]->nelem = 0;
-    if (blocks_[ib]->nsearch > 0) {
-      act.push_back(ib);
-      tot += 3 * (size_t)blocks_[ib]->nsearch;
-    }
+    if (blocks_[ib]->nsearch > 0) act.push_back(ib);
   }
   if (act.empty()) return;
   auto t0 = now();
-  // query-set OBBs: running FP64 sums whose order fixes their bits, one
-  // host thread per block
-  if (tot > pinned_cap_) {
-    if (pinned_) TG_HIP(hipHostFree(pinned_));
-    pinned_ = nullptr;
-    pinned_cap_ = 0;
-    const size_t want = tot + tot / 2;
-    TG_HIP(hipHostMalloc((void **)&pinned_, sizeof(double) * want, hipHostMallocDefault));
-    pinned_cap_ = want;
-  }
-  // large sets come over as three component arrays (transposed on the
-  // device), so their OBB sums can run one chain per thread
-  constexpr int OBB_SOA_MIN = 1 << 20;
-  std::vector<size_t> off(nbk, 0);
-  std::vector<char> soa(nbk, 0);
-  {
-    size_t o = 0;
-    for (int ib : act) {
-      Block &b = *blocks_[ib];
-      const int n = b.nsearch;
-      off[ib] = o;
-      if (n >= OBB_SOA_MIN) {
-        soa[ib] = 1;
-        soa_.reserve(3 * (size_t)n);
-        k_aos_to_soa<<<nb(n), BS, 0, s_>>>(n, b.xsearch.p, soa_.p);
-        TG_HIP(hipMemcpyAsync(pinned_ + o, soa_.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s_));
-      } else {
-        TG_HIP(hipMemcpyAsync(pinned_ + o, b.xsearch.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s_));
-      }
-      o += 3 * (size_t)n;
-    }
-  }
-  sync();
-  // the OBBs on host threads while the device deduplicates the query sets
+  // query-set OBBs, all sets in one device batch with the bits of the
+  // sequential running sums
   std::vector<OBB> obq(nbk);
-  std::vector<int> order = act;
-  std::sort(order.begin(), order.end(), [&](int a, int b) { return blocks_[a]->nsearch > blocks_[b]->nsearch; });
-  std::atomic<int> next{0};
-  const int hw = std::max(1, (int)std::thread::hardware_concurrency());
-  auto work = [&] {
-    for (int k; (k = next.fetch_add(1)) < (int)order.size();) {
-      const int ib = order[k];
-      const int n = blocks_[ib]->nsearch;
-      if (soa[ib]) {
-        const double *const c[3] = {pinned_ + off[ib], pinned_ + off[ib] + n, pinned_ + off[ib] + 2 * (size_t)n};
-        find_obb_soa(c, n, obq[ib], hw);
-      } else {
-        find_obb(pinned_ + off[ib], nullptr, n, obq[ib]);
-      }
-    }
-  };
-  const int nt = std::max(1, std::min((int)order.size(), hw));
-  std::vector<std::thread> th;
-  for (int t = 0; t < nt; t++) th.emplace_back(work);
+  {
+    std::vector<ObbSet> os;
+    std::vector<OBB> ob(act.size());
+    for (int ib : act) os.push_back(ObbSet{blocks_[ib]->nsearch, blocks_[ib]->xsearch.p});
+    long long st = 0;
+    find_obbs_device(os.data(), (int)os.size(), ob.data(), obbw_, s_, &st);
+    obb_stepped_ += st;
+    for (size_t a = 0; a < act.size(); a++) obq[act[a]] = ob[a];
+  }
   auto td0 = now();
   bool gid_mode = true;
   for (Block *o : blocks_) gid_mode = gid_mode && (o->unode_gid != nullptr);
@@ -2942,8 +2875,7 @@ void Engine::search_batched()
     sync();
   }
   auto td1 = now();
-  for (auto &t : th) t.join();
-  auto t1 = now();
+  auto t1 = td1;
   // cells overlapping each query OBB, as a LIFO list, and their boxes: the
   // flags of all blocks in one array (block a's cells from coff[a], each
   // block's reversed), one compaction, and each block's range of it
@@ -3003,8 +2935,7 @@ void Engine::search_batched()
   }
   sync();
   auto t3 = now();
-  // obb: staging and the host threads beyond the overlapped dedupe
-  prep_acc_[0] += ms(t0, td0) + ms(td1, t1);
+  prep_acc_[0] += ms(t0, td0);
   prep_acc_[1] += ms(t1, t2);
   prep_acc_[2] += ms(t2, t3);
   prep_acc_[3] += ms(td0, td1);
@@ -3040,15 +2971,12 @@ void Engine::dedupe_gid(Block &b)
 
 // search.C:107-233: query-set OBB -> cells whose OBB-frame box overlaps it,
 // as a LIFO list (descending cell index) -> element boxes -> ADT
-void Engine::build_search_structure(Block &b, const double *xq, int nq, int &nelem, int &depth,
-                                    DBuf<int> &elist, DBuf<double> &ebox, DBuf<int> &adtInt,
-                                    DBuf<double> &adtReal, DBuf<AdtNode> &nodes, double ext[6])
+void Engine::build_search_structure(Block &b, const OBB &obq, int &nelem, int &depth, DBuf<int> &elist,
+                                    DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal,
+                                    DBuf<AdtNode> &nodes, double ext[6])
 {
   auto now = [] { return std::chrono::steady_clock::now(); };
   auto ms = [](auto a, auto c) { return std::chrono::duration<double, std::milli>(c - a).count(); };
-  auto t0 = now();
-  OBB obq;
-  find_obb(xq, nullptr, nq, obq);
   auto t1 = now();
   const int N = b.ncells;
   iscratch_.reserve(N);
@@ -3069,7 +2997,6 @@ void Engine::build_search_structure(Block &b, const double *xq, int nq, int &nel
   if (ne) k_adt_nodes<<<nb(ne), BS, 0, s_>>>(ne, b.ti, adtInt.p, adtReal.p, ebox.p, elist.p, nodes.p);
   sync();
   auto t3 = now();
-  prep_acc_[0] += ms(t0, t1);
   prep_acc_[1] += ms(t1, t2);
   prep_acc_[2] += ms(t2, t3);
 }
@@ -3087,8 +3014,10 @@ int Engine::query_set_create(int btag, const double *xq, long nq)
   q->xq.reserve(3 * (size_t)nq);
   q->donor.reserve(nq);
   TG_HIP(hipMemcpyAsync(q->xq.p, xq, sizeof(double) * 3 * nq, hipMemcpyHostToDevice, s_));
-  build_search_structure(*b, xq, (int)nq, q->nelem, q->adt_depth, q->elist, q->ebox, q->adtInt, q->adtReal,
-                         q->nodes, q->ext);
+  OBB obq;
+  find_obb(xq, nullptr, (int)nq, obq);
+  build_search_structure(*b, obq, q->nelem, q->adt_depth, q->elist, q->ebox, q->adtInt, q->adtReal, q->nodes,
+                         q->ext);
   qsets_.push_back(q);
   return (int)qsets_.size() - 1;
 }
@@ -3493,6 +3422,7 @@ void Engine::perform_connectivity()
   exchange_search_data();
   auto t3 = now();
   for (double &a : prep_acc_) a = 0;
+  obb_stepped_ = 0;
   for (Block *b : blocks_) {
     b->ninterp = 0;
     b->interp_host_valid = false;
@@ -3544,6 +3474,7 @@ void Engine::perform_connectivity()
   t.prep_filter = prep_acc_[1];
   t.prep_adt = prep_acc_[2];
   t.prep_dedupe = prep_acc_[3];
+  t.obb_stepped = obb_stepped_;
   last_times = t;
   connected_ = true;
 }

@@ -6,7 +6,6 @@
 #include <string.h>
 
 #include <utility>
-#include <thread>
 #include <vector>
 
 namespace tg {
@@ -83,6 +82,24 @@ void jacobi3(double *a)
 
 }  // namespace
 
+void obb_axes(double *aa, OBB &o)
+{
+  jacobi3(aa);
+  for (int i = 0, m = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) o.vec[i][j] = aa[m++];
+}
+
+void obb_finish(OBB &o, const double xmin[3], const double xmax[3])
+{
+  double xd[3];
+  for (int j = 0; j < 3; j++) {
+    o.dxc[j] = (xmax[j] - xmin[j]) * 0.5 * 1.01;
+    xd[j] = (xmax[j] + xmin[j]) * 0.5;
+  }
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 3; k++) o.xc[j] += (xd[k] * o.vec[k][j]);
+}
+
 void find_obb(const double *x, const int *idx, int n, OBB &o)
 {
   auto P = [x, idx](int i) { return x + 3 * (size_t)(idx ? idx[i] : i); };
@@ -149,10 +166,8 @@ void find_obb(const double *x, const int *idx, int n, OBB &o)
   aa[1] = aa[3];
   aa[2] = aa[6];
   aa[5] = aa[7];
-  jacobi3(aa);
-  for (int i = 0, m = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) o.vec[i][j] = aa[m++];
-  double xmin[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, xmax[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE}, xd[3];
+  obb_axes(aa, o);
+  double xmin[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, xmax[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
   {
     const double xc[3] = {o.xc[0], o.xc[1], o.xc[2]};
     double vec[3][3];
@@ -176,104 +191,7 @@ void find_obb(const double *x, const int *idx, int n, OBB &o)
     xmax[1] = hi1;
     xmax[2] = hi2;
   }
-  for (int j = 0; j < 3; j++) {
-    o.dxc[j] = (xmax[j] - xmin[j]) * 0.5 * 1.01;
-    xd[j] = (xmax[j] + xmin[j]) * 0.5;
-  }
-  for (int j = 0; j < 3; j++)
-    for (int k = 0; k < 3; k++) o.xc[j] += (xd[k] * o.vec[k][j]);
-}
-
-// find_obb over coordinates held as three component arrays, for large sets:
-// every running sum is still one chain in index order (the centroid's three,
-// the covariance's six), but the chains run on threads of their own and each
-// reads only the components it needs; the extents pass, a min/max fold, runs
-// on nthreads chunks whose results are folded in chunk order (a fold keeps
-// the last element of its extreme class, so chunk folds combined in order
-// give the sequential fold's value and sign).
-void find_obb_soa(const double *const c[3], int n, OBB &o, int nthreads)
-{
-  if (n < 4) {
-    std::vector<double> x(3 * (size_t)std::max(n, 1));
-    for (int i = 0; i < n; i++)
-      for (int j = 0; j < 3; j++) x[3 * (size_t)i + j] = c[j][i];
-    find_obb(x.data(), nullptr, n, o);
-    return;
-  }
-  {
-    double s[3];
-    std::vector<std::thread> th;
-    for (int j = 0; j < 3; j++)
-      th.emplace_back([&, j] {
-        const double *v = c[j];
-        double a = 0;
-        for (int i = 0; i < n; i++) a += v[i];
-        s[j] = a;
-      });
-    for (auto &t : th) t.join();
-    for (int j = 0; j < 3; j++) o.xc[j] = s[j] / n;
-  }
-  double aa[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  {
-    // aa[0] x x, aa[4] y y, aa[8] z z, aa[3] x y, aa[6] x z, aa[7] y z
-    const int slot[6] = {0, 4, 8, 3, 6, 7}, pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
-    std::vector<std::thread> th;
-    for (int q = 0; q < 6; q++)
-      th.emplace_back([&, q] {
-        const double *u = c[pa[q]], *v = c[pb[q]];
-        const double cu = o.xc[pa[q]], cv = o.xc[pb[q]];
-        double a = 0;
-        for (int i = 0; i < n; i++) a += ((u[i] - cu) * (v[i] - cv));
-        aa[slot[q]] = a;
-      });
-    for (auto &t : th) t.join();
-  }
-  aa[1] = aa[3];
-  aa[2] = aa[6];
-  aa[5] = aa[7];
-  jacobi3(aa);
-  for (int i = 0, m = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) o.vec[i][j] = aa[m++];
-  const int nt = std::max(1, std::min(nthreads, n / 65536 + 1));
-  std::vector<double> lo(3 * (size_t)nt, BIGVALUE), hi(3 * (size_t)nt, -BIGVALUE);
-  {
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; t++)
-      th.emplace_back([&, t] {
-        const int i0 = (int)((long long)n * t / nt), i1 = (int)((long long)n * (t + 1) / nt);
-        double l0 = BIGVALUE, l1 = BIGVALUE, l2 = BIGVALUE, h0 = -BIGVALUE, h1 = -BIGVALUE, h2 = -BIGVALUE;
-        for (int i = i0; i < i1; i++) {
-          const double p[3] = {c[0][i], c[1][i], c[2][i]};
-          double e[3];
-          to_frame(p, o.xc, o.vec, e);
-          h0 = dmax(h0, e[0]);
-          l0 = dmin(l0, e[0]);
-          h1 = dmax(h1, e[1]);
-          l1 = dmin(l1, e[1]);
-          h2 = dmax(h2, e[2]);
-          l2 = dmin(l2, e[2]);
-        }
-        lo[3 * t] = l0;
-        lo[3 * t + 1] = l1;
-        lo[3 * t + 2] = l2;
-        hi[3 * t] = h0;
-        hi[3 * t + 1] = h1;
-        hi[3 * t + 2] = h2;
-      });
-    for (auto &t : th) t.join();
-  }
-  double xmin[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, xmax[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE}, xd[3];
-  for (int t = 0; t < nt; t++)
-    for (int j = 0; j < 3; j++) {
-      xmax[j] = dmax(xmax[j], hi[3 * t + j]);
-      xmin[j] = dmin(xmin[j], lo[3 * t + j]);
-    }
-  for (int j = 0; j < 3; j++) {
-    o.dxc[j] = (xmax[j] - xmin[j]) * 0.5 * 1.01;
-    xd[j] = (xmax[j] + xmin[j]) * 0.5;
-  }
-  for (int j = 0; j < 3; j++)
-    for (int k = 0; k < 3; k++) o.xc[j] += (xd[k] * o.vec[k][j]);
+  obb_finish(o, xmin, xmax);
 }
 
 int obb_intersect(const OBB &A, const OBB &B)

@@ -16,8 +16,11 @@ namespace tg {
 // kaiser.f:29-180).  x is AoS xyz; when idx is non-null the points are
 // x[3*idx[i]..], in that order.
 void find_obb(const double *x, const int *idx, int n, OBB &o);
-// the same OBB from component arrays c[0..2][n], on threads
-void find_obb_soa(const double *const c[3], int n, OBB &o, int nthreads);
+// find_obb's last two steps: the eigenvectors of the covariance matrix aa[9]
+// (column-major, overwritten) into o.vec (utils.c:106-118), and from the
+// frame extents the half widths and the shifted centre (utils.c:148-161)
+void obb_axes(double *aa, OBB &o);
+void obb_finish(OBB &o, const double xmin[3], const double xmax[3]);
 
 // 15-axis separating-axis test with eps 1e-12 (utils.c:297-384), including
 // the reference's |vA[i][j]|*D[j] projection of the centre offset.

@@ -101,7 +101,7 @@ def _ip(v: int):
 _I32 = {"isearch", "tagsearch", "donorId", "xtag", "elementList", "adtIntegers", "iblank_cell"}
 _AMR_F64 = {"rxyzCart", "rec_w"}
 PHASES = ("holemap", "boxes", "searchdata", "search_prep", "search_kernel", "donors", "total", "nsearch",
-          "prep_obb", "prep_filter", "prep_adt", "prep_dedupe")
+          "prep_obb", "prep_filter", "prep_adt", "prep_dedupe", "obb_stepped")
 
 
 class Tioga:
@@ -368,7 +368,7 @@ class Tioga:
         return ms, n.value, self.L.tioga_amd_last_search_kernel_ms()
 
     def phase_times(self) -> Dict[str, float]:
-        out = np.zeros(12)
+        out = np.zeros(len(PHASES))
         self.L.tioga_amd_phase_times(_p(out))
         self._check()
         return dict(zip(PHASES, out.tolist()))
