@@ -420,6 +420,7 @@ class Engine {
   void sync();
 
   hipStream_t s_ = nullptr;
+  hipStream_t s2_ = nullptr;  // work overlapped with s_ (the query-set OBB batch)
   std::vector<Block *> blocks_;
   std::map<int, int> tag_to_blk_;
   std::vector<HoleMap> hm_;
