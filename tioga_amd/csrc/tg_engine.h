@@ -415,6 +415,7 @@ class Engine {
   size_t event_slot_ = 0;
   double last_search_kernel_ms_ = 0;
   DBuf<unsigned> mk0_, mk1_;
+  DBuf<unsigned char> mset_;  // the per-set Morton table of a batched search
   DBuf<int> mv0_, mv1_;
   std::vector<QuerySet *> qsets_;
   void sync();

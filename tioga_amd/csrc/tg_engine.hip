@@ -928,6 +928,41 @@ __global__ void k_morton(int nq, const double *__restrict__ xs, double x0, doubl
   idx[i] = i;
 }
 
+// The query sets of a batched search, for their Morton keys in one launch:
+// set k's queries are [off, off + nq) of the concatenation.
+struct MortonSet {
+  const double *xs;
+  double lo[3], hi[3];
+  int off, nq;
+};
+
+// k_morton for every set at once, keyed (set << mb) | (Morton >> (30 - mb))
+// so that one sort orders each set's queries inside the set's own range
+// (idx = the query's index within its set)
+__global__ void k_morton_sets(int ns, int tot, const MortonSet *__restrict__ M, int mb, unsigned *__restrict__ key,
+                              int *__restrict__ idx)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= tot) return;
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (M[mid].off <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const MortonSet &m = M[lo];
+  const int i = g - m.off;
+  unsigned c[3];
+  for (int d = 0; d < 3; d++) {
+    double u = (m.xs[3 * (size_t)i + d] - m.lo[d]) / (m.hi[d] - m.lo[d]);
+    u = u < 0 ? 0 : (u > 0.999999 ? 0.999999 : u);
+    c[d] = (unsigned)(u * 1024.0);
+  }
+  const unsigned m30 = (spread10(c[0]) << 2) | (spread10(c[1]) << 1) | spread10(c[2]);
+  key[g] = ((unsigned)lo << mb) | (m30 >> (30 - mb));
+  idx[g] = i;
+}
+
 struct SearchView {
   int nq;
   const int *perm;  // processing order (query index of thread t), or null
@@ -1391,6 +1426,26 @@ __global__ void k_copy_dup(int nq, const int *__restrict__ xtag, int *__restrict
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nq) return;
+  int r = i;
+  while (xtag[r] != r) r = xtag[r];
+  if (r != i) donorId[i] = donorId[r];
+}
+
+// k_copy_dup for every set of a batched search (query g of the
+// concatenation belongs to the last set k with qb[k] <= g)
+__global__ void k_copy_dup_sets(const SearchView *__restrict__ sv, const int *__restrict__ qb, int ns, int nqt)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nqt) return;
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (qb[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const int i = g - qb[lo];
+  const int *xtag = sv[lo].xtag;
+  int *donorId = sv[lo].donorId;
   int r = i;
   while (xtag[r] != r) r = xtag[r];
   if (r != i) donorId[i] = donorId[r];
@@ -3167,14 +3222,22 @@ void Engine::launch_searches()
   mk1_.reserve(tot);
   mv0_.reserve(tot);
   mv1_.reserve(tot);
+  // Morton order inside each set, all sets in one sort: the set index in the
+  // high key bits, the Morton code truncated to the rest (the order only
+  // affects speed, each query's walk is independent)
+  const int ns = (int)act.size();
+  int rb = 0;
+  while ((1 << rb) < ns) rb++;
+  const int mb = std::min(30, 32 - rb);
+  if (mb < 12) throw std::runtime_error("tioga_amd: too many query sets in one search launch");
+  std::vector<MortonSet> ms(ns);
   std::vector<SearchView> sv;
   size_t off = 0;
-  for (Block *bp : act) {
-    Block &b = *bp;
+  for (int k = 0; k < ns; k++) {
+    Block &b = *act[k];
     const int nq = b.nsearch;
     const double *e = b.adtExt;
-    k_morton<<<nb(nq), BS, 0, s_>>>(nq, b.xsearch.p, e[0], e[1], e[2], e[3], e[4], e[5], mk0_.p + off, mv0_.p + off);
-    sort_pairs(mk0_.p + off, mk1_.p + off, mv0_.p + off, mv1_.p + off, nq, 30, s_);
+    ms[k] = MortonSet{b.xsearch.p, {e[0], e[2], e[4]}, {e[1], e[3], e[5]}, (int)off, nq};
     SearchView S;
     S.nq = nq;
     S.perm = mv1_.p + off;
@@ -3189,8 +3252,12 @@ void Engine::launch_searches()
     sv.push_back(S);
     off += nq;
   }
-  run_search(sv.data(), (int)sv.size(), sw_, s_, true);
-  for (Block *b : act) k_copy_dup<<<nb(b->nsearch), BS, 0, s_>>>(b->nsearch, b->xtag.p, b->donorId.p);
+  mset_.reserve(sizeof(MortonSet) * ns);
+  TG_HIP(hipMemcpyAsync(mset_.p, ms.data(), sizeof(MortonSet) * ns, hipMemcpyHostToDevice, s_));
+  k_morton_sets<<<nb(tot), BS, 0, s_>>>(ns, (int)tot, (const MortonSet *)mset_.p, mb, mk0_.p, mv0_.p);
+  sort_pairs(mk0_.p, mk1_.p, mv0_.p, mv1_.p, (int)tot, mb + rb, s_);
+  run_search(sv.data(), ns, sw_, s_, true);
+  k_copy_dup_sets<<<nb(tot), BS, 0, s_>>>((const SearchView *)sw_.sv.p, sw_.qb.p, ns, (int)tot);
 }
 
 // tioga::exchangeDonors (exchangeDonors.C:25-301), single rank
