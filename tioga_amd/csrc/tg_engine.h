@@ -300,6 +300,9 @@ class Engine {
   // together (one ADT build, one dedupe, ... for every block); false runs
   // them block by block.  Results are identical either way.
   bool batched = true;
+  // batched search: the query-set OBB batch on a second stream, overlapped
+  // with the dedupe (off for the serving slots, which overlap requests)
+  bool overlap_obb = true;
   int block_count() const { return (int)blocks_.size(); }
   int block_nnodes(int btag) const
   {

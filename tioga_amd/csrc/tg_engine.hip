@@ -2043,7 +2043,6 @@ struct Timer {
 Engine::Engine()
 {
   TG_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
-  TG_HIP(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking));
   err_.reserve(1);
 }
 
@@ -2917,15 +2916,23 @@ void Engine::search_batched()
   long long st = 0;
   double obb_ms = 0;
   std::exception_ptr obb_err;
-  std::thread obb_thread([&] {
+  auto obb_batch = [&](hipStream_t q) {
     try {
       auto a = now();
-      find_obbs_device(os.data(), (int)os.size(), ob.data(), obbw_, s2_, &st);
+      find_obbs_device(os.data(), (int)os.size(), ob.data(), obbw_, q, &st);
       obb_ms = ms(a, now());
     } catch (...) {
       obb_err = std::current_exception();
     }
-  });
+  };
+  std::thread obb_thread;
+  if (overlap_obb) {
+    // created on first use, so engines that never overlap hold one stream
+    if (!s2_) TG_HIP(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking));
+    obb_thread = std::thread(obb_batch, s2_);
+  } else {
+    obb_batch(s_);
+  }
   auto td0 = now();
   bool gid_mode = true;
   for (Block *o : blocks_) gid_mode = gid_mode && (o->unode_gid != nullptr);
@@ -2942,11 +2949,11 @@ void Engine::search_batched()
       sync();
     }
   } catch (...) {
-    obb_thread.join();
+    if (obb_thread.joinable()) obb_thread.join();
     throw;
   }
   auto td1 = now();
-  obb_thread.join();
+  if (obb_thread.joinable()) obb_thread.join();
   if (obb_err) std::rethrow_exception(obb_err);
   obb_stepped_ += st;
   for (size_t a = 0; a < act.size(); a++) obq[act[a]] = ob[a];
