@@ -300,9 +300,10 @@ class Engine {
   // together (one ADT build, one dedupe, ... for every block); false runs
   // them block by block.  Results are identical either way.
   bool batched = true;
-  // batched search: the query-set OBB batch on a second stream, overlapped
-  // with the dedupe (off for the serving slots, which overlap requests)
-  bool overlap_obb = true;
+  // batched search: the coincident-point dedupe on a second stream,
+  // overlapped with the OBBs, cell filter and ADT build (off for the serving
+  // slots, which overlap whole requests)
+  bool overlap_dedupe = true;
   int block_count() const { return (int)blocks_.size(); }
   int block_nnodes(int btag) const
   {
@@ -398,6 +399,7 @@ class Engine {
   void exchange_search_data();
   void search(Block &b);
   void search_batched();
+  void prepare_searches(const std::vector<int> &act);
   void dedupe_gid(Block &b);
   ObbScratch obbw_;           // query-set OBB batches
   DBuf<double> wallds_;       // hole-map cell sizes, per mesh tag
@@ -424,7 +426,7 @@ class Engine {
   void sync();
 
   hipStream_t s_ = nullptr;
-  hipStream_t s2_ = nullptr;  // work overlapped with s_ (the query-set OBB batch)
+  hipStream_t s2_ = nullptr;  // work overlapped with s_ (the batched dedupe)
   std::vector<Block *> blocks_;
   std::map<int, int> tag_to_blk_;
   std::vector<HoleMap> hm_;

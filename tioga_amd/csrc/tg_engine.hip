@@ -2905,58 +2905,73 @@ void Engine::search_batched()
     if (blocks_[ib]->nsearch > 0) act.push_back(ib);
   }
   if (act.empty()) return;
-  // query-set OBBs, all sets in one device batch with the bits of the
-  // sequential running sums, on a stream of their own driven by a helper
-  // thread while this thread runs the dedupe (both only read the query
-  // points; the cell filter below needs both)
-  std::vector<OBB> obq(nbk);
-  std::vector<ObbSet> os;
-  std::vector<OBB> ob(act.size());
-  for (int ib : act) os.push_back(ObbSet{blocks_[ib]->nsearch, blocks_[ib]->xsearch.p});
-  long long st = 0;
-  double obb_ms = 0;
-  std::exception_ptr obb_err;
-  auto obb_batch = [&](hipStream_t q) {
-    try {
-      auto a = now();
-      find_obbs_device(os.data(), (int)os.size(), ob.data(), obbw_, q, &st);
-      obb_ms = ms(a, now());
-    } catch (...) {
-      obb_err = std::current_exception();
-    }
-  };
-  std::thread obb_thread;
-  if (overlap_obb) {
-    // created on first use, so engines that never overlap hold one stream
-    if (!s2_) TG_HIP(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking));
-    obb_thread = std::thread(obb_batch, s2_);
-  } else {
-    obb_batch(s_);
-  }
-  auto td0 = now();
+  // the coincident-point dedupe of all query sets: on a stream of its own,
+  // driven by a helper thread, while this thread runs the OBB batch, the
+  // cell filter and the ADT build (the dedupe reads the query points and
+  // writes xtag and the raised resolutions, which only the searches read;
+  // those three stages never touch them)
   bool gid_mode = true;
   for (Block *o : blocks_) gid_mode = gid_mode && (o->unode_gid != nullptr);
-  try {
-    if (gid_mode) {
-      for (int ib : act) dedupe_gid(*blocks_[ib]);
-    } else {
-      std::vector<DedupeSet> sets;
-      for (int ib : act) {
-        Block &b = *blocks_[ib];
-        sets.push_back(DedupeSet{b.nsearch, b.xsearch.p, b.tagsearch.p, b.res_search.p, b.xtag.p});
-      }
-      unique_nodes_batch(sets.data(), (int)sets.size(), dedw_, s_);
-      sync();
+  std::vector<DedupeSet> dsets;
+  for (int ib : act) {
+    Block &b = *blocks_[ib];
+    dsets.push_back(DedupeSet{b.nsearch, b.xsearch.p, b.tagsearch.p, b.res_search.p, b.xtag.p});
+  }
+  double ded_ms = 0;
+  std::exception_ptr ded_err;
+  auto dedupe = [&](hipStream_t q) {
+    try {
+      auto a = now();
+      unique_nodes_batch(dsets.data(), (int)dsets.size(), dedw_, q);
+      TG_HIP(hipStreamSynchronize(q));
+      ded_ms = ms(a, now());
+    } catch (...) {
+      ded_err = std::current_exception();
     }
+  };
+  std::thread ded_thread;
+  if (gid_mode) {
+    auto a = now();
+    for (int ib : act) dedupe_gid(*blocks_[ib]);
+    ded_ms = ms(a, now());
+  } else if (overlap_dedupe) {
+    // created on first use, so engines that never overlap hold one stream
+    if (!s2_) TG_HIP(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking));
+    ded_thread = std::thread(dedupe, s2_);
+  } else {
+    dedupe(s_);
+  }
+  try {
+    prepare_searches(act);
   } catch (...) {
-    if (obb_thread.joinable()) obb_thread.join();
+    if (ded_thread.joinable()) ded_thread.join();
     throw;
   }
-  auto td1 = now();
-  if (obb_thread.joinable()) obb_thread.join();
-  if (obb_err) std::rethrow_exception(obb_err);
-  obb_stepped_ += st;
-  for (size_t a = 0; a < act.size(); a++) obq[act[a]] = ob[a];
+  if (ded_thread.joinable()) ded_thread.join();
+  if (ded_err) std::rethrow_exception(ded_err);
+  prep_acc_[3] += ded_ms;
+}
+
+// search_batched's OBB batch, cell filter and ADT build for the active
+// blocks act
+void Engine::prepare_searches(const std::vector<int> &act)
+{
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto ms = [](auto a, auto c) { return std::chrono::duration<double, std::milli>(c - a).count(); };
+  const int nbk = (int)blocks_.size();
+  // query-set OBBs, all sets in one device batch with the bits of the
+  // sequential running sums
+  auto t0 = now();
+  std::vector<OBB> obq(nbk);
+  {
+    std::vector<ObbSet> os;
+    std::vector<OBB> ob(act.size());
+    for (int ib : act) os.push_back(ObbSet{blocks_[ib]->nsearch, blocks_[ib]->xsearch.p});
+    long long st = 0;
+    find_obbs_device(os.data(), (int)os.size(), ob.data(), obbw_, s_, &st);
+    obb_stepped_ += st;
+    for (size_t a = 0; a < act.size(); a++) obq[act[a]] = ob[a];
+  }
   auto t1 = now();
   // cells overlapping each query OBB, as a LIFO list, and their boxes: the
   // flags of all blocks in one array (block a's cells from coff[a], each
@@ -3017,11 +3032,9 @@ void Engine::search_batched()
   }
   sync();
   auto t3 = now();
-  // the OBB batch's own wall time (overlapped with the dedupe)
-  prep_acc_[0] += obb_ms;
+  prep_acc_[0] += ms(t0, t1);
   prep_acc_[1] += ms(t1, t2);
   prep_acc_[2] += ms(t2, t3);
-  prep_acc_[3] += ms(td0, td1);
 }
 
 // search.C:48-74: first occurrence of each node global id

@@ -50,7 +50,7 @@ Server::Server(Engine &main, int bg_tag, int nslots, double budget_bytes)
     s->eng.nfringe = main.nfringe;
     s->eng.mexclude = main.mexclude;
     s->eng.batched = true;
-    s->eng.overlap_obb = false;
+    s->eng.overlap_dedupe = false;
     s->eng.adopt_block(main, bg_tag, bg_tag, 2, nullptr, nullptr, false);
     slots_.push_back(s);
   }
