@@ -535,7 +535,12 @@ void unique_nodes_batch(DedupeSet *sets, int nset, DedupeScratch &w, hipStream_t
   }
   const int n = base[nset];
   if (n == 0) return;
-  const bool single = (nset == 1);
+  // sets that already lie back to back (every array) are worked on in place
+  bool contiguous = true;
+  for (int k = 1; k < nset; k++)
+    contiguous = contiguous && sets[k].x == sets[0].x + 3 * (size_t)base[k] && sets[k].tag == sets[0].tag + base[k] &&
+                 sets[k].rtag == sets[0].rtag + base[k] && sets[k].itag == sets[0].itag + base[k];
+  const bool single = (nset == 1) || contiguous;
   const double *x;
   const int *tag;
   double *rtag;

@@ -47,6 +47,14 @@ struct DBuf {
     }
     return p;
   }
+  // a view of n elements at q inside another buffer (never grown or freed)
+  void view(T *q, size_t n)
+  {
+    release();
+    p = q;
+    cap = n;
+    own = false;
+  }
   void borrow(const DBuf &src)
   {
     release();
@@ -421,6 +429,8 @@ class Engine {
   double last_search_kernel_ms_ = 0;
   DBuf<unsigned> mk0_, mk1_;
   DBuf<unsigned char> mset_;  // the per-set Morton table of a batched search
+  DBuf<double> qx_, qres_;    // every block's query points / resolutions, back to back
+  DBuf<int> qtag_, qrep_;     // every block's query mesh tags / representatives
   DBuf<int> mv0_, mv1_;
   std::vector<QuerySet *> qsets_;
   void sync();

@@ -2839,15 +2839,26 @@ void Engine::exchange_search_data()
     dest[i] = ii;
     if (ii >= 0) blocks_[pairs_[ii].local]->nsearch += cnt[i];
   }
+  // the points, resolutions, mesh tags and representatives of all blocks'
+  // query sets lie back to back in one allocation each, block after block,
+  // so the batched dedupe works on them in place
+  size_t qtot = 0;
+  for (Block *bp : blocks_) qtot += bp->nsearch;
+  qx_.reserve(3 * qtot);
+  qres_.reserve(qtot);
+  qtag_.reserve(qtot);
+  qrep_.reserve(qtot);
+  size_t qo = 0;
   for (Block *bp : blocks_) {
     Block &b = *bp;
-    size_t n = std::max(b.nsearch, 1);
-    b.xsearch.reserve(3 * n);
-    b.res_search.reserve(n);
-    b.isearch.reserve(3 * n);
-    b.tagsearch.reserve(n);
-    b.donorId.reserve(n);
-    b.xtag.reserve(n);
+    const size_t n = b.nsearch;  // views of exactly the set (a block without queries gets none)
+    b.xsearch.view(qx_.p + 3 * qo, 3 * n);
+    b.res_search.view(qres_.p + qo, n);
+    b.tagsearch.view(qtag_.p + qo, n);
+    b.xtag.view(qrep_.p + qo, n);
+    qo += n;
+    b.isearch.reserve(3 * std::max(n, (size_t)1));
+    b.donorId.reserve(std::max(n, (size_t)1));
   }
   std::vector<int> off(blocks_.size(), 0);
   for (int i = 0; i < np; i++) {
