@@ -956,6 +956,16 @@ __global__ void k_adt_slot3(int n, const int *__restrict__ base, int ntree, int 
   adtInt[4 * ((size_t)b + adtInt[4 * (size_t)i]) + 3] = i - b;
 }
 
+// the root box of every tree of a concatenated batch (tree t's elements
+// start at tb[t]; an empty tree's slot is left alone)
+__global__ void k_adt_roots(int ntree, const int *__restrict__ tb, const double *__restrict__ adtReal,
+                            double *__restrict__ roots)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntree || tb[t + 1] == tb[t]) return;
+  for (int j = 0; j < 6; j++) roots[6 * t + j] = adtReal[6 * (size_t)tb[t] + j];
+}
+
 }  // namespace
 
 void build_adt_batch(AdtTree *trees, int ntree, AdtScratch &w, hipStream_t s)
@@ -981,8 +991,14 @@ void build_adt_batch(AdtTree *trees, int ntree, AdtScratch &w, hipStream_t s)
     for (int t = 0; t < ntree; t++) empty_ext(trees[t].ext);
     return;
   }
-  // one tree builds in its own arrays; several are concatenated
-  const bool single = (ntree == 1);
+  // one tree builds in its own arrays, and so do trees whose arrays already
+  // lie back to back; others are concatenated
+  bool contiguous = true;
+  for (int t = 1; t < ntree; t++)
+    contiguous = contiguous && trees[t].ebox == trees[0].ebox + 6 * (size_t)base[t] &&
+                 trees[t].adtInt == trees[0].adtInt + 4 * (size_t)base[t] &&
+                 trees[t].adtReal == trees[0].adtReal + 6 * (size_t)base[t];
+  const bool single = (ntree == 1) || contiguous;
   const double *ebox;
   int *adtInt;
   double *adtReal;
@@ -1122,10 +1138,11 @@ void build_adt_batch(AdtTree *trees, int ntree, AdtScratch &w, hipStream_t s)
         TG_HIP(hipMemcpyAsync(trees[t].adtReal, adtReal + 6 * (size_t)base[t], sizeof(double) * 6 * trees[t].n,
                               hipMemcpyDeviceToDevice, s));
       }
+  // every root's box (its first adtReal entry) gathered, one copy back
   std::vector<double> roots(6 * (size_t)ntree);
-  for (int t = 0; t < ntree; t++)
-    if (trees[t].n)
-      TG_HIP(hipMemcpyAsync(roots.data() + 6 * t, trees[t].adtReal, sizeof(double) * 6, hipMemcpyDeviceToHost, s));
+  w.roots.reserve(6 * (size_t)ntree);
+  k_adt_roots<<<nblocks(ntree, 64), 64, 0, s>>>(ntree, w.tb.p, adtReal, w.roots.p);
+  TG_HIP(hipMemcpyAsync(roots.data(), w.roots.p, sizeof(double) * 6 * ntree, hipMemcpyDeviceToHost, s));
   TG_HIP(hipStreamSynchronize(s));
   if (err) throw std::runtime_error("tioga_amd: ADT build: partition loop exceeded its bound");
   for (int t = 0; t < ntree; t++) {

@@ -47,6 +47,12 @@ struct DBuf {
     }
     return p;
   }
+  // reserve in memory of its own (a view is dropped first)
+  T *reserve_own(size_t n)
+  {
+    if (!own) release();
+    return reserve(n);
+  }
   // a view of n elements at q inside another buffer (never grown or freed)
   void view(T *q, size_t n)
   {
@@ -183,7 +189,7 @@ struct QuerySet {
 // Reusable device scratch of the ADT build (tg_adt.hip).
 struct AdtScratch {
   DBuf<int> ea, A, B, err, tb, choff, cntA, cntB, adtInt;
-  DBuf<double> X, ebox, adtReal;
+  DBuf<double> X, ebox, adtReal, roots;
   DBuf<int4> segs;  // segment descriptors (8 ints each)
   DBuf<int2> tab;
   DBuf<double4> state;
@@ -431,6 +437,10 @@ class Engine {
   DBuf<unsigned char> mset_;  // the per-set Morton table of a batched search
   DBuf<double> qx_, qres_;    // every block's query points / resolutions, back to back
   DBuf<int> qtag_, qrep_;     // every block's query mesh tags / representatives
+  DBuf<double> tebox_, treal_;  // every batched tree's element boxes / adtReal, back to back
+  DBuf<int> tint_, telist_;     // every batched tree's adtInt / element list, back to back
+  DBuf<AdtNode> tnodes_;        // every batched tree's search records, back to back
+  DBuf<unsigned char> ntab_;    // the per-tree table of the batched search records
   DBuf<int> mv0_, mv1_;
   std::vector<QuerySet *> qsets_;
   void sync();

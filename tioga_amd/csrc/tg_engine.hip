@@ -875,12 +875,10 @@ __global__ void k_seg_starts(int nseg, int n, const int *__restrict__ sorted, co
 static_assert(sizeof(AdtNode) == 160, "ADT node record must be 160 bytes");
 
 // ADT arrays (adtIntegers, adtReals, element boxes) -> node records
-__global__ void k_adt_nodes(int nelem, TypeInfo ti, const int *__restrict__ adtInt,
-                            const double *__restrict__ adtReal, const double *__restrict__ ebox,
-                            const int *__restrict__ elist, AdtNode *__restrict__ nodes)
+__device__ __forceinline__ void k_adt_node_one(int n, const TypeInfo &ti, const int *__restrict__ adtInt,
+                                               const double *__restrict__ adtReal, const double *__restrict__ ebox,
+                                               const int *__restrict__ elist, AdtNode *__restrict__ nodes)
 {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nelem) return;
   int4 a = reinterpret_cast<const int4 *>(adtInt)[n];
   AdtNode N;
   N.cell = elist[a.x];
@@ -896,6 +894,39 @@ __global__ void k_adt_nodes(int nelem, TypeInfo ti, const int *__restrict__ adtI
     N.rbox[j] = N.right > -1 ? adtReal[6 * (size_t)N.right + j] + pad : 0.0;
   }
   nodes[n] = N;
+}
+
+__global__ void k_adt_nodes(int nelem, TypeInfo ti, const int *__restrict__ adtInt,
+                            const double *__restrict__ adtReal, const double *__restrict__ ebox,
+                            const int *__restrict__ elist, AdtNode *__restrict__ nodes)
+{
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nelem) return;
+  k_adt_node_one(n, ti, adtInt, adtReal, ebox, elist, nodes);
+}
+
+// k_adt_nodes for every tree of a batch whose arrays lie back to back: tree
+// t's elements are [base[t], base[t + 1]) of the concatenation, its adtInt
+// entries tree-local as the reference stores them
+struct AdtNodesTree {
+  TypeInfo ti;
+  int base;
+};
+__global__ void k_adt_nodes_all(int ntree, int total, const AdtNodesTree *__restrict__ T,
+                                const int *__restrict__ adtInt, const double *__restrict__ adtReal,
+                                const double *__restrict__ ebox, const int *__restrict__ elist,
+                                AdtNode *__restrict__ nodes)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  int lo = 0, hi = ntree - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (T[mid].base <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const size_t b = T[lo].base;
+  k_adt_node_one(g - (int)b, T[lo].ti, adtInt + 4 * b, adtReal + 6 * b, ebox + 6 * b, elist + b, nodes + b);
 }
 
 // 30-bit Morton code of each query in the frame of the search structure's
@@ -3012,14 +3043,22 @@ void Engine::prepare_searches(const std::vector<int> &act)
     TG_HIP(hipMemcpyAsync(eoff.data(), dstart.p, sizeof(int) * (na + 1), hipMemcpyDeviceToHost, s_));
     sync();
   }
+  // every tree's element boxes and ADT arrays back to back (views into one
+  // allocation each, in the order of the compacted element list), so the
+  // batched build works on them in place
+  const size_t te = eoff[na];
+  telist_.reserve(te);
+  tebox_.reserve(6 * te);
+  tint_.reserve(4 * te);
+  treal_.reserve(6 * te);
   for (int a = 0; a < na; a++) {
     Block &b = *blocks_[act[a]];
     const int N = b.ncells, ne = eoff[a + 1] - eoff[a];
     b.nelem = ne;
-    b.elist.reserve(ne);
-    b.ebox.reserve(6 * (size_t)ne);
-    b.adtInt.reserve(4 * (size_t)ne);
-    b.adtReal.reserve(6 * (size_t)ne);
+    b.elist.view(telist_.p + eoff[a], ne);
+    b.ebox.view(tebox_.p + 6 * (size_t)eoff[a], 6 * (size_t)ne);
+    b.adtInt.view(tint_.p + 4 * (size_t)eoff[a], 4 * (size_t)ne);
+    b.adtReal.view(treal_.p + 6 * (size_t)eoff[a], 6 * (size_t)ne);
     if (ne)
       k_elem_box<<<nb(ne), BS, 0, s_>>>(ne, N, iscratch2_.p + eoff[a], coff[a], b.ti, b.conn.p, b.x.p, b.elist.p,
                                         b.ebox.p);
@@ -3031,15 +3070,22 @@ void Engine::prepare_searches(const std::vector<int> &act)
   sync();
   auto t2 = now();
   build_adt_batch(trees.data(), (int)trees.size(), adtw_, s_);
-  for (size_t a = 0; a < act.size(); a++) {
+  // the search records of every tree in one launch
+  tnodes_.reserve(te);
+  std::vector<AdtNodesTree> nt(na);
+  for (int a = 0; a < na; a++) {
     Block &b = *blocks_[act[a]];
     b.adt_depth = trees[a].depth;
     memcpy(b.adtExt, trees[a].ext, sizeof b.adtExt);
     if (b.adt_depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
-    b.nodes.reserve(b.nelem);
-    if (b.nelem)
-      k_adt_nodes<<<nb(b.nelem), BS, 0, s_>>>(b.nelem, b.ti, b.adtInt.p, b.adtReal.p, b.ebox.p, b.elist.p,
-                                              b.nodes.p);
+    b.nodes.view(tnodes_.p + eoff[a], b.nelem);
+    nt[a] = AdtNodesTree{b.ti, eoff[a]};
+  }
+  if (te) {
+    ntab_.reserve(sizeof(AdtNodesTree) * na);
+    TG_HIP(hipMemcpyAsync(ntab_.p, nt.data(), sizeof(AdtNodesTree) * na, hipMemcpyHostToDevice, s_));
+    k_adt_nodes_all<<<nb(te), BS, 0, s_>>>(na, (int)te, (const AdtNodesTree *)ntab_.p, tint_.p, treal_.p,
+                                            tebox_.p, telist_.p, tnodes_.p);
   }
   sync();
   auto t3 = now();
@@ -3091,16 +3137,16 @@ void Engine::build_search_structure(Block &b, const OBB &obq, int &nelem, int &d
   if (N) k_cell_filter<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, b.x.p, obq, iscratch_.p);
   int ne = select_flagged(iscratch_.p, N, iscratch2_.p, s_);
   nelem = ne;
-  elist.reserve(ne);
-  ebox.reserve(6 * (size_t)ne);
-  adtInt.reserve(4 * (size_t)ne);
-  adtReal.reserve(6 * (size_t)ne);
+  elist.reserve_own(ne);
+  ebox.reserve_own(6 * (size_t)ne);
+  adtInt.reserve_own(4 * (size_t)ne);
+  adtReal.reserve_own(6 * (size_t)ne);
   if (ne) k_elem_box<<<nb(ne), BS, 0, s_>>>(ne, N, iscratch2_.p, 0, b.ti, b.conn.p, b.x.p, elist.p, ebox.p);
   sync();
   auto t2 = now();
   depth = build_adt_device(ne, ebox.p, adtInt.p, adtReal.p, ext, adtw_, s_);
   if (depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
-  nodes.reserve(ne);
+  nodes.reserve_own(ne);
   if (ne) k_adt_nodes<<<nb(ne), BS, 0, s_>>>(ne, b.ti, adtInt.p, adtReal.p, ebox.p, elist.p, nodes.p);
   sync();
   auto t3 = now();
