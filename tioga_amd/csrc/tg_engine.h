@@ -441,6 +441,8 @@ class Engine {
   DBuf<int> tint_, telist_;     // every batched tree's adtInt / element list, back to back
   DBuf<AdtNode> tnodes_;        // every batched tree's search records, back to back
   DBuf<unsigned char> ntab_;    // the per-tree table of the batched search records
+  DBuf<unsigned char> ptab_;    // the per-block table of the batched cell filter / element boxes
+  DBuf<int> pseg_;              // their cell and element offsets
   DBuf<int> mv0_, mv1_;
   std::vector<QuerySet *> qsets_;
   void sync();

@@ -801,11 +801,9 @@ __global__ void k_fill_search(int n, const int *__restrict__ sel, const int *__r
 // ---------------------------------------------------------------------------
 
 // Cells whose box in the query-set OBB frame overlaps that OBB (search.C:123-167)
-__global__ void k_cell_filter(int ncells, TypeInfo ti, const int *__restrict__ conn, const double *__restrict__ x,
-                              OBB obq, int *__restrict__ flag)
+__device__ __forceinline__ void cell_filter_one(int c, int ncells, const TypeInfo &ti, const int *__restrict__ conn,
+                                                const double *__restrict__ x, const OBB &obq, int *__restrict__ flag)
 {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
   int nv = ti.nv[type_of(ti, c)];
   const int *cv = conn + 8 * (size_t)c;
   double lo[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, hi[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
@@ -829,13 +827,20 @@ __global__ void k_cell_filter(int ncells, TypeInfo ti, const int *__restrict__ c
                           fabs(xc[2]) <= (dxc[2] + obq.dxc[2]));
 }
 
-// element list and element boxes (search.C:183-219)
-__global__ void k_elem_box(int nelem, int ncells, const int *__restrict__ sel, int sel0, TypeInfo ti,
-                           const int *__restrict__ conn, const double *__restrict__ x, int *__restrict__ elist,
-                           double *__restrict__ ebox)
+__global__ void k_cell_filter(int ncells, TypeInfo ti, const int *__restrict__ conn, const double *__restrict__ x,
+                              OBB obq, int *__restrict__ flag)
 {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nelem) return;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  cell_filter_one(c, ncells, ti, conn, x, obq, flag);
+}
+
+// element list and element boxes (search.C:183-219)
+__device__ __forceinline__ void elem_box_one(int e, int ncells, const int *__restrict__ sel, int sel0,
+                                             const TypeInfo &ti, const int *__restrict__ conn,
+                                             const double *__restrict__ x, int *__restrict__ elist,
+                                             double *__restrict__ ebox)
+{
   int c = ncells - 1 - (sel[e] - sel0);
   elist[e] = c;
   int nv = ti.nv[type_of(ti, c)];
@@ -853,6 +858,57 @@ __global__ void k_elem_box(int nelem, int ncells, const int *__restrict__ sel, i
     ebox[6 * (size_t)e + j] = lo[j];
     ebox[6 * (size_t)e + 3 + j] = hi[j];
   }
+}
+
+__global__ void k_elem_box(int nelem, int ncells, const int *__restrict__ sel, int sel0, TypeInfo ti,
+                           const int *__restrict__ conn, const double *__restrict__ x, int *__restrict__ elist,
+                           double *__restrict__ ebox)
+{
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  elem_box_one(e, ncells, sel, sel0, ti, conn, x, elist, ebox);
+}
+
+// The blocks of a batched search preparation: block a's cells are
+// [coff, coff + ncells) of the concatenated flags, its elements
+// [eoff, eoff + nelem) of the compacted list (element arrays back to back).
+struct PrepBlk {
+  TypeInfo ti;
+  const int *conn;
+  const double *x;
+  OBB ob;
+  int coff, ncells, eoff;
+};
+
+__device__ __forceinline__ int prep_find(int na, const PrepBlk *__restrict__ T, int g, bool by_elem)
+{
+  int lo = 0, hi = na - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((by_elem ? T[mid].eoff : T[mid].coff) <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// k_cell_filter for every block of the batch
+__global__ void k_cell_filter_all(int na, int total, const PrepBlk *__restrict__ T, int *__restrict__ flag)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const PrepBlk &B = T[prep_find(na, T, g, false)];
+  cell_filter_one(g - B.coff, B.ncells, B.ti, B.conn, B.x, B.ob, flag + B.coff);
+}
+
+// k_elem_box for every block of the batch (sel: the compacted flag indices)
+__global__ void k_elem_box_all(int na, int total, const PrepBlk *__restrict__ T, const int *__restrict__ sel,
+                               int *__restrict__ elist, double *__restrict__ ebox)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const PrepBlk &B = T[prep_find(na, T, g, true)];
+  elem_box_one(g - B.eoff, B.ncells, sel + B.eoff, B.coff, B.ti, B.conn, B.x, elist + B.eoff,
+               ebox + 6 * (size_t)B.eoff);
 }
 
 // start[s] = first position j of the ascending list sorted[0..n) with
@@ -3028,19 +3084,22 @@ void Engine::prepare_searches(const std::vector<int> &act)
   const int totN = coff[na];
   iscratch_.reserve(std::max(totN, 1));
   iscratch2_.reserve(std::max(totN, 1));
+  // the per-block table of the batched filter and element boxes
+  std::vector<PrepBlk> pb(na);
   for (int a = 0; a < na; a++) {
     Block &b = *blocks_[act[a]];
-    if (b.ncells)
-      k_cell_filter<<<nb(b.ncells), BS, 0, s_>>>(b.ncells, b.ti, b.conn.p, b.x.p, obq[act[a]], iscratch_.p + coff[a]);
+    pb[a] = PrepBlk{b.ti, b.conn.p, b.x.p, obq[act[a]], coff[a], b.ncells, 0};
   }
+  ptab_.reserve(sizeof(PrepBlk) * na);
+  pseg_.reserve(2 * (size_t)(na + 1));
+  TG_HIP(hipMemcpyAsync(ptab_.p, pb.data(), sizeof(PrepBlk) * na, hipMemcpyHostToDevice, s_));
+  if (totN) k_cell_filter_all<<<nb(totN), BS, 0, s_>>>(na, totN, (const PrepBlk *)ptab_.p, iscratch_.p);
   const int totE = totN ? select_flagged(iscratch_.p, totN, iscratch2_.p, s_) : 0;
   if (totE) {
-    DBuf<int> dcoff, dstart;
-    dcoff.reserve(na + 1);
-    dstart.reserve(na + 1);
-    TG_HIP(hipMemcpyAsync(dcoff.p, coff.data(), sizeof(int) * (na + 1), hipMemcpyHostToDevice, s_));
-    k_seg_starts<<<nb(na + 1), BS, 0, s_>>>(na, totE, iscratch2_.p, dcoff.p, dstart.p);
-    TG_HIP(hipMemcpyAsync(eoff.data(), dstart.p, sizeof(int) * (na + 1), hipMemcpyDeviceToHost, s_));
+    int *dcoff = pseg_.p, *dstart = pseg_.p + (na + 1);
+    TG_HIP(hipMemcpyAsync(dcoff, coff.data(), sizeof(int) * (na + 1), hipMemcpyHostToDevice, s_));
+    k_seg_starts<<<nb(na + 1), BS, 0, s_>>>(na, totE, iscratch2_.p, dcoff, dstart);
+    TG_HIP(hipMemcpyAsync(eoff.data(), dstart, sizeof(int) * (na + 1), hipMemcpyDeviceToHost, s_));
     sync();
   }
   // every tree's element boxes and ADT arrays back to back (views into one
@@ -3053,19 +3112,21 @@ void Engine::prepare_searches(const std::vector<int> &act)
   treal_.reserve(6 * te);
   for (int a = 0; a < na; a++) {
     Block &b = *blocks_[act[a]];
-    const int N = b.ncells, ne = eoff[a + 1] - eoff[a];
+    const int ne = eoff[a + 1] - eoff[a];
     b.nelem = ne;
     b.elist.view(telist_.p + eoff[a], ne);
     b.ebox.view(tebox_.p + 6 * (size_t)eoff[a], 6 * (size_t)ne);
     b.adtInt.view(tint_.p + 4 * (size_t)eoff[a], 4 * (size_t)ne);
     b.adtReal.view(treal_.p + 6 * (size_t)eoff[a], 6 * (size_t)ne);
-    if (ne)
-      k_elem_box<<<nb(ne), BS, 0, s_>>>(ne, N, iscratch2_.p + eoff[a], coff[a], b.ti, b.conn.p, b.x.p, b.elist.p,
-                                        b.ebox.p);
+    pb[a].eoff = eoff[a];
     trees[a].n = ne;
     trees[a].ebox = b.ebox.p;
     trees[a].adtInt = b.adtInt.p;
     trees[a].adtReal = b.adtReal.p;
+  }
+  if (te) {
+    TG_HIP(hipMemcpyAsync(ptab_.p, pb.data(), sizeof(PrepBlk) * na, hipMemcpyHostToDevice, s_));
+    k_elem_box_all<<<nb(te), BS, 0, s_>>>(na, (int)te, (const PrepBlk *)ptab_.p, iscratch2_.p, telist_.p, tebox_.p);
   }
   sync();
   auto t2 = now();
