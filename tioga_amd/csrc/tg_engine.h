@@ -269,6 +269,24 @@ struct ObbScratch {
 void find_obbs_device(const ObbSet *sets, int nset, OBB *out, ObbScratch &w, hipStream_t s,
                       long long *stepped = nullptr);
 
+// Reusable device scratch of the donor exchange (kept between calls, so a
+// connectivity step allocates nothing once the sizes have been seen).
+struct DonorScratch {
+  DBuf<int> dtag, pflag, psel, dnoff, lnode, ldblk, lrem, dsel, tmin, dstart, didx, errs, ireset;
+  DBuf<double> lrres;
+  DBuf<unsigned> dkey0, dkey1;
+  DBuf<unsigned char> mt, mt1, cancel, rt, et, ct, it;
+};
+
+// Reusable device scratch of the other connectivity phases: hole maps, box
+// exchange, query gathering.
+struct StepScratch {
+  DBuf<unsigned char> maps, walls, boxes, qpairs;
+  DBuf<int> act, chg, cand0, rng0, first, pblk, qflag, qpos, qsel;
+  DBuf<int2> rng;
+  DBuf<OBB> obc;
+};
+
 // Reusable device scratch of the donor search: per-query candidate lists
 // and the queue of queries with more candidates than a list holds.
 struct SearchScratch {
@@ -416,6 +434,16 @@ class Engine {
   void prepare_searches(const std::vector<int> &act);
   void dedupe_gid(Block &b);
   ObbScratch obbw_;           // query-set OBB batches
+  DonorScratch donw_;         // donor exchange
+  StepScratch stw_;           // hole maps, boxes, query gathering
+  // a host table copied to device memory that b keeps (grown on demand)
+  template <class T>
+  void *put_table(DBuf<unsigned char> &b, const T &h)
+  {
+    b.reserve(sizeof(T));
+    TG_HIP(hipMemcpyAsync(b.p, &h, sizeof(T), hipMemcpyHostToDevice, s_));
+    return b.p;
+  }
   DBuf<double> wallds_;       // hole-map cell sizes, per mesh tag
   DBuf<unsigned char *> wallsams_;  // hole maps, per mesh tag
   void exchange_donors();

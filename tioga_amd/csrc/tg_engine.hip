@@ -2560,7 +2560,6 @@ void Engine::get_hole_map()
     if (hm_[i].exist) live.push_back(i);
   if (live.size() > 128) throw std::runtime_error("tioga_amd: more than 128 wall meshes");
   MapSet ms{};
-  DevStruct<MapSet> dms;
   MapSet *pms = nullptr;
   int allow = 7;  // isym stencil: 3 excludes z, 1 excludes x, 2 excludes y
   if (isym == 1) allow = 6;
@@ -2589,7 +2588,7 @@ void Engine::get_hole_map()
         maxslices = std::max(maxslices, h.nx[saxis] - 2);
       }
     }
-    pms = dms.put(ms, s_);
+    pms = (MapSet *)put_table(stw_.maps, ms);
     if (batched) k_maps_cells<<<dim3(nb(maxcells), ms.nmap), BS, 0, s_>>>(pms, 0);
   }
   {
@@ -2615,7 +2614,6 @@ void Engine::get_hole_map()
       W.map[q] = b.tag - 1;
     }
     if (W.nb > 0) {
-      DevStruct<WallSet> dw;
       hm_view_.reserve(maxtag);
       TG_HIP(hipMemcpyAsync(hm_view_.p, views.data(), sizeof(HoleMapView) * maxtag, hipMemcpyHostToDevice, s_));
       wallds_.reserve(dsv.size());
@@ -2623,7 +2621,8 @@ void Engine::get_hole_map()
       TG_HIP(hipMemcpyAsync(wallds_.p, dsv.data(), sizeof(double) * dsv.size(), hipMemcpyHostToDevice, s_));
       TG_HIP(hipMemcpyAsync(wallsams_.p, sams.data(), sizeof(unsigned char *) * sams.size(),
                             hipMemcpyHostToDevice, s_));
-      k_mark_wall<<<nb(W.off[W.nb]), BS, 0, s_>>>(dw.put(W, s_), hm_view_.p, wallds_.p, wallsams_.p);
+      k_mark_wall<<<nb(W.off[W.nb]), BS, 0, s_>>>((const WallSet *)put_table(stw_.walls, W), hm_view_.p,
+                                                  wallds_.p, wallsams_.p);
       sync();  // the tables above are host memory until the copies have run
     }
   }
@@ -2631,7 +2630,7 @@ void Engine::get_hole_map()
   if (batched && !live.empty()) {
     const int nm = ms.nmap;
     k_maps_cells<<<dim3(nb(maxcells), nm), BS, 0, s_>>>(pms, 1);
-    DBuf<int> act, chg;
+    DBuf<int> &act = stw_.act, &chg = stw_.chg;
     act.reserve(nm);
     chg.reserve(nm);
     std::vector<int> hact(nm, 1), hchg(nm);
@@ -2728,7 +2727,8 @@ void Engine::exchange_boxes()
     memcpy(p.obc.vec, B.obb.vec, sizeof(double) * 9);
   }
   if (batched) {  // one pass over each block's cells for all of its pairs
-    std::vector<DevStruct<BoxPairs> *> tabs;
+    std::vector<BoxPairs> tabs;
+    std::vector<int> tblk;
     for (int ib = 0; ib < nb_; ib++) {
       Block &A = *blocks_[ib];
       BoxPairs bp{};
@@ -2740,12 +2740,21 @@ void Engine::exchange_boxes()
           bp.np++;
         }
       if (bp.np < 2 || A.ncells == 0) continue;
-      tabs.push_back(new DevStruct<BoxPairs>);
-      k_reduced_obb_multi<<<std::min(nb(A.ncells), 1024u), BS, 0, s_>>>(A.ncells, A.ti, A.conn.p, A.x.p, A.obb,
-                                                                        tabs.back()->put(bp, s_), kscratch_.p);
+      tabs.push_back(bp);
+      tblk.push_back(ib);
+    }
+    if (!tabs.empty()) {
+      // every block's partner table in one upload
+      stw_.boxes.reserve(sizeof(BoxPairs) * tabs.size());
+      TG_HIP(hipMemcpyAsync(stw_.boxes.p, tabs.data(), sizeof(BoxPairs) * tabs.size(), hipMemcpyHostToDevice, s_));
+      const BoxPairs *dt = (const BoxPairs *)stw_.boxes.p;
+      for (size_t k = 0; k < tabs.size(); k++) {
+        Block &A = *blocks_[tblk[k]];
+        k_reduced_obb_multi<<<std::min(nb(A.ncells), 1024u), BS, 0, s_>>>(A.ncells, A.ti, A.conn.p, A.x.p, A.obb,
+                                                                          dt + k, kscratch_.p);
+      }
     }
     sync();
-    for (auto *t : tabs) delete t;
   }
   std::vector<unsigned long long> keys(6 * (size_t)std::max(np, 1));
   TG_HIP(hipMemcpyAsync(keys.data(), kscratch_.p, sizeof(unsigned long long) * keys.size(), hipMemcpyDeviceToHost,
@@ -2864,8 +2873,8 @@ void Engine::exchange_search_data()
   // one flag kernel over the candidates of every pair, one compaction
   {
     QueryPairs qp{};
-    DBuf<int> dcand0, drng0, first;
-    DBuf<int2> drng;
+    DBuf<int> &dcand0 = stw_.cand0, &drng0 = stw_.rng0, &first = stw_.first;
+    DBuf<int2> &drng = stw_.rng;
     dcand0.reserve(np + 1);
     drng0.reserve(np + 1);
     first.reserve(np + 1);
@@ -2887,17 +2896,16 @@ void Engine::exchange_search_data()
       hobc[i] = pairs_[i].obc;
       hblk[i] = pairs_[i].local;
     }
-    DBuf<OBB> dobc;
-    DBuf<int> dblk;
+    DBuf<OBB> &dobc = stw_.obc;
+    DBuf<int> &dblk = stw_.pblk;
     dobc.reserve(np);
     dblk.reserve(np);
     TG_HIP(hipMemcpyAsync(dobc.p, hobc.data(), sizeof(OBB) * np, hipMemcpyHostToDevice, s_));
     TG_HIP(hipMemcpyAsync(dblk.p, hblk.data(), sizeof(int) * np, hipMemcpyHostToDevice, s_));
     qp.obc = dobc.p;
     qp.blk = dblk.p;
-    DevStruct<QueryPairs> dq;
-    QueryPairs *pq = dq.put(qp, s_);
-    DBuf<int> flag, pos, sel;
+    QueryPairs *pq = (QueryPairs *)put_table(stw_.qpairs, qp);
+    DBuf<int> &flag = stw_.qflag, &pos = stw_.qpos, &sel = stw_.qsel;
     flag.reserve(ncand);
     pos.reserve(ncand);
     sel.reserve(ncand);
@@ -3430,8 +3438,8 @@ void Engine::exchange_donors()
     rt.hblk[ib] = b.head_blk.p;
     rt.hrem[ib] = b.head_remote.p;
   }
-  DevStruct<RecvTables> drt;
-  RecvTables *prt = drt.put(rt, s_);
+  DonorScratch &w = donw_;
+  RecvTables *prt = (RecvTables *)put_table(w.rt, rt);
   // comm1: donor records, arrival order = (donor block, search index)
   std::vector<DonorSrc> src(nbk);
   int arrival = 0;
@@ -3450,7 +3458,7 @@ void Engine::exchange_donors()
   // processDonors for every block; records in (receptor block, node) order
   std::vector<int> blk_tag(nbk);
   for (int ib = 0; ib < nbk; ib++) blk_tag[ib] = blocks_[ib]->tag;
-  DBuf<int> dtag;
+  DBuf<int> &dtag = w.dtag;
   dtag.reserve(nbk);
   TG_HIP(hipMemcpyAsync(dtag.p, blk_tag.data(), sizeof(int) * nbk, hipMemcpyHostToDevice, s_));
   HoleSet H{nmesh_, hm_view_.p};
@@ -3460,7 +3468,7 @@ void Engine::exchange_donors()
     if (noff[ib + 1] < noff[ib]) throw std::runtime_error("tioga_amd: too many nodes for one donor exchange");
   }
   const int totV = noff[nbk];
-  DBuf<int> pflag, psel, dnoff;
+  DBuf<int> &pflag = w.pflag, &psel = w.psel, &dnoff = w.dnoff;
   pflag.reserve(std::max(totV, 1));
   psel.reserve(std::max(totV, 1));
   dnoff.reserve(nbk + 1);
@@ -3468,7 +3476,13 @@ void Engine::exchange_donors()
   EmitTables et{};
   et.nbk = nbk;
   et.noff = dnoff.p;
-  DBuf<unsigned char> mt, mt1;
+  DBuf<unsigned char> &mt = w.mt, &mt1 = w.mt1;
+  {
+    size_t mv = 1;
+    for (Block *bp : blocks_) mv = std::max<size_t>(mv, bp->nnodes);
+    mt.reserve(mv);
+    mt1.reserve(mv);
+  }
   for (int ib = 0; ib < nbk; ib++) {
     Block &b = *blocks_[ib];
     const int V = b.nnodes, N = b.ncells;
@@ -3476,8 +3490,6 @@ void Engine::exchange_donors()
     et.hrem[ib] = b.head_remote.p;
     et.nodeRes[ib] = b.nodeRes.p;
     et.scale[ib] = b.resolutionScale;
-    mt.reserve(V);
-    mt1.reserve(V);
     if (V)
       k_process1<<<nb(V), BS, 0, s_>>>(V, b.tag, b.x.p, H, nwords, b.has_donor.p, b.rmax.p, b.tagmask.p,
                                        b.iblank.p, b.nodeRes.p, mt.p);
@@ -3493,20 +3505,19 @@ void Engine::exchange_donors()
   // the records of all receptor blocks, in (receptor block, node) order
   const int totrec = totV ? select_flagged(pflag.p, totV, psel.p, s_) : 0;
   RecordList L;
-  DBuf<int> lnode, ldblk, lrem;
-  DBuf<double> lrres;
+  DBuf<int> &lnode = w.lnode, &ldblk = w.ldblk, &lrem = w.lrem;
+  DBuf<double> &lrres = w.lrres;
   L.n = totrec;
   L.node = lnode.reserve(totrec);
   L.dblk = ldblk.reserve(totrec);
   L.remote = lrem.reserve(totrec);
   L.rres = lrres.reserve(totrec);
-  DevStruct<EmitTables> det;
-  const EmitTables *pet = det.put(et, s_);
+  const EmitTables *pet = (const EmitTables *)put_table(w.et, et);
   if (totrec) k_emit_records_all<<<nb(totrec), BS, 0, s_>>>(totrec, psel.p, pet, L);
   // comm2: findInterpData per donor block, records in arrival order: a
   // stable sort by donor block keeps each donor's records in list order
-  DBuf<int> dsel, tmin, dstart, didx;
-  DBuf<unsigned> dkey0, dkey1;
+  DBuf<int> &dsel = w.dsel, &tmin = w.tmin, &dstart = w.dstart, &didx = w.didx;
+  DBuf<unsigned> &dkey0 = w.dkey0, &dkey1 = w.dkey1;
   dsel.reserve(std::max(totrec, 1));
   didx.reserve(std::max(totrec, 1));
   dkey0.reserve(std::max(totrec, 1));
@@ -3526,10 +3537,14 @@ void Engine::exchange_donors()
   size_t maxV = 1;
   for (Block *b : blocks_) maxV = std::max<size_t>(maxV, b->nnodes);
   tmin.reserve(maxV);
-  DBuf<int> errs;
+  DBuf<int> &errs = w.errs;
   errs.reserve(nbk);
   fill<int>(errs.p, nbk, 0, s_);
-  std::vector<DBuf<unsigned char> *> cancelv(nbk);
+  // every block's cancel flags in one buffer (block ib's from coff[ib])
+  std::vector<size_t> canoff(nbk + 1, 0);
+  for (int ib = 0; ib < nbk; ib++) canoff[ib + 1] = canoff[ib] + blocks_[ib]->nnodes;
+  w.cancel.reserve(std::max<size_t>(canoff[nbk], 1));
+  std::vector<unsigned char *> cancelv(nbk);
   CancelTables ct{};
   for (int ib = 0; ib < nbk; ib++) {
     Block &D = *blocks_[ib];
@@ -3545,9 +3560,8 @@ void Engine::exchange_donors()
     fill<int>(D.interp2donor.p, std::max(D.nsearch, 1), -1, s_);
     tmin.reserve(D.nnodes);
     fill<int>(tmin.p, D.nnodes, BIGINT, s_);
-    cancelv[ib] = new DBuf<unsigned char>;
-    cancelv[ib]->reserve(D.nnodes);
-    fill<unsigned char>(cancelv[ib]->p, D.nnodes, 0, s_);
+    cancelv[ib] = w.cancel.p + canoff[ib];
+    fill<unsigned char>(cancelv[ib], D.nnodes, 0, s_);
     ct.i2d[ib] = D.interp2donor.p;
     ct.cancel[ib] = D.interp_cancel.p;
     if (nd == 0) continue;
@@ -3566,7 +3580,7 @@ void Engine::exchange_donors()
     V.xsearch = D.xsearch.p;
     V.scale = D.resolutionScale;
     V.tmin = tmin.p;
-    V.cancelv = cancelv[ib]->p;
+    V.cancelv = cancelv[ib];
     V.inode = D.interp_inode.p;
     V.nw = D.interp_nw.p;
     V.cancel = D.interp_cancel.p;
@@ -3592,19 +3606,22 @@ void Engine::exchange_donors()
       }
   }
   // comm3: cancellations, then coincident resets
-  DevStruct<CancelTables> dct;
-  CancelTables *pct = dct.put(ct, s_);
+  CancelTables *pct = (CancelTables *)put_table(w.ct, ct);
   for (int ib = 0; ib < nbk; ib++) {
     Block &b = *blocks_[ib];
     if (b.nnodes)
-      k_cancel<<<nb(b.nnodes), BS, 0, s_>>>(b.nnodes, cancelv[ib]->p, b.has_donor.p, b.head_blk.p,
+      k_cancel<<<nb(b.nnodes), BS, 0, s_>>>(b.nnodes, cancelv[ib], b.has_donor.p, b.head_blk.p,
                                             b.head_remote.p, pct);
   }
-  DBuf<int> ireset;
+  DBuf<int> &ireset = w.ireset;
+  {
+    size_t mq = 1;
+    for (Block *bp : blocks_) mq = std::max<size_t>(mq, bp->nsearch);
+    ireset.reserve(mq);
+  }
   for (int ib = 0; ib < nbk; ib++) {
     Block &b = *blocks_[ib];
     if (b.nsearch == 0) continue;
-    ireset.reserve(b.nsearch);
     fill<int>(ireset.p, b.nsearch, 1, s_);
     k_coinc1<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.interp2donor.p, b.xtag.p, b.interp_cancel.p, ireset.p);
     k_coinc2<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.interp2donor.p, b.xtag.p, ireset.p, b.interp_cancel.p);
@@ -3616,8 +3633,7 @@ void Engine::exchange_donors()
     if (b.nnodes) k_clear_neg<<<nb(b.nnodes), BS, 0, s_>>>(b.nnodes, b.iblank.p);
     it.ib[ib] = b.iblank.p;
   }
-  DevStruct<IblankTables> dit;
-  IblankTables *pit = dit.put(it, s_);
+  IblankTables *pit = (IblankTables *)put_table(w.it, it);
   for (int ib = 0; ib < nbk; ib++) {
     Block &b = *blocks_[ib];
     if (b.ninterp)
@@ -3625,7 +3641,6 @@ void Engine::exchange_donors()
                                                     b.interp_rblk.p, pit);
   }
   sync();
-  for (auto *c : cancelv) delete c;
 }
 
 void Engine::cell_iblanks(Block &b)
