@@ -276,6 +276,7 @@ struct DonorScratch {
   DBuf<double> lrres;
   DBuf<unsigned> dkey0, dkey1;
   DBuf<unsigned char> mt, mt1, cancel, rt, et, ct, it;
+  DBuf<unsigned char> tab[10];  // per-stage segment tables and fill chunks
 };
 
 // Reusable device scratch of the other connectivity phases: hole maps, box
@@ -435,6 +436,9 @@ class Engine {
   void dedupe_gid(Block &b);
   ObbScratch obbw_;           // query-set OBB batches
   DonorScratch donw_;         // donor exchange
+  // a per-block segment table copied to device memory that b keeps
+  template <class S>
+  const S *put_segs(DBuf<unsigned char> &b, const std::vector<S> &v);
   StepScratch stw_;           // hole maps, boxes, query gathering
   // a host table copied to device memory that b keeps (grown on demand)
   template <class T>

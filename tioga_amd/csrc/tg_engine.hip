@@ -1561,101 +1561,10 @@ struct RecvTables {  // per receptor block, indexed by block id
   int *harr[128], *hblk[128], *hrem[128];
 };
 
-__global__ void k_donor_pass1(DonorSrc D, int nwords, const RecvTables *__restrict__ R)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= D.nsearch || D.donorId[i] < 0) return;
-  int rb = D.isearch[3 * i + 2], nd = D.isearch[3 * i + 1];
-  R->has[rb][nd] = 1;
-  atomicMax(&R->rmax[rb][nd], dkey(D.res_search[D.xtag[i]]));
-  atomicMin(&R->hkey[rb][nd], dkey(D.cellRes[D.donorId[i]]));
-  int t = D.dtag - BASE;
-  atomicOr(&R->tmask[rb][(size_t)nd * nwords + (t >> 6)], 1ull << (t & 63));
-}
-
-__global__ void k_donor_pass2(DonorSrc D, const RecvTables *__restrict__ R)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= D.nsearch || D.donorId[i] < 0) return;
-  int rb = D.isearch[3 * i + 2], nd = D.isearch[3 * i + 1];
-  if (dkey(D.cellRes[D.donorId[i]]) == R->hkey[rb][nd]) atomicMin(&R->harr[rb][nd], D.arrival0 + i);
-}
-
-__global__ void k_donor_pass3(DonorSrc D, const RecvTables *__restrict__ R)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= D.nsearch || D.donorId[i] < 0) return;
-  int rb = D.isearch[3 * i + 2], nd = D.isearch[3 * i + 1];
-  if (R->harr[rb][nd] == D.arrival0 + i) {
-    R->hblk[rb][nd] = D.dblk;
-    R->hrem[rb][nd] = i;
-  }
-}
-
 struct HoleSet {
   int nmesh;
   const HoleMapView *hm;
 };
-
-// MeshBlock::processDonors, first part (bookKeeping.C:159-238): iblank from
-// the hole maps of the other meshes (those that did not donate, when the node
-// has donors) and nodeRes raised to the largest receptor resolution.
-__global__ void k_process1(int nnodes, int mytag, const double *__restrict__ x, HoleSet H, int nwords,
-                           const int *__restrict__ has, const unsigned long long *__restrict__ rmax,
-                           const unsigned long long *__restrict__ tmask, int *__restrict__ iblank,
-                           double *__restrict__ nodeRes, unsigned char *__restrict__ mtag)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nnodes) return;
-  int ib = 1;
-  double p[3];
-  load_xyz(x, i, p);
-  bool hd = has[i] != 0;
-  if (hd) nodeRes[i] = dmax(nodeRes[i], dunkey(rmax[i]));
-  for (int j = 0; j < H.nmesh; j++) {
-    if (j == mytag - BASE || !H.hm[j].exist) continue;
-    if (hd && ((tmask[(size_t)i * nwords + (j >> 6)] >> (j & 63)) & 1ull)) continue;
-    if (check_hole(p, H.hm[j])) {
-      ib = 0;
-      break;
-    }
-  }
-  iblank[i] = ib;
-  mtag[i] = (ib == 0);
-}
-
-// nfringe layers around hole nodes (bookKeeping.C:240-286)
-__global__ void k_fringe(int ncells, TypeInfo ti, const int *__restrict__ conn, const unsigned char *__restrict__ mt,
-                         unsigned char *__restrict__ mt1)
-{
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
-  int nv = ti.nv[type_of(ti, c)];
-  const int *cv = conn + 8 * (size_t)c;
-  int any = 0;
-  for (int m = 0; m < nv; m++) any |= (mt[cv[m]] == 1);
-  if (!any) return;
-  for (int m = 0; m < nv; m++)
-    if (mt[cv[m]] != 1) mt1[cv[m]] = 1;
-}
-
-// bookKeeping.C:288-340: fringe nodes become mandatory receptors; a node with
-// donors is a receptor when its best donor is finer than it.
-__global__ void k_process2(int nnodes, const unsigned char *__restrict__ mt1, const int *__restrict__ has,
-                           const unsigned long long *__restrict__ hkey, const int *__restrict__ hblk,
-                           const int *__restrict__ blk_tag, int *__restrict__ iblank, double *__restrict__ nodeRes,
-                           int *__restrict__ isrec)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nnodes) return;
-  if (mt1[i] && iblank[i]) nodeRes[i] = BIGVALUE;
-  int r = 0;
-  if (has[i] && iblank[i] != 0 && dunkey(hkey[i]) < nodeRes[i]) {
-    iblank[i] = -blk_tag[hblk[i]];
-    r = 1;
-  }
-  isrec[i] = r;
-}
 
 // Records sent back to donors (bookKeeping.C:318-340), in receptor order.
 struct RecordList {
@@ -1737,30 +1646,6 @@ struct InterpView {
   int *err;
 };
 
-// Mandatory receptors (receptorRes == BIGVALUE) reset their donor cell's
-// negative-iblank vertices to 1 and put every vertex with iblank <= 0 and
-// finite nodeRes on the cancel list (bookKeeping.C:456-468).  A record
-// later in the donor's record order sees such a vertex as 1: tmin[v] is the
-// first record that resets v.
-__global__ void k_interp_mandatory(InterpView V)
-{
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= V.n) return;
-  int k = V.sel[r];
-  double rr = fabs(V.L.rres[k]);
-  if (!(rr == BIGVALUE && V.scale == 1.0)) return;
-  int c = V.donorId[V.L.remote[k]];
-  int nv = V.ti.nv[type_of(V.ti, c)];
-  const int *cv = V.conn + 8 * (size_t)c;
-  for (int m = 0; m < nv; m++) {
-    int v = cv[m];
-    if (V.iblank[v] <= 0 && V.nodeRes[v] != BIGVALUE) {
-      if (V.iblank[v] < 0) atomicMin(&V.tmin[v], r);
-      V.cancelv[v] = 1;
-    }
-  }
-}
-
 // MeshBlock::findInterpData (bookKeeping.C:385-535) for record r, seeing the
 // iblank values the reference sees at that point of its sequential walk.
 // The cell's vertex count is a template constant, so every weight and vertex
@@ -1804,28 +1689,6 @@ __device__ __forceinline__ void interp_record(const InterpView &V, int r, int c,
   if (bad) atomicAdd(V.err, 1);
 }
 
-__global__ void k_interp(InterpView V)
-{
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= V.n) return;
-  const int k = V.sel[r];
-  const double rr2 = V.L.rres[k], rr = fabs(rr2);
-  const int irec = V.L.remote[k];
-  const int c = V.donorId[irec];
-  switch (V.ti.nv[type_of(V.ti, c)]) {
-    case 4: interp_record<4>(V, r, c, irec, rr2, rr); break;
-    case 5: interp_record<5>(V, r, c, irec, rr2, rr); break;
-    case 6: interp_record<6>(V, r, c, irec, rr2, rr); break;
-    default: interp_record<8>(V, r, c, irec, rr2, rr); break;
-  }
-}
-
-__global__ void k_interp_reset(int nnodes, const int *__restrict__ tmin, int *__restrict__ iblank)
-{
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < nnodes && tmin[v] != BIGINT) iblank[v] = 1;
-}
-
 struct CancelTables {
   int *i2d[128];
   int *cancel[128];
@@ -1863,12 +1726,6 @@ __global__ void k_coinc2(int nq, const int *__restrict__ i2d, const int *__restr
   if (p > -1 && cancel[p] == 1) cancel[p] = ireset[xtag[i]];
 }
 
-__global__ void k_clear_neg(int n, int *__restrict__ iblank)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && iblank[i] < 0) iblank[i] = 1;
-}
-
 struct IblankTables {
   int *ib[128];
 };
@@ -1878,6 +1735,271 @@ __global__ void k_set_receptors(int n, const int *__restrict__ cancel, const int
 {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r < n && !cancel[r]) T->ib[rblk[r]][rnode[r]] = -1;
+}
+
+// ---------------------------------------------------------------------------
+// The donor exchange for every block in one launch per stage.  A launch's
+// threads are cut into segments, one per block, each starting at a multiple
+// of BS, so a workgroup lies in one segment and finds it by a wave-uniform
+// binary search over the segment table (off = first thread, n = threads in
+// use).  Within a stage the blocks touch only their own arrays or write
+// other blocks' entries commutatively (stores of one value, atomic min /
+// max / or), and every read of another block's data comes from an earlier
+// stage, so one launch gives what one launch per block gives.
+// ---------------------------------------------------------------------------
+template <class S>
+__device__ __forceinline__ const S &seg_of(int ns, const S *__restrict__ T)
+{
+  const int g0 = blockIdx.x * BS;
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (T[mid].off <= g0) lo = mid;
+    else hi = mid - 1;
+  }
+  return T[lo];
+}
+
+// fills of many arrays: chunk k sets p[start, start + len) to v (width 1, 4
+// or 8 bytes)
+struct FillChunk {
+  void *p;
+  unsigned long long v;
+  long long start;
+  int len, width;
+};
+__global__ void k_fill_chunks(const FillChunk *__restrict__ C)
+{
+  const FillChunk c = C[blockIdx.x];
+  for (int i = threadIdx.x; i < c.len; i += blockDim.x) {
+    const long long j = c.start + i;
+    if (c.width == 1) ((unsigned char *)c.p)[j] = (unsigned char)c.v;
+    else if (c.width == 4) ((unsigned *)c.p)[j] = (unsigned)c.v;
+    else ((unsigned long long *)c.p)[j] = c.v;
+  }
+}
+
+struct DonorSeg {
+  DonorSrc D;
+  int off, n;
+};
+__global__ void k_donor_pass1_all(int ns, const DonorSeg *__restrict__ T, int nwords, const RecvTables *__restrict__ R)
+{
+  const DonorSeg &S = seg_of(ns, T);
+  const int i = blockIdx.x * BS + threadIdx.x - S.off;
+  const DonorSrc &D = S.D;
+  if (i >= S.n || D.donorId[i] < 0) return;
+  int rb = D.isearch[3 * i + 2], nd = D.isearch[3 * i + 1];
+  R->has[rb][nd] = 1;
+  atomicMax(&R->rmax[rb][nd], dkey(D.res_search[D.xtag[i]]));
+  atomicMin(&R->hkey[rb][nd], dkey(D.cellRes[D.donorId[i]]));
+  int t = D.dtag - BASE;
+  atomicOr(&R->tmask[rb][(size_t)nd * nwords + (t >> 6)], 1ull << (t & 63));
+}
+__global__ void k_donor_pass2_all(int ns, const DonorSeg *__restrict__ T, const RecvTables *__restrict__ R)
+{
+  const DonorSeg &S = seg_of(ns, T);
+  const int i = blockIdx.x * BS + threadIdx.x - S.off;
+  const DonorSrc &D = S.D;
+  if (i >= S.n || D.donorId[i] < 0) return;
+  int rb = D.isearch[3 * i + 2], nd = D.isearch[3 * i + 1];
+  if (dkey(D.cellRes[D.donorId[i]]) == R->hkey[rb][nd]) atomicMin(&R->harr[rb][nd], D.arrival0 + i);
+}
+__global__ void k_donor_pass3_all(int ns, const DonorSeg *__restrict__ T, const RecvTables *__restrict__ R)
+{
+  const DonorSeg &S = seg_of(ns, T);
+  const int i = blockIdx.x * BS + threadIdx.x - S.off;
+  const DonorSrc &D = S.D;
+  if (i >= S.n || D.donorId[i] < 0) return;
+  int rb = D.isearch[3 * i + 2], nd = D.isearch[3 * i + 1];
+  if (R->harr[rb][nd] == D.arrival0 + i) {
+    R->hblk[rb][nd] = D.dblk;
+    R->hrem[rb][nd] = i;
+  }
+}
+
+// processDonors, per receptor block (its nodes, then its cells)
+struct ProcSeg {
+  int off, n, mytag;
+  const double *x;
+  const int *has;
+  const unsigned long long *rmax, *tmask, *hkey;
+  const int *hblk;
+  int *iblank;
+  double *nodeRes;
+  unsigned char *mt, *mt1;
+  int *isrec;
+};
+// MeshBlock::processDonors, first part (bookKeeping.C:159-238): iblank from
+// the hole maps of the other meshes (those that did not donate, when the node
+// has donors) and nodeRes raised to the largest receptor resolution.
+__global__ void k_process1_all(int ns, const ProcSeg *__restrict__ T, HoleSet H, int nwords)
+{
+  const ProcSeg &S = seg_of(ns, T);
+  const int i = blockIdx.x * BS + threadIdx.x - S.off;
+  if (i >= S.n) return;
+  int ib = 1;
+  double p[3];
+  load_xyz(S.x, i, p);
+  bool hd = S.has[i] != 0;
+  if (hd) S.nodeRes[i] = dmax(S.nodeRes[i], dunkey(S.rmax[i]));
+  for (int j = 0; j < H.nmesh; j++) {
+    if (j == S.mytag - BASE || !H.hm[j].exist) continue;
+    if (hd && ((S.tmask[(size_t)i * nwords + (j >> 6)] >> (j & 63)) & 1ull)) continue;
+    if (check_hole(p, H.hm[j])) {
+      ib = 0;
+      break;
+    }
+  }
+  S.iblank[i] = ib;
+  S.mt[i] = (ib == 0);
+}
+struct FringeSeg {
+  int off, n;
+  TypeInfo ti;
+  const int *conn;
+  const unsigned char *mt;
+  unsigned char *mt1;
+};
+// nfringe layers around hole nodes (bookKeeping.C:240-286)
+__global__ void k_fringe_all(int ns, const FringeSeg *__restrict__ T)
+{
+  const FringeSeg &S = seg_of(ns, T);
+  const int c = blockIdx.x * BS + threadIdx.x - S.off;
+  if (c >= S.n) return;
+  int nv = S.ti.nv[type_of(S.ti, c)];
+  const int *cv = S.conn + 8 * (size_t)c;
+  int any = 0;
+  for (int m = 0; m < nv; m++) any |= (S.mt[cv[m]] == 1);
+  if (!any) return;
+  for (int m = 0; m < nv; m++)
+    if (S.mt[cv[m]] != 1) S.mt1[cv[m]] = 1;
+}
+// bookKeeping.C:288-340: fringe nodes become mandatory receptors; a node with
+// donors is a receptor when its best donor is finer than it.
+__global__ void k_process2_all(int ns, const ProcSeg *__restrict__ T, const int *__restrict__ blk_tag)
+{
+  const ProcSeg &S = seg_of(ns, T);
+  const int i = blockIdx.x * BS + threadIdx.x - S.off;
+  if (i >= S.n) return;
+  if (S.mt1[i] && S.iblank[i]) S.nodeRes[i] = BIGVALUE;
+  int r = 0;
+  if (S.has[i] && S.iblank[i] != 0 && dunkey(S.hkey[i]) < S.nodeRes[i]) {
+    S.iblank[i] = -blk_tag[S.hblk[i]];
+    r = 1;
+  }
+  S.isrec[i] = r;
+}
+
+// findInterpData per donor block (records) and its resets (nodes)
+struct InterpSeg {
+  int off, n;
+  InterpView V;
+};
+// Mandatory receptors (receptorRes == BIGVALUE) reset their donor cell's
+// negative-iblank vertices to 1 and put every vertex with iblank <= 0 and
+// finite nodeRes on the cancel list (bookKeeping.C:456-468).  A record
+// later in the donor's record order sees such a vertex as 1: tmin[v] is the
+// first record that resets v.
+__global__ void k_interp_mandatory_all(int ns, const InterpSeg *__restrict__ T)
+{
+  const InterpSeg &S = seg_of(ns, T);
+  const int r = blockIdx.x * BS + threadIdx.x - S.off;
+  if (r >= S.n) return;
+  const InterpView &V = S.V;
+  int k = V.sel[r];
+  double rr = fabs(V.L.rres[k]);
+  if (!(rr == BIGVALUE && V.scale == 1.0)) return;
+  int c = V.donorId[V.L.remote[k]];
+  int nv = V.ti.nv[type_of(V.ti, c)];
+  const int *cv = V.conn + 8 * (size_t)c;
+  for (int m = 0; m < nv; m++) {
+    int v = cv[m];
+    if (V.iblank[v] <= 0 && V.nodeRes[v] != BIGVALUE) {
+      if (V.iblank[v] < 0) atomicMin(&V.tmin[v], r);
+      V.cancelv[v] = 1;
+    }
+  }
+}
+__global__ void k_interp_all(int ns, const InterpSeg *__restrict__ T)
+{
+  const InterpSeg &S = seg_of(ns, T);
+  const int r = blockIdx.x * BS + threadIdx.x - S.off;
+  if (r >= S.n) return;
+  const InterpView &V = S.V;
+  const int k = V.sel[r];
+  const double rr2 = V.L.rres[k], rr = fabs(rr2);
+  const int irec = V.L.remote[k];
+  const int c = V.donorId[irec];
+  switch (V.ti.nv[type_of(V.ti, c)]) {
+    case 4: interp_record<4>(V, r, c, irec, rr2, rr); break;
+    case 5: interp_record<5>(V, r, c, irec, rr2, rr); break;
+    case 6: interp_record<6>(V, r, c, irec, rr2, rr); break;
+    default: interp_record<8>(V, r, c, irec, rr2, rr); break;
+  }
+}
+
+// node-wise stages of comm 3 / comm 4 (cancel lists, clearIblanks)
+struct NodeSeg {
+  int off, n;
+  const int *tmin;
+  const unsigned char *cancelv;
+  const int *has, *hblk, *hrem;
+  int *iblank;
+};
+__global__ void k_interp_reset_all(int ns, const NodeSeg *__restrict__ T)
+{
+  const NodeSeg &S = seg_of(ns, T);
+  const int v = blockIdx.x * BS + threadIdx.x - S.off;
+  if (v < S.n && S.tmin[v] != BIGINT) S.iblank[v] = 1;
+}
+__global__ void k_cancel_all(int ns, const NodeSeg *__restrict__ T, const CancelTables *__restrict__ C)
+{
+  const NodeSeg &S = seg_of(ns, T);
+  const int v = blockIdx.x * BS + threadIdx.x - S.off;
+  if (v >= S.n || !S.cancelv[v] || !S.has[v]) return;
+  int d = S.hblk[v];
+  int iptr = C->i2d[d][S.hrem[v]];
+  if (iptr > -1) C->cancel[d][iptr] = 1;
+}
+__global__ void k_clear_neg_all(int ns, const NodeSeg *__restrict__ T)
+{
+  const NodeSeg &S = seg_of(ns, T);
+  const int i = blockIdx.x * BS + threadIdx.x - S.off;
+  if (i < S.n && S.iblank[i] < 0) S.iblank[i] = 1;
+}
+
+// resetCoincident per block (its query points), and the receptor marks
+// (its interpolation records)
+struct QuerySeg {
+  int off, n;
+  int *i2d;
+  const int *xtag;
+  int *cancel;
+  int *ireset;
+  const int *rnode, *rblk;
+};
+__global__ void k_coinc1_all(int ns, const QuerySeg *__restrict__ T)
+{
+  const QuerySeg &S = seg_of(ns, T);
+  const int i = blockIdx.x * BS + threadIdx.x - S.off;
+  if (i >= S.n) return;
+  int p = S.i2d[i];
+  if (p > -1) atomicMin(&S.ireset[S.xtag[i]], S.cancel[p]);
+}
+__global__ void k_coinc2_all(int ns, const QuerySeg *__restrict__ T)
+{
+  const QuerySeg &S = seg_of(ns, T);
+  const int i = blockIdx.x * BS + threadIdx.x - S.off;
+  if (i >= S.n) return;
+  int p = S.i2d[i];
+  if (p > -1 && S.cancel[p] == 1) S.cancel[p] = S.ireset[S.xtag[i]];
+}
+__global__ void k_set_receptors_all(int ns, const QuerySeg *__restrict__ T, const IblankTables *__restrict__ I)
+{
+  const QuerySeg &S = seg_of(ns, T);
+  const int r = blockIdx.x * BS + threadIdx.x - S.off;
+  if (r < S.n && !S.cancel[r]) I->ib[S.rblk[r]][S.rnode[r]] = -1;
 }
 
 // MeshBlock::reduce_fringes (bookKeeping.C:650-760).  iblank_reduced starts
@@ -3406,12 +3528,50 @@ void Engine::launch_searches()
   k_copy_dup_sets<<<nb(tot), BS, 0, s_>>>((const SearchView *)sw_.sv.p, sw_.qb.p, ns, (int)tot);
 }
 
-// tioga::exchangeDonors (exchangeDonors.C:25-301), single rank
+// tioga::exchangeDonors (exchangeDonors.C:25-301), single rank.  Every
+// stage is one launch over all blocks (per-block segment tables).
+namespace {
+template <class S>
+int pack_segs(std::vector<S> &v)
+{
+  long long o = 0;
+  for (S &x : v) {
+    x.off = (int)o;
+    o += (long long)(x.n + BS - 1) / BS * BS;
+  }
+  if (o > 0x7fffffffLL) throw std::runtime_error("tioga_amd: donor exchange stage too large for one launch");
+  return (int)o;
+}
+}  // namespace
+
+template <class S>
+const S *Engine::put_segs(DBuf<unsigned char> &b, const std::vector<S> &v)
+{
+  b.reserve(sizeof(S) * std::max<size_t>(v.size(), 1));
+  if (!v.empty()) TG_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(S) * v.size(), hipMemcpyHostToDevice, s_));
+  return (const S *)b.p;
+}
+
 void Engine::exchange_donors()
 {
   const int nbk = (int)blocks_.size();
   if (nsend_ == 0 || nbk == 0) return;
   const int nwords = (nmesh_ + 63) / 64;
+  DonorScratch &w = donw_;
+  constexpr int FCH = 1 << 16;  // elements per fill chunk
+  // fills in three batches, each its own host table (alive until the end)
+  std::vector<FillChunk> fills[3];
+  int fb = 0;
+  auto add_fill = [&](void *p, size_t n, unsigned long long v, int width) {
+    for (size_t st = 0; st < n; st += FCH)
+      fills[fb].push_back(FillChunk{p, v, (long long)st, (int)std::min<size_t>(FCH, n - st), width});
+  };
+  auto run_fills = [&](DBuf<unsigned char> &tab) {
+    const std::vector<FillChunk> &fv = fills[fb++];
+    if (fv.empty()) return;
+    const FillChunk *dc = put_segs(tab, fv);
+    k_fill_chunks<<<(unsigned)fv.size(), BS, 0, s_>>>(dc);
+  };
   RecvTables rt{};
   for (int ib = 0; ib < nbk; ib++) {
     Block &b = *blocks_[ib];
@@ -3423,13 +3583,13 @@ void Engine::exchange_donors()
     b.head_blk.reserve(V);
     b.head_remote.reserve(V);
     b.tagmask.reserve((size_t)V * nwords);
-    fill<int>(b.has_donor.p, V, 0, s_);
-    fill<unsigned long long>(b.rmax.p, V, 0ull, s_);
-    fill<unsigned long long>(b.head_key.p, V, ~0ull, s_);
-    fill<int>(b.head_arr.p, V, BIGINT, s_);
-    fill<int>(b.head_blk.p, V, -1, s_);
-    fill<int>(b.head_remote.p, V, -1, s_);
-    fill<unsigned long long>(b.tagmask.p, (size_t)V * nwords, 0ull, s_);
+    add_fill(b.has_donor.p, V, 0, 4);
+    add_fill(b.rmax.p, V, 0ull, 8);
+    add_fill(b.head_key.p, V, ~0ull, 8);
+    add_fill(b.head_arr.p, V, (unsigned)BIGINT, 4);
+    add_fill(b.head_blk.p, V, 0xffffffffu, 4);
+    add_fill(b.head_remote.p, V, 0xffffffffu, 4);
+    add_fill(b.tagmask.p, (size_t)V * nwords, 0ull, 8);
     rt.has[ib] = b.has_donor.p;
     rt.rmax[ib] = b.rmax.p;
     rt.hkey[ib] = b.head_key.p;
@@ -3438,23 +3598,26 @@ void Engine::exchange_donors()
     rt.hblk[ib] = b.head_blk.p;
     rt.hrem[ib] = b.head_remote.p;
   }
-  DonorScratch &w = donw_;
+  run_fills(w.tab[0]);
   RecvTables *prt = (RecvTables *)put_table(w.rt, rt);
   // comm1: donor records, arrival order = (donor block, search index)
-  std::vector<DonorSrc> src(nbk);
+  std::vector<DonorSeg> dseg;
   int arrival = 0;
   for (int ib = 0; ib < nbk; ib++) {
     Block &D = *blocks_[ib];
-    src[ib] = DonorSrc{D.nsearch, arrival, ib, D.tag, D.donorId.p, D.isearch.p, D.xtag.p, D.cellRes.p,
+    const DonorSrc src{D.nsearch, arrival, ib, D.tag, D.donorId.p, D.isearch.p, D.xtag.p, D.cellRes.p,
                        D.res_search.p};
+    if (D.nsearch) dseg.push_back(DonorSeg{src, 0, D.nsearch});
     arrival += D.nsearch;
   }
-  for (int ib = 0; ib < nbk; ib++)
-    if (src[ib].nsearch) k_donor_pass1<<<nb(src[ib].nsearch), BS, 0, s_>>>(src[ib], nwords, prt);
-  for (int ib = 0; ib < nbk; ib++)
-    if (src[ib].nsearch) k_donor_pass2<<<nb(src[ib].nsearch), BS, 0, s_>>>(src[ib], prt);
-  for (int ib = 0; ib < nbk; ib++)
-    if (src[ib].nsearch) k_donor_pass3<<<nb(src[ib].nsearch), BS, 0, s_>>>(src[ib], prt);
+  if (!dseg.empty()) {
+    const int tot = pack_segs(dseg);
+    const DonorSeg *dt = put_segs(w.tab[1], dseg);
+    const int ns = (int)dseg.size();
+    k_donor_pass1_all<<<tot / BS, BS, 0, s_>>>(ns, dt, nwords, prt);
+    k_donor_pass2_all<<<tot / BS, BS, 0, s_>>>(ns, dt, prt);
+    k_donor_pass3_all<<<tot / BS, BS, 0, s_>>>(ns, dt, prt);
+  }
   // processDonors for every block; records in (receptor block, node) order
   std::vector<int> blk_tag(nbk);
   for (int ib = 0; ib < nbk; ib++) blk_tag[ib] = blocks_[ib]->tag;
@@ -3476,32 +3639,35 @@ void Engine::exchange_donors()
   EmitTables et{};
   et.nbk = nbk;
   et.noff = dnoff.p;
+  // the hole-cut and fringe marks of every block in one buffer (block ib's
+  // nodes from noff[ib])
   DBuf<unsigned char> &mt = w.mt, &mt1 = w.mt1;
-  {
-    size_t mv = 1;
-    for (Block *bp : blocks_) mv = std::max<size_t>(mv, bp->nnodes);
-    mt.reserve(mv);
-    mt1.reserve(mv);
-  }
+  mt.reserve(std::max(totV, 1));
+  mt1.reserve(std::max(totV, 1));
+  std::vector<ProcSeg> pseg;
+  std::vector<FringeSeg> fseg;
   for (int ib = 0; ib < nbk; ib++) {
     Block &b = *blocks_[ib];
-    const int V = b.nnodes, N = b.ncells;
     et.hblk[ib] = b.head_blk.p;
     et.hrem[ib] = b.head_remote.p;
     et.nodeRes[ib] = b.nodeRes.p;
     et.scale[ib] = b.resolutionScale;
-    if (V)
-      k_process1<<<nb(V), BS, 0, s_>>>(V, b.tag, b.x.p, H, nwords, b.has_donor.p, b.rmax.p, b.tagmask.p,
-                                       b.iblank.p, b.nodeRes.p, mt.p);
-    TG_HIP(hipMemcpyAsync(mt1.p, mt.p, V, hipMemcpyDeviceToDevice, s_));
-    for (int it = 0; it < nfringe; it++) {
-      if (N) k_fringe<<<nb(N), BS, 0, s_>>>(N, b.ti, b.conn.p, mt.p, mt1.p);
-      TG_HIP(hipMemcpyAsync(mt.p, mt1.p, V, hipMemcpyDeviceToDevice, s_));
-    }
-    if (V)
-      k_process2<<<nb(V), BS, 0, s_>>>(V, mt1.p, b.has_donor.p, b.head_key.p, b.head_blk.p, dtag.p, b.iblank.p,
-                                       b.nodeRes.p, pflag.p + noff[ib]);
+    if (b.nnodes)
+      pseg.push_back(ProcSeg{0, b.nnodes, b.tag, b.x.p, b.has_donor.p, b.rmax.p, b.tagmask.p, b.head_key.p,
+                             b.head_blk.p, b.iblank.p, b.nodeRes.p, mt.p + noff[ib], mt1.p + noff[ib],
+                             pflag.p + noff[ib]});
+    if (b.ncells) fseg.push_back(FringeSeg{0, b.ncells, b.ti, b.conn.p, mt.p + noff[ib], mt1.p + noff[ib]});
   }
+  const int totP = pack_segs(pseg), totF = pack_segs(fseg);
+  const ProcSeg *pt = put_segs(w.tab[2], pseg);
+  const FringeSeg *ft = put_segs(w.tab[3], fseg);
+  if (totP) k_process1_all<<<totP / BS, BS, 0, s_>>>((int)pseg.size(), pt, H, nwords);
+  if (totV) TG_HIP(hipMemcpyAsync(mt1.p, mt.p, totV, hipMemcpyDeviceToDevice, s_));
+  for (int it = 0; it < nfringe; it++) {
+    if (totF) k_fringe_all<<<totF / BS, BS, 0, s_>>>((int)fseg.size(), ft);
+    if (totV) TG_HIP(hipMemcpyAsync(mt.p, mt1.p, totV, hipMemcpyDeviceToDevice, s_));
+  }
+  if (totP) k_process2_all<<<totP / BS, BS, 0, s_>>>((int)pseg.size(), pt, dtag.p);
   // the records of all receptor blocks, in (receptor block, node) order
   const int totrec = totV ? select_flagged(pflag.p, totV, psel.p, s_) : 0;
   RecordList L;
@@ -3534,18 +3700,18 @@ void Engine::exchange_donors()
     sync();
   }
   for (int ib = 0; ib < nbk; ib++) ndv[ib] = doff[ib + 1] - doff[ib];
-  size_t maxV = 1;
-  for (Block *b : blocks_) maxV = std::max<size_t>(maxV, b->nnodes);
-  tmin.reserve(maxV);
+  // first reset records and cancel flags of every donor block, each in one
+  // buffer (block ib's nodes from noff[ib])
+  tmin.reserve(std::max(totV, 1));
+  w.cancel.reserve(std::max(totV, 1));
   DBuf<int> &errs = w.errs;
   errs.reserve(nbk);
-  fill<int>(errs.p, nbk, 0, s_);
-  // every block's cancel flags in one buffer (block ib's from coff[ib])
-  std::vector<size_t> canoff(nbk + 1, 0);
-  for (int ib = 0; ib < nbk; ib++) canoff[ib + 1] = canoff[ib] + blocks_[ib]->nnodes;
-  w.cancel.reserve(std::max<size_t>(canoff[nbk], 1));
-  std::vector<unsigned char *> cancelv(nbk);
+  add_fill(errs.p, nbk, 0, 4);
+  add_fill(tmin.p, totV, (unsigned)BIGINT, 4);
+  add_fill(w.cancel.p, totV, 0, 1);
   CancelTables ct{};
+  std::vector<InterpSeg> iseg;
+  std::vector<NodeSeg> nseg;
   for (int ib = 0; ib < nbk; ib++) {
     Block &D = *blocks_[ib];
     const int nd = ndv[ib];
@@ -3557,13 +3723,12 @@ void Engine::exchange_donors()
     D.interp_rnode.reserve(nd);
     D.interp_rblk.reserve(nd);
     D.interp2donor.reserve(std::max(D.nsearch, 1));
-    fill<int>(D.interp2donor.p, std::max(D.nsearch, 1), -1, s_);
-    tmin.reserve(D.nnodes);
-    fill<int>(tmin.p, D.nnodes, BIGINT, s_);
-    cancelv[ib] = w.cancel.p + canoff[ib];
-    fill<unsigned char>(cancelv[ib], D.nnodes, 0, s_);
+    add_fill(D.interp2donor.p, std::max(D.nsearch, 1), 0xffffffffu, 4);
     ct.i2d[ib] = D.interp2donor.p;
     ct.cancel[ib] = D.interp_cancel.p;
+    if (D.nnodes)
+      nseg.push_back(NodeSeg{0, D.nnodes, tmin.p + noff[ib], w.cancel.p + noff[ib], D.has_donor.p, D.head_blk.p,
+                             D.head_remote.p, D.iblank.p});
     if (nd == 0) continue;
     InterpView V;
     V.n = nd;
@@ -3579,8 +3744,8 @@ void Engine::exchange_donors()
     V.tagsearch = D.tagsearch.p;
     V.xsearch = D.xsearch.p;
     V.scale = D.resolutionScale;
-    V.tmin = tmin.p;
-    V.cancelv = cancelv[ib];
+    V.tmin = tmin.p + noff[ib];
+    V.cancelv = w.cancel.p + noff[ib];
     V.inode = D.interp_inode.p;
     V.nw = D.interp_nw.p;
     V.cancel = D.interp_cancel.p;
@@ -3589,10 +3754,17 @@ void Engine::exchange_donors()
     V.i2d = D.interp2donor.p;
     V.w = D.interp_w.p;
     V.err = errs.p + ib;
-    k_interp_mandatory<<<nb(nd), BS, 0, s_>>>(V);
-    k_interp<<<nb(nd), BS, 0, s_>>>(V);
-    k_interp_reset<<<nb(D.nnodes), BS, 0, s_>>>(D.nnodes, tmin.p, D.iblank.p);
+    iseg.push_back(InterpSeg{0, nd, V});
   }
+  run_fills(w.tab[4]);
+  const int totI = pack_segs(iseg), totN = pack_segs(nseg);
+  const InterpSeg *itab = put_segs(w.tab[5], iseg);
+  const NodeSeg *ntab = put_segs(w.tab[6], nseg);
+  if (totI) {
+    k_interp_mandatory_all<<<totI / BS, BS, 0, s_>>>((int)iseg.size(), itab);
+    k_interp_all<<<totI / BS, BS, 0, s_>>>((int)iseg.size(), itab);
+  }
+  if (totN) k_interp_reset_all<<<totN / BS, BS, 0, s_>>>((int)nseg.size(), ntab);
   {
     std::vector<int> nbad(nbk, 0);
     TG_HIP(hipMemcpyAsync(nbad.data(), errs.p, sizeof(int) * nbk, hipMemcpyDeviceToHost, s_));
@@ -3605,41 +3777,39 @@ void Engine::exchange_donors()
                 nbad[ib], blocks_[ib]->tag);
       }
   }
-  // comm3: cancellations, then coincident resets
+  // comm3: cancellations, then coincident resets (every block's reset
+  // words in one buffer, block ib's from qoff[ib])
   CancelTables *pct = (CancelTables *)put_table(w.ct, ct);
-  for (int ib = 0; ib < nbk; ib++) {
-    Block &b = *blocks_[ib];
-    if (b.nnodes)
-      k_cancel<<<nb(b.nnodes), BS, 0, s_>>>(b.nnodes, cancelv[ib], b.has_donor.p, b.head_blk.p,
-                                            b.head_remote.p, pct);
-  }
+  if (totN) k_cancel_all<<<totN / BS, BS, 0, s_>>>((int)nseg.size(), ntab, pct);
+  std::vector<size_t> qoff(nbk + 1, 0);
+  for (int ib = 0; ib < nbk; ib++) qoff[ib + 1] = qoff[ib] + blocks_[ib]->nsearch;
   DBuf<int> &ireset = w.ireset;
-  {
-    size_t mq = 1;
-    for (Block *bp : blocks_) mq = std::max<size_t>(mq, bp->nsearch);
-    ireset.reserve(mq);
-  }
-  for (int ib = 0; ib < nbk; ib++) {
-    Block &b = *blocks_[ib];
-    if (b.nsearch == 0) continue;
-    fill<int>(ireset.p, b.nsearch, 1, s_);
-    k_coinc1<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.interp2donor.p, b.xtag.p, b.interp_cancel.p, ireset.p);
-    k_coinc2<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.interp2donor.p, b.xtag.p, ireset.p, b.interp_cancel.p);
-  }
-  // comm4: clear negative iblanks, then mark live receptors
+  ireset.reserve(std::max<size_t>(qoff[nbk], 1));
+  add_fill(ireset.p, qoff[nbk], 1, 4);
+  std::vector<QuerySeg> qseg, rseg;
   IblankTables it{};
   for (int ib = 0; ib < nbk; ib++) {
     Block &b = *blocks_[ib];
-    if (b.nnodes) k_clear_neg<<<nb(b.nnodes), BS, 0, s_>>>(b.nnodes, b.iblank.p);
     it.ib[ib] = b.iblank.p;
-  }
-  IblankTables *pit = (IblankTables *)put_table(w.it, it);
-  for (int ib = 0; ib < nbk; ib++) {
-    Block &b = *blocks_[ib];
+    if (b.nsearch)
+      qseg.push_back(QuerySeg{0, b.nsearch, b.interp2donor.p, b.xtag.p, b.interp_cancel.p, ireset.p + qoff[ib],
+                              nullptr, nullptr});
     if (b.ninterp)
-      k_set_receptors<<<nb(b.ninterp), BS, 0, s_>>>(b.ninterp, b.interp_cancel.p, b.interp_rnode.p,
-                                                    b.interp_rblk.p, pit);
+      rseg.push_back(QuerySeg{0, b.ninterp, nullptr, nullptr, b.interp_cancel.p, nullptr, b.interp_rnode.p,
+                              b.interp_rblk.p});
   }
+  run_fills(w.tab[7]);
+  const int totQ = pack_segs(qseg), totR = pack_segs(rseg);
+  const QuerySeg *qt = put_segs(w.tab[8], qseg);
+  const QuerySeg *rtab = put_segs(w.tab[9], rseg);
+  if (totQ) {
+    k_coinc1_all<<<totQ / BS, BS, 0, s_>>>((int)qseg.size(), qt);
+    k_coinc2_all<<<totQ / BS, BS, 0, s_>>>((int)qseg.size(), qt);
+  }
+  // comm4: clear negative iblanks, then mark live receptors
+  if (totN) k_clear_neg_all<<<totN / BS, BS, 0, s_>>>((int)nseg.size(), ntab);
+  IblankTables *pit = (IblankTables *)put_table(w.it, it);
+  if (totR) k_set_receptors_all<<<totR / BS, BS, 0, s_>>>((int)rseg.size(), rtab, pit);
   sync();
 }
 
