@@ -2263,6 +2263,7 @@ Engine::~Engine()
   amr_release();
   if (s_) (void)hipStreamDestroy(s_);
   if (s2_) (void)hipStreamDestroy(s2_);
+  if (wb_) (void)hipHostFree(wb_);
 }
 
 void Engine::sync() { TG_HIP(hipStreamSynchronize(s_)); }
@@ -3858,20 +3859,62 @@ void Engine::perform_connectivity()
   for (Block *b : blocks_) cell_iblanks(*b);
   sync();
   auto t6 = now();
-  // write back to the user's arrays
-  for (Block *bp : blocks_) {
-    Block &b = *bp;
-    b.uq = nullptr;  // solutions are registered again after connectivity (tioga.C:152-155)
-    if (!b.writeback) continue;
-    TG_HIP(hipMemcpyAsync(b.uiblank, b.iblank.p, sizeof(int) * b.nnodes, hipMemcpyDeviceToHost, s_));
-    int *ibc = b.uiblank_cell;
-    if (!ibc) {
-      b.own_iblank_cell.resize(std::max(b.ncells, 1));
-      ibc = b.own_iblank_cell.data();
+  // write back to the user's arrays: every block's iblank and iblank_cell
+  // by DMA into one page-locked staging buffer, then into the user's
+  // (pageable) arrays on host threads
+  {
+    struct Part {
+      int *dst;
+      size_t off, n;
+    };
+    std::vector<Part> parts;
+    size_t tot = 0;
+    for (Block *bp : blocks_) {
+      Block &b = *bp;
+      b.uq = nullptr;  // solutions are registered again after connectivity (tioga.C:152-155)
+      if (!b.writeback) continue;
+      int *ibc = b.uiblank_cell;
+      if (!ibc) {
+        b.own_iblank_cell.resize(std::max(b.ncells, 1));
+        ibc = b.own_iblank_cell.data();
+      }
+      parts.push_back(Part{b.uiblank, tot, (size_t)b.nnodes});
+      tot += b.nnodes;
+      parts.push_back(Part{ibc, tot, (size_t)b.ncells});
+      tot += b.ncells;
     }
-    TG_HIP(hipMemcpyAsync(ibc, b.iblank_cell.p, sizeof(int) * b.ncells, hipMemcpyDeviceToHost, s_));
+    if (tot > wb_cap_) {
+      if (wb_) TG_HIP(hipHostFree(wb_));
+      wb_ = nullptr;
+      wb_cap_ = 0;
+      TG_HIP(hipHostMalloc((void **)&wb_, sizeof(int) * tot, hipHostMallocDefault));
+      wb_cap_ = tot;
+    }
+    size_t k = 0;
+    for (Block *bp : blocks_) {
+      Block &b = *bp;
+      if (!b.writeback) continue;
+      if (b.nnodes)
+        TG_HIP(hipMemcpyAsync(wb_ + parts[k].off, b.iblank.p, sizeof(int) * b.nnodes, hipMemcpyDeviceToHost, s_));
+      if (b.ncells)
+        TG_HIP(hipMemcpyAsync(wb_ + parts[k + 1].off, b.iblank_cell.p, sizeof(int) * b.ncells,
+                              hipMemcpyDeviceToHost, s_));
+      k += 2;
+    }
+    sync();
+    // the parts split into about equal byte ranges, one per thread
+    const int nt = (int)std::max<size_t>(1, std::min<size_t>(8, tot / (1 << 20)));
+    auto copy_range = [&](size_t lo, size_t hi) {  // [lo, hi) of the staged words
+      for (const Part &q : parts) {
+        const size_t a = std::max(lo, q.off), e = std::min(hi, q.off + q.n);
+        if (a < e) memcpy(q.dst + (a - q.off), wb_ + a, sizeof(int) * (e - a));
+      }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(copy_range, tot * t / nt, tot * (t + 1) / nt);
+    copy_range(0, tot / nt);
+    for (auto &x : th) x.join();
   }
-  sync();
   auto t7 = now();
   t.holemap = ms(t0, t1);
   t.boxes = ms(t1, t2);
