@@ -79,6 +79,33 @@ __device__ __forceinline__ double rl(double v, int j)
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// a double moved across lanes by a DPP pattern (lanes the pattern does not
+// write read 0.0)
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_mov(double v)
+{
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffLL), CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// inclusive and exclusive prefix sums over the 64 lanes: row shifts by 1,
+// 2, 4, 8, then each row's total broadcast into the rows above (every
+// partial sum covers a contiguous run of lanes ending at the lane itself)
+__device__ __forceinline__ void wave_scan(double v, double &incl, double &excl)
+{
+  double x = v;
+  x += dpp_mov<0x111>(x);        // row_shr:1
+  x += dpp_mov<0x112>(x);        // row_shr:2
+  x += dpp_mov<0x114>(x);        // row_shr:4
+  x += dpp_mov<0x118>(x);        // row_shr:8
+  x += dpp_mov<0x142, 0xa>(x);   // row_bcast:15 into rows 1 and 3
+  x += dpp_mov<0x143, 0xc>(x);   // row_bcast:31 into rows 2 and 3
+  incl = x;
+  excl = dpp_mov<0x138>(x);      // wave_shr:1 (lane 0 reads 0)
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
   for (int o = OW / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, OW);
@@ -254,13 +281,8 @@ __device__ double chunk_walk(double s, double t, int cnt, int lane)
     const double r = (y + MAGIC) - MAGIC;
     bool pass = sok && mine && fabs(y) < LIM && fabs(y - r) != 0.5;
     const double T = pass ? r : 0.0;
-    double incl = T;
-    for (int o = 1; o < OW; o <<= 1) {
-      const double u = __shfl_up(incl, o, OW);
-      if (lane >= o) incl += u;
-    }
-    double ex = __shfl_up(incl, 1, OW);
-    if (lane == 0) ex = 0.0;
+    double incl, ex;
+    wave_scan(T, incl, ex);
     pass = pass && (A + ex + (r < 0 ? r : 0.0) >= LOW) && (A + ex + (r > 0 ? r : 0.0) <= HIGH);
     const unsigned long long fm = __ballot(mine && !pass);
     const int f = fm ? __ffsll((long long)fm) - 1 : cnt;
@@ -357,13 +379,8 @@ __global__ void __launch_bounds__(OW) k_obb_walk(int ns, int G, const ObbSetDev 
         N = -t;
       }
       const double T = pass ? P + N : 0.0;
-      double incl = T;
-      for (int o = 1; o < OW; o <<= 1) {
-        const double u = __shfl_up(incl, o, OW);
-        if (lane >= o) incl += u;
-      }
-      double ex = __shfl_up(incl, 1, OW);
-      if (lane == 0) ex = 0.0;
+      double incl, ex;
+      wave_scan(T, incl, ex);
       pass = pass && (A + ex + N >= LOW) && (A + ex + P <= HIGH);
       const unsigned long long fm = __ballot(mine && !pass);
       const int f = fm ? __ffsll((long long)fm) - 1 : m;
