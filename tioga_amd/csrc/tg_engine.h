@@ -483,6 +483,7 @@ class Engine {
   hipStream_t s2_ = nullptr;  // work overlapped with s_ (the batched dedupe)
   int *wb_ = nullptr;         // page-locked staging of the iblank write-back
   size_t wb_cap_ = 0;
+  std::vector<hipEvent_t> wb_ev_;  // one per write-back piece
   std::vector<Block *> blocks_;
   std::map<int, int> tag_to_blk_;
   std::vector<HoleMap> hm_;

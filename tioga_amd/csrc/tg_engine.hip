@@ -11,7 +11,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <atomic>
 #include <exception>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -2264,6 +2266,7 @@ Engine::~Engine()
   if (s_) (void)hipStreamDestroy(s_);
   if (s2_) (void)hipStreamDestroy(s2_);
   if (wb_) (void)hipHostFree(wb_);
+  for (hipEvent_t e : wb_ev_) (void)hipEventDestroy(e);
 }
 
 void Engine::sync() { TG_HIP(hipStreamSynchronize(s_)); }
@@ -3890,30 +3893,57 @@ void Engine::perform_connectivity()
       TG_HIP(hipHostMalloc((void **)&wb_, sizeof(int) * tot, hipHostMallocDefault));
       wb_cap_ = tot;
     }
-    size_t k = 0;
-    for (Block *bp : blocks_) {
-      Block &b = *bp;
-      if (!b.writeback) continue;
-      if (b.nnodes)
-        TG_HIP(hipMemcpyAsync(wb_ + parts[k].off, b.iblank.p, sizeof(int) * b.nnodes, hipMemcpyDeviceToHost, s_));
-      if (b.ncells)
-        TG_HIP(hipMemcpyAsync(wb_ + parts[k + 1].off, b.iblank_cell.p, sizeof(int) * b.ncells,
-                              hipMemcpyDeviceToHost, s_));
-      k += 2;
+    // the DMA in pieces of at most 1 M words, an event after each, so the
+    // host copies of a piece start as soon as it has landed
+    struct Piece {
+      int *dst;
+      const int *src;
+      size_t off, n;
+    };
+    std::vector<Piece> pieces;
+    {
+      size_t k = 0;
+      for (Block *bp : blocks_) {
+        Block &b = *bp;
+        if (!b.writeback) continue;
+        const int *srcs[2] = {b.iblank.p, b.iblank_cell.p};
+        for (int h = 0; h < 2; h++, k++)
+          for (size_t a = 0; a < parts[k].n; a += (1 << 20))
+            pieces.push_back(Piece{parts[k].dst + a, srcs[h] + a, parts[k].off + a,
+                                   std::min<size_t>(1 << 20, parts[k].n - a)});
+      }
     }
-    sync();
-    // the parts split into about equal byte ranges, one per thread
-    const int nt = (int)std::max<size_t>(1, std::min<size_t>(8, tot / (1 << 20)));
-    auto copy_range = [&](size_t lo, size_t hi) {  // [lo, hi) of the staged words
-      for (const Part &q : parts) {
-        const size_t a = std::max(lo, q.off), e = std::min(hi, q.off + q.n);
-        if (a < e) memcpy(q.dst + (a - q.off), wb_ + a, sizeof(int) * (e - a));
+    while (wb_ev_.size() < pieces.size()) {
+      hipEvent_t e;
+      TG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      wb_ev_.push_back(e);
+    }
+    for (size_t k = 0; k < pieces.size(); k++) {
+      TG_HIP(hipMemcpyAsync(wb_ + pieces[k].off, pieces[k].src, sizeof(int) * pieces[k].n, hipMemcpyDeviceToHost,
+                            s_));
+      TG_HIP(hipEventRecord(wb_ev_[k], s_));
+    }
+    const int nt = (int)std::max<size_t>(1, std::min<size_t>(8, pieces.size()));
+    std::atomic<size_t> next{0};
+    std::exception_ptr cerr;
+    std::mutex cmu;
+    auto copier = [&] {
+      for (size_t k; (k = next.fetch_add(1)) < pieces.size();) {
+        const hipError_t e = hipEventSynchronize(wb_ev_[k]);
+        if (e != hipSuccess) {
+          std::lock_guard<std::mutex> g(cmu);
+          if (!cerr) cerr = std::make_exception_ptr(std::runtime_error("tioga_amd: iblank write-back failed"));
+          continue;
+        }
+        memcpy(pieces[k].dst, wb_ + pieces[k].off, sizeof(int) * pieces[k].n);
       }
     };
     std::vector<std::thread> th;
-    for (int t = 1; t < nt; t++) th.emplace_back(copy_range, tot * t / nt, tot * (t + 1) / nt);
-    copy_range(0, tot / nt);
+    for (int t = 1; t < nt; t++) th.emplace_back(copier);
+    copier();
     for (auto &x : th) x.join();
+    sync();
+    if (cerr) std::rethrow_exception(cerr);
   }
   auto t7 = now();
   t.holemap = ms(t0, t1);
