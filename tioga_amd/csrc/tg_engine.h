@@ -282,7 +282,7 @@ struct DonorScratch {
 // Reusable device scratch of the other connectivity phases: hole maps, box
 // exchange, query gathering.
 struct StepScratch {
-  DBuf<unsigned char> maps, walls, boxes, qpairs;
+  DBuf<unsigned char> maps, walls, boxes, qpairs, qfill, cib;
   DBuf<int> act, chg, cand0, rng0, first, pblk, qflag, qpos, qsel;
   DBuf<int2> rng;
   DBuf<OBB> obc;

@@ -778,26 +778,6 @@ __global__ void k_pair_runs(int npairs, int nsel, const int *__restrict__ sel, c
   first[i] = lo;
 }
 
-// exchangeSearchData.C:200-255: the receiver's search arrays for one packet
-__global__ void k_fill_search(int n, const int *__restrict__ sel, const int *__restrict__ invmap,
-                              const double *__restrict__ xs, const double *__restrict__ nodeRes, int off,
-                              int rblk, int rtag, double *__restrict__ xsearch, double *__restrict__ res_search,
-                              int *__restrict__ isearch, int *__restrict__ tagsearch)
-{
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  int nd = invmap[sel[j]];
-  size_t o = off + (size_t)j;
-  isearch[3 * o] = 0;
-  isearch[3 * o + 1] = nd;
-  isearch[3 * o + 2] = rblk;
-  tagsearch[o] = rtag;
-  xsearch[3 * o] = xs[3 * (size_t)nd];
-  xsearch[3 * o + 1] = xs[3 * (size_t)nd + 1];
-  xsearch[3 * o + 2] = xs[3 * (size_t)nd + 2];
-  res_search[o] = nodeRes[nd];
-}
-
 // ---------------------------------------------------------------------------
 // search: MeshBlock::search (search.C:107-265)
 // ---------------------------------------------------------------------------
@@ -1762,6 +1742,49 @@ __device__ __forceinline__ const S &seg_of(int ns, const S *__restrict__ T)
   return T[lo];
 }
 
+// exchangeSearchData.C:200-255: the receivers' search arrays, every block
+// pair's packet in one launch
+struct QFillSeg {
+  int off, n;
+  const int *sel, *invmap;
+  const double *xs, *nodeRes;
+  int qoff, rblk, rtag;
+  double *xsearch, *res_search;
+  int *isearch, *tagsearch;
+};
+__global__ void k_fill_search_all(int ns, const QFillSeg *__restrict__ T)
+{
+  const QFillSeg &S = seg_of(ns, T);
+  const int j = blockIdx.x * BS + threadIdx.x - S.off;
+  if (j >= S.n) return;
+  int nd = S.invmap[S.sel[j]];
+  size_t o = S.qoff + (size_t)j;
+  S.isearch[3 * o] = 0;
+  S.isearch[3 * o + 1] = nd;
+  S.isearch[3 * o + 2] = S.rblk;
+  S.tagsearch[o] = S.rtag;
+  S.xsearch[3 * o] = S.xs[3 * (size_t)nd];
+  S.xsearch[3 * o + 1] = S.xs[3 * (size_t)nd + 1];
+  S.xsearch[3 * o + 2] = S.xs[3 * (size_t)nd + 2];
+  S.res_search[o] = S.nodeRes[nd];
+}
+
+
+namespace {
+template <class S>
+int pack_segs(std::vector<S> &v)
+{
+  long long o = 0;
+  for (S &x : v) {
+    x.off = (int)o;
+    o += (long long)(x.n + BS - 1) / BS * BS;
+  }
+  if (o > 0x7fffffffLL) throw std::runtime_error("tioga_amd: donor exchange stage too large for one launch");
+  return (int)o;
+}
+}  // namespace
+
+
 // fills of many arrays: chunk k sets p[start, start + len) to v (width 1, 4
 // or 8 bytes)
 struct FillChunk {
@@ -2055,11 +2078,9 @@ __global__ void k_rf_clear(int n, const int *__restrict__ ibr, int *__restrict__
 }
 
 // MeshBlock::getCellIblanks (highOrder.C:88-150)
-__global__ void k_cell_iblank(int ncells, TypeInfo ti, const int *__restrict__ conn, const int *__restrict__ iblank,
-                              int *__restrict__ ibc)
+__device__ __forceinline__ void cell_iblank_one(int c, const TypeInfo &ti, const int *__restrict__ conn,
+                                                const int *__restrict__ iblank, int *__restrict__ ibc)
 {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
   int nv = ti.nv[type_of(ti, c)];
   const int *cv = conn + 8 * (size_t)c;
   int out = 1, cnt = 0;
@@ -2073,6 +2094,29 @@ __global__ void k_cell_iblank(int ncells, TypeInfo ti, const int *__restrict__ c
   }
   if (out && cnt == nv) out = -1;
   ibc[c] = out;
+}
+
+__global__ void k_cell_iblank(int ncells, TypeInfo ti, const int *__restrict__ conn, const int *__restrict__ iblank,
+                              int *__restrict__ ibc)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  cell_iblank_one(c, ti, conn, iblank, ibc);
+}
+
+// k_cell_iblank for every block
+struct CellIbSeg {
+  int off, n;
+  TypeInfo ti;
+  const int *conn, *iblank;
+  int *ibc;
+};
+__global__ void k_cell_iblank_all(int ns, const CellIbSeg *__restrict__ T)
+{
+  const CellIbSeg &S = seg_of(ns, T);
+  const int c = blockIdx.x * BS + threadIdx.x - S.off;
+  if (c >= S.n) return;
+  cell_iblank_one(c, S.ti, S.conn, S.iblank, S.ibc);
 }
 
 // ---------------------------------------------------------------------------
@@ -2913,6 +2957,14 @@ void Engine::exchange_boxes()
   }
 }
 
+template <class S>
+const S *Engine::put_segs(DBuf<unsigned char> &b, const std::vector<S> &v)
+{
+  b.reserve(sizeof(S) * std::max<size_t>(v.size(), 1));
+  if (!v.empty()) TG_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(S) * v.size(), hipMemcpyHostToDevice, s_));
+  return (const S *)b.p;
+}
+
 // tioga::exchangeSearchData (exchangeSearchData.C:27-263), single rank
 void Engine::exchange_search_data()
 {
@@ -3081,18 +3133,22 @@ void Engine::exchange_search_data()
     b.isearch.reserve(3 * std::max(n, (size_t)1));
     b.donorId.reserve(std::max(n, (size_t)1));
   }
+  // every pair's query points in one launch (a segment per pair)
   std::vector<int> off(blocks_.size(), 0);
+  std::vector<QFillSeg> qs;
   for (int i = 0; i < np; i++) {
     int ii = dest[i];
     if (ii < 0 || cnt[i] == 0) continue;
     Block &A = *blocks_[pairs_[i].local];
     int rb = pairs_[ii].local;
     Block &R = *blocks_[rb];
-    k_fill_search<<<nb(cnt[i]), BS, 0, s_>>>(cnt[i], qlist_.p + lst0[i], A.invmap.p, A.x.p, A.nodeRes.p, off[rb],
-                                             pairs_[ii].iblk_remote, pairs_[ii].tag_remote, R.xsearch.p,
-                                             R.res_search.p, R.isearch.p, R.tagsearch.p);
+    qs.push_back(QFillSeg{0, cnt[i], qlist_.p + lst0[i], A.invmap.p, A.x.p, A.nodeRes.p, off[rb],
+                          pairs_[ii].iblk_remote, pairs_[ii].tag_remote, R.xsearch.p, R.res_search.p, R.isearch.p,
+                          R.tagsearch.p});
     off[rb] += cnt[i];
   }
+  const int totq = pack_segs(qs);
+  if (totq) k_fill_search_all<<<totq / BS, BS, 0, s_>>>((int)qs.size(), put_segs(stw_.qfill, qs));
   sync();
 }
 
@@ -3534,28 +3590,6 @@ void Engine::launch_searches()
 
 // tioga::exchangeDonors (exchangeDonors.C:25-301), single rank.  Every
 // stage is one launch over all blocks (per-block segment tables).
-namespace {
-template <class S>
-int pack_segs(std::vector<S> &v)
-{
-  long long o = 0;
-  for (S &x : v) {
-    x.off = (int)o;
-    o += (long long)(x.n + BS - 1) / BS * BS;
-  }
-  if (o > 0x7fffffffLL) throw std::runtime_error("tioga_amd: donor exchange stage too large for one launch");
-  return (int)o;
-}
-}  // namespace
-
-template <class S>
-const S *Engine::put_segs(DBuf<unsigned char> &b, const std::vector<S> &v)
-{
-  b.reserve(sizeof(S) * std::max<size_t>(v.size(), 1));
-  if (!v.empty()) TG_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(S) * v.size(), hipMemcpyHostToDevice, s_));
-  return (const S *)b.p;
-}
-
 void Engine::exchange_donors()
 {
   const int nbk = (int)blocks_.size();
@@ -3859,7 +3893,17 @@ void Engine::perform_connectivity()
   }
   auto t5 = now();
   exchange_donors();
-  for (Block *b : blocks_) cell_iblanks(*b);
+  {
+    // every block's cell iblanks in one launch
+    std::vector<CellIbSeg> cs;
+    for (Block *bp : blocks_) {
+      Block &b = *bp;
+      b.iblank_cell.reserve(b.ncells);
+      if (b.ncells) cs.push_back(CellIbSeg{0, b.ncells, b.ti, b.conn.p, b.iblank.p, b.iblank_cell.p});
+    }
+    const int totc = pack_segs(cs);
+    if (totc) k_cell_iblank_all<<<totc / BS, BS, 0, s_>>>((int)cs.size(), put_segs(stw_.cib, cs));
+  }
   sync();
   auto t6 = now();
   // write back to the user's arrays: every block's iblank and iblank_cell
