@@ -881,11 +881,19 @@ __global__ void __launch_bounds__(BT) k_big_box_key(const int2 *__restrict__ tab
       m[j] = j < 3 ? min(m[j], o) : max(m[j], o);
     }
   }
+  // the waves' results folded in LDS: one atomic per key and workgroup (the
+  // keys of a long segment's first levels are shared by all its workgroups)
+  __shared__ unsigned long long wk[BT / 64][6];
   if ((threadIdx.x & 63) == 0)
-    for (int j = 0; j < 6; j++) {
-      if (j < 3) atomicMin(&bk[6 * tc.x + j], m[j]);
-      else atomicMax(&bk[6 * tc.x + j], m[j]);
-    }
+    for (int j = 0; j < 6; j++) wk[threadIdx.x >> 6][j] = m[j];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int j = threadIdx.x;
+    unsigned long long v = wk[0][j];
+    for (int q = 1; q < BT / 64; q++) v = j < 3 ? min(v, wk[q][j]) : max(v, wk[q][j]);
+    if (j < 3) atomicMin(&bk[6 * tc.x + j], v);
+    else atomicMax(&bk[6 * tc.x + j], v);
+  }
 }
 
 __global__ void __launch_bounds__(BT) k_big_box_pos(const int2 *__restrict__ tab, const Seg *__restrict__ segs,
