@@ -1,6 +1,6 @@
 """Device stages against the oracle at sizes the golden cases do not reach.
 
-The ADT builder splits segments longer than 65536 elements across many
+The ADT builder splits segments longer than 16384 elements across many
 workgroups and builds several trees in one level loop; the test runs it
 through the C ABI on trees of mixed sizes (long, medium, short, single,
 empty) whose boxes have heavy ties along every cut dimension and signed

@@ -41,7 +41,7 @@ constexpr int BT = 512;        // threads per workgroup for long segments
 constexpr int PER = 8;         // consecutive positions per thread per chunk
 constexpr int CH = BT * PER;   // positions per chunk of a multi-workgroup round
 constexpr int WAVEMAX = 2048;  // segments at most this long (and above SUB): one wave
-constexpr int BIGMIN = 1 << 16;  // segments longer than this: many workgroups
+constexpr int BIGMIN = 1 << 14;  // segments longer than this: many workgroups
 constexpr int FINISH = 16384;  // active range a single workgroup finishes a long segment's rounds from
 
 // base: offset of the segment's tree in the concatenated element arrays;
