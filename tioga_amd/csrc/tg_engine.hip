@@ -329,6 +329,44 @@ __global__ void k_hole_final(size_t n, unsigned char *__restrict__ sam)
 // column sweeps (the k_line_fill rule) until nothing changes, and stores it.
 constexpr int SLICE_MAX = 48 * 1024;
 
+// One sweep step of the slice fill over m <= FK cells from sl[i0] in
+// direction DIR (cell k at i0 + DIR * k * step, step 1 along a row, nu along
+// a column); cross neighbours at -+ xs; prev is the cell before the first.
+// A cell of value 0 next to a 1 (the previous cell in the sweep or a cross
+// neighbour) becomes 1.  Returns the last cell's value.
+constexpr int FK = 16;
+template <int DIR>
+__device__ __forceinline__ unsigned char fill_chunk(unsigned char *sl, int i0, int m, int xs, unsigned char prev,
+                                                    int &ch, int step = 1)
+{
+  unsigned char c[FK], a[FK], b[FK];
+#pragma unroll
+  for (int k = 0; k < FK; k++)
+    if (k < m) {
+      const int i = i0 + DIR * k * step;
+      c[k] = sl[i];
+      a[k] = sl[i - xs];
+      b[k] = sl[i + xs];
+    }
+  unsigned set = 0;
+#pragma unroll
+  for (int k = 0; k < FK; k++)
+    if (k < m) {
+      if (c[k] == 0 && (prev == 1 || a[k] == 1 || b[k] == 1)) {
+        c[k] = 1;
+        set |= 1u << k;
+      }
+      prev = c[k];
+    }
+  if (set) {
+    ch = 1;
+#pragma unroll
+    for (int k = 0; k < FK; k++)
+      if ((set >> k) & 1u) sl[i0 + DIR * k * step] = 1;
+  }
+  return prev;
+}
+
 __global__ void __launch_bounds__(256) k_fill_slices(int axis, int nmaps, unsigned char *const *__restrict__ sams,
                                                      const int *__restrict__ nxs)
 {
@@ -352,56 +390,29 @@ __global__ void __launch_bounds__(256) k_fill_slices(int axis, int nmaps, unsign
     if (t == 0) chg = 0;
     __syncthreads();
     int ch = 0;
-    // rows: along u, interior rows v = 1..nv-2, cross neighbours v +- 1
+    // rows: along u, interior rows v = 1..nv-2, cross neighbours v +- 1.
+    // Each row in register chunks of FK cells: loads first, the sequential
+    // update in registers, then the stores (so the chain never waits on an
+    // LDS store; cells of the neighbouring rows may be read before or after
+    // their owners update them, which the repeat-until-no-change loop allows)
     for (int v = 1 + t; v < nv - 1; v += blockDim.x) {
-      unsigned char prev = sl[v * nu];
-      for (int u = 1; u < nu - 1; u++) {
-        const int i = v * nu + u;
-        unsigned char c = sl[i];
-        if (c == 0 && (prev == 1 || sl[i - nu] == 1 || sl[i + nu] == 1)) {
-          c = 1;
-          sl[i] = 1;
-          ch = 1;
-        }
-        prev = c;
-      }
-      prev = sl[v * nu + nu - 1];
-      for (int u = nu - 2; u >= 1; u--) {
-        const int i = v * nu + u;
-        unsigned char c = sl[i];
-        if (c == 0 && (prev == 1 || sl[i - nu] == 1 || sl[i + nu] == 1)) {
-          c = 1;
-          sl[i] = 1;
-          ch = 1;
-        }
-        prev = c;
-      }
+      const int r0 = v * nu;
+      unsigned char prev = sl[r0];
+      for (int u0 = 1; u0 < nu - 1; u0 += FK)
+        prev = fill_chunk<+1>(sl, r0 + u0, min(FK, nu - 1 - u0), nu, prev, ch);
+      prev = sl[r0 + nu - 1];
+      for (int u0 = nu - 2; u0 >= 1; u0 -= FK)
+        prev = fill_chunk<-1>(sl, r0 + u0, min(FK, u0), nu, prev, ch);
     }
     __syncthreads();
     // columns: along v, interior columns u = 1..nu-2, cross neighbours u +- 1
     for (int u = 1 + t; u < nu - 1; u += blockDim.x) {
       unsigned char prev = sl[u];
-      for (int v = 1; v < nv - 1; v++) {
-        const int i = v * nu + u;
-        unsigned char c = sl[i];
-        if (c == 0 && (prev == 1 || sl[i - 1] == 1 || sl[i + 1] == 1)) {
-          c = 1;
-          sl[i] = 1;
-          ch = 1;
-        }
-        prev = c;
-      }
+      for (int v0 = 1; v0 < nv - 1; v0 += FK)
+        prev = fill_chunk<+1>(sl, v0 * nu + u, min(FK, nv - 1 - v0), 1, prev, ch, nu);
       prev = sl[(nv - 1) * nu + u];
-      for (int v = nv - 2; v >= 1; v--) {
-        const int i = v * nu + u;
-        unsigned char c = sl[i];
-        if (c == 0 && (prev == 1 || sl[i - 1] == 1 || sl[i + 1] == 1)) {
-          c = 1;
-          sl[i] = 1;
-          ch = 1;
-        }
-        prev = c;
-      }
+      for (int v0 = nv - 2; v0 >= 1; v0 -= FK)
+        prev = fill_chunk<-1>(sl, v0 * nu + u, min(FK, v0), 1, prev, ch, nu);
     }
     if (ch) atomicOr(&chg, 1);
     __syncthreads();
