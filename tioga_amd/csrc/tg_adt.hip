@@ -428,7 +428,7 @@ __device__ void wave_box(const int *ix, int nav, const double *eb, double *r)
   }
 }
 
-constexpr int WAVES = 4;  // waves per workgroup of the wave-level kernels
+constexpr int WAVES = 1;  // waves per workgroup of the wave-level kernels (more workgroups fit a CU's LDS)
 
 // Segments of SUB < nav <= WAVEMAX elements, one level: a wave each, in
 // global memory.
