@@ -370,7 +370,7 @@ __device__ __forceinline__ unsigned char fill_chunk(unsigned char *sl, int i0, i
 __global__ void __launch_bounds__(256) k_fill_slices(int axis, int nmaps, unsigned char *const *__restrict__ sams,
                                                      const int *__restrict__ nxs)
 {
-  __shared__ unsigned char sl[SLICE_MAX];
+  extern __shared__ unsigned char sl[];  // the slice (launch-sized, at most SLICE_MAX bytes)
   __shared__ int chg;
   const int m = blockIdx.y;
   const int n[3] = {nxs[3 * m], nxs[3 * m + 1], nxs[3 * m + 2]};
@@ -2751,6 +2751,7 @@ void Engine::get_hole_map()
   const int saxis = isym == 3 ? 2 : (isym == 1 ? 0 : (isym == 2 ? 1 : -1));
   bool slices = saxis >= 0;
   int maxcells = 0, maxlines[3] = {0, 0, 0}, maxslices = 0;
+  size_t maxslice = 1;  // bytes of the largest slice
   if (!live.empty()) {
     ms.nmap = (int)live.size();
     for (int k = 0; k < ms.nmap; k++) {
@@ -2765,7 +2766,9 @@ void Engine::get_hole_map()
         if (h.nx[a1] > 2 && h.nx[a2] > 2) maxlines[a] = std::max(maxlines[a], (h.nx[a1] - 2) * (h.nx[a2] - 2));
       }
       if (saxis >= 0) {
-        slices = slices && (long long)h.nx[(saxis + 1) % 3] * h.nx[(saxis + 2) % 3] <= SLICE_MAX;
+        const long long sb = (long long)h.nx[(saxis + 1) % 3] * h.nx[(saxis + 2) % 3];
+        slices = slices && sb <= SLICE_MAX;
+        maxslice = std::max<size_t>(maxslice, (size_t)std::min<long long>(sb, SLICE_MAX));
         maxslices = std::max(maxslices, h.nx[saxis] - 2);
       }
     }
@@ -2816,7 +2819,7 @@ void Engine::get_hole_map()
     chg.reserve(nm);
     std::vector<int> hact(nm, 1), hchg(nm);
     if (slices && maxslices > 0)
-      k_fill_slices<<<dim3(maxslices, nm), 256, 0, s_>>>(saxis, nm, &pms->sam[0], &pms->nx[0][0]);
+      k_fill_slices<<<dim3(maxslices, nm), 256, maxslice, s_>>>(saxis, nm, &pms->sam[0], &pms->nx[0][0]);
     for (; !slices;) {
       TG_HIP(hipMemcpyAsync(act.p, hact.data(), sizeof(int) * nm, hipMemcpyHostToDevice, s_));
       fill<int>(chg.p, nm, 0, s_);
@@ -2843,8 +2846,9 @@ void Engine::get_hole_map()
     if (slices) {
       int k = 0;
       while (live[k] != i) k++;
+      const size_t sb = (size_t)h.nx[(saxis + 1) % 3] * h.nx[(saxis + 2) % 3];  // <= SLICE_MAX here
       if (h.nx[saxis] > 2)
-        k_fill_slices<<<dim3(h.nx[saxis] - 2, 1), 256, 0, s_>>>(saxis, 1, &pms->sam[k], &pms->nx[k][0]);
+        k_fill_slices<<<dim3(h.nx[saxis] - 2, 1), 256, sb, s_>>>(saxis, 1, &pms->sam[k], &pms->nx[k][0]);
     }
     for (int round = 0; !slices; round++) {
       int hc = 0;
