@@ -484,6 +484,9 @@ class Engine {
   int *wb_ = nullptr;         // page-locked staging of the iblank write-back
   size_t wb_cap_ = 0;
   std::vector<hipEvent_t> wb_ev_;  // one per write-back piece
+  DBuf<signed char> wb8_;          // the write-back narrowed to bytes (device)
+  DBuf<int> wbflag_;               // set when a value does not fit a byte
+  DBuf<unsigned char> wbtab_;      // the pieces' table
   std::vector<Block *> blocks_;
   std::map<int, int> tag_to_blk_;
   std::vector<HoleMap> hm_;

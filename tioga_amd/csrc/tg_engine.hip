@@ -1804,6 +1804,23 @@ struct FillChunk {
   long long start;
   int len, width;
 };
+// iblank words narrowed to bytes for the write-back (piece k: n words from
+// src to out[off ...]); *wide is set when a value does not fit a byte
+struct PackSeg {
+  int off, n;
+  const int *src;
+  size_t dst;
+};
+__global__ void k_pack_i8(int ns, const PackSeg *__restrict__ T, signed char *__restrict__ out, int *__restrict__ wide)
+{
+  const PackSeg &S = seg_of(ns, T);
+  const int i = blockIdx.x * BS + threadIdx.x - S.off;
+  if (i >= S.n) return;
+  const int v = S.src[i];
+  out[S.dst + i] = (signed char)v;
+  if (v < -128 || v > 127) atomicOr(wide, 1);
+}
+
 __global__ void k_fill_chunks(const FillChunk *__restrict__ C)
 {
   const FillChunk c = C[blockIdx.x];
@@ -3977,9 +3994,30 @@ void Engine::perform_connectivity()
       TG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       wb_ev_.push_back(e);
     }
+    // iblank values are 1, 0, -1 or minus a mesh tag: when every one fits a
+    // byte (checked on the device), a quarter of the bytes cross the bus and
+    // the host widens them; otherwise the words go as they are
+    bool narrow = false;
+    if (tot) {
+      std::vector<PackSeg> ps;
+      for (const Piece &q : pieces) ps.push_back(PackSeg{0, (int)q.n, q.src, q.off});
+      const int totp = pack_segs(ps);
+      wb8_.reserve(tot);
+      wbflag_.reserve(1);
+      fill<int>(wbflag_.p, 1, 0, s_);
+      k_pack_i8<<<totp / BS, BS, 0, s_>>>((int)ps.size(), put_segs(wbtab_, ps), wb8_.p, wbflag_.p);
+      int wide = 1;
+      TG_HIP(hipMemcpyAsync(&wide, wbflag_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
+      sync();
+      narrow = (wide == 0);
+    }
+    signed char *wb8h = (signed char *)wb_;  // the narrowed words, front of the staging buffer
     for (size_t k = 0; k < pieces.size(); k++) {
-      TG_HIP(hipMemcpyAsync(wb_ + pieces[k].off, pieces[k].src, sizeof(int) * pieces[k].n, hipMemcpyDeviceToHost,
-                            s_));
+      if (narrow)
+        TG_HIP(hipMemcpyAsync(wb8h + pieces[k].off, wb8_.p + pieces[k].off, pieces[k].n, hipMemcpyDeviceToHost, s_));
+      else
+        TG_HIP(hipMemcpyAsync(wb_ + pieces[k].off, pieces[k].src, sizeof(int) * pieces[k].n, hipMemcpyDeviceToHost,
+                              s_));
       TG_HIP(hipEventRecord(wb_ev_[k], s_));
     }
     const int nt = (int)std::max<size_t>(1, std::min<size_t>(8, pieces.size()));
@@ -3994,7 +4032,13 @@ void Engine::perform_connectivity()
           if (!cerr) cerr = std::make_exception_ptr(std::runtime_error("tioga_amd: iblank write-back failed"));
           continue;
         }
-        memcpy(pieces[k].dst, wb_ + pieces[k].off, sizeof(int) * pieces[k].n);
+        if (narrow) {
+          const signed char *src = wb8h + pieces[k].off;
+          int *dst = pieces[k].dst;
+          for (size_t j = 0; j < pieces[k].n; j++) dst[j] = src[j];
+        } else {
+          memcpy(pieces[k].dst, wb_ + pieces[k].off, sizeof(int) * pieces[k].n);
+        }
       }
     };
     std::vector<std::thread> th;
