@@ -97,6 +97,10 @@ def build(name: str) -> Case:
         return Case(name, blocks, nvar=2, moves=_moves(blocks, 2, seed=9))
     if name == "three":
         return Case(name, three_blocks(), nvar=3)
+    if name == "bigtag":  # a mesh tag above 127: receptor iblanks of -200
+        blocks = W.tiny()
+        blocks[1].tag = 200
+        return Case(name, blocks, nvar=2)
     if name == "config1":
         return Case(name, W.config1(), nvar=5)
     # reduce_fringes (tioga.C:737-885) after the connectivity: fringe depths
@@ -123,7 +127,7 @@ def build(name: str) -> Case:
     raise KeyError(name)
 
 
-SMALL_CASES = ["tiny", "tiny_isym0", "tiny_isym1", "tiny_isym2", "mixed", "userres", "moving", "three"]
+SMALL_CASES = ["tiny", "tiny_isym0", "tiny_isym1", "tiny_isym2", "mixed", "userres", "moving", "three", "bigtag"]
 REDUCE_CASES = ["tiny_rf", "isym0_rf", "mixed_rf", "three_rf", "moving_rf", "nested_rf", "nested3_rf"]
 ALL_CASES = SMALL_CASES + ["config1"] + REDUCE_CASES
 
