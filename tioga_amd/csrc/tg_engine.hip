@@ -1339,14 +1339,27 @@ __global__ void __launch_bounds__(64) k_collect_more(SearchSet T, CandOut O)
 // the outcome goes to the candidate's own slot, so the binning order is free.
 __device__ __forceinline__ int type_bucket(int nv) { return nv == 4 ? 0 : (nv == 5 ? 1 : (nv == 6 ? 2 : 3)); }
 
+// per wave: how many of its lanes hold bucket q (lanes past n hold none),
+// and each lane's rank among them; one LDS atomic per bucket and wave
+__device__ __forceinline__ int bin_of(const CandOut &O, int j, int n)
+{
+  return j < n ? type_bucket((unsigned)O.cell[j] >> 28) : -1;
+}
+
 __global__ void __launch_bounds__(256) k_bin_count(CandOut O, int *__restrict__ bcount)
 {
   __shared__ int loc[4];
   if (threadIdx.x < 4) loc[threadIdx.x] = 0;
   __syncthreads();
   const int n = min(*O.used, O.cap);
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
-    atomicAdd(&loc[type_bucket((unsigned)O.cell[j] >> 28)], 1);
+  const int lane = threadIdx.x & 63;
+  for (int j0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); j0 < n; j0 += gridDim.x * blockDim.x) {
+    const int b = bin_of(O, j0 + lane, n);
+    for (int q = 0; q < 4; q++) {
+      const unsigned long long m = __ballot(b == q);
+      if (lane == 0 && m) atomicAdd(&loc[q], __popcll(m));
+    }
+  }
   __syncthreads();
   if (threadIdx.x < 4 && loc[threadIdx.x]) atomicAdd(&bcount[threadIdx.x], loc[threadIdx.x]);
 }
@@ -1359,9 +1372,17 @@ __global__ void __launch_bounds__(256) k_bin_fill(CandOut O, int *__restrict__ b
   __syncthreads();
   const int n = min(*O.used, O.cap);
   const int stride = gridDim.x * blockDim.x;
-  const int start = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = (1ull << lane) - 1;
+  const int start0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63);
   // pass 1: this block's counts; pass 2: its slice of each bucket
-  for (int j = start; j < n; j += stride) atomicAdd(&loc[type_bucket((unsigned)O.cell[j] >> 28)], 1);
+  for (int j0 = start0; j0 < n; j0 += stride) {
+    const int b = bin_of(O, j0 + lane, n);
+    for (int q = 0; q < 4; q++) {
+      const unsigned long long m = __ballot(b == q);
+      if (lane == 0 && m) atomicAdd(&loc[q], __popcll(m));
+    }
+  }
   __syncthreads();
   if (threadIdx.x < 4) {
     int bstart = 0;
@@ -1370,9 +1391,16 @@ __global__ void __launch_bounds__(256) k_bin_fill(CandOut O, int *__restrict__ b
     loc[threadIdx.x] = 0;
   }
   __syncthreads();
-  for (int j = start; j < n; j += stride) {
-    const int b = type_bucket((unsigned)O.cell[j] >> 28);
-    blist[base[b] + atomicAdd(&loc[b], 1)] = j;
+  for (int j0 = start0; j0 < n; j0 += stride) {
+    const int b = bin_of(O, j0 + lane, n);
+    for (int q = 0; q < 4; q++) {
+      const unsigned long long m = __ballot(b == q);
+      if (!m) continue;
+      int w = 0;
+      if (lane == 0) w = atomicAdd(&loc[q], __popcll(m));
+      w = __shfl(w, 0);
+      if (b == q) blist[base[q] + w + __popcll(m & lt)] = j0 + lane;
+    }
   }
 }
 
