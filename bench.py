@@ -143,34 +143,59 @@ def amr_gather_leg(shell, nvar: int = 8, updates: int = 1000):
             "traffic": profiled_traffic("amr_gather_config5.md", "traffic_bytes_per_update")}
 
 
-def cpu_baseline(blocks, sets, budget_s: float):
-    """The oracle's sequential donor search (tioga_oracle.c, restating
-    searchADTrecursion.C/checkContainment.C) on a bounded random sample of
-    each query set, 1 core; the search loop alone is timed."""
-    from oracle.oracle import Oracle, ptr
+def host_cpu():
+    """(search threads: the CPUs this process may run on, at most 16; those
+    CPUs; the CPU model name)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return max(1, min(16, n)), n, model
 
+
+def cpu_baseline(blocks, sets, gpu_donors, budget_s: float):
+    """The oracle's donor search (tioga_oracle.c, restating
+    search.C:107-233, searchADTrecursion.C:26-101 and checkContainment.C:27-96)
+    on a bounded random sample of each query set, searched against the tree
+    the GPU searches (the cells overlapping the OBB of the whole set), once
+    on 1 core and once on all the cores this process may use (POSIX threads
+    over contiguous chunks); the search loops alone are timed.  The sampled
+    donors are compared with the GPU's donors of the timed passes."""
+    from oracle.oracle import Oracle
+
+    nthr, nproc, model = host_cpu()
     o = Oracle(blocks)
     o.profile()
     rng = np.random.default_rng(0)
-    tot_q, tot_t = 0, 0.0
-    # calibrate on a small sample, then size the sample to the time budget
-    for btag, xq in sets:
+    tot_q, t1, tall, mism = 0, 0.0, 0.0, 0
+    for (btag, xq), gd in zip(sets, gpu_donors):
+        # calibrate the per-query cost on a small sample, then size the sample
         n0 = min(20000, xq.shape[0])
-        idx = rng.choice(xq.shape[0], n0, replace=False)
-        xs = np.ascontiguousarray(xq[idx])
-        d = np.zeros(n0, np.int32)
-        t = o.L.orc_search_points(o.h, btag, ptr(xs), n0, ptr(d))
+        idx = np.sort(rng.choice(xq.shape[0], n0, replace=False))
+        _, _, t = o.search_points_obb(btag, xq, xq[idx], 1)
         per = t / n0
         n = int(min(xq.shape[0], max(n0, budget_s / len(sets) / max(per, 1e-9))))
-        idx = rng.choice(xq.shape[0], n, replace=False)
+        idx = np.sort(rng.choice(xq.shape[0], n, replace=False))
         xs = np.ascontiguousarray(xq[idx])
-        d = np.zeros(n, np.int32)
-        t = o.L.orc_search_points(o.h, btag, ptr(xs), n, ptr(d))
+        d1, _, ts = o.search_points_obb(btag, xq, xs, 1)
+        dn, _, tp = o.search_points_obb(btag, xq, xs, nthr)
+        mism += int((d1 != gd[idx]).sum()) + int((dn != d1).sum())
         tot_q += n
-        tot_t += t
-        log(f"cpu baseline: {n} queries vs block {btag} in {t:.2f} s")
+        t1 += ts
+        tall += tp
+        log(f"cpu baseline: {n} queries vs block {btag}: 1 core {ts:.2f} s, {nthr} threads {tp:.2f} s")
     o.close()
-    return tot_q / tot_t, tot_q, tot_t
+    return {"one_core": tot_q / t1, "all_cores": tot_q / tall, "threads": nthr, "nproc": nproc, "cpu_model": model,
+            "sampled": tot_q, "seconds_one_core": t1, "seconds_all_cores": tall, "mismatches": mism}
 
 
 def batching_bench(args) -> int:
@@ -400,7 +425,23 @@ def main() -> int:
     torch.cuda.synchronize()
     t_conn = time.time() - t0
     phases = tg.phase_times()
-    log(f"preprocess {t_pre:.2f} s, performconnectivity {t_conn:.2f} s ({phases})")
+    log(f"preprocess {t_pre:.2f} s, first performconnectivity {t_conn:.2f} s ({phases})")
+    # performconnectivity warm: the same blocks again, scratch already sized;
+    # each step ends with iblank and iblank_cell written to the host arrays
+    for _ in range(args.warmup):
+        tg.connect()
+    conn_ms, conn_ph = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        tg.connect()
+        conn_ms.append(1000 * (time.perf_counter() - t0))
+        conn_ph.append(tg.phase_times())
+    conn_warm = {"steps": args.steps, "p50_ms": round(float(np.median(conn_ms)), 3),
+                 "p99_ms": round(float(np.percentile(conn_ms, 99)), 3),
+                 "searches": int(conn_ph[-1]["nsearch"]),
+                 "searches_per_s": round(conn_ph[-1]["nsearch"] / (float(np.median(conn_ms)) / 1000), 1),
+                 "phases_ms": {k: round(float(np.median([p[k] for p in conn_ph])), 3) for k in conn_ph[-1]}}
+    log(f"performconnectivity warm: {conn_warm}")
 
     # dataUpdate over the connectivity's receptors, 5 FP64 fields, q resident
     nvar = 5
@@ -426,25 +467,43 @@ def main() -> int:
     wall = time.time() - t0
     ms_per_step = 1000.0 * wall / args.steps
     value = nq / (ms_per_step / 1000.0)
+    # the donors the timed passes left (the path the rate is measured on)
+    timed = [tg.query_donors(i, xq.shape[0]) for i, (_, xq) in zip(qids, sets)]
+    timed_same = all(np.array_equal(a, b) for a, b in zip(timed, donors))
+    # queries inside their tree's root box (the others are rejected by one box test)
+    in_root = 0
+    for i, (_, xq) in zip(qids, sets):
+        e = tg.query_extents(i)
+        m = np.ones(xq.shape[0], bool)
+        for d in range(3):
+            m &= (xq[:, d] >= e[2 * d] - 1e-10) & (xq[:, d] <= e[2 * d + 1] + 1e-10)
+        in_root += int(m.sum())
 
     byblk = {b.tag: b for b in blocks}
     nelem_of = [tg.query_nelem(i) for i in qids]
-    algo = sum(search_bytes(byblk[btag], d, ne) for (btag, _), d, ne in zip(sets, donors, nelem_of))
+    algo = sum(search_bytes(byblk[btag], d, ne) for (btag, _), d, ne in zip(sets, timed, nelem_of))
     achieved = algo / (kern_ms / 1000.0) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 5),
             "traffic": profiled_traffic("search_pipeline_config2.md", "traffic_bytes_per_launch")
             if args.workload == "config2" else None,
-            "kernel": "k_search", "algorithmic_bytes_per_step": algo, "launches_per_step": len(qids),
+            "traffic_source": "profiles/search_pipeline_config2.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
+            "kernel": "search pipeline (k_collect, k_collect_more, k_bin_count, k_bin_fill, k_eval<4,5,6,8>, "
+                      "k_eval_long<5,6,8>, k_select)",
+            "algorithmic_bytes_per_step": algo, "launches_per_step": len(qids),
             "avg_launch_ms": round(kern_ms / len(qids), 4)}
 
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu:
-        rate, nsamp, tsamp = cpu_baseline(blocks, sets, args.cpu_budget)
-        cpu = {"value": round(rate, 1), "unit": "searches/s", "cores": 1, "kind": "port",
-               "sample": f"{nsamp} random queries drawn from both query sets, each set searched against the "
-                         f"oracle's ADT over the cells overlapping the sample's OBB; search loop only, "
-                         f"{tsamp:.1f} s"}
+        c = cpu_baseline(blocks, sets, timed, args.cpu_budget)
+        cpu = {"value": round(c["all_cores"], 1), "unit": "searches/s", "cores": c["threads"], "kind": "port",
+               "sample": f"{c['sampled']} random queries drawn from both query sets, each searched against the "
+                         f"tree the GPU searches (cells overlapping the OBB of the whole set); search loops only: "
+                         f"{c['seconds_all_cores']:.1f} s on {c['threads']} threads, "
+                         f"{c['seconds_one_core']:.1f} s on 1 core",
+               "one_core": round(c["one_core"], 1), "nproc": c["nproc"], "cpu_model": c["cpu_model"]}
+        parity = {"sampled": c["sampled"], "mismatches": c["mismatches"],
+                  "against": "oracle donors of the sampled queries (1 core and threaded) vs the GPU's timed passes"}
 
     bdu = dataupdate_bytes(tg, blocks, nvar)
     # tioga::reduce_fringes on this connectivity (last: it changes the
@@ -477,12 +536,16 @@ def main() -> int:
                    "blocks": {str(b.tag): {"nodes": b.nnodes, "cells": b.ncells} for b in blocks}},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "parity": parity,
+        "timed_donors_equal_checked": bool(timed_same),
+        "queries_in_root_box": in_root,
+        "searches_per_s_in_root_box": round(in_root / (ms_per_step / 1000.0), 1),
         "search_kernel_ms": round(kern_ms, 4),
         "search_pass_ms_device": round(pass_ms, 4),
-        "found": int(sum(int((d >= 0).sum()) for d in donors)),
-        "connectivity_ms": round(1000 * t_conn, 2),
-        "connectivity_phases_ms": {k: round(v, 3) for k, v in phases.items()},
-        "connectivity_searches": int(phases["nsearch"]),
+        "found": int(sum(int((d >= 0).sum()) for d in timed)),
+        "connectivity_first_ms": round(1000 * t_conn, 2),
+        "connectivity_first_phases_ms": {k: round(v, 3) for k, v in phases.items()},
+        "connectivity_warm": conn_warm,
         "preprocess_ms": round(1000 * t_pre, 2),
         "dataupdate": {"ms": round(du_ms, 4), "receptors": int(nrec), "nvar": nvar,
                        "algorithmic_bytes": bdu,

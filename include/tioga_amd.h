@@ -188,6 +188,17 @@ double tioga_amd_last_search_kernel_ms(void);
 void tioga_amd_query_set_clear(void);
 /* number of cells in a query set's search structure (-1: bad id) */
 int tioga_amd_query_set_nelem(int id);
+/* donors of query set id as the last pass over it left them (a search or
+   a timed pass of tioga_amd_time_query_sets), without searching again */
+void tioga_amd_query_set_donors(int id, int *donor);
+/* root extents of query set id's search structure, in the order of
+   adtExtents (xmin, xmax, ymin, ymax, zmin, zmax); returns its cell count
+   (-1: bad id) */
+int tioga_amd_query_set_extents(int id, double *ext);
+/* Capacity (entries) of the donor search's candidate list: cap > 0 sets the
+   capacity the next search starts from (a checked search grows it when its
+   candidates do not fit and walks again); returns the current capacity. */
+long tioga_amd_search_capacity(long cap);
 
 /* Host wall times (ms) of the last performconnectivity phases:
    out[0..6] = hole map, boxes, query gathering, search preparation,

@@ -58,6 +58,8 @@ def lib():
         L.orc_hole_map.restype = C.c_long
         L.orc_search_points.argtypes = [_P, C.c_int, _P, C.c_int, _P]
         L.orc_search_points.restype = C.c_double
+        L.orc_search_points_obb.argtypes = [_P, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int, _P]
+        L.orc_search_points_obb.restype = C.c_double
         L.orc_nodal_weights.argtypes = [_P, _P, _P, C.c_int]
         L.orc_cell_volume.argtypes = [_P, C.c_int]
         L.orc_cell_volume.restype = C.c_double
@@ -172,6 +174,17 @@ class Oracle:
         if n > 0:
             self.L.orc_block_array(self.h, btag, name.encode(), ptr(out))
         return out
+
+    def search_points_obb(self, btag: int, xobb: np.ndarray, xq: np.ndarray, nthreads: int = 1):
+        """Donors of xq against the tree MeshBlock::search builds for the
+        query set xobb (no dedupe): (donor[nq], tree size, seconds)."""
+        xobb = np.ascontiguousarray(xobb, np.float64).reshape(-1, 3)
+        xq = np.ascontiguousarray(xq, np.float64).reshape(-1, 3)
+        donor = np.zeros(xq.shape[0], np.int32)
+        ne = C.c_int(0)
+        t = self.L.orc_search_points_obb(self.h, btag, ptr(xobb), xobb.shape[0], ptr(xq), xq.shape[0], ptr(donor),
+                                         int(nthreads), C.byref(ne))
+        return donor, int(ne.value), float(t)
 
     def hole_map(self, mtag: int):
         nx = np.zeros(3, np.int32)

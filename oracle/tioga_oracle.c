@@ -23,6 +23,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <pthread.h>
 #include <string.h>
 #include <time.h>
 
@@ -2107,6 +2108,66 @@ long orc_hole_map(OrcTioga *t, int mtag, int nx[3], double ext[6], int *sam)
   if (ext) memcpy(ext, h->ext, sizeof(double) * 6);
   if (sam) memcpy(sam, h->sam, sizeof(int) * n);
   return n;
+}
+
+typedef struct {
+  const OrcBlock *b;
+  const double *xq;
+  int *donor;
+  int i0, i1;
+} SearchChunk;
+
+static void *search_chunk(void *arg)
+{
+  SearchChunk *c = (SearchChunk *)arg;
+  int i;
+  for (i = c->i0; i < c->i1; i++) c->donor[i] = search_adt(c->b, &c->xq[3 * i]);
+  return NULL;
+}
+
+/* MeshBlock::search (search.C:107-233) for the query set xobb[nobb]: the ADT
+   over the cells of block btag that overlap the OBB of xobb, then
+   searchADT (searchADTrecursion.C:26-101) for each of xq[nq], without the
+   coincident-point dedupe.  xq may be a subset (a sample) of xobb: its
+   searches run against the tree of the whole set.  Each search only reads
+   the block, so nthreads > 1 splits the queries into contiguous chunks over
+   POSIX threads.  Returns the elapsed seconds of the search loop. */
+double orc_search_points_obb(OrcTioga *t, int btag, const double *xobb, int nobb, const double *xq, int nq,
+                             int *donor, int nthreads, int *nelem)
+{
+  int k;
+  OrcBlock *b = find_block(t, btag, NULL);
+  if (nelem) *nelem = 0;
+  if (!b || nobb <= 0 || nq < 0) return 0.0;
+  if (!b->cellRes) tag_boundary(b);
+  build_block_adt(b, xobb, nobb);
+  if (nelem) *nelem = b->nelem;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  SearchChunk ch[256];
+  pthread_t th[256];
+  int started[256];
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  for (k = 0; k < nthreads; k++) {
+    ch[k].b = b;
+    ch[k].xq = xq;
+    ch[k].donor = donor;
+    ch[k].i0 = (int)((long long)nq * k / nthreads);
+    ch[k].i1 = (int)((long long)nq * (k + 1) / nthreads);
+  }
+  if (nthreads == 1) {
+    search_chunk(&ch[0]);
+  } else {
+    for (k = 0; k < nthreads; k++) {
+      started[k] = pthread_create(&th[k], NULL, search_chunk, &ch[k]) == 0;
+      if (!started[k]) search_chunk(&ch[k]); /* no thread: run the chunk here */
+    }
+    for (k = 0; k < nthreads; k++)
+      if (started[k]) pthread_join(th[k], NULL);
+  }
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }
 
 double orc_search_points(OrcTioga *t, int btag, const double *xq, int nq, int *donor)

@@ -81,6 +81,13 @@ int orc_weight_aborts(void);
    against the ADT of block `btag` built over all of its cells, sequentially,
    with containment tests; returns elapsed seconds. */
 double orc_search_points(OrcTioga *t, int btag, const double *xq, int nq, int *donor);
+/* The same search with the structure of another point set: the ADT of block
+   btag over the cells overlapping the OBB of xobb[nobb] (what
+   MeshBlock::search builds for the query set xobb), searched for each of
+   xq[nq] (a subset of xobb, say) without dedupe, on nthreads POSIX threads;
+   *nelem = the tree's element count.  Returns the search loop's seconds. */
+double orc_search_points_obb(OrcTioga *t, int btag, const double *xobb, int nobb, const double *xq, int nq,
+                             int *donor, int nthreads, int *nelem);
 
 /* ---- AMR Cartesian background, gather direction (single rank) ---- */
 /* tioga::register_amr_global_data (tioga.C:692-701): 11 ints and 6 doubles

@@ -509,6 +509,29 @@ int tioga_amd_query_set_nelem(int id)
   return r;
 }
 
+void tioga_amd_query_set_donors(int id, int *donor)
+{
+  guarded("tioga_amd_query_set_donors", [&] { tg_or_throw().engine()->query_set_donors(id, donor); });
+}
+
+int tioga_amd_query_set_extents(int id, double *ext)
+{
+  int r = -1;
+  guarded("tioga_amd_query_set_extents", [&] { r = tg_or_throw().engine()->query_set_extents(id, ext); });
+  return r;
+}
+
+long tioga_amd_search_capacity(long cap)
+{
+  long r = -1;
+  guarded("tioga_amd_search_capacity", [&] {
+    tg::Engine *e = tg_or_throw().engine();
+    if (cap > 0) e->set_search_capacity((size_t)cap);
+    r = (long)e->search_capacity();
+  });
+  return r;
+}
+
 void tioga_amd_phase_times(double *out)
 {
   guarded("tioga_amd_phase_times", [&] {

@@ -401,6 +401,14 @@ class Engine {
   // time_query_sets (the pass also orders the queries)
   double last_search_kernel_ms() const { return last_search_kernel_ms_; }
   int query_set_nelem(int id) const { return (id >= 0 && id < (int)qsets_.size()) ? qsets_[id]->nelem : -1; }
+  // donors of set id as the last pass over it (searched or timed) left them
+  void query_set_donors(int id, int *donor);
+  // root extents of set id's tree (adtExtents order); returns its size
+  int query_set_extents(int id, double ext[6]) const;
+  // candidate-list capacity (entries) the next donor search starts from; it
+  // grows whenever a checked search finds it too small
+  size_t search_capacity() const { return sw_.cap; }
+  void set_search_capacity(size_t cap) { sw_.cap = cap; }
 
   // AMR Cartesian background, gather direction (tg_amr.hip):
   // tioga::register_amr_global_data / set_amr_patch_count /

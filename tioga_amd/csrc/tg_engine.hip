@@ -1164,7 +1164,11 @@ constexpr int KSTAGE = 32;  // candidates per lane staged in LDS during the walk
 
 // Candidate lists of one search: a flat list of (cell | nv << 28, query)
 // entries, grouped per query in walk order, and per query its offset and
-// count (count 0: nothing to test).
+// count (count 0: nothing to test).  A slot that was reserved but not filled
+// (its query's list did not fit below the capacity) holds EMPTY_SLOT, so
+// every slot below min(used, cap) holds either a candidate or EMPTY_SLOT;
+// no kernel reads a value left from an earlier search.
+constexpr int EMPTY_SLOT = -1;  // nv field 15: no cell type has it
 struct CandOut {
   int *cell, *query, *status;
   int *qoff, *qcnt;
@@ -1294,7 +1298,12 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
   }
   const int off = base + pre;
   O.qoff[g] = off;
-  if (off + k > O.cap) return;  // flat list too small: detected by the host, which grows it and reruns
+  if (off + k > O.cap) {
+    // flat list too small (the host sees used > cap, grows it and walks
+    // again): the reserved slots below the capacity are marked empty
+    for (int j = off; j < O.cap && j < off + k; j++) O.cell[j] = EMPTY_SLOT;
+    return;
+  }
   for (int j = 0; j < k; j++) {
     O.cell[off + j] = stage[j][lane];
     O.query[off + j] = g;
@@ -1316,7 +1325,10 @@ __global__ void __launch_bounds__(64) k_collect_more(SearchSet T, CandOut O)
     const int cnt = O.qcnt[g];
     const int off = atomicAdd(O.used, cnt);
     O.qoff[g] = off;
-    if (off + cnt > O.cap) continue;
+    if (off + cnt > O.cap) {  // as in k_collect: reserved slots below the capacity marked empty
+      for (int j = off; j < O.cap && j < off + cnt; j++) O.cell[j] = EMPTY_SLOT;
+      continue;
+    }
     double p[3];
     load_xyz(S.xs, i, p);
     int sp = 0, nc = 0;
@@ -1343,7 +1355,9 @@ __device__ __forceinline__ int type_bucket(int nv) { return nv == 4 ? 0 : (nv ==
 // and each lane's rank among them; one LDS atomic per bucket and wave
 __device__ __forceinline__ int bin_of(const CandOut &O, int j, int n)
 {
-  return j < n ? type_bucket((unsigned)O.cell[j] >> 28) : -1;
+  if (j >= n) return -1;
+  const int c = O.cell[j];
+  return c == EMPTY_SLOT ? -1 : type_bucket((unsigned)c >> 28);
 }
 
 __global__ void __launch_bounds__(256) k_bin_count(CandOut O, int *__restrict__ bcount)
@@ -3500,8 +3514,10 @@ const int *Engine::order_queries(const double *xs, int nq, const double ext[6])
 void Engine::launch_query_set(QuerySet &q)
 {
   Block &b = *blocks_[q.blk];
-  if (q.nelem == 0) {
+  if (q.nelem == 0) {  // no cell overlaps the set: every donor -1 (the pair of events still brackets it)
+    if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
     fill<int>(q.donor.p, q.nq, -1, s_);
+    if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
     return;
   }
   SearchView S;
@@ -3527,6 +3543,21 @@ void Engine::query_set_search(int id, int *donor)
   launch_query_set(q);
   TG_HIP(hipMemcpyAsync(donor, q.donor.p, sizeof(int) * q.nq, hipMemcpyDeviceToHost, s_));
   sync();
+}
+
+void Engine::query_set_donors(int id, int *donor)
+{
+  if (id < 0 || id >= (int)qsets_.size()) throw std::runtime_error("tioga_amd: bad query set id");
+  QuerySet &q = *qsets_[id];
+  TG_HIP(hipMemcpyAsync(donor, q.donor.p, sizeof(int) * q.nq, hipMemcpyDeviceToHost, s_));
+  sync();
+}
+
+int Engine::query_set_extents(int id, double ext[6]) const
+{
+  if (id < 0 || id >= (int)qsets_.size()) throw std::runtime_error("tioga_amd: bad query set id");
+  memcpy(ext, qsets_[id]->ext, 6 * sizeof(double));
+  return qsets_[id]->nelem;
 }
 
 double Engine::time_query_sets(int nrep, long long *nq)

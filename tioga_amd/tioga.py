@@ -69,6 +69,11 @@ def load():
         L.tioga_amd_last_search_kernel_ms.restype = C.c_double
         L.tioga_amd_query_set_nelem.argtypes = [C.c_int]
         L.tioga_amd_query_set_nelem.restype = C.c_int
+        L.tioga_amd_query_set_donors.argtypes = [C.c_int, _P]
+        L.tioga_amd_query_set_extents.argtypes = [C.c_int, _P]
+        L.tioga_amd_query_set_extents.restype = C.c_int
+        L.tioga_amd_search_capacity.argtypes = [C.c_long]
+        L.tioga_amd_search_capacity.restype = C.c_long
         L.tioga_amd_weight_aborts.restype = C.c_int
         L.tioga_register_amr_global_data_.argtypes = [_P] * 7
         L.tioga_register_amr_patch_count_.argtypes = [_P]
@@ -359,6 +364,28 @@ class Tioga:
         n = self.L.tioga_amd_query_set_nelem(qid)
         self._check()
         return int(n)
+
+    def query_clear(self) -> None:
+        self.L.tioga_amd_query_set_clear()
+        self._check()
+
+    def query_donors(self, qid: int, nq: int) -> np.ndarray:
+        """Donors as the last pass over the set left them (no new search)."""
+        out = np.zeros(nq, np.int32)
+        self.L.tioga_amd_query_set_donors(qid, _p(out))
+        self._check()
+        return out
+
+    def query_extents(self, qid: int) -> np.ndarray:
+        ext = np.zeros(6)
+        self.L.tioga_amd_query_set_extents(qid, _p(ext))
+        self._check()
+        return ext
+
+    def search_capacity(self, cap: int = 0) -> int:
+        r = self.L.tioga_amd_search_capacity(int(cap))
+        self._check()
+        return int(r)
 
     def time_query_sets(self, nrep: int):
         """(ms per pass, queries per pass, ms per pass in the search kernel alone)"""
