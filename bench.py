@@ -367,6 +367,22 @@ def serving_bench(args) -> int:
     return 0
 
 
+def search_only(args, tg, blocks, desc) -> int:
+    """The headline passes alone (what the committed kernel profiles run)."""
+    sets = [(blocks[0].tag, blocks[1].x), (blocks[1].tag, blocks[0].x)]
+    qids = [tg.query_set(btag, xq) for btag, xq in sets]
+    for i, (_, xq) in zip(qids, sets):
+        tg.query_search(i, xq.shape[0])
+    for _ in range(args.warmup):
+        tg.time_query_sets(1)
+    pass_ms, nq, kern_ms = tg.time_query_sets(args.steps)
+    tg.close()
+    print(json.dumps({"workload": desc, "steps": args.steps, "queries_per_step": int(nq),
+                      "search_pass_ms_device": round(pass_ms, 4), "search_kernel_ms": round(kern_ms, 4)}),
+          flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -376,6 +392,9 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU search for the baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-amr", action="store_true", help="skip the AMR gather leg")
+    ap.add_argument("--search-only", action="store_true",
+                    help="only the headline query-set passes (profiling runs): no performconnectivity, dataUpdate, "
+                         "CPU baseline, reduce_fringes or AMR leg")
     ap.add_argument("--batched-only", action="store_true", help="config3: skip the block-by-block form")
     ap.add_argument("--slots", type=int, default=8, help="serving slots (config4)")
     ap.add_argument("--budget-gb", type=float, default=64.0, help="serving HBM budget (config4)")
@@ -420,6 +439,8 @@ def main() -> int:
     tg.preprocess()
     torch.cuda.synchronize()
     t_pre = time.time() - t0
+    if args.search_only:
+        return search_only(args, tg, blocks, desc)
     t0 = time.time()
     tg.connect()
     torch.cuda.synchronize()

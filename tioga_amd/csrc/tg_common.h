@@ -26,6 +26,20 @@ constexpr int HOLEMAPSIZE = 192;
 TG_HD double dmin(double a, double b) { return a < b ? a : b; }  // TIOGA_MIN
 TG_HD double dmax(double a, double b) { return a > b ? a : b; }  // TIOGA_MAX
 
+// The 16-bit grid coordinate of x in a tree's grid (lo, inv per axis):
+// clamp(((float)x - lo) * inv, 0, 65535), truncated.  Every step (rounding to
+// float, subtracting a constant, multiplying by a constant inv >= 0, the
+// clamps, truncation) is non-decreasing in x, so x >= b implies
+// grid_of(x) >= grid_of(b) and x <= b implies grid_of(x) <= grid_of(b): a box
+// test on grid coordinates passes whenever the FP64 test passes.
+TG_HD unsigned grid_of(double x, float lo, float inv)
+{
+  float u = ((float)x - lo) * inv;
+  u = u > 0.0f ? u : 0.0f;  // also maps NaN to 0
+  u = u < 65535.0f ? u : 65535.0f;
+  return (unsigned)u;
+}
+
 // Gaussian elimination on a 3x3 system with row swaps only when
 // |a_ii| < 1e-8 (math.c:25-81).  Returns 0 for a singular system.
 TG_HD int solve3(double a[3][3], double b[3])

@@ -87,16 +87,41 @@ struct TypeInfo {
   int cstart[MAXTYPES + 1];
 };
 
-// One ADT node as the search kernel reads it: 160 bytes, node ids in the
-// ADT's preorder; left/right are child node ids (-1: none) and the children's
-// subtree boxes sit in the parent, so one record decides a whole visit.  The
-// boxes are stored padded by the search tolerance (lower - TOL, upper + TOL).
-struct __align__(16) AdtNode {
-  double ebox[6];  // box of the node's own element
-  double lbox[6];  // subtree box of the left child
-  double rbox[6];  // subtree box of the right child
-  int cell, nv, left, right;
+// One ADT node as the search walk reads it (48 bytes).  The walk only
+// needs to know which nodes a point may enter and which elements' boxes may
+// hold it, so the bounds are kept coarse and conservative: every bound, padded
+// by the search tolerance as the reference tests it (lower - TOL,
+// upper + TOL), is mapped to the tree's 16-bit grid by the monotone map
+// grid_of() (tg_common.h); a point whose padded test passes also passes the
+// grid test, never the other way round.  The exact FP64 box test of each
+// listed element is made afterwards, by the evaluation.  Bounds are packed two
+// per word: (xlo | ylo << 16), (zlo | xhi << 16), (yhi | zhi << 16) for the
+// node's own element box (e), the left child's subtree box (l) and the right
+// child's (r).  cellnv = cell | nv << 28; left/right: child node ids in the
+// ADT's numbering (-1: none).
+struct __align__(16) WalkNode {
+  unsigned e[3], l[3], r[3];
+  int cellnv, left, right;
 };
+
+// The 16-bit grid of one tree: grid_of(x) = clamp(((float)x - lo) * inv,
+// 0, 65535), truncated (tg_common.h).
+struct GridFrame {
+  float lo[3], inv[3];
+};
+
+// The grid of a tree whose boxes lie inside ext (adtExtents order)
+inline GridFrame grid_frame(const double ext[6])
+{
+  GridFrame g;
+  for (int d = 0; d < 3; d++) {
+    const double span = ext[2 * d + 1] - ext[2 * d];
+    g.lo[d] = (float)ext[2 * d];
+    const double inv = span > 0.0 ? 65535.0 / span : 0.0;
+    g.inv[d] = inv < 3.0e38 ? (float)inv : 3.0e38f;
+  }
+  return g;
+}
 
 // Hole map of one mesh tag as the kernels see it.
 struct HoleMapView {
@@ -158,7 +183,7 @@ struct Block {
   int nelem = 0, adt_depth = 0;
   DBuf<int> elist, adtInt;
   DBuf<double> ebox, adtReal;
-  DBuf<AdtNode> nodes;
+  DBuf<WalkNode> nodes;
   double adtExt[6] = {0, 0, 0, 0, 0, 0};
 
   // donor bookkeeping (receptor side: one entry per node)
@@ -182,7 +207,7 @@ struct QuerySet {
   int nelem = 0, adt_depth = 0;
   DBuf<int> elist, adtInt;
   DBuf<double> ebox, adtReal;
-  DBuf<AdtNode> nodes;
+  DBuf<WalkNode> nodes;
   double ext[6] = {0, 0, 0, 0, 0, 0};
 };
 
@@ -464,7 +489,7 @@ class Engine {
   void launch_searches();
 
   void build_search_structure(Block &b, const OBB &obq, int &nelem, int &depth, DBuf<int> &elist,
-                              DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, DBuf<AdtNode> &nodes,
+                              DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, DBuf<WalkNode> &nodes,
                               double ext[6]);
   void launch_query_set(QuerySet &q);
   const int *order_queries(const double *xs, int nq, const double ext[6]);
@@ -479,7 +504,7 @@ class Engine {
   DBuf<int> qtag_, qrep_;     // every block's query mesh tags / representatives
   DBuf<double> tebox_, treal_;  // every batched tree's element boxes / adtReal, back to back
   DBuf<int> tint_, telist_;     // every batched tree's adtInt / element list, back to back
-  DBuf<AdtNode> tnodes_;        // every batched tree's search records, back to back
+  DBuf<WalkNode> tnodes_;        // every batched tree's search records, back to back
   DBuf<unsigned char> ntab_;    // the per-tree table of the batched search records
   DBuf<unsigned char> ptab_;    // the per-block table of the batched cell filter / element boxes
   DBuf<int> pseg_;              // their cell and element offsets

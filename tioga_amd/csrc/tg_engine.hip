@@ -921,37 +921,52 @@ __global__ void k_seg_starts(int nseg, int n, const int *__restrict__ sorted, co
   start[sg] = lo;
 }
 
-static_assert(sizeof(AdtNode) == 160, "ADT node record must be 160 bytes");
+static_assert(sizeof(WalkNode) == 48, "walk node record must be 48 bytes");
 
-// ADT arrays (adtIntegers, adtReals, element boxes) -> node records
-__device__ __forceinline__ void k_adt_node_one(int n, const TypeInfo &ti, const int *__restrict__ adtInt,
-                                               const double *__restrict__ adtReal, const double *__restrict__ ebox,
-                                               const int *__restrict__ elist, AdtNode *__restrict__ nodes)
+// two grid values in one word: a | b << 16
+__device__ __forceinline__ unsigned pack2(unsigned a, unsigned b) { return a | (b << 16); }
+
+// a box (lower xyz, upper xyz; FP64 as the reference stores it) padded by TOL
+// the way the walk's tests pad it, on the tree's grid, packed three words
+__device__ __forceinline__ void grid_box(const double *b, const GridFrame &g, unsigned w[3])
+{
+  unsigned q[6];
+  for (int j = 0; j < 6; j++) {
+    const double v = j < 3 ? b[j] - TOL : b[j] + TOL;
+    q[j] = grid_of(v, g.lo[j % 3], g.inv[j % 3]);
+  }
+  w[0] = pack2(q[0], q[1]);
+  w[1] = pack2(q[2], q[3]);
+  w[2] = pack2(q[4], q[5]);
+}
+
+// ADT arrays (adtIntegers, adtReals, element boxes) -> walk records
+__device__ __forceinline__ void k_adt_node_one(int n, const TypeInfo &ti, const GridFrame &g,
+                                               const int *__restrict__ adtInt, const double *__restrict__ adtReal,
+                                               const double *__restrict__ ebox, const int *__restrict__ elist,
+                                               WalkNode *__restrict__ nodes)
 {
   int4 a = reinterpret_cast<const int4 *>(adtInt)[n];
-  AdtNode N;
-  N.cell = elist[a.x];
-  N.nv = ti.nv[type_of(ti, N.cell)];
+  WalkNode N;
+  const int cell = elist[a.x];
+  N.cellnv = cell | (ti.nv[type_of(ti, cell)] << 28);
   N.left = a.y > -1 ? adtInt[4 * (size_t)a.y + 3] : -1;
   N.right = a.z > -1 ? adtInt[4 * (size_t)a.z + 3] : -1;
-  // bounds padded by TOL here, in the same FP64 operation the box test of
-  // the walk makes (lower - TOL, upper + TOL)
-  for (int j = 0; j < 6; j++) {
-    const double pad = j < 3 ? -TOL : TOL;
-    N.ebox[j] = ebox[6 * (size_t)a.x + j] + pad;
-    N.lbox[j] = N.left > -1 ? adtReal[6 * (size_t)N.left + j] + pad : 0.0;
-    N.rbox[j] = N.right > -1 ? adtReal[6 * (size_t)N.right + j] + pad : 0.0;
-  }
+  grid_box(ebox + 6 * (size_t)a.x, g, N.e);
+  if (N.left > -1) grid_box(adtReal + 6 * (size_t)N.left, g, N.l);
+  else N.l[0] = N.l[1] = N.l[2] = 0;
+  if (N.right > -1) grid_box(adtReal + 6 * (size_t)N.right, g, N.r);
+  else N.r[0] = N.r[1] = N.r[2] = 0;
   nodes[n] = N;
 }
 
-__global__ void k_adt_nodes(int nelem, TypeInfo ti, const int *__restrict__ adtInt,
+__global__ void k_adt_nodes(int nelem, TypeInfo ti, GridFrame g, const int *__restrict__ adtInt,
                             const double *__restrict__ adtReal, const double *__restrict__ ebox,
-                            const int *__restrict__ elist, AdtNode *__restrict__ nodes)
+                            const int *__restrict__ elist, WalkNode *__restrict__ nodes)
 {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nelem) return;
-  k_adt_node_one(n, ti, adtInt, adtReal, ebox, elist, nodes);
+  k_adt_node_one(n, ti, g, adtInt, adtReal, ebox, elist, nodes);
 }
 
 // k_adt_nodes for every tree of a batch whose arrays lie back to back: tree
@@ -959,12 +974,13 @@ __global__ void k_adt_nodes(int nelem, TypeInfo ti, const int *__restrict__ adtI
 // entries tree-local as the reference stores them
 struct AdtNodesTree {
   TypeInfo ti;
+  GridFrame g;
   int base;
 };
 __global__ void k_adt_nodes_all(int ntree, int total, const AdtNodesTree *__restrict__ T,
                                 const int *__restrict__ adtInt, const double *__restrict__ adtReal,
                                 const double *__restrict__ ebox, const int *__restrict__ elist,
-                                AdtNode *__restrict__ nodes)
+                                WalkNode *__restrict__ nodes)
 {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
@@ -975,7 +991,8 @@ __global__ void k_adt_nodes_all(int ntree, int total, const AdtNodesTree *__rest
     else hi = mid - 1;
   }
   const size_t b = T[lo].base;
-  k_adt_node_one(g - (int)b, T[lo].ti, adtInt + 4 * b, adtReal + 6 * b, ebox + 6 * b, elist + b, nodes + b);
+  k_adt_node_one(g - (int)b, T[lo].ti, T[lo].g, adtInt + 4 * b, adtReal + 6 * b, ebox + 6 * b, elist + b,
+                 nodes + b);
 }
 
 // 30-bit Morton code of each query in the frame of the search structure's
@@ -1049,7 +1066,8 @@ struct SearchView {
   const double *xs;
   const int *xtag;
   int *donorId;
-  const AdtNode *nodes;
+  const WalkNode *nodes;
+  GridFrame g;  // the tree's grid (the walk records' bounds are on it)
   double ext[6];
   const int *conn;
   const double *x;
@@ -1090,6 +1108,10 @@ __device__ __forceinline__ SearchView view_of(const SearchSet &T, int sid)
   S.xtag = as_global(v->xtag);
   S.donorId = as_global(v->donorId);
   S.nodes = as_global(v->nodes);
+  for (int d = 0; d < 3; d++) {
+    S.g.lo[d] = v->g.lo[d];
+    S.g.inv[d] = v->g.inv[d];
+  }
   for (int d = 0; d < 6; d++) S.ext[d] = v->ext[d];
   S.conn = as_global(v->conn);
   S.x = as_global(v->x);
@@ -1109,52 +1131,13 @@ __device__ __forceinline__ int wave_search(const SearchSet &T, int wv)
   return lo;
 }
 
-__device__ __forceinline__ bool in_box(const double p[3], const double *b)
+// The grid test of a packed box (grid_box layout) for a point at grid
+// coordinates q, without short-circuit branches: passes whenever the
+// point's FP64 test against the padded box passes.
+__device__ __forceinline__ bool in_grid(const unsigned q[3], const unsigned w[3])
 {
-  return p[0] >= b[0] - TOL && p[1] >= b[1] - TOL && p[2] >= b[2] - TOL && p[0] <= b[3] + TOL &&
-         p[1] <= b[4] + TOL && p[2] <= b[5] + TOL;
-}
-
-// the same test against bounds already padded by TOL (tree node records),
-// evaluated without short-circuit branches
-__device__ __forceinline__ bool in_padded(const double p[3], const double *b)
-{
-  return (p[0] >= b[0]) & (p[1] >= b[1]) & (p[2] >= b[2]) & (p[0] <= b[3]) & (p[1] <= b[4]) & (p[2] <= b[5]);
-}
-
-// MeshBlock::checkContainment (checkContainment.C:27-96) for a cell with NV
-// vertices: weights of p; accept when every weight lies in [-TOL, 1+TOL];
-// flag a near-zero weight in a BIGVALUE cell, which keeps the search going.
-// NV is a template constant so the vertex and weight arrays stay in registers.
-template <int NV>
-__device__ __forceinline__ void contain_t(const SearchView &S, int c, const double p[3], int &res, int &flag)
-{
-  const int *cv = S.conn + 8 * (size_t)c;
-  double xv[8][3], frac[8];
-#pragma unroll
-  for (int m = 0; m < NV; m++) load_xyz(S.x, cv[m], xv[m]);
-  nodal_weights(xv, p, frac, NV);
-  res = c;
-  flag = 0;
-  bool big = (S.cellRes[c] == BIGVALUE);
-#pragma unroll
-  for (int m = 0; m < NV; m++) {
-    if ((frac[m] + TOL) * (frac[m] - 1.0 - TOL) > 0) {
-      res = -1;
-      return;
-    }
-    if (fabs(frac[m]) < TOL && big) flag = 1;
-  }
-}
-
-__device__ __forceinline__ void contain(const SearchView &S, int c, int nv, const double p[3], int &res, int &flag)
-{
-  switch (nv) {
-    case 4: contain_t<4>(S, c, p, res, flag); break;
-    case 5: contain_t<5>(S, c, p, res, flag); break;
-    case 6: contain_t<6>(S, c, p, res, flag); break;
-    default: contain_t<8>(S, c, p, res, flag); break;
-  }
+  return (q[0] >= (w[0] & 0xffffu)) & (q[1] >= (w[0] >> 16)) & (q[2] >= (w[1] & 0xffffu)) &
+         (q[0] <= (w[1] >> 16)) & (q[1] <= (w[2] & 0xffffu)) & (q[2] <= (w[2] >> 16));
 }
 
 constexpr int WAVE = 64;
@@ -1193,6 +1176,17 @@ struct CandOut {
 // which keeps long candidate lists and slow Newton solves from serialising
 // inside one lane.
 //
+// The walk tests the coarse grid boxes of the walk records (WalkNode), which
+// pass whenever the reference's padded FP64 tests pass, so it visits a
+// superset of the reference's nodes, in the same preorder, and lists a
+// superset of its candidates.  k_eval makes the exact FP64 element-box test
+// of each listed candidate (the element box is the min/max of the cell's
+// vertices, search.C:186-214) and marks the ones that fail OUTSIDE, which
+// k_select skips.  What remains is exactly the reference's list: an element
+// whose padded box holds the point lies inside every ancestor's padded
+// subtree box (subtree boxes are min/max folds that include it), so the
+// reference's walk reaches it.
+//
 // k_collect: the 64 queries of a wave (consecutive in Morton order, so
 // spatially close) walk the tree together: the wave visits a node when any
 // lane's own walk would, carrying the mask of those lanes, so every lane sees
@@ -1206,7 +1200,7 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
   const int lane = threadIdx.x;
   const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, blockIdx.x));
   const SearchView S = view_of(T, sid);
-  const const_ptr<AdtNode> nodes = (const_ptr<AdtNode>)S.nodes;
+  const const_ptr<WalkNode> nodes = (const_ptr<WalkNode>)S.nodes;
   const int qb = T.qbase[sid];
   const int t = (blockIdx.x - T.wbase[sid]) * WAVE + lane;
   int i = -1;
@@ -1215,13 +1209,15 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
     if (T.ns > 1) T.qsid[qb + i] = sid;
     if (S.xtag && S.xtag[i] != i) i = -1;
   }
-  double p[3] = {0.0, 0.0, 0.0};
   bool inroot = false;
+  unsigned q[3] = {0u, 0u, 0u};  // the point on the tree's grid
   if (i >= 0) {
+    double p[3];
     load_xyz(S.xs, i, p);
-    inroot = true;
+    inroot = true;  // the root box test, FP64 as searchADTrecursion.C:38-43 makes it
     for (int d = 0; d < 3; d++) inroot = inroot && (p[d] >= S.ext[2 * d] - TOL);
     for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= S.ext[2 * d + 1] + TOL);
+    for (int d = 0; d < 3; d++) q[d] = grid_of(p[d], S.g.lo[d], S.g.inv[d]);
   }
   int nc = 0;
   // pending right subtrees: entry k (node, lane mask) held by lane k, so a
@@ -1234,28 +1230,27 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
   unsigned long long m = __ballot(inroot);
   bool more = m != 0ull;
   while (more) {
-    // the whole record in one batch of scalar loads, then the tests
-    AdtNode N;
+    // the whole 48-byte record in one batch of scalar loads, then the tests
+    WalkNode N;
     {
-      const_ptr<AdtNode> r = nodes + node;
-      for (int d = 0; d < 6; d++) {
-        N.ebox[d] = r->ebox[d];
-        N.lbox[d] = r->lbox[d];
-        N.rbox[d] = r->rbox[d];
+      const_ptr<WalkNode> r = nodes + node;
+      for (int d = 0; d < 3; d++) {
+        N.e[d] = r->e[d];
+        N.l[d] = r->l[d];
+        N.r[d] = r->r[d];
       }
-      N.cell = r->cell;
-      N.nv = r->nv;
+      N.cellnv = r->cellnv;
       N.left = r->left;
       N.right = r->right;
     }
     const bool act = (m >> lane) & 1ull;
-    if (act & in_padded(p, N.ebox)) {
-      if (nc < KSTAGE) stage[nc][lane] = N.cell | (N.nv << 28);
+    if (act & in_grid(q, N.e)) {
+      if (nc < KSTAGE) stage[nc][lane] = N.cellnv;
       nc++;
     }
     const int r = N.right, l = N.left;
-    const unsigned long long mr = (r > -1) ? __ballot(act & in_padded(p, N.rbox)) : 0ull;
-    const unsigned long long ml = (l > -1) ? __ballot(act & in_padded(p, N.lbox)) : 0ull;
+    const unsigned long long mr = (r > -1) ? __ballot(act & in_grid(q, N.r)) : 0ull;
+    const unsigned long long ml = (l > -1) ? __ballot(act & in_grid(q, N.l)) : 0ull;
     if (mr) {
       if (lane == sp) {
         vnode = r;
@@ -1316,11 +1311,11 @@ __global__ void __launch_bounds__(64) k_collect_more(SearchSet T, CandOut O)
 {
   __shared__ int stk[STACK][64];
   const int n = *O.novf, tid = threadIdx.x;
-  for (int q = blockIdx.x * 64 + tid; q < n; q += gridDim.x * 64) {
-    const int g = O.ovf[q];
+  for (int qi = blockIdx.x * 64 + tid; qi < n; qi += gridDim.x * 64) {
+    const int g = O.ovf[qi];
     const int sid = T.ns == 1 ? 0 : T.qsid[g];
     const SearchView S = view_of(T, sid);
-    const AdtNode *nodes = S.nodes;
+    const WalkNode *nodes = S.nodes;
     const int i = g - T.qbase[sid];
     const int cnt = O.qcnt[g];
     const int off = atomicAdd(O.used, cnt);
@@ -1330,18 +1325,20 @@ __global__ void __launch_bounds__(64) k_collect_more(SearchSet T, CandOut O)
       continue;
     }
     double p[3];
+    unsigned q[3];
     load_xyz(S.xs, i, p);
+    for (int d = 0; d < 3; d++) q[d] = grid_of(p[d], S.g.lo[d], S.g.inv[d]);
     int sp = 0, nc = 0;
     stk[sp++][tid] = 0;
     while (sp > 0) {
-      const AdtNode &N = nodes[stk[--sp][tid]];
-      if (in_padded(p, N.ebox)) {
-        O.cell[off + nc] = N.cell | (N.nv << 28);
+      const WalkNode &N = nodes[stk[--sp][tid]];
+      if (in_grid(q, N.e)) {
+        O.cell[off + nc] = N.cellnv;
         O.query[off + nc] = g;
         nc++;
       }
-      if (N.right > -1 && in_padded(p, N.rbox)) stk[sp++][tid] = N.right;
-      if (N.left > -1 && in_padded(p, N.lbox)) stk[sp++][tid] = N.left;
+      if (N.right > -1 && in_grid(q, N.r)) stk[sp++][tid] = N.right;
+      if (N.left > -1 && in_grid(q, N.l)) stk[sp++][tid] = N.left;
     }
   }
 }
@@ -1419,7 +1416,10 @@ __global__ void __launch_bounds__(256) k_bin_fill(CandOut O, int *__restrict__ b
 }
 
 // Outcome of a containment test from the weights (checkContainment.C:62-85):
-// 0 reject, 1 accepted with a near-zero weight in a BIGVALUE cell, 2 clean
+// 0 reject, 1 accepted with a near-zero weight in a BIGVALUE cell, 2 clean;
+// OUTSIDE: the point fails the element's padded box test, so the reference
+// never tests this cell (listed only by the coarse grid walk)
+constexpr int OUTSIDE = 3;
 template <int NV>
 __device__ __forceinline__ int test_status(const double frac[8], bool big)
 {
@@ -1470,6 +1470,24 @@ __global__ void __launch_bounds__(256, WAVES) k_eval(SearchSet T, CandOut O, con
     double p[3], xv[8][3], frac[8];
     load_xyz(S.xs, g - T.qbase[sid], p);
     cell_vertices<NV>(S, c, xv);
+    // the element box test of searchADTrecursion.C:66-71 on the box of
+    // search.C:199-214 (min/max of the vertices; the sign of a zero bound
+    // does not change x - TOL)
+    bool inb = true;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      double lo = xv[0][d], hi = xv[0][d];
+#pragma unroll
+      for (int mm = 1; mm < NV; mm++) {
+        lo = dmin(lo, xv[mm][d]);
+        hi = dmax(hi, xv[mm][d]);
+      }
+      inb = inb & (p[d] >= lo - TOL) & (p[d] <= hi + TOL);
+    }
+    if (!inb) {
+      O.status[j] = OUTSIDE;
+      continue;
+    }
     if (NV == 4) {
       nodal_weights(xv, p, frac, 4);
     } else {
@@ -1536,6 +1554,7 @@ __global__ void k_select(SearchSet T, CandOut O)
   }
   for (int k = 0; k < nc; k++) {
     const int st = O.status[off + k];
+    if (st == OUTSIDE) continue;
     const int cell = O.cell[off + k] & 0x0fffffff;
     res = st ? cell : -1;
     if (st == 2) break;
@@ -3404,7 +3423,7 @@ void Engine::prepare_searches(const std::vector<int> &act)
     memcpy(b.adtExt, trees[a].ext, sizeof b.adtExt);
     if (b.adt_depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
     b.nodes.view(tnodes_.p + eoff[a], b.nelem);
-    nt[a] = AdtNodesTree{b.ti, eoff[a]};
+    nt[a] = AdtNodesTree{b.ti, grid_frame(trees[a].ext), eoff[a]};
   }
   if (te) {
     ntab_.reserve(sizeof(AdtNodesTree) * na);
@@ -3451,7 +3470,7 @@ void Engine::dedupe_gid(Block &b)
 // as a LIFO list (descending cell index) -> element boxes -> ADT
 void Engine::build_search_structure(Block &b, const OBB &obq, int &nelem, int &depth, DBuf<int> &elist,
                                     DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal,
-                                    DBuf<AdtNode> &nodes, double ext[6])
+                                    DBuf<WalkNode> &nodes, double ext[6])
 {
   auto now = [] { return std::chrono::steady_clock::now(); };
   auto ms = [](auto a, auto c) { return std::chrono::duration<double, std::milli>(c - a).count(); };
@@ -3472,7 +3491,7 @@ void Engine::build_search_structure(Block &b, const OBB &obq, int &nelem, int &d
   depth = build_adt_device(ne, ebox.p, adtInt.p, adtReal.p, ext, adtw_, s_);
   if (depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
   nodes.reserve_own(ne);
-  if (ne) k_adt_nodes<<<nb(ne), BS, 0, s_>>>(ne, b.ti, adtInt.p, adtReal.p, ebox.p, elist.p, nodes.p);
+  if (ne) k_adt_nodes<<<nb(ne), BS, 0, s_>>>(ne, b.ti, grid_frame(ext), adtInt.p, adtReal.p, ebox.p, elist.p, nodes.p);
   sync();
   auto t3 = now();
   prep_acc_[1] += ms(t1, t2);
@@ -3527,6 +3546,7 @@ void Engine::launch_query_set(QuerySet &q)
   S.xtag = nullptr;
   S.donorId = q.donor.p;
   S.nodes = q.nodes.p;
+  S.g = grid_frame(q.ext);
   memcpy(S.ext, q.ext, sizeof S.ext);
   S.conn = b.conn.p;
   S.x = b.x.p;
@@ -3612,6 +3632,7 @@ void Engine::launch_search(Block &b)
   S.xtag = b.xtag.p;
   S.donorId = b.donorId.p;
   S.nodes = b.nodes.p;
+  S.g = grid_frame(b.adtExt);
   memcpy(S.ext, b.adtExt, sizeof S.ext);
   S.conn = b.conn.p;
   S.x = b.x.p;
@@ -3664,6 +3685,7 @@ void Engine::launch_searches()
     S.xtag = b.xtag.p;
     S.donorId = b.donorId.p;
     S.nodes = b.nodes.p;
+    S.g = grid_frame(b.adtExt);
     memcpy(S.ext, b.adtExt, sizeof S.ext);
     S.conn = b.conn.p;
     S.x = b.x.p;
