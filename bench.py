@@ -511,8 +511,8 @@ def main() -> int:
             "traffic_source": "profiles/search_pipeline_config2.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
             "kernel": "search pipeline (k_collect, k_collect_more, k_bin_count, k_bin_fill, k_eval<4,5,6,8>, "
                       "k_eval_long<5,6,8>, k_select)",
-            "algorithmic_bytes_per_step": algo, "launches_per_step": len(qids),
-            "avg_launch_ms": round(kern_ms / len(qids), 4)}
+            "algorithmic_bytes_per_step": algo, "launches_per_step": 1,
+            "avg_launch_ms": round(kern_ms, 4)}
 
     cpu, parity = None, None
     if not args.no_cpu:

@@ -91,3 +91,4 @@ def test_unique_nodes_matches_oracle(lib):
     ol.orc_unique_nodes_octree(ptr(x), ptr(tag), ptr(r2), ptr(t2), n)
     assert np.array_equal(t1, t2) and np.array_equal(r1, r2)
     assert (t1 != np.arange(n)).sum() > n // 3  # duplicates were found
+

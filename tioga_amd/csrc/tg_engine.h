@@ -316,10 +316,35 @@ struct StepScratch {
 // Reusable device scratch of the donor search: per-query candidate lists
 // and the queue of queries with more candidates than a list holds.
 struct SearchScratch {
-  DBuf<int> cell, query, status, qoff, qcnt, ovf, counters, bins, blist, lj, lcount, qb, wb, qsid;
+  DBuf<int> cell, query, status, qoff, qcnt, ovf, counters, bins, blist, lj, lcount, qb, wb, qsid, nin;
   DBuf<unsigned char> sv;  // the table of searches of one launch
   DBuf<double4> lst;
   size_t cap = 0;  // capacity of the flat candidate list
+  // the continued Newton solves run on a second stream (default priority),
+  // beside the tet tests; created on first use
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  SearchScratch() = default;
+  SearchScratch(const SearchScratch &) = delete;
+  SearchScratch &operator=(const SearchScratch &) = delete;
+  ~SearchScratch()
+  {
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (side) (void)hipStreamDestroy(side);
+  }
+};
+
+// The query sets of a search launch, for their Morton keys in one launch:
+// set k's queries are [off, off + nq) of the concatenation.  ext: the root
+// box of the set's tree (adtExtents order), which frames the Morton code and
+// decides which queries walk the tree at all.
+struct MortonSet {
+  const double *xs;
+  const int *xtag;  // representatives of coincident queries, or null
+  int *donorId;
+  double ext[6];
+  int off, nq;
 };
 
 struct AmrState;  // tg_amr.hip
@@ -491,8 +516,8 @@ class Engine {
   void build_search_structure(Block &b, const OBB &obq, int &nelem, int &depth, DBuf<int> &elist,
                               DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, DBuf<WalkNode> &nodes,
                               double ext[6]);
-  void launch_query_set(QuerySet &q);
-  const int *order_queries(const double *xs, int nq, const double ext[6]);
+  void launch_query_sets(const std::vector<QuerySet *> &qs);
+  const int *order_sets(const std::vector<MortonSet> &M, size_t tot);
   SearchScratch sw_;
   bool timing_ = false;  // inside time_query_sets: candidate capacity already established
   std::vector<hipEvent_t> *kernel_events_ = nullptr;

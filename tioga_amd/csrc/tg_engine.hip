@@ -1009,33 +1009,13 @@ __device__ __forceinline__ unsigned spread10(unsigned v)
   return v;
 }
 
-__global__ void k_morton(int nq, const double *__restrict__ xs, double x0, double x1, double y0, double y1, double z0,
-                         double z1, unsigned *__restrict__ key, int *__restrict__ idx)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nq) return;
-  double lo[3] = {x0, y0, z0}, hi[3] = {x1, y1, z1};
-  unsigned c[3];
-  for (int d = 0; d < 3; d++) {
-    double u = (xs[3 * (size_t)i + d] - lo[d]) / (hi[d] - lo[d]);
-    u = u < 0 ? 0 : (u > 0.999999 ? 0.999999 : u);
-    c[d] = (unsigned)(u * 1024.0);
-  }
-  key[i] = (spread10(c[0]) << 2) | (spread10(c[1]) << 1) | spread10(c[2]);
-  idx[i] = i;
-}
-
-// The query sets of a batched search, for their Morton keys in one launch:
-// set k's queries are [off, off + nq) of the concatenation.
-struct MortonSet {
-  const double *xs;
-  double lo[3], hi[3];
-  int off, nq;
-};
-
-// k_morton for every set at once, keyed (set << mb) | (Morton >> (30 - mb))
-// so that one sort orders each set's queries inside the set's own range
-// (idx = the query's index within its set)
+// Keys for one sort of every set: (set << (mb + 1)) | (skip << mb) |
+// (Morton >> (30 - mb)), so each set's queries stay in the set's own range,
+// the ones that walk the tree come first in Morton order, and the skipped
+// ones last: a coincident copy (its representative's donor is copied later,
+// search.C:251-265) or a point outside the root box, whose donor is -1
+// (searchADTrecursion.C:38-45; written here).  idx = the query's index
+// within its set.
 __global__ void k_morton_sets(int ns, int tot, const MortonSet *__restrict__ M, int mb, unsigned *__restrict__ key,
                               int *__restrict__ idx)
 {
@@ -1049,15 +1029,40 @@ __global__ void k_morton_sets(int ns, int tot, const MortonSet *__restrict__ M, 
   }
   const MortonSet &m = M[lo];
   const int i = g - m.off;
+  double p[3];
+  load_xyz(m.xs, i, p);
+  bool inroot = true;
+  for (int d = 0; d < 3; d++) inroot = inroot && (p[d] >= m.ext[2 * d] - TOL);
+  for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= m.ext[2 * d + 1] + TOL);
+  const bool dup = m.xtag && m.xtag[i] != i;
+  if (!inroot && !dup) m.donorId[i] = -1;
   unsigned c[3];
   for (int d = 0; d < 3; d++) {
-    double u = (m.xs[3 * (size_t)i + d] - m.lo[d]) / (m.hi[d] - m.lo[d]);
+    double u = (p[d] - m.ext[2 * d]) / (m.ext[2 * d + 1] - m.ext[2 * d]);
     u = u < 0 ? 0 : (u > 0.999999 ? 0.999999 : u);
     c[d] = (unsigned)(u * 1024.0);
   }
   const unsigned m30 = (spread10(c[0]) << 2) | (spread10(c[1]) << 1) | spread10(c[2]);
-  key[g] = ((unsigned)lo << mb) | (m30 >> (30 - mb));
+  key[g] = ((unsigned)lo << (mb + 1)) | ((inroot && !dup) ? 0u : (1u << mb)) | (m30 >> (30 - mb));
   idx[g] = i;
+}
+
+// nin[k] = the walking queries of set k: in the sorted keys of the set's
+// range they come first, so their count is the first position whose key
+// has the skip bit (a binary search; no atomics)
+__global__ void k_walk_counts(int ns, const MortonSet *__restrict__ M, int mb, const unsigned *__restrict__ key,
+                              int *__restrict__ nin)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ns) return;
+  const unsigned first_skip = ((unsigned)k << (mb + 1)) | (1u << mb);
+  int lo = M[k].off, hi = M[k].off + M[k].nq;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (key[mid] < first_skip) lo = mid + 1;
+    else hi = mid;
+  }
+  nin[k] = lo - M[k].off;
 }
 
 struct SearchView {
@@ -1077,11 +1082,15 @@ struct SearchView {
 // Several searches (one per block, each with its own tree and query set)
 // run by one launch of each kernel.  Queries are numbered globally: search s
 // owns [qbase[s], qbase[s+1]); in k_collect its queries take the waves
-// [wbase[s], wbase[s+1]), so every wave belongs to one search.  qsid maps a
-// global query to its search.
+// [wbase[s], wbase[s+1]), so every wave belongs to one search.  Within a
+// search, position t (processing order) holds query perm[t]; the first
+// nin[s] positions are the queries that walk the tree (k_morton_sets).  The
+// per-query walk results (qcnt, qoff) are indexed by global position
+// qbase[s] + t, candidates carry the global query id qbase[s] + perm[t], and
+// qsid maps either to its search (both ranges are [qbase[s], qbase[s+1])).
 struct SearchSet {
   const SearchView *sv;
-  const int *qbase, *wbase;
+  const int *qbase, *wbase, *nin;
   int *qsid;
   int ns, nqt;
 };
@@ -1202,13 +1211,11 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
   const SearchView S = view_of(T, sid);
   const const_ptr<WalkNode> nodes = (const_ptr<WalkNode>)S.nodes;
   const int qb = T.qbase[sid];
-  const int t = (blockIdx.x - T.wbase[sid]) * WAVE + lane;
-  int i = -1;
-  if (t < S.nq) {
-    i = S.perm ? S.perm[t] : t;
-    if (T.ns > 1) T.qsid[qb + i] = sid;
-    if (S.xtag && S.xtag[i] != i) i = -1;
-  }
+  const int t0 = (blockIdx.x - T.wbase[sid]) * WAVE, t = t0 + lane;
+  if (T.ns > 1 && t < S.nq) T.qsid[qb + t] = sid;
+  const int nin = T.nin[sid];
+  if (t0 >= nin) return;  // the whole wave holds queries that do not walk
+  const int i = t < nin ? S.perm[t] : -1;
   bool inroot = false;
   unsigned q[3] = {0u, 0u, 0u};  // the point on the tree's grid
   if (i >= 0) {
@@ -1285,14 +1292,14 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
   if (lane == 0 && total > 0) base = atomicAdd(O.used, total);
   base = __shfl(base, 0);
   if (i < 0) return;
-  const int g = qb + i;
-  O.qcnt[g] = nc;
+  const int gp = qb + t, g = qb + i;
+  O.qcnt[gp] = nc;
   if (nc > KSTAGE) {
-    O.ovf[atomicAdd(O.novf, 1)] = g;
+    O.ovf[atomicAdd(O.novf, 1)] = gp;
     return;
   }
   const int off = base + pre;
-  O.qoff[g] = off;
+  O.qoff[gp] = off;
   if (off + k > O.cap) {
     // flat list too small (the host sees used > cap, grows it and walks
     // again): the reserved slots below the capacity are marked empty
@@ -1312,14 +1319,15 @@ __global__ void __launch_bounds__(64) k_collect_more(SearchSet T, CandOut O)
   __shared__ int stk[STACK][64];
   const int n = *O.novf, tid = threadIdx.x;
   for (int qi = blockIdx.x * 64 + tid; qi < n; qi += gridDim.x * 64) {
-    const int g = O.ovf[qi];
-    const int sid = T.ns == 1 ? 0 : T.qsid[g];
+    const int gp = O.ovf[qi];
+    const int sid = T.ns == 1 ? 0 : T.qsid[gp];
     const SearchView S = view_of(T, sid);
     const WalkNode *nodes = S.nodes;
-    const int i = g - T.qbase[sid];
-    const int cnt = O.qcnt[g];
+    const int i = S.perm[gp - T.qbase[sid]];
+    const int g = T.qbase[sid] + i;
+    const int cnt = O.qcnt[gp];
     const int off = atomicAdd(O.used, cnt);
-    O.qoff[g] = off;
+    O.qoff[gp] = off;
     if (off + cnt > O.cap) {  // as in k_collect: reserved slots below the capacity marked empty
       for (int j = off; j < O.cap && j < off + cnt; j++) O.cell[j] = EMPTY_SLOT;
       continue;
@@ -1535,16 +1543,18 @@ __global__ void __launch_bounds__(256, 4) k_eval_long(SearchSet T, CandOut O, Lo
   }
 }
 
-// the reference's result from the candidates' outcomes, in walk order
+// the reference's result from the candidates' outcomes, in walk order, for
+// the queries that walked (the others' donors are set elsewhere)
 __global__ void k_select(SearchSet T, CandOut O)
 {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= T.nqt) return;
-  const int sid = T.ns == 1 ? 0 : T.qsid[g];
+  const int gp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gp >= T.nqt) return;
+  const int sid = T.ns == 1 ? 0 : T.qsid[gp];
+  const int t = gp - T.qbase[sid];
+  if (t >= T.nin[sid]) return;
   const SearchView S = view_of(T, sid);
-  const int i = g - T.qbase[sid];
-  if (S.xtag && S.xtag[i] != i) return;
-  const int nc = O.qcnt[g], off = O.qoff[g];
+  const int i = S.perm[t];
+  const int nc = O.qcnt[gp], off = O.qoff[gp];
   int res = -1;
   // a list that did not fit was not stored; the host sees the overflow in
   // the used count and searches again with a larger list
@@ -2298,7 +2308,7 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
   TG_HIP(hipMemcpyAsync(w.sv.p, S, sizeof(SearchView) * ns, hipMemcpyHostToDevice, s));
   TG_HIP(hipMemcpyAsync(w.qb.p, qb.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
   TG_HIP(hipMemcpyAsync(w.wb.p, wb.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
-  const SearchSet T{(const SearchView *)w.sv.p, w.qb.p, w.wb.p, w.qsid.p, ns, nqt};
+  const SearchSet T{(const SearchView *)w.sv.p, w.qb.p, w.wb.p, w.nin.p, w.qsid.p, ns, nqt};
   w.qoff.reserve(nqt);
   w.qcnt.reserve(nqt);
   w.ovf.reserve(nqt);
@@ -2333,13 +2343,26 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     w.lcount.reserve(4);
     fill<int>(w.lcount.p, 4, 0, s);
     LongQueue L{w.lj.p, w.lst.p, w.lcount.p, lcap};
-    k_eval<4, 8><<<2048, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    // the pyramids first: their continued Newton solves (a few thousand
+    // threads, each hundreds of dependent iterations, so bound by latency
+    // and not by the device) run on the side stream while the other cell
+    // types are tested
     k_eval<5, 4><<<2048, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    if (!w.side) {
+      TG_HIP(hipStreamCreateWithFlags(&w.side, hipStreamNonBlocking));
+      TG_HIP(hipEventCreateWithFlags(&w.fork, hipEventDisableTiming));
+      TG_HIP(hipEventCreateWithFlags(&w.join, hipEventDisableTiming));
+    }
+    TG_HIP(hipEventRecord(w.fork, s));
+    TG_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
+    k_eval_long<5><<<1024, 256, 0, w.side>>>(T, O, L);
+    TG_HIP(hipEventRecord(w.join, w.side));
     k_eval<6, 4><<<2048, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
     k_eval<8, 4><<<2048, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
-    k_eval_long<5><<<1024, 256, 0, s>>>(T, O, L);
     k_eval_long<6><<<1024, 256, 0, s>>>(T, O, L);
     k_eval_long<8><<<1024, 256, 0, s>>>(T, O, L);
+    k_eval<4, 8><<<2048, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    TG_HIP(hipStreamWaitEvent(s, w.join, 0));
     k_select<<<nb(nqt), BS, 0, s>>>(T, O);
     return;
   }
@@ -3519,40 +3542,80 @@ int Engine::query_set_create(int btag, const double *xq, long nq)
   return (int)qsets_.size() - 1;
 }
 
-const int *Engine::order_queries(const double *xs, int nq, const double ext[6])
+// Morton keys of the sets M (all queries in one buffer, set k at M[k].off),
+// the walk/skip partition and one sort: returns the processing order (set
+// k's at the same offset) and leaves each set's count of walking queries in
+// sw_.nin; donors of points outside their tree's root box are written.
+const int *Engine::order_sets(const std::vector<MortonSet> &M, size_t tot)
 {
-  mk0_.reserve(nq);
-  mk1_.reserve(nq);
-  mv0_.reserve(nq);
-  mv1_.reserve(nq);
-  k_morton<<<nb(nq), BS, 0, s_>>>(nq, xs, ext[0], ext[1], ext[2], ext[3], ext[4], ext[5], mk0_.p, mv0_.p);
-  sort_pairs(mk0_.p, mk1_.p, mv0_.p, mv1_.p, nq, 30, s_);
+  const int ns = (int)M.size();
+  if (tot > 0x7fffffff) throw std::runtime_error("tioga_amd: too many queries in one search launch");
+  int rb = 0;
+  while ((1 << rb) < ns) rb++;
+  // 7 Morton bits per axis (about one cell of the tree per step) keep the
+  // sort at three 8-bit passes for a few sets
+  const int mb = std::min(21, 31 - rb);
+  if (mb < 12) throw std::runtime_error("tioga_amd: too many query sets in one search launch");
+  mk0_.reserve(tot);
+  mk1_.reserve(tot);
+  mv0_.reserve(tot);
+  mv1_.reserve(tot);
+  sw_.nin.reserve(ns);
+  mset_.reserve(sizeof(MortonSet) * ns);
+  TG_HIP(hipMemcpyAsync(mset_.p, M.data(), sizeof(MortonSet) * ns, hipMemcpyHostToDevice, s_));
+  k_morton_sets<<<nb((long long)tot), BS, 0, s_>>>(ns, (int)tot, (const MortonSet *)mset_.p, mb, mk0_.p, mv0_.p);
+  sort_pairs(mk0_.p, mk1_.p, mv0_.p, mv1_.p, (int)tot, mb + 1 + rb, s_);
+  k_walk_counts<<<nb(ns), BS, 0, s_>>>(ns, (const MortonSet *)mset_.p, mb, mk1_.p, sw_.nin.p);
   return mv1_.p;
 }
 
-void Engine::launch_query_set(QuerySet &q)
+// Searches the query sets qs in one launch of each search kernel (sets of
+// uneven sizes, each with its own tree, side by side); the HIP events of a
+// timed pass bracket the search kernels.
+void Engine::launch_query_sets(const std::vector<QuerySet *> &qs)
 {
-  Block &b = *blocks_[q.blk];
-  if (q.nelem == 0) {  // no cell overlaps the set: every donor -1 (the pair of events still brackets it)
-    if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
-    fill<int>(q.donor.p, q.nq, -1, s_);
-    if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
+  std::vector<QuerySet *> act;
+  std::vector<MortonSet> ms;
+  size_t tot = 0;
+  for (QuerySet *q : qs) {
+    if (q->nelem == 0) {  // no cell overlaps the set: every donor -1
+      fill<int>(q->donor.p, q->nq, -1, s_);
+      continue;
+    }
+    MortonSet m{q->xq.p, nullptr, q->donor.p, {}, (int)tot, q->nq};
+    memcpy(m.ext, q->ext, sizeof m.ext);
+    ms.push_back(m);
+    act.push_back(q);
+    tot += q->nq;
+  }
+  if (act.empty()) {
+    if (kernel_events_) {
+      TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
+      TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
+    }
     return;
   }
-  SearchView S;
-  S.nq = q.nq;
-  S.perm = order_queries(q.xq.p, q.nq, q.ext);
-  S.xs = q.xq.p;
-  S.xtag = nullptr;
-  S.donorId = q.donor.p;
-  S.nodes = q.nodes.p;
-  S.g = grid_frame(q.ext);
-  memcpy(S.ext, q.ext, sizeof S.ext);
-  S.conn = b.conn.p;
-  S.x = b.x.p;
-  S.cellRes = b.cellRes.p;
+  const int *perm = order_sets(ms, tot);
+  std::vector<SearchView> sv;
+  for (size_t k = 0; k < act.size(); k++) {
+    QuerySet &q = *act[k];
+    Block &b = *blocks_[q.blk];
+    SearchView S;
+    S.nq = q.nq;
+    S.perm = perm + ms[k].off;
+    S.xs = q.xq.p;
+    S.xtag = nullptr;
+    S.donorId = q.donor.p;
+    S.nodes = q.nodes.p;
+    S.g = grid_frame(q.ext);
+    memcpy(S.ext, q.ext, sizeof S.ext);
+    S.conn = b.conn.p;
+    S.x = b.x.p;
+    S.cellRes = b.cellRes.p;
+    sv.push_back(S);
+  }
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
-  run_search(&S, 1, sw_, s_, !timing_);
+  run_search(sv.data(), (int)sv.size(), sw_, s_, !timing_);
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
 }
 
@@ -3560,7 +3623,7 @@ void Engine::query_set_search(int id, int *donor)
 {
   if (id < 0 || id >= (int)qsets_.size()) throw std::runtime_error("tioga_amd: bad query set id");
   QuerySet &q = *qsets_[id];
-  launch_query_set(q);
+  launch_query_sets({&q});
   TG_HIP(hipMemcpyAsync(donor, q.donor.p, sizeof(int) * q.nq, hipMemcpyDeviceToHost, s_));
   sync();
 }
@@ -3586,17 +3649,16 @@ double Engine::time_query_sets(int nrep, long long *nq)
   for (QuerySet *q : qsets_) n += q->nq;
   if (nq) *nq = n;
   if (nrep <= 0 || qsets_.empty()) return 0.0;
-  // events around every k_search launch give the kernel's own duration
-  std::vector<hipEvent_t> ev(2 * qsets_.size() * nrep);
+  // events around every pass's search launch give the kernels' own duration
+  std::vector<hipEvent_t> ev(2 * nrep);
   for (auto &e : ev) TG_HIP(hipEventCreate(&e));
-  // establish the candidate-list capacity with one checked pass per set
-  for (QuerySet *q : qsets_) launch_query_set(*q);
+  // establish the candidate-list capacity with one checked pass
+  launch_query_sets(qsets_);
   sync();
   kernel_events_ = &ev;
   timing_ = true;
   Timer tm(s_);
-  for (int r = 0; r < nrep; r++)
-    for (QuerySet *q : qsets_) launch_query_set(*q);
+  for (int r = 0; r < nrep; r++) launch_query_sets(qsets_);
   double pass = tm.stop() / nrep;
   timing_ = false;
   kernel_events_ = nullptr;
@@ -3625,9 +3687,11 @@ void Engine::launch_search(Block &b)
     fill<int>(b.donorId.p, b.nsearch, -1, s_);
     return;
   }
+  MortonSet ms{b.xsearch.p, b.xtag.p, b.donorId.p, {}, 0, b.nsearch};
+  memcpy(ms.ext, b.adtExt, sizeof ms.ext);
   SearchView S;
   S.nq = b.nsearch;
-  S.perm = order_queries(b.xsearch.p, b.nsearch, b.adtExt);
+  S.perm = order_sets({ms}, b.nsearch);
   S.xs = b.xsearch.p;
   S.xtag = b.xtag.p;
   S.donorId = b.donorId.p;
@@ -3657,30 +3721,24 @@ void Engine::launch_searches()
     tot += b->nsearch;
   }
   if (act.empty()) return;
-  if (tot > 0x7fffffff) throw std::runtime_error("tioga_amd: too many queries in one search launch");
-  mk0_.reserve(tot);
-  mk1_.reserve(tot);
-  mv0_.reserve(tot);
-  mv1_.reserve(tot);
-  // Morton order inside each set, all sets in one sort: the set index in the
-  // high key bits, the Morton code truncated to the rest (the order only
+  // Morton order inside each set, all sets in one sort (the order only
   // affects speed, each query's walk is independent)
   const int ns = (int)act.size();
-  int rb = 0;
-  while ((1 << rb) < ns) rb++;
-  const int mb = std::min(30, 32 - rb);
-  if (mb < 12) throw std::runtime_error("tioga_amd: too many query sets in one search launch");
   std::vector<MortonSet> ms(ns);
-  std::vector<SearchView> sv;
   size_t off = 0;
   for (int k = 0; k < ns; k++) {
     Block &b = *act[k];
-    const int nq = b.nsearch;
-    const double *e = b.adtExt;
-    ms[k] = MortonSet{b.xsearch.p, {e[0], e[2], e[4]}, {e[1], e[3], e[5]}, (int)off, nq};
+    ms[k] = MortonSet{b.xsearch.p, b.xtag.p, b.donorId.p, {}, (int)off, b.nsearch};
+    memcpy(ms[k].ext, b.adtExt, sizeof ms[k].ext);
+    off += b.nsearch;
+  }
+  const int *perm = order_sets(ms, tot);
+  std::vector<SearchView> sv;
+  for (int k = 0; k < ns; k++) {
+    Block &b = *act[k];
     SearchView S;
-    S.nq = nq;
-    S.perm = mv1_.p + off;
+    S.nq = b.nsearch;
+    S.perm = perm + ms[k].off;
     S.xs = b.xsearch.p;
     S.xtag = b.xtag.p;
     S.donorId = b.donorId.p;
@@ -3691,12 +3749,7 @@ void Engine::launch_searches()
     S.x = b.x.p;
     S.cellRes = b.cellRes.p;
     sv.push_back(S);
-    off += nq;
   }
-  mset_.reserve(sizeof(MortonSet) * ns);
-  TG_HIP(hipMemcpyAsync(mset_.p, ms.data(), sizeof(MortonSet) * ns, hipMemcpyHostToDevice, s_));
-  k_morton_sets<<<nb(tot), BS, 0, s_>>>(ns, (int)tot, (const MortonSet *)mset_.p, mb, mk0_.p, mv0_.p);
-  sort_pairs(mk0_.p, mk1_.p, mv0_.p, mv1_.p, (int)tot, mb + rb, s_);
   run_search(sv.data(), ns, sw_, s_, true);
   k_copy_dup_sets<<<nb(tot), BS, 0, s_>>>((const SearchView *)sw_.sv.p, sw_.qb.p, ns, (int)tot);
 }
