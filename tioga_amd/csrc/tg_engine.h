@@ -316,7 +316,7 @@ struct StepScratch {
 // Reusable device scratch of the donor search: per-query candidate lists
 // and the queue of queries with more candidates than a list holds.
 struct SearchScratch {
-  DBuf<int> cell, query, status, qoff, qcnt, ovf, counters, bins, blist, lj, lcount, qb, wb, qsid, nin;
+  DBuf<int> cell, query, status, qoff, qcnt, ovf, counters, bins, blist, lj, lcount, qb, qsid, nin, wstart;
   DBuf<unsigned char> sv;  // the table of searches of one launch
   DBuf<double4> lst;
   size_t cap = 0;  // capacity of the flat candidate list

@@ -1017,7 +1017,7 @@ __device__ __forceinline__ unsigned spread10(unsigned v)
 // (searchADTrecursion.C:38-45; written here).  idx = the query's index
 // within its set.
 __global__ void k_morton_sets(int ns, int tot, const MortonSet *__restrict__ M, int mb, unsigned *__restrict__ key,
-                              int *__restrict__ idx)
+                              int *__restrict__ idx, int *__restrict__ qsid)
 {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= tot) return;
@@ -1029,6 +1029,7 @@ __global__ void k_morton_sets(int ns, int tot, const MortonSet *__restrict__ M, 
   }
   const MortonSet &m = M[lo];
   const int i = g - m.off;
+  if (ns > 1) qsid[g] = lo;  // the set of query (and position) g
   double p[3];
   load_xyz(m.xs, i, p);
   bool inroot = true;
@@ -1049,20 +1050,32 @@ __global__ void k_morton_sets(int ns, int tot, const MortonSet *__restrict__ M, 
 
 // nin[k] = the walking queries of set k: in the sorted keys of the set's
 // range they come first, so their count is the first position whose key
-// has the skip bit (a binary search; no atomics)
-__global__ void k_walk_counts(int ns, const MortonSet *__restrict__ M, int mb, const unsigned *__restrict__ key,
-                              int *__restrict__ nin)
+// has the skip bit (a binary search; no atomics).  wstart[k] = the first of
+// set k's walking waves in the enumeration of all sets' walking waves
+// (wstart[ns] = their total).  One workgroup.
+__global__ void __launch_bounds__(256) k_walk_counts(int ns, const MortonSet *__restrict__ M, int mb,
+                                                      const unsigned *__restrict__ key, int *__restrict__ nin,
+                                                      int *__restrict__ wstart)
 {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= ns) return;
-  const unsigned first_skip = ((unsigned)k << (mb + 1)) | (1u << mb);
-  int lo = M[k].off, hi = M[k].off + M[k].nq;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (key[mid] < first_skip) lo = mid + 1;
-    else hi = mid;
+  for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+    const unsigned first_skip = ((unsigned)k << (mb + 1)) | (1u << mb);
+    int lo = M[k].off, hi = M[k].off + M[k].nq;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (key[mid] < first_skip) lo = mid + 1;
+      else hi = mid;
+    }
+    nin[k] = lo - M[k].off;
   }
-  nin[k] = lo - M[k].off;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int w = 0;
+    for (int k = 0; k < ns; k++) {
+      wstart[k] = w;
+      w += (nin[k] + 63) / 64;
+    }
+    wstart[ns] = w;
+  }
 }
 
 struct SearchView {
@@ -1081,16 +1094,16 @@ struct SearchView {
 
 // Several searches (one per block, each with its own tree and query set)
 // run by one launch of each kernel.  Queries are numbered globally: search s
-// owns [qbase[s], qbase[s+1]); in k_collect its queries take the waves
-// [wbase[s], wbase[s+1]), so every wave belongs to one search.  Within a
-// search, position t (processing order) holds query perm[t]; the first
-// nin[s] positions are the queries that walk the tree (k_morton_sets).  The
+// owns [qbase[s], qbase[s+1]).  Within a search, position t (processing
+// order) holds query perm[t]; the first nin[s] positions are the queries
+// that walk the tree (k_morton_sets), in waves of 64: search s's walking
+// waves are [wstart[s], wstart[s+1]) of the enumeration k_collect takes.  The
 // per-query walk results (qcnt, qoff) are indexed by global position
 // qbase[s] + t, candidates carry the global query id qbase[s] + perm[t], and
 // qsid maps either to its search (both ranges are [qbase[s], qbase[s+1])).
 struct SearchSet {
   const SearchView *sv;
-  const int *qbase, *wbase, *nin;
+  const int *qbase, *nin, *wstart;
   int *qsid;
   int ns, nqt;
 };
@@ -1128,13 +1141,13 @@ __device__ __forceinline__ SearchView view_of(const SearchSet &T, int sid)
   return S;
 }
 
-// the search owning wave wv (last s with wbase[s] <= wv)
+// the search owning walking wave wv (last s with wstart[s] <= wv)
 __device__ __forceinline__ int wave_search(const SearchSet &T, int wv)
 {
   int lo = 0, hi = T.ns - 1;
   while (lo < hi) {
     const int m = (lo + hi + 1) >> 1;
-    if (T.wbase[m] <= wv) lo = m;
+    if (T.wstart[m] <= wv) lo = m;
     else hi = m - 1;
   }
   return lo;
@@ -1207,14 +1220,16 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
 {
   __shared__ int stage[KSTAGE][WAVE];
   const int lane = threadIdx.x;
-  const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, blockIdx.x));
+  // workgroup wv walks walking wave wv; the grid is sized for every query,
+  // so the workgroups past the walking waves leave at once
+  const int wv = blockIdx.x;
+  if (wv >= T.wstart[T.ns]) return;
+  const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, wv));
   const SearchView S = view_of(T, sid);
   const const_ptr<WalkNode> nodes = (const_ptr<WalkNode>)S.nodes;
   const int qb = T.qbase[sid];
-  const int t0 = (blockIdx.x - T.wbase[sid]) * WAVE, t = t0 + lane;
-  if (T.ns > 1 && t < S.nq) T.qsid[qb + t] = sid;
+  const int t = (wv - T.wstart[sid]) * WAVE + lane;
   const int nin = T.nin[sid];
-  if (t0 >= nin) return;  // the whole wave holds queries that do not walk
   const int i = t < nin ? S.perm[t] : -1;
   bool inroot = false;
   unsigned q[3] = {0u, 0u, 0u};  // the point on the tree's grid
@@ -1291,24 +1306,25 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
   int base = 0;
   if (lane == 0 && total > 0) base = atomicAdd(O.used, total);
   base = __shfl(base, 0);
-  if (i < 0) return;
-  const int gp = qb + t, g = qb + i;
-  O.qcnt[gp] = nc;
-  if (nc > KSTAGE) {
-    O.ovf[atomicAdd(O.novf, 1)] = gp;
-    return;
-  }
-  const int off = base + pre;
-  O.qoff[gp] = off;
-  if (off + k > O.cap) {
-    // flat list too small (the host sees used > cap, grows it and walks
-    // again): the reserved slots below the capacity are marked empty
-    for (int j = off; j < O.cap && j < off + k; j++) O.cell[j] = EMPTY_SLOT;
-    return;
-  }
-  for (int j = 0; j < k; j++) {
-    O.cell[off + j] = stage[j][lane];
-    O.query[off + j] = g;
+  if (i >= 0) {
+    const int gp = qb + t, g = qb + i;
+    O.qcnt[gp] = nc;
+    if (nc > KSTAGE) {
+      O.ovf[atomicAdd(O.novf, 1)] = gp;
+    } else {
+      const int off = base + pre;
+      O.qoff[gp] = off;
+      if (off + k > O.cap) {
+        // flat list too small (the host sees used > cap, grows it and walks
+        // again): the reserved slots below the capacity are marked empty
+        for (int j = off; j < O.cap && j < off + k; j++) O.cell[j] = EMPTY_SLOT;
+      } else {
+        for (int j = 0; j < k; j++) {
+          O.cell[off + j] = stage[j][lane];
+          O.query[off + j] = g;
+        }
+      }
+    }
   }
 }
 
@@ -2292,27 +2308,29 @@ void fill(T *p, size_t n, T v, hipStream_t s)
 // when it did not fit, so no later kernel reads a list that overflowed
 void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bool check)
 {
-  std::vector<int> qb(ns + 1, 0), wb(ns + 1, 0);
+  std::vector<int> qb(ns + 1, 0);
   for (int k = 0; k < ns; k++) {
     if (S[k].nq < 0) throw std::runtime_error("tioga_amd: negative query count");
     qb[k + 1] = qb[k] + S[k].nq;
-    wb[k + 1] = wb[k] + (S[k].nq + WAVE - 1) / WAVE;
     if (qb[k + 1] < qb[k]) throw std::runtime_error("tioga_amd: too many queries in one search launch");
   }
-  const int nqt = qb[ns], nwave = wb[ns];
+  const int nqt = qb[ns];
   if (nqt <= 0) return;
+  // the processing order, walking counts and the query -> search map come
+  // from the ordering step (Engine::order_sets) of this launch
+  if (w.qsid.cap < (size_t)nqt || w.nin.cap < (size_t)ns || w.wstart.cap < (size_t)ns + 1)
+    throw std::runtime_error("tioga_amd: search launched without its ordering step");
   w.sv.reserve(sizeof(SearchView) * ns);
   w.qb.reserve(ns + 1);
-  w.wb.reserve(ns + 1);
-  w.qsid.reserve(nqt);
   TG_HIP(hipMemcpyAsync(w.sv.p, S, sizeof(SearchView) * ns, hipMemcpyHostToDevice, s));
   TG_HIP(hipMemcpyAsync(w.qb.p, qb.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
-  TG_HIP(hipMemcpyAsync(w.wb.p, wb.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
-  const SearchSet T{(const SearchView *)w.sv.p, w.qb.p, w.wb.p, w.nin.p, w.qsid.p, ns, nqt};
   w.qoff.reserve(nqt);
   w.qcnt.reserve(nqt);
   w.ovf.reserve(nqt);
   w.counters.reserve(2);
+  const SearchSet T{(const SearchView *)w.sv.p, w.qb.p, w.nin.p, w.wstart.p, w.qsid.p, ns, nqt};
+  long long nwave = 0;  // an upper bound of the walking waves
+  for (int k = 0; k < ns; k++) nwave += (S[k].nq + WAVE - 1) / WAVE;
   if (w.cap == 0) w.cap = std::max<size_t>(4 * (size_t)nqt, 1 << 20);
   for (;;) {
     w.cell.reserve(w.cap);
@@ -2321,7 +2339,7 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     CandOut O{w.cell.p, w.query.p, w.status.p, w.qoff.p, w.qcnt.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
               w.ovf.p, w.counters.p + 1};
     fill<int>(w.counters.p, 2, 0, s);
-    k_collect<<<nwave, WAVE, 0, s>>>(T, O);
+    k_collect<<<(unsigned)nwave, WAVE, 0, s>>>(T, O);
     k_collect_more<<<1024, 64, 0, s>>>(T, O);
     if (check) {
       int used = 0;
@@ -3563,9 +3581,12 @@ const int *Engine::order_sets(const std::vector<MortonSet> &M, size_t tot)
   sw_.nin.reserve(ns);
   mset_.reserve(sizeof(MortonSet) * ns);
   TG_HIP(hipMemcpyAsync(mset_.p, M.data(), sizeof(MortonSet) * ns, hipMemcpyHostToDevice, s_));
-  k_morton_sets<<<nb((long long)tot), BS, 0, s_>>>(ns, (int)tot, (const MortonSet *)mset_.p, mb, mk0_.p, mv0_.p);
+  sw_.wstart.reserve(ns + 1);
+  sw_.qsid.reserve(tot);
+  k_morton_sets<<<nb((long long)tot), BS, 0, s_>>>(ns, (int)tot, (const MortonSet *)mset_.p, mb, mk0_.p, mv0_.p,
+                                                   sw_.qsid.p);
   sort_pairs(mk0_.p, mk1_.p, mv0_.p, mv1_.p, (int)tot, mb + 1 + rb, s_);
-  k_walk_counts<<<nb(ns), BS, 0, s_>>>(ns, (const MortonSet *)mset_.p, mb, mk1_.p, sw_.nin.p);
+  k_walk_counts<<<1, 256, 0, s_>>>(ns, (const MortonSet *)mset_.p, mb, mk1_.p, sw_.nin.p, sw_.wstart.p);
   return mv1_.p;
 }
 
