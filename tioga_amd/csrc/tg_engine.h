@@ -194,6 +194,7 @@ struct Block {
   DBuf<int> interp_inode, interp_nw, interp_cancel, interp_rnode, interp_rblk, interp2donor;
   DBuf<double> interp_w;
   std::vector<InterpHost> interp_host;
+  DBuf<unsigned char> upd_mark;  // dataUpdate's donor-vertex marks
   bool interp_host_valid = false;
 };
 
@@ -335,6 +336,26 @@ struct SearchScratch {
   }
 };
 
+// dataUpdate (tg_update.hip): the interpolation records of one donor block
+// and the thread range [start, start + n) they take in a launch (start a
+// multiple of the workgroup size); voff = their first row of staged values.
+struct UpdSeg {
+  const int *inode;
+  const double *w;
+  const int *nw, *cancel, *rnode, *rblk;
+  int n, start, blk, voff;
+};
+constexpr int UPD_MAXBLK = 128;
+struct UpdTable {
+  double *q[UPD_MAXBLK];  // each block's solution (device)
+  int nnodes[UPD_MAXBLK];
+  int nseg;
+  UpdSeg seg[UPD_MAXBLK];
+};
+// tiogaInterface.C:178-190: 0 for an itype containing "row", 1 for
+// "column", -1 otherwise
+int update_interptype(const char *itype);
+
 // The query sets of a search launch, for their Morton keys in one launch:
 // set k's queries are [off, off + nq) of the concatenation.  ext: the root
 // box of the set's tree (adtExtents order), which frames the Morton code and
@@ -440,6 +461,7 @@ class Engine {
   // Re-run the dataUpdate gather/scatter on device-resident q `nrep` times
   // (q uploaded once from the registered arrays); mean device ms per pass.
   double time_data_update(int nvar, int interptype, int nrep, long long *nreceptors);
+
 
   // Query sets (extension): create returns an id; search runs the kernel
   // and copies donors out; time_query_sets times nrep passes over all sets.
@@ -564,6 +586,18 @@ class Engine {
   double prep_acc_[4] = {0, 0, 0, 0};
   long long obb_stepped_ = 0;
   std::vector<DBuf<double> *> qdev_;
+  // dataUpdate's plan for the current interpolation records (tg_update.hip)
+  struct UpdatePlan {
+    UpdTable host{};
+    DBuf<unsigned char> tab;      // the table on the device
+    DBuf<unsigned char *> marks;  // per block: donor-vertex marks
+    DBuf<int> flag;
+    DBuf<double> vals;            // staged values (two-kernel form)
+    long long nthreads = 0;
+    int nrec = 0;
+    bool valid = false, staged = false;
+  } upd_;
+  void prepare_update();
   DBuf<int> qlist_;  // query positions of every pair (exchange_search_data)
   // AMR Cartesian background (tg_amr.hip)
   AmrState *amr_ = nullptr;

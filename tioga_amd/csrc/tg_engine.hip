@@ -2242,53 +2242,6 @@ __global__ void k_cell_iblank_all(int ns, const CellIbSeg *__restrict__ T)
 // Records are stored structure-of-arrays (slot m of record r at m*n + r), so
 // a wave's index and weight loads are coalesced; the sum over the weights
 // runs in slot order m = 0..nw-1 as the reference sums it.
-__global__ void k_gather(int n, int nvar, int interptype, int nnodes, const int *__restrict__ inode,
-                         const double *__restrict__ w, const int *__restrict__ nw, const int *__restrict__ cancel,
-                         const double *__restrict__ q, double *__restrict__ out)
-{
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n || cancel[r]) return;
-  const int m_n = nw[r];
-  int v[8];
-  double wt[8];
-#pragma unroll
-  for (int m = 0; m < 8; m++)
-    if (m < m_n) {
-      v[m] = inode[(size_t)m * n + r];
-      wt[m] = w[(size_t)m * n + r];
-    }
-  for (int k = 0; k < nvar; k++) {
-    double acc = 0;
-#pragma unroll
-    for (int m = 0; m < 8; m++)
-      if (m < m_n) {
-        double qv = interptype == 0 ? q[(size_t)v[m] * nvar + k] : q[(size_t)k * nnodes + v[m]];
-        acc += qv * wt[m];
-      }
-    out[(size_t)r * nvar + k] = acc;
-  }
-}
-
-struct QTables {
-  double *q[128];
-  int nnodes[128];
-};
-
-__global__ void k_scatter(int n, int nvar, int interptype, const int *__restrict__ cancel,
-                          const int *__restrict__ rnode, const int *__restrict__ rblk, const double *__restrict__ vals,
-                          const QTables *__restrict__ T)
-{
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n || cancel[r]) return;
-  int b = rblk[r], nd = rnode[r];
-  double *q = T->q[b];
-  for (int k = 0; k < nvar; k++) {
-    double v = vals[(size_t)r * nvar + k];
-    if (interptype == 0) q[(size_t)nd * nvar + k] = v;
-    else q[(size_t)k * T->nnodes[b] + nd] = v;
-  }
-}
-
 template <class T>
 __global__ void k_fill(size_t n, T *p, T v)
 {
@@ -2489,6 +2442,7 @@ void Engine::register_grid(int btag, int nnodes, double *xyz, int *ibl, int nwbc
   b->topo_uploaded = false;
   b->unres = b->ucres = nullptr;
   connected_ = false;
+  upd_.valid = false;
 }
 
 void Engine::set_gids(int btag, uint64_t *cell_gid, uint64_t *node_gid)
@@ -2758,6 +2712,7 @@ void Engine::adopt_block(const Engine &src, int src_tag, int as_tag, int share, 
   b->ninterp = 0;
   b->interp_host_valid = false;
   connected_ = false;
+  upd_.valid = false;
 }
 
 __global__ void k_nonunit(int n, const int *__restrict__ ib, int *__restrict__ flag)
@@ -4219,6 +4174,7 @@ void Engine::perform_connectivity()
   t.obb_stepped = obb_stepped_;
   last_times = t;
   connected_ = true;
+  prepare_update();
 }
 
 // tioga::reduce_fringes (tioga.C:737-885), one rank: each block's fringe
@@ -4295,52 +4251,7 @@ void Engine::reduce_fringes()
     delete ibr[ib];
     delete cancelv[ib];
   }
-}
-
-// tioga::dataUpdate (tioga.C:336-524), at_points = 0
-void Engine::data_update(int nvar, int interptype)
-{
-  for (Block *b : blocks_)
-    if (b->uq == nullptr) return;
-  if (nsend_ == 0 || nvar <= 0) return;
-  const int nbk = (int)blocks_.size();
-  while ((int)qdev_.size() < nbk) qdev_.push_back(new DBuf<double>);
-  QTables qt{};
-  size_t tot = 0;
-  for (int ib = 0; ib < nbk; ib++) {
-    Block &b = *blocks_[ib];
-    size_t n = (size_t)b.nnodes * nvar;
-    qdev_[ib]->reserve(n);
-    TG_HIP(hipMemcpyAsync(qdev_[ib]->p, b.uq, sizeof(double) * n, hipMemcpyHostToDevice, s_));
-    qt.q[ib] = qdev_[ib]->p;
-    qt.nnodes[ib] = b.nnodes;
-    tot += (size_t)b.ninterp * nvar;
-  }
-  DBuf<double> vals;
-  vals.reserve(tot);
-  for (int ib = 0, off = 0; ib < nbk; ib++) {
-    Block &b = *blocks_[ib];
-    if (b.ninterp)
-      k_gather<<<nb(b.ninterp), BS, 0, s_>>>(b.ninterp, nvar, interptype, b.nnodes, b.interp_inode.p, b.interp_w.p,
-                                             b.interp_nw.p, b.interp_cancel.p, qdev_[ib]->p,
-                                             vals.p + (size_t)off * nvar);
-    off += b.ninterp;
-  }
-  DevStruct<QTables> dq;
-  QTables *pq = dq.put(qt, s_);
-  for (int ib = 0, off = 0; ib < nbk; ib++) {
-    Block &b = *blocks_[ib];
-    if (b.ninterp)
-      k_scatter<<<nb(b.ninterp), BS, 0, s_>>>(b.ninterp, nvar, interptype, b.interp_cancel.p, b.interp_rnode.p,
-                                              b.interp_rblk.p, vals.p + (size_t)off * nvar, pq);
-    off += b.ninterp;
-  }
-  for (int ib = 0; ib < nbk; ib++) {
-    Block &b = *blocks_[ib];
-    TG_HIP(hipMemcpyAsync(b.uq, qdev_[ib]->p, sizeof(double) * (size_t)b.nnodes * nvar, hipMemcpyDeviceToHost,
-                          s_));
-  }
-  sync();
+  prepare_update();
 }
 
 static void fetch_interp(Block &b, hipStream_t s)
@@ -4512,51 +4423,5 @@ double Engine::time_search(int nrep, long long *nsearch)
   return tm.stop() / nrep;
 }
 
-double Engine::time_data_update(int nvar, int interptype, int nrep, long long *nrecv)
-{
-  const int nbk = (int)blocks_.size();
-  long long nr = 0;
-  for (Block *b : blocks_) nr += b->ninterp;
-  if (nrecv) *nrecv = nr;
-  for (Block *b : blocks_)
-    if (b->uq == nullptr) return 0.0;
-  if (nsend_ == 0 || nvar <= 0 || nrep <= 0) return 0.0;
-  while ((int)qdev_.size() < nbk) qdev_.push_back(new DBuf<double>);
-  QTables qt{};
-  size_t tot = 0;
-  for (int ib = 0; ib < nbk; ib++) {
-    Block &b = *blocks_[ib];
-    size_t n = (size_t)b.nnodes * nvar;
-    qdev_[ib]->reserve(n);
-    TG_HIP(hipMemcpyAsync(qdev_[ib]->p, b.uq, sizeof(double) * n, hipMemcpyHostToDevice, s_));
-    qt.q[ib] = qdev_[ib]->p;
-    qt.nnodes[ib] = b.nnodes;
-    tot += (size_t)b.ninterp * nvar;
-  }
-  DBuf<double> vals;
-  vals.reserve(tot);
-  DevStruct<QTables> dq;
-  QTables *pq = dq.put(qt, s_);
-  sync();
-  Timer tm(s_);
-  for (int rep = 0; rep < nrep; rep++) {
-    for (int ib = 0, off = 0; ib < nbk; ib++) {
-      Block &b = *blocks_[ib];
-      if (b.ninterp)
-        k_gather<<<nb(b.ninterp), BS, 0, s_>>>(b.ninterp, nvar, interptype, b.nnodes, b.interp_inode.p,
-                                               b.interp_w.p, b.interp_nw.p, b.interp_cancel.p, qdev_[ib]->p,
-                                               vals.p + (size_t)off * nvar);
-      off += b.ninterp;
-    }
-    for (int ib = 0, off = 0; ib < nbk; ib++) {
-      Block &b = *blocks_[ib];
-      if (b.ninterp)
-        k_scatter<<<nb(b.ninterp), BS, 0, s_>>>(b.ninterp, nvar, interptype, b.interp_cancel.p, b.interp_rnode.p,
-                                                b.interp_rblk.p, vals.p + (size_t)off * nvar, pq);
-      off += b.ninterp;
-    }
-  }
-  return tm.stop() / nrep;
-}
 
 }  // namespace tg
