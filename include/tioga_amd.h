@@ -238,6 +238,22 @@ int tioga_amd_device_unique_nodes(int nset, const int *n, const double *x, const
    Returns 0, or -1 with the error status set.  Needs a GPU. */
 int tioga_amd_device_find_obbs(int nset, const int *n, const double *x, double *out, long long *stepped);
 
+/* dataUpdate without device allocation -- no counterpart in the reference.
+   For callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc): the same
+   update as tioga_dataupdate_mb_ (itype "row" or "column", nvar fields),
+   applied in place to the caller's device arrays q_dev[b] (the solution of
+   the b-th registered block, in registration order, each
+   nnodes(b) * nvar doubles).  Its staging (the per-block table and, when
+   some receptor is also a donor vertex, the staged values) lives in one
+   fixed device array of tioga_amd_dataupdate_noalloc_max_bytes() bytes
+   (64 MiB) that each GPU holds until the process ends; an update that needs
+   more is rejected with an error.  Calls that overlap on one GPU share that
+   array and race.  Synchronous; returns 0, or -1 with the error status set.
+   Needs a performconnectivity first. */
+int tioga_amd_dataupdate_mb_noalloc(int *nvar, char *itype, double **q_dev);
+long tioga_amd_dataupdate_noalloc_max_bytes(void);
+
 /* Donor weights outside [-0.2, 1.2] seen (the reference aborts on one). */
 int tioga_amd_weight_aborts(void);
 

@@ -236,6 +236,22 @@ void tioga_dataupdate_mb_(int *nvar, char *itype)
   });
 }
 
+int tioga_amd_dataupdate_mb_noalloc(int *nvar, char *itype, double **q_dev)
+{
+  int rc = -1;
+  guarded("tioga_amd_dataupdate_mb_noalloc", [&] {
+    const int t = tg::update_interptype(itype);
+    if (t < 0) throw std::runtime_error("tioga_amd: dataUpdate: unknown data orientation (itype)");
+    if (tg_or_throw().iamrGlobal())
+      throw std::runtime_error("tioga_amd: the allocation-free dataUpdate serves the unstructured blocks only");
+    tg_or_throw().engine()->data_update_noalloc(*nvar, t, q_dev);
+    rc = 0;
+  });
+  return rc;
+}
+
+long tioga_amd_dataupdate_noalloc_max_bytes(void) { return (long)tg::update_scratch_bytes(); }
+
 void tioga_dataupdate_(double *q, int *nvar, char *itype)
 {
   guarded("tioga_dataupdate", [&] {

@@ -352,6 +352,8 @@ struct UpdTable {
   int nseg;
   UpdSeg seg[UPD_MAXBLK];
 };
+// the staging bytes of the allocation-free dataUpdate (its fixed maximum)
+size_t update_scratch_bytes();
 // tiogaInterface.C:178-190: 0 for an itype containing "row", 1 for
 // "column", -1 otherwise
 int update_interptype(const char *itype);
@@ -461,6 +463,9 @@ class Engine {
   // Re-run the dataUpdate gather/scatter on device-resident q `nrep` times
   // (q uploaded once from the registered arrays); mean device ms per pass.
   double time_data_update(int nvar, int interptype, int nrep, long long *nreceptors);
+  // dataUpdate on the caller's device solutions qdev[block] (registration
+  // order), staging in a fixed device array: allocates no device memory
+  void data_update_noalloc(int nvar, int interptype, double *const *qdev);
 
 
   // Query sets (extension): create returns an id; search runs the kernel

@@ -18,10 +18,16 @@
 // block comes from a table of per-block segments whose thread ranges start
 // at multiples of the workgroup size (a wave-uniform binary search).
 //
-// Engine::data_update (tioga_dataupdate_mb_) copies the registered host
-// arrays to device buffers that the engine keeps, updates them and copies
-// them back; the segment table and the staging buffer are the engine's too,
-// grown on demand and kept, so repeated calls allocate nothing new.
+// Two entry points share the kernels:
+//   * Engine::data_update (tioga_dataupdate_mb_) copies the registered host
+//     arrays to device buffers that the engine keeps, updates them and
+//     copies them back; the segment table and the staging buffer are the
+//     engine's too, grown on demand and kept, so repeated calls allocate
+//     nothing new;
+//   * Engine::data_update_noalloc (tioga_amd_dataupdate_mb_noalloc) updates
+//     the caller's device arrays in place, with the segment table and the
+//     staged values in `scratch`, a file-scope device array of 64 MiB; it
+//     allocates no device memory.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,12 +38,46 @@
 
 #include "tg_engine.h"
 
+// The staging array of the allocation-free dataUpdate (see below).  In host
+// code its name is only a handle for hipGetSymbolAddress; kernels get the
+// device address that call returns.
+static __device__ __align__(256) unsigned char scratch[size_t(64) << 20];
+
 namespace tg {
 
 namespace {
 
 constexpr int UBS = 256;
 inline unsigned unb(long long n) { return (unsigned)((n + UBS - 1) / UBS); }
+
+// The staging area of the allocation-free dataUpdate: the segment table,
+// then the staged values of the two-kernel form.  Each GPU the library runs
+// on holds its own copy until the process ends.  Calls that overlap on one
+// GPU share it and race; the table is written at the start of every call
+// and the values before they are read, so nothing needs clearing.
+constexpr size_t UPDATE_SCRATCH_BYTES = sizeof(::scratch);
+static_assert(UPDATE_SCRATCH_BYTES == size_t(64) << 20, "dataUpdate staging is 64 MiB");
+
+constexpr int MAXDEV = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAXDEV] = {};
+
+// the device address of this device's `scratch` (looked up once per device,
+// with that device current)
+void *scratch_address()
+{
+  int dev = 0;
+  TG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAXDEV) throw std::runtime_error("tioga_amd: device index out of range for dataUpdate staging");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    TG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+    if (!p) throw std::runtime_error("tioga_amd: no device address for the dataUpdate staging array");
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
 
 // the segment holding thread g (segments start at multiples of the
 // workgroup size, so it is the same for a whole workgroup)
@@ -297,6 +337,34 @@ double Engine::time_data_update(int nvar, int interptype, int nrep, long long *n
   (void)hipEventDestroy(b);
   return ms / nrep;
 }
+
+// tioga::dataUpdate on the caller's device arrays qdev[ib] (block ib in
+// registration order), with every staging buffer in `scratch`: no device
+// memory is allocated.  Synchronous, like the host-array form.
+void Engine::data_update_noalloc(int nvar, int interptype, double *const *qdev)
+{
+  if (!connected_ || !upd_.valid) throw std::runtime_error("tioga_amd: dataUpdate needs performconnectivity first");
+  if (nvar <= 0) throw std::runtime_error("tioga_amd: dataUpdate needs nvar >= 1");
+  if (interptype != 0 && interptype != 1) throw std::runtime_error("tioga_amd: dataUpdate: unknown interpolation type");
+  const int nbk = (int)blocks_.size();
+  if (nsend_ == 0 || upd_.host.nseg == 0) return;
+  for (int ib = 0; ib < nbk; ib++)
+    if (!qdev || !qdev[ib]) throw std::runtime_error("tioga_amd: dataUpdate needs a device solution for every block");
+  const size_t tab_bytes = (sizeof(UpdTable) + 255) / 256 * 256;
+  const size_t need = tab_bytes + (upd_.staged ? sizeof(double) * (size_t)upd_.nrec * nvar : 0);
+  if (need > UPDATE_SCRATCH_BYTES)
+    throw std::runtime_error("tioga_amd: dataUpdate without allocation needs " + std::to_string(need) +
+                             " bytes of staging, more than its " + std::to_string(UPDATE_SCRATCH_BYTES));
+  unsigned char *base = (unsigned char *)scratch_address();
+  UpdTable T = upd_.host;
+  for (int ib = 0; ib < nbk; ib++) T.q[ib] = qdev[ib];
+  TG_HIP(hipMemcpyAsync(base, &T, sizeof T, hipMemcpyHostToDevice, s_));
+  launch_update((const UpdTable *)base, upd_.nthreads, upd_.staged, nvar, interptype, (double *)(base + tab_bytes),
+                s_);
+  sync();
+}
+
+size_t update_scratch_bytes() { return UPDATE_SCRATCH_BYTES; }
 
 int update_interptype(const char *itype) { return parse_itype(itype); }
 
