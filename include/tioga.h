@@ -41,7 +41,8 @@ class tioga {
   }
 
   // tioga.C:64-86; node_gid selects the global-id coincident-point dedupe
-  // of search.C:48-74 (cell_gid is accepted and unused, as in the reference)
+  // of search.C:48-74 (when every block has one), and cell_gid is what
+  // getReceptorInfo reports for each donor cell
   void registerGridData(int btag, int nnodes, double *xyz, int *ibl, int nwbc, int nobc, int *wbcnode,
                         int *obcnode, int ntypes, int *nv, int *nc, int **vconn, uint64_t *cell_gid = NULL,
                         uint64_t *node_gid = NULL);

@@ -22,6 +22,8 @@
 #ifndef TIOGA_AMD_H
 #define TIOGA_AMD_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -253,6 +255,18 @@ int tioga_amd_device_find_obbs(int nset, const int *n, const double *x, double *
    Needs a performconnectivity first. */
 int tioga_amd_dataupdate_mb_noalloc(int *nvar, char *itype, double **q_dev);
 long tioga_amd_dataupdate_noalloc_max_bytes(void);
+
+/* Global ids of block btag, as the C++ registerGridData(..., cell_gid,
+   node_gid) takes them (tioga.h:121-123; kept by pointer, must stay valid).
+   When every block has node ids, coincident queries are deduped by node id
+   (search.C:48-74, the reference's TIOGA_HAS_NODEGID build) instead of by
+   position.  Call after registering the block. */
+void tioga_amd_register_gids(int btag, uint64_t *cell_gid, uint64_t *node_gid);
+/* tioga::getReceptorInfo (tioga.C:572-671): per live receptor (node, mesh
+   tag of its block, the donor cell's 64-bit global id as two ints in memory
+   order; the cell index when the block has no cell ids).  Returns the
+   number of ints (-1 on error) and copies them when out != NULL. */
+long tioga_amd_receptor_info(int *out);
 
 /* Donor weights outside [-0.2, 1.2] seen (the reference aborts on one). */
 int tioga_amd_weight_aborts(void);

@@ -23,6 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+REF_DRIVER_GID = os.path.join(HERE, "_ref", "gid", "ref_driver")  # built with TIOGA_HAS_NODEGID
 
 _P = C.c_void_p
 _lib = None
@@ -60,6 +61,10 @@ def lib():
         L.orc_search_points.restype = C.c_double
         L.orc_search_points_obb.argtypes = [_P, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int, _P]
         L.orc_search_points_obb.restype = C.c_double
+        L.orc_set_gids.argtypes = [_P, C.c_int, _P, _P]
+        L.orc_unique_nodes_gid.argtypes = [_P, _P, _P, C.c_int]
+        L.orc_get_receptor_info.argtypes = [_P, _P]
+        L.orc_get_receptor_info.restype = C.c_long
         L.orc_nodal_weights.argtypes = [_P, _P, _P, C.c_int]
         L.orc_cell_volume.argtypes = [_P, C.c_int]
         L.orc_cell_volume.restype = C.c_double
@@ -121,6 +126,8 @@ class Oracle:
             if b.nres is not None:
                 self.L.orc_set_resolutions(self.h, b.tag, ptr(b.nres), ptr(b.cres))
             self.L.orc_set_cell_iblank(self.h, b.tag, ptr(ibc))
+            if b.cell_gid is not None or b.node_gid is not None:
+                self.L.orc_set_gids(self.h, b.tag, ptr(b.cell_gid), ptr(b.node_gid))
             self.iblank.append(ib)
             self.iblank_cell.append(ibc)
 
@@ -185,6 +192,13 @@ class Oracle:
         t = self.L.orc_search_points_obb(self.h, btag, ptr(xobb), xobb.shape[0], ptr(xq), xq.shape[0], ptr(donor),
                                          int(nthreads), C.byref(ne))
         return donor, int(ne.value), float(t)
+
+    def receptor_info(self) -> np.ndarray:
+        n = self.L.orc_get_receptor_info(self.h, None)
+        out = np.zeros(max(n, 0), np.int32)
+        if n > 0:
+            self.L.orc_get_receptor_info(self.h, ptr(out))
+        return out
 
     def hole_map(self, mtag: int):
         nx = np.zeros(3, np.int32)
@@ -277,6 +291,8 @@ class Oracle:
             r[f"hm{m}.existWall"] = np.array([0 if hm is None else 1], np.int32)
             if hm is not None:
                 r[f"hm{m}.nx"], r[f"hm{m}.extents"], r[f"hm{m}.sam"] = hm
+        if any(b.cell_gid is not None for b in self.blocks):
+            r["receptor_info"] = self.receptor_info()
         if qs is not None:
             for i, q in enumerate(self.data_update(qs, nvar, interptype)):
                 r[f"b{i}.q"] = q
@@ -293,7 +309,8 @@ def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptyp
     """amr = (AmrGrid, patch fields per local patch) selects the AMR gather
     mode of the driver (format version 2); reduce runs reduce_fringes after
     each step's connectivity (format version 3)."""
-    version = 3 if reduce else (1 if amr is None else 2)
+    gids = any(b.cell_gid is not None or b.node_gid is not None for b in blocks)
+    version = 4 if gids else (3 if reduce else (1 if amr is None else 2))
     with open(path, "wb") as f:
         w = lambda fmt, *v: f.write(struct.pack("<" + fmt, *v))
         w("9i", 0x41474954, version, len(blocks), isym, nfringe, mexclude, nvar, interptype,
@@ -327,6 +344,12 @@ def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptyp
                 f.write(np.ascontiguousarray(q, np.float64).tobytes())
         if version >= 3:
             w("i", int(reduce))
+        if version >= 4:
+            w("i", 1)
+            for b in blocks:
+                b = b.normalized()
+                f.write(b.cell_gid.tobytes())
+                f.write(b.node_gid.tobytes())
         for step in moves:
             for xb in step:
                 if xb is None:
@@ -361,6 +384,9 @@ def run_reference(case_path: str, out_path: str, timeout: float = 600.0,
     (in cwd: the reference writes a cartGrid.dat there for AMR cases)."""
     env = dict(os.environ)
     env.setdefault("HYDRA_LAUNCHER", "fork")
-    subprocess.run([REF_DRIVER, case_path, out_path], check=True, timeout=timeout, env=env, cwd=cwd,
+    with open(case_path, "rb") as f:
+        version = struct.unpack("<2i", f.read(8))[1]
+    drv = REF_DRIVER_GID if version >= 4 else REF_DRIVER
+    subprocess.run([drv, case_path, out_path], check=True, timeout=timeout, env=env, cwd=cwd,
                    stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     return read_result(out_path)

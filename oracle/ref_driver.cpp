@@ -30,6 +30,12 @@
 //   after its dataUpdate with tioga::reduce_fringes (tioga.C:737-885), the
 //   full dump again and a dataUpdate on a fresh copy of q, all under the
 //   name prefix "rf.".
+//   version 4 adds, after reduce: int32 has_gids; if set, per block:
+//              uint64 cell_gid[ncells], uint64 node_gid[nnodes], passed to
+//              registerGridData (tioga.h:121-123), and every dump adds
+//              "receptor_info" (tioga::getReceptorInfo, tioga.C:572-671).
+//              Cases with node ids need the gid build of this driver
+//              (-DTIOGA_HAS_NODEGID: the dedupe of search.C:48-74).
 // Result file: records {int32 namelen, name, int32 dtype(0=i32,1=f64),
 //              int64 count, payload}.  Step s > 0 prefixes names with "s<s>.".
 #include <cstdint>
@@ -80,6 +86,7 @@ struct Writer {
 
 struct Block {
   int tag, nnodes, nwbc, nobc, ntypes, has_res, ncells;
+  std::vector<uint64_t> cell_gid, node_gid;
   std::vector<double> x, nres, cres, q;
   std::vector<int> wbc, obc, iblank, iblank_cell;
   std::vector<int> nv, nc;
@@ -87,9 +94,14 @@ struct Block {
   std::vector<int *> vconn;
 };
 
-void dump(tioga *tg, std::vector<Block> &blocks, Writer &w, double tconn)
+void dump(tioga *tg, std::vector<Block> &blocks, Writer &w, double tconn, bool gids)
 {
   w.f64("time_connectivity", &tconn, 1);
+  if (gids) {
+    std::vector<int> ri;
+    tg->getReceptorInfo(ri);
+    w.i32("receptor_info", ri.data(), (int64_t)ri.size());
+  }
   for (size_t b = 0; b < blocks.size(); b++) {
     Block &B = blocks[b];
     auto it = tg->tag_iblk_map.find(B.tag);
@@ -247,7 +259,7 @@ int main(int argc, char **argv)
   if (!r.f) { perror(argv[1]); return 2; }
   if (r.get<int32_t>() != 0x41474954) { fprintf(stderr, "bad case\n"); return 2; }
   const int version = r.get<int32_t>();
-  if (version < 1 || version > 3) { fprintf(stderr, "bad case version\n"); return 2; }
+  if (version < 1 || version > 4) { fprintf(stderr, "bad case version\n"); return 2; }
   int nblocks = r.get<int32_t>(), isym = r.get<int32_t>(), nfringe = r.get<int32_t>();
   int mexclude = r.get<int32_t>(), nvar = r.get<int32_t>(), itype = r.get<int32_t>();
   int nsteps = r.get<int32_t>();
@@ -291,6 +303,12 @@ int main(int argc, char **argv)
     }
   }
   const int reduce = version >= 3 ? r.get<int32_t>() : 0;
+  const int has_gids = version >= 4 ? r.get<int32_t>() : 0;
+  if (has_gids)
+    for (auto &B : blocks) {
+      r.arr(B.cell_gid, B.ncells);
+      r.arr(B.node_gid, B.nnodes);
+    }
   Writer w{fopen(argv[2], "wb"), ""};
   tioga *tg = new tioga;
   tg->setCommunicator(MPI_COMM_WORLD, 0, 1);
@@ -299,7 +317,8 @@ int main(int argc, char **argv)
   tg->setMexclude(&mexclude);
   for (auto &B : blocks) {
     tg->registerGridData(B.tag, B.nnodes, B.x.data(), B.iblank.data(), B.nwbc, B.nobc, B.wbc.data(),
-                         B.obc.data(), B.ntypes, B.nv.data(), B.nc.data(), B.vconn.data());
+                         B.obc.data(), B.ntypes, B.nv.data(), B.nc.data(), B.vconn.data(),
+                         has_gids ? B.cell_gid.data() : NULL, has_gids ? B.node_gid.data() : NULL);
     if (B.has_res) tg->setResolutions(B.tag, B.nres.data(), B.cres.data());
     tg->set_cell_iblank(B.tag, B.iblank_cell.data());
   }
@@ -323,7 +342,7 @@ int main(int argc, char **argv)
     double t0 = MPI_Wtime();
     tg->performConnectivity();
     double tconn = MPI_Wtime() - t0;
-    dump(tg, blocks, w, tconn);
+    dump(tg, blocks, w, tconn, has_gids != 0);
     // dataUpdate on a fresh copy of q so every step starts from the same field
     std::vector<std::vector<double>> qs(blocks.size());
     for (size_t b = 0; b < blocks.size(); b++) {
@@ -337,7 +356,7 @@ int main(int argc, char **argv)
       const std::string p0 = w.prefix;
       tg->reduce_fringes();
       w.prefix = p0 + "rf.";
-      dump(tg, blocks, w, 0.0);
+      dump(tg, blocks, w, 0.0, has_gids != 0);
       for (size_t b = 0; b < blocks.size(); b++) {
         qs[b] = blocks[b].q;
         tg->registerSolution(blocks[b].tag, qs[b].data());

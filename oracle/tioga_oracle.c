@@ -770,6 +770,8 @@ typedef struct {
   int *cancelList;
   int ncancel, cancelCap;
   double *q;
+  /* global ids (registerGridData's cell_gid / node_gid, tioga.h:121-123) */
+  const unsigned long long *cell_gid, *node_gid;
   /* receptors of the AMR Cartesian background (cartOps.C:61-160) */
   int ncart;
   int *receptorIdCart, *donorIdCart;
@@ -1571,7 +1573,47 @@ static void build_block_adt(OrcBlock *b, const double *xq, int nq)
 }
 
 /* search.C:78-269 (NODEGID off: octree dedupe) */
-static void block_search(OrcBlock *b)
+/* search.C:48-74 (uniquenode_map): the first query carrying each global id
+   represents all that carry it; the representative's resolution becomes
+   the max over its copies and every copy takes it.  First occurrence in
+   query order, found by an open-addressing table of the ids (any exact set
+   lookup gives the same first occurrences). */
+void orc_unique_nodes_gid(const unsigned long long *gid, double *res, int *itag, int n)
+{
+  int i;
+  size_t cap = 16;
+  while (cap < 2 * (size_t)(n > 0 ? n : 1)) cap <<= 1;
+  int *slot = (int *)malloc(sizeof(int) * cap);
+  for (size_t k = 0; k < cap; k++) slot[k] = -1;
+  for (i = 0; i < n; i++) {
+    unsigned long long g = gid[i];
+    size_t h = (size_t)((g * 0x9E3779B97F4A7C15ULL) >> 17) & (cap - 1);
+    while (slot[h] >= 0 && gid[slot[h]] != g) h = (h + 1) & (cap - 1);
+    if (slot[h] >= 0) {
+      int f = slot[h];
+      itag[i] = f;
+      if (res[f] < res[i]) res[f] = res[i]; /* std::max(res[f], res[i]) */
+    } else {
+      slot[h] = i;
+      itag[i] = i;
+    }
+  }
+  for (i = 0; i < n; i++) res[i] = res[itag[i]];
+  free(slot);
+}
+
+/* the node-id mode of a single-rank run: the reference's TIOGA_HAS_NODEGID
+   build, which every block's node ids select here */
+static int gid_mode(const OrcTioga *t)
+{
+  int ib;
+  if (t->nblocks == 0) return 0;
+  for (ib = 0; ib < t->nblocks; ib++)
+    if (!t->b[ib]->node_gid) return 0;
+  return 1;
+}
+
+static void block_search(OrcTioga *t, OrcBlock *b)
 {
   int i;
   if (b->nsearch == 0) return;
@@ -1580,7 +1622,15 @@ static void block_search(OrcBlock *b)
   free(b->xtag);
   b->donorId = (int *)malloc(sizeof(int) * b->nsearch);
   b->xtag = (int *)malloc(sizeof(int) * b->nsearch);
-  orc_unique_nodes_octree(b->xsearch, b->tagsearch, b->res_search, b->xtag, b->nsearch);
+  if (gid_mode(t)) {
+    /* gid_search: the sender's node id of every query (exchangeSearchData.C:191-231) */
+    unsigned long long *gs = (unsigned long long *)malloc(sizeof(unsigned long long) * b->nsearch);
+    for (i = 0; i < b->nsearch; i++) gs[i] = t->b[b->isearch[3 * i + 2]]->node_gid[b->isearch[3 * i + 1]];
+    orc_unique_nodes_gid(gs, b->res_search, b->xtag, b->nsearch);
+    free(gs);
+  } else {
+    orc_unique_nodes_octree(b->xsearch, b->tagsearch, b->res_search, b->xtag, b->nsearch);
+  }
   for (i = 0; i < b->nsearch; i++) {
     if (b->xtag[i] == i) b->donorId[i] = search_adt(b, &b->xsearch[3 * i]);
     else b->donorId[i] = b->donorId[b->xtag[i]];
@@ -1968,7 +2018,7 @@ void orc_perform_connectivity(OrcTioga *t)
     free(b->interp);
     b->interp = NULL;
     b->ninterp = 0;
-    block_search(b);
+    block_search(t, b);
   }
   exchange_donors(t);
   for (ib = 0; ib < t->nblocks; ib++) cell_iblanks(t->b[ib]);
@@ -2058,6 +2108,43 @@ void orc_get_donor_info(OrcTioga *t, int btag, int *receptors, int *indices, dou
     receptors[k++] = it->rinfo[2];
     receptors[k++] = it->nweights;
   }
+}
+
+/* tioga::getReceptorInfo (tioga.C:572-671) on one rank, from
+   MeshBlock::getReceptorInfo (dataUpdate.C:161-178) of every block in
+   registration order: per live record (receptor node, mesh tag of the
+   receptor block, the donor cell's global id as two ints in memory order).
+   Without cell ids the reference reads a null pointer; here the cell index
+   stands in.  Returns the number of ints; copies them when out != NULL. */
+long orc_get_receptor_info(OrcTioga *t, int *out)
+{
+  long k = 0;
+  int ib, i;
+  if (t->nsend == 0) return 0;
+  for (ib = 0; ib < t->nblocks; ib++) {
+    OrcBlock *b = t->b[ib];
+    for (i = 0; i < b->ninterp; i++) {
+      Interp *it = &b->interp[i];
+      if (it->cancel) continue;
+      if (out) {
+        int don = it->inode[it->nweights];
+        unsigned long long g = b->cell_gid ? b->cell_gid[don] : (unsigned long long)don;
+        out[k] = it->rinfo[1];
+        out[k + 1] = t->b[it->rinfo[2]]->tag;
+        memcpy(&out[k + 2], &g, sizeof g);
+      }
+      k += 4;
+    }
+  }
+  return k;
+}
+
+void orc_set_gids(OrcTioga *t, int btag, const unsigned long long *cell_gid, const unsigned long long *node_gid)
+{
+  OrcBlock *b = find_block(t, btag, NULL);
+  if (!b) return;
+  b->cell_gid = cell_gid;
+  b->node_gid = node_gid;
 }
 
 #define COPY_OUT(ptr, n, type)                                   \

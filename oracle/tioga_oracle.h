@@ -63,6 +63,16 @@ void orc_data_update(OrcTioga *t, int nvar, int interptype);
 void orc_get_donor_count(OrcTioga *t, int btag, int *dcount, int *fcount);
 void orc_get_donor_info(OrcTioga *t, int btag, int *receptors, int *indices, double *frac, int *dcount);
 
+/* registerGridData's global ids (tioga.h:121-123; kept by pointer).  When
+   every block has node ids the search dedupes coincident queries by node id
+   (search.C:48-74), as the reference's TIOGA_HAS_NODEGID build does. */
+void orc_set_gids(OrcTioga *t, int btag, const unsigned long long *cell_gid, const unsigned long long *node_gid);
+/* uniquenode_map (search.C:48-74) */
+void orc_unique_nodes_gid(const unsigned long long *gid, double *res, int *itag, int n);
+/* tioga::getReceptorInfo (tioga.C:572-671): 4 ints per live receptor;
+   returns the count, copies when out != NULL */
+long orc_get_receptor_info(OrcTioga *t, int *out);
+
 /* Inspection of internal per-block state (for parity of intermediate arrays).
    Returns the number of elements; copies into out when out != NULL.
    names: "nodeRes","cellRes","xsearch","res_search","isearch","tagsearch",

@@ -78,6 +78,41 @@ def nested_boxes(shell: bool = False):
     return out
 
 
+def _position_gids(blocks, base: int = 1 << 33):
+    """Node global ids from positions (coordinates rounded to 1e-9): nodes
+    at the same place, in one block (the shell's per-azimuth pole copies)
+    or in two (the seam of a split box), share an id, as a global node
+    numbering gives them; ids above 2^32.  Cell ids (tag << 40) + 3*cell + 1."""
+    ids = {}
+    for b in blocks:
+        key = np.round(b.x * 1e9).astype(np.int64)
+        g = np.empty(b.nnodes, np.uint64)
+        for i, k in enumerate(map(tuple, key)):
+            g[i] = ids.setdefault(k, base + len(ids))
+        b.node_gid = g
+        b.cell_gid = (np.uint64(b.tag) << np.uint64(40)) + np.uint64(3) * np.arange(b.ncells, dtype=np.uint64) + \
+            np.uint64(1)
+    return blocks
+
+
+def split_box(n: int, cut: int, tags=(2, 3)):
+    """The n^3 box of cart_box cut at cell column i = cut into two blocks of
+    their own tags; the nodes of the cut plane belong to both (abutting
+    partitions as separate meshes)."""
+    full = W.cart_box(n, tag=tags[0])
+    conn = full.types[0][1] - 1
+    m = n + 1
+    ci = np.arange(n ** 3) % n
+    out = []
+    for tag, sel in zip(tags, (ci < cut, ci >= cut)):
+        c = conn[sel]
+        used = np.unique(c)
+        remap = np.full(m ** 3, -1, np.int64)
+        remap[used] = np.arange(used.size)
+        out.append(W.Block(tag, full.x[used].copy(), [(8, (remap[c] + 1).astype(np.int32))]))
+    return out
+
+
 def build(name: str) -> Case:
     if name == "tiny":
         return Case(name, W.tiny(), nvar=5)
@@ -124,12 +159,24 @@ def build(name: str) -> Case:
         return Case(name, nested_boxes(), nvar=2, nfringe=0, mexclude=5, reduce=True)
     if name == "nested3_rf":
         return Case(name, nested_boxes(shell=True), nvar=3, nfringe=0, mexclude=4, reduce=True)
+    # registerGridData with global ids (tioga.h:121-123): the reference built
+    # with TIOGA_HAS_NODEGID dedupes queries by node id (search.C:48-74) and
+    # getReceptorInfo packs the donor cells' 64-bit ids (tioga.C:572-671)
+    if name == "tiny_gid":  # pole copies share node ids
+        return Case(name, _position_gids(W.tiny()), nvar=3)
+    if name == "seam_gid":  # a box split in two meshes: the seam nodes of both share ids
+        blocks = [W.latlong_shell(12, 8, 5, 0.2, 0.45, 1.1, tag=1, center=(0.02, 0.05, 0.01))] + split_box(12, 6)
+        return Case(name, _position_gids(blocks), nvar=2)
+    if name == "seam_gid_rf":
+        blocks = [W.latlong_shell(12, 8, 5, 0.2, 0.45, 1.1, tag=1, center=(0.02, 0.05, 0.01))] + split_box(12, 6)
+        return Case(name, _position_gids(blocks), nvar=2, nfringe=2, reduce=True)
     raise KeyError(name)
 
 
 SMALL_CASES = ["tiny", "tiny_isym0", "tiny_isym1", "tiny_isym2", "mixed", "userres", "moving", "three", "bigtag"]
 REDUCE_CASES = ["tiny_rf", "isym0_rf", "mixed_rf", "three_rf", "moving_rf", "nested_rf", "nested3_rf"]
-ALL_CASES = SMALL_CASES + ["config1"] + REDUCE_CASES
+GID_CASES = ["tiny_gid", "seam_gid", "seam_gid_rf"]
+ALL_CASES = SMALL_CASES + ["config1"] + REDUCE_CASES + GID_CASES
 
 
 # ---------------------------------------------------------------------------

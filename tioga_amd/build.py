@@ -59,6 +59,26 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+ROOT = os.path.dirname(HERE)
+CXX_DRIVER_SRC = os.path.join(ROOT, "tests", "cxx", "tioga_class_driver.cpp")
+CXX_DRIVER = os.path.join(ROOT, "tests", "cxx", "tioga_class_driver")
+
+
+def build_cxx_driver(verbose: bool = False) -> str:
+    """The C++ caller of TIOGA::tioga that tests/test_cxx_class.py runs: a
+    host program (g++) linked against libtioga_amd.so in this tree."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", CXX_DRIVER_SRC, "-o", CXX_DRIVER + ".tmp", "-L" + HERE,
+           "-ltioga_amd", "-Wl,-rpath,$ORIGIN/../../tioga_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(CXX_DRIVER + ".tmp", CXX_DRIVER)
+    if verbose:
+        print("built", CXX_DRIVER)
+    return CXX_DRIVER
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_cxx_driver(verbose=True)
     sys.exit(0)

@@ -269,6 +269,23 @@ void tioga_writeoutputfiles_(int *, char *itype)
   fprintf(stderr, "tioga_amd: tioga_writeoutputfiles_ writes no Tecplot files in this engine\n");
 }
 
+void tioga_amd_register_gids(int btag, uint64_t *cell_gid, uint64_t *node_gid)
+{
+  guarded("tioga_amd_register_gids", [&] { tg_or_throw().engine()->set_gids(btag, cell_gid, node_gid); });
+}
+
+long tioga_amd_receptor_info(int *out)
+{
+  long r = -1;
+  guarded("tioga_amd_receptor_info", [&] {
+    std::vector<int> v;
+    tg_or_throw().getReceptorInfo(v);
+    if (out && !v.empty()) memcpy(out, v.data(), sizeof(int) * v.size());
+    r = (long)v.size();
+  });
+  return r;
+}
+
 void tioga_getdonorcount_(int *btag, int *dcount, int *fcount)
 {
   guarded("tioga_getdonorcount", [&] { tg_or_throw().getDonorCount(*btag, dcount, fcount); });

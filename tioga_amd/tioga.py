@@ -69,6 +69,9 @@ def load():
         L.tioga_amd_last_search_kernel_ms.restype = C.c_double
         L.tioga_amd_query_set_nelem.argtypes = [C.c_int]
         L.tioga_amd_query_set_nelem.restype = C.c_int
+        L.tioga_amd_register_gids.argtypes = [C.c_int, _P, _P]
+        L.tioga_amd_receptor_info.argtypes = [_P]
+        L.tioga_amd_receptor_info.restype = C.c_long
         L.tioga_amd_query_set_donors.argtypes = [C.c_int, _P]
         L.tioga_amd_query_set_extents.argtypes = [C.c_int, _P]
         L.tioga_amd_query_set_extents.restype = C.c_int
@@ -160,6 +163,9 @@ class Tioga:
         self._check()
         if b.nres is not None:
             self.L.tioga_setresolutions_multi_(_ip(b.tag), _p(b.nres), _p(b.cres))
+            self._check()
+        if b.cell_gid is not None or b.node_gid is not None:
+            self.L.tioga_amd_register_gids(b.tag, _p(b.cell_gid), _p(b.node_gid))
             self._check()
         self.blocks.append(b)
         self.iblank.append(ib)
@@ -365,6 +371,15 @@ class Tioga:
         self._check()
         return int(n)
 
+    def receptor_info(self) -> np.ndarray:
+        n = self.L.tioga_amd_receptor_info(None)
+        self._check()
+        out = np.zeros(max(n, 0), np.int32)
+        if n > 0:
+            self.L.tioga_amd_receptor_info(_p(out))
+            self._check()
+        return out
+
     def query_clear(self) -> None:
         self.L.tioga_amd_query_set_clear()
         self._check()
@@ -419,6 +434,8 @@ class Tioga:
             rec, ind, frac = self.donor_info(b.tag)
             r[p + "dcount"] = np.array([rec.size // 4], np.int32)
             r[p + "receptors"], r[p + "indices"], r[p + "frac"] = rec, ind, frac
+        if any(b.cell_gid is not None for b in self.blocks):
+            r["receptor_info"] = self.receptor_info()
         mt = max(b.tag for b in self.blocks)
         for m in range(1, mt + 1):
             hm = self.hole_map(m)

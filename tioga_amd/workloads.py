@@ -58,6 +58,8 @@ class Block:
     obc: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     nres: Optional[np.ndarray] = None
     cres: Optional[np.ndarray] = None
+    cell_gid: Optional[np.ndarray] = None  # uint64 global ids (registerGridData, tioga.h:121-123)
+    node_gid: Optional[np.ndarray] = None
 
     @property
     def nnodes(self) -> int:
@@ -77,6 +79,8 @@ class Block:
             np.ascontiguousarray(self.obc, dtype=np.int32),
             None if self.nres is None else np.ascontiguousarray(self.nres, dtype=np.float64),
             None if self.cres is None else np.ascontiguousarray(self.cres, dtype=np.float64),
+            None if self.cell_gid is None else np.ascontiguousarray(self.cell_gid, dtype=np.uint64),
+            None if self.node_gid is None else np.ascontiguousarray(self.node_gid, dtype=np.uint64),
         )
 
 
