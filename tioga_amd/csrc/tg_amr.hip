@@ -314,13 +314,37 @@ __global__ void __launch_bounds__(ABS) k_amr_gather(int nrec, int nvar, int inte
   }
   double *out = T->q[meta >> 3];
   const int nn = T->nnodes[meta >> 3];
-  for (int n = 0; n < nvar; n++) {
-    const double *qn = qp + (size_t)d3 * n;
-    double acc = 0;
+  // fields in chunks of 8: all sums of a chunk first, then the receptor's
+  // values of the chunk written together (in the row layout a whole
+  // 64-byte run, as 16-byte stores when the run is so aligned)
+  for (int n0 = 0; n0 < nvar; n0 += 8) {
+    const int kn = nvar - n0 < 8 ? nvar - n0 : 8;
+    double acc[8];
 #pragma unroll
-    for (int m = 0; m < 8; m++) acc += qn[off[m]] * w[m];
-    if (interptype == 0) out[(size_t)node * nvar + n] = acc;
-    else out[(size_t)nn * n + node] = acc;
+    for (int kk = 0; kk < 8; kk++) {
+      acc[kk] = 0;
+      if (kk < kn) {
+        const double *qn = qp + (size_t)d3 * (n0 + kk);
+#pragma unroll
+        for (int m = 0; m < 8; m++) acc[kk] += qn[off[m]] * w[m];
+      }
+    }
+    if (interptype == 0) {
+      double *o = out + (size_t)node * nvar + n0;
+      if (kn == 8 && (((uintptr_t)o) & 15) == 0) {
+        double2 *o2 = reinterpret_cast<double2 *>(o);
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) o2[kk] = make_double2(acc[2 * kk], acc[2 * kk + 1]);
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++)
+          if (kk < kn) o[kk] = acc[kk];
+      }
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < 8; kk++)
+        if (kk < kn) out[(size_t)nn * (n0 + kk) + node] = acc[kk];
+    }
   }
 }
 
