@@ -371,6 +371,9 @@ def search_only(args, tg, blocks, desc) -> int:
     """The headline passes alone (what the committed kernel profiles run)."""
     sets = [(blocks[0].tag, blocks[1].x), (blocks[1].tag, blocks[0].x)]
     qids = [tg.query_set(btag, xq) for btag, xq in sets]
+    from tioga_amd import autotune
+
+    autotune.apply(tg, args.workload)
     for i, (_, xq) in zip(qids, sets):
         tg.query_search(i, xq.shape[0])
     for _ in range(args.warmup):
@@ -477,6 +480,12 @@ def main() -> int:
     t0 = time.time()
     qids = [tg.query_set(btag, xq) for btag, xq in sets]
     log(f"query sets built in {time.time() - t0:.1f} s")
+    # the search's launch parameters from the autotuner's table (they change
+    # no result; tioga_amd/autotune.py measured them on this workload)
+    from tioga_amd import autotune
+
+    tuning = autotune.apply(tg, args.workload)
+    tuning["source"] = "tioga_amd/search_tuning.json" if autotune.load(args.workload) else "defaults"
     donors = [tg.query_search(i, xq.shape[0]) for i, (_, xq) in zip(qids, sets)]
 
     for _ in range(args.warmup):
@@ -559,6 +568,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "parity": parity,
         "timed_donors_equal_checked": bool(timed_same),
+        "search_tuning": tuning,
         "queries_in_root_box": in_root,
         "searches_per_s_in_root_box": round(in_root / (ms_per_step / 1000.0), 1),
         "search_kernel_ms": round(kern_ms, 4),

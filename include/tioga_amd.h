@@ -201,6 +201,13 @@ int tioga_amd_query_set_extents(int id, double *ext);
    capacity the next search starts from (a checked search grows it when its
    candidates do not fit and walks again); returns the current capacity. */
 long tioga_amd_search_capacity(long cap);
+/* Launch parameters of the donor search, none of which changes a result:
+   {Morton bits of the query order (0..30), candidates staged per lane by the
+   tree walk (16, 24 or 32), workgroups of each containment-test kernel
+   (64..65536)}.  set (may be NULL) applies 3 values, get (may be NULL)
+   receives the current ones; returns 0, or -1 with the error status set.
+   tioga_amd/autotune.py chooses them by measurement. */
+int tioga_amd_search_tuning(const int *set, int *get);
 
 /* Host wall times (ms) of the last performconnectivity phases:
    out[0..6] = hole map, boxes, query gathering, search preparation,

@@ -190,3 +190,23 @@ def test_query_sets_candidate_list_regrows():
     finally:
         t.close()
         o.close()
+
+
+@pytest.mark.parametrize("tuning", [dict(morton_bits=0, kstage=16, eval_grid=1024),
+                                    dict(morton_bits=30, kstage=24, eval_grid=8192),
+                                    dict(morton_bits=9, kstage=32, eval_grid=64)])
+def test_query_sets_any_tuning(tuning):
+    """The search's launch parameters (Morton bits, walk staging depth,
+    evaluation grid) change no donor."""
+    print(f"\n[query sets] tuning {tuning}", flush=True)
+    blocks = _blocks("mid")
+    o = _oracle(blocks)
+    t = _engine(blocks)
+    try:
+        got = t.search_tuning(**tuning)
+        assert got == tuning
+        sets = [(blocks[0].tag, blocks[1].x), (blocks[1].tag, blocks[0].x)]
+        _check_sets(t, o, sets, f"tuning {tuning}")
+    finally:
+        t.close()
+        o.close()

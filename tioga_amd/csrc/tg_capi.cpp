@@ -554,6 +554,30 @@ int tioga_amd_query_set_extents(int id, double *ext)
   return r;
 }
 
+int tioga_amd_search_tuning(const int *set, int *get)
+{
+  int rc = -1;
+  guarded("tioga_amd_search_tuning", [&] {
+    tg::SearchTuning &t = tg_or_throw().engine()->search_tuning();
+    if (set) {
+      if (set[0] < 0 || set[0] > 30) throw std::runtime_error("tioga_amd: Morton bits must be 0..30");
+      if (set[1] != 16 && set[1] != 24 && set[1] != 32)
+        throw std::runtime_error("tioga_amd: walk staging must be 16, 24 or 32 candidates");
+      if (set[2] < 64 || set[2] > 65536) throw std::runtime_error("tioga_amd: evaluation grid must be 64..65536");
+      t.morton_bits = set[0];
+      t.kstage = set[1];
+      t.eval_grid = set[2];
+    }
+    if (get) {
+      get[0] = t.morton_bits;
+      get[1] = t.kstage;
+      get[2] = t.eval_grid;
+    }
+    rc = 0;
+  });
+  return rc;
+}
+
 long tioga_amd_search_capacity(long cap)
 {
   long r = -1;

@@ -316,7 +316,17 @@ struct StepScratch {
 
 // Reusable device scratch of the donor search: per-query candidate lists
 // and the queue of queries with more candidates than a list holds.
+// Launch parameters of the donor search, none of which changes a result:
+// Morton bits of the query order, candidates staged per lane by the walk
+// (16, 24 or 32), workgroups of the containment-test kernels.
+struct SearchTuning {
+  int morton_bits = 21;
+  int kstage = 32;
+  int eval_grid = 2048;
+};
+
 struct SearchScratch {
+  SearchTuning tune;
   DBuf<int> cell, query, status, qoff, qcnt, ovf, counters, bins, blist, lj, lcount, qb, qsid, nin, wstart;
   DBuf<unsigned char> sv;  // the table of searches of one launch
   DBuf<double4> lst;
@@ -485,6 +495,7 @@ class Engine {
   // candidate-list capacity (entries) the next donor search starts from; it
   // grows whenever a checked search finds it too small
   size_t search_capacity() const { return sw_.cap; }
+  SearchTuning &search_tuning() { return sw_.tune; }
   void set_search_capacity(size_t cap) { sw_.cap = cap; }
 
   // AMR Cartesian background, gather direction (tg_amr.hip):

@@ -1044,7 +1044,8 @@ __global__ void k_morton_sets(int ns, int tot, const MortonSet *__restrict__ M, 
     c[d] = (unsigned)(u * 1024.0);
   }
   const unsigned m30 = (spread10(c[0]) << 2) | (spread10(c[1]) << 1) | spread10(c[2]);
-  key[g] = ((unsigned)lo << (mb + 1)) | ((inroot && !dup) ? 0u : (1u << mb)) | (m30 >> (30 - mb));
+  const unsigned mk = mb > 0 ? (m30 >> (30 - mb)) : 0u;
+  key[g] = ((unsigned)lo << (mb + 1)) | ((inroot && !dup) ? 0u : (1u << mb)) | mk;
   idx[g] = i;
 }
 
@@ -1165,7 +1166,6 @@ __device__ __forceinline__ bool in_grid(const unsigned q[3], const unsigned w[3]
 constexpr int WAVE = 64;
 constexpr int STACK = 32;
 static_assert(STACK <= WAVE, "k_collect keeps its stack one entry per lane");
-constexpr int KSTAGE = 32;  // candidates per lane staged in LDS during the walk
 
 // Candidate lists of one search: a flat list of (cell | nv << 28, query)
 // entries, grouped per query in walk order, and per query its offset and
@@ -1179,7 +1179,7 @@ struct CandOut {
   int *qoff, *qcnt;
   int *used;  // entries allocated so far
   int cap;    // capacity of the flat list
-  int *ovf, *novf;  // queries with more than KSTAGE candidates
+  int *ovf, *novf;  // queries with more candidates than k_collect stages
 };
 
 // ADT::searchADT / searchIntersections (searchADTrecursion.C:26-101).  The
@@ -1215,7 +1215,10 @@ struct CandOut {
 // exactly its own preorder walk while node records are wave-uniform scalar
 // loads.  Candidates are staged in LDS and copied out to a chunk the wave
 // allocates at the end; a lane with more than KSTAGE candidates is queued for
-// k_collect_more, which lists all of them.
+// k_collect_more, which lists all of them.  KSTAGE (16, 24 or 32 entries per
+// lane, 4-8 KB of LDS per wave) trades occupancy against queries sent to the
+// slower one-lane walk; the result does not depend on it.
+template <int KSTAGE>
 __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
 {
   __shared__ int stage[KSTAGE][WAVE];
@@ -2292,7 +2295,11 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     CandOut O{w.cell.p, w.query.p, w.status.p, w.qoff.p, w.qcnt.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
               w.ovf.p, w.counters.p + 1};
     fill<int>(w.counters.p, 2, 0, s);
-    k_collect<<<(unsigned)nwave, WAVE, 0, s>>>(T, O);
+    switch (w.tune.kstage) {
+      case 16: k_collect<16><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
+      case 24: k_collect<24><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
+      default: k_collect<32><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
+    }
     k_collect_more<<<1024, 64, 0, s>>>(T, O);
     if (check) {
       int used = 0;
@@ -2318,7 +2325,8 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     // threads, each hundreds of dependent iterations, so bound by latency
     // and not by the device) run on the side stream while the other cell
     // types are tested
-    k_eval<5, 4><<<2048, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    const unsigned eg = (unsigned)w.tune.eval_grid;
+    k_eval<5, 4><<<eg, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
     if (!w.side) {
       TG_HIP(hipStreamCreateWithFlags(&w.side, hipStreamNonBlocking));
       TG_HIP(hipEventCreateWithFlags(&w.fork, hipEventDisableTiming));
@@ -2328,11 +2336,11 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     TG_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
     k_eval_long<5><<<1024, 256, 0, w.side>>>(T, O, L);
     TG_HIP(hipEventRecord(w.join, w.side));
-    k_eval<6, 4><<<2048, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
-    k_eval<8, 4><<<2048, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    k_eval<6, 4><<<eg, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    k_eval<8, 4><<<eg, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
     k_eval_long<6><<<1024, 256, 0, s>>>(T, O, L);
     k_eval_long<8><<<1024, 256, 0, s>>>(T, O, L);
-    k_eval<4, 8><<<2048, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    k_eval<4, 8><<<eg, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
     TG_HIP(hipStreamWaitEvent(s, w.join, 0));
     k_select<<<nb(nqt), BS, 0, s>>>(T, O);
     return;
@@ -3525,10 +3533,11 @@ const int *Engine::order_sets(const std::vector<MortonSet> &M, size_t tot)
   if (tot > 0x7fffffff) throw std::runtime_error("tioga_amd: too many queries in one search launch");
   int rb = 0;
   while ((1 << rb) < ns) rb++;
-  // 7 Morton bits per axis (about one cell of the tree per step) keep the
-  // sort at three 8-bit passes for a few sets
-  const int mb = std::min(21, 31 - rb);
-  if (mb < 12) throw std::runtime_error("tioga_amd: too many query sets in one search launch");
+  // Morton bits (sw_.tune.morton_bits, 0-30; 21 = 7 per axis, about one
+  // cell of the tree per step, three 8-bit sort passes for a few sets):
+  // the order only affects speed
+  const int mb = std::min(std::max(sw_.tune.morton_bits, 0), 31 - rb);
+  if (31 - rb < 12) throw std::runtime_error("tioga_amd: too many query sets in one search launch");
   mk0_.reserve(tot);
   mk1_.reserve(tot);
   mv0_.reserve(tot);

@@ -75,6 +75,8 @@ def load():
         L.tioga_amd_query_set_donors.argtypes = [C.c_int, _P]
         L.tioga_amd_query_set_extents.argtypes = [C.c_int, _P]
         L.tioga_amd_query_set_extents.restype = C.c_int
+        L.tioga_amd_search_tuning.argtypes = [_P, _P]
+        L.tioga_amd_search_tuning.restype = C.c_int
         L.tioga_amd_search_capacity.argtypes = [C.c_long]
         L.tioga_amd_search_capacity.restype = C.c_long
         L.tioga_amd_weight_aborts.restype = C.c_int
@@ -396,6 +398,21 @@ class Tioga:
         self.L.tioga_amd_query_set_extents(qid, _p(ext))
         self._check()
         return ext
+
+    def search_tuning(self, morton_bits: Optional[int] = None, kstage: Optional[int] = None,
+                      eval_grid: Optional[int] = None) -> Dict[str, int]:
+        """Sets the given search launch parameters; returns all three."""
+        cur = np.zeros(3, np.int32)
+        self.L.tioga_amd_search_tuning(None, _p(cur))
+        self._check()
+        want = cur.copy()
+        for i, v in enumerate((morton_bits, kstage, eval_grid)):
+            if v is not None:
+                want[i] = v
+        if not np.array_equal(want, cur):
+            self.L.tioga_amd_search_tuning(_p(want), _p(cur))
+            self._check()
+        return {"morton_bits": int(cur[0]), "kstage": int(cur[1]), "eval_grid": int(cur[2])}
 
     def search_capacity(self, cap: int = 0) -> int:
         r = self.L.tioga_amd_search_capacity(int(cap))
