@@ -208,6 +208,10 @@ long tioga_amd_search_capacity(long cap);
    receives the current ones; returns 0, or -1 with the error status set.
    tioga_amd/autotune.py chooses them by measurement. */
 int tioga_amd_search_tuning(const int *set, int *get);
+/* The current device's architecture name (e.g. "gfx950", without feature
+   suffixes) into out (len bytes); returns 0, or -1 with the error status set.
+   Needs a GPU. */
+int tioga_amd_device_arch(char *out, int len);
 
 /* Host wall times (ms) of the last performconnectivity phases:
    out[0..6] = hole map, boxes, query gathering, search preparation,

@@ -33,15 +33,20 @@ DEFAULT = {"morton_bits": 21, "kstage": 32, "eval_grid": 2048}
 
 
 def arch() -> str:
-    """The device architecture name the table is keyed by (gfx950 here)."""
-    try:
-        import torch
+    """The device architecture name the table is keyed by (gfx950 here), as
+    the engine's HIP runtime reports it."""
+    import ctypes as C
 
-        if torch.cuda.is_available():
-            return torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
-    except Exception:
-        pass
-    return "unknown"
+    from tioga_amd.tioga import load as load_lib
+
+    L = load_lib()
+    L.tioga_amd_device_arch.argtypes = [C.c_char_p, C.c_int]
+    L.tioga_amd_device_arch.restype = C.c_int
+    buf = C.create_string_buffer(64)
+    if L.tioga_amd_device_arch(buf, 64) != 0:
+        L.tioga_amd_clear_status()
+        return "unknown"
+    return buf.value.decode()
 
 
 def _measure(tg, params: Dict[str, int], steps: int) -> float:

@@ -554,6 +554,25 @@ int tioga_amd_query_set_extents(int id, double *ext)
   return r;
 }
 
+int tioga_amd_device_arch(char *out, int len)
+{
+  int rc = -1;
+  guarded("tioga_amd_device_arch", [&] {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    TG_HIP(hipGetDevice(&dev));
+    TG_HIP(hipGetDeviceProperties(&prop, dev));
+    std::string a(prop.gcnArchName);
+    a = a.substr(0, a.find(':'));
+    if (out && len > 0) {
+      strncpy(out, a.c_str(), (size_t)len - 1);
+      out[len - 1] = 0;
+    }
+    rc = 0;
+  });
+  return rc;
+}
+
 int tioga_amd_search_tuning(const int *set, int *get)
 {
   int rc = -1;
