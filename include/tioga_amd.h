@@ -208,6 +208,12 @@ long tioga_amd_search_capacity(long cap);
    receives the current ones; returns 0, or -1 with the error status set.
    tioga_amd/autotune.py chooses them by measurement. */
 int tioga_amd_search_tuning(const int *set, int *get);
+/* tioga_preprocess_grids_ for a solver that moved one block only: block
+   btag is preprocessed in full (tioga.C:95-110), every other block keeps the
+   geometry of its last preprocess (its coordinates, resolutions and
+   topology must not have changed since) and gets iblank = 1, which is what
+   a full preprocess gives them. */
+void tioga_amd_preprocess_block(int btag);
 /* The current device's architecture name (e.g. "gfx950", without feature
    suffixes) into out (len bytes); returns 0, or -1 with the error status set.
    Needs a GPU. */

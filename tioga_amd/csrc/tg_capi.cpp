@@ -554,6 +554,16 @@ int tioga_amd_query_set_extents(int id, double *ext)
   return r;
 }
 
+void tioga_amd_preprocess_block(int btag)
+{
+  guarded("tioga_amd_preprocess_block", [&] {
+    TIOGA::tioga &t = tg_or_throw();
+    t.engine()->mexclude = t.mexclude;
+    t.engine()->nfringe = t.nfringe;
+    t.engine()->profile(btag);
+  });
+}
+
 int tioga_amd_device_arch(char *out, int len)
 {
   int rc = -1;

@@ -444,7 +444,7 @@ class Engine {
   void set_resolutions_first(double *nres, double *cres);
   void set_cell_iblank(int btag, int *ibc);
   void set_cell_iblank_first(int *ibc);
-  void profile();
+  void profile(int only_tag = -1);
   void perform_connectivity();
   void reduce_fringes();
   void register_solution(int btag, double *q);

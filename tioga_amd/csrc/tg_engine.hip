@@ -2622,12 +2622,17 @@ void Engine::tag_boundary(Block &b)
   sync();
 }
 
-// tioga::profile (tioga.C:95-110)
-void Engine::profile()
+// tioga::profile (tioga.C:95-110).  only_tag >= 0: the blocks other than
+// that one keep their preprocessed geometry (their coordinates, resolutions
+// and topology must not have changed since it was computed) and only get
+// iblank = 1, as MeshBlock::preprocess sets it (MeshBlock.C:80); for them the
+// result equals a full preprocess, which recomputes the same values.
+void Engine::profile(int only_tag)
 {
   for (Block *bp : blocks_) {
     Block &b = *bp;
-    if (b.share == 2) {  // another engine's geometry: only this engine's iblank
+    const bool keep = only_tag >= 0 && b.tag != only_tag && b.topo_uploaded && b.cellRes.p != nullptr;
+    if (b.share == 2 || keep) {  // another engine's (or kept) geometry: only this engine's iblank
       b.iblank.reserve(b.nnodes);
       fill<int>(b.iblank.p, b.nnodes, 1, s_);
       if (b.writeback && b.uiblank)

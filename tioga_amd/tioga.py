@@ -75,6 +75,7 @@ def load():
         L.tioga_amd_query_set_donors.argtypes = [C.c_int, _P]
         L.tioga_amd_query_set_extents.argtypes = [C.c_int, _P]
         L.tioga_amd_query_set_extents.restype = C.c_int
+        L.tioga_amd_preprocess_block.argtypes = [C.c_int]
         L.tioga_amd_search_tuning.argtypes = [_P, _P]
         L.tioga_amd_search_tuning.restype = C.c_int
         L.tioga_amd_search_capacity.argtypes = [C.c_long]
@@ -186,6 +187,11 @@ class Tioga:
     def set_coords(self, bi: int, x: np.ndarray) -> None:
         """Move block bi in place (the library keeps the user's pointer)."""
         self.blocks[bi].x[...] = x
+
+    def preprocess_block(self, btag: int):
+        """Preprocess after only block btag moved (tioga_amd_preprocess_block)."""
+        self.L.tioga_amd_preprocess_block(btag)
+        self._check()
 
     def preprocess(self):
         self.L.tioga_preprocess_grids_()
