@@ -349,7 +349,30 @@ def serving_leg(slots: int, budget_gb: float, rates, seconds: float, max_request
     burst = stream(W.motion_requests(64, len(shells), seed=60), np.zeros(64))
     log(f"serving burst of 64: {burst}")
     tg.serve_close()
+    # resident-tree mode: one tree over all background cells, built at open
+    t0 = time.perf_counter()
+    tg.serve_open(bg.tag, nslots=slots, budget_gb=budget_gb, resident_tree=True)
+    open_ms = 1000 * (time.perf_counter() - t0)
+    for s_ in shells:
+        tg.serve_wait(tg.serve_submit(s_.tag, np.eye(3), cen[s_.tag], np.zeros(3)), size[s_.tag])
+    r_alone, r_s, r_tie, r_clean = [], 0, 0, 0
+    for tag, R, sh in W.motion_requests(16, len(shells), seed=40):
+        r = tg.serve_wait(tg.serve_submit(tag, R, cen[tag], sh), size[tag])
+        r_alone.append(1000 * (r["t_done"] - r["t_submit"]))
+        r_s += r["nsearch"]
+        r_tie += r["tieband"]
+        r_clean += r["tieband"] == 0
+    r_burst = stream(W.motion_requests(64, len(shells), seed=60), np.zeros(64))
+    tg.serve_close()
     tg.close()
+    resident = {"open_ms": round(open_ms, 1), "alone_p50_ms": round(float(np.median(r_alone)), 3),
+                "alone_max_ms": round(float(max(r_alone)), 3),
+                "alone_searches_per_s": round(r_s / (sum(r_alone) / 1000), 1),
+                "saturation_burst": r_burst, "tieband_queries": int(r_tie), "searches": int(r_s),
+                "requests_without_tieband": f"{r_clean}/16",
+                "what": "one tree over all background cells built at open; a request with tieband 0 "
+                        "has the per-request tree's results"}
+    log(f"resident-tree serving: {resident}")
     # the plain GPU form of the same kind of requests
     plain, engines = [], {}
     for tag, R, sh in W.motion_requests(nplain, len(shells), seed=40):
@@ -384,7 +407,8 @@ def serving_leg(slots: int, budget_gb: float, rates, seconds: float, max_request
                            "requests": len(plain),
                            "what": "one engine per shell (shell + background), move, preprocess the shell "
                                    "alone, performconnectivity; sequential"},
-            "served_vs_plain_searches": round(burst["searches_per_s"] / plain_sps, 3)}
+            "served_vs_plain_searches": round(burst["searches_per_s"] / plain_sps, 3),
+            "resident_tree": resident}
 
 
 def serving_bench(args) -> int:

@@ -142,8 +142,10 @@ double tioga_amd_time_dataupdate(int nvar, int interptype, int nrep, long long *
    connectivity with the background block alone (the two-block
    performconnectivity on the moved coordinates, x' = R (x - c) + c + t per
    node), served by nslots concurrent slots that share the background's
-   resident device data; requests start in arrival order while their device
-   memory estimates fit budget_gb.  Open after preprocess; the registered
+   resident device data; requests start in arrival order while the device
+   memory of every slot (running: the request's charge, the most a request
+   on that block has held; idle: what the slot keeps) fits budget_gb; one
+   request always runs.  Open after preprocess; the registered
    blocks must not change while serving.  submit returns a request id (-1 on
    error); wait blocks until that request is done and fills the moved
    block's iblank (nshell entries), the background nodes whose iblank is not
@@ -154,6 +156,17 @@ long tioga_amd_serve_submit(int shell_tag, const double *rot9, const double *cen
 int tioga_amd_serve_wait(long id, int *shell_iblank, int nshell, int *nbg, int *bg_nodes, int *bg_vals, int cap,
                          double *times, long long *nsearch);
 void tioga_amd_serve_close(void);
+/* tioga_amd_serve_open with flags: bit 0 = resident-tree mode (searches
+   against the background walk one tree over all its cells, built at open,
+   instead of a tree per request; a request's donors may then differ from the
+   per-request tree's only at queries whose donor depends on the walk order,
+   which tioga_amd_serve_wait2 counts). */
+int tioga_amd_serve_open2(int bg_tag, int nslots, double budget_gb, int flags);
+/* tioga_amd_serve_wait with stats[4] = donor searches, tie-band queries
+   (resident-tree mode), device bytes the admission charged the request, and
+   device bytes its slot held at most while serving it. */
+int tioga_amd_serve_wait2(long id, int *shell_iblank, int nshell, int *nbg, int *bg_nodes, int *bg_vals, int cap,
+                          double *times, long long *stats);
 
 /* Batched form (on != 0, the default): the stages of performconnectivity
    run for all blocks together (one ADT build, one coincident-point dedupe

@@ -427,16 +427,21 @@ double tioga_amd_time_dataupdate(int nvar, int interptype, int nrep, long long *
 
 // ---- serving (tg_serve.h) ----
 
-int tioga_amd_serve_open(int bg_tag, int nslots, double budget_gb)
+int tioga_amd_serve_open2(int bg_tag, int nslots, double budget_gb, int flags)
 {
   int rc = -1;
   guarded("tioga_amd_serve_open", [&] {
     delete g_srv;
     g_srv = nullptr;
-    g_srv = new tg::Server(*tg_or_throw().engine(), bg_tag, nslots, budget_gb * 1e9);
+    g_srv = new tg::Server(*tg_or_throw().engine(), bg_tag, nslots, budget_gb * 1e9, (flags & 1) != 0);
     rc = 0;
   });
   return rc;
+}
+
+int tioga_amd_serve_open(int bg_tag, int nslots, double budget_gb)
+{
+  return tioga_amd_serve_open2(bg_tag, nslots, budget_gb, 0);
 }
 
 long tioga_amd_serve_submit(int shell_tag, const double *rot, const double *center, const double *shift)
@@ -449,8 +454,11 @@ long tioga_amd_serve_submit(int shell_tag, const double *rot, const double *cent
   return id;
 }
 
-int tioga_amd_serve_wait(long id, int *shell_iblank, int nshell, int *nbg, int *bg_nodes, int *bg_vals, int cap,
-                         double *times, long long *nsearch)
+}  // extern "C"
+
+namespace {
+int serve_wait_impl(long id, int *shell_iblank, int nshell, int *nbg, int *bg_nodes, int *bg_vals, int cap,
+                    double *times, long long *nsearch, long long *stats)
 {
   int rc = -1;
   guarded("tioga_amd_serve_wait", [&] {
@@ -471,9 +479,31 @@ int tioga_amd_serve_wait(long id, int *shell_iblank, int nshell, int *nbg, int *
     times[2] = r.t_done;
     times[3] = r.connectivity_ms;
     *nsearch = r.nsearch;
+    if (stats) {
+      stats[0] = r.nsearch;
+      stats[1] = r.tieband;
+      stats[2] = (long long)r.est_bytes;
+      stats[3] = (long long)r.peak_bytes;
+    }
     rc = 0;
   });
   return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int tioga_amd_serve_wait(long id, int *shell_iblank, int nshell, int *nbg, int *bg_nodes, int *bg_vals, int cap,
+                         double *times, long long *nsearch)
+{
+  return serve_wait_impl(id, shell_iblank, nshell, nbg, bg_nodes, bg_vals, cap, times, nsearch, nullptr);
+}
+
+int tioga_amd_serve_wait2(long id, int *shell_iblank, int nshell, int *nbg, int *bg_nodes, int *bg_vals, int cap,
+                          double *times, long long *stats)
+{
+  long long ns = 0;
+  return serve_wait_impl(id, shell_iblank, nshell, nbg, bg_nodes, bg_vals, cap, times, &ns, stats);
 }
 
 void tioga_amd_serve_close(void)

@@ -26,6 +26,20 @@ namespace tg {
 void hip_check(hipError_t e, const char *what, const char *file, int line);
 #define TG_HIP(x) ::tg::hip_check((x), #x, __FILE__, __LINE__)
 
+// Device bytes allocated through DBuf and the library primitives by the
+// calling host thread (net of frees on that thread) and its high-water mark:
+// a serving slot runs its engine on one thread, so these measure what a
+// request holds and needs.
+struct DevBytes {
+  long long now = 0, peak = 0;
+  void add(long long n)
+  {
+    now += n;
+    if (now > peak) peak = now;
+  }
+};
+DevBytes &thread_dev_bytes();
+
 // Owning device buffer that grows on demand.
 template <class T>
 struct DBuf {
@@ -44,6 +58,7 @@ struct DBuf {
       release();
       TG_HIP(hipMalloc((void **)&p, n * sizeof(T)));
       cap = n;
+      thread_dev_bytes().add((long long)(n * sizeof(T)));
     }
     return p;
   }
@@ -70,7 +85,10 @@ struct DBuf {
   }
   void release()
   {
-    if (p && own) (void)hipFree(p);
+    if (p && own) {
+      (void)hipFree(p);
+      thread_dev_bytes().add(-(long long)(cap * sizeof(T)));
+    }
     p = nullptr;
     cap = 0;
     own = true;
@@ -138,6 +156,16 @@ struct InterpHost {
   double w[9];
 };
 
+// A search structure kept over every cell of a block (a served background):
+// searches against the block walk it instead of a tree built per query set.
+struct ResidentTree {
+  int nelem = 0, depth = 0;
+  double ext[6] = {0, 0, 0, 0, 0, 0};
+  DBuf<int> elist, adtInt;
+  DBuf<double> ebox, adtReal;
+  DBuf<WalkNode> nodes;
+};
+
 struct Block {
   // ---- registration: the user's memory, kept by pointer ----
   int tag = 0, nnodes = 0, ncells = 0;
@@ -185,6 +213,10 @@ struct Block {
   DBuf<double> ebox, adtReal;
   DBuf<WalkNode> nodes;
   double adtExt[6] = {0, 0, 0, 0, 0, 0};
+  // searches against this block walk this tree (read only; another engine
+  // owns it) when set; queries whose donor could depend on the walk order
+  // are counted (tie band)
+  const ResidentTree *resident = nullptr;
 
   // donor bookkeeping (receptor side: one entry per node)
   DBuf<int> has_donor, head_arr, head_blk, head_remote;
@@ -436,6 +468,33 @@ class Engine {
     auto it = tag_to_blk_.find(btag);
     return it == tag_to_blk_.end() ? nullptr : blocks_[it->second]->ux;
   }
+  const double *block_device_coords(int btag) const
+  {
+    auto it = tag_to_blk_.find(btag);
+    return it == tag_to_blk_.end() ? nullptr : blocks_[it->second]->x.p;
+  }
+  const int *block_wall_nodes(int btag, int *nwbc) const
+  {
+    auto it = tag_to_blk_.find(btag);
+    if (it == tag_to_blk_.end()) return nullptr;
+    *nwbc = blocks_[it->second]->nwbc;
+    return blocks_[it->second]->uwbc;
+  }
+  // the search structure over every cell of block btag (after preprocess)
+  void build_resident_tree(int btag, ResidentTree &t);
+  // searches against block btag walk t (owned by another engine) from now on
+  void use_resident_tree(int btag, const ResidentTree *t);
+  // tioga::profile for a block moved rigidly on the device: block btag's
+  // coordinates become x' = R (x0 - c) + c + t per node (x0: device
+  // coordinates of the same topology, row-major R), then it is preprocessed;
+  // every other block only gets iblank = 1 (profile(btag) semantics).  The
+  // host coordinates the block was registered or adopted with must hold the
+  // moved wall nodes (the hole map's wall bounds read them).
+  void profile_moved(int btag, const double *x0_dev, const double rot[9], const double c[3], const double t[3]);
+  // queries of the last search whose donor could depend on the walk order
+  // of a resident tree (more than one accepting candidate, or a near-zero
+  // weight in a BIGVALUE cell)
+  long long last_tieband() const { return last_tieband_; }
 
   void register_grid(int btag, int nnodes, double *xyz, int *ibl, int nwbc, int nobc, int *wbc, int *obc,
                      int ntypes, const int *nv, const int *nc, int **vconn);
@@ -602,6 +661,10 @@ class Engine {
   double prep_acc_[4] = {0, 0, 0, 0};
   long long obb_stepped_ = 0;
   std::vector<DBuf<double> *> qdev_;
+  DBuf<int> tieband_;  // tie-band count of the last search over resident trees
+  DBuf<double> moved_;  // the last rigid move (R, c, t) on the device
+  long long last_tieband_ = 0;
+  void attach_resident(Block &b);
   // dataUpdate's plan for the current interpolation records (tg_update.hip)
   struct UpdatePlan {
     UpdTable host{};

@@ -25,6 +25,12 @@
 
 namespace tg {
 
+DevBytes &thread_dev_bytes()
+{
+  static thread_local DevBytes b;
+  return b;
+}
+
 void hip_check(hipError_t e, const char *what, const char *file, int line)
 {
   if (e == hipSuccess) return;
@@ -1091,6 +1097,7 @@ struct SearchView {
   const int *conn;
   const double *x;
   const double *cellRes;
+  int *tieband = nullptr;  // counter of tie-band queries (resident trees), or null
 };
 
 // Several searches (one per block, each with its own tree and query set)
@@ -1139,6 +1146,7 @@ __device__ __forceinline__ SearchView view_of(const SearchSet &T, int sid)
   S.conn = as_global(v->conn);
   S.x = as_global(v->x);
   S.cellRes = as_global(v->cellRes);
+  S.tieband = v->tieband;
   return S;
 }
 
@@ -1581,13 +1589,24 @@ __global__ void k_select(SearchSet T, CandOut O)
     S.donorId[i] = -1;
     return;
   }
+  // with a tie-band counter every candidate is looked at: the donor depends
+  // on the walk order when more than one candidate accepts the point or one
+  // accepts it with a near-zero weight in a BIGVALUE cell
+  bool done = false, flag1 = false;
+  int nacc = 0;
   for (int k = 0; k < nc; k++) {
     const int st = O.status[off + k];
     if (st == OUTSIDE) continue;
-    const int cell = O.cell[off + k] & 0x0fffffff;
-    res = st ? cell : -1;
-    if (st == 2) break;
+    nacc += st != 0;
+    flag1 = flag1 || st == 1;
+    if (!done) {
+      const int cell = O.cell[off + k] & 0x0fffffff;
+      res = st ? cell : -1;
+      done = st == 2;
+    }
+    if (done && !S.tieband) break;
   }
+  if (S.tieband && (nacc > 1 || flag1)) atomicAdd(S.tieband, 1);
   S.donorId[i] = res;
 }
 
@@ -2655,6 +2674,51 @@ void Engine::profile(int only_tag)
   }
 }
 
+// x' = R (x0 - c) + c + t per node, in this order (the host form of the
+// serving loop's moves, evaluated the same way)
+__global__ void k_rigid(int n, const double *__restrict__ x0, const double *__restrict__ m, double *__restrict__ x)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double d0 = x0[3 * (size_t)i] - m[9], d1 = x0[3 * (size_t)i + 1] - m[10], d2 = x0[3 * (size_t)i + 2] - m[11];
+  for (int j = 0; j < 3; j++) {
+    const double y = m[3 * j] * d0 + m[3 * j + 1] * d1 + m[3 * j + 2] * d2;
+    x[3 * (size_t)i + j] = y + m[9 + j] + m[12 + j];
+  }
+}
+
+void Engine::profile_moved(int btag, const double *x0_dev, const double rot[9], const double c[3],
+                           const double t[3])
+{
+  Block *bm = find(btag);
+  if (!bm || !bm->topo_uploaded) throw std::runtime_error("tioga_amd: moved block not adopted");
+  for (Block *bp : blocks_) {
+    Block &b = *bp;
+    b.iblank.reserve(b.nnodes);
+    fill<int>(b.iblank.p, b.nnodes, 1, s_);
+    if (b.writeback && b.uiblank)
+      for (int i = 0; i < b.nnodes; i++) b.uiblank[i] = 1;  // MeshBlock.C:80
+  }
+  Block &b = *bm;
+  double m[15];
+  memcpy(m, rot, 9 * sizeof(double));
+  memcpy(m + 9, c, 3 * sizeof(double));
+  memcpy(m + 12, t, 3 * sizeof(double));
+  moved_.reserve(15);
+  TG_HIP(hipMemcpyAsync(moved_.p, m, sizeof m, hipMemcpyHostToDevice, s_));
+  b.x.reserve_own(3 * (size_t)b.nnodes);
+  if (b.nnodes) k_rigid<<<nb(b.nnodes), BS, 0, s_>>>(b.nnodes, x0_dev, moved_.p, b.x.p);
+  // the block OBB (findOBB, utils.c:25-165) from the device coordinates, with
+  // the bits of the host running sums
+  if (b.nnodes) {
+    const ObbSet os{b.nnodes, b.x.p};
+    find_obbs_device(&os, 1, &b.obb, obbw_, s_, nullptr);
+  }
+  tag_boundary(b);
+  connected_ = false;
+  upd_.valid = false;
+}
+
 void Engine::adopt_block(const Engine &src, int src_tag, int as_tag, int share, double *ux, int *uiblank,
                          bool writeback, int slot)
 {
@@ -3261,13 +3325,18 @@ void Engine::search(Block &b)
   b.nelem = 0;
   if (Q == 0) return;
   auto t0 = std::chrono::steady_clock::now();
-  OBB obq;
-  const ObbSet os{Q, b.xsearch.p};
-  long long st = 0;
-  find_obbs_device(&os, 1, &obq, obbw_, s_, &st);
-  obb_stepped_ += st;
-  prep_acc_[0] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  build_search_structure(b, obq, b.nelem, b.adt_depth, b.elist, b.ebox, b.adtInt, b.adtReal, b.nodes, b.adtExt);
+  if (b.resident) {
+    attach_resident(b);
+  } else {
+    OBB obq;
+    const ObbSet os{Q, b.xsearch.p};
+    long long st = 0;
+    find_obbs_device(&os, 1, &obq, obbw_, s_, &st);
+    obb_stepped_ += st;
+    prep_acc_[0] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    build_search_structure(b, obq, b.nelem, b.adt_depth, b.elist, b.ebox, b.adtInt, b.adtReal, b.nodes,
+                           b.adtExt);
+  }
   bool gid_mode = true;
   for (Block *o : blocks_) gid_mode = gid_mode && (o->unode_gid != nullptr);
   if (gid_mode) {
@@ -3331,8 +3400,13 @@ void Engine::search_batched()
   } else {
     dedupe(s_);
   }
+  std::vector<int> build;  // blocks whose tree is built for this query set
+  for (int ib : act) {
+    if (blocks_[ib]->resident) attach_resident(*blocks_[ib]);
+    else build.push_back(ib);
+  }
   try {
-    prepare_searches(act);
+    if (!build.empty()) prepare_searches(build);
   } catch (...) {
     if (ded_thread.joinable()) ded_thread.join();
     throw;
@@ -3473,6 +3547,39 @@ void Engine::dedupe_gid(Block &b)
   TG_HIP(hipMemcpyAsync(b.xtag.p, xt.data(), sizeof(int) * Q, hipMemcpyHostToDevice, s_));
   TG_HIP(hipMemcpyAsync(b.res_search.p, rs.data(), sizeof(double) * Q, hipMemcpyHostToDevice, s_));
   sync();
+}
+
+// The search structure over every cell of block btag: the cells overlapping
+// the block's own OBB (all of them) in the reference's LIFO order, their
+// ADT and walk records.
+void Engine::build_resident_tree(int btag, ResidentTree &t)
+{
+  Block *b = find(btag);
+  if (!b || !b->topo_uploaded || b->cellRes.p == nullptr) throw std::runtime_error("tioga_amd: preprocess first");
+  build_search_structure(*b, b->obb, t.nelem, t.depth, t.elist, t.ebox, t.adtInt, t.adtReal, t.nodes, t.ext);
+  if (t.nelem != b->ncells) throw std::runtime_error("tioga_amd: resident tree misses cells of its block");
+}
+
+void Engine::use_resident_tree(int btag, const ResidentTree *t)
+{
+  Block *b = find(btag);
+  if (!b) throw std::runtime_error("tioga_amd: resident tree for an unknown tag");
+  b->resident = t;
+  connected_ = false;
+}
+
+// a block's search arrays as views of its resident tree
+void Engine::attach_resident(Block &b)
+{
+  const ResidentTree &t = *b.resident;
+  b.nelem = t.nelem;
+  b.adt_depth = t.depth;
+  memcpy(b.adtExt, t.ext, sizeof b.adtExt);
+  b.elist.view(t.elist.p, t.nelem);
+  b.ebox.view(t.ebox.p, 6 * (size_t)t.nelem);
+  b.adtInt.view(t.adtInt.p, 4 * (size_t)t.nelem);
+  b.adtReal.view(t.adtReal.p, 6 * (size_t)t.nelem);
+  b.nodes.view(t.nodes.p, t.nelem);
 }
 
 // search.C:107-233: query-set OBB -> cells whose OBB-frame box overlaps it,
@@ -3679,7 +3786,12 @@ void Engine::launch_search(Block &b)
   }
   MortonSet ms{b.xsearch.p, b.xtag.p, b.donorId.p, {}, 0, b.nsearch};
   memcpy(ms.ext, b.adtExt, sizeof ms.ext);
+  if (b.resident) {
+    tieband_.reserve(1);
+    fill<int>(tieband_.p, 1, 0, s_);
+  }
   SearchView S;
+  S.tieband = b.resident ? tieband_.p : nullptr;
   S.nq = b.nsearch;
   S.perm = order_sets({ms}, b.nsearch);
   S.xs = b.xsearch.p;
@@ -3693,6 +3805,12 @@ void Engine::launch_search(Block &b)
   S.cellRes = b.cellRes.p;
   run_search(&S, 1, sw_, s_, true);
   k_copy_dup<<<nb(b.nsearch), BS, 0, s_>>>(b.nsearch, b.xtag.p, b.donorId.p);
+  if (b.resident) {
+    int tb = 0;
+    TG_HIP(hipMemcpyAsync(&tb, tieband_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
+    sync();
+    last_tieband_ += tb;
+  }
 }
 
 // every block's donor search in one launch of each search kernel: each
@@ -3723,10 +3841,17 @@ void Engine::launch_searches()
     off += b.nsearch;
   }
   const int *perm = order_sets(ms, tot);
+  bool any_resident = false;
+  for (Block *b : act) any_resident = any_resident || b->resident;
+  if (any_resident) {
+    tieband_.reserve(1);
+    fill<int>(tieband_.p, 1, 0, s_);
+  }
   std::vector<SearchView> sv;
   for (int k = 0; k < ns; k++) {
     Block &b = *act[k];
     SearchView S;
+    S.tieband = b.resident ? tieband_.p : nullptr;
     S.nq = b.nsearch;
     S.perm = perm + ms[k].off;
     S.xs = b.xsearch.p;
@@ -3742,6 +3867,13 @@ void Engine::launch_searches()
   }
   run_search(sv.data(), ns, sw_, s_, true);
   k_copy_dup_sets<<<nb(tot), BS, 0, s_>>>((const SearchView *)sw_.sv.p, sw_.qb.p, ns, (int)tot);
+  last_tieband_ = 0;
+  if (any_resident) {
+    int tb = 0;
+    TG_HIP(hipMemcpyAsync(&tb, tieband_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
+    sync();
+    last_tieband_ = tb;
+  }
 }
 
 // tioga::exchangeDonors (exchangeDonors.C:25-301), single rank.  Every

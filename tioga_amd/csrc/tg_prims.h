@@ -1,11 +1,14 @@
 // tg_prims.h -- device-wide primitives (stable compaction, stable radix sort
 // of key/value pairs) used by the connectivity stages.  Implemented over
-// hipCUB in tg_prims.hip so the rest of the engine compiles without it.
+// rocPRIM in tg_prims.hip so the rest of the engine compiles without it.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace tg {
+
+struct DevBytes;
+DevBytes &thread_dev_bytes();  // tg_engine.h
 
 // Device scratch for the library calls; grows on demand and is kept.  Work
 // issued on several streams at once gives each stream its own.

@@ -1,7 +1,8 @@
-// tg_prims.hip -- hipCUB-backed device primitives (see tg_prims.h).
+// tg_prims.hip -- device primitives over rocPRIM (see tg_prims.h).
 #include "tg_prims.h"
+#include "tg_engine.h"
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/rocprim.hpp>
 
 #include <stdexcept>
 #include <string>
@@ -23,25 +24,32 @@ thread_local int *g_count = nullptr;
 void *PrimTemp::get(size_t want)
 {
   if (want > n) {
-    if (p) check(hipFree(p), "hipFree");
+    if (p) {
+      check(hipFree(p), "hipFree");
+      thread_dev_bytes().add(-(long long)n);
+    }
     p = nullptr;
     n = 0;
     check(hipMalloc(&p, want), "hipMalloc(temp)");
     n = want;
+    thread_dev_bytes().add((long long)want);
   }
   return p;
 }
 
 PrimTemp::~PrimTemp()
 {
-  if (p) (void)hipFree(p);
+  if (p) {
+    (void)hipFree(p);
+    thread_dev_bytes().add(-(long long)n);
+  }
 }
 
 size_t sort_pairs_bytes(int n, int bits)
 {
   size_t bytes = 0;
-  check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const unsigned *)nullptr, (unsigned *)nullptr,
-                                           (const int *)nullptr, (int *)nullptr, n, 0, bits),
+  check(rocprim::radix_sort_pairs(nullptr, bytes, (const unsigned *)nullptr, (unsigned *)nullptr,
+                                  (const int *)nullptr, (int *)nullptr, (size_t)n, 0u, (unsigned)bits),
         "sort(size)");
   return bytes;
 }
@@ -50,10 +58,10 @@ int select_flagged(const int *flags, int n, int *out, hipStream_t s)
 {
   if (n <= 0) return 0;
   if (!g_count) check(hipMalloc(&g_count, sizeof(int)), "hipMalloc(count)");
-  hipcub::CountingInputIterator<int> it(0);
+  rocprim::counting_iterator<int> it(0);
   size_t bytes = 0;
-  check(hipcub::DeviceSelect::Flagged(nullptr, bytes, it, flags, out, g_count, n, s), "select(size)");
-  check(hipcub::DeviceSelect::Flagged(g_temp.get(bytes), bytes, it, flags, out, g_count, n, s), "select");
+  check(rocprim::select(nullptr, bytes, it, flags, out, g_count, (size_t)n, s), "select(size)");
+  check(rocprim::select(g_temp.get(bytes), bytes, it, flags, out, g_count, (size_t)n, s), "select");
   int c = 0;
   check(hipMemcpyAsync(&c, g_count, sizeof(int), hipMemcpyDeviceToHost, s), "count D2H");
   check(hipStreamSynchronize(s), "sync");
@@ -65,10 +73,12 @@ void sort_pairs(const unsigned *keys_in, unsigned *keys_out, const int *vals_in,
 {
   if (n <= 0) return;
   size_t bytes = 0;
-  check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in, keys_out, vals_in, vals_out, n, 0, bits, s),
+  check(rocprim::radix_sort_pairs(nullptr, bytes, keys_in, keys_out, vals_in, vals_out, (size_t)n, 0u,
+                                  (unsigned)bits, s),
         "sort(size)");
   PrimTemp &t = temp ? *temp : g_temp;
-  check(hipcub::DeviceRadixSort::SortPairs(t.get(bytes), bytes, keys_in, keys_out, vals_in, vals_out, n, 0, bits, s),
+  check(rocprim::radix_sort_pairs(t.get(bytes), bytes, keys_in, keys_out, vals_in, vals_out, (size_t)n, 0u,
+                                  (unsigned)bits, s),
         "sort");
 }
 
@@ -76,8 +86,8 @@ void exclusive_sum(const int *in, int *out, int n, hipStream_t s)
 {
   if (n <= 0) return;
   size_t bytes = 0;
-  check(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, s), "scan(size)");
-  check(hipcub::DeviceScan::ExclusiveSum(g_temp.get(bytes), bytes, in, out, n, s), "scan");
+  check(rocprim::exclusive_scan(nullptr, bytes, in, out, 0, (size_t)n, rocprim::plus<int>(), s), "scan(size)");
+  check(rocprim::exclusive_scan(g_temp.get(bytes), bytes, in, out, 0, (size_t)n, rocprim::plus<int>(), s), "scan");
 }
 
 }  // namespace tg

@@ -20,6 +20,9 @@ struct ServeResult {
   std::vector<int> shell_iblank;     // iblank of every node of the moved block
   std::vector<int> bg_nodes, bg_vals;  // background nodes whose iblank is not 1
   long long nsearch = 0;
+  long long tieband = 0;     // resident-tree mode: queries whose donor could depend on the walk order
+  double est_bytes = 0;      // device memory the admission charged for the request
+  double peak_bytes = 0;     // device memory the slot held at most during the request
   double connectivity_ms = 0;
   double t_submit = 0, t_start = 0, t_done = 0;  // seconds, steady clock
 };
@@ -28,8 +31,10 @@ class Server {
  public:
   // main: an engine whose blocks are registered and preprocessed; bg_tag the
   // resident background; nslots concurrent requests at most; budget_bytes
-  // the device memory the running requests' estimates may add up to.
-  Server(Engine &main, int bg_tag, int nslots, double budget_bytes);
+  // the device memory the slots may hold together (see Admission in
+  // tg_serve.cpp); resident_tree: searches against the background walk one
+  // tree over all its cells, built here, instead of a tree per request.
+  Server(Engine &main, int bg_tag, int nslots, double budget_bytes, bool resident_tree = false);
   ~Server();
   Server(const Server &) = delete;
   Server &operator=(const Server &) = delete;
@@ -53,13 +58,19 @@ class Server {
     std::vector<double> x;
     std::vector<int> ib;
     std::thread th;
+    double retained = 0;  // device bytes the slot holds between requests
+    double charge = 0;    // what the admission counts for the slot now
   };
   void worker(Slot &s);
   void run(Slot &s, Req &r);
+  double estimate(int tag) const;  // under mu_
 
   Engine &main_;
   int bg_tag_;
-  double budget_, reserved_ = 0;
+  double budget_;
+  bool resident_;
+  ResidentTree tree_;
+  std::map<int, double> seen_peak_;  // per moving block: the most a request has held
   int active_ = 0;
   long next_id_ = 1;
   bool stop_ = false;

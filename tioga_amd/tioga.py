@@ -94,6 +94,10 @@ def load():
         L.tioga_amd_serve_submit.restype = C.c_long
         L.tioga_amd_serve_wait.argtypes = [C.c_long, _P, C.c_int, _P, _P, _P, C.c_int, _P, _P]
         L.tioga_amd_serve_wait.restype = C.c_int
+        L.tioga_amd_serve_open2.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int]
+        L.tioga_amd_serve_open2.restype = C.c_int
+        L.tioga_amd_serve_wait2.argtypes = [C.c_long, _P, C.c_int, _P, _P, _P, C.c_int, _P, _P]
+        L.tioga_amd_serve_wait2.restype = C.c_int
         L.tioga_amd_serve_close.argtypes = []
         L.tioga_amd_amr_array.argtypes = [C.c_int, C.c_char_p, _P]
         L.tioga_amd_amr_array.restype = C.c_long
@@ -298,8 +302,8 @@ class Tioga:
         return r
 
     # ---- serving: moving blocks over the resident background ----
-    def serve_open(self, bg_tag: int, nslots: int = 4, budget_gb: float = 64.0) -> None:
-        self.L.tioga_amd_serve_open(bg_tag, nslots, budget_gb)
+    def serve_open(self, bg_tag: int, nslots: int = 4, budget_gb: float = 64.0, resident_tree: bool = False) -> None:
+        self.L.tioga_amd_serve_open2(bg_tag, nslots, budget_gb, 1 if resident_tree else 0)
         self._check()
 
     def serve_submit(self, shell_tag: int, rot, center, shift) -> int:
@@ -316,13 +320,13 @@ class Tioga:
         bn = np.zeros(cap, np.int32)
         bv = np.zeros(cap, np.int32)
         tm = np.zeros(4)
-        ns = C.c_longlong(0)
-        self.L.tioga_amd_serve_wait(rid, _p(ib), nshell, _p(nbg), _p(bn), _p(bv), cap, _p(tm), C.byref(ns))
+        st = np.zeros(4, np.int64)
+        self.L.tioga_amd_serve_wait2(rid, _p(ib), nshell, _p(nbg), _p(bn), _p(bv), cap, _p(tm), _p(st))
         self._check()
         k = min(int(nbg[0]), cap)
         return {"shell_iblank": ib[:nshell], "bg_nodes": bn[:k].copy(), "bg_vals": bv[:k].copy(),
                 "t_submit": tm[0], "t_start": tm[1], "t_done": tm[2], "connectivity_ms": tm[3],
-                "nsearch": ns.value}
+                "nsearch": int(st[0]), "tieband": int(st[1]), "est_bytes": int(st[2]), "peak_bytes": int(st[3])}
 
     def serve_close(self) -> None:
         self.L.tioga_amd_serve_close()
