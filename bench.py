@@ -367,7 +367,7 @@ def serving_leg(slots: int, budget_gb: float, rates, seconds: float, max_request
     tg.close()
     resident = {"open_ms": round(open_ms, 1), "alone_p50_ms": round(float(np.median(r_alone)), 3),
                 "alone_max_ms": round(float(max(r_alone)), 3),
-                "alone_searches_per_s": round(r_s / (sum(r_alone) / 1000), 1),
+                "alone_searches_per_s": round(float(r_s / (sum(r_alone) / 1000)), 1),
                 "saturation_burst": r_burst, "tieband_queries": int(r_tie), "searches": int(r_s),
                 "requests_without_tieband": f"{r_clean}/16",
                 "what": "one tree over all background cells built at open; a request with tieband 0 "
@@ -642,7 +642,10 @@ def main() -> int:
         "preprocess_ms": round(1000 * t_pre, 2),
         "dataupdate": {"ms": round(du_ms, 4), "receptors": int(nrec), "nvar": nvar,
                        "algorithmic_bytes": bdu,
-                       "GBps": round(bdu / (du_ms / 1000) / 1e9, 2) if du_ms > 0 else None},
+                       "GBps": round(bdu / (du_ms / 1000) / 1e9, 2) if du_ms > 0 else None,
+                       "frac": round(bdu / (du_ms / 1000) / 1e9 / HBM_PEAK_GBPS, 4) if du_ms > 0 else None,
+                       "kernel": "k_update<0> (fused gather and scatter)",
+                       "traffic": profiled_traffic("dataupdate_config2.md", "traffic_bytes_per_update")},
         "reduce_fringes": rf,
         "amr_gather": amr,
         "batching": batching,
