@@ -28,8 +28,19 @@
 extern "C" {
 #endif
 
-/* MPICH's MPI_Comm is an int; the handle is not used (one rank). */
+/* The communicator of tioga_init_ (tiogaInterface.C:61).  The library runs
+   one rank and never reads the handle, and on x86-64 an int (MPICH's
+   MPI_Comm) and a pointer (Open MPI's) both travel in the first integer
+   register, so one library binds to either.  With <mpi.h> included first the
+   declaration takes the caller's own MPI_Comm; otherwise int, or the type
+   TIOGA_AMD_COMM_T names. */
+#if defined(TIOGA_AMD_COMM_T)
+typedef TIOGA_AMD_COMM_T tioga_comm_t;
+#elif defined(MPI_VERSION) && !defined(TIOGA_AMD_BUILDING)
+typedef MPI_Comm tioga_comm_t;
+#else
 typedef int tioga_comm_t;
+#endif
 
 /* tiogaInterface.C:36-59 */
 void tioga_init_f90_(int *scomm);

@@ -71,3 +71,21 @@ def test_status_api_without_device(lib):
     assert lib.tioga_amd_status(buf, 256) == 1
     assert b"tioga_init_" in buf.value
     lib.tioga_amd_clear_status()
+
+
+@pytest.mark.parametrize("prelude", [
+    "",  # no MPI header: an int handle (MPICH's MPI_Comm)
+    "#define MPI_VERSION 3\ntypedef int MPI_Comm;\n",  # MPICH-like
+    "#define MPI_VERSION 3\ntypedef struct ompi_communicator_t *MPI_Comm;\n",  # Open MPI-like
+])
+def test_header_binds_any_mpi_comm(tmp_path, prelude):
+    """tioga_init_ takes the caller's MPI_Comm when <mpi.h> comes first
+    (tiogaInterface.C:61); the C header compiles with each MPI's handle type."""
+    import subprocess
+
+    src = tmp_path / "use.c"
+    src.write_text(prelude + '#include "tioga_amd.h"\n'
+                   "void f(void) { tioga_comm_t c = 0; tioga_init_(c); }\n")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#define TIOGA_AMD_BUILDING  // the library side of tioga_comm_t: an int, never read
 #include "../../include/tioga.h"
 #include "../../include/tioga_amd.h"
 #include "tg_engine.h"
