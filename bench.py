@@ -437,9 +437,11 @@ def search_only(args, tg, blocks, desc) -> int:
     for _ in range(args.warmup):
         tg.time_query_sets(1)
     pass_ms, nq, kern_ms = tg.time_query_sets(args.steps)
+    stats = tg.search_stats()
     tg.close()
     print(json.dumps({"workload": desc, "steps": args.steps, "queries_per_step": int(nq),
-                      "search_pass_ms_device": round(pass_ms, 4), "search_kernel_ms": round(kern_ms, 4)}),
+                      "search_pass_ms_device": round(pass_ms, 4), "search_kernel_ms": round(kern_ms, 4),
+                      "search_stats": stats}),
           flush=True)
     return 0
 
@@ -559,6 +561,7 @@ def main() -> int:
     value = nq / (ms_per_step / 1000.0)
     # the donors the timed passes left (the path the rate is measured on)
     timed = [tg.query_donors(i, xq.shape[0]) for i, (_, xq) in zip(qids, sets)]
+    stats = tg.search_stats()
     timed_same = all(np.array_equal(a, b) for a, b in zip(timed, donors))
     # queries inside their tree's root box (the others are rejected by one box test)
     in_root = 0
@@ -632,6 +635,7 @@ def main() -> int:
         "timed_donors_equal_checked": bool(timed_same),
         "search_tuning": tuning,
         "queries_in_root_box": in_root,
+        "search_stats": stats,
         "searches_per_s_in_root_box": round(in_root / (ms_per_step / 1000.0), 1),
         "search_kernel_ms": round(kern_ms, 4),
         "search_pass_ms_device": round(pass_ms, 4),

@@ -29,7 +29,7 @@ SPACE = {
     "kstage": [16, 24, 32],
     "eval_grid": [1024, 2048, 4096, 8192],
 }
-DEFAULT = {"morton_bits": 21, "kstage": 32, "eval_grid": 2048}
+DEFAULT = {"morton_bits": 21, "kstage": 16, "eval_grid": 4096}
 
 
 def arch() -> str:

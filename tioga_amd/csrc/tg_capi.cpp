@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <exception>
 #include <mutex>
 #include <string>
@@ -559,6 +560,16 @@ void tioga_amd_query_set_clear(void)
   guarded("tioga_amd_query_set_clear", [] { tg_or_throw().engine()->query_set_clear(); });
 }
 
+int tioga_amd_search_stats(long long *out)
+{
+  int r = -1;
+  guarded("tioga_amd_search_stats", [&] {
+    tg_or_throw().engine()->search_stats(out);
+    r = 0;
+  });
+  return r;
+}
+
 double tioga_amd_last_search_kernel_ms(void)
 {
   double r = 0;
@@ -645,6 +656,17 @@ long tioga_amd_search_capacity(long cap)
     tg::Engine *e = tg_or_throw().engine();
     if (cap > 0) e->set_search_capacity((size_t)cap);
     r = (long)e->search_capacity();
+  });
+  return r;
+}
+
+long tioga_amd_search_spill(long chunks)
+{
+  long r = -1;
+  guarded("tioga_amd_search_spill", [&] {
+    tg::Engine *e = tg_or_throw().engine();
+    if (chunks > 0) e->set_search_spill_chunks((int)std::min<long>(chunks, 262144));
+    r = (long)e->search_spill_chunks();
   });
   return r;
 }

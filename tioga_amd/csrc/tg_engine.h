@@ -353,16 +353,18 @@ struct StepScratch {
 // (16, 24 or 32), workgroups of the containment-test kernels.
 struct SearchTuning {
   int morton_bits = 21;
-  int kstage = 32;
-  int eval_grid = 2048;
+  int kstage = 16;
+  int eval_grid = 4096;
 };
 
 struct SearchScratch {
   SearchTuning tune;
-  DBuf<int> cell, query, status, qoff, qcnt, ovf, counters, bins, blist, lj, lcount, qb, qsid, nin, wstart;
+  DBuf<int> cell, query, status, qoff, qcnt, ovf, counters, bins, blist, lj, lcount, qb, qsid, nin, wstart, spill;
+  int spill_chunks = 0;  // chunks of the walk's spill pool
   DBuf<unsigned char> sv;  // the table of searches of one launch
   DBuf<double4> lst;
   size_t cap = 0;  // capacity of the flat candidate list
+  int last_ns = 0, last_nqt = 0;  // searches and queries of the last launch
   // the continued Newton solves run on a second stream (default priority),
   // beside the tet tests; created on first use
   hipStream_t side = nullptr;
@@ -546,6 +548,13 @@ class Engine {
   // mean device ms per pass of the k_search launches alone in the last
   // time_query_sets (the pass also orders the queries)
   double last_search_kernel_ms() const { return last_search_kernel_ms_; }
+  // counts of the last search launch: out[0] = candidates listed, out[1] =
+  // queries with more candidates than the walk stages, out[2] = queries that
+  // walked a tree, out[3] = queries, out[4..7] = candidates whose test
+  // rejected the point, accepted it with a near-zero weight in a BIGVALUE
+  // cell, accepted it cleanly, found it outside the element box
+  // (synchronises the stream)
+  void search_stats(long long out[8]);
   int query_set_nelem(int id) const { return (id >= 0 && id < (int)qsets_.size()) ? qsets_[id]->nelem : -1; }
   // donors of set id as the last pass over it (searched or timed) left them
   void query_set_donors(int id, int *donor);
@@ -556,6 +565,10 @@ class Engine {
   size_t search_capacity() const { return sw_.cap; }
   SearchTuning &search_tuning() { return sw_.tune; }
   void set_search_capacity(size_t cap) { sw_.cap = cap; }
+  // chunks of the walk's spill pool the next search starts from (0 until
+  // the first search sizes it); a checked search grows it when it runs out
+  int search_spill_chunks() const { return sw_.spill_chunks; }
+  void set_search_spill_chunks(int n) { sw_.spill_chunks = n; }
 
   // AMR Cartesian background, gather direction (tg_amr.hip):
   // tioga::register_amr_global_data / set_amr_patch_count /

@@ -1187,8 +1187,16 @@ struct CandOut {
   int *qoff, *qcnt;
   int *used;  // entries allocated so far
   int cap;    // capacity of the flat list
-  int *ovf, *novf;  // queries with more candidates than k_collect stages
+  int *ovf, *novf;  // queries whose candidates k_collect could not hold
+  // Spill pool of k_collect: chunks of SPILL_K entries for each of a wave's
+  // 64 lanes (entry k of lane l at chunk * SPILL_K * 64 + k * 64 + l), taken
+  // by a wave when one of its lanes has more than KSTAGE candidates.
+  int *spill;
+  int spill_chunks;  // chunks in the pool
+  int *nspill;       // chunks taken so far
 };
+constexpr int SPILL_K = 16;    // entries per lane in one spill chunk
+constexpr int SPILL_MAX = 64;  // chunks one wave may hold (KSTAGE + 1024 entries per lane)
 
 // ADT::searchADT / searchIntersections (searchADTrecursion.C:26-101).  The
 // reference walks the tree in preorder (a node's own element, then its left
@@ -1221,15 +1229,19 @@ struct CandOut {
 // spatially close) walk the tree together: the wave visits a node when any
 // lane's own walk would, carrying the mask of those lanes, so every lane sees
 // exactly its own preorder walk while node records are wave-uniform scalar
-// loads.  Candidates are staged in LDS and copied out to a chunk the wave
-// allocates at the end; a lane with more than KSTAGE candidates is queued for
-// k_collect_more, which lists all of them.  KSTAGE (16, 24 or 32 entries per
-// lane, 4-8 KB of LDS per wave) trades occupancy against queries sent to the
-// slower one-lane walk; the result does not depend on it.
+// loads.  Candidates are staged in LDS (KSTAGE per lane, 4-8 KB per wave) and
+// copied out to a chunk the wave allocates at the end.  A lane's candidates
+// beyond KSTAGE go to spill chunks the wave takes from a global pool (one
+// atomic per SPILL_K entries of its longest list), in the same lane-major
+// layout as the stage.  Only when the pool is exhausted, or a list exceeds
+// KSTAGE + SPILL_K * SPILL_MAX, is the lane queued for k_collect_more, which
+// walks its query again alone and lists all its candidates.  KSTAGE trades
+// occupancy against spill traffic; the result does not depend on it.
 template <int KSTAGE>
 __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
 {
   __shared__ int stage[KSTAGE][WAVE];
+  __shared__ int chunk[SPILL_MAX];  // the wave's spill chunks, in order
   const int lane = threadIdx.x;
   // workgroup wv walks walking wave wv; the grid is sized for every query,
   // so the workgroups past the walking waves leave at once
@@ -1253,15 +1265,19 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
     for (int d = 0; d < 3; d++) q[d] = grid_of(p[d], S.g.lo[d], S.g.inv[d]);
   }
   int nc = 0;
+  // spill chunks held (wave-uniform); spill_ok false once the pool or the
+  // wave's chunk table is exhausted (lanes past KSTAGE then go to the queue)
+  int nchunk = 0;
+  bool spill_ok = true;
   // pending right subtrees: entry k (node, lane mask) held by lane k, so a
   // push is a select and a pop a readlane; the left child, when any lane
   // enters it, is visited next without the stack (preorder)
   int vnode = 0;
   unsigned vmlo = 0, vmhi = 0;
   int sp = 0;
-  int node = 0;
   unsigned long long m = __ballot(inroot);
   bool more = m != 0ull;
+  int node = 0;
   while (more) {
     // the whole 48-byte record in one batch of scalar loads, then the tests
     WalkNode N;
@@ -1277,10 +1293,31 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
       N.right = r->right;
     }
     const bool act = (m >> lane) & 1ull;
-    if (act & in_grid(q, N.e)) {
-      if (nc < KSTAGE) stage[nc][lane] = N.cellnv;
-      nc++;
+    const bool hit = act & in_grid(q, N.e);
+    if (hit & (nc < KSTAGE)) stage[nc][lane] = N.cellnv;
+    // a lane past the stage writes entry e = nc - KSTAGE to chunk e / SPILL_K;
+    // lists grow by one entry per node, so a new chunk is needed exactly when
+    // some lane reaches entry SPILL_K * nchunk
+    const int e = nc - KSTAGE;
+    if (__ballot(hit & (e >= 0)) && spill_ok) {
+      if (__ballot(hit & (e == SPILL_K * nchunk))) {
+        int id = O.spill_chunks;
+        if (nchunk < SPILL_MAX) {
+          if (lane == 0) id = atomicAdd(O.nspill, 1);
+          id = __builtin_amdgcn_readfirstlane(id);
+        }
+        if (id < O.spill_chunks) {
+          if (lane == 0) chunk[nchunk] = id;
+          __syncthreads();  // one wave per workgroup: orders lane 0's write before the reads
+          nchunk++;
+        } else {
+          spill_ok = false;
+        }
+      }
+      if (spill_ok && hit && e >= 0)
+        O.spill[((size_t)chunk[e / SPILL_K] * SPILL_K + e % SPILL_K) * WAVE + lane] = N.cellnv;
     }
+    nc += hit;
     const int r = N.right, l = N.left;
     const unsigned long long mr = (r > -1) ? __ballot(act & in_grid(q, N.r)) : 0ull;
     const unsigned long long ml = (l > -1) ? __ballot(act & in_grid(q, N.l)) : 0ull;
@@ -1305,7 +1342,8 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
     }
   }
   // one chunk of the flat list per wave
-  const int k = (nc <= KSTAGE) ? nc : 0;  // overflowing lanes are listed later
+  const bool held = spill_ok || nc <= KSTAGE;  // every candidate staged or spilled
+  const int k = held ? nc : 0;                 // the others are listed later
   int pre = k;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -1320,7 +1358,7 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
   if (i >= 0) {
     const int gp = qb + t, g = qb + i;
     O.qcnt[gp] = nc;
-    if (nc > KSTAGE) {
+    if (!held) {
       O.ovf[atomicAdd(O.novf, 1)] = gp;
     } else {
       const int off = base + pre;
@@ -1330,8 +1368,14 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
         // again): the reserved slots below the capacity are marked empty
         for (int j = off; j < O.cap && j < off + k; j++) O.cell[j] = EMPTY_SLOT;
       } else {
-        for (int j = 0; j < k; j++) {
+        const int ks = k < KSTAGE ? k : KSTAGE;
+        for (int j = 0; j < ks; j++) {
           O.cell[off + j] = stage[j][lane];
+          O.query[off + j] = g;
+        }
+        for (int j = KSTAGE; j < k; j++) {
+          const int e2 = j - KSTAGE;
+          O.cell[off + j] = O.spill[((size_t)chunk[e2 / SPILL_K] * SPILL_K + e2 % SPILL_K) * WAVE + lane];
           O.query[off + j] = g;
         }
       }
@@ -1339,8 +1383,8 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
   }
 }
 
-// the walk of one query, listing all its candidates (queries with more than
-// KSTAGE); a grid-stride loop over the queue
+// the walk of one query, listing all its candidates (queries whose lists
+// k_collect could not hold); a grid-stride loop over the queue
 __global__ void __launch_bounds__(64) k_collect_more(SearchSet T, CandOut O)
 {
   __shared__ int stk[STACK][64];
@@ -1543,6 +1587,23 @@ __global__ void __launch_bounds__(256, WAVES) k_eval(SearchSet T, CandOut O, con
     }
     O.status[j] = test_status<NV>(frac, S.cellRes[c] == BIGVALUE);
   }
+}
+
+// how many listed candidates ended with each status (0 reject, 1 accepted
+// with a near-zero weight, 2 clean, 3 outside the element box); a diagnostic
+__global__ void __launch_bounds__(256) k_status_hist(CandOut O, int *__restrict__ hist)
+{
+  __shared__ int loc[4];
+  if (threadIdx.x < 4) loc[threadIdx.x] = 0;
+  __syncthreads();
+  const int n = min(*O.used, O.cap);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    if (O.cell[j] == EMPTY_SLOT) continue;
+    const int st = O.status[j];
+    if (st >= 0 && st < 4) atomicAdd(&loc[st], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < 4 && loc[threadIdx.x]) atomicAdd(&hist[threadIdx.x], loc[threadIdx.x]);
 }
 
 // the queued solves, continued from their saved state to the end
@@ -2290,6 +2351,8 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     if (qb[k + 1] < qb[k]) throw std::runtime_error("tioga_amd: too many queries in one search launch");
   }
   const int nqt = qb[ns];
+  w.last_ns = ns;
+  w.last_nqt = nqt;
   if (nqt <= 0) return;
   // the processing order, walking counts and the query -> search map come
   // from the ordering step (Engine::order_sets) of this launch
@@ -2302,7 +2365,8 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
   w.qoff.reserve(nqt);
   w.qcnt.reserve(nqt);
   w.ovf.reserve(nqt);
-  w.counters.reserve(2);
+  w.counters.reserve(3);
+  if (w.spill_chunks == 0) w.spill_chunks = 16384;  // 64 MiB; grown when a checked pass runs out
   const SearchSet T{(const SearchView *)w.sv.p, w.qb.p, w.nin.p, w.wstart.p, w.qsid.p, ns, nqt};
   long long nwave = 0;  // an upper bound of the walking waves
   for (int k = 0; k < ns; k++) nwave += (S[k].nq + WAVE - 1) / WAVE;
@@ -2311,21 +2375,25 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     w.cell.reserve(w.cap);
     w.query.reserve(w.cap);
     w.status.reserve(w.cap);
+    w.spill.reserve((size_t)w.spill_chunks * SPILL_K * WAVE);
     CandOut O{w.cell.p, w.query.p, w.status.p, w.qoff.p, w.qcnt.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
-              w.ovf.p, w.counters.p + 1};
-    fill<int>(w.counters.p, 2, 0, s);
+              w.ovf.p, w.counters.p + 1, w.spill.p, w.spill_chunks, w.counters.p + 2};
+    fill<int>(w.counters.p, 3, 0, s);
     switch (w.tune.kstage) {
       case 16: k_collect<16><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
       case 24: k_collect<24><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
       default: k_collect<32><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
     }
-    k_collect_more<<<1024, 64, 0, s>>>(T, O);
+    k_collect_more<<<8192, 64, 0, s>>>(T, O);
     if (check) {
-      int used = 0;
-      TG_HIP(hipMemcpyAsync(&used, w.counters.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      int c[3] = {0, 0, 0};
+      TG_HIP(hipMemcpyAsync(c, w.counters.p, sizeof c, hipMemcpyDeviceToHost, s));
       TG_HIP(hipStreamSynchronize(s));
-      if ((size_t)used > w.cap) {  // grow the flat list and walk again
-        w.cap = (size_t)used + used / 4;
+      // a spill pool that ran out only sent lists to k_collect_more; the
+      // next launch gets a larger one (at most 1 GiB)
+      if (c[2] > w.spill_chunks) w.spill_chunks = std::min(2 * c[2], 262144);
+      if ((size_t)c[0] > w.cap) {  // grow the flat list and walk again
+        w.cap = (size_t)c[0] + c[0] / 4;
         continue;
       }
     }
@@ -3714,6 +3782,30 @@ void Engine::launch_query_sets(const std::vector<QuerySet *> &qs)
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
   run_search(sv.data(), (int)sv.size(), sw_, s_, !timing_);
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
+}
+
+void Engine::search_stats(long long out[8])
+{
+  SearchScratch &w = sw_;
+  for (int k = 0; k < 8; k++) out[k] = 0;
+  out[3] = w.last_nqt;
+  if (w.last_nqt <= 0 || w.last_ns <= 0) return;
+  int c[2] = {0, 0}, h[4] = {0, 0, 0, 0};
+  std::vector<int> nin(w.last_ns);
+  DBuf<int> hist;
+  hist.reserve(4);
+  fill<int>(hist.p, 4, 0, s_);
+  CandOut O{w.cell.p, w.query.p, w.status.p, w.qoff.p, w.qcnt.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
+            w.ovf.p, w.counters.p + 1, w.spill.p, w.spill_chunks, w.counters.p + 2};
+  k_status_hist<<<1024, 256, 0, s_>>>(O, hist.p);
+  TG_HIP(hipMemcpyAsync(c, w.counters.p, sizeof c, hipMemcpyDeviceToHost, s_));
+  TG_HIP(hipMemcpyAsync(h, hist.p, sizeof h, hipMemcpyDeviceToHost, s_));
+  TG_HIP(hipMemcpyAsync(nin.data(), w.nin.p, sizeof(int) * w.last_ns, hipMemcpyDeviceToHost, s_));
+  TG_HIP(hipStreamSynchronize(s_));
+  out[0] = c[0];
+  out[1] = c[1];
+  for (int v : nin) out[2] += v;
+  for (int k = 0; k < 4; k++) out[4 + k] = h[k];
 }
 
 void Engine::query_set_search(int id, int *donor)

@@ -67,6 +67,10 @@ def load():
         L.tioga_amd_query_set_clear.argtypes = []
         L.tioga_amd_last_search_kernel_ms.argtypes = []
         L.tioga_amd_last_search_kernel_ms.restype = C.c_double
+        L.tioga_amd_search_stats.argtypes = [_P]
+        L.tioga_amd_search_spill.argtypes = [C.c_long]
+        L.tioga_amd_search_spill.restype = C.c_long
+        L.tioga_amd_search_stats.restype = C.c_int
         L.tioga_amd_query_set_nelem.argtypes = [C.c_int]
         L.tioga_amd_query_set_nelem.restype = C.c_int
         L.tioga_amd_register_gids.argtypes = [C.c_int, _P, _P]
@@ -424,6 +428,11 @@ class Tioga:
             self._check()
         return {"morton_bits": int(cur[0]), "kstage": int(cur[1]), "eval_grid": int(cur[2])}
 
+    def search_spill(self, chunks: int = 0) -> int:
+        r = self.L.tioga_amd_search_spill(int(chunks))
+        self._check()
+        return int(r)
+
     def search_capacity(self, cap: int = 0) -> int:
         r = self.L.tioga_amd_search_capacity(int(cap))
         self._check()
@@ -435,6 +444,17 @@ class Tioga:
         ms = self.L.tioga_amd_time_query_sets(nrep, C.byref(n))
         self._check()
         return ms, n.value, self.L.tioga_amd_last_search_kernel_ms()
+
+    def search_stats(self) -> Dict[str, int]:
+        """Counts of the last search launch: candidates listed by the walk,
+        queries with more than the staged candidates, walking queries,
+        queries, and the candidates by test outcome."""
+        out = np.zeros(8, np.int64)
+        self.L.tioga_amd_search_stats(_p(out))
+        self._check()
+        return {"candidates": int(out[0]), "overflow_queries": int(out[1]), "walking_queries": int(out[2]),
+                "queries": int(out[3]), "rejected": int(out[4]), "accepted_near_zero": int(out[5]),
+                "accepted": int(out[6]), "outside_element_box": int(out[7])}
 
     def phase_times(self) -> Dict[str, float]:
         out = np.zeros(len(PHASES))
