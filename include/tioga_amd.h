@@ -217,7 +217,10 @@ double tioga_amd_last_search_kernel_ms(void);
    coincident copy), out[3] = queries, out[4..7] = listed candidates whose
    containment test rejected the point, accepted it with a near-zero weight
    in a BIGVALUE cell, accepted it cleanly, or found it outside the element's
-   padded box (8 values); returns 0, or -1 with the error status set */
+   padded box, out[8] = candidates ranked after their query's lowest-ranked
+   clean hit, which the reference's walk never tests, out[9] = 1 when the
+   candidates were listed from bin grids, 0 by the tree walk (10 values);
+   returns 0, or -1 with the error status set */
 int tioga_amd_search_stats(long long *out);
 void tioga_amd_query_set_clear(void);
 /* number of cells in a query set's search structure (-1: bad id) */
@@ -246,6 +249,13 @@ long tioga_amd_search_spill(long chunks);
    receives the current ones; returns 0, or -1 with the error status set.
    tioga_amd/autotune.py chooses them by measurement. */
 int tioga_amd_search_tuning(const int *set, int *get);
+/* How the donor search lists each query's candidate cells, which changes no
+   result: bins > 0 lists them from a bin grid built with each search tree,
+   about `bins` elements per bin (a few hundred MB for a tree of millions of
+   cells); 0 walks the tree instead.  bins >= 0 sets it (the registered
+   query sets' grids are built again), -1 only queries; returns the current
+   value, or -1 with the error status set. */
+int tioga_amd_search_bins(int bins);
 /* tioga_preprocess_grids_ for a solver that moved one block only: block
    btag is preprocessed in full (tioga.C:95-110), every other block keeps the
    geometry of its last preprocess (its coordinates, resolutions and

@@ -192,12 +192,14 @@ def test_query_sets_candidate_list_regrows():
         o.close()
 
 
-@pytest.mark.parametrize("tuning", [dict(morton_bits=0, kstage=16, eval_grid=1024),
-                                    dict(morton_bits=30, kstage=24, eval_grid=8192),
-                                    dict(morton_bits=9, kstage=32, eval_grid=64)])
+@pytest.mark.parametrize("tuning", [dict(morton_bits=0, kstage=16, eval_grid=1024, bins=0),
+                                    dict(morton_bits=30, kstage=24, eval_grid=8192, bins=1),
+                                    dict(morton_bits=9, kstage=32, eval_grid=64, bins=64),
+                                    dict(morton_bits=21, kstage=16, eval_grid=4096, bins=4096)])
 def test_query_sets_any_tuning(tuning):
-    """The search's launch parameters (Morton bits, walk staging depth,
-    evaluation grid) change no donor."""
+    """The search's parameters (candidates from bin grids of any density or
+    from the tree walk, Morton bits, staging depth, evaluation grid) change
+    no donor."""
     print(f"\n[query sets] tuning {tuning}", flush=True)
     blocks = _blocks("mid")
     o = _oracle(blocks)
@@ -207,6 +209,7 @@ def test_query_sets_any_tuning(tuning):
         assert got == tuning
         sets = [(blocks[0].tag, blocks[1].x), (blocks[1].tag, blocks[0].x)]
         _check_sets(t, o, sets, f"tuning {tuning}")
+        assert t.search_stats()["listing"] == ("bins" if tuning["bins"] else "walk")
     finally:
         t.close()
         o.close()

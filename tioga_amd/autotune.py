@@ -1,10 +1,11 @@
 """Launch parameters of the donor search, chosen by measurement.
 
-The search's launch parameters -- the Morton bits of the query order (the
-sort's passes against the walk's coherence), the candidates each lane of
-the tree walk stages in LDS (occupancy against queries left to the one-lane
-walk), and the workgroups of each containment-test kernel -- change no
-result, only speed.  :func:`tune` measures the registered query sets of an
+The search's parameters -- how candidates are listed (bin grids of about
+`bins` elements per bin, or the tree walk when 0), the Morton bits of the
+query order (the sort's passes against the listing's coherence), the
+candidates each lane stages in LDS (occupancy against spill traffic), and
+the workgroups of each containment-test kernel -- change no result, only
+speed.  :func:`tune` measures the registered query sets of an
 engine over a grid of values by coordinate descent (each parameter swept
 with the others held, twice round) and returns the fastest; the table
 ``search_tuning.json`` beside this file keeps the chosen values per device
@@ -25,11 +26,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "search_tuning.json")
 
 SPACE = {
+    "bins": [0, 1, 2, 4, 8],
     "morton_bits": [0, 9, 15, 21, 30],
     "kstage": [16, 24, 32],
     "eval_grid": [1024, 2048, 4096, 8192],
 }
-DEFAULT = {"morton_bits": 21, "kstage": 16, "eval_grid": 4096}
+DEFAULT = {"morton_bits": 21, "kstage": 16, "eval_grid": 4096, "bins": 2}
 
 
 def arch() -> str:
@@ -60,11 +62,11 @@ def tune(tg, steps: int = 5, rounds: int = 2, log=None) -> Tuple[Dict[str, int],
     """Coordinate descent over SPACE on the engine's registered query sets;
     returns (best parameters, every measurement as {params..., "ms"})."""
     best = dict(tg.search_tuning())
-    seen: Dict[Tuple[int, int, int], float] = {}
+    seen: Dict[Tuple[int, ...], float] = {}
     trials: List[dict] = []
 
     def cost(p):
-        key = (p["morton_bits"], p["kstage"], p["eval_grid"])
+        key = tuple(p[k] for k in sorted(SPACE))
         if key not in seen:
             seen[key] = _measure(tg, p, steps)
             trials.append(dict(p, ms=round(seen[key], 4)))
@@ -95,7 +97,7 @@ def load(workload: str, device: Optional[str] = None) -> Optional[Dict[str, int]
     except OSError:
         return None
     entry = t.get(device or arch(), {}).get(workload)
-    return None if entry is None else {k: int(entry[k]) for k in DEFAULT}
+    return None if entry is None else {k: int(entry.get(k, DEFAULT[k])) for k in DEFAULT}
 
 
 def apply(tg, workload: str, device: Optional[str] = None) -> Dict[str, int]:

@@ -649,6 +649,17 @@ int tioga_amd_search_tuning(const int *set, int *get)
   return rc;
 }
 
+int tioga_amd_search_bins(int bins)
+{
+  int r = -1;
+  guarded("tioga_amd_search_bins", [&] {
+    tg::Engine *e = tg_or_throw().engine();
+    if (bins >= 0) e->set_search_bins(bins);
+    r = e->search_tuning().bins;
+  });
+  return r;
+}
+
 long tioga_amd_search_capacity(long cap)
 {
   long r = -1;

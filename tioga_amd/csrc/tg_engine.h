@@ -156,6 +156,18 @@ struct InterpHost {
   double w[9];
 };
 
+// Bins of one tree's walk records, for listing candidates without the walk:
+// the tree's 16-bit grid cut into 2^(16 - shift) bins per axis; bin b lists,
+// in any order, (packed grid box, node id | nv << 28) of every element whose
+// grid box overlaps it, at ent[off[b] .. off[b + 1]).  off holds indices into
+// the ent array of the batch the tree was built in (its own when built alone).
+struct BinGrid {
+  int shift = 16;
+  long long nbins = 0;  // 0: none built
+  DBuf<int> off;
+  DBuf<uint4> ent;
+};
+
 // A search structure kept over every cell of a block (a served background):
 // searches against the block walk it instead of a tree built per query set.
 struct ResidentTree {
@@ -164,6 +176,7 @@ struct ResidentTree {
   DBuf<int> elist, adtInt;
   DBuf<double> ebox, adtReal;
   DBuf<WalkNode> nodes;
+  BinGrid bins;
 };
 
 struct Block {
@@ -212,6 +225,7 @@ struct Block {
   DBuf<int> elist, adtInt;
   DBuf<double> ebox, adtReal;
   DBuf<WalkNode> nodes;
+  BinGrid bins;
   double adtExt[6] = {0, 0, 0, 0, 0, 0};
   // searches against this block walk this tree (read only; another engine
   // owns it) when set; queries whose donor could depend on the walk order
@@ -241,6 +255,7 @@ struct QuerySet {
   DBuf<int> elist, adtInt;
   DBuf<double> ebox, adtReal;
   DBuf<WalkNode> nodes;
+  BinGrid bins;
   double ext[6] = {0, 0, 0, 0, 0, 0};
 };
 
@@ -355,6 +370,9 @@ struct SearchTuning {
   int morton_bits = 21;
   int kstage = 16;
   int eval_grid = 4096;
+  // candidates listed from bin grids with about this many elements per bin
+  // (built with each tree; 0: by the tree walk)
+  int bins = 2;
 };
 
 struct SearchScratch {
@@ -365,6 +383,7 @@ struct SearchScratch {
   DBuf<double4> lst;
   size_t cap = 0;  // capacity of the flat candidate list
   int last_ns = 0, last_nqt = 0;  // searches and queries of the last launch
+  bool last_bins = false;         // the last launch listed candidates from bin grids
   // the continued Newton solves run on a second stream (default priority),
   // beside the tet tests; created on first use
   hipStream_t side = nullptr;
@@ -552,9 +571,11 @@ class Engine {
   // queries with more candidates than the walk stages, out[2] = queries that
   // walked a tree, out[3] = queries, out[4..7] = candidates whose test
   // rejected the point, accepted it with a near-zero weight in a BIGVALUE
-  // cell, accepted it cleanly, found it outside the element box
+  // cell, accepted it cleanly, found it outside the element box, out[8] =
+  // candidates ranked after their query's lowest-ranked clean hit, out[9] =
+  // 1 when the candidates came from bin grids, 0 from the tree walk
   // (synchronises the stream)
-  void search_stats(long long out[8]);
+  void search_stats(long long out[10]);
   int query_set_nelem(int id) const { return (id >= 0 && id < (int)qsets_.size()) ? qsets_[id]->nelem : -1; }
   // donors of set id as the last pass over it (searched or timed) left them
   void query_set_donors(int id, int *donor);
@@ -564,6 +585,9 @@ class Engine {
   // grows whenever a checked search finds it too small
   size_t search_capacity() const { return sw_.cap; }
   SearchTuning &search_tuning() { return sw_.tune; }
+  // the bin density of the search (0: the tree walk); the query sets' bin
+  // grids are built again at a new density, trees built later use it
+  void set_search_bins(int bins);
   void set_search_capacity(size_t cap) { sw_.cap = cap; }
   // chunks of the walk's spill pool the next search starts from (0 until
   // the first search sizes it); a checked search grows it when it runs out
@@ -625,7 +649,12 @@ class Engine {
 
   void build_search_structure(Block &b, const OBB &obq, int &nelem, int &depth, DBuf<int> &elist,
                               DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, DBuf<WalkNode> &nodes,
-                              double ext[6]);
+                              double ext[6], BinGrid &bins);
+  void build_bins(int ntree, const WalkNode *const *nodes, const int *nelem, BinGrid &store, BinGrid *const *views);
+  BinGrid tbins_;                      // the bin grids of every batched tree
+  DBuf<int> bcnt_;                     // bin counts of a bin build
+  DBuf<unsigned char> btab_;           // the tree table of a bin build
+  DBuf<unsigned long long> bsums_;     // entries per tree and shift of a bin build
   void launch_query_sets(const std::vector<QuerySet *> &qs);
   const int *order_sets(const std::vector<MortonSet> &M, size_t tot);
   SearchScratch sw_;

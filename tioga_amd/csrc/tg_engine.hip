@@ -1098,6 +1098,11 @@ struct SearchView {
   const double *x;
   const double *cellRes;
   int *tieband = nullptr;  // counter of tie-band queries (resident trees), or null
+  // the tree's bin grid (k_find), or boff null: bin b's list is
+  // bent[boff[b] .. boff[b + 1]), 2^(16 - bshift) bins per axis
+  const int *boff = nullptr;
+  const uint4 *bent = nullptr;
+  int bshift = 16;
 };
 
 // Several searches (one per block, each with its own tree and query set)
@@ -1147,6 +1152,9 @@ __device__ __forceinline__ SearchView view_of(const SearchSet &T, int sid)
   S.x = as_global(v->x);
   S.cellRes = as_global(v->cellRes);
   S.tieband = v->tieband;
+  S.boff = as_global(v->boff);
+  S.bent = as_global(v->bent);
+  S.bshift = v->bshift;
   return S;
 }
 
@@ -1202,46 +1210,128 @@ constexpr int SPILL_MAX = 64;  // chunks one wave may hold (KSTAGE + 1024 entrie
 // reference walks the tree in preorder (a node's own element, then its left
 // and right subtrees whose boxes hold the point), tests every element whose
 // box holds the point, lets each test overwrite the result and stops at the
-// first clean hit.  Box tests have no side effects, so the same walk without
-// the tests lists the candidates in the order the reference tests them, and
-// the reference's result is a function of that list and the candidates'
-// test outcomes alone: the first clean hit if there is one, otherwise the
-// outcome of the last candidate (its cell when accepted with a near-zero
-// weight in a BIGVALUE cell, else -1).  So the search runs as
-//   k_collect   the walk, listing candidates in walk order,
-//   k_eval      every candidate's containment test, one thread each,
-//   k_select    the rule above, per query,
+// first clean hit.  Box tests have no side effects, and the ADT numbers its
+// nodes in preorder (buildADTrecursion.c:55-58 counts a node before its
+// subtrees), so the reference tests exactly the candidates -- the elements
+// whose padded box holds the point -- in increasing node id.  An element
+// whose padded box holds the point lies inside every ancestor's padded
+// subtree box (subtree boxes are min/max folds that include it), so the walk
+// reaches every candidate.  The reference's result is therefore a function of
+// the candidate set and each candidate's node id ("rank") and test outcome:
+// the cell of the lowest-ranked clean hit, and when there is none, the
+// outcome of the highest-ranked candidate (its cell when accepted with a
+// near-zero weight in a BIGVALUE cell, else -1).  So the search runs as
+//   k_collect / k_find  list each query's candidates (as rank | nv << 28),
+//   k_eval              every candidate's containment test, one thread each,
+//   k_select            the rule above, per query, in any listing order,
 // which keeps long candidate lists and slow Newton solves from serialising
 // inside one lane.
 //
-// The walk tests the coarse grid boxes of the walk records (WalkNode), which
-// pass whenever the reference's padded FP64 tests pass, so it visits a
-// superset of the reference's nodes, in the same preorder, and lists a
-// superset of its candidates.  k_eval makes the exact FP64 element-box test
-// of each listed candidate (the element box is the min/max of the cell's
-// vertices, search.C:186-214) and marks the ones that fail OUTSIDE, which
-// k_select skips.  What remains is exactly the reference's list: an element
-// whose padded box holds the point lies inside every ancestor's padded
-// subtree box (subtree boxes are min/max folds that include it), so the
-// reference's walk reaches it.
+// Both listing kernels test the coarse 16-bit grid boxes of the walk records
+// (WalkNode), which pass whenever the reference's padded FP64 test passes,
+// so they list a superset of the candidates.  k_eval makes the exact FP64
+// element-box test of each listed candidate (the element box is the min/max
+// of the cell's vertices, search.C:186-214) and marks the ones that fail
+// OUTSIDE, which k_select skips.
 //
+// Listing: candidates are staged in LDS (KSTAGE per lane, 2-8 KB per wave)
+// and copied out to a chunk of the flat list the wave allocates at the end.
+// A lane's candidates beyond KSTAGE go to spill chunks the wave takes from a
+// global pool (one atomic per SPILL_K entries of its longest list), in the
+// same lane-major layout as the stage.  Only when the pool is exhausted, or a
+// list exceeds KSTAGE + SPILL_K * SPILL_MAX, is the lane queued for
+// k_collect_more, which walks its query again alone and lists all its
+// candidates.  KSTAGE trades occupancy against spill traffic; the result does
+// not depend on it.
+template <int KSTAGE>
+struct Stager {
+  int *stage;  // LDS, [KSTAGE][WAVE]
+  int *chunk;  // LDS, the wave's spill chunks in order
+  int lane, nc = 0, nchunk = 0;
+  bool spill_ok = true;  // false once the pool or the wave's chunk table ran out
+  // one listing step of the wave: each lane adds at most one candidate, so a
+  // new spill chunk is needed exactly when some lane reaches entry
+  // SPILL_K * nchunk past the stage.  Called in wave-uniform control flow.
+  __device__ __forceinline__ void emit(bool hit, int val, const CandOut &O)
+  {
+    if (hit & (nc < KSTAGE)) stage[nc * WAVE + lane] = val;
+    const int e = nc - KSTAGE;
+    if (__ballot(hit & (e >= 0)) && spill_ok) {
+      if (__ballot(hit & (e == SPILL_K * nchunk))) {
+        int id = O.spill_chunks;
+        if (nchunk < SPILL_MAX) {
+          if (lane == 0) id = atomicAdd(O.nspill, 1);
+          id = __builtin_amdgcn_readfirstlane(id);
+        }
+        if (id < O.spill_chunks) {
+          if (lane == 0) chunk[nchunk] = id;
+          __syncthreads();  // one wave per workgroup: orders lane 0's write before the reads
+          nchunk++;
+        } else {
+          spill_ok = false;
+        }
+      }
+      if (spill_ok && hit && e >= 0) O.spill[((size_t)chunk[e / SPILL_K] * SPILL_K + e % SPILL_K) * WAVE + lane] = val;
+    }
+    nc += hit;
+  }
+  // one chunk of the flat list per wave; valid: the lane holds query g at
+  // position gp.  Called in wave-uniform control flow.
+  __device__ __forceinline__ void flush(const CandOut &O, bool valid, int gp, int g)
+  {
+    const bool held = spill_ok || nc <= KSTAGE;  // every candidate staged or spilled
+    const int k = (valid && held) ? nc : 0;      // the others are listed later
+    int pre = k;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      int v = __shfl_up(pre, o);
+      if (lane >= o) pre += v;
+    }
+    const int total = __shfl(pre, 63);
+    pre -= k;
+    int base = 0;
+    if (lane == 0 && total > 0) base = atomicAdd(O.used, total);
+    base = __shfl(base, 0);
+    if (!valid) return;
+    O.qcnt[gp] = nc;
+    if (!held) {
+      O.ovf[atomicAdd(O.novf, 1)] = gp;
+      return;
+    }
+    const int off = base + pre;
+    O.qoff[gp] = off;
+    if (off + k > O.cap) {
+      // flat list too small (the host sees used > cap, grows it and lists
+      // again): the reserved slots below the capacity are marked empty
+      for (int j = off; j < O.cap && j < off + k; j++) O.cell[j] = EMPTY_SLOT;
+      return;
+    }
+    const int ks = k < KSTAGE ? k : KSTAGE;
+    for (int j = 0; j < ks; j++) {
+      O.cell[off + j] = stage[j * WAVE + lane];
+      O.query[off + j] = g;
+    }
+    for (int j = KSTAGE; j < k; j++) {
+      const int e2 = j - KSTAGE;
+      O.cell[off + j] = O.spill[((size_t)chunk[e2 / SPILL_K] * SPILL_K + e2 % SPILL_K) * WAVE + lane];
+      O.query[off + j] = g;
+    }
+  }
+};
+
+// the listed value of a walk record: its node id (the rank) with the vertex count
+__device__ __forceinline__ int rank_nv(int node, int cellnv) { return node | (cellnv & (int)0xf0000000); }
+
 // k_collect: the 64 queries of a wave (consecutive in Morton order, so
-// spatially close) walk the tree together: the wave visits a node when any
+// spatially close) walk the ADT together: the wave visits a node when any
 // lane's own walk would, carrying the mask of those lanes, so every lane sees
 // exactly its own preorder walk while node records are wave-uniform scalar
-// loads.  Candidates are staged in LDS (KSTAGE per lane, 4-8 KB per wave) and
-// copied out to a chunk the wave allocates at the end.  A lane's candidates
-// beyond KSTAGE go to spill chunks the wave takes from a global pool (one
-// atomic per SPILL_K entries of its longest list), in the same lane-major
-// layout as the stage.  Only when the pool is exhausted, or a list exceeds
-// KSTAGE + SPILL_K * SPILL_MAX, is the lane queued for k_collect_more, which
-// walks its query again alone and lists all its candidates.  KSTAGE trades
-// occupancy against spill traffic; the result does not depend on it.
+// loads.
 template <int KSTAGE>
 __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
 {
-  __shared__ int stage[KSTAGE][WAVE];
-  __shared__ int chunk[SPILL_MAX];  // the wave's spill chunks, in order
+  __shared__ int stage[KSTAGE * WAVE];
+  __shared__ int chunk[SPILL_MAX];
   const int lane = threadIdx.x;
   // workgroup wv walks walking wave wv; the grid is sized for every query,
   // so the workgroups past the walking waves leave at once
@@ -1264,11 +1354,7 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
     for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= S.ext[2 * d + 1] + TOL);
     for (int d = 0; d < 3; d++) q[d] = grid_of(p[d], S.g.lo[d], S.g.inv[d]);
   }
-  int nc = 0;
-  // spill chunks held (wave-uniform); spill_ok false once the pool or the
-  // wave's chunk table is exhausted (lanes past KSTAGE then go to the queue)
-  int nchunk = 0;
-  bool spill_ok = true;
+  Stager<KSTAGE> st{stage, chunk, lane};
   // pending right subtrees: entry k (node, lane mask) held by lane k, so a
   // push is a select and a pop a readlane; the left child, when any lane
   // enters it, is visited next without the stack (preorder)
@@ -1293,31 +1379,7 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
       N.right = r->right;
     }
     const bool act = (m >> lane) & 1ull;
-    const bool hit = act & in_grid(q, N.e);
-    if (hit & (nc < KSTAGE)) stage[nc][lane] = N.cellnv;
-    // a lane past the stage writes entry e = nc - KSTAGE to chunk e / SPILL_K;
-    // lists grow by one entry per node, so a new chunk is needed exactly when
-    // some lane reaches entry SPILL_K * nchunk
-    const int e = nc - KSTAGE;
-    if (__ballot(hit & (e >= 0)) && spill_ok) {
-      if (__ballot(hit & (e == SPILL_K * nchunk))) {
-        int id = O.spill_chunks;
-        if (nchunk < SPILL_MAX) {
-          if (lane == 0) id = atomicAdd(O.nspill, 1);
-          id = __builtin_amdgcn_readfirstlane(id);
-        }
-        if (id < O.spill_chunks) {
-          if (lane == 0) chunk[nchunk] = id;
-          __syncthreads();  // one wave per workgroup: orders lane 0's write before the reads
-          nchunk++;
-        } else {
-          spill_ok = false;
-        }
-      }
-      if (spill_ok && hit && e >= 0)
-        O.spill[((size_t)chunk[e / SPILL_K] * SPILL_K + e % SPILL_K) * WAVE + lane] = N.cellnv;
-    }
-    nc += hit;
+    st.emit(act & in_grid(q, N.e), rank_nv(node, N.cellnv), O);
     const int r = N.right, l = N.left;
     const unsigned long long mr = (r > -1) ? __ballot(act & in_grid(q, N.r)) : 0ull;
     const unsigned long long ml = (l > -1) ? __ballot(act & in_grid(q, N.l)) : 0ull;
@@ -1341,50 +1403,67 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
       more = false;
     }
   }
-  // one chunk of the flat list per wave
-  const bool held = spill_ok || nc <= KSTAGE;  // every candidate staged or spilled
-  const int k = held ? nc : 0;                 // the others are listed later
-  int pre = k;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    int v = __shfl_up(pre, o);
-    if (lane >= o) pre += v;
-  }
-  const int total = __shfl(pre, 63);
-  pre -= k;
-  int base = 0;
-  if (lane == 0 && total > 0) base = atomicAdd(O.used, total);
-  base = __shfl(base, 0);
+  st.flush(O, i >= 0, qb + t, qb + i);
+}
+
+// k_find: the same candidate sets from the tree's bin grid instead of the
+// walk.  The bins split the tree's 16-bit grid into 2^(16 - bshift) cells per
+// axis; bin b lists (packed grid box, rank | nv << 28) of every element whose
+// grid box overlaps it, in any order.  A point lies in exactly one bin, and
+// every grid box holding the point overlaps that bin, so scanning the bin's
+// list with the grid test lists the walk's candidates.  Each lane scans its
+// own bin's list (independent loads, four at a time); the wave steps through
+// the longest list of its lanes.
+template <int KSTAGE>
+__global__ void __launch_bounds__(WAVE, 8) k_find(SearchSet T, CandOut O)
+{
+  __shared__ int stage[KSTAGE * WAVE];
+  __shared__ int chunk[SPILL_MAX];
+  const int lane = threadIdx.x;
+  const int wv = blockIdx.x;
+  if (wv >= T.wstart[T.ns]) return;
+  const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, wv));
+  const SearchView S = view_of(T, sid);
+  const int qb = T.qbase[sid];
+  const int t = (wv - T.wstart[sid]) * WAVE + lane;
+  const int nin = T.nin[sid];
+  const int i = t < nin ? S.perm[t] : -1;
+  unsigned q[3] = {0u, 0u, 0u};
+  int start = 0, len = 0;
   if (i >= 0) {
-    const int gp = qb + t, g = qb + i;
-    O.qcnt[gp] = nc;
-    if (!held) {
-      O.ovf[atomicAdd(O.novf, 1)] = gp;
-    } else {
-      const int off = base + pre;
-      O.qoff[gp] = off;
-      if (off + k > O.cap) {
-        // flat list too small (the host sees used > cap, grows it and walks
-        // again): the reserved slots below the capacity are marked empty
-        for (int j = off; j < O.cap && j < off + k; j++) O.cell[j] = EMPTY_SLOT;
-      } else {
-        const int ks = k < KSTAGE ? k : KSTAGE;
-        for (int j = 0; j < ks; j++) {
-          O.cell[off + j] = stage[j][lane];
-          O.query[off + j] = g;
-        }
-        for (int j = KSTAGE; j < k; j++) {
-          const int e2 = j - KSTAGE;
-          O.cell[off + j] = O.spill[((size_t)chunk[e2 / SPILL_K] * SPILL_K + e2 % SPILL_K) * WAVE + lane];
-          O.query[off + j] = g;
-        }
-      }
+    double p[3];
+    load_xyz(S.xs, i, p);
+    bool inroot = true;  // the root box test, FP64 as searchADTrecursion.C:38-43 makes it
+    for (int d = 0; d < 3; d++) inroot = inroot && (p[d] >= S.ext[2 * d] - TOL);
+    for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= S.ext[2 * d + 1] + TOL);
+    for (int d = 0; d < 3; d++) q[d] = grid_of(p[d], S.g.lo[d], S.g.inv[d]);
+    if (inroot && S.boff) {
+      const int sh = S.bshift, R = 65536 >> sh;
+      const int bin = (int)(((q[2] >> sh) * (unsigned)R + (q[1] >> sh)) * (unsigned)R + (q[0] >> sh));
+      start = S.boff[bin];
+      len = S.boff[bin + 1] - start;
     }
   }
+  int lmax = len;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) lmax = max(lmax, __shfl_xor(lmax, o));
+  Stager<KSTAGE> st{stage, chunk, lane};
+  const uint4 *ent = S.bent;
+  for (int k = 0; k < lmax; k += 4) {
+    uint4 e[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) e[u] = (k + u < len) ? ent[start + k + u] : make_uint4(0xffffu, 0u, 0u, 0u);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const unsigned w[3] = {e[u].x, e[u].y, e[u].z};
+      st.emit((k + u < len) && in_grid(q, w), (int)e[u].w, O);
+    }
+  }
+  st.flush(O, i >= 0, qb + t, qb + i);
 }
 
 // the walk of one query, listing all its candidates (queries whose lists
-// k_collect could not hold); a grid-stride loop over the queue
+// k_collect or k_find could not hold); a grid-stride loop over the queue
 __global__ void __launch_bounds__(64) k_collect_more(SearchSet T, CandOut O)
 {
   __shared__ int stk[STACK][64];
@@ -1410,9 +1489,10 @@ __global__ void __launch_bounds__(64) k_collect_more(SearchSet T, CandOut O)
     int sp = 0, nc = 0;
     stk[sp++][tid] = 0;
     while (sp > 0) {
-      const WalkNode &N = nodes[stk[--sp][tid]];
+      const int node = stk[--sp][tid];
+      const WalkNode &N = nodes[node];
       if (in_grid(q, N.e)) {
-        O.cell[off + nc] = N.cellnv;
+        O.cell[off + nc] = rank_nv(node, N.cellnv);
         O.query[off + nc] = g;
         nc++;
       }
@@ -1511,6 +1591,12 @@ __device__ __forceinline__ int test_status(const double frac[8], bool big)
   return flag ? 1 : 2;
 }
 
+// the cell of a listed candidate (rank | nv << 28): its walk record's cell
+__device__ __forceinline__ int cell_of(const SearchView &S, int ranknv)
+{
+  return S.nodes[ranknv & 0x0fffffff].cellnv & 0x0fffffff;
+}
+
 template <int NV>
 __device__ __forceinline__ void cell_vertices(const SearchView &S, int c, double xv[8][3])
 {
@@ -1543,9 +1629,9 @@ __global__ void __launch_bounds__(256, WAVES) k_eval(SearchSet T, CandOut O, con
   const int n = bcount[B];
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const int j = blist[b0 + k];
-    const int c = O.cell[j] & 0x0fffffff;
     const int g = O.query[j], sid = T.ns == 1 ? 0 : T.qsid[g];
     const SearchView S = view_of(T, sid);
+    const int c = cell_of(S, O.cell[j]);
     double p[3], xv[8][3], frac[8];
     load_xyz(S.xs, g - T.qbase[sid], p);
     cell_vertices<NV>(S, c, xv);
@@ -1606,6 +1692,28 @@ __global__ void __launch_bounds__(256) k_status_hist(CandOut O, int *__restrict_
   if (threadIdx.x < 4 && loc[threadIdx.x]) atomicAdd(&hist[threadIdx.x], loc[threadIdx.x]);
 }
 
+// candidates ranked after their query's lowest-ranked clean hit (the
+// reference's walk stops at that hit and never tests them); a diagnostic
+__global__ void k_after_hit(SearchSet T, CandOut O, unsigned long long *__restrict__ out)
+{
+  const int gp = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = 0;
+  if (gp < T.nqt) {
+    const int sid = T.ns == 1 ? 0 : T.qsid[gp];
+    if (gp - T.qbase[sid] < T.nin[sid]) {
+      const int off = O.qoff[gp], cnt = O.qcnt[gp];
+      if (off >= 0 && off + cnt <= O.cap) {
+        int hit = 0x7fffffff;
+        for (int j = 0; j < cnt; j++)
+          if (O.status[off + j] == 2) hit = min(hit, O.cell[off + j] & 0x0fffffff);
+        for (int j = 0; j < cnt; j++) v += (O.cell[off + j] & 0x0fffffff) > hit;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+}
+
 // the queued solves, continued from their saved state to the end
 template <int NV>
 __global__ void __launch_bounds__(256, 4) k_eval_long(SearchSet T, CandOut O, LongQueue L)
@@ -1615,9 +1723,9 @@ __global__ void __launch_bounds__(256, 4) k_eval_long(SearchSet T, CandOut O, Lo
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const int j = L.j[B * L.cap + k];
     const double4 st = L.st[B * L.cap + k];
-    const int c = O.cell[j] & 0x0fffffff;
     const int g = O.query[j], sid = T.ns == 1 ? 0 : T.qsid[g];
     const SearchView S = view_of(T, sid);
+    const int c = cell_of(S, O.cell[j]);
     double p[3], xv[8][3], frac[8], f[8][3];
     load_xyz(S.xs, g - T.qbase[sid], p);
     cell_vertices<NV>(S, c, xv);
@@ -1650,25 +1758,128 @@ __global__ void k_select(SearchSet T, CandOut O)
     S.donorId[i] = -1;
     return;
   }
-  // with a tie-band counter every candidate is looked at: the donor depends
-  // on the walk order when more than one candidate accepts the point or one
-  // accepts it with a near-zero weight in a BIGVALUE cell
-  bool done = false, flag1 = false;
-  int nacc = 0;
+  // the lowest-ranked clean hit, else the highest-ranked candidate's outcome
+  // (the candidates' order in the list is free); with a tie-band counter the
+  // queries whose donor would depend on the walk order are counted: more
+  // than one candidate accepts the point, or one accepts it with a near-zero
+  // weight in a BIGVALUE cell
+  int hit = 0x7fffffff, last = -1, last_st = 0, nacc = 0;
+  bool flag1 = false;
   for (int k = 0; k < nc; k++) {
     const int st = O.status[off + k];
     if (st == OUTSIDE) continue;
+    const int rank = O.cell[off + k] & 0x0fffffff;
     nacc += st != 0;
     flag1 = flag1 || st == 1;
-    if (!done) {
-      const int cell = O.cell[off + k] & 0x0fffffff;
-      res = st ? cell : -1;
-      done = st == 2;
+    if (st == 2 && rank < hit) hit = rank;
+    if (rank > last) {
+      last = rank;
+      last_st = st;
     }
-    if (done && !S.tieband) break;
   }
+  if (hit != 0x7fffffff) res = S.nodes[hit].cellnv & 0x0fffffff;
+  else if (last >= 0 && last_st == 1) res = S.nodes[last].cellnv & 0x0fffffff;
   if (S.tieband && (nacc > 1 || flag1)) atomicAdd(S.tieband, 1);
   S.donorId[i] = res;
+}
+
+// ---- bin grids of walk records (k_find's lists) ----
+struct BinTree {
+  const WalkNode *nodes;
+  int nelem, shift;
+  long long bbase;  // the tree's first bin in the batch's count / offset arrays
+  int nbase;        // the tree's first element in the batch's enumeration
+};
+
+__device__ __forceinline__ int bin_tree(int ntree, const BinTree *T, int g)
+{
+  int lo = 0, hi = ntree - 1;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (T[m].nbase <= g) lo = m;
+    else hi = m - 1;
+  }
+  return lo;
+}
+
+// the bin ranges of a packed grid box (grid_box layout) at a shift
+__device__ __forceinline__ void box_bins(const unsigned w[3], int sh, int lo[3], int hi[3])
+{
+  lo[0] = (int)((w[0] & 0xffffu) >> sh);
+  lo[1] = (int)((w[0] >> 16) >> sh);
+  lo[2] = (int)((w[1] & 0xffffu) >> sh);
+  hi[0] = (int)((w[1] >> 16) >> sh);
+  hi[1] = (int)((w[2] & 0xffffu) >> sh);
+  hi[2] = (int)((w[2] >> 16) >> sh);
+}
+
+// entries a tree would have at shifts shift .. shift + 3 (sums[4 * tree + k])
+__global__ void k_bin_sizes(int ntree, int total, const BinTree *__restrict__ T, unsigned long long *__restrict__ sums)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = g < total;
+  const int a = ok ? bin_tree(ntree, T, g) : 0;
+  unsigned long long c[4] = {0, 0, 0, 0};
+  if (ok) {
+    const WalkNode &N = T[a].nodes[g - T[a].nbase];
+    for (int k = 0; k < 4; k++) {
+      const int sh = min(T[a].shift + k, 16);
+      int lo[3], hi[3];
+      box_bins(N.e, sh, lo, hi);
+      c[k] = (unsigned long long)(hi[0] - lo[0] + 1) * (unsigned long long)(hi[1] - lo[1] + 1) *
+             (unsigned long long)(hi[2] - lo[2] + 1);
+    }
+  }
+  // one atomic per wave and shift when the wave's elements share a tree
+  const int a0 = __builtin_amdgcn_readfirstlane(a);
+  if (__ballot(ok && a != a0) == 0ull) {
+    for (int k = 0; k < 4; k++) {
+      unsigned long long v = c[k];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if ((threadIdx.x & 63) == 0 && v) atomicAdd(&sums[4 * a0 + k], v);
+    }
+  } else if (ok) {
+    for (int k = 0; k < 4; k++) atomicAdd(&sums[4 * a + k], c[k]);
+  }
+}
+
+__global__ void k_bin_count(int ntree, int total, const BinTree *__restrict__ T, int *__restrict__ cnt)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const int a = bin_tree(ntree, T, g);
+  const WalkNode &N = T[a].nodes[g - T[a].nbase];
+  const int sh = T[a].shift, R = 65536 >> sh;
+  int lo[3], hi[3];
+  box_bins(N.e, sh, lo, hi);
+  int *c = cnt + T[a].bbase;
+  for (int z = lo[2]; z <= hi[2]; z++)
+    for (int y = lo[1]; y <= hi[1]; y++)
+      for (int x = lo[0]; x <= hi[0]; x++) atomicAdd(&c[(z * R + y) * R + x], 1);
+}
+
+// each element into every bin its grid box overlaps; cnt (the counts) is
+// counted down to place the entries, in any order
+__global__ void k_bin_fill(int ntree, int total, const BinTree *__restrict__ T, const int *__restrict__ off,
+                           int *__restrict__ cnt, uint4 *__restrict__ ent)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const int a = bin_tree(ntree, T, g);
+  const int n = g - T[a].nbase;
+  const WalkNode &N = T[a].nodes[n];
+  const int sh = T[a].shift, R = 65536 >> sh;
+  int lo[3], hi[3];
+  box_bins(N.e, sh, lo, hi);
+  const uint4 v = make_uint4(N.e[0], N.e[1], N.e[2], (unsigned)rank_nv(n, N.cellnv));
+  const int *o = off + T[a].bbase;
+  int *c = cnt + T[a].bbase;
+  for (int z = lo[2]; z <= hi[2]; z++)
+    for (int y = lo[1]; y <= hi[1]; y++)
+      for (int x = lo[0]; x <= hi[0]; x++) {
+        const int b = (z * R + y) * R + x;
+        ent[o[b] + atomicSub(&c[b], 1) - 1] = v;
+      }
 }
 
 // coincident queries take their representative's donor (search.C:251-265)
@@ -2370,6 +2581,10 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
   const SearchSet T{(const SearchView *)w.sv.p, w.qb.p, w.nin.p, w.wstart.p, w.qsid.p, ns, nqt};
   long long nwave = 0;  // an upper bound of the walking waves
   for (int k = 0; k < ns; k++) nwave += (S[k].nq + WAVE - 1) / WAVE;
+  // candidates from the bin grids when every tree has one, else by the walk
+  bool bins = w.tune.bins > 0;
+  for (int k = 0; k < ns; k++) bins = bins && S[k].boff != nullptr;
+  w.last_bins = bins;
   if (w.cap == 0) w.cap = std::max<size_t>(4 * (size_t)nqt, 1 << 20);
   for (;;) {
     w.cell.reserve(w.cap);
@@ -2379,10 +2594,18 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     CandOut O{w.cell.p, w.query.p, w.status.p, w.qoff.p, w.qcnt.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
               w.ovf.p, w.counters.p + 1, w.spill.p, w.spill_chunks, w.counters.p + 2};
     fill<int>(w.counters.p, 3, 0, s);
-    switch (w.tune.kstage) {
-      case 16: k_collect<16><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
-      case 24: k_collect<24><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
-      default: k_collect<32><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
+    if (bins) {
+      switch (w.tune.kstage) {
+        case 16: k_find<16><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
+        case 24: k_find<24><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
+        default: k_find<32><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
+      }
+    } else {
+      switch (w.tune.kstage) {
+        case 16: k_collect<16><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
+        case 24: k_collect<24><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
+        default: k_collect<32><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
+      }
     }
     k_collect_more<<<8192, 64, 0, s>>>(T, O);
     if (check) {
@@ -2432,6 +2655,14 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     k_select<<<nb(nqt), BS, 0, s>>>(T, O);
     return;
   }
+}
+
+// a search's bin grid, when one was built
+inline void view_bins(SearchView &S, const BinGrid &g)
+{
+  S.boff = g.nbins ? g.off.p : nullptr;
+  S.bent = g.nbins ? g.ent.p : nullptr;
+  S.bshift = g.shift;
 }
 
 // a small device-side copy of a host struct, valid until the next call
@@ -3403,7 +3634,7 @@ void Engine::search(Block &b)
     obb_stepped_ += st;
     prep_acc_[0] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     build_search_structure(b, obq, b.nelem, b.adt_depth, b.elist, b.ebox, b.adtInt, b.adtReal, b.nodes,
-                           b.adtExt);
+                           b.adtExt, b.bins);
   }
   bool gid_mode = true;
   for (Block *o : blocks_) gid_mode = gid_mode && (o->unode_gid != nullptr);
@@ -3582,6 +3813,18 @@ void Engine::prepare_searches(const std::vector<int> &act)
     k_adt_nodes_all<<<nb(te), BS, 0, s_>>>(na, (int)te, (const AdtNodesTree *)ntab_.p, tint_.p, treal_.p,
                                             tebox_.p, telist_.p, tnodes_.p);
   }
+  {  // the bin grids of every tree, one batch
+    std::vector<const WalkNode *> np(na);
+    std::vector<int> ne(na);
+    std::vector<BinGrid *> bv(na);
+    for (int a = 0; a < na; a++) {
+      Block &b = *blocks_[act[a]];
+      np[a] = b.nodes.p;
+      ne[a] = b.nelem;
+      bv[a] = &b.bins;
+    }
+    build_bins(na, np.data(), ne.data(), tbins_, bv.data());
+  }
   sync();
   auto t3 = now();
   prep_acc_[0] += ms(t0, t1);
@@ -3624,7 +3867,8 @@ void Engine::build_resident_tree(int btag, ResidentTree &t)
 {
   Block *b = find(btag);
   if (!b || !b->topo_uploaded || b->cellRes.p == nullptr) throw std::runtime_error("tioga_amd: preprocess first");
-  build_search_structure(*b, b->obb, t.nelem, t.depth, t.elist, t.ebox, t.adtInt, t.adtReal, t.nodes, t.ext);
+  build_search_structure(*b, b->obb, t.nelem, t.depth, t.elist, t.ebox, t.adtInt, t.adtReal, t.nodes, t.ext,
+                         t.bins);
   if (t.nelem != b->ncells) throw std::runtime_error("tioga_amd: resident tree misses cells of its block");
 }
 
@@ -3648,13 +3892,83 @@ void Engine::attach_resident(Block &b)
   b.adtInt.view(t.adtInt.p, 4 * (size_t)t.nelem);
   b.adtReal.view(t.adtReal.p, 6 * (size_t)t.nelem);
   b.nodes.view(t.nodes.p, t.nelem);
+  b.bins.shift = t.bins.shift;
+  b.bins.nbins = t.bins.nbins;
+  if (t.bins.nbins) {
+    b.bins.off.view(t.bins.off.p, (size_t)t.bins.nbins + 1);
+    b.bins.ent.view(t.bins.ent.p, t.bins.ent.cap);
+  }
+}
+
+// The bin grids of ntree trees (walk records nodes[a], nelem[a] of them),
+// built into store's arrays; views[a] gets tree a's grid (views of store's
+// arrays unless views[a] is store itself).  A tree's bins per axis are the
+// power of two nearest cbrt(nelem / tune.bins), made coarser (up to 3 times)
+// while its entries exceed 32 per element.  tune.bins == 0 builds none.
+void Engine::build_bins(int ntree, const WalkNode *const *nodes, const int *nelem, BinGrid &store,
+                        BinGrid *const *views)
+{
+  for (int a = 0; a < ntree; a++) views[a]->nbins = 0;
+  const int per = sw_.tune.bins;
+  std::vector<BinTree> T(ntree);
+  long long total = 0;
+  for (int a = 0; a < ntree; a++) {
+    const int n = nelem[a];
+    const double r = per > 0 && n > 0 ? std::cbrt((double)n / per) : 1.0;
+    const int k = std::max(0, std::min(8, (int)std::lround(std::log2(std::max(r, 1.0)))));
+    T[a] = BinTree{nodes[a], n, 16 - k, 0, (int)total};
+    total += n;
+  }
+  if (per <= 0 || total == 0) return;
+  if (total > 0x7fffffffLL) throw std::runtime_error("tioga_amd: too many tree elements for one bin batch");
+  btab_.reserve(sizeof(BinTree) * ntree);
+  bsums_.reserve(4 * (size_t)ntree);
+  TG_HIP(hipMemcpyAsync(btab_.p, T.data(), sizeof(BinTree) * ntree, hipMemcpyHostToDevice, s_));
+  fill<unsigned long long>(bsums_.p, 4 * (size_t)ntree, 0ull, s_);
+  k_bin_sizes<<<nb(total), BS, 0, s_>>>(ntree, (int)total, (const BinTree *)btab_.p, bsums_.p);
+  std::vector<unsigned long long> sums(4 * (size_t)ntree);
+  TG_HIP(hipMemcpyAsync(sums.data(), bsums_.p, sizeof(unsigned long long) * sums.size(), hipMemcpyDeviceToHost, s_));
+  sync();
+  long long nbin = 0, nent = 0;
+  for (int a = 0; a < ntree; a++) {
+    const unsigned long long cap = 32ull * (unsigned long long)T[a].nelem + 65536ull;
+    int k = 0;
+    while (k < 3 && (sums[4 * a + k] > cap || T[a].shift + k > 16)) k++;
+    T[a].shift = std::min(T[a].shift + k, 16);
+    const long long R = 65536 >> T[a].shift;
+    T[a].bbase = nbin;
+    nbin += R * R * R;
+    nent += (long long)sums[4 * a + k];
+  }
+  if (nbin + 1 > 0x7fffffffLL || nent > 0x7fffffffLL)
+    throw std::runtime_error("tioga_amd: bin grids too large");
+  bcnt_.reserve((size_t)nbin + 1);
+  store.off.reserve_own((size_t)nbin + 1);
+  store.ent.reserve_own((size_t)std::max<long long>(nent, 1));
+  fill<int>(bcnt_.p, (size_t)nbin + 1, 0, s_);
+  TG_HIP(hipMemcpyAsync(btab_.p, T.data(), sizeof(BinTree) * ntree, hipMemcpyHostToDevice, s_));
+  k_bin_count<<<nb(total), BS, 0, s_>>>(ntree, (int)total, (const BinTree *)btab_.p, bcnt_.p);
+  exclusive_sum(bcnt_.p, store.off.p, (int)(nbin + 1), s_);
+  k_bin_fill<<<nb(total), BS, 0, s_>>>(ntree, (int)total, (const BinTree *)btab_.p, store.off.p, bcnt_.p,
+                                      store.ent.p);
+  for (int a = 0; a < ntree; a++) {
+    BinGrid &v = *views[a];
+    if (T[a].nelem == 0) continue;
+    const long long R = 65536 >> T[a].shift;
+    v.shift = T[a].shift;
+    v.nbins = R * R * R;
+    if (&v != &store) {
+      v.off.view(store.off.p + T[a].bbase, (size_t)v.nbins + 1);
+      v.ent.view(store.ent.p, store.ent.cap);
+    }
+  }
 }
 
 // search.C:107-233: query-set OBB -> cells whose OBB-frame box overlaps it,
 // as a LIFO list (descending cell index) -> element boxes -> ADT
 void Engine::build_search_structure(Block &b, const OBB &obq, int &nelem, int &depth, DBuf<int> &elist,
                                     DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal,
-                                    DBuf<WalkNode> &nodes, double ext[6])
+                                    DBuf<WalkNode> &nodes, double ext[6], BinGrid &bins)
 {
   auto now = [] { return std::chrono::steady_clock::now(); };
   auto ms = [](auto a, auto c) { return std::chrono::duration<double, std::milli>(c - a).count(); };
@@ -3676,6 +3990,9 @@ void Engine::build_search_structure(Block &b, const OBB &obq, int &nelem, int &d
   if (depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
   nodes.reserve_own(ne);
   if (ne) k_adt_nodes<<<nb(ne), BS, 0, s_>>>(ne, b.ti, grid_frame(ext), adtInt.p, adtReal.p, ebox.p, elist.p, nodes.p);
+  const WalkNode *np = nodes.p;
+  BinGrid *bp = &bins;
+  build_bins(1, &np, &ne, bins, &bp);
   sync();
   auto t3 = now();
   prep_acc_[1] += ms(t1, t2);
@@ -3698,7 +4015,7 @@ int Engine::query_set_create(int btag, const double *xq, long nq)
   OBB obq;
   find_obb(xq, nullptr, (int)nq, obq);
   build_search_structure(*b, obq, q->nelem, q->adt_depth, q->elist, q->ebox, q->adtInt, q->adtReal, q->nodes,
-                         q->ext);
+                         q->ext, q->bins);
   qsets_.push_back(q);
   return (int)qsets_.size() - 1;
 }
@@ -3772,6 +4089,7 @@ void Engine::launch_query_sets(const std::vector<QuerySet *> &qs)
     S.xtag = nullptr;
     S.donorId = q.donor.p;
     S.nodes = q.nodes.p;
+    view_bins(S, q.bins);
     S.g = grid_frame(q.ext);
     memcpy(S.ext, q.ext, sizeof S.ext);
     S.conn = b.conn.p;
@@ -3784,10 +4102,24 @@ void Engine::launch_query_sets(const std::vector<QuerySet *> &qs)
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
 }
 
-void Engine::search_stats(long long out[8])
+void Engine::set_search_bins(int bins)
+{
+  if (bins < 0 || bins > 4096) throw std::runtime_error("tioga_amd: bin density must be 0..4096");
+  if (bins == sw_.tune.bins) return;
+  sw_.tune.bins = bins;
+  for (QuerySet *q : qsets_) {
+    const WalkNode *np = q->nodes.p;
+    BinGrid *bp = &q->bins;
+    build_bins(1, &np, &q->nelem, q->bins, &bp);
+  }
+  sync();
+}
+
+void Engine::search_stats(long long out[10])
 {
   SearchScratch &w = sw_;
-  for (int k = 0; k < 8; k++) out[k] = 0;
+  for (int k = 0; k < 10; k++) out[k] = 0;
+  out[9] = w.last_bins ? 1 : 0;
   out[3] = w.last_nqt;
   if (w.last_nqt <= 0 || w.last_ns <= 0) return;
   int c[2] = {0, 0}, h[4] = {0, 0, 0, 0};
@@ -3798,6 +4130,13 @@ void Engine::search_stats(long long out[8])
   CandOut O{w.cell.p, w.query.p, w.status.p, w.qoff.p, w.qcnt.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
             w.ovf.p, w.counters.p + 1, w.spill.p, w.spill_chunks, w.counters.p + 2};
   k_status_hist<<<1024, 256, 0, s_>>>(O, hist.p);
+  DBuf<unsigned long long> ah;
+  ah.reserve(1);
+  fill<unsigned long long>(ah.p, 1, 0ull, s_);
+  const SearchSet T{(const SearchView *)w.sv.p, w.qb.p, w.nin.p, w.wstart.p, w.qsid.p, w.last_ns, w.last_nqt};
+  k_after_hit<<<nb(w.last_nqt), BS, 0, s_>>>(T, O, ah.p);
+  unsigned long long after = 0;
+  TG_HIP(hipMemcpyAsync(&after, ah.p, sizeof after, hipMemcpyDeviceToHost, s_));
   TG_HIP(hipMemcpyAsync(c, w.counters.p, sizeof c, hipMemcpyDeviceToHost, s_));
   TG_HIP(hipMemcpyAsync(h, hist.p, sizeof h, hipMemcpyDeviceToHost, s_));
   TG_HIP(hipMemcpyAsync(nin.data(), w.nin.p, sizeof(int) * w.last_ns, hipMemcpyDeviceToHost, s_));
@@ -3806,6 +4145,7 @@ void Engine::search_stats(long long out[8])
   out[1] = c[1];
   for (int v : nin) out[2] += v;
   for (int k = 0; k < 4; k++) out[4 + k] = h[k];
+  out[8] = (long long)after;
 }
 
 void Engine::query_set_search(int id, int *donor)
@@ -3890,6 +4230,7 @@ void Engine::launch_search(Block &b)
   S.xtag = b.xtag.p;
   S.donorId = b.donorId.p;
   S.nodes = b.nodes.p;
+  view_bins(S, b.bins);
   S.g = grid_frame(b.adtExt);
   memcpy(S.ext, b.adtExt, sizeof S.ext);
   S.conn = b.conn.p;
@@ -3950,6 +4291,7 @@ void Engine::launch_searches()
     S.xtag = b.xtag.p;
     S.donorId = b.donorId.p;
     S.nodes = b.nodes.p;
+    view_bins(S, b.bins);
     S.g = grid_frame(b.adtExt);
     memcpy(S.ext, b.adtExt, sizeof S.ext);
     S.conn = b.conn.p;

@@ -68,6 +68,8 @@ def load():
         L.tioga_amd_last_search_kernel_ms.argtypes = []
         L.tioga_amd_last_search_kernel_ms.restype = C.c_double
         L.tioga_amd_search_stats.argtypes = [_P]
+        L.tioga_amd_search_bins.argtypes = [C.c_int]
+        L.tioga_amd_search_bins.restype = C.c_int
         L.tioga_amd_search_spill.argtypes = [C.c_long]
         L.tioga_amd_search_spill.restype = C.c_long
         L.tioga_amd_search_stats.restype = C.c_int
@@ -414,8 +416,11 @@ class Tioga:
         return ext
 
     def search_tuning(self, morton_bits: Optional[int] = None, kstage: Optional[int] = None,
-                      eval_grid: Optional[int] = None) -> Dict[str, int]:
-        """Sets the given search launch parameters; returns all three."""
+                      eval_grid: Optional[int] = None, bins: Optional[int] = None) -> Dict[str, int]:
+        """Sets the given search parameters; returns all four (bins: elements
+        per bin of the candidate bin grids, 0 = the tree walk)."""
+        nb = self.L.tioga_amd_search_bins(-1 if bins is None else int(bins))
+        self._check()
         cur = np.zeros(3, np.int32)
         self.L.tioga_amd_search_tuning(None, _p(cur))
         self._check()
@@ -426,7 +431,7 @@ class Tioga:
         if not np.array_equal(want, cur):
             self.L.tioga_amd_search_tuning(_p(want), _p(cur))
             self._check()
-        return {"morton_bits": int(cur[0]), "kstage": int(cur[1]), "eval_grid": int(cur[2])}
+        return {"morton_bits": int(cur[0]), "kstage": int(cur[1]), "eval_grid": int(cur[2]), "bins": int(nb)}
 
     def search_spill(self, chunks: int = 0) -> int:
         r = self.L.tioga_amd_search_spill(int(chunks))
@@ -449,12 +454,13 @@ class Tioga:
         """Counts of the last search launch: candidates listed by the walk,
         queries with more than the staged candidates, walking queries,
         queries, and the candidates by test outcome."""
-        out = np.zeros(8, np.int64)
+        out = np.zeros(10, np.int64)
         self.L.tioga_amd_search_stats(_p(out))
         self._check()
         return {"candidates": int(out[0]), "overflow_queries": int(out[1]), "walking_queries": int(out[2]),
                 "queries": int(out[3]), "rejected": int(out[4]), "accepted_near_zero": int(out[5]),
-                "accepted": int(out[6]), "outside_element_box": int(out[7])}
+                "accepted": int(out[6]), "outside_element_box": int(out[7]), "after_first_hit": int(out[8]),
+                "listing": "bins" if out[9] else "walk"}
 
     def phase_times(self) -> Dict[str, float]:
         out = np.zeros(len(PHASES))
