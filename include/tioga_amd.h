@@ -211,9 +211,9 @@ double tioga_amd_time_query_sets(int nrep, long long *nq);
 /* device ms per pass of the search kernel launches alone in the last
    tioga_amd_time_query_sets (HIP events around each launch) */
 double tioga_amd_last_search_kernel_ms(void);
-/* counts of the last search launch: out[0] = candidate cells listed by the
-   tree walk, out[1] = queries with more candidates than the walk stages per
-   lane, out[2] = queries that walked a tree (inside its root box, not a
+/* counts of the last search launch: out[0] = candidate cells listed,
+   out[1] = listing work items (walking waves, or their bin-list segments),
+   out[2] = queries that searched a tree (inside its root box, not a
    coincident copy), out[3] = queries, out[4..7] = listed candidates whose
    containment test rejected the point, accepted it with a near-zero weight
    in a BIGVALUE cell, accepted it cleanly, or found it outside the element's
@@ -236,16 +236,10 @@ int tioga_amd_query_set_extents(int id, double *ext);
    capacity the next search starts from (a checked search grows it when its
    candidates do not fit and walks again); returns the current capacity. */
 long tioga_amd_search_capacity(long cap);
-/* Spill pool of the donor search's tree walk, in chunks of 16 candidates for
-   each of a wave's 64 queries (4 KiB): chunks > 0 sets the pool the next
-   search starts from (at most 262144; a checked search that runs out lists
-   the lists it could not hold by walking those queries again, then grows the
-   pool); returns the current size (0 before the first search). */
-long tioga_amd_search_spill(long chunks);
 /* Launch parameters of the donor search, none of which changes a result:
-   {Morton bits of the query order (0..30), candidates staged per lane by the
-   tree walk (16, 24 or 32), workgroups of each containment-test kernel
-   (64..65536)}.  set (may be NULL) applies 3 values, get (may be NULL)
+   {Morton bits of the query order (0..30), bin-list entries per query in one
+   work item of the bin listing (16..65536), workgroups of each
+   containment-test kernel (64..65536)}.  set (may be NULL) applies 3 values, get (may be NULL)
    receives the current ones; returns 0, or -1 with the error status set.
    tioga_amd/autotune.py chooses them by measurement. */
 int tioga_amd_search_tuning(const int *set, int *get);

@@ -192,14 +192,15 @@ def test_query_sets_candidate_list_regrows():
         o.close()
 
 
-@pytest.mark.parametrize("tuning", [dict(morton_bits=0, kstage=16, eval_grid=1024, bins=0),
-                                    dict(morton_bits=30, kstage=24, eval_grid=8192, bins=1),
-                                    dict(morton_bits=9, kstage=32, eval_grid=64, bins=64),
-                                    dict(morton_bits=21, kstage=16, eval_grid=4096, bins=4096)])
+@pytest.mark.parametrize("tuning", [dict(morton_bits=0, seg=256, eval_grid=1024, bins=0),
+                                    dict(morton_bits=30, seg=1024, eval_grid=8192, bins=1),
+                                    dict(morton_bits=9, seg=64, eval_grid=64, bins=64),
+                                    dict(morton_bits=21, seg=16, eval_grid=4096, bins=4096)])
 def test_query_sets_any_tuning(tuning):
     """The search's parameters (candidates from bin grids of any density or
-    from the tree walk, Morton bits, staging depth, evaluation grid) change
-    no donor."""
+    from the tree walk, bin lists cut into work items of any length, Morton
+    bits, evaluation grid) change no donor; very coarse bins cut into short
+    items make many items per wave."""
     print(f"\n[query sets] tuning {tuning}", flush=True)
     blocks = _blocks("mid")
     o = _oracle(blocks)
@@ -209,44 +210,10 @@ def test_query_sets_any_tuning(tuning):
         assert got == tuning
         sets = [(blocks[0].tag, blocks[1].x), (blocks[1].tag, blocks[0].x)]
         _check_sets(t, o, sets, f"tuning {tuning}")
-        assert t.search_stats()["listing"] == ("bins" if tuning["bins"] else "walk")
-    finally:
-        t.close()
-        o.close()
-
-
-def test_query_sets_spill_pool_runs_out():
-    """The walk's spill pool (candidates past the per-lane stage) starting
-    at one chunk: the checked search lists what the pool could not hold by
-    walking those queries again, then grows the pool; the timed passes spill
-    into the larger pool.  Every pass equals the oracle, and the larger pool
-    leaves fewer queries to the second walk."""
-    print("\n[query sets] spill pool from 1 chunk", flush=True)
-    blocks = _blocks("mid")
-    o = _oracle(blocks)
-    t = _engine(blocks)
-    try:
-        t.search_tuning(morton_bits=21, kstage=16, eval_grid=2048)  # a short stage: many lists spill
-        assert t.search_spill(1) == 1
-        sets = [(blocks[0].tag, blocks[1].x), (blocks[1].tag, blocks[0].x)]
-        ids = [t.query_set(btag, xq) for btag, xq in sets]
-        want, small = [], []
-        for qid, (btag, xq) in zip(ids, sets):
-            d_ref, _, _ = o.search_points_obb(btag, xq, xq, _threads())
-            want.append(d_ref)
-            t.search_spill(1)  # each set's checked search starts from one chunk
-            got = t.query_search(qid, xq.shape[0])
-            small.append(t.search_stats())
-            assert np.array_equal(got, d_ref)
-            assert t.search_spill() >= 1
-        assert sum(st["overflow_queries"] for st in small) > 0, small
-        t.time_query_sets(2)  # both sets in one launch, the grown pool
-        big = t.search_stats()
-        for qid, (_, xq), w in zip(ids, sets, want):
-            assert np.array_equal(t.query_donors(qid, xq.shape[0]), w)
-        assert big["overflow_queries"] < sum(st["overflow_queries"] for st in small), (small, big)
-        assert big["candidates"] == sum(st["candidates"] for st in small)
-        t.query_clear()
+        st = t.search_stats()
+        assert st["listing"] == ("bins" if tuning["bins"] else "walk")
+        if tuning["bins"] == 4096:
+            assert st["listing_items"] > 2 * (st["walking_queries"] // 64 + 1), st
     finally:
         t.close()
         o.close()

@@ -1,138 +1,150 @@
-"""The slot accounting of the donor search's candidate list, restated on the
-host (no GPU).
+"""The donor search's candidate list and selection, restated on the host (no
+GPU).
 
-k_collect gives each wave of 64 queries one chunk of the flat candidate list
-(an atomic add of the wave's total to the `used` counter; each lane's slots
-follow by an exclusive prefix sum).  A lane's first KSTAGE candidates are
-staged in LDS; the rest go to spill chunks of SPILL_K entries per lane that
-the wave takes from a global pool (one atomic per chunk, at most SPILL_MAX
-per wave), and are copied into the lane's slots with the staged ones.  When
-the pool runs out, or a wave would need more than SPILL_MAX chunks, the
-wave's lanes with more than KSTAGE candidates take no slots there and are
-listed by k_collect_more (one atomic add each).  A list that does not fit
-below the capacity is not stored; the slots it reserved below the capacity
-are marked empty.  The binning and evaluation kernels then read every slot
-below min(used, cap).
+Listing (k_collect / k_find): each wave gathers its lanes' (rank | nv << 28,
+query) pairs in an LDS buffer of WBUF pairs, one step adding at most one pair
+per lane, and appends the buffer to the flat list with one atomic add of its
+length to `used` whenever more than WBUF - 64 pairs are held, and at the
+end; the copy sorts the pairs by lane, keeping each lane's step order.
+Pairs that land at or past the capacity are dropped; the host then sees used > cap, grows the list and
+lists again before any test runs.  The binning and evaluation kernels read
+every slot below min(used, cap).
 
-This model replays that allocation with random candidate counts, random
-atomic orders, spill pools from empty to ample, and capacities far below the
-total, and checks that every slot the later kernels read was written by this
-search (a candidate of the query that owns it, or the empty mark), that a
-query whose list fits finds exactly its own candidates at [qoff, qoff +
-count) in order, and that spill chunks are never shared between waves."""
+Selection (k_eval's per-query words, k_select): the reference walks the ADT
+in preorder, tests every candidate (an element whose padded box holds the
+point) in increasing node id, and stops at the first clean hit; its result is
+that hit's cell, or else the outcome of the last candidate tested (its cell
+when accepted with a near-zero weight, status 1, else -1).  The device folds
+the outcomes, in any order, into best = the lowest rank of a clean hit, then
+for the queries without one into last = the highest (rank << 2 | status)
+among candidates inside their element box (status OUTSIDE = 3 takes no
+part), and picks best's cell, else last's cell when its status is 1, else
+-1.
+
+The model replays the listing with random candidate counts, interleavings and
+capacities, and checks that every slot read later was written once by this
+launch, and that with room for everything each query's pairs are exactly its
+candidates.  It then checks the order-free selection against the sequential
+rule on shuffled lists with random outcomes, and that a planted mistake (the
+highest clean hit instead of the lowest) is caught."""
 import numpy as np
 import pytest
 
 WAVE = 64
-KSTAGE = 24
-SPILL_K = 16
-SPILL_MAX = 4
-EMPTY = -1
+WBUF = 512
+OUTSIDE = 3
 
 
-def replay(counts, cap, pool, rng):
-    """counts[q]: candidates of query q (queries in wave order); pool: spill
-    chunks.  Returns the slot contents (None = never written), write counts,
-    qoff, used, and the queries listed by k_collect_more."""
-    nq = counts.size
+def replay(lists, cap, rng):
+    """lists[q]: the candidate values of query q (queries in wave order).
+    Returns the slots (None = never written), write counts and used."""
+    nq = len(lists)
     nwave = (nq + WAVE - 1) // WAVE
     slots = [None] * cap
+    flush_of = [None] * cap
     writes = np.zeros(cap, np.int64)
-    qoff = np.full(nq, -1, np.int64)
     used = 0
-    taken = 0  # the pool's chunk counter
-    owner = {}  # spill chunk -> wave
-    ovf = []
-
-    def write(j, v):
-        slots[j] = v
-        writes[j] += 1
-
-    # waves reach their atomics in any order
-    for w in rng.permutation(nwave):
-        lanes = np.arange(w * WAVE, min((w + 1) * WAVE, nq))
-        c = counts[lanes]
-        # the walk: a chunk is taken when the longest list reaches entry
-        # SPILL_K * nchunk past the stage; a failed take ends spilling
-        need = int(-(-max(int(c.max()) - KSTAGE, 0) // SPILL_K))
-        chunks, ok = [], True
-        for _ in range(need):
-            if len(chunks) == SPILL_MAX:
-                ok = False
-                break
-            cid = taken
-            taken += 1
-            if cid >= pool:
-                ok = False
-                break
-            assert cid not in owner
-            owner[cid] = w
-            chunks.append(cid)
-        spill = {}
-        if ok:  # spilled entries, lane-major inside each chunk
-            for lane, q in enumerate(lanes):
-                for e in range(max(int(counts[q]) - KSTAGE, 0)):
-                    spill[(chunks[e // SPILL_K] * SPILL_K + e % SPILL_K) * WAVE + lane] = (int(q), KSTAGE + e)
-        held = ok | (c <= KSTAGE)
-        k = np.where(held, c, 0)
-        pre = np.concatenate([[0], np.cumsum(k)[:-1]])
+    # each wave: its steps, then its flushes; waves interleave at flushes
+    pending = []
+    for w in range(nwave):
+        lanes = range(w * WAVE, min((w + 1) * WAVE, nq))
+        steps = max((len(lists[q]) for q in lanes), default=0)
+        buf, flushes = [], []
+        for k in range(steps):
+            for q in lanes:  # lane order inside a step (the ballot prefix)
+                if k < len(lists[q]):
+                    buf.append((q, lists[q][k]))
+            if len(buf) > WBUF - WAVE:
+                flushes.append(buf)
+                buf = []
+        if buf:
+            flushes.append(buf)
+        pending.append(flushes)
+    # the waves' flushes reach the atomic in any interleaving
+    order = [w for w in range(nwave) for _ in pending[w]]
+    rng.shuffle(order)
+    nxt = [0] * nwave
+    for f, w in enumerate(order):
+        buf = pending[w][nxt[w]]
+        nxt[w] += 1
         base = used
-        used += int(k.sum())
-        for lane, (q, kq, p, h) in enumerate(zip(lanes, k, pre, held)):
-            if not h:
-                ovf.append(q)
-                continue
-            off = base + int(p)
-            qoff[q] = off
-            if off + kq > cap:
-                for j in range(off, min(cap, off + kq)):
-                    write(j, EMPTY)
-                continue
-            for j in range(min(kq, KSTAGE)):
-                write(off + j, (int(q), j))  # from the stage
-            for j in range(KSTAGE, kq):
-                e = j - KSTAGE
-                write(off + j, spill[(chunks[e // SPILL_K] * SPILL_K + e % SPILL_K) * WAVE + lane])
-    # k_collect_more: the queued queries in any order
-    for q in rng.permutation(np.array(ovf, np.int64)) if ovf else []:
-        cnt = int(counts[q])
-        off = used
-        used += cnt
-        qoff[q] = off
-        if off + cnt > cap:
-            for j in range(off, min(cap, off + cnt)):
-                write(j, EMPTY)
-            continue
-        for j in range(cnt):
-            write(off + j, (int(q), j))
-    return slots, writes, qoff, used, ovf
+        used += len(buf)
+        buf = sorted(buf, key=lambda qv: qv[0] % WAVE)  # stable: by lane, step order kept
+        for j, v in enumerate(buf):
+            if base + j < cap:
+                slots[base + j] = v
+                flush_of[base + j] = f
+                writes[base + j] += 1
+    return slots, writes, used, flush_of
 
 
-@pytest.mark.parametrize("seed", range(12))
-def test_every_read_slot_was_written_by_this_search(seed):
+@pytest.mark.parametrize("seed", range(10))
+def test_every_read_slot_was_written_by_this_launch(seed):
     rng = np.random.default_rng(seed)
-    nq = int(rng.integers(1, 700))
-    # mostly short lists, some empty, some above KSTAGE
-    counts = rng.choice([0, 1, 2, 3, 5, 8, 23, 24, 25, 40, 41, 90, 200], size=nq,
-                        p=[.2, .2, .15, .1, .1, .08, .04, .03, .03, .03, .02, .01, .01]).astype(np.int64)
+    nq = int(rng.integers(1, 500))
+    counts = rng.choice([0, 1, 2, 3, 5, 8, 30, 90, 700], size=nq,
+                        p=[.2, .2, .15, .15, .1, .1, .05, .03, .02])
+    lists = [[int(q) * 10000 + k for k in range(int(c))] for q, c in enumerate(counts)]
     total = int(counts.sum())
-    caps = sorted({1, 7, max(1, total // 5), max(1, total // 2), max(1, total - 1), total, total + 10})
-    for cap, pool in [(c, p) for c in caps for p in (0, 1, 3, 10 ** 6)]:
-        slots, writes, qoff, used, ovf = replay(counts, cap, pool, rng)
+    for cap in sorted({1, 7, max(1, total // 5), max(1, total // 2), max(1, total - 1), total, total + 10}):
+        slots, writes, used, flush_of = replay(lists, cap, rng)
         assert used == total
-        if pool == 0:  # nothing spills: exactly the lists past the stage are re-walked
-            assert sorted(ovf) == sorted(np.nonzero(counts > KSTAGE)[0].tolist())
         n = min(used, cap)
-        # every slot the binning reads was written exactly once by this search
-        assert all(s is not None for s in slots[:n]), (seed, cap, pool)
-        assert (writes[:n] == 1).all(), (seed, cap, pool)
-        # a query whose list fits finds exactly its own candidates, in order
-        for q in range(nq):
-            c = int(counts[q])
-            if c == 0:
-                continue
-            if qoff[q] + c <= cap:
-                assert slots[qoff[q]:qoff[q] + c] == [(q, j) for j in range(c)], (seed, cap, pool, q)
-        # with room for everything, no slot is marked empty
-        if cap >= total:
-            assert EMPTY not in slots[:n]
+        assert all(s is not None for s in slots[:n]), (seed, cap)
+        assert (writes[:n] == 1).all(), (seed, cap)
+        if cap >= total:  # with room for everything: each query's pairs are exactly its candidates
+            got = {}
+            for q, v in slots[:n]:
+                got.setdefault(q, []).append(v)
+            for q, lst in enumerate(lists):
+                assert sorted(got.get(q, [])) == lst, (seed, q)
+            # a query's pairs keep its step order
+            for q in range(len(lists)):
+                mine = [v for (qq, v) in slots[:n] if qq == q]
+                assert mine == lists[q], (seed, q)
+
+def sequential(cands):
+    """The reference's walk over its candidates in preorder (rank order):
+    cands = [(rank, cell, status)], OUTSIDE entries never tested."""
+    res = -1
+    for rank, cell, st in sorted(cands):
+        if st == OUTSIDE:
+            continue
+        res = cell if st else -1
+        if st == 2:
+            break
+    return res
+
+
+def order_free(cands, lowest=True):
+    best, last = None, None
+    for rank, cell, st in cands:
+        if st == OUTSIDE:
+            continue
+        if st == 2 and (best is None or (rank < best[0] if lowest else rank > best[0])):
+            best = (rank, cell)
+        if last is None or rank > last[0]:
+            last = (rank, cell, st)
+    if best is not None:
+        return best[1]
+    if last is not None and last[2] == 1:
+        return last[1]
+    return -1
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_order_free_selection_is_the_sequential_rule(seed):
+    rng = np.random.default_rng(100 + seed)
+    planted_caught = False
+    for _ in range(3000):
+        n = int(rng.integers(0, 12))
+        ranks = rng.choice(1000, size=n, replace=False)
+        cells = rng.integers(0, 10 ** 6, size=n)
+        st = rng.choice([0, 1, 2, OUTSIDE], size=n, p=[.55, .1, .2, .15])
+        cands = [(int(r), int(c), int(s)) for r, c, s in zip(ranks, cells, st)]
+        want = sequential(cands)
+        rng.shuffle(cands)  # the listing order is free
+        assert order_free(cands) == want, cands
+        if order_free(cands, lowest=False) != want:
+            planted_caught = True
+    assert planted_caught  # the checks see a selection that takes the wrong clean hit

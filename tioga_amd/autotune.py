@@ -28,10 +28,10 @@ TABLE = os.path.join(HERE, "search_tuning.json")
 SPACE = {
     "bins": [0, 1, 2, 4, 8],
     "morton_bits": [0, 9, 15, 21, 30],
-    "kstage": [16, 24, 32],
+    "seg": [64, 128, 256, 512, 1024],
     "eval_grid": [1024, 2048, 4096, 8192],
 }
-DEFAULT = {"morton_bits": 21, "kstage": 16, "eval_grid": 4096, "bins": 2}
+DEFAULT = {"morton_bits": 15, "seg": 256, "eval_grid": 8192, "bins": 2}
 
 
 def arch() -> str:

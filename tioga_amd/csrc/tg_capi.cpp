@@ -632,16 +632,15 @@ int tioga_amd_search_tuning(const int *set, int *get)
     tg::SearchTuning &t = tg_or_throw().engine()->search_tuning();
     if (set) {
       if (set[0] < 0 || set[0] > 30) throw std::runtime_error("tioga_amd: Morton bits must be 0..30");
-      if (set[1] != 16 && set[1] != 24 && set[1] != 32)
-        throw std::runtime_error("tioga_amd: walk staging must be 16, 24 or 32 candidates");
+      if (set[1] < 16 || set[1] > 65536) throw std::runtime_error("tioga_amd: listing segment must be 16..65536 entries");
       if (set[2] < 64 || set[2] > 65536) throw std::runtime_error("tioga_amd: evaluation grid must be 64..65536");
       t.morton_bits = set[0];
-      t.kstage = set[1];
+      t.seg = set[1];
       t.eval_grid = set[2];
     }
     if (get) {
       get[0] = t.morton_bits;
-      get[1] = t.kstage;
+      get[1] = t.seg;
       get[2] = t.eval_grid;
     }
     rc = 0;
@@ -667,17 +666,6 @@ long tioga_amd_search_capacity(long cap)
     tg::Engine *e = tg_or_throw().engine();
     if (cap > 0) e->set_search_capacity((size_t)cap);
     r = (long)e->search_capacity();
-  });
-  return r;
-}
-
-long tioga_amd_search_spill(long chunks)
-{
-  long r = -1;
-  guarded("tioga_amd_search_spill", [&] {
-    tg::Engine *e = tg_or_throw().engine();
-    if (chunks > 0) e->set_search_spill_chunks((int)std::min<long>(chunks, 262144));
-    r = (long)e->search_spill_chunks();
   });
   return r;
 }

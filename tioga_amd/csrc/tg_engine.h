@@ -176,6 +176,7 @@ struct ResidentTree {
   DBuf<int> elist, adtInt;
   DBuf<double> ebox, adtReal;
   DBuf<WalkNode> nodes;
+  DBuf<int> rcell;  // the cell of each walk record (node id order)
   BinGrid bins;
 };
 
@@ -225,6 +226,7 @@ struct Block {
   DBuf<int> elist, adtInt;
   DBuf<double> ebox, adtReal;
   DBuf<WalkNode> nodes;
+  DBuf<int> rcell;  // the cell of each walk record (node id order)
   BinGrid bins;
   double adtExt[6] = {0, 0, 0, 0, 0, 0};
   // searches against this block walk this tree (read only; another engine
@@ -255,6 +257,7 @@ struct QuerySet {
   DBuf<int> elist, adtInt;
   DBuf<double> ebox, adtReal;
   DBuf<WalkNode> nodes;
+  DBuf<int> rcell;  // the cell of each walk record (node id order)
   BinGrid bins;
   double ext[6] = {0, 0, 0, 0, 0, 0};
 };
@@ -361,15 +364,14 @@ struct StepScratch {
   DBuf<OBB> obc;
 };
 
-// Reusable device scratch of the donor search: per-query candidate lists
-// and the queue of queries with more candidates than a list holds.
-// Launch parameters of the donor search, none of which changes a result:
-// Morton bits of the query order, candidates staged per lane by the walk
-// (16, 24 or 32), workgroups of the containment-test kernels.
+// Parameters of the donor search, none of which changes a result: Morton
+// bits of the query order, bin-list entries per lane in one work item of the
+// bin listing, workgroups of the containment-test kernels, and the bin
+// density (0: candidates listed by the tree walk).
 struct SearchTuning {
-  int morton_bits = 21;
-  int kstage = 16;
-  int eval_grid = 4096;
+  int morton_bits = 15;
+  int seg = 256;  // bin-list entries per lane in one work item of the listing
+  int eval_grid = 8192;
   // candidates listed from bin grids with about this many elements per bin
   // (built with each tree; 0: by the tree walk)
   int bins = 2;
@@ -377,13 +379,15 @@ struct SearchTuning {
 
 struct SearchScratch {
   SearchTuning tune;
-  DBuf<int> cell, query, status, qoff, qcnt, ovf, counters, bins, blist, lj, lcount, qb, qsid, nin, wstart, spill;
-  int spill_chunks = 0;  // chunks of the walk's spill pool
+  DBuf<int> cell, query, status, counters, bins, blist, lj, lcount, qb, qsid, nin, wstart;
+  DBuf<int> best, last, acc;     // per query outcome words of a launch
+  DBuf<int> nseg, segstart;      // the bin listing's work items per walking wave
   DBuf<unsigned char> sv;  // the table of searches of one launch
   DBuf<double4> lst;
   size_t cap = 0;  // capacity of the flat candidate list
   int last_ns = 0, last_nqt = 0;  // searches and queries of the last launch
   bool last_bins = false;         // the last launch listed candidates from bin grids
+  long long last_nwave = 0;       // waves the last launch was sized for
   // the continued Newton solves run on a second stream (default priority),
   // beside the tet tests; created on first use
   hipStream_t side = nullptr;
@@ -589,10 +593,6 @@ class Engine {
   // grids are built again at a new density, trees built later use it
   void set_search_bins(int bins);
   void set_search_capacity(size_t cap) { sw_.cap = cap; }
-  // chunks of the walk's spill pool the next search starts from (0 until
-  // the first search sizes it); a checked search grows it when it runs out
-  int search_spill_chunks() const { return sw_.spill_chunks; }
-  void set_search_spill_chunks(int n) { sw_.spill_chunks = n; }
 
   // AMR Cartesian background, gather direction (tg_amr.hip):
   // tioga::register_amr_global_data / set_amr_patch_count /
@@ -649,7 +649,7 @@ class Engine {
 
   void build_search_structure(Block &b, const OBB &obq, int &nelem, int &depth, DBuf<int> &elist,
                               DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, DBuf<WalkNode> &nodes,
-                              double ext[6], BinGrid &bins);
+                              DBuf<int> &rcell, double ext[6], BinGrid &bins);
   void build_bins(int ntree, const WalkNode *const *nodes, const int *nelem, BinGrid &store, BinGrid *const *views);
   BinGrid tbins_;                      // the bin grids of every batched tree
   DBuf<int> bcnt_;                     // bin counts of a bin build
@@ -669,6 +669,7 @@ class Engine {
   DBuf<double> tebox_, treal_;  // every batched tree's element boxes / adtReal, back to back
   DBuf<int> tint_, telist_;     // every batched tree's adtInt / element list, back to back
   DBuf<WalkNode> tnodes_;        // every batched tree's search records, back to back
+  DBuf<int> trcell_;             // every batched tree's record cells, back to back
   DBuf<unsigned char> ntab_;    // the per-tree table of the batched search records
   DBuf<unsigned char> ptab_;    // the per-block table of the batched cell filter / element boxes
   DBuf<int> pseg_;              // their cell and element offsets

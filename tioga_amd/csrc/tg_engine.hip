@@ -950,12 +950,13 @@ __device__ __forceinline__ void grid_box(const double *b, const GridFrame &g, un
 __device__ __forceinline__ void k_adt_node_one(int n, const TypeInfo &ti, const GridFrame &g,
                                                const int *__restrict__ adtInt, const double *__restrict__ adtReal,
                                                const double *__restrict__ ebox, const int *__restrict__ elist,
-                                               WalkNode *__restrict__ nodes)
+                                               WalkNode *__restrict__ nodes, int *__restrict__ rcell)
 {
   int4 a = reinterpret_cast<const int4 *>(adtInt)[n];
   WalkNode N;
   const int cell = elist[a.x];
   N.cellnv = cell | (ti.nv[type_of(ti, cell)] << 28);
+  rcell[n] = cell;
   N.left = a.y > -1 ? adtInt[4 * (size_t)a.y + 3] : -1;
   N.right = a.z > -1 ? adtInt[4 * (size_t)a.z + 3] : -1;
   grid_box(ebox + 6 * (size_t)a.x, g, N.e);
@@ -968,11 +969,11 @@ __device__ __forceinline__ void k_adt_node_one(int n, const TypeInfo &ti, const 
 
 __global__ void k_adt_nodes(int nelem, TypeInfo ti, GridFrame g, const int *__restrict__ adtInt,
                             const double *__restrict__ adtReal, const double *__restrict__ ebox,
-                            const int *__restrict__ elist, WalkNode *__restrict__ nodes)
+                            const int *__restrict__ elist, WalkNode *__restrict__ nodes, int *__restrict__ rcell)
 {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nelem) return;
-  k_adt_node_one(n, ti, g, adtInt, adtReal, ebox, elist, nodes);
+  k_adt_node_one(n, ti, g, adtInt, adtReal, ebox, elist, nodes, rcell);
 }
 
 // k_adt_nodes for every tree of a batch whose arrays lie back to back: tree
@@ -986,7 +987,7 @@ struct AdtNodesTree {
 __global__ void k_adt_nodes_all(int ntree, int total, const AdtNodesTree *__restrict__ T,
                                 const int *__restrict__ adtInt, const double *__restrict__ adtReal,
                                 const double *__restrict__ ebox, const int *__restrict__ elist,
-                                WalkNode *__restrict__ nodes)
+                                WalkNode *__restrict__ nodes, int *__restrict__ rcell)
 {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
@@ -998,7 +999,7 @@ __global__ void k_adt_nodes_all(int ntree, int total, const AdtNodesTree *__rest
   }
   const size_t b = T[lo].base;
   k_adt_node_one(g - (int)b, T[lo].ti, T[lo].g, adtInt + 4 * b, adtReal + 6 * b, ebox + 6 * b, elist + b,
-                 nodes + b);
+                 nodes + b, rcell + b);
 }
 
 // 30-bit Morton code of each query in the frame of the search structure's
@@ -1092,6 +1093,7 @@ struct SearchView {
   const int *xtag;
   int *donorId;
   const WalkNode *nodes;
+  const int *rcell;  // the cell of each walk record
   GridFrame g;  // the tree's grid (the walk records' bounds are on it)
   double ext[6];
   const int *conn;
@@ -1109,11 +1111,10 @@ struct SearchView {
 // run by one launch of each kernel.  Queries are numbered globally: search s
 // owns [qbase[s], qbase[s+1]).  Within a search, position t (processing
 // order) holds query perm[t]; the first nin[s] positions are the queries
-// that walk the tree (k_morton_sets), in waves of 64: search s's walking
-// waves are [wstart[s], wstart[s+1]) of the enumeration k_collect takes.  The
-// per-query walk results (qcnt, qoff) are indexed by global position
-// qbase[s] + t, candidates carry the global query id qbase[s] + perm[t], and
-// qsid maps either to its search (both ranges are [qbase[s], qbase[s+1])).
+// that search the tree (k_morton_sets), in waves of 64: search s's walking
+// waves are [wstart[s], wstart[s+1]) of the enumeration the listing kernels
+// take.  Candidates and the per-query outcome words carry the global position
+// qbase[s] + t, and qsid maps a global position to its search.
 struct SearchSet {
   const SearchView *sv;
   const int *qbase, *nin, *wstart;
@@ -1143,6 +1144,7 @@ __device__ __forceinline__ SearchView view_of(const SearchSet &T, int sid)
   S.xtag = as_global(v->xtag);
   S.donorId = as_global(v->donorId);
   S.nodes = as_global(v->nodes);
+  S.rcell = as_global(v->rcell);
   for (int d = 0; d < 3; d++) {
     S.g.lo[d] = v->g.lo[d];
     S.g.inv[d] = v->g.inv[d];
@@ -1183,28 +1185,24 @@ constexpr int WAVE = 64;
 constexpr int STACK = 32;
 static_assert(STACK <= WAVE, "k_collect keeps its stack one entry per lane");
 
-// Candidate lists of one search: a flat list of (cell | nv << 28, query)
-// entries, grouped per query in walk order, and per query its offset and
-// count (count 0: nothing to test).  A slot that was reserved but not filled
-// (its query's list did not fit below the capacity) holds EMPTY_SLOT, so
-// every slot below min(used, cap) holds either a candidate or EMPTY_SLOT;
-// no kernel reads a value left from an earlier search.
+// Candidates of one search launch: a flat list of entries j = (rank | nv << 28,
+// global position of the query, test outcome), appended in any order.  Per
+// query, by its position (Morton order, so neighbouring candidates' words are
+// neighbours), the
+// tests fold into best (the lowest rank of a clean hit; INT_MAX: none), then,
+// for the queries without one, into last ((rank << 2) | outcome of the
+// highest-ranked candidate inside its element box; -1: none), and with a
+// tie-band counter into acc (candidates accepting the point, plus 2^20 per
+// near-zero-weight acceptance).  The list
+// has a capacity: entries appended past it are dropped, and the host sees
+// used > cap, grows the list and lists again before any test runs.
 constexpr int EMPTY_SLOT = -1;  // nv field 15: no cell type has it
 struct CandOut {
   int *cell, *query, *status;
-  int *qoff, *qcnt;
-  int *used;  // entries allocated so far
+  int *used;  // entries appended so far
   int cap;    // capacity of the flat list
-  int *ovf, *novf;  // queries whose candidates k_collect could not hold
-  // Spill pool of k_collect: chunks of SPILL_K entries for each of a wave's
-  // 64 lanes (entry k of lane l at chunk * SPILL_K * 64 + k * 64 + l), taken
-  // by a wave when one of its lanes has more than KSTAGE candidates.
-  int *spill;
-  int spill_chunks;  // chunks in the pool
-  int *nspill;       // chunks taken so far
+  int *best, *last, *acc;  // per global position (acc: null without a tie-band counter)
 };
-constexpr int SPILL_K = 16;    // entries per lane in one spill chunk
-constexpr int SPILL_MAX = 64;  // chunks one wave may hold (KSTAGE + 1024 entries per lane)
 
 // ADT::searchADT / searchIntersections (searchADTrecursion.C:26-101).  The
 // reference walks the tree in preorder (a node's own element, then its left
@@ -1221,9 +1219,10 @@ constexpr int SPILL_MAX = 64;  // chunks one wave may hold (KSTAGE + 1024 entrie
 // the cell of the lowest-ranked clean hit, and when there is none, the
 // outcome of the highest-ranked candidate (its cell when accepted with a
 // near-zero weight in a BIGVALUE cell, else -1).  So the search runs as
-//   k_collect / k_find  list each query's candidates (as rank | nv << 28),
+//   k_collect / k_find  list (rank | nv << 28, query) pairs, in any order,
 //   k_eval              every candidate's containment test, one thread each,
-//   k_select            the rule above, per query, in any listing order,
+//                       folded into its query's best / last words (atomics),
+//   k_select            the rule above, per query, from those two words,
 // which keeps long candidate lists and slow Newton solves from serialising
 // inside one lane.
 //
@@ -1232,90 +1231,64 @@ constexpr int SPILL_MAX = 64;  // chunks one wave may hold (KSTAGE + 1024 entrie
 // so they list a superset of the candidates.  k_eval makes the exact FP64
 // element-box test of each listed candidate (the element box is the min/max
 // of the cell's vertices, search.C:186-214) and marks the ones that fail
-// OUTSIDE, which k_select skips.
+// OUTSIDE, which take no part in the selection.
 //
-// Listing: candidates are staged in LDS (KSTAGE per lane, 2-8 KB per wave)
-// and copied out to a chunk of the flat list the wave allocates at the end.
-// A lane's candidates beyond KSTAGE go to spill chunks the wave takes from a
-// global pool (one atomic per SPILL_K entries of its longest list), in the
-// same lane-major layout as the stage.  Only when the pool is exhausted, or a
-// list exceeds KSTAGE + SPILL_K * SPILL_MAX, is the lane queued for
-// k_collect_more, which walks its query again alone and lists all its
-// candidates.  KSTAGE trades occupancy against spill traffic; the result does
-// not depend on it.
-template <int KSTAGE>
-struct Stager {
-  int *stage;  // LDS, [KSTAGE][WAVE]
-  int *chunk;  // LDS, the wave's spill chunks in order
-  int lane, nc = 0, nchunk = 0;
-  bool spill_ok = true;  // false once the pool or the wave's chunk table ran out
-  // one listing step of the wave: each lane adds at most one candidate, so a
-  // new spill chunk is needed exactly when some lane reaches entry
-  // SPILL_K * nchunk past the stage.  Called in wave-uniform control flow.
-  __device__ __forceinline__ void emit(bool hit, int val, const CandOut &O)
+// A wave gathers its lanes' candidates in an LDS buffer of WBUF pairs and
+// appends the buffer to the flat list (one atomic) when it is nearly full and
+// at the end.  The copy places the pairs sorted by lane, in step order within
+// a lane (each pair keeps its index among its lane's pairs, and the lanes'
+// counts give their offsets), so a query's candidates mostly lie together in
+// the list and its containment tests run side by side (one point,
+// neighbouring cells).
+constexpr int WBUF = 512;
+constexpr unsigned FIND_GRID = 16384;  // workgroups of k_find (each takes work items in turn)
+struct WaveBuf {
+  int *val, *qid, *key;  // LDS, WBUF entries each: value, position, lane | index << 6
+  int *off;              // LDS, WAVE: each lane's first slot in the sorted order
+  int lane;
+  int n = 0;     // wave-uniform: pairs held
+  int mine = 0;  // this lane's pairs held
+  // each lane adds at most one pair; called in wave-uniform control flow
+  __device__ __forceinline__ void emit(bool hit, int v, int g, const CandOut &O)
   {
-    if (hit & (nc < KSTAGE)) stage[nc * WAVE + lane] = val;
-    const int e = nc - KSTAGE;
-    if (__ballot(hit & (e >= 0)) && spill_ok) {
-      if (__ballot(hit & (e == SPILL_K * nchunk))) {
-        int id = O.spill_chunks;
-        if (nchunk < SPILL_MAX) {
-          if (lane == 0) id = atomicAdd(O.nspill, 1);
-          id = __builtin_amdgcn_readfirstlane(id);
-        }
-        if (id < O.spill_chunks) {
-          if (lane == 0) chunk[nchunk] = id;
-          __syncthreads();  // one wave per workgroup: orders lane 0's write before the reads
-          nchunk++;
-        } else {
-          spill_ok = false;
-        }
-      }
-      if (spill_ok && hit && e >= 0) O.spill[((size_t)chunk[e / SPILL_K] * SPILL_K + e % SPILL_K) * WAVE + lane] = val;
+    const unsigned long long m = __ballot(hit);
+    if (!m) return;
+    if (hit) {
+      const int k = n + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+      val[k] = v;
+      qid[k] = g;
+      key[k] = lane | (mine << 6);
+      mine++;
     }
-    nc += hit;
+    n += __popcll(m);
+    if (n > WBUF - WAVE) flush(O);
   }
-  // one chunk of the flat list per wave; valid: the lane holds query g at
-  // position gp.  Called in wave-uniform control flow.
-  __device__ __forceinline__ void flush(const CandOut &O, bool valid, int gp, int g)
+  // called in wave-uniform control flow; one wave per workgroup
+  __device__ __forceinline__ void flush(const CandOut &O)
   {
-    const bool held = spill_ok || nc <= KSTAGE;  // every candidate staged or spilled
-    const int k = (valid && held) ? nc : 0;      // the others are listed later
-    int pre = k;
+    if (n == 0) return;
+    int pre = mine;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      int v = __shfl_up(pre, o);
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const int v = __shfl_up(pre, o);
       if (lane >= o) pre += v;
     }
-    const int total = __shfl(pre, 63);
-    pre -= k;
+    off[lane] = pre - mine;
     int base = 0;
-    if (lane == 0 && total > 0) base = atomicAdd(O.used, total);
+    if (lane == 0) base = atomicAdd(O.used, n);
     base = __shfl(base, 0);
-    if (!valid) return;
-    O.qcnt[gp] = nc;
-    if (!held) {
-      O.ovf[atomicAdd(O.novf, 1)] = gp;
-      return;
+    __syncthreads();  // the lanes' pair writes and offsets before the copy reads them
+    for (int j = lane; j < n; j += WAVE) {
+      const int kj = key[j];
+      const int d = base + off[kj & (WAVE - 1)] + (kj >> 6);
+      if (d < O.cap) {
+        O.cell[d] = val[j];
+        O.query[d] = qid[j];
+      }
     }
-    const int off = base + pre;
-    O.qoff[gp] = off;
-    if (off + k > O.cap) {
-      // flat list too small (the host sees used > cap, grows it and lists
-      // again): the reserved slots below the capacity are marked empty
-      for (int j = off; j < O.cap && j < off + k; j++) O.cell[j] = EMPTY_SLOT;
-      return;
-    }
-    const int ks = k < KSTAGE ? k : KSTAGE;
-    for (int j = 0; j < ks; j++) {
-      O.cell[off + j] = stage[j * WAVE + lane];
-      O.query[off + j] = g;
-    }
-    for (int j = KSTAGE; j < k; j++) {
-      const int e2 = j - KSTAGE;
-      O.cell[off + j] = O.spill[((size_t)chunk[e2 / SPILL_K] * SPILL_K + e2 % SPILL_K) * WAVE + lane];
-      O.query[off + j] = g;
-    }
+    __syncthreads();  // the copy before the buffer is written again
+    n = 0;
+    mine = 0;
   }
 };
 
@@ -1327,11 +1300,9 @@ __device__ __forceinline__ int rank_nv(int node, int cellnv) { return node | (ce
 // lane's own walk would, carrying the mask of those lanes, so every lane sees
 // exactly its own preorder walk while node records are wave-uniform scalar
 // loads.
-template <int KSTAGE>
 __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
 {
-  __shared__ int stage[KSTAGE * WAVE];
-  __shared__ int chunk[SPILL_MAX];
+  __shared__ int bval[WBUF], bqid[WBUF], bkey[WBUF], boff[WAVE];
   const int lane = threadIdx.x;
   // workgroup wv walks walking wave wv; the grid is sized for every query,
   // so the workgroups past the walking waves leave at once
@@ -1354,7 +1325,8 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
     for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= S.ext[2 * d + 1] + TOL);
     for (int d = 0; d < 3; d++) q[d] = grid_of(p[d], S.g.lo[d], S.g.inv[d]);
   }
-  Stager<KSTAGE> st{stage, chunk, lane};
+  const int gp = qb + t;  // the query's global position (its outcome words' index)
+  WaveBuf buf{bval, bqid, bkey, boff, lane};
   // pending right subtrees: entry k (node, lane mask) held by lane k, so a
   // push is a select and a pop a readlane; the left child, when any lane
   // enters it, is visited next without the stack (preorder)
@@ -1379,7 +1351,7 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
       N.right = r->right;
     }
     const bool act = (m >> lane) & 1ull;
-    st.emit(act & in_grid(q, N.e), rank_nv(node, N.cellnv), O);
+    buf.emit(act & in_grid(q, N.e), rank_nv(node, N.cellnv), gp, O);
     const int r = N.right, l = N.left;
     const unsigned long long mr = (r > -1) ? __ballot(act & in_grid(q, N.r)) : 0ull;
     const unsigned long long ml = (l > -1) ? __ballot(act & in_grid(q, N.l)) : 0ull;
@@ -1403,103 +1375,103 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
       more = false;
     }
   }
-  st.flush(O, i >= 0, qb + t, qb + i);
+  buf.flush(O);
 }
 
-// k_find: the same candidate sets from the tree's bin grid instead of the
-// walk.  The bins split the tree's 16-bit grid into 2^(16 - bshift) cells per
-// axis; bin b lists (packed grid box, rank | nv << 28) of every element whose
-// grid box overlaps it, in any order.  A point lies in exactly one bin, and
-// every grid box holding the point overlaps that bin, so scanning the bin's
-// list with the grid test lists the walk's candidates.  Each lane scans its
-// own bin's list (independent loads, four at a time); the wave steps through
-// the longest list of its lanes.
-template <int KSTAGE>
-__global__ void __launch_bounds__(WAVE, 8) k_find(SearchSet T, CandOut O)
+// k_find lists the same candidate sets from the tree's bin grid instead of
+// the walk.  The bins split the tree's 16-bit grid into 2^(16 - bshift)
+// cells per axis; bin b lists (packed grid box, rank | nv << 28) of every
+// element whose grid box overlaps it, in any order.  A point lies in exactly
+// one bin, and every grid box holding the point overlaps that bin, so
+// scanning the bin's list with the grid test lists the walk's candidates.
+//
+// The lanes of a walking wave scan their own bins' lists (independent loads,
+// four at a time).  Lists differ widely in length (where many boxes overlap,
+// as at the axis of a lat-long mesh, a list holds hundreds of elements), so a
+// wave's scan is cut into segments of seg entries per lane, and each segment
+// is a work item of its own: k_find_cost counts each walking wave's segments,
+// a scan numbers the items, and k_find's workgroups take the items in turn.
+__device__ __forceinline__ void find_lane(const SearchView &S, int i, unsigned q[3], int &start, int &len)
 {
-  __shared__ int stage[KSTAGE * WAVE];
-  __shared__ int chunk[SPILL_MAX];
-  const int lane = threadIdx.x;
-  const int wv = blockIdx.x;
-  if (wv >= T.wstart[T.ns]) return;
-  const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, wv));
+  start = 0;
+  len = 0;
+  if (i < 0) return;
+  double p[3];
+  load_xyz(S.xs, i, p);
+  bool inroot = true;  // the root box test, FP64 as searchADTrecursion.C:38-43 makes it
+  for (int d = 0; d < 3; d++) inroot = inroot && (p[d] >= S.ext[2 * d] - TOL);
+  for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= S.ext[2 * d + 1] + TOL);
+  for (int d = 0; d < 3; d++) q[d] = grid_of(p[d], S.g.lo[d], S.g.inv[d]);
+  if (!inroot || !S.boff) return;
+  const int sh = S.bshift, R = 65536 >> sh;
+  const int bin = (int)(((q[2] >> sh) * (unsigned)R + (q[1] >> sh)) * (unsigned)R + (q[0] >> sh));
+  start = S.boff[bin];
+  len = S.boff[bin + 1] - start;
+}
+
+// nseg[w] = segments of walking wave w (at least one), 0 past the walking waves
+__global__ void __launch_bounds__(256) k_find_cost(SearchSet T, int nwave, int seg, int *__restrict__ nseg)
+{
+  const int w = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+  if (w >= nwave) return;
+  const int nwalk = T.wstart[T.ns];
+  if (w >= nwalk) {
+    if (lane == 0) nseg[w] = 0;
+    return;
+  }
+  const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, w));
   const SearchView S = view_of(T, sid);
-  const int qb = T.qbase[sid];
-  const int t = (wv - T.wstart[sid]) * WAVE + lane;
-  const int nin = T.nin[sid];
-  const int i = t < nin ? S.perm[t] : -1;
+  const int t = (w - T.wstart[sid]) * WAVE + lane;
+  const int i = t < T.nin[sid] ? S.perm[t] : -1;
   unsigned q[3] = {0u, 0u, 0u};
-  int start = 0, len = 0;
-  if (i >= 0) {
-    double p[3];
-    load_xyz(S.xs, i, p);
-    bool inroot = true;  // the root box test, FP64 as searchADTrecursion.C:38-43 makes it
-    for (int d = 0; d < 3; d++) inroot = inroot && (p[d] >= S.ext[2 * d] - TOL);
-    for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= S.ext[2 * d + 1] + TOL);
-    for (int d = 0; d < 3; d++) q[d] = grid_of(p[d], S.g.lo[d], S.g.inv[d]);
-    if (inroot && S.boff) {
-      const int sh = S.bshift, R = 65536 >> sh;
-      const int bin = (int)(((q[2] >> sh) * (unsigned)R + (q[1] >> sh)) * (unsigned)R + (q[0] >> sh));
-      start = S.boff[bin];
-      len = S.boff[bin + 1] - start;
-    }
-  }
-  int lmax = len;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) lmax = max(lmax, __shfl_xor(lmax, o));
-  Stager<KSTAGE> st{stage, chunk, lane};
-  const uint4 *ent = S.bent;
-  for (int k = 0; k < lmax; k += 4) {
-    uint4 e[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) e[u] = (k + u < len) ? ent[start + k + u] : make_uint4(0xffffu, 0u, 0u, 0u);
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const unsigned w[3] = {e[u].x, e[u].y, e[u].z};
-      st.emit((k + u < len) && in_grid(q, w), (int)e[u].w, O);
-    }
-  }
-  st.flush(O, i >= 0, qb + t, qb + i);
+  int start, len;
+  find_lane(S, i, q, start, len);
+  for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o));
+  if (lane == 0) nseg[w] = len > seg ? (len + seg - 1) / seg : 1;
 }
 
-// the walk of one query, listing all its candidates (queries whose lists
-// k_collect or k_find could not hold); a grid-stride loop over the queue
-__global__ void __launch_bounds__(64) k_collect_more(SearchSet T, CandOut O)
+__global__ void __launch_bounds__(WAVE, 8) k_find(SearchSet T, CandOut O, int nwave, int seg,
+                                                   const int *__restrict__ segstart)
 {
-  __shared__ int stk[STACK][64];
-  const int n = *O.novf, tid = threadIdx.x;
-  for (int qi = blockIdx.x * 64 + tid; qi < n; qi += gridDim.x * 64) {
-    const int gp = O.ovf[qi];
-    const int sid = T.ns == 1 ? 0 : T.qsid[gp];
-    const SearchView S = view_of(T, sid);
-    const WalkNode *nodes = S.nodes;
-    const int i = S.perm[gp - T.qbase[sid]];
-    const int g = T.qbase[sid] + i;
-    const int cnt = O.qcnt[gp];
-    const int off = atomicAdd(O.used, cnt);
-    O.qoff[gp] = off;
-    if (off + cnt > O.cap) {  // as in k_collect: reserved slots below the capacity marked empty
-      for (int j = off; j < O.cap && j < off + cnt; j++) O.cell[j] = EMPTY_SLOT;
-      continue;
+  __shared__ int bval[WBUF], bqid[WBUF], bkey[WBUF], boff[WAVE];
+  const int lane = threadIdx.x;
+  WaveBuf buf{bval, bqid, bkey, boff, lane};
+  const int items = segstart[nwave];
+  for (int it = blockIdx.x; it < items; it += gridDim.x) {
+    // the walking wave of item it (last w with segstart[w] <= it)
+    int lo = 0, hi = nwave - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (segstart[mid] <= it) lo = mid;
+      else hi = mid - 1;
     }
-    double p[3];
-    unsigned q[3];
-    load_xyz(S.xs, i, p);
-    for (int d = 0; d < 3; d++) q[d] = grid_of(p[d], S.g.lo[d], S.g.inv[d]);
-    int sp = 0, nc = 0;
-    stk[sp++][tid] = 0;
-    while (sp > 0) {
-      const int node = stk[--sp][tid];
-      const WalkNode &N = nodes[node];
-      if (in_grid(q, N.e)) {
-        O.cell[off + nc] = rank_nv(node, N.cellnv);
-        O.query[off + nc] = g;
-        nc++;
+    const int w = __builtin_amdgcn_readfirstlane(lo);
+    const int k0 = (it - segstart[w]) * seg;
+    const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, w));
+    const SearchView S = view_of(T, sid);
+    const int t = (w - T.wstart[sid]) * WAVE + lane;
+    const int i = t < T.nin[sid] ? S.perm[t] : -1;
+    unsigned q[3] = {0u, 0u, 0u};
+    int start, len;
+    find_lane(S, i, q, start, len);
+    const int gp = T.qbase[sid] + t;
+    const int end = min(len, k0 + seg);
+    int lmax = end;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) lmax = max(lmax, __shfl_xor(lmax, o));
+    const uint4 *ent = S.bent;
+    for (int k = k0; k < lmax; k += 4) {
+      uint4 e[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) e[u] = (k + u < end) ? ent[start + k + u] : make_uint4(0xffffu, 0u, 0u, 0u);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const unsigned wb[3] = {e[u].x, e[u].y, e[u].z};
+        buf.emit((k + u < end) && in_grid(q, wb), (int)e[u].w, gp, O);
       }
-      if (N.right > -1 && in_grid(q, N.r)) stk[sp++][tid] = N.right;
-      if (N.left > -1 && in_grid(q, N.l)) stk[sp++][tid] = N.left;
     }
   }
+  buf.flush(O);
 }
 
 // Candidate tests binned by cell type (bucket 0..3 = 4, 5, 6, 8 vertices),
@@ -1591,11 +1563,35 @@ __device__ __forceinline__ int test_status(const double frac[8], bool big)
   return flag ? 1 : 2;
 }
 
-// the cell of a listed candidate (rank | nv << 28): its walk record's cell
-__device__ __forceinline__ int cell_of(const SearchView &S, int ranknv)
+// a tested candidate's outcome: its status, and a clean hit into its query's
+// best word (and the tie-band count); last is formed afterwards, only for
+// the queries without a clean hit (k_last)
+__device__ __forceinline__ void record_outcome(const CandOut &O, int j, int st)
 {
-  return S.nodes[ranknv & 0x0fffffff].cellnv & 0x0fffffff;
+  O.status[j] = st;
+  if (st == OUTSIDE || st == 0) return;
+  const int g = O.query[j];
+  if (st == 2) atomicMin(&O.best[g], O.cell[j] & 0x0fffffff);
+  if (O.acc) atomicAdd(&O.acc[g], st == 1 ? (1 << 20) + 1 : 1);
 }
+
+// last = the highest (rank << 2 | status) among a query's candidates inside
+// their element box, for the queries without a clean hit (the only ones
+// k_select reads it for)
+__global__ void __launch_bounds__(256) k_last(CandOut O)
+{
+  const int n = min(*O.used, O.cap);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const int st = O.status[j];
+    if (st == OUTSIDE) continue;
+    const int g = O.query[j];
+    if (O.best[g] != 0x7fffffff) continue;
+    atomicMax(&O.last[g], ((O.cell[j] & 0x0fffffff) << 2) | st);
+  }
+}
+
+// the cell of a listed candidate (rank | nv << 28): its walk record's cell
+__device__ __forceinline__ int cell_of(const SearchView &S, int ranknv) { return S.rcell[ranknv & 0x0fffffff]; }
 
 template <int NV>
 __device__ __forceinline__ void cell_vertices(const SearchView &S, int c, double xv[8][3])
@@ -1629,11 +1625,11 @@ __global__ void __launch_bounds__(256, WAVES) k_eval(SearchSet T, CandOut O, con
   const int n = bcount[B];
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const int j = blist[b0 + k];
-    const int g = O.query[j], sid = T.ns == 1 ? 0 : T.qsid[g];
+    const int gp = O.query[j], sid = T.ns == 1 ? 0 : T.qsid[gp];
     const SearchView S = view_of(T, sid);
     const int c = cell_of(S, O.cell[j]);
     double p[3], xv[8][3], frac[8];
-    load_xyz(S.xs, g - T.qbase[sid], p);
+    load_xyz(S.xs, S.perm[gp - T.qbase[sid]], p);
     cell_vertices<NV>(S, c, xv);
     // the element box test of searchADTrecursion.C:66-71 on the box of
     // search.C:199-214 (min/max of the vertices; the sign of a zero bound
@@ -1650,7 +1646,7 @@ __global__ void __launch_bounds__(256, WAVES) k_eval(SearchSet T, CandOut O, con
       inb = inb & (p[d] >= lo - TOL) & (p[d] <= hi + TOL);
     }
     if (!inb) {
-      O.status[j] = OUTSIDE;
+      record_outcome(O, j, OUTSIDE);
       continue;
     }
     if (NV == 4) {
@@ -1671,7 +1667,7 @@ __global__ void __launch_bounds__(256, WAVES) k_eval(SearchSet T, CandOut O, con
       newton3_finish(it, ok, u, v, w);
       iso_weights(u, v, w, frac, NV);
     }
-    O.status[j] = test_status<NV>(frac, S.cellRes[c] == BIGVALUE);
+    record_outcome(O, j, test_status<NV>(frac, S.cellRes[c] == BIGVALUE));
   }
 }
 
@@ -1694,22 +1690,12 @@ __global__ void __launch_bounds__(256) k_status_hist(CandOut O, int *__restrict_
 
 // candidates ranked after their query's lowest-ranked clean hit (the
 // reference's walk stops at that hit and never tests them); a diagnostic
-__global__ void k_after_hit(SearchSet T, CandOut O, unsigned long long *__restrict__ out)
+__global__ void k_after_hit(CandOut O, unsigned long long *__restrict__ out)
 {
-  const int gp = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = min(*O.used, O.cap);
   unsigned long long v = 0;
-  if (gp < T.nqt) {
-    const int sid = T.ns == 1 ? 0 : T.qsid[gp];
-    if (gp - T.qbase[sid] < T.nin[sid]) {
-      const int off = O.qoff[gp], cnt = O.qcnt[gp];
-      if (off >= 0 && off + cnt <= O.cap) {
-        int hit = 0x7fffffff;
-        for (int j = 0; j < cnt; j++)
-          if (O.status[off + j] == 2) hit = min(hit, O.cell[off + j] & 0x0fffffff);
-        for (int j = 0; j < cnt; j++) v += (O.cell[off + j] & 0x0fffffff) > hit;
-      }
-    }
-  }
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    if (O.cell[j] != EMPTY_SLOT) v += (O.cell[j] & 0x0fffffff) > O.best[O.query[j]];
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
 }
@@ -1723,11 +1709,11 @@ __global__ void __launch_bounds__(256, 4) k_eval_long(SearchSet T, CandOut O, Lo
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const int j = L.j[B * L.cap + k];
     const double4 st = L.st[B * L.cap + k];
-    const int g = O.query[j], sid = T.ns == 1 ? 0 : T.qsid[g];
+    const int gp = O.query[j], sid = T.ns == 1 ? 0 : T.qsid[gp];
     const SearchView S = view_of(T, sid);
     const int c = cell_of(S, O.cell[j]);
     double p[3], xv[8][3], frac[8], f[8][3];
-    load_xyz(S.xs, g - T.qbase[sid], p);
+    load_xyz(S.xs, S.perm[gp - T.qbase[sid]], p);
     cell_vertices<NV>(S, c, xv);
     iso_coeffs(xv, p, f, NV);
     double u = st.x, v = st.y, w = st.z;
@@ -1735,12 +1721,15 @@ __global__ void __launch_bounds__(256, 4) k_eval_long(SearchSet T, CandOut O, Lo
     newton3_steps(f, u, v, w, it, NEWTON_MAX, ok);
     newton3_finish(it, ok, u, v, w);
     iso_weights(u, v, w, frac, NV);
-    O.status[j] = test_status<NV>(frac, S.cellRes[c] == BIGVALUE);
+    record_outcome(O, j, test_status<NV>(frac, S.cellRes[c] == BIGVALUE));
   }
 }
 
-// the reference's result from the candidates' outcomes, in walk order, for
-// the queries that walked (the others' donors are set elsewhere)
+// the reference's result for the queries that walked (the others' donors are
+// set elsewhere): the lowest-ranked clean hit, else the highest-ranked
+// candidate's outcome; with a tie-band counter the queries whose donor would
+// depend on the walk order are counted (more than one candidate accepts the
+// point, or one accepts it with a near-zero weight in a BIGVALUE cell)
 __global__ void k_select(SearchSet T, CandOut O)
 {
   const int gp = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1750,36 +1739,14 @@ __global__ void k_select(SearchSet T, CandOut O)
   if (t >= T.nin[sid]) return;
   const SearchView S = view_of(T, sid);
   const int i = S.perm[t];
-  const int nc = O.qcnt[gp], off = O.qoff[gp];
+  const int best = O.best[gp], last = O.last[gp];
   int res = -1;
-  // a list that did not fit was not stored; the host sees the overflow in
-  // the used count and searches again with a larger list
-  if ((long long)off + nc > O.cap) {
-    S.donorId[i] = -1;
-    return;
+  if (best != 0x7fffffff) res = S.rcell[best];
+  else if (last >= 0 && (last & 3) == 1) res = S.rcell[last >> 2];
+  if (S.tieband && O.acc) {
+    const int a = O.acc[gp];
+    if ((a & 0xfffff) > 1 || (a >> 20) > 0) atomicAdd(S.tieband, 1);
   }
-  // the lowest-ranked clean hit, else the highest-ranked candidate's outcome
-  // (the candidates' order in the list is free); with a tie-band counter the
-  // queries whose donor would depend on the walk order are counted: more
-  // than one candidate accepts the point, or one accepts it with a near-zero
-  // weight in a BIGVALUE cell
-  int hit = 0x7fffffff, last = -1, last_st = 0, nacc = 0;
-  bool flag1 = false;
-  for (int k = 0; k < nc; k++) {
-    const int st = O.status[off + k];
-    if (st == OUTSIDE) continue;
-    const int rank = O.cell[off + k] & 0x0fffffff;
-    nacc += st != 0;
-    flag1 = flag1 || st == 1;
-    if (st == 2 && rank < hit) hit = rank;
-    if (rank > last) {
-      last = rank;
-      last_st = st;
-    }
-  }
-  if (hit != 0x7fffffff) res = S.nodes[hit].cellnv & 0x0fffffff;
-  else if (last >= 0 && last_st == 1) res = S.nodes[last].cellnv & 0x0fffffff;
-  if (S.tieband && (nacc > 1 || flag1)) atomicAdd(S.tieband, 1);
   S.donorId[i] = res;
 }
 
@@ -2573,11 +2540,17 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
   w.qb.reserve(ns + 1);
   TG_HIP(hipMemcpyAsync(w.sv.p, S, sizeof(SearchView) * ns, hipMemcpyHostToDevice, s));
   TG_HIP(hipMemcpyAsync(w.qb.p, qb.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
-  w.qoff.reserve(nqt);
-  w.qcnt.reserve(nqt);
-  w.ovf.reserve(nqt);
-  w.counters.reserve(3);
-  if (w.spill_chunks == 0) w.spill_chunks = 16384;  // 64 MiB; grown when a checked pass runs out
+  w.counters.reserve(1);
+  // per query: lowest clean-hit rank, highest tested (rank, outcome), and the
+  // tie-band count when a search has a counter
+  bool tieband = false;
+  for (int k = 0; k < ns; k++) tieband = tieband || S[k].tieband != nullptr;
+  w.best.reserve(nqt);
+  w.last.reserve(nqt);
+  if (tieband) w.acc.reserve(nqt);
+  fill<int>(w.best.p, nqt, 0x7fffffff, s);
+  fill<int>(w.last.p, nqt, -1, s);
+  if (tieband) fill<int>(w.acc.p, nqt, 0, s);
   const SearchSet T{(const SearchView *)w.sv.p, w.qb.p, w.nin.p, w.wstart.p, w.qsid.p, ns, nqt};
   long long nwave = 0;  // an upper bound of the walking waves
   for (int k = 0; k < ns; k++) nwave += (S[k].nq + WAVE - 1) / WAVE;
@@ -2585,38 +2558,30 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
   bool bins = w.tune.bins > 0;
   for (int k = 0; k < ns; k++) bins = bins && S[k].boff != nullptr;
   w.last_bins = bins;
+  w.last_nwave = nwave;
+  if (bins) {  // the listing's work items: segments of the walking waves' bin lists
+    w.nseg.reserve((size_t)nwave + 1);
+    w.segstart.reserve((size_t)nwave + 1);
+    fill<int>(w.nseg.p + nwave, 1, 0, s);
+    k_find_cost<<<nb(nwave * WAVE), BS, 0, s>>>(T, (int)nwave, w.tune.seg, w.nseg.p);
+    exclusive_sum(w.nseg.p, w.segstart.p, (int)nwave + 1, s);
+  }
   if (w.cap == 0) w.cap = std::max<size_t>(4 * (size_t)nqt, 1 << 20);
   for (;;) {
     w.cell.reserve(w.cap);
     w.query.reserve(w.cap);
     w.status.reserve(w.cap);
-    w.spill.reserve((size_t)w.spill_chunks * SPILL_K * WAVE);
-    CandOut O{w.cell.p, w.query.p, w.status.p, w.qoff.p, w.qcnt.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
-              w.ovf.p, w.counters.p + 1, w.spill.p, w.spill_chunks, w.counters.p + 2};
-    fill<int>(w.counters.p, 3, 0, s);
-    if (bins) {
-      switch (w.tune.kstage) {
-        case 16: k_find<16><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
-        case 24: k_find<24><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
-        default: k_find<32><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
-      }
-    } else {
-      switch (w.tune.kstage) {
-        case 16: k_collect<16><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
-        case 24: k_collect<24><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
-        default: k_collect<32><<<(unsigned)nwave, WAVE, 0, s>>>(T, O); break;
-      }
-    }
-    k_collect_more<<<8192, 64, 0, s>>>(T, O);
+    CandOut O{w.cell.p, w.query.p, w.status.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
+              w.best.p, w.last.p, tieband ? w.acc.p : nullptr};
+    fill<int>(w.counters.p, 1, 0, s);
+    if (bins) k_find<<<FIND_GRID, WAVE, 0, s>>>(T, O, (int)nwave, w.tune.seg, w.segstart.p);
+    else k_collect<<<(unsigned)nwave, WAVE, 0, s>>>(T, O);
     if (check) {
-      int c[3] = {0, 0, 0};
-      TG_HIP(hipMemcpyAsync(c, w.counters.p, sizeof c, hipMemcpyDeviceToHost, s));
+      int used = 0;
+      TG_HIP(hipMemcpyAsync(&used, w.counters.p, sizeof used, hipMemcpyDeviceToHost, s));
       TG_HIP(hipStreamSynchronize(s));
-      // a spill pool that ran out only sent lists to k_collect_more; the
-      // next launch gets a larger one (at most 1 GiB)
-      if (c[2] > w.spill_chunks) w.spill_chunks = std::min(2 * c[2], 262144);
-      if ((size_t)c[0] > w.cap) {  // grow the flat list and walk again
-        w.cap = (size_t)c[0] + c[0] / 4;
+      if ((size_t)used > w.cap) {  // grow the flat list and list again (nothing was tested yet)
+        w.cap = (size_t)used + used / 4;
         continue;
       }
     }
@@ -2652,6 +2617,7 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     k_eval_long<8><<<1024, 256, 0, s>>>(T, O, L);
     k_eval<4, 8><<<eg, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
     TG_HIP(hipStreamWaitEvent(s, w.join, 0));
+    k_last<<<4096, 256, 0, s>>>(O);
     k_select<<<nb(nqt), BS, 0, s>>>(T, O);
     return;
   }
@@ -3633,7 +3599,7 @@ void Engine::search(Block &b)
     find_obbs_device(&os, 1, &obq, obbw_, s_, &st);
     obb_stepped_ += st;
     prep_acc_[0] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    build_search_structure(b, obq, b.nelem, b.adt_depth, b.elist, b.ebox, b.adtInt, b.adtReal, b.nodes,
+    build_search_structure(b, obq, b.nelem, b.adt_depth, b.elist, b.ebox, b.adtInt, b.adtReal, b.nodes, b.rcell,
                            b.adtExt, b.bins);
   }
   bool gid_mode = true;
@@ -3798,6 +3764,7 @@ void Engine::prepare_searches(const std::vector<int> &act)
   build_adt_batch(trees.data(), (int)trees.size(), adtw_, s_);
   // the search records of every tree in one launch
   tnodes_.reserve(te);
+  trcell_.reserve(te);
   std::vector<AdtNodesTree> nt(na);
   for (int a = 0; a < na; a++) {
     Block &b = *blocks_[act[a]];
@@ -3805,13 +3772,14 @@ void Engine::prepare_searches(const std::vector<int> &act)
     memcpy(b.adtExt, trees[a].ext, sizeof b.adtExt);
     if (b.adt_depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
     b.nodes.view(tnodes_.p + eoff[a], b.nelem);
+    b.rcell.view(trcell_.p + eoff[a], b.nelem);
     nt[a] = AdtNodesTree{b.ti, grid_frame(trees[a].ext), eoff[a]};
   }
   if (te) {
     ntab_.reserve(sizeof(AdtNodesTree) * na);
     TG_HIP(hipMemcpyAsync(ntab_.p, nt.data(), sizeof(AdtNodesTree) * na, hipMemcpyHostToDevice, s_));
     k_adt_nodes_all<<<nb(te), BS, 0, s_>>>(na, (int)te, (const AdtNodesTree *)ntab_.p, tint_.p, treal_.p,
-                                            tebox_.p, telist_.p, tnodes_.p);
+                                            tebox_.p, telist_.p, tnodes_.p, trcell_.p);
   }
   {  // the bin grids of every tree, one batch
     std::vector<const WalkNode *> np(na);
@@ -3867,8 +3835,8 @@ void Engine::build_resident_tree(int btag, ResidentTree &t)
 {
   Block *b = find(btag);
   if (!b || !b->topo_uploaded || b->cellRes.p == nullptr) throw std::runtime_error("tioga_amd: preprocess first");
-  build_search_structure(*b, b->obb, t.nelem, t.depth, t.elist, t.ebox, t.adtInt, t.adtReal, t.nodes, t.ext,
-                         t.bins);
+  build_search_structure(*b, b->obb, t.nelem, t.depth, t.elist, t.ebox, t.adtInt, t.adtReal, t.nodes, t.rcell,
+                         t.ext, t.bins);
   if (t.nelem != b->ncells) throw std::runtime_error("tioga_amd: resident tree misses cells of its block");
 }
 
@@ -3892,6 +3860,7 @@ void Engine::attach_resident(Block &b)
   b.adtInt.view(t.adtInt.p, 4 * (size_t)t.nelem);
   b.adtReal.view(t.adtReal.p, 6 * (size_t)t.nelem);
   b.nodes.view(t.nodes.p, t.nelem);
+  b.rcell.view(t.rcell.p, t.nelem);
   b.bins.shift = t.bins.shift;
   b.bins.nbins = t.bins.nbins;
   if (t.bins.nbins) {
@@ -3968,7 +3937,7 @@ void Engine::build_bins(int ntree, const WalkNode *const *nodes, const int *nele
 // as a LIFO list (descending cell index) -> element boxes -> ADT
 void Engine::build_search_structure(Block &b, const OBB &obq, int &nelem, int &depth, DBuf<int> &elist,
                                     DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal,
-                                    DBuf<WalkNode> &nodes, double ext[6], BinGrid &bins)
+                                    DBuf<WalkNode> &nodes, DBuf<int> &rcell, double ext[6], BinGrid &bins)
 {
   auto now = [] { return std::chrono::steady_clock::now(); };
   auto ms = [](auto a, auto c) { return std::chrono::duration<double, std::milli>(c - a).count(); };
@@ -3989,7 +3958,10 @@ void Engine::build_search_structure(Block &b, const OBB &obq, int &nelem, int &d
   depth = build_adt_device(ne, ebox.p, adtInt.p, adtReal.p, ext, adtw_, s_);
   if (depth + 2 > STACK) throw std::runtime_error("tioga_amd: ADT deeper than the search stack");
   nodes.reserve_own(ne);
-  if (ne) k_adt_nodes<<<nb(ne), BS, 0, s_>>>(ne, b.ti, grid_frame(ext), adtInt.p, adtReal.p, ebox.p, elist.p, nodes.p);
+  rcell.reserve_own(ne);
+  if (ne)
+    k_adt_nodes<<<nb(ne), BS, 0, s_>>>(ne, b.ti, grid_frame(ext), adtInt.p, adtReal.p, ebox.p, elist.p, nodes.p,
+                                       rcell.p);
   const WalkNode *np = nodes.p;
   BinGrid *bp = &bins;
   build_bins(1, &np, &ne, bins, &bp);
@@ -4015,7 +3987,7 @@ int Engine::query_set_create(int btag, const double *xq, long nq)
   OBB obq;
   find_obb(xq, nullptr, (int)nq, obq);
   build_search_structure(*b, obq, q->nelem, q->adt_depth, q->elist, q->ebox, q->adtInt, q->adtReal, q->nodes,
-                         q->ext, q->bins);
+                         q->rcell, q->ext, q->bins);
   qsets_.push_back(q);
   return (int)qsets_.size() - 1;
 }
@@ -4089,6 +4061,7 @@ void Engine::launch_query_sets(const std::vector<QuerySet *> &qs)
     S.xtag = nullptr;
     S.donorId = q.donor.p;
     S.nodes = q.nodes.p;
+    S.rcell = q.rcell.p;
     view_bins(S, q.bins);
     S.g = grid_frame(q.ext);
     memcpy(S.ext, q.ext, sizeof S.ext);
@@ -4119,31 +4092,35 @@ void Engine::search_stats(long long out[10])
 {
   SearchScratch &w = sw_;
   for (int k = 0; k < 10; k++) out[k] = 0;
-  out[9] = w.last_bins ? 1 : 0;
   out[3] = w.last_nqt;
+  out[9] = w.last_bins ? 1 : 0;
   if (w.last_nqt <= 0 || w.last_ns <= 0) return;
-  int c[2] = {0, 0}, h[4] = {0, 0, 0, 0};
+  int used = 0, items = 0, h[4] = {0, 0, 0, 0};
   std::vector<int> nin(w.last_ns);
   DBuf<int> hist;
   hist.reserve(4);
   fill<int>(hist.p, 4, 0, s_);
-  CandOut O{w.cell.p, w.query.p, w.status.p, w.qoff.p, w.qcnt.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
-            w.ovf.p, w.counters.p + 1, w.spill.p, w.spill_chunks, w.counters.p + 2};
+  CandOut O{w.cell.p, w.query.p, w.status.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
+            w.best.p, w.last.p, nullptr};
   k_status_hist<<<1024, 256, 0, s_>>>(O, hist.p);
   DBuf<unsigned long long> ah;
   ah.reserve(1);
   fill<unsigned long long>(ah.p, 1, 0ull, s_);
-  const SearchSet T{(const SearchView *)w.sv.p, w.qb.p, w.nin.p, w.wstart.p, w.qsid.p, w.last_ns, w.last_nqt};
-  k_after_hit<<<nb(w.last_nqt), BS, 0, s_>>>(T, O, ah.p);
+  k_after_hit<<<1024, 256, 0, s_>>>(O, ah.p);
   unsigned long long after = 0;
   TG_HIP(hipMemcpyAsync(&after, ah.p, sizeof after, hipMemcpyDeviceToHost, s_));
-  TG_HIP(hipMemcpyAsync(c, w.counters.p, sizeof c, hipMemcpyDeviceToHost, s_));
+  TG_HIP(hipMemcpyAsync(&used, w.counters.p, sizeof used, hipMemcpyDeviceToHost, s_));
   TG_HIP(hipMemcpyAsync(h, hist.p, sizeof h, hipMemcpyDeviceToHost, s_));
   TG_HIP(hipMemcpyAsync(nin.data(), w.nin.p, sizeof(int) * w.last_ns, hipMemcpyDeviceToHost, s_));
+  if (w.last_bins) {
+    TG_HIP(hipMemcpyAsync(&items, w.segstart.p + w.last_nwave, sizeof items, hipMemcpyDeviceToHost, s_));
+  }
   TG_HIP(hipStreamSynchronize(s_));
-  out[0] = c[0];
-  out[1] = c[1];
-  for (int v : nin) out[2] += v;
+  out[0] = used;
+  long long walking = 0;
+  for (int v : nin) walking += v;
+  out[1] = w.last_bins ? items : (walking + WAVE - 1) / WAVE;
+  out[2] = walking;
   for (int k = 0; k < 4; k++) out[4 + k] = h[k];
   out[8] = (long long)after;
 }
@@ -4230,6 +4207,7 @@ void Engine::launch_search(Block &b)
   S.xtag = b.xtag.p;
   S.donorId = b.donorId.p;
   S.nodes = b.nodes.p;
+  S.rcell = b.rcell.p;
   view_bins(S, b.bins);
   S.g = grid_frame(b.adtExt);
   memcpy(S.ext, b.adtExt, sizeof S.ext);
@@ -4291,6 +4269,7 @@ void Engine::launch_searches()
     S.xtag = b.xtag.p;
     S.donorId = b.donorId.p;
     S.nodes = b.nodes.p;
+    S.rcell = b.rcell.p;
     view_bins(S, b.bins);
     S.g = grid_frame(b.adtExt);
     memcpy(S.ext, b.adtExt, sizeof S.ext);

@@ -70,8 +70,6 @@ def load():
         L.tioga_amd_search_stats.argtypes = [_P]
         L.tioga_amd_search_bins.argtypes = [C.c_int]
         L.tioga_amd_search_bins.restype = C.c_int
-        L.tioga_amd_search_spill.argtypes = [C.c_long]
-        L.tioga_amd_search_spill.restype = C.c_long
         L.tioga_amd_search_stats.restype = C.c_int
         L.tioga_amd_query_set_nelem.argtypes = [C.c_int]
         L.tioga_amd_query_set_nelem.restype = C.c_int
@@ -415,7 +413,7 @@ class Tioga:
         self._check()
         return ext
 
-    def search_tuning(self, morton_bits: Optional[int] = None, kstage: Optional[int] = None,
+    def search_tuning(self, morton_bits: Optional[int] = None, seg: Optional[int] = None,
                       eval_grid: Optional[int] = None, bins: Optional[int] = None) -> Dict[str, int]:
         """Sets the given search parameters; returns all four (bins: elements
         per bin of the candidate bin grids, 0 = the tree walk)."""
@@ -425,18 +423,13 @@ class Tioga:
         self.L.tioga_amd_search_tuning(None, _p(cur))
         self._check()
         want = cur.copy()
-        for i, v in enumerate((morton_bits, kstage, eval_grid)):
+        for i, v in enumerate((morton_bits, seg, eval_grid)):
             if v is not None:
                 want[i] = v
         if not np.array_equal(want, cur):
             self.L.tioga_amd_search_tuning(_p(want), _p(cur))
             self._check()
-        return {"morton_bits": int(cur[0]), "kstage": int(cur[1]), "eval_grid": int(cur[2]), "bins": int(nb)}
-
-    def search_spill(self, chunks: int = 0) -> int:
-        r = self.L.tioga_amd_search_spill(int(chunks))
-        self._check()
-        return int(r)
+        return {"morton_bits": int(cur[0]), "seg": int(cur[1]), "eval_grid": int(cur[2]), "bins": int(nb)}
 
     def search_capacity(self, cap: int = 0) -> int:
         r = self.L.tioga_amd_search_capacity(int(cap))
@@ -451,13 +444,13 @@ class Tioga:
         return ms, n.value, self.L.tioga_amd_last_search_kernel_ms()
 
     def search_stats(self) -> Dict[str, int]:
-        """Counts of the last search launch: candidates listed by the walk,
-        queries with more than the staged candidates, walking queries,
-        queries, and the candidates by test outcome."""
+        """Counts of the last search launch: candidates listed, listing work
+        items, queries that searched a tree, queries, and the candidates by
+        test outcome."""
         out = np.zeros(10, np.int64)
         self.L.tioga_amd_search_stats(_p(out))
         self._check()
-        return {"candidates": int(out[0]), "overflow_queries": int(out[1]), "walking_queries": int(out[2]),
+        return {"candidates": int(out[0]), "listing_items": int(out[1]), "walking_queries": int(out[2]),
                 "queries": int(out[3]), "rejected": int(out[4]), "accepted_near_zero": int(out[5]),
                 "accepted": int(out[6]), "outside_element_box": int(out[7]), "after_first_hit": int(out[8]),
                 "listing": "bins" if out[9] else "walk"}
