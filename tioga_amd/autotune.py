@@ -31,7 +31,7 @@ SPACE = {
     "seg": [64, 128, 256, 512, 1024],
     "eval_grid": [1024, 2048, 4096, 8192],
 }
-DEFAULT = {"morton_bits": 15, "seg": 256, "eval_grid": 8192, "bins": 2}
+DEFAULT = {"morton_bits": 15, "seg": 256, "eval_grid": 8192, "bins": 8}
 
 
 def arch() -> str:

@@ -374,7 +374,7 @@ struct SearchTuning {
   int eval_grid = 8192;
   // candidates listed from bin grids with about this many elements per bin
   // (built with each tree; 0: by the tree walk)
-  int bins = 2;
+  int bins = 8;
 };
 
 struct SearchScratch {
@@ -649,12 +649,14 @@ class Engine {
 
   void build_search_structure(Block &b, const OBB &obq, int &nelem, int &depth, DBuf<int> &elist,
                               DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, DBuf<WalkNode> &nodes,
-                              DBuf<int> &rcell, double ext[6], BinGrid &bins);
+                              DBuf<int> &rcell, double ext[6], BinGrid &bins, long long nq);
   void build_bins(int ntree, const WalkNode *const *nodes, const int *nelem, BinGrid &store, BinGrid *const *views);
   BinGrid tbins_;                      // the bin grids of every batched tree
-  DBuf<int> bcnt_;                     // bin counts of a bin build
+  DBuf<int> bcnt_, beoff_;             // a bin build's entries per element and their prefix
   DBuf<unsigned char> btab_;           // the tree table of a bin build
-  DBuf<unsigned long long> bsums_;     // entries per tree and shift of a bin build
+  DBuf<long long> bsums_;              // entries per tree of a bin build
+  DBuf<unsigned> bkey_, bkey2_;        // a bin build's (bin, element) pairs, sorted by bin
+  DBuf<int> bval_, bval2_;
   void launch_query_sets(const std::vector<QuerySet *> &qs);
   const int *order_sets(const std::vector<MortonSet> &M, size_t tot);
   SearchScratch sw_;

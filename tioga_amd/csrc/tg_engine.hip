@@ -1240,6 +1240,13 @@ struct CandOut {
 // counts give their offsets), so a query's candidates mostly lie together in
 // the list and its containment tests run side by side (one point,
 // neighbouring cells).
+// A tree built for one search gets bins when it has at least one query per
+// BIN_ELEMENTS_PER_QUERY elements: with fewer queries, building the bins
+// costs more than they save the search (measured on a 3.9 M-element tree
+// with one query per element: bins gain 0.4 ms of 37 ms; 65 trees of 1e4 to
+// 7e6 elements: no difference either way).
+constexpr long long BIN_ELEMENTS_PER_QUERY = 2;
+
 constexpr int WBUF = 512;
 constexpr unsigned FIND_GRID = 16384;  // workgroups of k_find (each takes work items in turn)
 struct WaveBuf {
@@ -1300,7 +1307,7 @@ __device__ __forceinline__ int rank_nv(int node, int cellnv) { return node | (ce
 // lane's own walk would, carrying the mask of those lanes, so every lane sees
 // exactly its own preorder walk while node records are wave-uniform scalar
 // loads.
-__global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
+__global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O, int skip_binned)
 {
   __shared__ int bval[WBUF], bqid[WBUF], bkey[WBUF], boff[WAVE];
   const int lane = threadIdx.x;
@@ -1310,6 +1317,7 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O)
   if (wv >= T.wstart[T.ns]) return;
   const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, wv));
   const SearchView S = view_of(T, sid);
+  if (skip_binned && S.boff) return;  // listed from the tree's bins by k_find
   const const_ptr<WalkNode> nodes = (const_ptr<WalkNode>)S.nodes;
   const int qb = T.qbase[sid];
   const int t = (wv - T.wstart[sid]) * WAVE + lane;
@@ -1421,6 +1429,10 @@ __global__ void __launch_bounds__(256) k_find_cost(SearchSet T, int nwave, int s
   }
   const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, w));
   const SearchView S = view_of(T, sid);
+  if (!S.boff) {  // a tree without bins: its waves walk (k_collect)
+    if (lane == 0) nseg[w] = 0;
+    return;
+  }
   const int t = (w - T.wstart[sid]) * WAVE + lane;
   const int i = t < T.nin[sid] ? S.perm[t] : -1;
   unsigned q[3] = {0u, 0u, 0u};
@@ -1780,73 +1792,71 @@ __device__ __forceinline__ void box_bins(const unsigned w[3], int sh, int lo[3],
   hi[2] = (int)((w[2] >> 16) >> sh);
 }
 
-// entries a tree would have at shifts shift .. shift + 3 (sums[4 * tree + k])
-__global__ void k_bin_sizes(int ntree, int total, const BinTree *__restrict__ T, unsigned long long *__restrict__ sums)
-{
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool ok = g < total;
-  const int a = ok ? bin_tree(ntree, T, g) : 0;
-  unsigned long long c[4] = {0, 0, 0, 0};
-  if (ok) {
-    const WalkNode &N = T[a].nodes[g - T[a].nbase];
-    for (int k = 0; k < 4; k++) {
-      const int sh = min(T[a].shift + k, 16);
-      int lo[3], hi[3];
-      box_bins(N.e, sh, lo, hi);
-      c[k] = (unsigned long long)(hi[0] - lo[0] + 1) * (unsigned long long)(hi[1] - lo[1] + 1) *
-             (unsigned long long)(hi[2] - lo[2] + 1);
-    }
-  }
-  // one atomic per wave and shift when the wave's elements share a tree
-  const int a0 = __builtin_amdgcn_readfirstlane(a);
-  if (__ballot(ok && a != a0) == 0ull) {
-    for (int k = 0; k < 4; k++) {
-      unsigned long long v = c[k];
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-      if ((threadIdx.x & 63) == 0 && v) atomicAdd(&sums[4 * a0 + k], v);
-    }
-  } else if (ok) {
-    for (int k = 0; k < 4; k++) atomicAdd(&sums[4 * a + k], c[k]);
-  }
-}
-
-__global__ void k_bin_count(int ntree, int total, const BinTree *__restrict__ T, int *__restrict__ cnt)
+// entries of each element at its tree's shift
+__global__ void k_bin_nb(int ntree, int total, const BinTree *__restrict__ T, int *__restrict__ nbe)
 {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const int a = bin_tree(ntree, T, g);
-  const WalkNode &N = T[a].nodes[g - T[a].nbase];
-  const int sh = T[a].shift, R = 65536 >> sh;
   int lo[3], hi[3];
-  box_bins(N.e, sh, lo, hi);
-  int *c = cnt + T[a].bbase;
-  for (int z = lo[2]; z <= hi[2]; z++)
-    for (int y = lo[1]; y <= hi[1]; y++)
-      for (int x = lo[0]; x <= hi[0]; x++) atomicAdd(&c[(z * R + y) * R + x], 1);
+  box_bins(T[a].nodes[g - T[a].nbase].e, T[a].shift, lo, hi);
+  nbe[g] = (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
 }
 
-// each element into every bin its grid box overlaps; cnt (the counts) is
-// counted down to place the entries, in any order
-__global__ void k_bin_fill(int ntree, int total, const BinTree *__restrict__ T, const int *__restrict__ off,
-                           int *__restrict__ cnt, uint4 *__restrict__ ent)
+// entries of each tree: the element prefix at its ends (tot[a] = eoff[end] - eoff[start])
+__global__ void k_bin_totals(int ntree, const BinTree *__restrict__ T, const int *__restrict__ eoff,
+                             long long *__restrict__ tot)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ntree) return;
+  tot[a] = (long long)eoff[T[a].nbase + T[a].nelem] - eoff[T[a].nbase];
+}
+
+// one (global bin, element) pair per bin of each element, at the element's prefix
+__global__ void k_bin_pairs(int ntree, int total, const BinTree *__restrict__ T, const int *__restrict__ eoff,
+                            unsigned *__restrict__ key, int *__restrict__ val)
 {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const int a = bin_tree(ntree, T, g);
-  const int n = g - T[a].nbase;
-  const WalkNode &N = T[a].nodes[n];
   const int sh = T[a].shift, R = 65536 >> sh;
   int lo[3], hi[3];
-  box_bins(N.e, sh, lo, hi);
-  const uint4 v = make_uint4(N.e[0], N.e[1], N.e[2], (unsigned)rank_nv(n, N.cellnv));
-  const int *o = off + T[a].bbase;
-  int *c = cnt + T[a].bbase;
+  box_bins(T[a].nodes[g - T[a].nbase].e, sh, lo, hi);
+  int o = eoff[g];
   for (int z = lo[2]; z <= hi[2]; z++)
     for (int y = lo[1]; y <= hi[1]; y++)
       for (int x = lo[0]; x <= hi[0]; x++) {
-        const int b = (z * R + y) * R + x;
-        ent[o[b] + atomicSub(&c[b], 1) - 1] = v;
+        key[o] = (unsigned)(T[a].bbase + (z * R + y) * R + x);
+        val[o] = g;
+        o++;
       }
+}
+
+// off[b] = the first sorted pair of bin b (its lower bound), off[nbin] = n
+__global__ void k_bin_offsets(int nbin, int n, const unsigned *__restrict__ key, int *__restrict__ off)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nbin) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (key[m] < (unsigned)b) lo = m + 1;
+    else hi = m;
+  }
+  off[b] = lo;
+}
+
+// entry j of the sorted pairs: its element's packed grid box and rank | nv
+__global__ void k_bin_entries(int ntree, int n, const BinTree *__restrict__ T, const int *__restrict__ val,
+                              uint4 *__restrict__ ent)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int g = val[j];
+  const int a = bin_tree(ntree, T, g);
+  const int e = g - T[a].nbase;
+  const WalkNode &N = T[a].nodes[e];
+  ent[j] = make_uint4(N.e[0], N.e[1], N.e[2], (unsigned)rank_nv(e, N.cellnv));
 }
 
 // coincident queries take their representative's donor (search.C:251-265)
@@ -2554,9 +2564,14 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
   const SearchSet T{(const SearchView *)w.sv.p, w.qb.p, w.nin.p, w.wstart.p, w.qsid.p, ns, nqt};
   long long nwave = 0;  // an upper bound of the walking waves
   for (int k = 0; k < ns; k++) nwave += (S[k].nq + WAVE - 1) / WAVE;
-  // candidates from the bin grids when every tree has one, else by the walk
-  bool bins = w.tune.bins > 0;
-  for (int k = 0; k < ns; k++) bins = bins && S[k].boff != nullptr;
+  // candidates from the bin grids of the trees that have one (k_find), the
+  // other trees' by the walk (k_collect)
+  bool bins = false, walk = false;
+  for (int k = 0; k < ns; k++) {
+    const bool b = w.tune.bins > 0 && S[k].boff != nullptr;
+    bins = bins || b;
+    walk = walk || !b;
+  }
   w.last_bins = bins;
   w.last_nwave = nwave;
   if (bins) {  // the listing's work items: segments of the walking waves' bin lists
@@ -2575,7 +2590,7 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
               w.best.p, w.last.p, tieband ? w.acc.p : nullptr};
     fill<int>(w.counters.p, 1, 0, s);
     if (bins) k_find<<<FIND_GRID, WAVE, 0, s>>>(T, O, (int)nwave, w.tune.seg, w.segstart.p);
-    else k_collect<<<(unsigned)nwave, WAVE, 0, s>>>(T, O);
+    if (walk) k_collect<<<(unsigned)nwave, WAVE, 0, s>>>(T, O, bins ? 1 : 0);
     if (check) {
       int used = 0;
       TG_HIP(hipMemcpyAsync(&used, w.counters.p, sizeof used, hipMemcpyDeviceToHost, s));
@@ -3600,7 +3615,7 @@ void Engine::search(Block &b)
     obb_stepped_ += st;
     prep_acc_[0] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     build_search_structure(b, obq, b.nelem, b.adt_depth, b.elist, b.ebox, b.adtInt, b.adtReal, b.nodes, b.rcell,
-                           b.adtExt, b.bins);
+                           b.adtExt, b.bins, b.nsearch);
   }
   bool gid_mode = true;
   for (Block *o : blocks_) gid_mode = gid_mode && (o->unode_gid != nullptr);
@@ -3788,7 +3803,7 @@ void Engine::prepare_searches(const std::vector<int> &act)
     for (int a = 0; a < na; a++) {
       Block &b = *blocks_[act[a]];
       np[a] = b.nodes.p;
-      ne[a] = b.nelem;
+      ne[a] = BIN_ELEMENTS_PER_QUERY * b.nsearch >= b.nelem ? b.nelem : 0;  // as build_search_structure decides
       bv[a] = &b.bins;
     }
     build_bins(na, np.data(), ne.data(), tbins_, bv.data());
@@ -3836,7 +3851,7 @@ void Engine::build_resident_tree(int btag, ResidentTree &t)
   Block *b = find(btag);
   if (!b || !b->topo_uploaded || b->cellRes.p == nullptr) throw std::runtime_error("tioga_amd: preprocess first");
   build_search_structure(*b, b->obb, t.nelem, t.depth, t.elist, t.ebox, t.adtInt, t.adtReal, t.nodes, t.rcell,
-                         t.ext, t.bins);
+                         t.ext, t.bins, -1);
   if (t.nelem != b->ncells) throw std::runtime_error("tioga_amd: resident tree misses cells of its block");
 }
 
@@ -3873,7 +3888,10 @@ void Engine::attach_resident(Block &b)
 // built into store's arrays; views[a] gets tree a's grid (views of store's
 // arrays unless views[a] is store itself).  A tree's bins per axis are the
 // power of two nearest cbrt(nelem / tune.bins), made coarser (up to 3 times)
-// while its entries exceed 32 per element.  tune.bins == 0 builds none.
+// while its entries exceed 32 per element.  The (bin, element) pairs of all
+// trees are sorted by bin (a stable radix sort, so each bin lists its
+// elements in node order) and each bin's start found by binary search: no
+// contended atomics.  tune.bins == 0 builds none.
 void Engine::build_bins(int ntree, const WalkNode *const *nodes, const int *nelem, BinGrid &store,
                         BinGrid *const *views)
 {
@@ -3889,37 +3907,54 @@ void Engine::build_bins(int ntree, const WalkNode *const *nodes, const int *nele
     total += n;
   }
   if (per <= 0 || total == 0) return;
-  if (total > 0x7fffffffLL) throw std::runtime_error("tioga_amd: too many tree elements for one bin batch");
+  if (total >= 0x7fffffffLL) throw std::runtime_error("tioga_amd: too many tree elements for one bin batch");
   btab_.reserve(sizeof(BinTree) * ntree);
-  bsums_.reserve(4 * (size_t)ntree);
-  TG_HIP(hipMemcpyAsync(btab_.p, T.data(), sizeof(BinTree) * ntree, hipMemcpyHostToDevice, s_));
-  fill<unsigned long long>(bsums_.p, 4 * (size_t)ntree, 0ull, s_);
-  k_bin_sizes<<<nb(total), BS, 0, s_>>>(ntree, (int)total, (const BinTree *)btab_.p, bsums_.p);
-  std::vector<unsigned long long> sums(4 * (size_t)ntree);
-  TG_HIP(hipMemcpyAsync(sums.data(), bsums_.p, sizeof(unsigned long long) * sums.size(), hipMemcpyDeviceToHost, s_));
-  sync();
-  long long nbin = 0, nent = 0;
+  bsums_.reserve(ntree);
+  bcnt_.reserve((size_t)total + 1);
+  beoff_.reserve((size_t)total + 1);
+  std::vector<long long> tot(ntree);
+  long long nent = 0;
+  for (int round = 0;; round++) {  // coarser bins for the trees whose entries exceed the cap
+    TG_HIP(hipMemcpyAsync(btab_.p, T.data(), sizeof(BinTree) * ntree, hipMemcpyHostToDevice, s_));
+    fill<int>(bcnt_.p + total, 1, 0, s_);
+    k_bin_nb<<<nb(total), BS, 0, s_>>>(ntree, (int)total, (const BinTree *)btab_.p, bcnt_.p);
+    exclusive_sum(bcnt_.p, beoff_.p, (int)total + 1, s_);
+    k_bin_totals<<<nb(ntree), BS, 0, s_>>>(ntree, (const BinTree *)btab_.p, beoff_.p, (long long *)bsums_.p);
+    TG_HIP(hipMemcpyAsync(tot.data(), bsums_.p, sizeof(long long) * ntree, hipMemcpyDeviceToHost, s_));
+    sync();
+    bool again = false;
+    nent = 0;
+    for (int a = 0; a < ntree; a++) {
+      nent += tot[a];
+      const long long cap = 32LL * T[a].nelem + 65536;
+      if (tot[a] > cap && T[a].shift < 16 && round < 3) {
+        T[a].shift++;
+        again = true;
+      }
+    }
+    if (!again) break;
+  }
+  long long nbin = 0;
   for (int a = 0; a < ntree; a++) {
-    const unsigned long long cap = 32ull * (unsigned long long)T[a].nelem + 65536ull;
-    int k = 0;
-    while (k < 3 && (sums[4 * a + k] > cap || T[a].shift + k > 16)) k++;
-    T[a].shift = std::min(T[a].shift + k, 16);
     const long long R = 65536 >> T[a].shift;
     T[a].bbase = nbin;
     nbin += R * R * R;
-    nent += (long long)sums[4 * a + k];
   }
-  if (nbin + 1 > 0x7fffffffLL || nent > 0x7fffffffLL)
-    throw std::runtime_error("tioga_amd: bin grids too large");
-  bcnt_.reserve((size_t)nbin + 1);
-  store.off.reserve_own((size_t)nbin + 1);
-  store.ent.reserve_own((size_t)std::max<long long>(nent, 1));
-  fill<int>(bcnt_.p, (size_t)nbin + 1, 0, s_);
+  if (nbin + 1 > 0x7fffffffLL || nent >= 0x7fffffffLL) throw std::runtime_error("tioga_amd: bin grids too large");
   TG_HIP(hipMemcpyAsync(btab_.p, T.data(), sizeof(BinTree) * ntree, hipMemcpyHostToDevice, s_));
-  k_bin_count<<<nb(total), BS, 0, s_>>>(ntree, (int)total, (const BinTree *)btab_.p, bcnt_.p);
-  exclusive_sum(bcnt_.p, store.off.p, (int)(nbin + 1), s_);
-  k_bin_fill<<<nb(total), BS, 0, s_>>>(ntree, (int)total, (const BinTree *)btab_.p, store.off.p, bcnt_.p,
-                                      store.ent.p);
+  const size_t ne = (size_t)std::max<long long>(nent, 1);
+  bkey_.reserve(ne);
+  bkey2_.reserve(ne);
+  bval_.reserve(ne);
+  bval2_.reserve(ne);
+  store.off.reserve_own((size_t)nbin + 1);
+  store.ent.reserve_own(ne);
+  k_bin_pairs<<<nb(total), BS, 0, s_>>>(ntree, (int)total, (const BinTree *)btab_.p, beoff_.p, bkey_.p, bval_.p);
+  int bits = 1;
+  while ((1LL << bits) <= nbin) bits++;
+  sort_pairs(bkey_.p, bkey2_.p, bval_.p, bval2_.p, (int)nent, bits, s_);
+  k_bin_offsets<<<nb(nbin + 1), BS, 0, s_>>>((int)nbin, (int)nent, bkey2_.p, store.off.p);
+  k_bin_entries<<<nb(nent), BS, 0, s_>>>(ntree, (int)nent, (const BinTree *)btab_.p, bval2_.p, store.ent.p);
   for (int a = 0; a < ntree; a++) {
     BinGrid &v = *views[a];
     if (T[a].nelem == 0) continue;
@@ -3937,7 +3972,8 @@ void Engine::build_bins(int ntree, const WalkNode *const *nodes, const int *nele
 // as a LIFO list (descending cell index) -> element boxes -> ADT
 void Engine::build_search_structure(Block &b, const OBB &obq, int &nelem, int &depth, DBuf<int> &elist,
                                     DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal,
-                                    DBuf<WalkNode> &nodes, DBuf<int> &rcell, double ext[6], BinGrid &bins)
+                                    DBuf<WalkNode> &nodes, DBuf<int> &rcell, double ext[6], BinGrid &bins,
+                                    long long nq)
 {
   auto now = [] { return std::chrono::steady_clock::now(); };
   auto ms = [](auto a, auto c) { return std::chrono::duration<double, std::milli>(c - a).count(); };
@@ -3962,9 +3998,12 @@ void Engine::build_search_structure(Block &b, const OBB &obq, int &nelem, int &d
   if (ne)
     k_adt_nodes<<<nb(ne), BS, 0, s_>>>(ne, b.ti, grid_frame(ext), adtInt.p, adtReal.p, ebox.p, elist.p, nodes.p,
                                        rcell.p);
+  // bins when the tree serves many searches (nq < 0), or one search with at
+  // least one query per BIN_ELEMENTS_PER_QUERY elements; else its queries walk it
   const WalkNode *np = nodes.p;
   BinGrid *bp = &bins;
-  build_bins(1, &np, &ne, bins, &bp);
+  const int nbin_el = (nq < 0 || BIN_ELEMENTS_PER_QUERY * nq >= ne) ? ne : 0;
+  build_bins(1, &np, &nbin_el, bins, &bp);
   sync();
   auto t3 = now();
   prep_acc_[1] += ms(t1, t2);
@@ -3987,7 +4026,7 @@ int Engine::query_set_create(int btag, const double *xq, long nq)
   OBB obq;
   find_obb(xq, nullptr, (int)nq, obq);
   build_search_structure(*b, obq, q->nelem, q->adt_depth, q->elist, q->ebox, q->adtInt, q->adtReal, q->nodes,
-                         q->rcell, q->ext, q->bins);
+                         q->rcell, q->ext, q->bins, -1);
   qsets_.push_back(q);
   return (int)qsets_.size() - 1;
 }
