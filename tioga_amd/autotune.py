@@ -27,11 +27,11 @@ TABLE = os.path.join(HERE, "search_tuning.json")
 
 SPACE = {
     "bins": [0, 1, 2, 4, 8],
-    "morton_bits": [0, 9, 15, 21, 30],
+    "morton_bits": [0, 9, 12, 13, 14, 15, 21, 30],
     "seg": [64, 128, 256, 512, 1024],
     "eval_grid": [1024, 2048, 4096, 8192],
 }
-DEFAULT = {"morton_bits": 15, "seg": 256, "eval_grid": 8192, "bins": 8}
+DEFAULT = {"morton_bits": 14, "seg": 256, "eval_grid": 8192, "bins": 8}
 
 
 def arch() -> str:
@@ -52,10 +52,11 @@ def arch() -> str:
 
 
 def _measure(tg, params: Dict[str, int], steps: int) -> float:
+    """Device time of one whole pass (ordering, listing, tests, selection)."""
     tg.search_tuning(**params)
     tg.time_query_sets(1)  # warm-up at these parameters
-    _, _, kern_ms = tg.time_query_sets(steps)
-    return kern_ms
+    pass_ms, _, _ = tg.time_query_sets(steps)
+    return pass_ms
 
 
 def tune(tg, steps: int = 5, rounds: int = 2, log=None) -> Tuple[Dict[str, int], List[dict]]:

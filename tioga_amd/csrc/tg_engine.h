@@ -369,7 +369,7 @@ struct StepScratch {
 // bin listing, workgroups of the containment-test kernels, and the bin
 // density (0: candidates listed by the tree walk).
 struct SearchTuning {
-  int morton_bits = 15;
+  int morton_bits = 14;
   int seg = 256;  // bin-list entries per lane in one work item of the listing
   int eval_grid = 8192;
   // candidates listed from bin grids with about this many elements per bin
