@@ -649,7 +649,8 @@ def main() -> int:
                        "GBps": round(bdu / (du_ms / 1000) / 1e9, 2) if du_ms > 0 else None,
                        "frac": round(bdu / (du_ms / 1000) / 1e9 / HBM_PEAK_GBPS, 4) if du_ms > 0 else None,
                        "kernel": "k_update<0> (fused gather and scatter)",
-                       "traffic": profiled_traffic("dataupdate_config2.md", "traffic_bytes_per_update")},
+                       "traffic": profiled_traffic("dataupdate_config2.md", "traffic_bytes_per_update")
+                       if args.workload == "config2" else None},
         "reduce_fringes": rf,
         "amr_gather": amr,
         "batching": batching,

@@ -250,6 +250,10 @@ void run_amr(tioga *tg, std::vector<Block> &blocks, Amr &A, int nvar, int itype,
 
 int main(int argc, char **argv)
 {
+  // REF_DRIVER_TIMES set: wall times of profile, performConnectivity and
+  // dataUpdate to stderr (the timers build adds the reference's own per-phase
+  // lines, tioga.C:715-735)
+  const bool times = getenv("REF_DRIVER_TIMES") != nullptr;
   if (argc < 3) {
     fprintf(stderr, "usage: ref_driver case.bin out.bin\n");
     return 2;
@@ -338,10 +342,13 @@ int main(int argc, char **argv)
       }
       w.prefix = "s" + std::to_string(s) + ".";
     }
+    const double tp0 = MPI_Wtime();
     tg->profile();
+    const double tprof = MPI_Wtime() - tp0;
     double t0 = MPI_Wtime();
     tg->performConnectivity();
     double tconn = MPI_Wtime() - t0;
+    if (times) fprintf(stderr, "ref_driver step %d: profile %.6f s, performConnectivity %.6f s\n", s, tprof, tconn);
     dump(tg, blocks, w, tconn, has_gids != 0);
     // dataUpdate on a fresh copy of q so every step starts from the same field
     std::vector<std::vector<double>> qs(blocks.size());
@@ -349,7 +356,9 @@ int main(int argc, char **argv)
       qs[b] = blocks[b].q;
       tg->registerSolution(blocks[b].tag, qs[b].data());
     }
+    const double tu0 = MPI_Wtime();
     tg->dataUpdate(nvar, itype);
+    if (times) fprintf(stderr, "ref_driver step %d: dataUpdate %.6f s\n", s, MPI_Wtime() - tu0);
     for (size_t b = 0; b < blocks.size(); b++)
       w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
     if (reduce) {
