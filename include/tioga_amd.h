@@ -179,6 +179,22 @@ int tioga_amd_serve_open2(int bg_tag, int nslots, double budget_gb, int flags);
 int tioga_amd_serve_wait2(long id, int *shell_iblank, int nshell, int *nbg, int *bg_nodes, int *bg_vals, int cap,
                           double *times, long long *stats);
 
+/* Virtual ranks: one process holds the blocks of every rank of an MPI run
+   (parallelComm.C:96-170 is the exchange between them).  Per-block calls
+   that follow (registration, resolutions, cell iblanks, solutions, donor
+   count and info, receptor info) act on the blocks of this rank, each named
+   by its mesh tag as the reference's rank names its own blocks
+   (tag_iblk_map, tioga.h:78); a mesh tag may sit on several ranks, a
+   partitioned mesh.  Collective calls (preprocess, performconnectivity,
+   dataupdate, reduce_fringes) run every rank together.  Results per rank
+   equal the reference run with that many MPI ranks: every array, the rank
+   and per-rank block labels in donor info and the search arrays, and the
+   packets of tioga_amd_packet.  Rank 0, the default, alone is the
+   single-rank reference; tioga_init_ resets it.  Each rank holds at most
+   one block per mesh tag, as in the reference. */
+void tioga_amd_set_rank(int rank);
+int tioga_amd_rank(void);
+
 /* Batched form (on != 0, the default): the stages of performconnectivity
    run for all blocks together (one ADT build, one coincident-point dedupe
    for every block's query set, ...); 0 runs them block by block.  The

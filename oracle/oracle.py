@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import re
 import struct
 import subprocess
 from typing import Dict, List, Optional, Sequence
@@ -305,12 +306,14 @@ class Oracle:
 
 def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptype: int = 0,
                isym: int = 3, nfringe: int = 1, mexclude: int = 3,
-               moves: Sequence[Sequence[Optional[np.ndarray]]] = (), amr=None, reduce: bool = False) -> None:
+               moves: Sequence[Sequence[Optional[np.ndarray]]] = (), amr=None, reduce: bool = False,
+               ranks: Optional[Sequence[int]] = None) -> None:
     """amr = (AmrGrid, patch fields per local patch) selects the AMR gather
     mode of the driver (format version 2); reduce runs reduce_fringes after
-    each step's connectivity (format version 3)."""
+    each step's connectivity (format version 3); ranks (the MPI rank of each
+    block) selects the multi-rank mode (format version 5)."""
     gids = any(b.cell_gid is not None or b.node_gid is not None for b in blocks)
-    version = 4 if gids else (3 if reduce else (1 if amr is None else 2))
+    version = 5 if ranks is not None else (4 if gids else (3 if reduce else (1 if amr is None else 2)))
     with open(path, "wb") as f:
         w = lambda fmt, *v: f.write(struct.pack("<" + fmt, *v))
         w("9i", 0x41474954, version, len(blocks), isym, nfringe, mexclude, nvar, interptype,
@@ -345,11 +348,14 @@ def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptyp
         if version >= 3:
             w("i", int(reduce))
         if version >= 4:
-            w("i", 1)
-            for b in blocks:
+            w("i", int(gids))
+            for b in blocks if gids else ():
                 b = b.normalized()
                 f.write(b.cell_gid.tobytes())
                 f.write(b.node_gid.tobytes())
+        if version >= 5:
+            w("i", 1)
+            f.write(np.asarray(ranks, np.int32).tobytes())
         for step in moves:
             for xb in step:
                 if xb is None:
@@ -378,15 +384,71 @@ def read_result(path: str) -> Dict[str, np.ndarray]:
     return out
 
 
+MPIEXEC = "/opt/conda/bin/mpiexec"
+
+
 def run_reference(case_path: str, out_path: str, timeout: float = 600.0,
-                  cwd: Optional[str] = None) -> Dict[str, np.ndarray]:
+                  cwd: Optional[str] = None, nranks: int = 0, gids: Optional[bool] = None):
     """Run the compiled reference (oracle/_ref/ref_driver) on one rank
-    (in cwd: the reference writes a cartGrid.dat there for AMR cases)."""
+    (in cwd: the reference writes a cartGrid.dat there for AMR cases).
+
+    nranks > 0 (a version-5 case) runs it under mpiexec on that many ranks
+    and returns (results, packets): every rank's arrays merged, each name
+    carrying its rank as "r<k>." after the step and reduce_fringes prefixes,
+    and packets[k] = the sendRecvPackets calls of rank k (parse_packets).
+    gids selects the global-id build (default: a version 4 case)."""
     env = dict(os.environ)
     env.setdefault("HYDRA_LAUNCHER", "fork")
     with open(case_path, "rb") as f:
         version = struct.unpack("<2i", f.read(8))[1]
-    drv = REF_DRIVER_GID if version >= 4 else REF_DRIVER
-    subprocess.run([drv, case_path, out_path], check=True, timeout=timeout, env=env, cwd=cwd,
-                   stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
-    return read_result(out_path)
+    if gids is None:
+        gids = version == 4
+    drv = REF_DRIVER_GID if gids else REF_DRIVER
+    if nranks <= 0:
+        subprocess.run([drv, case_path, out_path], check=True, timeout=timeout, env=env, cwd=cwd,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+        return read_result(out_path)
+    subprocess.run([MPIEXEC, "-n", str(nranks), drv, case_path, out_path], check=True, timeout=timeout,
+                   env=env, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    res: Dict[str, np.ndarray] = {}
+    packets = []
+    for k in range(nranks):
+        for name, a in read_result(f"{out_path}.r{k}").items():
+            res[re.sub(r"^((?:s\d+\.)?(?:rf\.)?)", rf"\1r{k}.", name)] = a
+        packets.append(parse_packets(f"{out_path}.r{k}.pk"))
+    return res, packets
+
+
+def parse_packets(path: str):
+    """The packet log of one rank (ref_driver.cpp) as a list with one entry
+    per parallelComm::sendRecvPackets call, each {dest rank: (int32 payload,
+    float64 payload, nints, nreals)}: the counts come from the call's first
+    group of sends, the payloads from its second, in destination order."""
+    with open(path, "rb") as f:
+        data = f.read()
+    groups, cur, o = [], [], 0
+    while o < len(data):
+        (kind,) = struct.unpack_from("<i", data, o)
+        o += 4
+        if kind == 2:
+            groups.append(cur)
+            cur = []
+            continue
+        dest, tag, dt, n = struct.unpack_from("<4i", data, o)
+        o += 16
+        nb = n * (8 if dt == 1 else 4)
+        cur.append((dest, tag, np.frombuffer(data[o:o + nb], np.float64 if dt == 1 else np.int32).copy()))
+        o += nb
+    if cur or len(groups) % 2:
+        raise ValueError(f"{path}: unpaired send groups")
+    calls = []
+    for g in range(0, len(groups), 2):
+        counts, payload = groups[g], iter(groups[g + 1])
+        call = {}
+        for dest, _tag, c in counts:
+            ni, nr = int(c[0]), int(c[1])
+            ints = next(payload)[2] if ni > 0 else np.zeros(0, np.int32)
+            reals = next(payload)[2] if nr > 0 else np.zeros(0, np.float64)
+            call[int(dest)] = (ints, reals, ni, nr)
+        calls.append(call)
+    return calls

@@ -2,7 +2,7 @@
 //
 // Runs the compiled reference TIOGA (oracle/_ref/libtioga_ref.so, built from
 // the sources under /root/reference/src by oracle/Makefile) on one MPI rank
-// over a case file, and dumps every array the parity tests compare: the
+// (or, for version 5 cases, on several) over a case file, and dumps every array the parity tests compare: the
 // user-visible outputs (iblank, iblank_cell, donor info, updated q) and the
 // internal per-block state (search arrays, donorId, xtag, ADT arrays,
 // resolutions, hole maps).  Internal members are reached by compiling the
@@ -36,8 +36,24 @@
 //              "receptor_info" (tioga::getReceptorInfo, tioga.C:572-671).
 //              Cases with node ids need the gid build of this driver
 //              (-DTIOGA_HAS_NODEGID: the dedupe of search.C:48-74).
+//   version 5 adds, after the node ids: int32 has_ranks; if set:
+//              int32 rank[nblocks], the MPI rank that registers each block.
+//              Run under mpiexec with that many ranks: every rank registers
+//              its own blocks in file order (a mesh tag may sit on several
+//              ranks: a partitioned mesh), runs the same call sequence, and
+//              writes its results to out.bin.r<rank> (only the blocks it
+//              owns, under their case-file index) and the packets it sends
+//              to out.bin.r<rank>.pk (below).
 // Result file: records {int32 namelen, name, int32 dtype(0=i32,1=f64),
 //              int64 count, payload}.  Step s > 0 prefixes names with "s<s>.".
+// Packet log (multi-rank mode): every MPI_Isend the reference makes, in call
+//              order, as {int32 1, dest, tag, dtype(0=int,1=double), count,
+//              payload}, and every MPI_Waitall as {int32 2}.  Each
+//              parallelComm::sendRecvPackets call (parallelComm.C:96-170) is
+//              two such groups: the per-destination (nints, nreals) counts,
+//              then the int and double payloads.  Captured through the MPI
+//              profiling interface: these two functions are defined here and
+//              forward to PMPI_Isend / PMPI_Waitall.
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -52,6 +68,31 @@
 #undef private
 
 using namespace TIOGA;
+
+namespace {
+FILE *g_pk = nullptr;  // packet log of this rank (multi-rank mode)
+}
+
+extern "C" int MPI_Isend(const void *buf, int count, MPI_Datatype dt, int dest, int tag, MPI_Comm comm,
+                         MPI_Request *req)
+{
+  if (g_pk) {
+    const int dtype = dt == MPI_DOUBLE ? 1 : (dt == MPI_INT ? 0 : -1);
+    const int32_t h[5] = {1, dest, tag, dtype, count};
+    fwrite(h, 4, 5, g_pk);
+    if (count > 0 && dtype >= 0) fwrite(buf, dtype ? 8 : 4, count, g_pk);
+  }
+  return PMPI_Isend(buf, count, dt, dest, tag, comm, req);
+}
+
+extern "C" int MPI_Waitall(int n, MPI_Request *reqs, MPI_Status *st)
+{
+  if (g_pk) {
+    const int32_t h = 2;
+    fwrite(&h, 4, 1, g_pk);
+  }
+  return PMPI_Waitall(n, reqs, st);
+}
 
 namespace {
 
@@ -85,7 +126,7 @@ struct Writer {
 };
 
 struct Block {
-  int tag, nnodes, nwbc, nobc, ntypes, has_res, ncells;
+  int tag, nnodes, nwbc, nobc, ntypes, has_res, ncells, rank = 0;
   std::vector<uint64_t> cell_gid, node_gid;
   std::vector<double> x, nres, cres, q;
   std::vector<int> wbc, obc, iblank, iblank_cell;
@@ -104,6 +145,7 @@ void dump(tioga *tg, std::vector<Block> &blocks, Writer &w, double tconn, bool g
   }
   for (size_t b = 0; b < blocks.size(); b++) {
     Block &B = blocks[b];
+    if (B.rank != tg->myid) continue;
     auto it = tg->tag_iblk_map.find(B.tag);
     MeshBlock *mb = tg->mblocks[it->second].get();
     std::string p = "b" + std::to_string(b) + ".";
@@ -263,7 +305,7 @@ int main(int argc, char **argv)
   if (!r.f) { perror(argv[1]); return 2; }
   if (r.get<int32_t>() != 0x41474954) { fprintf(stderr, "bad case\n"); return 2; }
   const int version = r.get<int32_t>();
-  if (version < 1 || version > 4) { fprintf(stderr, "bad case version\n"); return 2; }
+  if (version < 1 || version > 5) { fprintf(stderr, "bad case version\n"); return 2; }
   int nblocks = r.get<int32_t>(), isym = r.get<int32_t>(), nfringe = r.get<int32_t>();
   int mexclude = r.get<int32_t>(), nvar = r.get<int32_t>(), itype = r.get<int32_t>();
   int nsteps = r.get<int32_t>();
@@ -313,13 +355,28 @@ int main(int argc, char **argv)
       r.arr(B.cell_gid, B.ncells);
       r.arr(B.node_gid, B.nnodes);
     }
-  Writer w{fopen(argv[2], "wb"), ""};
+  const int has_ranks = version >= 5 ? r.get<int32_t>() : 0;
+  if (has_ranks)
+    for (auto &B : blocks) B.rank = r.get<int32_t>();
+  int myrank = 0, nprocs = 1;
+  MPI_Comm_rank(MPI_COMM_WORLD, &myrank);
+  MPI_Comm_size(MPI_COMM_WORLD, &nprocs);
+  if (!has_ranks && nprocs != 1) { fprintf(stderr, "ref_driver: a case without ranks runs on one rank\n"); return 2; }
+  for (auto &B : blocks)
+    if (B.rank < 0 || B.rank >= nprocs) { fprintf(stderr, "ref_driver: block rank outside the run\n"); return 2; }
+  std::string outp = argv[2];
+  if (has_ranks) {
+    outp += ".r" + std::to_string(myrank);
+    g_pk = fopen((outp + ".pk").c_str(), "wb");
+  }
+  Writer w{fopen(outp.c_str(), "wb"), ""};
   tioga *tg = new tioga;
-  tg->setCommunicator(MPI_COMM_WORLD, 0, 1);
+  tg->setCommunicator(MPI_COMM_WORLD, myrank, nprocs);
   tg->setSymmetry(isym);
   tg->setNfringe(&nfringe);
   tg->setMexclude(&mexclude);
   for (auto &B : blocks) {
+    if (B.rank != myrank) continue;
     tg->registerGridData(B.tag, B.nnodes, B.x.data(), B.iblank.data(), B.nwbc, B.nobc, B.wbc.data(),
                          B.obc.data(), B.ntypes, B.nv.data(), B.nc.data(), B.vconn.data(),
                          has_gids ? B.cell_gid.data() : NULL, has_gids ? B.node_gid.data() : NULL);
@@ -327,6 +384,7 @@ int main(int argc, char **argv)
     tg->set_cell_iblank(B.tag, B.iblank_cell.data());
   }
   if (has_amr) {
+    if (has_ranks) { fprintf(stderr, "ref_driver: AMR cases run on one rank\n"); return 2; }
     run_amr(tg, blocks, amr, nvar, itype, w);
     nsteps = -1;
   }
@@ -353,6 +411,7 @@ int main(int argc, char **argv)
     // dataUpdate on a fresh copy of q so every step starts from the same field
     std::vector<std::vector<double>> qs(blocks.size());
     for (size_t b = 0; b < blocks.size(); b++) {
+      if (blocks[b].rank != myrank) continue;
       qs[b] = blocks[b].q;
       tg->registerSolution(blocks[b].tag, qs[b].data());
     }
@@ -360,24 +419,27 @@ int main(int argc, char **argv)
     tg->dataUpdate(nvar, itype);
     if (times) fprintf(stderr, "ref_driver step %d: dataUpdate %.6f s\n", s, MPI_Wtime() - tu0);
     for (size_t b = 0; b < blocks.size(); b++)
-      w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
+      if (blocks[b].rank == myrank) w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
     if (reduce) {
       const std::string p0 = w.prefix;
       tg->reduce_fringes();
       w.prefix = p0 + "rf.";
       dump(tg, blocks, w, 0.0, has_gids != 0);
       for (size_t b = 0; b < blocks.size(); b++) {
+        if (blocks[b].rank != myrank) continue;
         qs[b] = blocks[b].q;
         tg->registerSolution(blocks[b].tag, qs[b].data());
       }
       tg->dataUpdate(nvar, itype);
       for (size_t b = 0; b < blocks.size(); b++)
-        w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
+        if (blocks[b].rank == myrank) w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
       w.prefix = p0;
     }
   }
   fclose(w.f);
   fclose(r.f);
+  if (g_pk) fclose(g_pk);
+  g_pk = nullptr;
   delete tg;
   MPI_Finalize();
   return 0;

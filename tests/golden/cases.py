@@ -27,6 +27,7 @@ class Case:
     mexclude: int = 3
     moves: List[List[Optional[np.ndarray]]] = field(default_factory=list)
     reduce: bool = False  # tioga::reduce_fringes after each connectivity
+    ranks: Optional[List[int]] = None  # MPI rank of each block (multi-rank cases)
 
     @property
     def qs(self) -> List[np.ndarray]:
@@ -113,6 +114,34 @@ def split_box(n: int, cut: int, tags=(2, 3)):
     return out
 
 
+def ranked_blocks(three: bool = False):
+    """Partitioned meshes on several MPI ranks, each rank holding at most
+    one block per mesh tag.  Two ranks: a lat-long shell (tag 1) on rank 0
+    and a 12^3 box (tag 2) cut at cell column 5 into a partition on each rank
+    (the cut nodes are sent by both, coincident queries of one mesh).  Three
+    ranks: an off-centre shell (tag 1) cut radially into an inner partition
+    with the wall (rank 2) and an outer one without wall nodes (rank 0), so
+    the wall bounds of tag 1 are reduced with the zeros the wall-less
+    partition reports (MeshBlock.C:640-647, holeMap.C:110-112); a second
+    shell (tag 3) on rank 1; the box cut in three, one partition per rank.
+    The blocks are listed out of rank order (each rank registers its own in
+    list order)."""
+    if not three:
+        shell = W.latlong_shell(12, 8, 5, 0.2, 0.45, 1.1, tag=1, center=(0.02, 0.05, 0.01))
+        box = W.cart_box(12, tag=2)
+        ci = np.arange(box.ncells) % 12
+        parts = [W.partition(box, ci < 5), W.partition(box, ci >= 5)]
+        return [parts[1], shell, parts[0]], [1, 0, 0]
+    s1 = W.latlong_shell(12, 8, 6, 0.12, 0.3, 1.1, tag=1, center=(0.42, 0.1, -0.05))
+    kr = np.arange(s1.ncells) // (12 * 8)  # radial layer of each cell
+    s3 = W.latlong_shell(10, 6, 4, 0.1, 0.25, 1.05, tag=3, center=(-0.35, -0.2, 0.1))
+    box = W.cart_box(14, tag=2)
+    ci = np.arange(box.ncells) % 14
+    b0, b1, b2 = (W.partition(box, m) for m in (ci < 4, (ci >= 4) & (ci < 9), ci >= 9))
+    inner, outer = W.partition(s1, kr < 3), W.partition(s1, kr >= 3)
+    return [b1, outer, s3, b0, inner, b2], [1, 0, 1, 0, 2, 2]
+
+
 def build(name: str) -> Case:
     if name == "tiny":
         return Case(name, W.tiny(), nvar=5)
@@ -170,6 +199,19 @@ def build(name: str) -> Case:
     if name == "seam_gid_rf":
         blocks = [W.latlong_shell(12, 8, 5, 0.2, 0.45, 1.1, tag=1, center=(0.02, 0.05, 0.01))] + split_box(12, 6)
         return Case(name, _position_gids(blocks), nvar=2, nfringe=2, reduce=True)
+    # several MPI ranks (the reference under mpiexec; the engine's virtual ranks)
+    if name == "ranks2":
+        blocks, ranks = ranked_blocks()
+        return Case(name, blocks, nvar=3, ranks=ranks)
+    if name == "ranks3":
+        blocks, ranks = ranked_blocks(three=True)
+        return Case(name, blocks, nvar=2, ranks=ranks)
+    if name == "ranks3_rf":
+        blocks, ranks = ranked_blocks(three=True)
+        return Case(name, blocks, nvar=2, nfringe=2, reduce=True, ranks=ranks)
+    if name == "ranks3_gid":
+        blocks, ranks = ranked_blocks(three=True)
+        return Case(name, _position_gids(blocks), nvar=2, ranks=ranks)
     raise KeyError(name)
 
 
@@ -177,6 +219,9 @@ SMALL_CASES = ["tiny", "tiny_isym0", "tiny_isym1", "tiny_isym2", "mixed", "userr
 REDUCE_CASES = ["tiny_rf", "isym0_rf", "mixed_rf", "three_rf", "moving_rf", "nested_rf", "nested3_rf"]
 GID_CASES = ["tiny_gid", "seam_gid", "seam_gid_rf"]
 ALL_CASES = SMALL_CASES + ["config1"] + REDUCE_CASES + GID_CASES
+# several ranks: pinned by the reference run under mpiexec (the oracle
+# restates the single-rank path only)
+RANK_CASES = ["ranks2", "ranks3", "ranks3_rf", "ranks3_gid"]
 
 
 # ---------------------------------------------------------------------------

@@ -49,12 +49,12 @@ def run_oracle_case(case):
     return out
 
 
-def compare_to_golden(name: str, got: dict, skip=()):
+def compare_to_golden(name: str, got: dict, skip=(), skip_prefix=()):
     """Returns a list of mismatch descriptions (empty when bit-exact)."""
     meta, full = load_golden(name)
     bad = []
     for k, m in meta.items():
-        if any(k.endswith(s) for s in skip):
+        if any(k.endswith(s) for s in skip) or any(k.startswith(s) for s in skip_prefix):
             continue
         if k not in got:
             bad.append(f"{k}: missing")
@@ -82,8 +82,9 @@ def run_engine_case(case, batched: bool = True):
     try:
         t.set_batching(batched)
         t.set_params(case.isym, case.nfringe, case.mexclude)
-        for b in case.blocks:
-            t.register_grid(b)
+        ranks = getattr(case, "ranks", None) or [0] * len(case.blocks)
+        for b, r in zip(case.blocks, ranks):
+            t.register_grid(b, rank=r)
         for s in range(len(case.moves) + 1):
             if s > 0:
                 for bi, xb in enumerate(case.moves[s - 1]):

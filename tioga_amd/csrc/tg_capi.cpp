@@ -516,6 +516,21 @@ void tioga_amd_serve_close(void)
   });
 }
 
+void tioga_amd_set_rank(int rank)
+{
+  guarded("tioga_amd_set_rank", [&] {
+    if (rank < 0) throw std::runtime_error("negative rank");
+    tg_or_throw().engine()->cur_rank = rank;
+  });
+}
+
+int tioga_amd_rank(void)
+{
+  int r = 0;
+  guarded("tioga_amd_rank", [&] { r = tg_or_throw().engine()->cur_rank; });
+  return r;
+}
+
 void tioga_amd_set_batching(int on)
 {
   guarded("tioga_amd_set_batching", [&] { tg_or_throw().engine()->batched = (on != 0); });

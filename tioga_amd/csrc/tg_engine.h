@@ -183,6 +183,7 @@ struct ResidentTree {
 struct Block {
   // ---- registration: the user's memory, kept by pointer ----
   int tag = 0, nnodes = 0, ncells = 0;
+  int rank = 0, lidx = 0;  // virtual rank, and index among that rank's blocks
   TypeInfo ti{};
   double *ux = nullptr;
   int *uiblank = nullptr;
@@ -478,33 +479,53 @@ class Engine {
   // slots, which overlap whole requests)
   bool overlap_dedupe = true;
   int block_count() const { return (int)blocks_.size(); }
+  // Virtual ranks.  Every block belongs to one rank of a virtual
+  // communicator; per-block calls (registration, solutions, donor info, ...)
+  // name a block by its mesh tag on the current rank, as a rank of the
+  // reference names its own blocks (tag_iblk_map, tioga.h:78).  A mesh tag may
+  // sit on several ranks (a partitioned mesh).  Blocks are kept in rank-major
+  // order, each rank's in its registration order, which is the order of the
+  // reference's OBB allgather (exchangeBoxes.C:99-117); rank 0 alone is the
+  // single-rank reference.
+  int cur_rank = 0;
+  static long long block_key(int rank, int tag) { return ((long long)rank << 32) | (unsigned)tag; }
+  int index_of(int btag) const
+  {
+    auto it = tag_to_blk_.find(block_key(cur_rank, btag));
+    return it == tag_to_blk_.end() ? -1 : it->second;
+  }
   int block_nnodes(int btag) const
   {
-    auto it = tag_to_blk_.find(btag);
-    return it == tag_to_blk_.end() ? -1 : blocks_[it->second]->nnodes;
+    const int i = index_of(btag);
+    return i < 0 ? -1 : blocks_[i]->nnodes;
   }
   int block_ncells(int btag) const
   {
-    auto it = tag_to_blk_.find(btag);
-    return it == tag_to_blk_.end() ? -1 : blocks_[it->second]->ncells;
+    const int i = index_of(btag);
+    return i < 0 ? -1 : blocks_[i]->ncells;
   }
   const double *block_coords(int btag) const
   {
-    auto it = tag_to_blk_.find(btag);
-    return it == tag_to_blk_.end() ? nullptr : blocks_[it->second]->ux;
+    const int i = index_of(btag);
+    return i < 0 ? nullptr : blocks_[i]->ux;
   }
   const double *block_device_coords(int btag) const
   {
-    auto it = tag_to_blk_.find(btag);
-    return it == tag_to_blk_.end() ? nullptr : blocks_[it->second]->x.p;
+    const int i = index_of(btag);
+    return i < 0 ? nullptr : blocks_[i]->x.p;
   }
   const int *block_wall_nodes(int btag, int *nwbc) const
   {
-    auto it = tag_to_blk_.find(btag);
-    if (it == tag_to_blk_.end()) return nullptr;
-    *nwbc = blocks_[it->second]->nwbc;
-    return blocks_[it->second]->uwbc;
+    const int i = index_of(btag);
+    if (i < 0) return nullptr;
+    *nwbc = blocks_[i]->nwbc;
+    return blocks_[i]->uwbc;
   }
+  // ranks holding blocks, and rank r's communication partners after the box
+  // exchange: the ranks one of its blocks shares an OBB pair with, ascending
+  // (sndMap = rcvMap, exchangeBoxes.C:166-176)
+  int nranks() const;
+  const std::vector<int> &partners(int r) const;
   // the search structure over every cell of block btag (after preprocess)
   void build_resident_tree(int btag, ResidentTree &t);
   // searches against block btag walk t (owned by another engine) from now on
@@ -688,7 +709,10 @@ class Engine {
   DBuf<int> wbflag_;               // set when a value does not fit a byte
   DBuf<unsigned char> wbtab_;      // the pieces' table
   std::vector<Block *> blocks_;
-  std::map<int, int> tag_to_blk_;
+  std::map<long long, int> tag_to_blk_;  // block_key(rank, tag) -> index in blocks_
+  std::vector<std::vector<int>> partners_;  // per rank (exchange_boxes)
+  void reindex_blocks();
+  int recv_pair(int i) const;
   std::vector<HoleMap> hm_;
   DBuf<HoleMapView> hm_view_;
   int nmesh_ = 0;

@@ -2707,10 +2707,35 @@ void Engine::sync() { TG_HIP(hipStreamSynchronize(s_)); }
 
 Block *Engine::find(int btag, int *iblk)
 {
-  auto it = tag_to_blk_.find(btag);
-  if (it == tag_to_blk_.end()) return nullptr;
-  if (iblk) *iblk = it->second;
-  return blocks_[it->second];
+  const int i = index_of(btag);
+  if (i < 0) return nullptr;
+  if (iblk) *iblk = i;
+  return blocks_[i];
+}
+
+// Block indices, keys and per-rank indices from blocks_ (rank-major order).
+void Engine::reindex_blocks()
+{
+  tag_to_blk_.clear();
+  std::map<int, int> nper;
+  for (int i = 0; i < (int)blocks_.size(); i++) {
+    Block &b = *blocks_[i];
+    b.lidx = nper[b.rank]++;
+    tag_to_blk_[block_key(b.rank, b.tag)] = i;
+  }
+}
+
+int Engine::nranks() const
+{
+  int r = 0;
+  for (const Block *b : blocks_) r = std::max(r, b->rank + 1);
+  return std::max(r, 1);
+}
+
+const std::vector<int> &Engine::partners(int r) const
+{
+  static const std::vector<int> none;
+  return r >= 0 && r < (int)partners_.size() ? partners_[r] : none;
 }
 
 // tioga::registerGridData (tioga.C:64-86)
@@ -2719,13 +2744,20 @@ void Engine::register_grid(int btag, int nnodes, double *xyz, int *ibl, int nwbc
 {
   if (ntypes < 1 || ntypes > MAXTYPES) throw std::runtime_error("tioga_amd: ntypes out of range");
   if (btag < 1) throw std::runtime_error("tioga_amd: mesh tags must be >= 1");
+  if (cur_rank < 0) throw std::runtime_error("tioga_amd: negative rank");
   int iblk;
   Block *b = find(btag, &iblk);
   if (!b) {
     if (blocks_.size() >= 128) throw std::runtime_error("tioga_amd: at most 128 blocks");
+    if (amr_mode() && cur_rank != 0) throw std::runtime_error("tioga_amd: the AMR path runs on one rank");
     b = new Block;
-    blocks_.push_back(b);
-    tag_to_blk_[btag] = (int)blocks_.size() - 1;
+    b->rank = cur_rank;
+    // after every block of a rank <= cur_rank: rank-major, registration order within a rank
+    size_t at = 0;
+    while (at < blocks_.size() && blocks_[at]->rank <= cur_rank) at++;
+    blocks_.insert(blocks_.begin() + at, b);
+    b->tag = btag;
+    reindex_blocks();
   }
   b->tag = btag;
   b->nnodes = nnodes;
@@ -3002,24 +3034,27 @@ void Engine::profile_moved(int btag, const double *x0_dev, const double rot[9], 
 void Engine::adopt_block(const Engine &src, int src_tag, int as_tag, int share, double *ux, int *uiblank,
                          bool writeback, int slot)
 {
-  auto it = src.tag_to_blk_.find(src_tag);
-  if (it == src.tag_to_blk_.end()) throw std::runtime_error("tioga_amd: adopt of an unknown tag");
-  const Block &S = *src.blocks_[it->second];
+  if (src.nranks() > 1 || cur_rank != 0) throw std::runtime_error("tioga_amd: serving runs on one rank");
+  const int si = src.index_of(src_tag);
+  if (si < 0) throw std::runtime_error("tioga_amd: adopt of an unknown tag");
+  const Block &S = *src.blocks_[si];
   if (!S.topo_uploaded || S.nodeRes.p == nullptr) throw std::runtime_error("tioga_amd: adopt before preprocess");
   if (share != 1 && share != 2) throw std::runtime_error("tioga_amd: share must be 1 or 2");
   Block *b;
   if (slot >= 0 && slot < (int)blocks_.size()) {
     b = blocks_[slot];
-    tag_to_blk_.erase(b->tag);
+    tag_to_blk_.erase(block_key(0, b->tag));
   } else {
     if (blocks_.size() >= 128) throw std::runtime_error("tioga_amd: at most 128 blocks");
     b = new Block;
     blocks_.push_back(b);
     slot = (int)blocks_.size() - 1;
   }
-  if (tag_to_blk_.count(as_tag)) throw std::runtime_error("tioga_amd: adopt under a tag in use");
-  tag_to_blk_[as_tag] = slot;
+  if (tag_to_blk_.count(block_key(0, as_tag))) throw std::runtime_error("tioga_amd: adopt under a tag in use");
+  tag_to_blk_[block_key(0, as_tag)] = slot;
   b->tag = as_tag;
+  b->rank = 0;
+  b->lidx = slot;
   b->nnodes = S.nnodes;
   b->ncells = S.ncells;
   b->ti = S.ti;
@@ -3116,27 +3151,34 @@ void Engine::get_hole_map()
   hm_ = std::vector<HoleMap>(maxtag);
   // reuse the old maps' storage
   for (size_t i = 0; i < old.size() && i < hm_.size(); i++) std::swap(hm_[i].sam.p, old[i].sam.p), std::swap(hm_[i].sam.cap, old[i].sam.cap);
+  // Per mesh tag: a wall exists if any block of the tag has wall nodes, and
+  // the bounds are the min / max over the tag's blocks of their wall bounds,
+  // where a block without wall nodes reports zeros (MeshBlock.C:640-647).
+  // With one block per tag this is the single-rank map; with a tag on several
+  // ranks it is the MPI_MAX / MPI_MIN / MPI_MAX allreduce of holeMap.C:86-112.
   std::vector<double> bmin(3 * maxtag, BIGVALUE), bmax(3 * maxtag, -BIGVALUE);
+  for (HoleMap &h : hm_) h.exist = 0;
   for (Block *bp : blocks_) {
     Block &b = *bp;
     HoleMap &h = hm_[b.tag - 1];
-    if (b.nwbc <= 0) {  // MeshBlock.C:640-647
-      h.exist = 0;
-      for (int j = 0; j < 3; j++) bmin[3 * (b.tag - 1) + j] = bmax[3 * (b.tag - 1) + j] = 0;
-      continue;
-    }
-    h.exist = 1;
-    double lo[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, hi[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
-    for (int i = 0; i < b.nwbc; i++) {
-      const double *p = b.ux + 3 * (size_t)(b.uwbc[i] - BASE);
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    if (b.nwbc > 0) {
+      h.exist = 1;
       for (int j = 0; j < 3; j++) {
-        lo[j] = dmin(lo[j], p[j]);
-        hi[j] = dmax(hi[j], p[j]);
+        lo[j] = BIGVALUE;
+        hi[j] = -BIGVALUE;
+      }
+      for (int i = 0; i < b.nwbc; i++) {
+        const double *p = b.ux + 3 * (size_t)(b.uwbc[i] - BASE);
+        for (int j = 0; j < 3; j++) {
+          lo[j] = dmin(lo[j], p[j]);
+          hi[j] = dmax(hi[j], p[j]);
+        }
       }
     }
     for (int j = 0; j < 3; j++) {
-      bmin[3 * (b.tag - 1) + j] = lo[j];
-      bmax[3 * (b.tag - 1) + j] = hi[j];
+      bmin[3 * (b.tag - 1) + j] = std::min(bmin[3 * (b.tag - 1) + j], lo[j]);
+      bmax[3 * (b.tag - 1) + j] = std::max(bmax[3 * (b.tag - 1) + j], hi[j]);
     }
   }
   std::vector<HoleMapView> views(maxtag);
@@ -3301,25 +3343,58 @@ void Engine::get_hole_map()
   sync();
 }
 
-// tioga::exchangeBoxes (exchangeBoxes.C:34-272), single rank
+// The pair that receives pair i's packets: the packet goes to the rank of
+// pair i's OBB, whose map (intBoxMap) holds its pairs by receive key, the
+// last one inserted under a key (exchangeBoxes.C:206-243).
+int Engine::recv_pair(int i) const
+{
+  const int key = pairs_[i].send_tag, dst = blocks_[pairs_[i].ob]->rank;
+  int ii = -1;
+  for (int j = 0; j < (int)pairs_.size(); j++)
+    if (pairs_[j].recv_tag == key && blocks_[pairs_[j].local]->rank == dst) ii = j;
+  return ii;
+}
+
+// tioga::exchangeBoxes (exchangeBoxes.C:34-272) for every rank
 void Engine::exchange_boxes()
 {
   const int nb_ = (int)blocks_.size();
   int maxtag = -1;
   for (Block *b : blocks_) maxtag = std::max(maxtag, std::abs(b->tag));
+  const int mxtgsqr = maxtag * maxtag;
+  // each rank's pairs in its own order (every OBB of the allgather, then
+  // its blocks), rank after rank; a rank's partners are the ranks of the
+  // OBBs it pairs with (exchangeBoxes.C:131-176)
+  const int nr = nranks();
   pairs_.clear();
-  for (int ob = 0; ob < nb_; ob++)
-    for (int ib = 0; ib < nb_; ib++) {
-      Block &A = *blocks_[ib], &B = *blocks_[ob];
-      if (std::abs(B.tag) == A.tag) continue;
-      if (obb_intersect(A.obb, B.obb) || obb_intersect(B.obb, A.obb)) {
-        Pair p;
-        p.local = ib;
-        p.ob = ob;
-        pairs_.push_back(p);
+  partners_.assign(nr, std::vector<int>());
+  for (int r = 0; r < nr; r++) {
+    std::vector<char> flag(nr, 0);
+    for (int ob = 0; ob < nb_; ob++)
+      for (int ib = 0; ib < nb_; ib++) {
+        Block &A = *blocks_[ib], &B = *blocks_[ob];
+        if (A.rank != r) continue;
+        if (std::abs(B.tag) == A.tag) continue;
+        if (obb_intersect(A.obb, B.obb) || obb_intersect(B.obb, A.obb)) {
+          Pair p;
+          p.local = ib;
+          p.ob = ob;
+          pairs_.push_back(p);
+          flag[B.rank] = 1;
+        }
       }
-    }
+    for (int k = 0; k < nr; k++)
+      if (flag[k]) partners_[r].push_back(k);
+  }
   nsend_ = pairs_.empty() ? 0 : 1;
+  // exchangeDonors returns at once on a rank without partners
+  // (exchangeDonors.C:36), so its blocks are never hole-cut there; that
+  // case is not reproduced
+  for (int r = 0; r < nr && nsend_; r++)
+    if (partners_[r].empty())
+      for (const Block *b : blocks_)
+        if (b->rank == r)
+          throw std::runtime_error("tioga_amd: a rank whose blocks overlap no other block (not supported)");
   const int np = (int)pairs_.size();
   kscratch_.reserve(6 * (size_t)std::max(np, 1));
   std::vector<unsigned long long> init(6 * (size_t)std::max(np, 1));
@@ -3335,8 +3410,8 @@ void Engine::exchange_boxes()
   for (int i = 0; i < np; i++) {
     Pair &p = pairs_[i];
     Block &A = *blocks_[p.local], &B = *blocks_[p.ob];
-    p.recv_tag = maxtag * (A.tag - 1) + B.tag - 1;
-    p.send_tag = maxtag * (B.tag - 1) + (A.tag - 1);
+    p.recv_tag = mxtgsqr * B.rank + maxtag * (A.tag - 1) + B.tag - 1;
+    p.send_tag = mxtgsqr * A.rank + maxtag * (B.tag - 1) + (A.tag - 1);
     if (A.ncells && !(batched && npart[p.local] > 1))
       k_reduced_obb<<<nb(A.ncells), BS, 0, s_>>>(A.ncells, A.ti, A.conn.p, A.x.p, A.obb, B.obb,
                                                  kscratch_.p + 6 * (size_t)i);
@@ -3387,12 +3462,8 @@ void Engine::exchange_boxes()
       rd[6 * i + 3 + j] = (bb[j + 3] - bb[j]) * 0.51;
     }
   }
-  // packet i lands on the pair whose receive key is its send key; std::map
-  // keeps the last pair inserted under a key (exchangeBoxes.C:206-243)
   for (int i = 0; i < np; i++) {
-    int key = pairs_[i].send_tag, ii = -1;
-    for (int j = 0; j < np; j++)
-      if (pairs_[j].recv_tag == key) ii = j;
+    const int ii = recv_pair(i);
     if (ii < 0) continue;
     pairs_[ii].iblk_remote = pairs_[i].local;
     pairs_[ii].tag_remote = blocks_[pairs_[i].local]->tag;
@@ -3552,9 +3623,7 @@ void Engine::exchange_search_data()
     }
   }
   for (int i = 0; i < np; i++) {
-    int key = pairs_[i].send_tag, ii = -1;
-    for (int j = 0; j < np; j++)
-      if (pairs_[j].recv_tag == key) ii = j;
+    const int ii = recv_pair(i);
     dest[i] = ii;
     if (ii >= 0) blocks_[pairs_[ii].local]->nsearch += cnt[i];
   }
@@ -4897,7 +4966,9 @@ void Engine::get_donor_count(int btag, int *dcount, int *fcount)
     }
 }
 
-// tioga::getDonorInfo (tioga.C:551-570, dataUpdate.C:136-159); rank 0
+// tioga::getDonorInfo (tioga.C:551-570, dataUpdate.C:136-159): per live
+// record (receptor's rank, receptor node, receptor block's index on its
+// rank, nweights)
 void Engine::get_donor_info(int btag, int *receptors, int *indices, double *frac, int *dcount)
 {
   (void)dcount;
@@ -4912,16 +4983,19 @@ void Engine::get_donor_info(int btag, int *receptors, int *indices, double *frac
       frac[j] = h.w[m];
       j++;
     }
-    receptors[k++] = h.rinfo[0];
+    const Block &R = *blocks_[h.rinfo[2]];
+    receptors[k++] = R.rank;
     receptors[k++] = h.rinfo[1];
-    receptors[k++] = h.rinfo[2];
+    receptors[k++] = R.lidx;
     receptors[k++] = h.nweights;
   }
 }
 
-// tioga::getReceptorInfo (tioga.C:572-671, dataUpdate.C:161-178), single
-// rank: per live record (node, receptor mesh tag, donor cell global id as two
-// ints); the cell index stands in for the global id without cell_gid.
+// tioga::getReceptorInfo (tioga.C:572-671, dataUpdate.C:161-178) on the
+// current rank: per live record whose receptor is on this rank (node,
+// receptor mesh tag, donor cell global id as two ints), in the order the
+// packets arrive: donor rank, donor block, record; the cell index stands in
+// for the global id without cell_gid.
 void Engine::get_receptor_info(std::vector<int> &receptors)
 {
   receptors.clear();
@@ -4930,7 +5004,7 @@ void Engine::get_receptor_info(std::vector<int> &receptors)
     Block &b = *bp;
     fetch_interp(b, s_);
     for (auto &h : b.interp_host) {
-      if (h.cancel) continue;
+      if (h.cancel || blocks_[h.rinfo[2]]->rank != cur_rank) continue;
       receptors.push_back(h.rinfo[1]);
       receptors.push_back(blocks_[h.rinfo[2]]->tag);
       int don = h.inode[h.nweights];
@@ -4963,7 +5037,21 @@ long Engine::block_array(int btag, const std::string &nm, void *out)
   if (nm == "cellRes") return copy_out(b.cellRes, b.ncells, out, s_);
   if (nm == "xsearch") return copy_out(b.xsearch, 3 * Q, out, s_);
   if (nm == "res_search") return copy_out(b.res_search, Q, out, s_);
-  if (nm == "isearch") return copy_out(b.isearch, 3 * Q, out, s_);
+  if (nm == "isearch") {
+    // the device holds (0, node, sender block index); the reference's
+    // (partner index of the sender's rank, node, sender's index on its rank)
+    const long n = copy_out(b.isearch, 3 * Q, out, s_);
+    if (out && nranks() > 1) {
+      int *o = (int *)out;
+      const std::vector<int> &pm = partners(b.rank);
+      for (size_t q = 0; q < Q; q++) {
+        const Block &S = *blocks_[o[3 * q + 2]];
+        o[3 * q] = (int)(std::lower_bound(pm.begin(), pm.end(), S.rank) - pm.begin());
+        o[3 * q + 2] = S.lidx;
+      }
+    }
+    return n;
+  }
   if (nm == "tagsearch") return copy_out(b.tagsearch, Q, out, s_);
   if (nm == "donorId") return copy_out(b.donorId, Q, out, s_);
   if (nm == "xtag") return copy_out(b.xtag, Q, out, s_);

@@ -103,6 +103,8 @@ def load():
         L.tioga_amd_serve_wait2.argtypes = [C.c_long, _P, C.c_int, _P, _P, _P, C.c_int, _P, _P]
         L.tioga_amd_serve_wait2.restype = C.c_int
         L.tioga_amd_serve_close.argtypes = []
+        L.tioga_amd_set_rank.argtypes = [C.c_int]
+        L.tioga_amd_rank.restype = C.c_int
         L.tioga_amd_amr_array.argtypes = [C.c_int, C.c_char_p, _P]
         L.tioga_amd_amr_array.restype = C.c_long
         _lib = L
@@ -134,6 +136,7 @@ class Tioga:
         self._check()
         self._keep: List[object] = []
         self.blocks = []
+        self.ranks: List[int] = []  # virtual rank of each registered block
         self.iblank: List[np.ndarray] = []
         self.iblank_cell: List[np.ndarray] = []
         self.q: Dict[int, np.ndarray] = {}
@@ -157,8 +160,16 @@ class Tioga:
     def __exit__(self, *a):
         self.close()
 
+    def set_rank(self, rank: int) -> None:
+        """Virtual rank that the per-block calls which follow act on."""
+        self.L.tioga_amd_set_rank(int(rank))
+        self._check()
+
     # ---- registration (tiogaInterface.C:108-130) ----
-    def register_grid(self, block, bid: Optional[int] = None) -> None:
+    def register_grid(self, block, bid: Optional[int] = None, rank: int = 0) -> None:
+        """Register a block on virtual rank `rank` (slot bid, by default the
+        next free slot of that rank)."""
+        self.set_rank(rank)
         b = block.normalized()
         ib = np.ones(b.nnodes, np.int32)
         args = []
@@ -166,7 +177,7 @@ class Tioga:
             nvp, ncp = C.c_int(nv), C.c_int(c.shape[0])
             self._keep += [nvp, ncp, c]
             args += [C.byref(nvp), C.byref(ncp), _p(c)]
-        bid = len(self.blocks) + 1 if bid is None else bid
+        bid = self.ranks.count(rank) + 1 if bid is None else bid
         self._keep += [b, ib]
         self.L.tioga_registergrid_data_mb_(_ip(bid), _ip(b.tag), _ip(b.nnodes), _p(b.x), _p(ib),
                                            _ip(len(b.wbc)), _ip(len(b.obc)), _p(b.wbc), _p(b.obc),
@@ -179,6 +190,7 @@ class Tioga:
             self.L.tioga_amd_register_gids(b.tag, _p(b.cell_gid), _p(b.node_gid))
             self._check()
         self.blocks.append(b)
+        self.ranks.append(rank)
         self.iblank.append(ib)
 
     def set_params(self, isym: int = 3, nfringe: int = 1, mexclude: int = 3):
@@ -227,7 +239,8 @@ class Tioga:
 
     def data_update_copies(self, qs: Sequence[np.ndarray], nvar: int, interptype: int = 0):
         out = [np.ascontiguousarray(q, np.float64).copy() for q in qs]
-        for b, q in zip(self.blocks, out):
+        for b, r, q in zip(self.blocks, self.ranks, out):
+            self.set_rank(r)
             self.register_solution(b.tag, q)
         self.data_update(nvar, interptype)
         return out
@@ -465,9 +478,32 @@ class Tioga:
         return int(self.L.tioga_amd_weight_aborts())
 
     def results(self, qs=None, nvar: int = 0, interptype: int = 0) -> Dict[str, np.ndarray]:
-        """Every array the reference driver dumps, under the same names."""
+        """Every array the reference driver dumps, under the same names; with
+        blocks on several virtual ranks, each rank's arrays under "r<rank>."
+        as the multi-rank driver's files merge them."""
+        ranks = sorted(set(self.ranks))
+        if ranks != [0]:
+            r: Dict[str, np.ndarray] = {}
+            for k in ranks:
+                for nm, a in self._results_of_rank(k).items():
+                    r[f"r{k}." + nm] = a
+            if qs is not None:
+                for i, q in enumerate(self.data_update_copies(qs, nvar, interptype)):
+                    r[f"r{self.ranks[i]}.b{i}.q"] = q
+            self.set_rank(0)
+            return r
+        r = self._results_of_rank(0)
+        if qs is not None:
+            for i, q in enumerate(self.data_update_copies(qs, nvar, interptype)):
+                r[f"b{i}.q"] = q
+        return r
+
+    def _results_of_rank(self, rank: int) -> Dict[str, np.ndarray]:
+        self.set_rank(rank)
         r: Dict[str, np.ndarray] = {}
         for i, b in enumerate(self.blocks):
+            if self.ranks[i] != rank:
+                continue
             p = f"b{i}."
             r[p + "iblank"] = self.iblank[i].copy()
             r[p + "iblank_cell"] = self.array(b.tag, "iblank_cell")
@@ -488,7 +524,4 @@ class Tioga:
             r[f"hm{m}.existWall"] = np.array([0 if hm is None else 1], np.int32)
             if hm is not None:
                 r[f"hm{m}.nx"], r[f"hm{m}.extents"], r[f"hm{m}.sam"] = hm
-        if qs is not None:
-            for i, q in enumerate(self.data_update_copies(qs, nvar, interptype)):
-                r[f"b{i}.q"] = q
         return r

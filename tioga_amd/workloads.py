@@ -139,6 +139,42 @@ def cart_box(n: int, tag: int, lo: float = -1.0, hi: float = 1.0, jitter: float 
     return Block(tag, x, [(8, conn)])
 
 
+def partition(b: Block, cell_mask: np.ndarray) -> Block:
+    """The partition of block b holding the cells where cell_mask (one flag
+    per cell, in type order) is set, as a rank of a partitioned mesh holds
+    it: the nodes those cells use, in ascending original order, the
+    connectivity renumbered to them, the wall and overset-boundary nodes and
+    user resolutions and global ids that fall in the partition.  Nodes on a
+    cut belong to every partition that touches them."""
+    cell_mask = np.asarray(cell_mask, bool)
+    used = np.zeros(b.nnodes, bool)
+    c0 = 0
+    for nv, c in b.types:
+        m = cell_mask[c0:c0 + c.shape[0]]
+        used[(c[m] - 1).ravel()] = True
+        c0 += c.shape[0]
+    remap = np.full(b.nnodes, -1, np.int64)
+    keep = np.nonzero(used)[0]
+    remap[keep] = np.arange(keep.size)
+    types, c0 = [], 0
+    for nv, c in b.types:
+        m = cell_mask[c0:c0 + c.shape[0]]
+        c0 += c.shape[0]
+        if m.any():
+            types.append((nv, (remap[c[m] - 1] + 1).astype(np.int32)))
+
+    def nodes(lst):
+        lst = np.asarray(lst, np.int64) - 1
+        lst = lst[used[lst]]
+        return (remap[lst] + 1).astype(np.int32)
+
+    return Block(b.tag, b.x[keep].copy(), types, nodes(b.wbc), nodes(b.obc),
+                 None if b.nres is None else b.nres[keep].copy(),
+                 None if b.cres is None else b.cres[cell_mask].copy(),
+                 None if b.cell_gid is None else b.cell_gid[cell_mask].copy(),
+                 None if b.node_gid is None else b.node_gid[keep].copy())
+
+
 def _radii(r0: float, r1: float, nr: int, stretch: float) -> np.ndarray:
     k = np.arange(nr + 1, dtype=np.float64)
     if stretch == 1.0:
