@@ -195,6 +195,23 @@ int tioga_amd_serve_wait2(long id, int *shell_iblank, int nshell, int *nbg, int 
 void tioga_amd_set_rank(int rank);
 int tioga_amd_rank(void);
 
+/* The packets virtual ranks exchange, in the reference's PACKET layout
+   (codetypes.h:114-120) and order (parallelComm.C:96-170): what rank src
+   sends rank dst in one exchange.  tioga_amd_record_packets(1) keeps what
+   the exchanges send from then on (0 stops and frees it).  exchange is one
+   of "boxes" (exchangeBoxes.C:184-243), "search" (exchangeSearchData.C:
+   84-112), "donors1" .. "donors4" (the four rounds of exchangeDonors.C),
+   "receptor_info" (tioga.C:572-650), "update" (tioga.C:368-427) and
+   "rf.donors3" (the cancellations of reduce_fringes, tioga.C:764-807); the
+   exchanges after reduce_fringes are its "donors4", "update" and
+   "receptor_info" asked again.  counts[0..1] = the packet's (nints, nreals)
+   header, counts[2..3] = the ints and doubles written to ints / reals
+   (either may be NULL to size them first; "boxes" writes the six doubles
+   per pair that the reference fills of the eighteen it sends).  Returns 0,
+   or -1 when src and dst do not exchange. */
+void tioga_amd_record_packets(int on);
+int tioga_amd_packet(const char *exchange, int src, int dst, int *ints, double *reals, long long *counts);
+
 /* Batched form (on != 0, the default): the stages of performconnectivity
    run for all blocks together (one ADT build, one coincident-point dedupe
    for every block's query set, ...); 0 runs them block by block.  The

@@ -82,9 +82,14 @@ def run_engine_case(case, batched: bool = True):
     try:
         t.set_batching(batched)
         t.set_params(case.isym, case.nfringe, case.mexclude)
-        ranks = getattr(case, "ranks", None) or [0] * len(case.blocks)
+        ranked = getattr(case, "ranks", None) is not None
+        ranks = case.ranks if ranked else [0] * len(case.blocks)
+        gids = any(b.node_gid is not None for b in case.blocks)
         for b, r in zip(case.blocks, ranks):
             t.register_grid(b, rank=r)
+        if ranked:
+            t.record_packets(True)
+        conn_ex = ["boxes", "search", "donors1", "donors2", "donors3", "donors4"] + (["receptor_info"] if gids else [])
         for s in range(len(case.moves) + 1):
             if s > 0:
                 for bi, xb in enumerate(case.moves[s - 1]):
@@ -95,10 +100,16 @@ def run_engine_case(case, batched: bool = True):
             pre = "" if s == 0 else f"s{s}."
             for k, v in t.results(case.qs, case.nvar, case.interptype).items():
                 out[pre + k] = v
+            if ranked:  # what the exchanges sent (the dataUpdate ran inside results)
+                for ex in conn_ex + ["update"]:
+                    out.update(t.packets(ex, pre + ex))
             if getattr(case, "reduce", False):
                 t.reduce_fringes()
                 for k, v in t.results(case.qs, case.nvar, case.interptype).items():
                     out[pre + "rf." + k] = v
+                if ranked:
+                    for ex in ["rf.donors3", "donors4"] + (["receptor_info"] if gids else []) + ["update"]:
+                        out.update(t.packets(ex, pre + ("rf." + ex if not ex.startswith("rf.") else ex)))
         out["weight_aborts"] = t.weight_aborts()
     finally:
         t.close()

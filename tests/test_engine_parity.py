@@ -22,7 +22,7 @@ def test_engine_matches_reference(name, batched):
     print(f"\n[engine parity] case {name} batched={batched}", flush=True)
     got = run_engine_case(cases.build(name), batched)
     assert got["weight_aborts"] == 0
-    bad = compare_to_golden(name, got, skip_prefix=("pk.",))
+    bad = compare_to_golden(name, got)
     assert not bad, "\n".join(bad[:20])
 
 

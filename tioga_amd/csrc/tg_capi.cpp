@@ -531,6 +531,30 @@ int tioga_amd_rank(void)
   return r;
 }
 
+void tioga_amd_record_packets(int on)
+{
+  guarded("tioga_amd_record_packets", [&] { tg_or_throw().engine()->record_packets(on != 0); });
+}
+
+int tioga_amd_packet(const char *exchange, int src, int dst, int *ints, double *reals, long long *counts)
+{
+  int r = -1;
+  guarded("tioga_amd_packet", [&] {
+    std::vector<int> iv;
+    std::vector<double> rv;
+    long long hdr[2];
+    if (!tg_or_throw().engine()->packet(exchange, src, dst, iv, rv, hdr)) return;
+    counts[0] = hdr[0];
+    counts[1] = hdr[1];
+    counts[2] = (long long)iv.size();
+    counts[3] = (long long)rv.size();
+    if (ints && !iv.empty()) memcpy(ints, iv.data(), sizeof(int) * iv.size());
+    if (reals && !rv.empty()) memcpy(reals, rv.data(), sizeof(double) * rv.size());
+    r = 0;
+  });
+  return r;
+}
+
 void tioga_amd_set_batching(int on)
 {
   guarded("tioga_amd_set_batching", [&] { tg_or_throw().engine()->batched = (on != 0); });

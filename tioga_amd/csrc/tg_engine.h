@@ -451,6 +451,26 @@ struct HoleMap {
   DBuf<unsigned char> sam;
 };
 
+// What the exchanges of the last performConnectivity, reduce_fringes and
+// dataUpdate sent between virtual ranks, kept while packet recording is on:
+// the state each packet is read from at the moment the reference builds it
+// (later stages change some of it in place).
+struct PacketLog {
+  bool on = false;
+  bool connectivity = false, reduced = false, updated = false;
+  std::vector<double> pair_rd;               // per pair: the reduced OBB sent (xc, dxc)
+  std::vector<int> pair_cnt, pair_off;       // per pair: query points sent, their offset at the receiver
+  std::vector<std::vector<double>> sent_res; // per block: res_search as received (before the dedupe)
+  std::vector<std::vector<int>> iblank_pd;   // per block: iblank after processDonors
+  std::vector<std::vector<double>> nres_pd;  // per block: nodeRes after processDonors
+  // the receptor records of processDonors (comm2) in send order
+  std::vector<int> rec_blk, rec_node, rec_dblk, rec_remote;
+  std::vector<double> rec_rres;
+  std::vector<std::vector<int>> rf_cancel;   // per block: receptors reduce_fringes drops, ascending
+  int nvar = 0;
+  std::vector<std::vector<double>> upd_vals; // per donor block: values of its live records (dataUpdate)
+};
+
 // Per-phase device time of the last performConnectivity (ms, HIP events).
 struct PhaseTimes {
   double holemap = 0, boxes = 0, searchdata = 0, search_prep = 0, search_kernel = 0, donors = 0,
@@ -526,6 +546,18 @@ class Engine {
   // (sndMap = rcvMap, exchangeBoxes.C:166-176)
   int nranks() const;
   const std::vector<int> &partners(int r) const;
+  // Packets between virtual ranks in the reference's PACKET layout
+  // (codetypes.h:114-120): recording on keeps what the exchanges send;
+  // packet() rebuilds the packet virtual rank src sends to dst in exchange
+  // `ex` ("boxes", "search", "donors1".."donors4", "receptor_info",
+  // "update", "rf.donors3").  counts = the (nints, nreals) header; returns
+  // false when src and dst do not exchange.
+  void record_packets(bool on);
+  bool packet(const std::string &ex, int src, int dst, std::vector<int> &ints, std::vector<double> &reals,
+              long long counts[2]);
+  PacketLog pk_;
+  void interp_records(Block &b);
+  void record_update(int nvar, int interptype);
   // the search structure over every cell of block btag (after preprocess)
   void build_resident_tree(int btag, ResidentTree &t);
   // searches against block btag walk t (owned by another engine) from now on

@@ -3462,6 +3462,7 @@ void Engine::exchange_boxes()
       rd[6 * i + 3 + j] = (bb[j + 3] - bb[j]) * 0.51;
     }
   }
+  pk_.pair_rd = rd;
   for (int i = 0; i < np; i++) {
     const int ii = recv_pair(i);
     if (ii < 0) continue;
@@ -3651,8 +3652,11 @@ void Engine::exchange_search_data()
   // every pair's query points in one launch (a segment per pair)
   std::vector<int> off(blocks_.size(), 0);
   std::vector<QFillSeg> qs;
+  pk_.pair_cnt = cnt;
+  pk_.pair_off.assign(np, -1);
   for (int i = 0; i < np; i++) {
     int ii = dest[i];
+    if (ii >= 0) pk_.pair_off[i] = off[pairs_[ii].local];
     if (ii < 0 || cnt[i] == 0) continue;
     Block &A = *blocks_[pairs_[i].local];
     int rb = pairs_[ii].local;
@@ -3665,6 +3669,17 @@ void Engine::exchange_search_data()
   const int totq = pack_segs(qs);
   if (totq) k_fill_search_all<<<totq / BS, BS, 0, s_>>>((int)qs.size(), put_segs(stw_.qfill, qs));
   sync();
+  if (pk_.on) {  // the resolutions as sent, before the dedupe raises them
+    pk_.sent_res.assign(blocks_.size(), std::vector<double>());
+    for (size_t ib = 0; ib < blocks_.size(); ib++) {
+      Block &b = *blocks_[ib];
+      pk_.sent_res[ib].resize(b.nsearch);
+      if (b.nsearch)
+        TG_HIP(hipMemcpyAsync(pk_.sent_res[ib].data(), b.res_search.p, sizeof(double) * b.nsearch,
+                              hipMemcpyDeviceToHost, s_));
+    }
+    sync();
+  }
 }
 
 // MeshBlock::search (search.C:78-269), NODEGID off
@@ -4515,6 +4530,21 @@ void Engine::exchange_donors()
     if (totV) TG_HIP(hipMemcpyAsync(mt.p, mt1.p, totV, hipMemcpyDeviceToDevice, s_));
   }
   if (totP) k_process2_all<<<totP / BS, BS, 0, s_>>>((int)pseg.size(), pt, dtag.p);
+  if (pk_.on) {  // iblank and nodeRes as findInterpData sees them
+    pk_.iblank_pd.assign(nbk, std::vector<int>());
+    pk_.nres_pd.assign(nbk, std::vector<double>());
+    for (int ib = 0; ib < nbk; ib++) {
+      Block &b = *blocks_[ib];
+      pk_.iblank_pd[ib].resize(b.nnodes);
+      pk_.nres_pd[ib].resize(b.nnodes);
+      if (!b.nnodes) continue;
+      TG_HIP(hipMemcpyAsync(pk_.iblank_pd[ib].data(), b.iblank.p, sizeof(int) * b.nnodes, hipMemcpyDeviceToHost,
+                            s_));
+      TG_HIP(hipMemcpyAsync(pk_.nres_pd[ib].data(), b.nodeRes.p, sizeof(double) * b.nnodes,
+                            hipMemcpyDeviceToHost, s_));
+    }
+    sync();
+  }
   // the records of all receptor blocks, in (receptor block, node) order
   const int totrec = totV ? select_flagged(pflag.p, totV, psel.p, s_) : 0;
   RecordList L;
@@ -4527,6 +4557,26 @@ void Engine::exchange_donors()
   L.rres = lrres.reserve(totrec);
   const EmitTables *pet = (const EmitTables *)put_table(w.et, et);
   if (totrec) k_emit_records_all<<<nb(totrec), BS, 0, s_>>>(totrec, psel.p, pet, L);
+  if (pk_.on) {
+    std::vector<int> g(totrec);
+    pk_.rec_blk.resize(totrec);
+    pk_.rec_node.resize(totrec);
+    pk_.rec_dblk.resize(totrec);
+    pk_.rec_remote.resize(totrec);
+    pk_.rec_rres.resize(totrec);
+    if (totrec) {
+      TG_HIP(hipMemcpyAsync(g.data(), psel.p, sizeof(int) * totrec, hipMemcpyDeviceToHost, s_));
+      TG_HIP(hipMemcpyAsync(pk_.rec_node.data(), L.node, sizeof(int) * totrec, hipMemcpyDeviceToHost, s_));
+      TG_HIP(hipMemcpyAsync(pk_.rec_dblk.data(), L.dblk, sizeof(int) * totrec, hipMemcpyDeviceToHost, s_));
+      TG_HIP(hipMemcpyAsync(pk_.rec_remote.data(), L.remote, sizeof(int) * totrec, hipMemcpyDeviceToHost, s_));
+      TG_HIP(hipMemcpyAsync(pk_.rec_rres.data(), L.rres, sizeof(double) * totrec, hipMemcpyDeviceToHost, s_));
+      sync();
+    }
+    for (int j = 0; j < totrec; j++)
+      pk_.rec_blk[j] = (int)(std::upper_bound(noff.begin(), noff.end(), g[j]) - noff.begin()) - 1;
+    pk_.connectivity = true;
+    pk_.reduced = pk_.updated = false;
+  }
   // comm2: findInterpData per donor block, records in arrival order: a
   // stable sort by donor block keeps each donor's records in list order
   DBuf<int> &dsel = w.dsel, &tmin = w.tmin, &dstart = w.dstart, &didx = w.didx;
@@ -4876,6 +4926,19 @@ void Engine::reduce_fringes()
     }
     k_rf_cancel<<<nb(V), BS, 0, s_>>>(V, b.iblank.p, ibr[ib]->p, cancelv[ib]->p);
   }
+  if (pk_.on) {  // the receptors each block drops, ascending (its new cancel list)
+    pk_.rf_cancel.assign(nbk, std::vector<int>());
+    for (int ib = 0; ib < nbk; ib++) {
+      const int V = blocks_[ib]->nnodes;
+      std::vector<unsigned char> m(V);
+      if (V) TG_HIP(hipMemcpyAsync(m.data(), cancelv[ib]->p, V, hipMemcpyDeviceToHost, s_));
+      sync();
+      for (int i = 0; i < V; i++)
+        if (m[i]) pk_.rf_cancel[ib].push_back(i);
+    }
+    pk_.reduced = true;
+    pk_.updated = false;
+  }
   DevStruct<CancelTables> dct;
   CancelTables *pct = dct.put(ct, s_);
   for (int ib = 0; ib < nbk; ib++) {
@@ -4951,6 +5014,8 @@ static void fetch_interp(Block &b, hipStream_t s)
   }
   b.interp_host_valid = true;
 }
+
+void Engine::interp_records(Block &b) { fetch_interp(b, s_); }
 
 // tioga::getDonorCount (tioga.C:533-549, dataUpdate.C:121-134)
 void Engine::get_donor_count(int btag, int *dcount, int *fcount)

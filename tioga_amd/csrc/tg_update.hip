@@ -277,6 +277,7 @@ void Engine::data_update(int nvar, int interptype)
   if (nsend_ == 0 || nvar <= 0) return;
   if (!upd_.valid) prepare_update();
   const int nbk = (int)blocks_.size();
+  if (pk_.on) record_update(nvar, interptype);
   while ((int)qdev_.size() < nbk) qdev_.push_back(new DBuf<double>);
   UpdTable &T = upd_.host;
   for (int ib = 0; ib < nbk; ib++) {

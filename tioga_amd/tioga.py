@@ -105,6 +105,9 @@ def load():
         L.tioga_amd_serve_close.argtypes = []
         L.tioga_amd_set_rank.argtypes = [C.c_int]
         L.tioga_amd_rank.restype = C.c_int
+        L.tioga_amd_record_packets.argtypes = [C.c_int]
+        L.tioga_amd_packet.argtypes = [C.c_char_p, C.c_int, C.c_int, _P, _P, _P]
+        L.tioga_amd_packet.restype = C.c_int
         L.tioga_amd_amr_array.argtypes = [C.c_int, C.c_char_p, _P]
         L.tioga_amd_amr_array.restype = C.c_long
         _lib = L
@@ -164,6 +167,38 @@ class Tioga:
         """Virtual rank that the per-block calls which follow act on."""
         self.L.tioga_amd_set_rank(int(rank))
         self._check()
+
+    def record_packets(self, on: bool = True) -> None:
+        """Keep what the exchanges between virtual ranks send (tioga_amd_packet)."""
+        self.L.tioga_amd_record_packets(1 if on else 0)
+        self._check()
+
+    def packet(self, exchange: str, src: int, dst: int):
+        """(ints, reals, (nints, nreals)) that rank src sends rank dst in one
+        exchange of the last connectivity, or None when they do not exchange."""
+        cnt = np.zeros(4, np.int64)
+        if self.L.tioga_amd_packet(exchange.encode(), src, dst, None, None, _p(cnt)) != 0:
+            self._check()
+            return None
+        ints = np.zeros(int(cnt[2]), np.int32)
+        reals = np.zeros(int(cnt[3]), np.float64)
+        self.L.tioga_amd_packet(exchange.encode(), src, dst, _p(ints), _p(reals), _p(cnt))
+        self._check()
+        return ints, reals, (int(cnt[0]), int(cnt[1]))
+
+    def packets(self, exchange: str, name: Optional[str] = None) -> Dict[str, np.ndarray]:
+        """Every packet of one exchange as pk.<name>.<src>.<dst>.{ints,reals,counts}."""
+        out: Dict[str, np.ndarray] = {}
+        nr = max(self.ranks) + 1 if self.ranks else 1
+        for src in range(nr):
+            for dst in range(nr):
+                got = self.packet(exchange, src, dst)
+                if got is None:
+                    continue
+                p = f"pk.{name or exchange}.{src}.{dst}."
+                out[p + "ints"], out[p + "reals"] = got[0], got[1]
+                out[p + "counts"] = np.array(got[2], np.int32)
+        return out
 
     # ---- registration (tiogaInterface.C:108-130) ----
     def register_grid(self, block, bid: Optional[int] = None, rank: int = 0) -> None:
