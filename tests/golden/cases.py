@@ -212,6 +212,10 @@ def build(name: str) -> Case:
     if name == "ranks3_gid":
         blocks, ranks = ranked_blocks(three=True)
         return Case(name, _position_gids(blocks), nvar=2, ranks=ranks)
+    if name == "ranks4_apart":  # a fourth rank whose block overlaps no other (exchangeDonors.C:36)
+        blocks, ranks = ranked_blocks(three=True)
+        far = W.cart_box(4, tag=4, lo=1.3, hi=1.6)
+        return Case(name, blocks + [far], nvar=2, nfringe=2, reduce=True, ranks=ranks + [3])
     raise KeyError(name)
 
 
@@ -221,7 +225,7 @@ GID_CASES = ["tiny_gid", "seam_gid", "seam_gid_rf"]
 ALL_CASES = SMALL_CASES + ["config1"] + REDUCE_CASES + GID_CASES
 # several ranks: pinned by the reference run under mpiexec (the oracle
 # restates the single-rank path only)
-RANK_CASES = ["ranks2", "ranks3", "ranks3_rf", "ranks3_gid"]
+RANK_CASES = ["ranks2", "ranks3", "ranks3_rf", "ranks3_gid", "ranks4_apart"]
 
 
 # ---------------------------------------------------------------------------

@@ -61,6 +61,8 @@ def packet_arrays(packets, gids: bool, nmoves: int, reduce: bool):
     names = packet_stages(gids, nmoves, reduce)
     out = {}
     for src, calls in enumerate(packets):
+        if not any(calls):  # a rank without partners sends nothing (and skips some exchanges)
+            continue
         if len(calls) != len(names):
             raise RuntimeError(f"rank {src}: {len(calls)} exchanges, expected {len(names)}")
         for nm, call in zip(names, calls):

@@ -3387,14 +3387,6 @@ void Engine::exchange_boxes()
       if (flag[k]) partners_[r].push_back(k);
   }
   nsend_ = pairs_.empty() ? 0 : 1;
-  // exchangeDonors returns at once on a rank without partners
-  // (exchangeDonors.C:36), so its blocks are never hole-cut there; that
-  // case is not reproduced
-  for (int r = 0; r < nr && nsend_; r++)
-    if (partners_[r].empty())
-      for (const Block *b : blocks_)
-        if (b->rank == r)
-          throw std::runtime_error("tioga_amd: a rank whose blocks overlap no other block (not supported)");
   const int np = (int)pairs_.size();
   kscratch_.reserve(6 * (size_t)std::max(np, 1));
   std::vector<unsigned long long> init(6 * (size_t)std::max(np, 1));
@@ -4514,6 +4506,9 @@ void Engine::exchange_donors()
     et.hrem[ib] = b.head_remote.p;
     et.nodeRes[ib] = b.nodeRes.p;
     et.scale[ib] = b.resolutionScale;
+    // exchangeDonors returns at once on a rank without partners
+    // (exchangeDonors.C:36): its blocks are neither hole-cut nor fringed
+    if (partners(b.rank).empty()) continue;
     if (b.nnodes)
       pseg.push_back(ProcSeg{0, b.nnodes, b.tag, b.x.p, b.has_donor.p, b.rmax.p, b.tagmask.p, b.head_key.p,
                              b.head_blk.p, b.iblank.p, b.nodeRes.p, mt.p + noff[ib], mt1.p + noff[ib],
