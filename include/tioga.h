@@ -4,8 +4,8 @@
 // Public methods, argument meaning and defaults follow the reference class
 // (src/tioga.h:47-261).  Private state is an engine handle, so users
 // recompile against this header.  Methods whose feature is outside this
-// engine (high-order and p4est callbacks, the AMR donor callback) are not
-// declared, so a build that needs them fails at compile time.
+// engine (the high-order and p4est callbacks, the AMR donor callback) are
+// not declared, so a build that needs them fails at compile time.
 #ifndef TIOGA_AMD_TIOGA_H
 #define TIOGA_AMD_TIOGA_H
 
@@ -49,7 +49,13 @@ class tioga {
   void registerSolution(int btag, double *q);  // tioga.C:88-93
   void profile(void);                          // tioga.C:95-110
   void performConnectivity(void);              // tioga.C:112-160
-  void dataUpdate(int nvar, int interptype, int at_points = 0);  // tioga.C:336-524
+  // tioga.C:336-524; at_points = 1 updates the point receptors of
+  // performConnectivityHighOrder
+  void dataUpdate(int nvar, int interptype, int at_points = 0);
+  // tioga.C:162-178 without high-order callbacks: the receptor nodes of the
+  // last performConnectivity searched again as points, donors restricted to
+  // cells of cell iblank 1 (highOrder.C:219-419)
+  void performConnectivityHighOrder(void);
   void getDonorCount(int btag, int *dcount, int *fcount);        // tioga.C:533-549
   void getDonorInfo(int btag, int *receptors, int *indices, double *frac, int *dcount);  // tioga.C:551-570
   void getReceptorInfo(std::vector<int> &receptors);  // tioga.C:572-671 (single rank)

@@ -108,10 +108,18 @@ void tioga_register_amr_local_data_(int *ipatch, int *global_id, int *iblank, do
    tioga_dataupdate_mb_ runs dataUpdate_AMR (tiogaInterface.C:194-204). */
 void tioga_performconnectivity_amr_(void);
 
-/* Out of scope of this engine (high-order and p4est callbacks, the AMR donor
-   callback of pdegree > 0 patches); they link, print that the feature is not
-   available and set the error status.  tiogaInterface.C:160-163, 273-300. */
+/* tiogaInterface.C:160-163: tioga::performConnectivityHighOrder
+   (tioga.C:162-178) for a solver without high-order callbacks: the receptor
+   nodes of the last performconnectivity are searched again as points, and
+   each found in a donor cell of cell iblank 1 gets a point donor record
+   (highOrder.C:219-419).  As in the reference, tioga_dataupdate_mb_ then
+   still updates the node receptors (ihighGlobal stays 0 without callbacks);
+   tioga_amd_dataupdate_points runs the point update. */
 void tioga_performconnectivity_highorder_(void);
+
+/* Out of scope of this engine (the high-order and p4est callbacks, the AMR
+   donor callback of pdegree > 0 patches); they link, print that the feature
+   is not available and set the error status.  tiogaInterface.C:273-300. */
 void tioga_set_highorder_callback_(void (*f1)(int *, int *), void (*f2)(int *, int *, double *),
                                    void (*f3)(int *, double *, int *, double *),
                                    void (*f4)(int *, double *, int *, int *, double *, double *, int *),
@@ -210,6 +218,20 @@ int tioga_amd_rank(void);
    per pair that the reference fills of the eighteen it sends).  Returns 0,
    or -1 when src and dst do not exchange. */
 void tioga_amd_record_packets(int on);
+
+/* tioga::dataUpdate(nvar, itype, at_points = 1) (tioga.C:336-524) through
+   the C ABI, which reaches it in the reference only when high-order
+   callbacks are set: after tioga_performconnectivity_highorder_, every
+   point receptor takes its value from its point donor; picked nodes no
+   donor reached become field nodes (iblank 1, highOrder.C:193-206). */
+void tioga_amd_dataupdate_points(int *nvar, char *itype);
+
+/* The point path's arrays of block btag on the current rank, under the
+   reference's member names: "ntotalPoints", "rxyz", "ninterp2",
+   "interp2_inode", "interp2_weights" (nweights per record, record after
+   record) and "interp2_rinfo" (3 per record).  Returns the element count
+   (out may be NULL), -1 for an unknown block or name. */
+long tioga_amd_points_array(int btag, const char *name, void *out);
 int tioga_amd_packet(const char *exchange, int src, int dst, int *ints, double *reals, long long *counts);
 
 /* Batched form (on != 0, the default): the stages of performconnectivity

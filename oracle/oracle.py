@@ -307,13 +307,14 @@ class Oracle:
 def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptype: int = 0,
                isym: int = 3, nfringe: int = 1, mexclude: int = 3,
                moves: Sequence[Sequence[Optional[np.ndarray]]] = (), amr=None, reduce: bool = False,
-               ranks: Optional[Sequence[int]] = None) -> None:
+               ranks: Optional[Sequence[int]] = None, points: bool = False) -> None:
     """amr = (AmrGrid, patch fields per local patch) selects the AMR gather
     mode of the driver (format version 2); reduce runs reduce_fringes after
     each step's connectivity (format version 3); ranks (the MPI rank of each
-    block) selects the multi-rank mode (format version 5)."""
+    block) selects the multi-rank mode (format version 5); points runs the
+    point path after each step (format version 6)."""
     gids = any(b.cell_gid is not None or b.node_gid is not None for b in blocks)
-    version = 5 if ranks is not None else (4 if gids else (3 if reduce else (1 if amr is None else 2)))
+    version = 6 if points else (5 if ranks is not None else (4 if gids else (3 if reduce else (1 if amr is None else 2))))
     with open(path, "wb") as f:
         w = lambda fmt, *v: f.write(struct.pack("<" + fmt, *v))
         w("9i", 0x41474954, version, len(blocks), isym, nfringe, mexclude, nvar, interptype,
@@ -354,8 +355,11 @@ def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptyp
                 f.write(b.cell_gid.tobytes())
                 f.write(b.node_gid.tobytes())
         if version >= 5:
-            w("i", 1)
-            f.write(np.asarray(ranks, np.int32).tobytes())
+            w("i", int(ranks is not None))
+            if ranks is not None:
+                f.write(np.asarray(ranks, np.int32).tobytes())
+        if version >= 6:
+            w("i", int(points))
         for step in moves:
             for xb in step:
                 if xb is None:
@@ -402,7 +406,7 @@ def run_reference(case_path: str, out_path: str, timeout: float = 600.0,
     with open(case_path, "rb") as f:
         version = struct.unpack("<2i", f.read(8))[1]
     if gids is None:
-        gids = version == 4
+        gids = version == 4  # version 5 and 6 cases name it
     drv = REF_DRIVER_GID if gids else REF_DRIVER
     if nranks <= 0:
         subprocess.run([drv, case_path, out_path], check=True, timeout=timeout, env=env, cwd=cwd,

@@ -46,6 +46,13 @@
 //              to out.bin.r<rank>.pk (below).
 // Result file: records {int32 namelen, name, int32 dtype(0=i32,1=f64),
 //              int64 count, payload}.  Step s > 0 prefixes names with "s<s>.".
+//   version 6 adds, after the ranks: int32 points; if set, every step goes on
+//              after its dataUpdate with tioga::performConnectivityHighOrder
+//              (tioga.C:162-178, no high-order callbacks: the receptor nodes
+//              searched again as points), a dump of the point path under
+//              the prefix "hp." (dump_points), and tioga::dataUpdate with
+//              at_points = 1 on a fresh copy of q, whose q and iblank (point
+//              orphans become 1) are dumped as hp.b<i>.q / hp.b<i>.iblank.
 // Packet log (multi-rank mode): every MPI_Isend the reference makes, in call
 //              order, as {int32 1, dest, tag, dtype(0=int,1=double), count,
 //              payload}, and every MPI_Waitall as {int32 2}.  Each
@@ -196,6 +203,42 @@ void dump(tioga *tg, std::vector<Block> &blocks, Writer &w, double tconn, bool g
   }
 }
 
+// The point path of one rank (highOrder.C:219-419): picked points, the
+// point search's arrays and the point donor records (interpList2).
+void dump_points(tioga *tg, std::vector<Block> &blocks, Writer &w)
+{
+  for (size_t b = 0; b < blocks.size(); b++) {
+    Block &B = blocks[b];
+    if (B.rank != tg->myid) continue;
+    MeshBlock *mb = tg->mblocks[tg->tag_iblk_map.find(B.tag)->second].get();
+    std::string p = "b" + std::to_string(b) + ".";
+    int np = mb->ntotalPoints, ns = mb->nsearch, n2 = mb->ninterp2;
+    w.i32(p + "ntotalPoints", &np, 1);
+    w.f64(p + "rxyz", mb->rxyz, 3 * (int64_t)np);
+    w.i32(p + "nsearch", &ns, 1);
+    if (ns > 0) {
+      w.f64(p + "xsearch", mb->xsearch, 3 * (int64_t)ns);
+      w.f64(p + "res_search", mb->res_search, ns);
+      w.i32(p + "isearch", mb->isearch, 3 * (int64_t)ns);
+      w.i32(p + "tagsearch", mb->tagsearch, ns);
+      w.i32(p + "donorId", mb->donorId, ns);
+      w.i32(p + "xtag", mb->xtag, ns);
+    }
+    std::vector<int> inode, rinfo;
+    std::vector<double> wt;
+    for (int i = 0; i < n2; i++) {
+      INTERPLIST &l = mb->interpList2[i];
+      inode.insert(inode.end(), l.inode, l.inode + l.nweights);
+      wt.insert(wt.end(), l.weights, l.weights + l.nweights);
+      rinfo.insert(rinfo.end(), l.receptorInfo, l.receptorInfo + 3);
+    }
+    w.i32(p + "ninterp2", &n2, 1);
+    w.i32(p + "interp2_inode", inode.data(), (int64_t)inode.size());
+    w.f64(p + "interp2_weights", wt.data(), (int64_t)wt.size());
+    w.i32(p + "interp2_rinfo", rinfo.data(), (int64_t)rinfo.size());
+  }
+}
+
 struct Amr {
   int nf = 0, qstride = 0, ngrids = 0, qnodesize = 0, npatch = 0;
   std::vector<double> qnode, rdata;
@@ -305,7 +348,7 @@ int main(int argc, char **argv)
   if (!r.f) { perror(argv[1]); return 2; }
   if (r.get<int32_t>() != 0x41474954) { fprintf(stderr, "bad case\n"); return 2; }
   const int version = r.get<int32_t>();
-  if (version < 1 || version > 5) { fprintf(stderr, "bad case version\n"); return 2; }
+  if (version < 1 || version > 6) { fprintf(stderr, "bad case version\n"); return 2; }
   int nblocks = r.get<int32_t>(), isym = r.get<int32_t>(), nfringe = r.get<int32_t>();
   int mexclude = r.get<int32_t>(), nvar = r.get<int32_t>(), itype = r.get<int32_t>();
   int nsteps = r.get<int32_t>();
@@ -358,6 +401,7 @@ int main(int argc, char **argv)
   const int has_ranks = version >= 5 ? r.get<int32_t>() : 0;
   if (has_ranks)
     for (auto &B : blocks) B.rank = r.get<int32_t>();
+  const int points = version >= 6 ? r.get<int32_t>() : 0;
   int myrank = 0, nprocs = 1;
   MPI_Comm_rank(MPI_COMM_WORLD, &myrank);
   MPI_Comm_size(MPI_COMM_WORLD, &nprocs);
@@ -420,6 +464,24 @@ int main(int argc, char **argv)
     if (times) fprintf(stderr, "ref_driver step %d: dataUpdate %.6f s\n", s, MPI_Wtime() - tu0);
     for (size_t b = 0; b < blocks.size(); b++)
       if (blocks[b].rank == myrank) w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
+    if (points) {
+      const std::string p0 = w.prefix;
+      tg->performConnectivityHighOrder();
+      w.prefix = p0 + "hp.";
+      dump_points(tg, blocks, w);
+      for (size_t b = 0; b < blocks.size(); b++) {
+        if (blocks[b].rank != myrank) continue;
+        qs[b] = blocks[b].q;
+        tg->registerSolution(blocks[b].tag, qs[b].data());
+      }
+      tg->dataUpdate(nvar, itype, 1);
+      for (size_t b = 0; b < blocks.size(); b++) {
+        if (blocks[b].rank != myrank) continue;
+        w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
+        w.i32("b" + std::to_string(b) + ".iblank", blocks[b].iblank.data(), blocks[b].nnodes);
+      }
+      w.prefix = p0;
+    }
     if (reduce) {
       const std::string p0 = w.prefix;
       tg->reduce_fringes();

@@ -28,6 +28,7 @@ class Case:
     moves: List[List[Optional[np.ndarray]]] = field(default_factory=list)
     reduce: bool = False  # tioga::reduce_fringes after each connectivity
     ranks: Optional[List[int]] = None  # MPI rank of each block (multi-rank cases)
+    points: bool = False  # performConnectivityHighOrder and a point update after each step
 
     @property
     def qs(self) -> List[np.ndarray]:
@@ -199,6 +200,12 @@ def build(name: str) -> Case:
     if name == "seam_gid_rf":
         blocks = [W.latlong_shell(12, 8, 5, 0.2, 0.45, 1.1, tag=1, center=(0.02, 0.05, 0.01))] + split_box(12, 6)
         return Case(name, _position_gids(blocks), nvar=2, nfringe=2, reduce=True)
+    # the point path (performConnectivityHighOrder without callbacks, then
+    # dataUpdate at points): receptor nodes searched again as points
+    if name == "points":
+        return Case(name, W.tiny(), nvar=3, points=True)
+    if name == "points_three":
+        return Case(name, three_blocks(), nvar=2, interptype=1, points=True)
     # several MPI ranks (the reference under mpiexec; the engine's virtual ranks)
     if name == "ranks2":
         blocks, ranks = ranked_blocks()
@@ -223,6 +230,9 @@ SMALL_CASES = ["tiny", "tiny_isym0", "tiny_isym1", "tiny_isym2", "mixed", "userr
 REDUCE_CASES = ["tiny_rf", "isym0_rf", "mixed_rf", "three_rf", "moving_rf", "nested_rf", "nested3_rf"]
 GID_CASES = ["tiny_gid", "seam_gid", "seam_gid_rf"]
 ALL_CASES = SMALL_CASES + ["config1"] + REDUCE_CASES + GID_CASES
+# the point path: pinned by the reference directly (the oracle restates the
+# node path only)
+POINT_CASES = ["points", "points_three"]
 # several ranks: pinned by the reference run under mpiexec (the oracle
 # restates the single-rank path only)
 RANK_CASES = ["ranks2", "ranks3", "ranks3_rf", "ranks3_gid", "ranks4_apart"]

@@ -86,13 +86,13 @@ def main(names):
             else:
                 c = cases.build(name)
                 O.write_case(cp, c.blocks, c.qs, c.nvar, c.interptype, c.isym, c.nfringe, c.mexclude,
-                             c.moves, reduce=c.reduce, ranks=c.ranks)
+                             c.moves, reduce=c.reduce, ranks=c.ranks, points=c.points)
             if getattr(c, "ranks", None) is not None:
                 gids = any(b.node_gid is not None for b in c.blocks)
                 res, packets = O.run_reference(cp, op, cwd=td, nranks=max(c.ranks) + 1, gids=gids)
                 res.update(packet_arrays(packets, gids, len(c.moves), c.reduce))
             else:
-                res = O.run_reference(cp, op, cwd=td)
+                res = O.run_reference(cp, op, cwd=td, gids=any(b.node_gid is not None for b in c.blocks))
         meta = {}
         full = {}
         for k, a in sorted(res.items()):
@@ -108,4 +108,4 @@ def main(names):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or cases.ALL_CASES + cases.AMR_CASES + cases.RANK_CASES)
+    main(sys.argv[1:] or cases.ALL_CASES + cases.AMR_CASES + cases.POINT_CASES + cases.RANK_CASES)

@@ -103,6 +103,10 @@ def run_engine_case(case, batched: bool = True):
             if ranked:  # what the exchanges sent (the dataUpdate ran inside results)
                 for ex in conn_ex + ["update"]:
                     out.update(t.packets(ex, pre + ex))
+            if getattr(case, "points", False):  # performConnectivityHighOrder, point update
+                t.connect_points()
+                for k, v in t.points_results(case.qs, case.nvar, case.interptype).items():
+                    out[pre + "hp." + k] = v
             if getattr(case, "reduce", False):
                 t.reduce_fringes()
                 for k, v in t.results(case.qs, case.nvar, case.interptype).items():

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("batched", [True, False], ids=["batched", "per_block"])
-@pytest.mark.parametrize("name", cases.ALL_CASES + cases.RANK_CASES)
+@pytest.mark.parametrize("name", cases.ALL_CASES + cases.POINT_CASES + cases.RANK_CASES)
 def test_engine_matches_reference(name, batched):
     print(f"\n[engine parity] case {name} batched={batched}", flush=True)
     got = run_engine_case(cases.build(name), batched)

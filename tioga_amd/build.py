@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.environ.get("TIOGA_AMD_OUT", os.path.join(HERE, "libtioga_amd.so"))
 BUILD = os.environ.get("TIOGA_AMD_BUILD_DIR", os.path.join(HERE, "_build"))
 
-SOURCES = ["tg_engine.hip", "tg_amr.hip", "tg_adt.hip", "tg_dedupe.hip", "tg_obb.hip", "tg_prims.hip", "tg_update.hip", "tg_seq.cpp", "tg_serve.cpp", "tg_packets.cpp", "tg_capi.cpp"]
+SOURCES = ["tg_engine.hip", "tg_amr.hip", "tg_adt.hip", "tg_dedupe.hip", "tg_obb.hip", "tg_prims.hip", "tg_update.hip", "tg_points.hip", "tg_seq.cpp", "tg_serve.cpp", "tg_packets.cpp", "tg_capi.cpp"]
 ARCH = os.environ.get("TIOGA_AMD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
          "-Wno-unused-function", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -68,9 +68,11 @@ void tioga::performConnectivity(void) { eng_->perform_connectivity(); }
 
 void tioga::dataUpdate(int nvar, int interptype, int at_points)
 {
-  if (at_points) throw std::runtime_error("tioga_amd: dataUpdate at_points=1 (high-order path) is not available");
-  eng_->data_update(nvar, interptype);
+  if (at_points) eng_->data_update_points(nvar, interptype);
+  else eng_->data_update(nvar, interptype);
 }
+
+void tioga::performConnectivityHighOrder(void) { eng_->perform_connectivity_points(); }
 
 void tioga::getDonorCount(int btag, int *dcount, int *fcount) { eng_->get_donor_count(btag, dcount, fcount); }
 
@@ -368,7 +370,26 @@ void tioga_performconnectivity_amr_(void)
   guarded("tioga_performconnectivity_amr", [] { tg_or_throw().performConnectivityAMR(); });
 }
 
-void tioga_performconnectivity_highorder_(void) { not_available("tioga_performconnectivity_highorder"); }
+void tioga_performconnectivity_highorder_(void)
+{
+  guarded("tioga_performconnectivity_highorder", [] { tg_or_throw().performConnectivityHighOrder(); });
+}
+
+void tioga_amd_dataupdate_points(int *nvar, char *itype)
+{
+  guarded("tioga_amd_dataupdate_points", [&] {
+    const int it = parse_itype(itype);
+    if (it < 0) throw std::runtime_error("unknown data orientation");
+    tg_or_throw().dataUpdate(*nvar, it, 1);
+  });
+}
+
+long tioga_amd_points_array(int btag, const char *name, void *out)
+{
+  long r = -1;
+  guarded("tioga_amd_points_array", [&] { r = tg_or_throw().engine()->points_array(btag, name, out); });
+  return r;
+}
 void tioga_set_highorder_callback_(void (*)(int *, int *), void (*)(int *, int *, double *),
                                    void (*)(int *, double *, int *, double *),
                                    void (*)(int *, double *, int *, int *, double *, double *, int *),

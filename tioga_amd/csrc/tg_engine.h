@@ -245,6 +245,13 @@ struct Block {
   std::vector<InterpHost> interp_host;
   DBuf<unsigned char> upd_mark;  // dataUpdate's donor-vertex marks
   bool interp_host_valid = false;
+  // point receptors (performConnectivityHighOrder without callbacks,
+  // tg_points.hip): the picked nodes and their coordinates, and the point
+  // donor records of this block (interpList2): vertices and weights
+  // (slot m of record r at m * ninterp2 + r), receptor point and block
+  int npts = 0, ninterp2 = 0;
+  DBuf<int> pnode, p_inode, p_nw, p_rpt, p_rblk;
+  DBuf<double> pxyz, p_w;
 };
 
 // A standalone donor search: nq points against one block, with the search
@@ -558,6 +565,15 @@ class Engine {
   PacketLog pk_;
   void interp_records(Block &b);
   void record_update(int nvar, int interptype);
+  // tioga::performConnectivityHighOrder and dataUpdate(at_points = 1) for a
+  // solver without high-order callbacks (tg_points.hip)
+  void perform_connectivity_points();
+  void data_update_points(int nvar, int interptype);
+  long points_array(int btag, const std::string &name, void *out);
+  bool points_ready_ = false;
+  DBuf<int> plist_, psel_;
+  std::vector<DBuf<double> *> ptmp_;
+  std::vector<DBuf<int> *> pitmp_;
   // the search structure over every cell of block btag (after preprocess)
   void build_resident_tree(int btag, ResidentTree &t);
   // searches against block btag walk t (owned by another engine) from now on

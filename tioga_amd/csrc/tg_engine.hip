@@ -2696,6 +2696,8 @@ Engine::~Engine()
   query_set_clear();
   for (Block *b : blocks_) delete b;
   for (auto *q : qdev_) delete q;
+  for (auto *q : ptmp_) delete q;
+  for (auto *q : pitmp_) delete q;
   amr_release();
   if (s_) (void)hipStreamDestroy(s_);
   if (s2_) (void)hipStreamDestroy(s2_);
@@ -4718,6 +4720,7 @@ void Engine::perform_connectivity()
   auto now = [] { return std::chrono::steady_clock::now(); };
   auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
   auto t0 = now();
+  points_ready_ = false;
   get_hole_map();
   auto t1 = now();
   exchange_boxes();

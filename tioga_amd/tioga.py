@@ -105,6 +105,8 @@ def load():
         L.tioga_amd_serve_close.argtypes = []
         L.tioga_amd_set_rank.argtypes = [C.c_int]
         L.tioga_amd_rank.restype = C.c_int
+        L.tioga_amd_points_array.argtypes = [C.c_int, C.c_char_p, _P]
+        L.tioga_amd_points_array.restype = C.c_long
         L.tioga_amd_record_packets.argtypes = [C.c_int]
         L.tioga_amd_packet.argtypes = [C.c_char_p, C.c_int, C.c_int, _P, _P, _P]
         L.tioga_amd_packet.restype = C.c_int
@@ -255,6 +257,55 @@ class Tioga:
     def connect(self):
         self.L.tioga_performconnectivity_()
         self._check()
+
+    def connect_points(self):
+        """tioga_performconnectivity_highorder_: the receptor nodes searched
+        again as points (no high-order callbacks)."""
+        self.L.tioga_performconnectivity_highorder_()
+        self._check()
+
+    def data_update_points(self, nvar: int, interptype: int = 0):
+        """dataUpdate(nvar, itype, at_points=1) (tioga_amd_dataupdate_points)."""
+        self.L.tioga_amd_dataupdate_points(_ip(nvar), b"row" if interptype == 0 else b"column")
+        self._check()
+
+    def points_array(self, btag: int, name: str) -> np.ndarray:
+        n = self.L.tioga_amd_points_array(btag, name.encode(), None)
+        self._check()
+        out = np.zeros(max(n, 0), np.float64 if name in ("rxyz", "interp2_weights") else np.int32)
+        if n > 0:
+            self.L.tioga_amd_points_array(btag, name.encode(), _p(out))
+            self._check()
+        return out
+
+    def points_results(self, qs, nvar: int, interptype: int = 0) -> Dict[str, np.ndarray]:
+        """The point path's arrays as the reference driver dumps them (hp.*):
+        per block the picked points, the point search arrays and the point
+        donor records, then q and iblank after a point update of fresh
+        copies of qs."""
+        r: Dict[str, np.ndarray] = {}
+        for i, b in enumerate(self.blocks):
+            self.set_rank(self.ranks[i])
+            p = f"b{i}."
+            for nm in ("ntotalPoints", "rxyz"):
+                r[p + nm] = self.points_array(b.tag, nm)
+            for nm in ("xsearch", "res_search", "isearch", "tagsearch", "donorId", "xtag"):
+                a = self.array(b.tag, nm)
+                if a.size:
+                    r[p + nm] = a
+            r[p + "nsearch"] = np.array([self.array(b.tag, "donorId").size], np.int32)
+            for nm in ("ninterp2", "interp2_inode", "interp2_weights", "interp2_rinfo"):
+                r[p + nm] = self.points_array(b.tag, nm)
+        out = [np.ascontiguousarray(q, np.float64).copy() for q in qs]
+        for b, rk, q in zip(self.blocks, self.ranks, out):
+            self.set_rank(rk)
+            self.register_solution(b.tag, q)
+        self.data_update_points(nvar, interptype)
+        for i in range(len(self.blocks)):
+            r[f"b{i}.q"] = out[i]
+            r[f"b{i}.iblank"] = self.iblank[i].copy()
+        self.set_rank(0)
+        return r
 
     def reduce_fringes(self):
         """tioga_reduce_fringes_ (tiogaInterface.C:302-305): thin the fringes
