@@ -4,8 +4,8 @@
 // Public methods, argument meaning and defaults follow the reference class
 // (src/tioga.h:47-261).  Private state is an engine handle, so users
 // recompile against this header.  Methods whose feature is outside this
-// engine (the high-order and p4est callbacks, the AMR donor callback) are
-// not declared, so a build that needs them fails at compile time.
+// engine (the p4est callbacks, the AMR donor callback) are not declared, so
+// a build that needs them fails at compile time.
 #ifndef TIOGA_AMD_TIOGA_H
 #define TIOGA_AMD_TIOGA_H
 
@@ -52,10 +52,18 @@ class tioga {
   // tioga.C:336-524; at_points = 1 updates the point receptors of
   // performConnectivityHighOrder
   void dataUpdate(int nvar, int interptype, int at_points = 0);
-  // tioga.C:162-178 without high-order callbacks: the receptor nodes of the
-  // last performConnectivity searched again as points, donors restricted to
-  // cells of cell iblank 1 (highOrder.C:219-419)
+  // tioga.C:162-178: without high-order callbacks the receptor nodes of the
+  // last performConnectivity are searched again as points, with them the
+  // callbacks' points of the receptor cells; donors are cells of cell
+  // iblank 1 (highOrder.C:219-419)
   void performConnectivityHighOrder(void);
+  // tioga.h:210-222: the solver's high-order callbacks (host functions),
+  // called where the reference calls them: get_nodes_per_cell,
+  // get_receptor_nodes, donor_inclusion_test, donor_frac, convert_to_modal
+  void setcallback(void (*f1)(int *, int *), void (*f2)(int *, int *, double *),
+                   void (*f3)(int *, double *, int *, double *),
+                   void (*f4)(int *, double *, int *, int *, double *, double *, int *),
+                   void (*f5)(int *, int *, double *, int *, int *, double *));
   void getDonorCount(int btag, int *dcount, int *fcount);        // tioga.C:533-549
   void getDonorInfo(int btag, int *receptors, int *indices, double *frac, int *dcount);  // tioga.C:551-570
   void getReceptorInfo(std::vector<int> &receptors);  // tioga.C:572-671 (single rank)

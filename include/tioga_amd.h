@@ -109,21 +109,29 @@ void tioga_register_amr_local_data_(int *ipatch, int *global_id, int *iblank, do
 void tioga_performconnectivity_amr_(void);
 
 /* tiogaInterface.C:160-163: tioga::performConnectivityHighOrder
-   (tioga.C:162-178) for a solver without high-order callbacks: the receptor
-   nodes of the last performconnectivity are searched again as points, and
-   each found in a donor cell of cell iblank 1 gets a point donor record
-   (highOrder.C:219-419).  As in the reference, tioga_dataupdate_mb_ then
-   still updates the node receptors (ihighGlobal stays 0 without callbacks);
-   tioga_amd_dataupdate_points runs the point update. */
+   (tioga.C:162-178).  Without high-order callbacks the receptor nodes of the
+   last performconnectivity are searched again as points, and each found in
+   a donor cell of cell iblank 1 gets a point donor record with its
+   isoparametric weights (highOrder.C:219-419); tioga_dataupdate_mb_ then
+   still updates the node receptors (ihighGlobal stays 0 without callbacks)
+   and tioga_amd_dataupdate_points runs the point update. */
 void tioga_performconnectivity_highorder_(void);
 
-/* Out of scope of this engine (the high-order and p4est callbacks, the AMR
-   donor callback of pdegree > 0 patches); they link, print that the feature
-   is not available and set the error status.  tiogaInterface.C:273-300. */
+/* tiogaInterface.C:273-284: the solver's high-order callbacks, host
+   functions the engine calls where the reference calls them:
+   get_nodes_per_cell and get_receptor_nodes for the points of every cell of
+   cell iblank -1, donor_inclusion_test on each point's candidate cells in
+   the reference's walk order (the device search lists them), donor_frac
+   per point donor, convert_to_modal per updated cell.  After the next
+   performconnectivity, tioga_dataupdate_mb_ runs the point update
+   (tiogaInterface.C:207-216), row layout only as in the reference. */
 void tioga_set_highorder_callback_(void (*f1)(int *, int *), void (*f2)(int *, int *, double *),
                                    void (*f3)(int *, double *, int *, double *),
                                    void (*f4)(int *, double *, int *, int *, double *, double *, int *),
                                    void (*f5)(int *, int *, double *, int *, int *, double *));
+/* Out of scope of this engine (the p4est callbacks, the AMR donor callback
+   of pdegree > 0 patches); they link, print that the feature is not
+   available and set the error status.  tiogaInterface.C:286-300. */
 void tioga_set_p4est_(void);
 void tioga_set_amr_callback_(void (*f1)(int *, double *, int *, double *));
 void tioga_set_p4est_search_callback_(void (*f1)(double *xsearch, int *process_id, int *cell_id, int *npts),

@@ -307,7 +307,7 @@ class Oracle:
 def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptype: int = 0,
                isym: int = 3, nfringe: int = 1, mexclude: int = 3,
                moves: Sequence[Sequence[Optional[np.ndarray]]] = (), amr=None, reduce: bool = False,
-               ranks: Optional[Sequence[int]] = None, points: bool = False) -> None:
+               ranks: Optional[Sequence[int]] = None, points: int = 0) -> None:
     """amr = (AmrGrid, patch fields per local patch) selects the AMR gather
     mode of the driver (format version 2); reduce runs reduce_fringes after
     each step's connectivity (format version 3); ranks (the MPI rank of each
@@ -359,7 +359,7 @@ def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptyp
             if ranks is not None:
                 f.write(np.asarray(ranks, np.int32).tobytes())
         if version >= 6:
-            w("i", int(points))
+            w("i", int(points))  # 1: the point path, 2: with the test callbacks
         for step in moves:
             for xb in step:
                 if xb is None:

@@ -48,11 +48,16 @@
 //              int64 count, payload}.  Step s > 0 prefixes names with "s<s>.".
 //   version 6 adds, after the ranks: int32 points; if set, every step goes on
 //              after its dataUpdate with tioga::performConnectivityHighOrder
-//              (tioga.C:162-178, no high-order callbacks: the receptor nodes
-//              searched again as points), a dump of the point path under
-//              the prefix "hp." (dump_points), and tioga::dataUpdate with
-//              at_points = 1 on a fresh copy of q, whose q and iblank (point
-//              orphans become 1) are dumped as hp.b<i>.q / hp.b<i>.iblank.
+//              (tioga.C:162-178), a dump of the point path under the prefix
+//              "hp." (dump_points), and tioga::dataUpdate with at_points = 1
+//              on a fresh copy of q, whose q, iblank and iblank_cell are
+//              dumped as hp.b<i>.q / .iblank / .iblank_cell.  points = 1: no
+//              high-order callbacks (the receptor nodes searched again as
+//              points); points = 2: the callbacks of tests/cxx/ho_callbacks.c
+//              are set (tioga::setcallback) on a case of two single-type
+//              blocks, block 0 the receptor mesh and block 1 the donor mesh,
+//              and the node dataUpdate is left out (with callbacks the C ABI
+//              sends dataUpdate to the point update).
 // Packet log (multi-rank mode): every MPI_Isend the reference makes, in call
 //              order, as {int32 1, dest, tag, dtype(0=int,1=double), count,
 //              payload}, and every MPI_Waitall as {int32 2}.  Each
@@ -75,6 +80,16 @@
 #undef private
 
 using namespace TIOGA;
+
+extern "C" {  // tests/cxx/ho_callbacks.c
+void ho_cb_set(const double *rx, const int *rconn, int rnv, const double *dx, const int *dconn, int dnv, int dnnodes,
+               int nvar);
+void ho_nodes_per_cell(int *icell, int *npts);
+void ho_receptor_nodes(int *icell, int *npts, double *xyz);
+void ho_inclusion(int *icell, double *x, int *pass, double *rst);
+void ho_frac(int *icell, double *x, int *nweights, int *inode, double *frac, double *rst, int *ndim);
+void ho_modal(int *icell, int *npts_cell, double *qtmp, int *npts, int *index_out, double *qout);
+}
 
 namespace {
 FILE *g_pk = nullptr;  // packet log of this rank (multi-rank mode)
@@ -228,7 +243,8 @@ void dump_points(tioga *tg, std::vector<Block> &blocks, Writer &w)
     std::vector<double> wt;
     for (int i = 0; i < n2; i++) {
       INTERPLIST &l = mb->interpList2[i];
-      inode.insert(inode.end(), l.inode, l.inode + l.nweights);
+      // with callbacks a record holds one solution index (highOrder.C:365)
+      inode.insert(inode.end(), l.inode, l.inode + (mb->ihigh ? 1 : l.nweights));
       wt.insert(wt.end(), l.weights, l.weights + l.nweights);
       rinfo.insert(rinfo.end(), l.receptorInfo, l.receptorInfo + 3);
     }
@@ -427,6 +443,15 @@ int main(int argc, char **argv)
     if (B.has_res) tg->setResolutions(B.tag, B.nres.data(), B.cres.data());
     tg->set_cell_iblank(B.tag, B.iblank_cell.data());
   }
+  if (points == 2) {
+    if (blocks.size() != 2 || blocks[0].ntypes != 1 || blocks[1].ntypes != 1 || has_ranks) {
+      fprintf(stderr, "ref_driver: callback cases have two single-type blocks on one rank\n");
+      return 2;
+    }
+    ho_cb_set(blocks[0].x.data(), blocks[0].conn[0].data(), blocks[0].nv[0], blocks[1].x.data(),
+              blocks[1].conn[0].data(), blocks[1].nv[0], blocks[1].nnodes, nvar);
+    tg->setcallback(ho_nodes_per_cell, ho_receptor_nodes, ho_inclusion, ho_frac, ho_modal);
+  }
   if (has_amr) {
     if (has_ranks) { fprintf(stderr, "ref_driver: AMR cases run on one rank\n"); return 2; }
     run_amr(tg, blocks, amr, nvar, itype, w);
@@ -454,15 +479,15 @@ int main(int argc, char **argv)
     dump(tg, blocks, w, tconn, has_gids != 0);
     // dataUpdate on a fresh copy of q so every step starts from the same field
     std::vector<std::vector<double>> qs(blocks.size());
-    for (size_t b = 0; b < blocks.size(); b++) {
+    for (size_t b = 0; b < blocks.size() && points != 2; b++) {
       if (blocks[b].rank != myrank) continue;
       qs[b] = blocks[b].q;
       tg->registerSolution(blocks[b].tag, qs[b].data());
     }
     const double tu0 = MPI_Wtime();
-    tg->dataUpdate(nvar, itype);
+    if (points != 2) tg->dataUpdate(nvar, itype);
     if (times) fprintf(stderr, "ref_driver step %d: dataUpdate %.6f s\n", s, MPI_Wtime() - tu0);
-    for (size_t b = 0; b < blocks.size(); b++)
+    for (size_t b = 0; b < blocks.size() && points != 2; b++)
       if (blocks[b].rank == myrank) w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
     if (points) {
       const std::string p0 = w.prefix;
@@ -479,6 +504,7 @@ int main(int argc, char **argv)
         if (blocks[b].rank != myrank) continue;
         w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
         w.i32("b" + std::to_string(b) + ".iblank", blocks[b].iblank.data(), blocks[b].nnodes);
+        w.i32("b" + std::to_string(b) + ".iblank_cell", blocks[b].iblank_cell.data(), blocks[b].ncells);
       }
       w.prefix = p0;
     }

@@ -28,7 +28,10 @@ class Case:
     moves: List[List[Optional[np.ndarray]]] = field(default_factory=list)
     reduce: bool = False  # tioga::reduce_fringes after each connectivity
     ranks: Optional[List[int]] = None  # MPI rank of each block (multi-rank cases)
-    points: bool = False  # performConnectivityHighOrder and a point update after each step
+    # performConnectivityHighOrder and a point update after each step: 1
+    # without callbacks, 2 with tests/cxx/ho_callbacks.c (two single-type
+    # blocks, block 0 the receptor mesh, block 1 the donor mesh)
+    points: int = 0
 
     @property
     def qs(self) -> List[np.ndarray]:
@@ -203,9 +206,13 @@ def build(name: str) -> Case:
     # the point path (performConnectivityHighOrder without callbacks, then
     # dataUpdate at points): receptor nodes searched again as points
     if name == "points":
-        return Case(name, W.tiny(), nvar=3, points=True)
+        return Case(name, W.tiny(), nvar=3, points=1)
     if name == "points_three":
-        return Case(name, three_blocks(), nvar=2, interptype=1, points=True)
+        return Case(name, three_blocks(), nvar=2, interptype=1, points=1)
+    if name == "points_cb":  # high-order callbacks
+        return Case(name, W.tiny(), nvar=3, points=2)
+    if name == "points_cb_jitter":  # callbacks; overlapping donor-cell boxes in a jittered box
+        return Case(name, [W.tiny()[0], W.cart_box(10, tag=2, jitter=0.15, seed=8)], nvar=2, nfringe=2, points=2)
     # several MPI ranks (the reference under mpiexec; the engine's virtual ranks)
     if name == "ranks2":
         blocks, ranks = ranked_blocks()
@@ -232,7 +239,7 @@ GID_CASES = ["tiny_gid", "seam_gid", "seam_gid_rf"]
 ALL_CASES = SMALL_CASES + ["config1"] + REDUCE_CASES + GID_CASES
 # the point path: pinned by the reference directly (the oracle restates the
 # node path only)
-POINT_CASES = ["points", "points_three"]
+POINT_CASES = ["points", "points_three", "points_cb", "points_cb_jitter"]
 # several ranks: pinned by the reference run under mpiexec (the oracle
 # restates the single-rank path only)
 RANK_CASES = ["ranks2", "ranks3", "ranks3_rf", "ranks3_gid", "ranks4_apart"]

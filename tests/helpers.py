@@ -72,6 +72,27 @@ def compare_to_golden(name: str, got: dict, skip=(), skip_prefix=()):
     return bad
 
 
+def set_test_callbacks(t, case):
+    """Hand the engine the high-order callbacks of tests/cxx/ho_callbacks.c
+    (the ones the reference driver sets), on block 0 as the receptor mesh and
+    block 1 as the donor mesh; returns what must stay alive."""
+    import ctypes as C
+
+    lib = C.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "cxx", "libho_callbacks.so"))
+    r, d = case.blocks[0].normalized(), case.blocks[1].normalized()
+    keep = [lib, r, d]
+    P = C.c_void_p
+    lib.ho_cb_set.argtypes = [P, P, C.c_int, P, P, C.c_int, C.c_int, C.c_int]
+    lib.ho_cb_set(r.x.ctypes.data, r.types[0][1].ctypes.data, r.types[0][0], d.x.ctypes.data,
+                  d.types[0][1].ctypes.data, d.types[0][0], d.nnodes, case.nvar)
+    fns = [C.cast(getattr(lib, n), P) for n in
+           ("ho_nodes_per_cell", "ho_receptor_nodes", "ho_inclusion", "ho_frac", "ho_modal")]
+    t.L.tioga_set_highorder_callback_.argtypes = [P] * 5
+    t.L.tioga_set_highorder_callback_(*fns)
+    t._check()
+    return keep
+
+
 def run_engine_case(case, batched: bool = True):
     """Every array ref_driver.cpp dumps, computed by the HIP engine through
     its C ABI, in the driver's call sequence (batched or block-by-block form)."""
@@ -87,6 +108,8 @@ def run_engine_case(case, batched: bool = True):
         gids = any(b.node_gid is not None for b in case.blocks)
         for b, r in zip(case.blocks, ranks):
             t.register_grid(b, rank=r)
+        callbacks = getattr(case, "points", 0) == 2
+        keep = set_test_callbacks(t, case) if callbacks else None
         if ranked:
             t.record_packets(True)
         conn_ex = ["boxes", "search", "donors1", "donors2", "donors3", "donors4"] + (["receptor_info"] if gids else [])
@@ -98,7 +121,8 @@ def run_engine_case(case, batched: bool = True):
             t.preprocess()
             t.connect()
             pre = "" if s == 0 else f"s{s}."
-            for k, v in t.results(case.qs, case.nvar, case.interptype).items():
+            # with callbacks the C ABI's dataupdate is the point update: no node update here
+            for k, v in t.results(None if callbacks else case.qs, case.nvar, case.interptype).items():
                 out[pre + k] = v
             if ranked:  # what the exchanges sent (the dataUpdate ran inside results)
                 for ex in conn_ex + ["update"]:
@@ -115,6 +139,7 @@ def run_engine_case(case, batched: bool = True):
                     for ex in ["rf.donors3", "donors4"] + (["receptor_info"] if gids else []) + ["update"]:
                         out.update(t.packets(ex, pre + ("rf." + ex if not ex.startswith("rf.") else ex)))
         out["weight_aborts"] = t.weight_aborts()
+        del keep
     finally:
         t.close()
     return out

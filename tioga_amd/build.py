@@ -78,7 +78,26 @@ def build_cxx_driver(verbose: bool = False) -> str:
     return CXX_DRIVER
 
 
+HO_CB_SRC = os.path.join(ROOT, "tests", "cxx", "ho_callbacks.c")
+HO_CB_LIB = os.path.join(ROOT, "tests", "cxx", "libho_callbacks.so")
+
+
+def build_test_callbacks(verbose: bool = False) -> str:
+    """The high-order callbacks of the parity cases (tests/cxx/ho_callbacks.c),
+    the same source the reference driver links, as a library the tests hand
+    to tioga_set_highorder_callback_."""
+    cmd = ["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", HO_CB_SRC, "-o", HO_CB_LIB + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(HO_CB_LIB + ".tmp", HO_CB_LIB)
+    if verbose:
+        print("built", HO_CB_LIB)
+    return HO_CB_LIB
+
+
 if __name__ == "__main__":
     build(verbose=True)
     build_cxx_driver(verbose=True)
+    build_test_callbacks(verbose=True)
     sys.exit(0)

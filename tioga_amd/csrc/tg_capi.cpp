@@ -74,6 +74,21 @@ void tioga::dataUpdate(int nvar, int interptype, int at_points)
 
 void tioga::performConnectivityHighOrder(void) { eng_->perform_connectivity_points(); }
 
+void tioga::setcallback(void (*f1)(int *, int *), void (*f2)(int *, int *, double *),
+                        void (*f3)(int *, double *, int *, double *),
+                        void (*f4)(int *, double *, int *, int *, double *, double *, int *),
+                        void (*f5)(int *, int *, double *, int *, int *, double *))
+{
+  if (!f1 || !f2 || !f3 || !f4 || !f5) throw std::runtime_error("tioga_amd: a high-order callback is null");
+  tg::HighOrderCallbacks &h = eng_->ho_;
+  h.nodes_per_cell = f1;
+  h.receptor_nodes = f2;
+  h.inclusion = f3;
+  h.frac = f4;
+  h.modal = f5;
+  h.set = true;
+}
+
 void tioga::getDonorCount(int btag, int *dcount, int *fcount) { eng_->get_donor_count(btag, dcount, fcount); }
 
 void tioga::getDonorInfo(int btag, int *receptors, int *indices, double *frac, int *dcount)
@@ -232,11 +247,17 @@ void tioga_dataupdate_mb_(int *nvar, char *itype)
       printf("#tiogaInterface.C:dataupdate_:unknown data orientation\n");
       return;
     }
-    // tiogaInterface.C:194-204
-    if (tg_or_throw().iamrGlobal())
+    // tiogaInterface.C:194-216
+    if (tg_or_throw().engine()->ho_global_) {
+      if (tg_or_throw().iamrGlobal())
+        printf("Data udpate between high-order near-body and AMR cartesian Not implemented yet\n");
+      else
+        tg_or_throw().dataUpdate(*nvar, t, 1);
+    } else if (tg_or_throw().iamrGlobal()) {
       tg_or_throw().dataUpdate_AMR(*nvar, t);
-    else
+    } else {
       tg_or_throw().dataUpdate(*nvar, t);
+    }
   });
 }
 
@@ -390,12 +411,12 @@ long tioga_amd_points_array(int btag, const char *name, void *out)
   guarded("tioga_amd_points_array", [&] { r = tg_or_throw().engine()->points_array(btag, name, out); });
   return r;
 }
-void tioga_set_highorder_callback_(void (*)(int *, int *), void (*)(int *, int *, double *),
-                                   void (*)(int *, double *, int *, double *),
-                                   void (*)(int *, double *, int *, int *, double *, double *, int *),
-                                   void (*)(int *, int *, double *, int *, int *, double *))
+void tioga_set_highorder_callback_(void (*f1)(int *, int *), void (*f2)(int *, int *, double *),
+                                   void (*f3)(int *, double *, int *, double *),
+                                   void (*f4)(int *, double *, int *, int *, double *, double *, int *),
+                                   void (*f5)(int *, int *, double *, int *, int *, double *))
 {
-  not_available("tioga_set_highorder_callback");
+  guarded("tioga_set_highorder_callback", [&] { tg_or_throw().setcallback(f1, f2, f3, f4, f5); });
 }
 void tioga_set_p4est_(void) { not_available("tioga_set_p4est"); }
 void tioga_set_amr_callback_(void (*)(int *, double *, int *, double *)) { not_available("tioga_set_amr_callback"); }

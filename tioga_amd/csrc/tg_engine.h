@@ -252,6 +252,22 @@ struct Block {
   int npts = 0, ninterp2 = 0;
   DBuf<int> pnode, p_inode, p_nw, p_rpt, p_rblk;
   DBuf<double> pxyz, p_w;
+  // with high-order callbacks (host): receptor cells (1-based), their point
+  // counts and points, and the point donor records: first solution index
+  // (1-based), weights at h2_woff[r] .. h2_woff[r + 1], receptor point and block
+  std::vector<int> hctag, hppc, h2_inode, h2_woff, h2_rpt, h2_rblk;
+  std::vector<double> hrxyz, h2_w;
+};
+
+// The solver's high-order callbacks (tiogaInterface.C:273-277, MeshBlock.h:
+// 78-83), host functions.
+struct HighOrderCallbacks {
+  void (*nodes_per_cell)(int *, int *) = nullptr;
+  void (*receptor_nodes)(int *, int *, double *) = nullptr;
+  void (*inclusion)(int *, double *, int *, double *) = nullptr;
+  void (*frac)(int *, double *, int *, int *, double *, double *, int *) = nullptr;
+  void (*modal)(int *, int *, double *, int *, int *, double *) = nullptr;
+  bool set = false;
 };
 
 // A standalone donor search: nq points against one block, with the search
@@ -571,6 +587,13 @@ class Engine {
   void data_update_points(int nvar, int interptype);
   long points_array(int btag, const std::string &name, void *out);
   bool points_ready_ = false;
+  // the high-order callbacks, and the reference's ihighGlobal: a
+  // performConnectivity has run with them set (tioga.C:138), which sends
+  // tioga_dataupdate_mb_ to the point update (tiogaInterface.C:207-216)
+  HighOrderCallbacks ho_;
+  bool ho_global_ = false;
+  void points_by_callbacks(Block &D);
+  void data_update_points_callbacks(int nvar, int interptype);
   DBuf<int> plist_, psel_;
   std::vector<DBuf<double> *> ptmp_;
   std::vector<DBuf<int> *> pitmp_;
@@ -715,6 +738,7 @@ class Engine {
   void cell_iblanks(Block &b);
   void launch_search(Block &b);
   void launch_searches();
+  void list_candidates(Block &b, std::vector<int> &qi, std::vector<int> &rank, std::vector<int> &cell);
 
   void build_search_structure(Block &b, const OBB &obq, int &nelem, int &depth, DBuf<int> &elist,
                               DBuf<double> &ebox, DBuf<int> &adtInt, DBuf<double> &adtReal, DBuf<WalkNode> &nodes,

@@ -2530,7 +2530,7 @@ void fill(T *p, size_t n, T v, hipStream_t s)
 // path, then the queries that overflowed it); with check, the host reads
 // the candidate count after the walk and walks again with a larger list
 // when it did not fit, so no later kernel reads a list that overflowed
-void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bool check)
+void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bool check, bool list_only = false)
 {
   std::vector<int> qb(ns + 1, 0);
   for (int k = 0; k < ns; k++) {
@@ -2600,6 +2600,7 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
         continue;
       }
     }
+    if (list_only) return;
     w.bins.reserve(8);
     w.blist.reserve(w.cap);
     fill<int>(w.bins.p, 8, 0, s);
@@ -2636,6 +2637,20 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     k_select<<<nb(nqt), BS, 0, s>>>(T, O);
     return;
   }
+}
+
+// the listed candidates of a single search: search point index, preorder
+// rank and cell of each (the element-box test is left to the caller)
+__global__ void k_export_cands(int n, SearchSet T, CandOut O, int *__restrict__ qi, int *__restrict__ rank,
+                               int *__restrict__ cell)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const SearchView S = view_of(T, 0);
+  const int v = O.cell[j];
+  qi[j] = S.perm[O.query[j]];
+  rank[j] = v & 0x0fffffff;
+  cell[j] = cell_of(S, v);
 }
 
 // a search's bin grid, when one was built
@@ -4341,6 +4356,55 @@ void Engine::launch_search(Block &b)
   }
 }
 
+// The candidates the device search lists for block b's query set (after
+// search(b) built its tree): per candidate its search point, preorder rank
+// and cell, for a containment test outside the engine (the high-order
+// callbacks).  Points outside the tree's root box and coincident copies list
+// none.
+void Engine::list_candidates(Block &b, std::vector<int> &qi, std::vector<int> &rank, std::vector<int> &cell)
+{
+  qi.clear();
+  rank.clear();
+  cell.clear();
+  if (b.nsearch == 0 || b.nelem == 0) return;
+  MortonSet ms{b.xsearch.p, b.xtag.p, b.donorId.p, {}, 0, b.nsearch};
+  memcpy(ms.ext, b.adtExt, sizeof ms.ext);
+  SearchView S;
+  S.nq = b.nsearch;
+  S.perm = order_sets({ms}, b.nsearch);
+  S.xs = b.xsearch.p;
+  S.xtag = b.xtag.p;
+  S.donorId = b.donorId.p;
+  S.nodes = b.nodes.p;
+  S.rcell = b.rcell.p;
+  view_bins(S, b.bins);
+  S.g = grid_frame(b.adtExt);
+  memcpy(S.ext, b.adtExt, sizeof S.ext);
+  S.conn = b.conn.p;
+  S.x = b.x.p;
+  S.cellRes = b.cellRes.p;
+  run_search(&S, 1, sw_, s_, true, true);
+  int used = 0;
+  TG_HIP(hipMemcpyAsync(&used, sw_.counters.p, sizeof used, hipMemcpyDeviceToHost, s_));
+  sync();
+  if (used <= 0) return;
+  DBuf<int> dq, dr, dc;
+  dq.reserve(used);
+  dr.reserve(used);
+  dc.reserve(used);
+  const SearchSet T{(const SearchView *)sw_.sv.p, sw_.qb.p, sw_.nin.p, sw_.wstart.p, sw_.qsid.p, 1, b.nsearch};
+  CandOut O{sw_.cell.p, sw_.query.p, sw_.status.p, sw_.counters.p, (int)std::min<size_t>(sw_.cap, 0x7fffffff),
+            sw_.best.p, sw_.last.p, nullptr};
+  k_export_cands<<<nb(used), BS, 0, s_>>>(used, T, O, dq.p, dr.p, dc.p);
+  qi.resize(used);
+  rank.resize(used);
+  cell.resize(used);
+  TG_HIP(hipMemcpyAsync(qi.data(), dq.p, sizeof(int) * used, hipMemcpyDeviceToHost, s_));
+  TG_HIP(hipMemcpyAsync(rank.data(), dr.p, sizeof(int) * used, hipMemcpyDeviceToHost, s_));
+  TG_HIP(hipMemcpyAsync(cell.data(), dc.p, sizeof(int) * used, hipMemcpyDeviceToHost, s_));
+  sync();
+}
+
 // every block's donor search in one launch of each search kernel: each
 // block's queries in Morton order within its own range of one buffer
 void Engine::launch_searches()
@@ -4889,6 +4953,7 @@ void Engine::perform_connectivity()
   t.obb_stepped = obb_stepped_;
   last_times = t;
   connected_ = true;
+  ho_global_ = ho_.set;
   prepare_update();
 }
 

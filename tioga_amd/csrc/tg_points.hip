@@ -23,7 +23,25 @@
 //      MeshBlock::clearOrphans (highOrder.C:193-206: a picked node no record
 //      reached gets iblank 1) and MeshBlock::updatePointData
 //      (highOrder.C:544-556: a picked node still at -1 takes its values).
+//
+// With the solver's high-order callbacks set (tioga_set_highorder_callback_)
+// the same stages call them, on the host, where the reference does:
+//   1. the points of each cell with cell iblank -1 (get_nodes_per_cell for
+//      every such cell, then get_receptor_nodes, highOrder.C:220-245);
+//   3. the device search lists each point's candidate cells; for each point
+//      in order the host tests them in preorder rank with the exact element
+//      box and donor_inclusion_test, up to the first that passes
+//      (searchADTrecursion.C:52-101, checkContainment.C:86-93): the
+//      reference's call sequence.  A coincident copy takes its
+//      representative's donor and rst (the reference leaves its rst unset);
+//   4. donor_frac per point donor (highOrder.C:363-383);
+//   5. the values sum q[first + m * nvar + k] * w_m (highOrder.C:445-469,
+//      row layout only, as in the reference); clearOrphans marks a receptor
+//      cell 0 when its last orphan point lies in another mesh's hole map,
+//      else 1 (highOrder.C:157-192); convert_to_modal writes the values of
+//      cells still at -1 (highOrder.C:527-543).
 #include <algorithm>
+#include <cstring>
 #include <stdexcept>
 
 #include "tg_engine.h"
@@ -200,13 +218,39 @@ void Engine::perform_connectivity_points()
 {
   if (!connected_) throw std::runtime_error("tioga_amd: performConnectivityHighOrder needs performconnectivity first");
   const int nbk = (int)blocks_.size();
+  const bool cb = ho_.set;
   // 1. picked points
   DBuf<int> flag;
   for (Block *bp : blocks_) {
     Block &b = *bp;
+    b.npts = 0;
+    if (cb) {  // MeshBlock::getInternalNodes with callbacks (highOrder.C:220-245)
+      std::vector<int> ibc(b.ncells);
+      if (b.ncells)
+        TG_HIP(hipMemcpyAsync(ibc.data(), b.iblank_cell.p, sizeof(int) * b.ncells, hipMemcpyDeviceToHost, s_));
+      sync();
+      b.hctag.clear();
+      for (int i = 0; i < b.ncells; i++)
+        if (ibc[i] == -1) b.hctag.push_back(i + BASE);
+      b.hppc.assign(b.hctag.size(), 0);
+      for (size_t i = 0; i < b.hctag.size(); i++) {
+        ho_.nodes_per_cell(&b.hctag[i], &b.hppc[i]);
+        b.npts += b.hppc[i];
+      }
+      b.hrxyz.assign(3 * (size_t)b.npts, 0.0);
+      size_t m = 0;
+      for (size_t i = 0; i < b.hctag.size(); i++) {
+        ho_.receptor_nodes(&b.hctag[i], &b.hppc[i], &b.hrxyz[m]);
+        m += 3 * (size_t)b.hppc[i];
+      }
+      b.pxyz.reserve(3 * (size_t)std::max(b.npts, 1));
+      if (b.npts)
+        TG_HIP(hipMemcpyAsync(b.pxyz.p, b.hrxyz.data(), sizeof(double) * 3 * b.npts, hipMemcpyHostToDevice, s_));
+      sync();
+      continue;
+    }
     b.pnode.reserve(std::max(b.nnodes, 1));
     flag.reserve(std::max(b.nnodes, 1));
-    b.npts = 0;
     if (b.nnodes) {
       k_pick<<<pnb(b.nnodes), PBS, 0, s_>>>(b.nnodes, b.iblank.p, flag.p);
       b.npts = select_flagged(flag.p, b.nnodes, b.pnode.p, s_);
@@ -266,6 +310,14 @@ void Engine::perform_connectivity_points()
     at[rb] += cnt[i];
   }
   sync();
+  if (cb) {  // 3, 4 with callbacks: block by block, as tioga.C:171-176 runs them
+    for (Block *b : blocks_) {
+      search(*b);
+      points_by_callbacks(*b);
+    }
+    points_ready_ = true;
+    return;
+  }
   // 3. the search
   if (batched) {
     search_batched();
@@ -298,12 +350,197 @@ void Engine::perform_connectivity_points()
   points_ready_ = true;
 }
 
+namespace {
+constexpr int NFRAC = 1331;  // highOrder.C:25: the callbacks' weight and modal buffers
+
+// utils.c:170-185 (C truncation toward zero, then the range test)
+int check_hole_host(const double *x, const int nx[3], const unsigned char *sam, const double ext[6])
+{
+  double dx[3];
+  int ix[3];
+  for (int i = 0; i < 3; i++) dx[i] = (ext[i + 3] - ext[i]) / nx[i];
+  for (int i = 0; i < 3; i++) {
+    ix[i] = (int)((x[i] - ext[i]) / dx[i]);
+    if (ix[i] < 0 || ix[i] > nx[i] - 1) return 0;
+  }
+  return sam[(size_t)ix[2] * nx[1] * nx[0] + (size_t)ix[1] * nx[0] + ix[0]];
+}
+}  // namespace
+
+// MeshBlock::search's walk with donor_inclusion_test, then
+// MeshBlock::processPointDonors with donor_frac, for block D's point queries
+void Engine::points_by_callbacks(Block &D)
+{
+  D.h2_inode.clear();
+  D.h2_woff.assign(1, 0);
+  D.h2_rpt.clear();
+  D.h2_rblk.clear();
+  D.h2_w.clear();
+  D.ninterp2 = 0;
+  const int Q = D.nsearch;
+  if (Q == 0) return;
+  std::vector<int> qi, rank, cell;
+  list_candidates(D, qi, rank, cell);
+  std::vector<double> xs(3 * (size_t)Q);
+  std::vector<int> xt(Q), is(3 * (size_t)Q), ibc(D.ncells);
+  TG_HIP(hipMemcpyAsync(xs.data(), D.xsearch.p, sizeof(double) * xs.size(), hipMemcpyDeviceToHost, s_));
+  TG_HIP(hipMemcpyAsync(xt.data(), D.xtag.p, sizeof(int) * Q, hipMemcpyDeviceToHost, s_));
+  TG_HIP(hipMemcpyAsync(is.data(), D.isearch.p, sizeof(int) * is.size(), hipMemcpyDeviceToHost, s_));
+  if (D.ncells)
+    TG_HIP(hipMemcpyAsync(ibc.data(), D.iblank_cell.p, sizeof(int) * D.ncells, hipMemcpyDeviceToHost, s_));
+  sync();
+  std::vector<int> ord(qi.size());
+  for (size_t j = 0; j < ord.size(); j++) ord[j] = (int)j;
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return qi[a] != qi[b] ? qi[a] < qi[b] : rank[a] < rank[b]; });
+  std::vector<int> donor(Q, -1);
+  std::vector<double> rst(3 * (size_t)Q, 0.0);
+  for (size_t k = 0; k < ord.size();) {
+    const int i = qi[ord[k]];
+    bool found = false;
+    for (; k < ord.size() && qi[ord[k]] == i; k++) {
+      if (found) continue;
+      const int c = cell[ord[k]];
+      int t = 0;
+      while (t < D.ti.ntypes - 1 && c >= D.ti.cstart[t + 1]) t++;
+      const int nv = D.ti.nv[t];
+      const int *cv = D.uconn[t] + (size_t)nv * (c - D.ti.cstart[t]);
+      // the element box (search.C:199-214), tested with TOL (searchADTrecursion.C:60-66)
+      bool inb = true;
+      for (int d = 0; d < 3; d++) {
+        double lo = BIGVALUE, hi = -BIGVALUE;
+        for (int m = 0; m < nv; m++) {
+          const double v = D.ux[3 * (size_t)(cv[m] - BASE) + d];
+          lo = dmin(lo, v);
+          hi = dmax(hi, v);
+        }
+        inb = inb && xs[3 * (size_t)i + d] >= lo - TOL && xs[3 * (size_t)i + d] <= hi + TOL;
+      }
+      if (!inb) continue;
+      int icell1 = c + BASE, pass = 0;
+      ho_.inclusion(&icell1, &xs[3 * (size_t)i], &pass, &rst[3 * (size_t)i]);
+      if (pass) {
+        donor[i] = c;
+        found = true;
+      }
+    }
+  }
+  for (int i = 0; i < Q; i++)
+    if (xt[i] != i) {
+      donor[i] = donor[xt[i]];
+      for (int j = 0; j < 3; j++) rst[3 * (size_t)i + j] = rst[3 * (size_t)xt[i] + j];
+    }
+  TG_HIP(hipMemcpyAsync(D.donorId.p, donor.data(), sizeof(int) * Q, hipMemcpyHostToDevice, s_));
+  sync();
+  std::vector<double> frac(NFRAC);
+  for (int i = 0; i < Q; i++) {
+    if (donor[i] < 0 || ibc[donor[i]] != 1) continue;
+    int icell = donor[i] + BASE, nw = 0, inode0 = 0, ndim = NFRAC;
+    ho_.frac(&icell, &xs[3 * (size_t)i], &nw, &inode0, frac.data(), &rst[3 * (size_t)i], &ndim);
+    if (nw < 0 || nw > NFRAC) throw std::runtime_error("tioga_amd: donor_frac returned a weight count out of range");
+    D.h2_inode.push_back(inode0);
+    D.h2_w.insert(D.h2_w.end(), frac.begin(), frac.begin() + nw);
+    D.h2_woff.push_back((int)D.h2_w.size());
+    D.h2_rpt.push_back(is[3 * (size_t)i + 1]);
+    D.h2_rblk.push_back(is[3 * (size_t)i + 2]);
+  }
+  D.ninterp2 = (int)D.h2_inode.size();
+}
+
+// tioga::dataUpdate(at_points = 1) with the callbacks: the reference's
+// high-order data has a row layout only (highOrder.C:445-471)
+void Engine::data_update_points_callbacks(int nvar, int interptype)
+{
+  if (interptype != 0)
+    throw std::runtime_error("tioga_amd: the high-order point update has a row layout only (highOrder.C:445)");
+  const int nbk = (int)blocks_.size();
+  std::vector<std::vector<double>> qtmp(nbk);
+  std::vector<std::vector<int>> itmp(nbk);
+  for (int ib = 0; ib < nbk; ib++) {
+    qtmp[ib].assign((size_t)blocks_[ib]->npts * nvar, 0.0);
+    itmp[ib].assign(blocks_[ib]->npts, 0);
+  }
+  std::vector<double> qq(nvar);
+  for (int ib = 0; ib < nbk; ib++) {
+    const Block &D = *blocks_[ib];
+    for (int r = 0; r < D.ninterp2; r++) {
+      const size_t inode = (size_t)(D.h2_inode[r] - BASE);
+      for (int k = 0; k < nvar; k++) qq[k] = 0;
+      for (int m = 0; m < D.h2_woff[r + 1] - D.h2_woff[r]; m++) {
+        const double w = D.h2_w[D.h2_woff[r] + m];
+        for (int k = 0; k < nvar; k++) qq[k] += D.uq[inode + (size_t)m * nvar + k] * w;
+      }
+      const int b = D.h2_rblk[r], p = D.h2_rpt[r];
+      std::copy(qq.begin(), qq.end(), qtmp[b].begin() + (size_t)p * nvar);
+      itmp[b][p] = 1;
+    }
+  }
+  // the hole maps, on the host
+  std::vector<std::vector<unsigned char>> sam(hm_.size());
+  for (size_t k = 0; k < hm_.size(); k++)
+    if (hm_[k].exist) {
+      sam[k].resize((size_t)hm_[k].nx[0] * hm_[k].nx[1] * hm_[k].nx[2]);
+      TG_HIP(hipMemcpyAsync(sam[k].data(), hm_[k].sam.p, sam[k].size(), hipMemcpyDeviceToHost, s_));
+    }
+  sync();
+  std::vector<double> qout((size_t)nvar * NFRAC);
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &R = *blocks_[ib];
+    std::vector<int> ibc(R.ncells);
+    if (R.ncells)
+      TG_HIP(hipMemcpyAsync(ibc.data(), R.iblank_cell.p, sizeof(int) * R.ncells, hipMemcpyDeviceToHost, s_));
+    sync();
+    // MeshBlock::clearOrphans (highOrder.C:157-192)
+    size_t m = 0;
+    for (size_t i = 0; i < R.hctag.size(); i++) {
+      int reject = 0;
+      for (int j = 0; j < R.hppc[i]; j++, m++) {
+        if (itmp[ib][m] != 0) continue;
+        reject = 2;
+        for (size_t k = 0; k < hm_.size(); k++)
+          if ((int)k != R.tag - BASE && hm_[k].exist &&
+              check_hole_host(&R.hrxyz[3 * m], hm_[k].nx, sam[k].data(), hm_[k].ext)) {
+            reject = 1;
+            break;
+          }
+      }
+      if (reject == 1) ibc[R.hctag[i] - 1] = 0;
+      else if (reject == 2) ibc[R.hctag[i] - 1] = 1;
+    }
+    // MeshBlock::updatePointData (highOrder.C:527-543); npts is set once
+    // and then carries what the callback leaves in it, as there
+    int npts = NFRAC;
+    m = 0;
+    for (size_t i = 0; i < R.hctag.size(); i++) {
+      if (ibc[R.hctag[i] - 1] == -1) {
+        int index_out = 0;
+        ho_.modal(&R.hctag[i], &R.hppc[i], &qtmp[ib][m], &npts, &index_out, qout.data());
+        if (npts < 0 || npts > NFRAC) throw std::runtime_error("tioga_amd: convert_to_modal returned too many points");
+        index_out -= BASE;
+        size_t k = 0;
+        for (int j = 0; j < npts; j++)
+          for (int n = 0; n < nvar; n++) R.uq[(size_t)index_out + (size_t)j * nvar + n] = qout[k++];
+      }
+      m += (size_t)R.hppc[i] * nvar;
+    }
+    if (R.ncells) {
+      TG_HIP(hipMemcpyAsync(R.iblank_cell.p, ibc.data(), sizeof(int) * R.ncells, hipMemcpyHostToDevice, s_));
+      int *dst = R.uiblank_cell ? R.uiblank_cell : (R.own_iblank_cell.size() >= (size_t)R.ncells ? R.own_iblank_cell.data() : nullptr);
+      if (dst && R.writeback) memcpy(dst, ibc.data(), sizeof(int) * R.ncells);
+    }
+    sync();
+  }
+}
+
 void Engine::data_update_points(int nvar, int interptype)
 {
   if (!points_ready_) throw std::runtime_error("tioga_amd: dataUpdate at points needs performConnectivityHighOrder");
   for (Block *b : blocks_)
     if (b->uq == nullptr) return;  // tioga.C:348-352
   if (nsend_ == 0 || nvar <= 0) return;
+  if (ho_.set) {
+    data_update_points_callbacks(nvar, interptype);
+    return;
+  }
   const int nbk = (int)blocks_.size();
   const int column = interptype == 1;
   while ((int)qdev_.size() < nbk) qdev_.push_back(new DBuf<double>);
@@ -373,6 +610,30 @@ long Engine::points_array(int btag, const std::string &nm, void *out)
   if (nm == "ninterp2") {
     if (out) *(int *)out = b.ninterp2;
     return 1;
+  }
+  if (ho_.set) {  // callback records: one solution index each
+    const int n = b.ninterp2;
+    if (nm == "interp2_inode") {
+      if (out) std::copy(b.h2_inode.begin(), b.h2_inode.end(), (int *)out);
+      return n;
+    }
+    if (nm == "interp2_weights") {
+      if (out) std::copy(b.h2_w.begin(), b.h2_w.end(), (double *)out);
+      return (long)b.h2_w.size();
+    }
+    if (nm == "interp2_rinfo") {
+      if (out) {
+        const std::vector<int> &pm = partners(b.rank);
+        for (int r = 0; r < n; r++) {
+          const Block &R = *blocks_[b.h2_rblk[r]];
+          ((int *)out)[3 * r] = nranks() > 1 ? (int)(std::lower_bound(pm.begin(), pm.end(), R.rank) - pm.begin()) : 0;
+          ((int *)out)[3 * r + 1] = b.h2_rpt[r];
+          ((int *)out)[3 * r + 2] = R.lidx;
+        }
+      }
+      return 3 * (long)n;
+    }
+    return -1;
   }
   const int n = b.ninterp2;
   std::vector<int> nw(n), inode(8 * (size_t)n), rpt(n), rblk(n);

@@ -301,9 +301,11 @@ class Tioga:
             self.set_rank(rk)
             self.register_solution(b.tag, q)
         self.data_update_points(nvar, interptype)
-        for i in range(len(self.blocks)):
+        for i, b in enumerate(self.blocks):
+            self.set_rank(self.ranks[i])
             r[f"b{i}.q"] = out[i]
             r[f"b{i}.iblank"] = self.iblank[i].copy()
+            r[f"b{i}.iblank_cell"] = self.array(b.tag, "iblank_cell")
         self.set_rank(0)
         return r
 
