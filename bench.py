@@ -432,6 +432,7 @@ def search_only(args, tg, blocks, desc) -> int:
     from tioga_amd import autotune
 
     autotune.apply(tg, args.workload)
+    tg.search_graph(not args.no_graph)
     for i, (_, xq) in zip(qids, sets):
         tg.query_search(i, xq.shape[0])
     for _ in range(args.warmup):
@@ -441,6 +442,7 @@ def search_only(args, tg, blocks, desc) -> int:
     tg.close()
     print(json.dumps({"workload": desc, "steps": args.steps, "queries_per_step": int(nq),
                       "search_pass_ms_device": round(pass_ms, 4), "search_kernel_ms": round(kern_ms, 4),
+                      "search_launch": "kernel by kernel" if args.no_graph else "HIP graph replay",
                       "search_stats": stats}),
           flush=True)
     return 0
@@ -454,6 +456,8 @@ def main() -> int:
     ap.add_argument("--workload", default="config2")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU search for the baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="timed passes launch the search kernels one by one instead of replaying a HIP graph")
     ap.add_argument("--no-amr", action="store_true", help="skip the AMR gather leg")
     ap.add_argument("--search-only", action="store_true",
                     help="only the headline query-set passes (profiling runs): no performconnectivity, dataUpdate, "
@@ -548,6 +552,7 @@ def main() -> int:
 
     tuning = autotune.apply(tg, args.workload)
     tuning["source"] = "tioga_amd/search_tuning.json" if autotune.load(args.workload) else "defaults"
+    tg.search_graph(not args.no_graph)
     donors = [tg.query_search(i, xq.shape[0]) for i, (_, xq) in zip(qids, sets)]
 
     for _ in range(args.warmup):
@@ -634,6 +639,7 @@ def main() -> int:
         "parity": parity,
         "timed_donors_equal_checked": bool(timed_same),
         "search_tuning": tuning,
+        "search_launch": "kernel by kernel" if args.no_graph else "HIP graph replay (recorded once after the checked pass)",
         "queries_in_root_box": in_root,
         "search_stats": stats,
         "searches_per_s_in_root_box": round(in_root / (ms_per_step / 1000.0), 1),

@@ -242,6 +242,11 @@ void tioga_amd_dataupdate_points(int *nvar, char *itype);
 long tioga_amd_points_array(int btag, const char *name, void *out);
 int tioga_amd_packet(const char *exchange, int src, int dst, int *ints, double *reals, long long *counts);
 
+/* tioga_amd_time_query_sets' timed passes replay their search launch from
+   a HIP graph recorded after the checked pass (on != 0, the default) or
+   launch its kernels one by one (0).  Results are the same. */
+void tioga_amd_search_graph(int on);
+
 /* Batched form (on != 0, the default): the stages of performconnectivity
    run for all blocks together (one ADT build, one coincident-point dedupe
    for every block's query set, ...); 0 runs them block by block.  The

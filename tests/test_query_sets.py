@@ -217,3 +217,31 @@ def test_query_sets_any_tuning(tuning):
     finally:
         t.close()
         o.close()
+
+
+def test_query_sets_graph_replays():
+    """The timed passes replay the search launch from a HIP graph recorded
+    after the checked pass: several replays of one graph, and graphs
+    recorded again with eager searches in between, all leave the oracle's
+    donors; so does the kernel-by-kernel form."""
+    print("\n[query sets] graph replays", flush=True)
+    blocks = _blocks("mid")
+    o = _oracle(blocks)
+    t = _engine(blocks)
+    try:
+        sets = [(blocks[0].tag, blocks[1].x), (blocks[1].tag, blocks[0].x)]
+        ids = [t.query_set(btag, xq) for btag, xq in sets]
+        want = [o.search_points_obb(btag, xq, xq, _threads())[0] for btag, xq in sets]
+        for rep in range(3):
+            t.time_query_sets(4)
+            for qid, (_, xq), d in zip(ids, sets, want):
+                assert np.array_equal(t.query_donors(qid, xq.shape[0]), d), f"replay round {rep}, set {qid}"
+            assert np.array_equal(t.query_search(ids[rep % 2], sets[rep % 2][1].shape[0]), want[rep % 2])
+        t.search_graph(False)
+        t.time_query_sets(2)
+        for qid, (_, xq), d in zip(ids, sets, want):
+            assert np.array_equal(t.query_donors(qid, xq.shape[0]), d), f"kernel by kernel, set {qid}"
+        t.search_graph(True)
+    finally:
+        t.close()
+        o.close()

@@ -597,6 +597,11 @@ int tioga_amd_packet(const char *exchange, int src, int dst, int *ints, double *
   return r;
 }
 
+void tioga_amd_search_graph(int on)
+{
+  guarded("tioga_amd_search_graph", [&] { tg_or_throw().engine()->search_graph = (on != 0); });
+}
+
 void tioga_amd_set_batching(int on)
 {
   guarded("tioga_amd_set_batching", [&] { tg_or_throw().engine()->batched = (on != 0); });

@@ -403,6 +403,9 @@ struct SearchTuning {
 
 struct SearchScratch {
   SearchTuning tune;
+  // a launch recorded into a HIP graph reads the search tables the previous
+  // launch of the same sets uploaded (identical), so it copies none itself
+  bool skip_uploads = false;
   DBuf<int> cell, query, status, counters, bins, blist, lj, lcount, qb, qsid, nin, wstart;
   DBuf<int> best, last, acc;     // per query outcome words of a launch
   DBuf<int> nseg, segstart;      // the bin listing's work items per walking wave
@@ -612,6 +615,9 @@ class Engine {
   // of a resident tree (more than one accepting candidate, or a near-zero
   // weight in a BIGVALUE cell)
   long long last_tieband() const { return last_tieband_; }
+  // timed query-set passes replay the search launch from a HIP graph (true)
+  // or launch its kernels one by one (false); results are the same
+  bool search_graph = true;
 
   void register_grid(int btag, int nnodes, double *xyz, int *ibl, int nwbc, int nobc, int *wbc, int *obc,
                      int ntypes, const int *nv, const int *nc, int **vconn);
@@ -754,6 +760,11 @@ class Engine {
   const int *order_sets(const std::vector<MortonSet> &M, size_t tot);
   SearchScratch sw_;
   bool timing_ = false;  // inside time_query_sets: candidate capacity already established
+  // the timed passes replay the search launch from a HIP graph, recorded
+  // once per time_query_sets call after its checked pass
+  hipGraph_t qs_graph_ = nullptr;
+  hipGraphExec_t qs_exec_ = nullptr;
+  void drop_search_graph();
   std::vector<hipEvent_t> *kernel_events_ = nullptr;
   size_t event_slot_ = 0;
   double last_search_kernel_ms_ = 0;

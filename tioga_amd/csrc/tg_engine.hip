@@ -2548,8 +2548,10 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     throw std::runtime_error("tioga_amd: search launched without its ordering step");
   w.sv.reserve(sizeof(SearchView) * ns);
   w.qb.reserve(ns + 1);
-  TG_HIP(hipMemcpyAsync(w.sv.p, S, sizeof(SearchView) * ns, hipMemcpyHostToDevice, s));
-  TG_HIP(hipMemcpyAsync(w.qb.p, qb.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
+  if (!w.skip_uploads) {
+    TG_HIP(hipMemcpyAsync(w.sv.p, S, sizeof(SearchView) * ns, hipMemcpyHostToDevice, s));
+    TG_HIP(hipMemcpyAsync(w.qb.p, qb.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
+  }
   w.counters.reserve(1);
   // per query: lowest clean-hit rank, highest tested (rank, outcome), and the
   // tie-band count when a search has a counter
@@ -2708,6 +2710,7 @@ Engine::Engine()
 
 Engine::~Engine()
 {
+  drop_search_graph();
   query_set_clear();
   for (Block *b : blocks_) delete b;
   for (auto *q : qdev_) delete q;
@@ -4203,8 +4206,31 @@ void Engine::launch_query_sets(const std::vector<QuerySet *> &qs)
     sv.push_back(S);
   }
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
-  run_search(sv.data(), (int)sv.size(), sw_, s_, !timing_);
+  if (timing_ && search_graph) {
+    // the unchecked launch of a timed pass, recorded once into a graph (its
+    // tables are the checked pass's, on the device already) and replayed
+    if (!qs_exec_) {
+      sw_.skip_uploads = true;
+      TG_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+      run_search(sv.data(), (int)sv.size(), sw_, s_, false);
+      const hipError_t e = hipStreamEndCapture(s_, &qs_graph_);
+      sw_.skip_uploads = false;
+      TG_HIP(e);
+      TG_HIP(hipGraphInstantiate(&qs_exec_, qs_graph_, nullptr, nullptr, 0));
+    }
+    TG_HIP(hipGraphLaunch(qs_exec_, s_));
+  } else {
+    run_search(sv.data(), (int)sv.size(), sw_, s_, !timing_);
+  }
   if (kernel_events_) TG_HIP(hipEventRecord((*kernel_events_)[event_slot_++], s_));
+}
+
+void Engine::drop_search_graph()
+{
+  if (qs_exec_) (void)hipGraphExecDestroy(qs_exec_);
+  if (qs_graph_) (void)hipGraphDestroy(qs_graph_);
+  qs_exec_ = nullptr;
+  qs_graph_ = nullptr;
 }
 
 void Engine::set_search_bins(int bins)
@@ -4290,7 +4316,9 @@ double Engine::time_query_sets(int nrep, long long *nq)
   // events around every pass's search launch give the kernels' own duration
   std::vector<hipEvent_t> ev(2 * nrep);
   for (auto &e : ev) TG_HIP(hipEventCreate(&e));
-  // establish the candidate-list capacity with one checked pass
+  // establish the candidate-list capacity with one checked pass; the graph
+  // of the timed launches is recorded after it (buffers may have grown)
+  drop_search_graph();
   launch_query_sets(qsets_);
   sync();
   kernel_events_ = &ev;
@@ -4309,6 +4337,7 @@ double Engine::time_query_sets(int nrep, long long *nq)
   for (auto &e : ev) (void)hipEventDestroy(e);
   last_search_kernel_ms_ = k / nrep;
   event_slot_ = 0;
+  drop_search_graph();
   return pass;
 }
 

@@ -103,6 +103,7 @@ def load():
         L.tioga_amd_serve_wait2.argtypes = [C.c_long, _P, C.c_int, _P, _P, _P, C.c_int, _P, _P]
         L.tioga_amd_serve_wait2.restype = C.c_int
         L.tioga_amd_serve_close.argtypes = []
+        L.tioga_amd_search_graph.argtypes = [C.c_int]
         L.tioga_amd_set_rank.argtypes = [C.c_int]
         L.tioga_amd_rank.restype = C.c_int
         L.tioga_amd_points_array.argtypes = [C.c_int, C.c_char_p, _P]
@@ -531,6 +532,12 @@ class Tioga:
             self.L.tioga_amd_search_tuning(_p(want), _p(cur))
             self._check()
         return {"morton_bits": int(cur[0]), "seg": int(cur[1]), "eval_grid": int(cur[2]), "bins": int(nb)}
+
+    def search_graph(self, on: bool = True) -> None:
+        """Timed query-set passes replay their search launch from a HIP graph
+        (on) or launch its kernels one by one."""
+        self.L.tioga_amd_search_graph(1 if on else 0)
+        self._check()
 
     def search_capacity(self, cap: int = 0) -> int:
         r = self.L.tioga_amd_search_capacity(int(cap))
