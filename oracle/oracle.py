@@ -307,14 +307,15 @@ class Oracle:
 def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptype: int = 0,
                isym: int = 3, nfringe: int = 1, mexclude: int = 3,
                moves: Sequence[Sequence[Optional[np.ndarray]]] = (), amr=None, reduce: bool = False,
-               ranks: Optional[Sequence[int]] = None, points: int = 0) -> None:
+               ranks: Optional[Sequence[int]] = None, points: int = 0, amr_full: bool = False) -> None:
     """amr = (AmrGrid, patch fields per local patch) selects the AMR gather
     mode of the driver (format version 2); reduce runs reduce_fringes after
     each step's connectivity (format version 3); ranks (the MPI rank of each
     block) selects the multi-rank mode (format version 5); points runs the
-    point path after each step (format version 6)."""
+    point path after each step (format version 6); amr_full runs both
+    directions of the AMR path (format version 7)."""
     gids = any(b.cell_gid is not None or b.node_gid is not None for b in blocks)
-    version = 6 if points else (5 if ranks is not None else (4 if gids else (3 if reduce else (1 if amr is None else 2))))
+    version = 7 if amr_full else (6 if points else (5 if ranks is not None else (4 if gids else (3 if reduce else (1 if amr is None else 2)))))
     with open(path, "wb") as f:
         w = lambda fmt, *v: f.write(struct.pack("<" + fmt, *v))
         w("9i", 0x41474954, version, len(blocks), isym, nfringe, mexclude, nvar, interptype,
@@ -360,6 +361,8 @@ def write_case(path: str, blocks, qs: Sequence[np.ndarray], nvar: int, interptyp
                 f.write(np.asarray(ranks, np.int32).tobytes())
         if version >= 6:
             w("i", int(points))  # 1: the point path, 2: with the test callbacks
+        if version >= 7:
+            w("i", int(amr_full))
         for step in moves:
             for xb in step:
                 if xb is None:

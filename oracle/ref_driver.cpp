@@ -58,6 +58,14 @@
 //              blocks, block 0 the receptor mesh and block 1 the donor mesh,
 //              and the node dataUpdate is left out (with callbacks the C ABI
 //              sends dataUpdate to the point update).
+//   version 7 adds, after points: int32 amr_full; if set (an AMR case with
+//              one near-body block), run_amr runs tioga::performConnectivity
+//              first (the hole maps) and then performConnectivityAMR's own
+//              sequence up to its closing MPI_Abort (tioga.C:187-207: both
+//              directions, exchangeAMRDonors included), and also dumps the
+//              near-body search of the Cartesian points, interpListCart,
+//              every patch's iblank and, after dataUpdate_AMR, every
+//              patch's q (amr.patch<p>.*).
 // Packet log (multi-rank mode): every MPI_Isend the reference makes, in call
 //              order, as {int32 1, dest, tag, dtype(0=int,1=double), count,
 //              payload}, and every MPI_Waitall as {int32 2}.  Each
@@ -270,13 +278,19 @@ struct Amr {
 // step is a call into the compiled reference except the loop that fills the
 // patch interpolation lists, which is exchangeAMRDonors.C:120-192 written
 // for a single rank (packets to self, so arrival order = packing order).
-void run_amr(tioga *tg, std::vector<Block> &blocks, Amr &A, int nvar, int itype, Writer &w)
+void run_amr_full(tioga *tg, std::vector<Block> &blocks, Amr &A, int nvar, int itype, Writer &w);
+
+void run_amr(tioga *tg, std::vector<Block> &blocks, Amr &A, int nvar, int itype, Writer &w, bool full)
 {
   tg->register_amr_global_data(A.nf, A.qstride, A.qnode.data(), A.idata.data(), A.rdata.data(), A.ngrids,
                                A.qnodesize);
   tg->set_amr_patch_count(A.npatch);
   for (int p = 0; p < A.npatch; p++) tg->register_amr_local_data(p, A.pgid[p], A.ibl[p].data(), A.q[p].data());
   tg->profile();
+  if (full) {
+    run_amr_full(tg, blocks, A, nvar, itype, w);
+    return;
+  }
   double t0 = MPI_Wtime();
   tg->cg->preprocess();
   for (int i = 0; i < tg->ncart; i++) tg->cb[i].preprocess(tg->cg);
@@ -347,6 +361,93 @@ void run_amr(tioga *tg, std::vector<Block> &blocks, Amr &A, int nvar, int itype,
     w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
 }
 
+// tioga::performConnectivityAMR (tioga.C:180-216) as the reference runs
+// it, after a performConnectivity that leaves the hole maps its Cartesian
+// donor processing reads (CartBlock.C:308-380), without the closing
+// MPI_Barrier / MPI_Abort; then tioga::dataUpdate_AMR.
+void run_amr_full(tioga *tg, std::vector<Block> &blocks, Amr &A, int nvar, int itype, Writer &w)
+{
+  tg->performConnectivity();
+  double t0 = MPI_Wtime();
+  tg->cg->preprocess();
+  for (int i = 0; i < tg->ncart; i++) tg->cb[i].preprocess(tg->cg);
+  for (int ib = 0; ib < tg->nblocks; ib++) {
+    MeshBlock *mb = tg->mblocks[ib].get();
+    mb->getCartReceptors(tg->cg, tg->pc_cart);
+    mb->ihigh = tg->ihigh;
+    mb->search();
+    mb->getUnresolvedMandatoryReceptors();
+    tg->cg->search(mb->rxyzCart, mb->donorIdCart, mb->ntotalPointsCart);
+  }
+  tg->exchangeAMRDonors();
+  for (int ib = 0; ib < tg->nblocks; ib++) tg->mblocks[ib]->getCellIblanks();
+  double tconn = MPI_Wtime() - t0;
+  w.f64("time_connectivity", &tconn, 1);
+  for (size_t b = 0; b < blocks.size(); b++) {
+    Block &B = blocks[b];
+    MeshBlock *mb = tg->mblocks[tg->tag_iblk_map.find(B.tag)->second].get();
+    std::string p = "b" + std::to_string(b) + ".";
+    w.i32(p + "iblank", B.iblank.data(), B.nnodes);
+    w.i32(p + "iblank_cell", B.iblank_cell.data(), B.ncells);
+    w.f64(p + "nodeRes", mb->nodeRes, B.nnodes);
+    int n = mb->ntotalPointsCart;
+    w.i32(p + "ncart", &n, 1);
+    w.i32(p + "receptorIdCart", mb->receptorIdCart, n);
+    w.i32(p + "donorIdCart", mb->donorIdCart, n);
+    w.f64(p + "rxyzCart", mb->rxyzCart, 3 * (int64_t)n);
+    int ns = mb->nsearch;
+    w.i32(p + "nsearch", &ns, 1);
+    if (ns > 0) {
+      w.f64(p + "xsearch", mb->xsearch, 3 * (int64_t)ns);
+      w.i32(p + "isearch", mb->isearch, 3 * (int64_t)ns);
+      w.i32(p + "donorId", mb->donorId, ns);
+      w.i32(p + "xtag", mb->xtag, ns);
+    }
+    std::vector<int> rinfo, inode;
+    std::vector<double> wt;
+    for (int i = 0; i < mb->ninterpCart; i++) {
+      INTERPLIST &l = mb->interpListCart[i];
+      rinfo.insert(rinfo.end(), l.receptorInfo, l.receptorInfo + 3);
+      inode.insert(inode.end(), l.inode, l.inode + l.nweights);
+      wt.insert(wt.end(), l.weights, l.weights + l.nweights);
+    }
+    int nc = mb->ninterpCart;
+    w.i32(p + "ninterpCart", &nc, 1);
+    w.i32(p + "interpCart_rinfo", rinfo.data(), (int64_t)rinfo.size());
+    w.i32(p + "interpCart_inode", inode.data(), (int64_t)inode.size());
+    w.f64(p + "interpCart_w", wt.data(), (int64_t)wt.size());
+  }
+  std::vector<int> rpatch, rinfo, rijk;
+  std::vector<double> rw;
+  for (int i = 0; i < tg->ncart; i++)
+    for (INTERPLIST2 *l = tg->cb[i].interpList; l; l = l->next) {
+      rpatch.push_back(i);
+      rinfo.insert(rinfo.end(), l->receptorInfo, l->receptorInfo + 3);
+      rijk.insert(rijk.end(), l->inode, l->inode + 3 * l->nweights);
+      rw.insert(rw.end(), l->weights, l->weights + l->nweights);
+    }
+  w.i32("amr.rec_patch", rpatch.data(), (int64_t)rpatch.size());
+  w.i32("amr.rec_info", rinfo.data(), (int64_t)rinfo.size());
+  w.i32("amr.rec_ijk", rijk.data(), (int64_t)rijk.size());
+  w.f64("amr.rec_w", rw.data(), (int64_t)rw.size());
+  for (int p = 0; p < A.npatch; p++)
+    w.i32("amr.patch" + std::to_string(p) + ".iblank", A.ibl[p].data(), (int64_t)A.ibl[p].size());
+  if (!tg->qblock) {
+    tg->qblock = (double **)malloc(sizeof(double *) * tg->nblocks);
+    for (int ib = 0; ib < tg->nblocks; ib++) tg->qblock[ib] = NULL;
+  }
+  std::vector<std::vector<double>> qs(blocks.size());
+  for (size_t b = 0; b < blocks.size(); b++) {
+    qs[b] = blocks[b].q;
+    tg->registerSolution(blocks[b].tag, qs[b].data());
+  }
+  tg->dataUpdate_AMR(nvar, itype);
+  for (size_t b = 0; b < blocks.size(); b++)
+    w.f64("b" + std::to_string(b) + ".q", qs[b].data(), (int64_t)qs[b].size());
+  for (int p = 0; p < A.npatch; p++)
+    w.f64("amr.patch" + std::to_string(p) + ".q", A.q[p].data(), (int64_t)A.q[p].size());
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -364,7 +465,7 @@ int main(int argc, char **argv)
   if (!r.f) { perror(argv[1]); return 2; }
   if (r.get<int32_t>() != 0x41474954) { fprintf(stderr, "bad case\n"); return 2; }
   const int version = r.get<int32_t>();
-  if (version < 1 || version > 6) { fprintf(stderr, "bad case version\n"); return 2; }
+  if (version < 1 || version > 7) { fprintf(stderr, "bad case version\n"); return 2; }
   int nblocks = r.get<int32_t>(), isym = r.get<int32_t>(), nfringe = r.get<int32_t>();
   int mexclude = r.get<int32_t>(), nvar = r.get<int32_t>(), itype = r.get<int32_t>();
   int nsteps = r.get<int32_t>();
@@ -418,6 +519,7 @@ int main(int argc, char **argv)
   if (has_ranks)
     for (auto &B : blocks) B.rank = r.get<int32_t>();
   const int points = version >= 6 ? r.get<int32_t>() : 0;
+  const int amr_full = version >= 7 ? r.get<int32_t>() : 0;
   int myrank = 0, nprocs = 1;
   MPI_Comm_rank(MPI_COMM_WORLD, &myrank);
   MPI_Comm_size(MPI_COMM_WORLD, &nprocs);
@@ -454,7 +556,7 @@ int main(int argc, char **argv)
   }
   if (has_amr) {
     if (has_ranks) { fprintf(stderr, "ref_driver: AMR cases run on one rank\n"); return 2; }
-    run_amr(tg, blocks, amr, nvar, itype, w);
+    run_amr(tg, blocks, amr, nvar, itype, w, amr_full != 0);
     nsteps = -1;
   }
   for (int s = 0; s <= nsteps; s++) {

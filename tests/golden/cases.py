@@ -257,6 +257,9 @@ class AmrCase:
     grid: object           # tioga_amd.workloads.AmrGrid
     nvar: int
     interptype: int = 0
+    # both directions (performConnectivity for the hole maps, then the
+    # reference's performConnectivityAMR sequence: one near-body block)
+    full: bool = False
 
     @property
     def qs(self) -> List[np.ndarray]:
@@ -292,7 +295,25 @@ def build_amr(name: str) -> AmrCase:
         grid = W.amr_hierarchy((-1.0, -1.0, -1.0), 0.25, [((0, 0, 0), (8, 8, 8)), ((4, 4, 4), (12, 12, 12))],
                                patch=4, nf=1, shuffle_seed=5, local_seed=6)
         return AmrCase(name, [W.all_receptors(W.cart_box(8, tag=1, lo=-0.5, hi=0.5))], grid, nvar=2)
+    # both directions: patch cells inside the near-body block take values from it
+    if name == "amr_cart":
+        # the shell of amr_fringe: its wall cuts holes in the patches
+        grid = W.amr_hierarchy((-1.0, -1.0, -1.0), 0.25,
+                               [((0, 0, 0), (8, 8, 8)), ((2, 2, 2), (14, 14, 14)), ((8, 8, 8), (24, 24, 24))],
+                               patch=4, nf=1, shuffle_seed=1, local_seed=2)
+        return AmrCase(name, [W.latlong_shell(16, 8, 6, 0.25, 0.6, 1.1, tag=1)], grid, nvar=5, full=True)
+    if name == "amr_cart_col":
+        # a mixed-element shell off the origin, two ghost layers, column layout
+        grid = W.amr_hierarchy((-1.0, -1.0, -1.0), 0.2,
+                               [((0, 0, 0), (10, 10, 10)), ((4, 4, 4), (16, 16, 16)),
+                                ((12, 12, 12), (28, 28, 28))],
+                               patch=4, nf=2, shuffle_seed=3, local_seed=4)
+        return AmrCase(name, [W.mixed_shell(12, 8, layers=(2, 2, 1, 2), tag=1)], grid, nvar=3, interptype=1,
+                       full=True)
     raise KeyError(name)
 
 
 AMR_CASES = ["amr_fringe", "amr_all", "amr_faces"]
+# both directions: pinned by the reference directly (the oracle restates the
+# gather direction only)
+AMR_FULL_CASES = ["amr_cart", "amr_cart_col"]

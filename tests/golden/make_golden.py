@@ -80,9 +80,9 @@ def main(names):
     for name in names:
         with tempfile.TemporaryDirectory() as td:
             cp, op = os.path.join(td, "case.bin"), os.path.join(td, "out.bin")
-            if name in cases.AMR_CASES:
+            if name in cases.AMR_CASES + cases.AMR_FULL_CASES:
                 c = cases.build_amr(name)
-                O.write_case(cp, c.blocks, c.qs, c.nvar, c.interptype, amr=(c.grid, c.patch_q))
+                O.write_case(cp, c.blocks, c.qs, c.nvar, c.interptype, amr=(c.grid, c.patch_q), amr_full=c.full)
             else:
                 c = cases.build(name)
                 O.write_case(cp, c.blocks, c.qs, c.nvar, c.interptype, c.isym, c.nfringe, c.mexclude,
@@ -108,4 +108,4 @@ def main(names):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or cases.ALL_CASES + cases.AMR_CASES + cases.POINT_CASES + cases.RANK_CASES)
+    main(sys.argv[1:] or cases.ALL_CASES + cases.AMR_CASES + cases.AMR_FULL_CASES + cases.POINT_CASES + cases.RANK_CASES)

@@ -167,13 +167,16 @@ def run_engine_amr_case(case):
     from tioga_amd.tioga import Tioga
 
     t = Tioga()
+    full = getattr(case, "full", False)
     try:
         for b in case.blocks:
             t.register_grid(b)
         t.register_amr(case.grid, case.patch_q)
         t.preprocess()
+        if full:  # the hole maps the Cartesian donor processing reads
+            t.connect()
         t.connect_amr()
-        out = t.amr_results(case.qs, case.nvar, case.interptype)
+        out = t.amr_results(case.qs, case.nvar, case.interptype, full=full)
     finally:
         t.close()
     return out

@@ -72,10 +72,13 @@ def test_engine_matches_oracle_random_seeds():
             assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
 
 
-@pytest.mark.parametrize("name", cases.AMR_CASES)
+@pytest.mark.parametrize("name", cases.AMR_CASES + cases.AMR_FULL_CASES)
 def test_engine_amr_matches_reference(name):
     """AMR gather direction: receptors, donor patches, patch interpolation
-    records (list order, stencils, weights), iblanks and updated q."""
+    records (list order, stencils, weights), iblanks and updated q; in the
+    full cases also the Cartesian-receptor direction: the near-body search
+    of the patch cells, patch iblanks (holes, receptors), cancelled donors,
+    interpListCart and the updated patch solutions."""
     print(f"\n[engine parity] AMR case {name}", flush=True)
     got = run_engine_amr_case(cases.build_amr(name))
     bad = compare_to_golden(name, got)

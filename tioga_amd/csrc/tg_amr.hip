@@ -34,10 +34,13 @@
 //   * patch solutions packed in one buffer, nvar fields of d3nf doubles each.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <memory>
 #include <stdexcept>
 
 #include "tg_engine.h"
 #include "tg_prims.h"
+#include "tg_seq.h"
 
 namespace tg {
 
@@ -89,6 +92,22 @@ struct AmrState {
   DBuf<long long> qoff;
   DBuf<int> err;
   DBuf<AmrQTables> qt;
+  // the Cartesian-receptor direction: per patch cell (local patch p's cells
+  // from coff[p], x fastest) the donor count, donor mesh tags, whether a
+  // donor of resolution < BIGVALUE exists, and the cancel mark; the patch
+  // iblanks (d3nf per patch from ioff[p]); per block the records of the
+  // interpolation list to Cartesian cells (interpListCart)
+  std::vector<long long> coff, ioff;
+  DBuf<long long> dcoff, dioff;
+  DBuf<int> ccnt, cgood, ccancel, pibl_dev;
+  DBuf<unsigned long long> ctags;
+  struct CartRecs {
+    int n = 0;
+    DBuf<int> inode, nw, rpt, rlp;  // inode[m * n + r]; receptor: patch q index, local patch
+    DBuf<double> w;
+  };
+  std::vector<std::unique_ptr<CartRecs>> crec;  // per block
+  std::vector<double> cvals;   // the last update's values at Cartesian receptors (host)
 };
 
 namespace {
@@ -354,6 +373,427 @@ AmrState &Engine::amr()
 {
   if (!amr_) amr_ = new AmrState;
   return *amr_;
+}
+
+namespace {
+
+// ---- the Cartesian-receptor direction ----
+
+// MeshBlock::getCartReceptors (getCartReceptors.C:85-154) for pdegree 0: the
+// centre of patch cell (i, j, k) (get_amr_index_xyz.c:1-56 with one point),
+// cells in the reference's loop order (x outermost, z innermost), kept when
+// the centre lies in the block's OBB
+__device__ __forceinline__ void cart_cell(const AmrGridDev &g, int t, int &i, int &j, int &k)
+{
+  k = t % g.dims[2];
+  j = (t / g.dims[2]) % g.dims[1];
+  i = t / (g.dims[2] * g.dims[1]);
+}
+
+__device__ __forceinline__ void cart_centre(const AmrGridDev &g, double qn0, int i, int j, int k, double x[3])
+{
+  const double cx = g.xlo[0] + i * g.dx[0], cy = g.xlo[1] + j * g.dx[1], cz = g.xlo[2] + k * g.dx[2];
+  const double hx = 0.5 * g.dx[0], hy = 0.5 * g.dx[1], hz = 0.5 * g.dx[2];
+  x[2] = cz + hz * (1.0 + qn0);
+  x[1] = cy + hy * (1.0 + qn0);
+  x[0] = cx + hx * (1.0 + qn0);
+}
+
+// the grids of one block's Cartesian search, cells numbered grid after grid
+struct CartGridList {
+  int n;                     // grids
+  const AmrGridDev *g;       // their records
+  const long long *off;      // first cell number of each, and the total (n + 1)
+  const int *lid;            // their local patch ids
+};
+
+__device__ __forceinline__ int cart_grid_of(const CartGridList &L, long long t)
+{
+  int lo = 0, hi = L.n - 1;  // last grid whose first cell is <= t
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (L.off[m] <= t) lo = m;
+    else hi = m - 1;
+  }
+  return lo;
+}
+
+__global__ void k_cart_flags(long long n, CartGridList L, double qn0, OBB b, int *__restrict__ flag)
+{
+  const long long u = (long long)blockIdx.x * ABS + threadIdx.x;
+  if (u >= n) return;
+  const int gi = cart_grid_of(L, u);
+  const AmrGridDev g = L.g[gi];
+  int i, j, k;
+  cart_cell(g, (int)(u - L.off[gi]), i, j, k);
+  double x[3], xd[3] = {0, 0, 0};
+  cart_centre(g, qn0, i, j, k, x);
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++) xd[a] += (x[c] - b.xc[c]) * b.vec[a][c];
+  flag[u] = fabs(xd[0]) <= b.dxc[0] && fabs(xd[1]) <= b.dxc[1] && fabs(xd[2]) <= b.dxc[2];
+}
+
+// search points: the centre, isearch (rank, local patch, q index of the
+// cell), resolution BIGVALUE and mesh tag -1 (the reference leaves both
+// unset; they matter only to coincident points, which a patch hierarchy
+// does not produce)
+__global__ void k_cart_points(int n, const int *__restrict__ sel, CartGridList L, double qn0, int nf,
+                              double *__restrict__ xs, int *__restrict__ is, double *__restrict__ rs,
+                              int *__restrict__ ts)
+{
+  const int q = blockIdx.x * ABS + threadIdx.x;
+  if (q >= n) return;
+  const int gi = cart_grid_of(L, sel[q]);
+  const AmrGridDev g = L.g[gi];
+  int i, j, k;
+  cart_cell(g, (int)(sel[q] - L.off[gi]), i, j, k);
+  double x[3];
+  cart_centre(g, qn0, i, j, k, x);
+  const int sx = g.dims[0] + 2 * nf, sy = g.dims[1] + 2 * nf;
+  xs[3 * (size_t)q] = x[0];
+  xs[3 * (size_t)q + 1] = x[1];
+  xs[3 * (size_t)q + 2] = x[2];
+  is[3 * (size_t)q] = 0;
+  is[3 * (size_t)q + 1] = L.lid[gi];
+  is[3 * (size_t)q + 2] = sy * sx * (k + nf) + sx * (j + nf) + (i + nf);
+  rs[q] = BIGVALUE;
+  ts[q] = -1;
+}
+
+// the patch cell (local patch p, unghosted index) of a Cartesian search
+// point: amr_index_to_ijklmn (get_amr_index_xyz.c:58-110) for pdegree 0
+__device__ __forceinline__ long long cart_cell_of(const int *is, const int *__restrict__ pd3,
+                                                  const long long *__restrict__ coff, const int *__restrict__ pgid,
+                                                  const AmrGridDev *__restrict__ G, int nf, int q)
+{
+  const int lp = is[3 * (size_t)q + 1], idx = is[3 * (size_t)q + 2];
+  const AmrGridDev &g = G[pgid[lp]];
+  const int sx = g.dims[0] + 2 * nf, sy = g.dims[1] + 2 * nf;
+  const int r = idx % pd3[lp];
+  const int kk = r / (sx * sy) - nf, jj = (r % (sx * sy)) / sx - nf, ii = r % sx - nf;
+  return coff[lp] + ((long long)kk * g.dims[1] + jj) * g.dims[0] + ii;
+}
+
+// CartBlock::insertInDonorList (CartBlock.C:270-306): what processDonors
+// reads of a cell's list -- whether it is empty, the donor mesh tags, and
+// whether some donor resolution is below BIGVALUE
+__global__ void k_cart_donors(int n, const int *__restrict__ donor, const int *__restrict__ is,
+                              const double *__restrict__ cellRes, int tag, const int *__restrict__ pd3,
+                              const long long *__restrict__ coff, const int *__restrict__ pgid,
+                              const AmrGridDev *__restrict__ G, int nf, int *__restrict__ cnt,
+                              unsigned long long *__restrict__ tags, int *__restrict__ good)
+{
+  const int q = blockIdx.x * ABS + threadIdx.x;
+  if (q >= n || donor[q] == -1) return;
+  const long long c = cart_cell_of(is, pd3, coff, pgid, G, nf, q);
+  atomicAdd(&cnt[c], 1);
+  atomicOr(&tags[c], 1ull << ((tag - 1) & 63));
+  if (cellRes[donor[q]] < BIGVALUE) atomicOr(&good[c], 1);
+}
+
+struct CartHoles {
+  int nmesh;
+  const HoleMapView *hm;
+};
+
+__device__ __forceinline__ int cart_in_hole(const HoleMapView &h, const double x[3])
+{
+  double dx[3];
+  int ix[3];
+  for (int i = 0; i < 3; i++) dx[i] = (h.ext[i + 3] - h.ext[i]) / h.nx[i];
+  for (int i = 0; i < 3; i++) {
+    ix[i] = (int)((x[i] - h.ext[i]) / dx[i]);
+    if (ix[i] < 0 || ix[i] > h.nx[i] - 1) return 0;
+  }
+  return h.sam[(size_t)ix[2] * h.nx[1] * h.nx[0] + (size_t)ix[1] * h.nx[0] + ix[0]];
+}
+
+// CartBlock::processDonors (CartBlock.C:308-504) for pdegree 0, one thread
+// per cell of one patch: a cell without donors inside a hole map, or with
+// donors but inside the hole map of a mesh that did not donate, becomes a
+// hole (0); a non-hole cell with a donor of resolution < BIGVALUE becomes a
+// receptor (-1); the donors of every other cell are cancelled
+__global__ void k_cart_process(int n, AmrGridDev g, double qn0, int nf, long long c0, CartHoles H,
+                               const int *__restrict__ cnt, const unsigned long long *__restrict__ tags,
+                               const int *__restrict__ good, int *__restrict__ cancel, int *__restrict__ ibl)
+{
+  const int t = blockIdx.x * ABS + threadIdx.x;
+  if (t >= n) return;
+  const int k = t / (g.dims[0] * g.dims[1]), j = (t / g.dims[0]) % g.dims[1], i = t % g.dims[0];
+  double x[3];
+  cart_centre(g, qn0, i, j, k, x);
+  const long long c = c0 + t;
+  const int ib = (k + nf) * (g.dims[1] + 2 * nf) * (g.dims[0] + 2 * nf) + (j + nf) * (g.dims[0] + 2 * nf) + i + nf;
+  for (int h = 0; h < H.nmesh; h++) {
+    if (!H.hm[h].exist) continue;
+    if (cnt[c] != 0 && h < 64 && ((tags[c] >> h) & 1ull)) continue;
+    if (cart_in_hole(H.hm[h], x)) {
+      ibl[ib] = 0;
+      break;
+    }
+  }
+  if (ibl[ib] == 0) cancel[c] = 1;
+  else if (good[c]) ibl[ib] = -1;
+  else cancel[c] = 1;
+}
+
+__global__ void k_cart_cancel(int n, int *__restrict__ donor, const int *__restrict__ is, const int *__restrict__ pd3,
+                              const long long *__restrict__ coff, const int *__restrict__ pgid,
+                              const AmrGridDev *__restrict__ G, int nf, const int *__restrict__ cancel,
+                              int *__restrict__ flag)
+{
+  const int q = blockIdx.x * ABS + threadIdx.x;
+  if (q >= n) return;
+  if (donor[q] != -1 && cancel[cart_cell_of(is, pd3, coff, pgid, G, nf, q)]) donor[q] = -1;
+  flag[q] = donor[q] != -1;
+}
+
+// MeshBlock::findInterpListCart (cartOps.C:203-278): the donor cell's
+// isoparametric weights at the Cartesian point, record after record
+template <int NV>
+__device__ void cart_record(int n, int r, int c, const int *conn, const double *x, const double xp[3], int *inode,
+                            double *w, int *nw)
+{
+  double xv[8][3], frac[8];
+  const int *cv = conn + 8 * (size_t)c;
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    const size_t v = cv[m];
+    xv[m][0] = x[3 * v];
+    xv[m][1] = x[3 * v + 1];
+    xv[m][2] = x[3 * v + 2];
+  }
+  nodal_weights(xv, xp, frac, NV);
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    inode[(size_t)m * n + r] = cv[m];
+    w[(size_t)m * n + r] = frac[m];
+  }
+  nw[r] = NV;
+}
+
+__global__ void k_cart_records(int n, const int *__restrict__ sel, TypeInfo ti, const int *__restrict__ conn,
+                               const double *__restrict__ x, const double *__restrict__ xs, const int *__restrict__ is,
+                               const int *__restrict__ donor, int *__restrict__ inode, double *__restrict__ w,
+                               int *__restrict__ nw, int *__restrict__ rpt, int *__restrict__ rlp)
+{
+  const int r = blockIdx.x * ABS + threadIdx.x;
+  if (r >= n) return;
+  const int q = sel[r], c = donor[q];
+  const int t = type_of_cell(ti, c);
+  const double xp[3] = {xs[3 * (size_t)q], xs[3 * (size_t)q + 1], xs[3 * (size_t)q + 2]};
+  switch (ti.nv[t]) {
+    case 4: cart_record<4>(n, r, c, conn, x, xp, inode, w, nw); break;
+    case 5: cart_record<5>(n, r, c, conn, x, xp, inode, w, nw); break;
+    case 6: cart_record<6>(n, r, c, conn, x, xp, inode, w, nw); break;
+    default: cart_record<8>(n, r, c, conn, x, xp, inode, w, nw); break;
+  }
+  rpt[r] = is[3 * (size_t)q + 2];
+  rlp[r] = is[3 * (size_t)q + 1];
+}
+
+// the values MeshBlock::getInterpolatedSolutionAMR sends a Cartesian
+// receptor (cartOps.C:366-385, 412-428): per field, vertices in order
+__global__ void k_cart_values(int n, int nvar, int column, int nnodes, const double *__restrict__ q,
+                              const int *__restrict__ inode, const double *__restrict__ w, const int *__restrict__ nw,
+                              double *__restrict__ out)
+{
+  const int r = blockIdx.x * ABS + threadIdx.x;
+  if (r >= n) return;
+  for (int k = 0; k < nvar; k++) {
+    double qq = 0;
+    for (int m = 0; m < nw[r]; m++) {
+      const size_t v = inode[(size_t)m * n + r];
+      qq += (column ? q[(size_t)k * nnodes + v] : q[v * nvar + k]) * w[(size_t)m * n + r];
+    }
+    out[(size_t)r * nvar + k] = qq;
+  }
+}
+
+}  // namespace
+
+// The Cartesian-receptor direction of performConnectivityAMR on one rank
+// (tioga.C:190-207): getCartReceptors and the search of each block, the donor
+// half of exchangeAMRDonors (exchangeAMRDonors.C:120-287: donor lists on the
+// patch cells, CartBlock::processDonors, cancellation), findInterpListCart.
+// Every block's donor entries follow each other in block order (the
+// reference packs interpolation and donor entries block by block and reads
+// them back as all of one kind, then all of the other, exchangeAMRDonors.C:
+// 118-199, which agrees only with one near-body block).
+void Engine::amr_cart_receptors()
+{
+  AmrState &A = amr();
+  const int nbk = (int)blocks_.size();
+  const int G = A.ngrids;
+  const double qn0 = A.qnode.empty() ? 0.0 : A.qnode[0];
+  std::vector<AmrGridDev> gt(std::max(G, 1));
+  for (int g = 0; g < G; g++)
+    for (int n = 0; n < 3; n++) {
+      gt[g].xlo[n] = A.xlo[3 * g + n];
+      gt[g].dx[n] = A.dx[3 * g + n];
+      gt[g].dims[n] = A.dims[3 * g + n];
+      gt[g].level = A.level[g];
+    }
+  // 1. each block's Cartesian points, laid out as node queries are
+  DBuf<int> flag, sel;
+  std::vector<std::unique_ptr<DBuf<int>>> cells;  // per block: selected cell numbers
+  std::vector<std::vector<AmrGridDev>> bg(nbk);
+  std::vector<std::vector<long long>> boff(nbk);
+  std::vector<std::vector<int>> blid(nbk);
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    cells.emplace_back(new DBuf<int>);
+    boff[ib].push_back(0);
+    for (int g = 0; g < G; g++) {  // the grids whose box meets the block's OBB (getCartReceptors.C:89-101)
+      OBB oc{};
+      for (int n = 0; n < 3; n++) {
+        oc.vec[n][n] = 1.0;
+        oc.dxc[n] = A.dx[3 * g + n] * A.dims[3 * g + n] * 0.5;
+        oc.xc[n] = A.xlo[3 * g + n] + oc.dxc[n];
+      }
+      if (!(obb_intersect(b.obb, oc) || obb_intersect(oc, b.obb))) continue;
+      bg[ib].push_back(gt[g]);
+      blid[ib].push_back(A.lid[g]);
+      boff[ib].push_back(boff[ib].back() + (long long)A.dims[3 * g] * A.dims[3 * g + 1] * A.dims[3 * g + 2]);
+    }
+    const long long tot = boff[ib].back();
+    if (tot > 0x7fffffffLL) throw std::runtime_error("tioga_amd: too many Cartesian candidate cells for one block");
+    b.nsearch = 0;
+    if (!tot) continue;
+    DBuf<AmrGridDev> dg;
+    DBuf<long long> doff;
+    DBuf<int> dlid;
+    const int ng = (int)bg[ib].size();
+    dg.reserve(ng);
+    doff.reserve(ng + 1);
+    dlid.reserve(ng);
+    TG_HIP(hipMemcpyAsync(dg.p, bg[ib].data(), sizeof(AmrGridDev) * ng, hipMemcpyHostToDevice, s_));
+    TG_HIP(hipMemcpyAsync(doff.p, boff[ib].data(), sizeof(long long) * (ng + 1), hipMemcpyHostToDevice, s_));
+    TG_HIP(hipMemcpyAsync(dlid.p, blid[ib].data(), sizeof(int) * ng, hipMemcpyHostToDevice, s_));
+    const CartGridList L{ng, dg.p, doff.p, dlid.p};
+    flag.reserve(tot);
+    cells[ib]->reserve(tot);
+    k_cart_flags<<<anb(tot), ABS, 0, s_>>>(tot, L, qn0, b.obb, flag.p);
+    b.nsearch = select_flagged(flag.p, (int)tot, cells[ib]->p, s_);
+  }
+  size_t qtot = 0;
+  for (Block *b : blocks_) qtot += b->nsearch;
+  qx_.reserve(3 * qtot);
+  qres_.reserve(qtot);
+  qtag_.reserve(qtot);
+  qrep_.reserve(qtot);
+  size_t qo = 0;
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    const size_t n = b.nsearch;
+    b.xsearch.view(qx_.p + 3 * qo, 3 * n);
+    b.res_search.view(qres_.p + qo, n);
+    b.tagsearch.view(qtag_.p + qo, n);
+    b.xtag.view(qrep_.p + qo, n);
+    qo += n;
+    b.isearch.reserve(3 * std::max(n, (size_t)1));
+    b.donorId.reserve(std::max(n, (size_t)1));
+    if (!n) continue;
+    DBuf<AmrGridDev> dg;
+    DBuf<long long> doff;
+    DBuf<int> dlid;
+    const int ng = (int)bg[ib].size();
+    dg.reserve(ng);
+    doff.reserve(ng + 1);
+    dlid.reserve(ng);
+    TG_HIP(hipMemcpyAsync(dg.p, bg[ib].data(), sizeof(AmrGridDev) * ng, hipMemcpyHostToDevice, s_));
+    TG_HIP(hipMemcpyAsync(doff.p, boff[ib].data(), sizeof(long long) * (ng + 1), hipMemcpyHostToDevice, s_));
+    TG_HIP(hipMemcpyAsync(dlid.p, blid[ib].data(), sizeof(int) * ng, hipMemcpyHostToDevice, s_));
+    const CartGridList L{ng, dg.p, doff.p, dlid.p};
+    k_cart_points<<<anb(n), ABS, 0, s_>>>((int)n, cells[ib]->p, L, qn0, A.nf, b.xsearch.p, b.isearch.p,
+                                          b.res_search.p, b.tagsearch.p);
+    sync();
+  }
+  // 2. the searches (MeshBlock::search)
+  if (batched) {
+    search_batched();
+    launch_searches();
+  } else {
+    for (Block *b : blocks_) search(*b);
+    for (Block *b : blocks_) launch_search(*b);
+  }
+  // 3. donor lists on the patch cells, processDonors, cancellation
+  A.coff.assign(A.npatch + 1, 0);
+  A.ioff.assign(A.npatch + 1, 0);
+  std::vector<int> pd3(std::max(A.npatch, 1));
+  for (int p = 0; p < A.npatch; p++) {
+    const int g = A.pgid[p];
+    A.coff[p + 1] = A.coff[p] + (long long)A.dims[3 * g] * A.dims[3 * g + 1] * A.dims[3 * g + 2];
+    pd3[p] = (A.dims[3 * g] + 2 * A.nf) * (A.dims[3 * g + 1] + 2 * A.nf) * (A.dims[3 * g + 2] + 2 * A.nf);
+    A.ioff[p + 1] = A.ioff[p] + pd3[p];
+  }
+  const long long NC = A.coff[A.npatch], NI = A.ioff[A.npatch];
+  if (NC > (1ll << 31) || NI > (1ll << 31)) throw std::runtime_error("tioga_amd: too many AMR patch cells");
+  A.dcoff.reserve(A.npatch + 1);
+  A.dioff.reserve(A.npatch + 1);
+  A.ccnt.reserve(std::max(NC, 1ll));
+  A.cgood.reserve(std::max(NC, 1ll));
+  A.ccancel.reserve(std::max(NC, 1ll));
+  A.ctags.reserve(std::max(NC, 1ll));
+  A.pibl_dev.reserve(std::max(NI, 1ll));
+  DBuf<int> dpd3;
+  dpd3.reserve(std::max(A.npatch, 1));
+  TG_HIP(hipMemcpyAsync(A.dcoff.p, A.coff.data(), sizeof(long long) * (A.npatch + 1), hipMemcpyHostToDevice, s_));
+  TG_HIP(hipMemcpyAsync(A.dioff.p, A.ioff.data(), sizeof(long long) * (A.npatch + 1), hipMemcpyHostToDevice, s_));
+  if (A.npatch) TG_HIP(hipMemcpyAsync(dpd3.p, pd3.data(), sizeof(int) * A.npatch, hipMemcpyHostToDevice, s_));
+  if (NC) {
+    TG_HIP(hipMemsetAsync(A.ccnt.p, 0, sizeof(int) * NC, s_));
+    TG_HIP(hipMemsetAsync(A.cgood.p, 0, sizeof(int) * NC, s_));
+    TG_HIP(hipMemsetAsync(A.ccancel.p, 0, sizeof(int) * NC, s_));
+    TG_HIP(hipMemsetAsync(A.ctags.p, 0, sizeof(unsigned long long) * NC, s_));
+  }
+  for (int p = 0; p < A.npatch; p++) {  // the patch iblanks as registered
+    if (!A.pibl[p]) throw std::runtime_error("tioga_amd: AMR patch iblank not registered");
+    TG_HIP(hipMemcpyAsync(A.pibl_dev.p + A.ioff[p], A.pibl[p], sizeof(int) * pd3[p], hipMemcpyHostToDevice, s_));
+  }
+  for (Block *bp : blocks_) {
+    Block &b = *bp;
+    if (b.tag > 64 && b.nsearch) throw std::runtime_error("tioga_amd: AMR donor meshes need tags <= 64");
+    if (b.nsearch)
+      k_cart_donors<<<anb(b.nsearch), ABS, 0, s_>>>(b.nsearch, b.donorId.p, b.isearch.p, b.cellRes.p, b.tag, dpd3.p,
+                                                    A.dcoff.p, A.ppgid.p, A.grids.p, A.nf, A.ccnt.p, A.ctags.p,
+                                                    A.cgood.p);
+  }
+  const CartHoles H{nmesh_, hm_view_.p};
+  if (nmesh_ > 64) throw std::runtime_error("tioga_amd: AMR hole maps need mesh tags <= 64");
+  for (int p = 0; p < A.npatch; p++) {
+    const int g = A.pgid[p];
+    const int nc = (int)(A.coff[p + 1] - A.coff[p]);
+    if (nc)
+      k_cart_process<<<anb(nc), ABS, 0, s_>>>(nc, gt[g], qn0, A.nf, A.coff[p], H, A.ccnt.p, A.ctags.p, A.cgood.p,
+                                              A.ccancel.p, A.pibl_dev.p + A.ioff[p]);
+  }
+  for (int p = 0; p < A.npatch; p++)
+    TG_HIP(hipMemcpyAsync(A.pibl[p], A.pibl_dev.p + A.ioff[p], sizeof(int) * pd3[p], hipMemcpyDeviceToHost, s_));
+  // 4. findInterpListCart
+  A.crec.clear();
+  for (int ib = 0; ib < nbk; ib++) A.crec.emplace_back(new AmrState::CartRecs);
+  for (int ib = 0; ib < nbk; ib++) {
+    Block &b = *blocks_[ib];
+    AmrState::CartRecs &R = *A.crec[ib];
+    R.n = 0;
+    if (!b.nsearch) continue;
+    flag.reserve(b.nsearch);
+    sel.reserve(b.nsearch);
+    k_cart_cancel<<<anb(b.nsearch), ABS, 0, s_>>>(b.nsearch, b.donorId.p, b.isearch.p, dpd3.p, A.dcoff.p, A.ppgid.p,
+                                                  A.grids.p, A.nf, A.ccancel.p, flag.p);
+    const int n = select_flagged(flag.p, b.nsearch, sel.p, s_);
+    R.n = n;
+    if (!n) continue;
+    R.inode.reserve(8 * (size_t)n);
+    R.w.reserve(8 * (size_t)n);
+    R.nw.reserve(n);
+    R.rpt.reserve(n);
+    R.rlp.reserve(n);
+    k_cart_records<<<anb(n), ABS, 0, s_>>>(n, sel.p, b.ti, b.conn.p, b.x.p, b.xsearch.p, b.isearch.p, b.donorId.p,
+                                           R.inode.p, R.w.p, R.nw.p, R.rpt.p, R.rlp.p);
+  }
+  sync();
 }
 
 void Engine::amr_release()
@@ -677,6 +1117,8 @@ void Engine::perform_connectivity_amr()
     }
   }
   if (A.nrec == 0) A.rec_count.assign(A.npatch, 0);
+  // the other direction: patch cells that take values from the near-body blocks
+  amr_cart_receptors();
   // setCartIblanks, getCellIblanks, write back
   for (int ib = 0; ib < nbk; ib++) {
     Block &b = *blocks_[ib];
@@ -766,12 +1208,43 @@ void Engine::data_update_amr(int nvar, int interptype)
   }
   amr_upload_q(nvar);
   amr_tables();
+  // the Cartesian receptors' values, from the near-body solutions before the
+  // gather writes their receptors (every value is formed before any is
+  // written, as the packets of tioga.C:286-321 are)
+  size_t ncv = 0;
+  std::vector<size_t> cv0(A.crec.size() + 1, 0);
+  for (size_t ib = 0; ib < A.crec.size(); ib++) cv0[ib + 1] = cv0[ib] + (size_t)A.crec[ib]->n * nvar;
+  ncv = cv0.back();
+  DBuf<double> cvals;
+  cvals.reserve(std::max<size_t>(ncv, 1));
+  for (size_t ib = 0; ib < A.crec.size(); ib++) {
+    const AmrState::CartRecs &R = *A.crec[ib];
+    if (R.n)
+      k_cart_values<<<anb(R.n), ABS, 0, s_>>>(R.n, nvar, interptype == 1, blocks_[ib]->nnodes, qdev_[ib]->p,
+                                              R.inode.p, R.w.p, R.nw.p, cvals.p + cv0[ib]);
+  }
   if (A.nrec) amr_gather(nvar, interptype);
   for (int ib = 0; ib < nbk; ib++) {
     Block &b = *blocks_[ib];
     TG_HIP(hipMemcpyAsync(b.uq, qdev_[ib]->p, sizeof(double) * (size_t)b.nnodes * nvar, hipMemcpyDeviceToHost, s_));
   }
+  A.cvals.resize(ncv);
+  if (ncv) TG_HIP(hipMemcpyAsync(A.cvals.data(), cvals.p, sizeof(double) * ncv, hipMemcpyDeviceToHost, s_));
   sync();
+  // CartBlock::update (CartBlock.C:146-151): block after block, record after record
+  for (size_t ib = 0; ib < A.crec.size(); ib++) {
+    const AmrState::CartRecs &R = *A.crec[ib];
+    if (!R.n) continue;
+    std::vector<int> rpt(R.n), rlp(R.n);
+    TG_HIP(hipMemcpy(rpt.data(), R.rpt.p, sizeof(int) * R.n, hipMemcpyDeviceToHost));
+    TG_HIP(hipMemcpy(rlp.data(), R.rlp.p, sizeof(int) * R.n, hipMemcpyDeviceToHost));
+    for (int r = 0; r < R.n; r++) {
+      const int p = rlp[r], g = A.pgid[p];
+      const size_t d3 = (size_t)(A.dims[3 * g] + 2 * A.nf) * (A.dims[3 * g + 1] + 2 * A.nf) * (A.dims[3 * g + 2] + 2 * A.nf);
+      if (!A.pq[p]) throw std::runtime_error("tioga_amd: AMR patch solution not registered");
+      for (int k = 0; k < nvar; k++) A.pq[p][(size_t)rpt[r] + d3 * k] = A.cvals[cv0[ib] + (size_t)r * nvar + k];
+    }
+  }
 }
 
 // Device-resident dataUpdate_AMR: q of the blocks and patches uploaded once,
@@ -818,6 +1291,44 @@ long Engine::amr_array(int btag, const std::string &name, void *out)
 {
   if (!amr_ || !amr_->connected) return 0;
   AmrState &A = *amr_;
+  if (name == "patch_iblank") {  // btag: the local patch index
+    if (btag < 0 || btag >= A.npatch) return -1;
+    const long long n = A.ioff.size() > (size_t)btag + 1 ? A.ioff[btag + 1] - A.ioff[btag] : 0;
+    if (out && n) memcpy(out, A.pibl[btag], sizeof(int) * n);
+    return (long)n;
+  }
+  if (btag >= 1 && name.compare(0, 11, "interpCart_") == 0) {  // MeshBlock::interpListCart
+    int ib = -1;
+    if (!find(btag, &ib)) return -1;
+    if ((size_t)ib >= A.crec.size()) return 0;
+    const AmrState::CartRecs &R = *A.crec[ib];
+    const int n = R.n;
+    std::vector<int> nw(n), inode(8 * (size_t)n), rpt(n), rlp(n);
+    std::vector<double> w(8 * (size_t)n);
+    if (n) {
+      TG_HIP(hipMemcpy(nw.data(), R.nw.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+      TG_HIP(hipMemcpy(inode.data(), R.inode.p, sizeof(int) * 8 * n, hipMemcpyDeviceToHost));
+      TG_HIP(hipMemcpy(w.data(), R.w.p, sizeof(double) * 8 * n, hipMemcpyDeviceToHost));
+      TG_HIP(hipMemcpy(rpt.data(), R.rpt.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+      TG_HIP(hipMemcpy(rlp.data(), R.rlp.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    }
+    if (name == "interpCart_rinfo") {  // (rank, q index, local patch) as cartOps.C:258-260 stores them
+      for (int r = 0; out && r < n; r++) {
+        ((int *)out)[3 * r] = 0;
+        ((int *)out)[3 * r + 1] = rpt[r];
+        ((int *)out)[3 * r + 2] = rlp[r];
+      }
+      return 3l * n;
+    }
+    long k = 0;
+    for (int r = 0; r < n; r++)
+      for (int m = 0; m < nw[r]; m++, k++)
+        if (out) {
+          if (name == "interpCart_inode") ((int *)out)[k] = inode[(size_t)m * n + r];
+          else if (name == "interpCart_w") ((double *)out)[k] = w[(size_t)m * n + r];
+        }
+    return name == "interpCart_inode" || name == "interpCart_w" ? k : -1;
+  }
   if (btag >= 1) {
     int ib = -1;
     Block *b = find(btag, &ib);

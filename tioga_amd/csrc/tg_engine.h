@@ -837,6 +837,7 @@ class Engine {
   void amr_upload_q(int nvar);
   void amr_tables();
   void amr_gather(int nvar, int interptype);
+  void amr_cart_receptors();
 };
 
 }  // namespace tg

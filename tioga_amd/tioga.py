@@ -126,7 +126,7 @@ def _ip(v: int):
 
 
 _I32 = {"isearch", "tagsearch", "donorId", "xtag", "elementList", "adtIntegers", "iblank_cell"}
-_AMR_F64 = {"rxyzCart", "rec_w"}
+_AMR_F64 = {"rxyzCart", "rec_w", "interpCart_w"}
 PHASES = ("holemap", "boxes", "searchdata", "search_prep", "search_kernel", "donors", "total", "nsearch",
           "prep_obb", "prep_filter", "prep_adt", "prep_dedupe", "obb_stepped")
 
@@ -143,6 +143,8 @@ class Tioga:
         self._keep: List[object] = []
         self.blocks = []
         self.ranks: List[int] = []  # virtual rank of each registered block
+        self.patch_q: List[np.ndarray] = []    # AMR patches' solutions and iblanks, as registered
+        self.patch_ibl: List[np.ndarray] = []
         self.iblank: List[np.ndarray] = []
         self.iblank_cell: List[np.ndarray] = []
         self.q: Dict[int, np.ndarray] = {}
@@ -367,6 +369,8 @@ class Tioga:
             ibl = np.ones(grid.d3nf(g), np.int32)
             q = np.ascontiguousarray(q, np.float64)
             self._keep += [ibl, q]
+            self.patch_q.append(q)
+            self.patch_ibl.append(ibl)
             self.L.tioga_register_amr_local_data_(_ip(lp), _ip(g), _p(ibl), _p(q))
             self._check()
 
@@ -389,8 +393,11 @@ class Tioga:
         self._check()
         return ms, n.value
 
-    def amr_results(self, qs=None, nvar: int = 0, interptype: int = 0) -> Dict[str, np.ndarray]:
-        """The arrays the reference driver dumps for the AMR gather path."""
+    def amr_results(self, qs=None, nvar: int = 0, interptype: int = 0, full: bool = False) -> Dict[str, np.ndarray]:
+        """The arrays the reference driver dumps for the AMR path; full adds
+        the Cartesian-receptor direction: the near-body search of the patch
+        cells, interpListCart, every patch's iblank and, after the update,
+        every patch's q."""
         r: Dict[str, np.ndarray] = {}
         for i, b in enumerate(self.blocks):
             p = f"b{i}."
@@ -400,11 +407,26 @@ class Tioga:
             for nm in ("receptorIdCart", "donorIdCart", "rxyzCart"):
                 r[p + nm] = self.amr_array(b.tag, nm)
             r[p + "ncart"] = np.array([r[p + "receptorIdCart"].size], np.int32)
+            if full:
+                for nm in ("xsearch", "isearch", "donorId", "xtag"):
+                    a = self.array(b.tag, nm)
+                    if a.size:
+                        r[p + nm] = a
+                r[p + "nsearch"] = np.array([self.array(b.tag, "donorId").size], np.int32)
+                for nm in ("interpCart_rinfo", "interpCart_inode", "interpCart_w"):
+                    r[p + nm] = self.amr_array(b.tag, nm)
+                r[p + "ninterpCart"] = np.array([r[p + "interpCart_rinfo"].size // 3], np.int32)
         for nm in ("rec_patch", "rec_info", "rec_ijk", "rec_w"):
             r["amr." + nm] = self.amr_array(0, nm)
+        if full:
+            for lp, ibl in enumerate(self.patch_ibl):
+                r[f"amr.patch{lp}.iblank"] = ibl.copy()
         if qs is not None:
             for i, q in enumerate(self.data_update_copies(qs, nvar, interptype)):
                 r[f"b{i}.q"] = q
+            if full:
+                for lp, q in enumerate(self.patch_q):
+                    r[f"amr.patch{lp}.q"] = q.copy()
         return r
 
     # ---- serving: moving blocks over the resident background ----
