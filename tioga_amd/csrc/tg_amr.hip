@@ -509,20 +509,31 @@ __device__ __forceinline__ int cart_in_hole(const HoleMapView &h, const double x
 }
 
 // CartBlock::processDonors (CartBlock.C:308-504) for pdegree 0, one thread
-// per cell of one patch: a cell without donors inside a hole map, or with
-// donors but inside the hole map of a mesh that did not donate, becomes a
-// hole (0); a non-hole cell with a donor of resolution < BIGVALUE becomes a
-// receptor (-1); the donors of every other cell are cancelled
-__global__ void k_cart_process(int n, AmrGridDev g, double qn0, int nf, long long c0, CartHoles H,
+// per patch cell (every patch's cells, numbered patch after patch from
+// coff): a cell without donors inside a hole map, or with donors but inside
+// the hole map of a mesh that did not donate, becomes a hole (0); a non-hole
+// cell with a donor of resolution < BIGVALUE becomes a receptor (-1); the
+// donors of every other cell are cancelled
+__global__ void k_cart_process(long long n, int npatch, const long long *__restrict__ coff,
+                               const long long *__restrict__ ioff, const int *__restrict__ pgid,
+                               const AmrGridDev *__restrict__ G, double qn0, int nf, CartHoles H,
                                const int *__restrict__ cnt, const unsigned long long *__restrict__ tags,
-                               const int *__restrict__ good, int *__restrict__ cancel, int *__restrict__ ibl)
+                               const int *__restrict__ good, int *__restrict__ cancel, int *__restrict__ ibl_all)
 {
-  const int t = blockIdx.x * ABS + threadIdx.x;
-  if (t >= n) return;
+  const long long c = (long long)blockIdx.x * ABS + threadIdx.x;
+  if (c >= n) return;
+  int lo = 0, hi = npatch - 1;  // the last patch whose first cell is <= c
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (coff[m] <= c) lo = m;
+    else hi = m - 1;
+  }
+  const AmrGridDev g = G[pgid[lo]];
+  const int t = (int)(c - coff[lo]);
   const int k = t / (g.dims[0] * g.dims[1]), j = (t / g.dims[0]) % g.dims[1], i = t % g.dims[0];
   double x[3];
   cart_centre(g, qn0, i, j, k, x);
-  const long long c = c0 + t;
+  int *ibl = ibl_all + ioff[lo];
   const int ib = (k + nf) * (g.dims[1] + 2 * nf) * (g.dims[0] + 2 * nf) + (j + nf) * (g.dims[0] + 2 * nf) + i + nf;
   for (int h = 0; h < H.nmesh; h++) {
     if (!H.hm[h].exist) continue;
@@ -747,10 +758,13 @@ void Engine::amr_cart_receptors()
     TG_HIP(hipMemsetAsync(A.ccancel.p, 0, sizeof(int) * NC, s_));
     TG_HIP(hipMemsetAsync(A.ctags.p, 0, sizeof(unsigned long long) * NC, s_));
   }
-  for (int p = 0; p < A.npatch; p++) {  // the patch iblanks as registered
+  // the patch iblanks as registered, through one host staging copy each way
+  std::vector<int> ibl_host((size_t)std::max(NI, 1ll));
+  for (int p = 0; p < A.npatch; p++) {
     if (!A.pibl[p]) throw std::runtime_error("tioga_amd: AMR patch iblank not registered");
-    TG_HIP(hipMemcpyAsync(A.pibl_dev.p + A.ioff[p], A.pibl[p], sizeof(int) * pd3[p], hipMemcpyHostToDevice, s_));
+    memcpy(ibl_host.data() + A.ioff[p], A.pibl[p], sizeof(int) * pd3[p]);
   }
+  if (NI) TG_HIP(hipMemcpyAsync(A.pibl_dev.p, ibl_host.data(), sizeof(int) * NI, hipMemcpyHostToDevice, s_));
   for (Block *bp : blocks_) {
     Block &b = *bp;
     if (b.tag > 64 && b.nsearch) throw std::runtime_error("tioga_amd: AMR donor meshes need tags <= 64");
@@ -761,15 +775,12 @@ void Engine::amr_cart_receptors()
   }
   const CartHoles H{nmesh_, hm_view_.p};
   if (nmesh_ > 64) throw std::runtime_error("tioga_amd: AMR hole maps need mesh tags <= 64");
-  for (int p = 0; p < A.npatch; p++) {
-    const int g = A.pgid[p];
-    const int nc = (int)(A.coff[p + 1] - A.coff[p]);
-    if (nc)
-      k_cart_process<<<anb(nc), ABS, 0, s_>>>(nc, gt[g], qn0, A.nf, A.coff[p], H, A.ccnt.p, A.ctags.p, A.cgood.p,
-                                              A.ccancel.p, A.pibl_dev.p + A.ioff[p]);
-  }
-  for (int p = 0; p < A.npatch; p++)
-    TG_HIP(hipMemcpyAsync(A.pibl[p], A.pibl_dev.p + A.ioff[p], sizeof(int) * pd3[p], hipMemcpyDeviceToHost, s_));
+  if (NC)
+    k_cart_process<<<anb(NC), ABS, 0, s_>>>(NC, A.npatch, A.dcoff.p, A.dioff.p, A.ppgid.p, A.grids.p, qn0, A.nf, H,
+                                            A.ccnt.p, A.ctags.p, A.cgood.p, A.ccancel.p, A.pibl_dev.p);
+  if (NI) TG_HIP(hipMemcpyAsync(ibl_host.data(), A.pibl_dev.p, sizeof(int) * NI, hipMemcpyDeviceToHost, s_));
+  sync();
+  for (int p = 0; p < A.npatch; p++) memcpy(A.pibl[p], ibl_host.data() + A.ioff[p], sizeof(int) * pd3[p]);
   // 4. findInterpListCart
   A.crec.clear();
   for (int ib = 0; ib < nbk; ib++) A.crec.emplace_back(new AmrState::CartRecs);
