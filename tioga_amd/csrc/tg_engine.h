@@ -769,6 +769,8 @@ class Engine {
   size_t event_slot_ = 0;
   double last_search_kernel_ms_ = 0;
   DBuf<unsigned> mk0_, mk1_;
+  DBuf<unsigned long long> wmask_;  // the unsorted ordering: walking masks per wave of queries
+  DBuf<int> wcnt_, wpre_;            // their populations and exclusive sum
   DBuf<unsigned char> mset_;  // the per-set Morton table of a batched search
   DBuf<double> qx_, qres_;    // every block's query points / resolutions, back to back
   DBuf<int> qtag_, qrep_;     // every block's query mesh tags / representatives

@@ -1086,6 +1086,92 @@ __global__ void __launch_bounds__(256) k_walk_counts(int ns, const MortonSet *__
   }
 }
 
+// The processing order without Morton bits is each set's walking queries in
+// index order (what a stable sort on (set, skip) gives), found by a
+// compaction instead of a sort.  k_walk_mask: per wave of 64 queries, the
+// mask of the walking ones and its population; donors of the points outside
+// the root box are written as in k_morton_sets.
+__global__ void __launch_bounds__(256) k_walk_mask(int ns, int tot, const MortonSet *__restrict__ M,
+                                                    unsigned long long *__restrict__ mask, int *__restrict__ cnt)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  bool walk = false;
+  if (g < tot) {
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (M[mid].off <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    const MortonSet &m = M[lo];
+    const int i = g - m.off;
+    double p[3];
+    load_xyz(m.xs, i, p);
+    bool inroot = true;
+    for (int d = 0; d < 3; d++) inroot = inroot && (p[d] >= m.ext[2 * d] - TOL);
+    for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= m.ext[2 * d + 1] + TOL);
+    const bool dup = m.xtag && m.xtag[i] != i;
+    if (!inroot && !dup) m.donorId[i] = -1;
+    walk = inroot && !dup;
+  }
+  const unsigned long long b = __ballot(walk);
+  if ((threadIdx.x & 63) == 0 && g < tot) {
+    mask[g >> 6] = b;
+    cnt[g >> 6] = __popcll(b);
+  }
+}
+
+// walking queries before index x of the concatenated sets (pre: the
+// exclusive sum of the per-wave populations, pre[nw] = their total)
+__device__ __forceinline__ int walkers_before(int x, int tot, const unsigned long long *__restrict__ mask,
+                                              const int *__restrict__ pre)
+{
+  if (x >= tot) return pre[(tot + 63) >> 6];
+  return pre[x >> 6] + __popcll(mask[x >> 6] & ((1ull << (x & 63)) - 1));
+}
+
+// each walking query's position: its set's offset + the walking queries of
+// its set before it; idx = its index within the set, qsid = its set
+__global__ void __launch_bounds__(256) k_walk_scatter(int ns, int tot, const MortonSet *__restrict__ M,
+                                                       const unsigned long long *__restrict__ mask,
+                                                       const int *__restrict__ pre, int *__restrict__ idx,
+                                                       int *__restrict__ qsid)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= tot) return;
+  const unsigned long long mw = mask[g >> 6];
+  if (!((mw >> (g & 63)) & 1ull)) return;
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (M[mid].off <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const int off = M[lo].off;
+  const int t = walkers_before(g, tot, mask, pre) - walkers_before(off, tot, mask, pre);
+  idx[off + t] = g - off;
+  if (ns > 1) qsid[off + t] = lo;
+}
+
+// nin and wstart as k_walk_counts forms them, from the compaction's counts
+__global__ void __launch_bounds__(256) k_walk_counts_mask(int ns, int tot, const MortonSet *__restrict__ M,
+                                                           const unsigned long long *__restrict__ mask,
+                                                           const int *__restrict__ pre, int *__restrict__ nin,
+                                                           int *__restrict__ wstart)
+{
+  for (int k = threadIdx.x; k < ns; k += blockDim.x)
+    nin[k] = walkers_before(M[k].off + M[k].nq, tot, mask, pre) - walkers_before(M[k].off, tot, mask, pre);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int w = 0;
+    for (int k = 0; k < ns; k++) {
+      wstart[k] = w;
+      w += (nin[k] + 63) / 64;
+    }
+    wstart[ns] = w;
+  }
+}
+
 struct SearchView {
   int nq;
   const int *perm;  // processing order (query index of thread t), or null
@@ -1417,8 +1503,12 @@ __device__ __forceinline__ void find_lane(const SearchView &S, int i, unsigned q
   len = S.boff[bin + 1] - start;
 }
 
-// nseg[w] = segments of walking wave w (at least one), 0 past the walking waves
-__global__ void __launch_bounds__(256) k_find_cost(SearchSet T, int nwave, int seg, int *__restrict__ nseg)
+// nseg[w] = segments of walking wave w (at least one), 0 past the walking
+// waves; and each walking query's outcome words start out empty (best,
+// last and the tie-band count; no other position is read)
+__global__ void __launch_bounds__(256) k_find_cost(SearchSet T, int nwave, int seg, int *__restrict__ nseg,
+                                                   int *__restrict__ best, int *__restrict__ last,
+                                                   int *__restrict__ acc)
 {
   const int w = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
   if (w >= nwave) return;
@@ -1429,11 +1519,17 @@ __global__ void __launch_bounds__(256) k_find_cost(SearchSet T, int nwave, int s
   }
   const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, w));
   const SearchView S = view_of(T, sid);
+  const int t = (w - T.wstart[sid]) * WAVE + lane;
+  if (t < T.nin[sid]) {
+    const int gp = T.qbase[sid] + t;
+    best[gp] = 0x7fffffff;
+    last[gp] = -1;
+    if (acc) acc[gp] = 0;
+  }
   if (!S.boff) {  // a tree without bins: its waves walk (k_collect)
     if (lane == 0) nseg[w] = 0;
     return;
   }
-  const int t = (w - T.wstart[sid]) * WAVE + lane;
   const int i = t < T.nin[sid] ? S.perm[t] : -1;
   unsigned q[3] = {0u, 0u, 0u};
   int start, len;
@@ -1742,24 +1838,27 @@ __global__ void __launch_bounds__(256, 4) k_eval_long(SearchSet T, CandOut O, Lo
 // candidate's outcome; with a tie-band counter the queries whose donor would
 // depend on the walk order are counted (more than one candidate accepts the
 // point, or one accepts it with a near-zero weight in a BIGVALUE cell)
-__global__ void k_select(SearchSet T, CandOut O)
+__global__ void __launch_bounds__(256) k_select(SearchSet T, CandOut O)
 {
-  const int gp = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gp >= T.nqt) return;
-  const int sid = T.ns == 1 ? 0 : T.qsid[gp];
-  const int t = gp - T.qbase[sid];
-  if (t >= T.nin[sid]) return;
-  const SearchView S = view_of(T, sid);
-  const int i = S.perm[t];
-  const int best = O.best[gp], last = O.last[gp];
-  int res = -1;
-  if (best != 0x7fffffff) res = S.rcell[best];
-  else if (last >= 0 && (last & 3) == 1) res = S.rcell[last >> 2];
-  if (S.tieband && O.acc) {
-    const int a = O.acc[gp];
-    if ((a & 0xfffff) > 1 || (a >> 20) > 0) atomicAdd(S.tieband, 1);
+  const int lane = threadIdx.x & (WAVE - 1), nwalk = T.wstart[T.ns];
+  const int step = gridDim.x * (blockDim.x / WAVE);
+  for (int w = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE; w < nwalk; w += step) {
+    const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, w));
+    const int t = (w - T.wstart[sid]) * WAVE + lane;
+    if (t >= T.nin[sid]) continue;
+    const SearchView S = view_of(T, sid);
+    const int gp = T.qbase[sid] + t;
+    const int i = S.perm[t];
+    const int best = O.best[gp], last = O.last[gp];
+    int res = -1;
+    if (best != 0x7fffffff) res = S.rcell[best];
+    else if (last >= 0 && (last & 3) == 1) res = S.rcell[last >> 2];
+    if (S.tieband && O.acc) {
+      const int a = O.acc[gp];
+      if ((a & 0xfffff) > 1 || (a >> 20) > 0) atomicAdd(S.tieband, 1);
+    }
+    S.donorId[i] = res;
   }
-  S.donorId[i] = res;
 }
 
 // ---- bin grids of walk records (k_find's lists) ----
@@ -2560,9 +2659,6 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
   w.best.reserve(nqt);
   w.last.reserve(nqt);
   if (tieband) w.acc.reserve(nqt);
-  fill<int>(w.best.p, nqt, 0x7fffffff, s);
-  fill<int>(w.last.p, nqt, -1, s);
-  if (tieband) fill<int>(w.acc.p, nqt, 0, s);
   const SearchSet T{(const SearchView *)w.sv.p, w.qb.p, w.nin.p, w.wstart.p, w.qsid.p, ns, nqt};
   long long nwave = 0;  // an upper bound of the walking waves
   for (int k = 0; k < ns; k++) nwave += (S[k].nq + WAVE - 1) / WAVE;
@@ -2580,8 +2676,13 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     w.nseg.reserve((size_t)nwave + 1);
     w.segstart.reserve((size_t)nwave + 1);
     fill<int>(w.nseg.p + nwave, 1, 0, s);
-    k_find_cost<<<nb(nwave * WAVE), BS, 0, s>>>(T, (int)nwave, w.tune.seg, w.nseg.p);
+    k_find_cost<<<nb(nwave * WAVE), BS, 0, s>>>(T, (int)nwave, w.tune.seg, w.nseg.p, w.best.p, w.last.p,
+                                                tieband ? w.acc.p : nullptr);
     exclusive_sum(w.nseg.p, w.segstart.p, (int)nwave + 1, s);
+  } else {
+    fill<int>(w.best.p, nqt, 0x7fffffff, s);
+    fill<int>(w.last.p, nqt, -1, s);
+    if (tieband) fill<int>(w.acc.p, nqt, 0, s);
   }
   if (w.cap == 0) w.cap = std::max<size_t>(4 * (size_t)nqt, 1 << 20);
   for (;;) {
@@ -2636,7 +2737,7 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     k_eval<4, 8><<<eg, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
     TG_HIP(hipStreamWaitEvent(s, w.join, 0));
     k_last<<<4096, 256, 0, s>>>(O);
-    k_select<<<nb(nqt), BS, 0, s>>>(T, O);
+    k_select<<<(unsigned)std::min<long long>((nwave + 3) / 4, 8192), 256, 0, s>>>(T, O);
     return;
   }
 }
@@ -4128,9 +4229,11 @@ int Engine::query_set_create(int btag, const double *xq, long nq)
 }
 
 // Morton keys of the sets M (all queries in one buffer, set k at M[k].off),
-// the walk/skip partition and one sort: returns the processing order (set
-// k's at the same offset) and leaves each set's count of walking queries in
-// sw_.nin; donors of points outside their tree's root box are written.
+// the walk/skip partition and one sort (with no Morton bits, a compaction of
+// the walking queries in index order): returns the processing order (set
+// k's at the same offset; only its first nin[k] positions are read) and
+// leaves each set's count of walking queries in sw_.nin; donors of points
+// outside their tree's root box are written.
 const int *Engine::order_sets(const std::vector<MortonSet> &M, size_t tot)
 {
   const int ns = (int)M.size();
@@ -4142,15 +4245,29 @@ const int *Engine::order_sets(const std::vector<MortonSet> &M, size_t tot)
   // the order only affects speed
   const int mb = std::min(std::max(sw_.tune.morton_bits, 0), 31 - rb);
   if (31 - rb < 12) throw std::runtime_error("tioga_amd: too many query sets in one search launch");
-  mk0_.reserve(tot);
-  mk1_.reserve(tot);
-  mv0_.reserve(tot);
   mv1_.reserve(tot);
   sw_.nin.reserve(ns);
   mset_.reserve(sizeof(MortonSet) * ns);
   TG_HIP(hipMemcpyAsync(mset_.p, M.data(), sizeof(MortonSet) * ns, hipMemcpyHostToDevice, s_));
   sw_.wstart.reserve(ns + 1);
   sw_.qsid.reserve(tot);
+  if (mb == 0) {  // no Morton order: a compaction
+    const size_t nw = (tot + 63) / 64;
+    wmask_.reserve(nw);
+    wcnt_.reserve(nw + 1);
+    wpre_.reserve(nw + 1);
+    fill<int>(wcnt_.p + nw, 1, 0, s_);
+    k_walk_mask<<<nb((long long)tot), BS, 0, s_>>>(ns, (int)tot, (const MortonSet *)mset_.p, wmask_.p, wcnt_.p);
+    exclusive_sum(wcnt_.p, wpre_.p, (int)nw + 1, s_);
+    k_walk_scatter<<<nb((long long)tot), BS, 0, s_>>>(ns, (int)tot, (const MortonSet *)mset_.p, wmask_.p,
+                                                      wpre_.p, mv1_.p, sw_.qsid.p);
+    k_walk_counts_mask<<<1, 256, 0, s_>>>(ns, (int)tot, (const MortonSet *)mset_.p, wmask_.p, wpre_.p,
+                                          sw_.nin.p, sw_.wstart.p);
+    return mv1_.p;
+  }
+  mk0_.reserve(tot);
+  mk1_.reserve(tot);
+  mv0_.reserve(tot);
   k_morton_sets<<<nb((long long)tot), BS, 0, s_>>>(ns, (int)tot, (const MortonSet *)mset_.p, mb, mk0_.p, mv0_.p,
                                                    sw_.qsid.p);
   sort_pairs(mk0_.p, mk1_.p, mv0_.p, mv1_.p, (int)tot, mb + 1 + rb, s_);
