@@ -1279,16 +1279,63 @@ static_assert(STACK <= WAVE, "k_collect keeps its stack one entry per lane");
 // for the queries without one, into last ((rank << 2) | outcome of the
 // highest-ranked candidate inside its element box; -1: none), and with a
 // tie-band counter into acc (candidates accepting the point, plus 2^20 per
-// near-zero-weight acceptance).  The list
-// has a capacity: entries appended past it are dropped, and the host sees
-// used > cap, grows the list and lists again before any test runs.
-constexpr int EMPTY_SLOT = -1;  // nv field 15: no cell type has it
+// near-zero-weight acceptance).
+// The list is cut into NREG regions of rcap slots, region r holding slots
+// r * rcap .. (r + 1) * rcap - 1 and appended to through its own counter
+// used[r * USED_STRIDE] (a cache line each): workgroup b of a listing launch
+// appends to region b % NREG, so the appends of all waves do not queue on
+// one atomic counter.  A region's slots past its count are unused; entries
+// appended past rcap are dropped, and the host sees a count above rcap,
+// grows the list and lists again before any test runs.
+constexpr int NREG = 32, USED_STRIDE = 32;
+// workgroups of the launches over the list, multiples of NREG
+constexpr unsigned BIN_GRID = 2048, LAST_GRID = 4096, STATS_GRID = 1024;
+static_assert(BIN_GRID % NREG == 0 && LAST_GRID % NREG == 0 && STATS_GRID % NREG == 0, "grids over the list");
 struct CandOut {
   int *cell, *query, *status;
-  int *used;  // entries appended so far
-  int cap;    // capacity of the flat list
+  int *used;  // per region: entries appended so far, at used[r * USED_STRIDE]
+  int rcap;   // capacity of each region
   int *best, *last, *acc;  // per global position (acc: null without a tie-band counter)
 };
+
+// slots per region of a list of about cap entries (at most 2^31 - 1 slots)
+inline int cand_region_cap(size_t cap)
+{
+  const size_t r = std::max<size_t>((cap + NREG - 1) / NREG, 1);
+  return (int)std::min<size_t>(r, 0x7fffffff / NREG);
+}
+
+// entries held by the regions of a list (counts as the device left them)
+inline long long cand_held(const int *used, int rcap)
+{
+  long long n = 0;
+  for (int r = 0; r < NREG; r++) n += std::min(used[r * USED_STRIDE], rcap);
+  return n;
+}
+
+// every slot of the list, the entries region r holds (its first slot is
+// r * rcap), and whether slot j holds a listed candidate
+__device__ __forceinline__ int cand_slots(const CandOut &O) { return NREG * O.rcap; }
+__device__ __forceinline__ int region_held(const CandOut &O, int r) { return min(O.used[r * USED_STRIDE], O.rcap); }
+
+// A launch over the list (gridDim.x a multiple of NREG) gives region
+// b % NREG to workgroup b and the others of its residue: the workgroup's
+// first slot (its first thread's), the end of the region's entries, and
+// the stride of its loop
+struct RegionLoop {
+  int first, end, stride;
+};
+__device__ __forceinline__ RegionLoop region_loop(const CandOut &O)
+{
+  const int r = (int)blockIdx.x % NREG, lb = (int)blockIdx.x / NREG;
+  const int nb = (int)gridDim.x / NREG, bs = (int)blockDim.x;
+  return RegionLoop{r * O.rcap + lb * bs, r * O.rcap + region_held(O, r), nb * bs};
+}
+__device__ __forceinline__ bool slot_live(const CandOut &O, int j)
+{
+  const int r = j / O.rcap;
+  return j - r * O.rcap < O.used[r * USED_STRIDE];
+}
 
 // ADT::searchADT / searchIntersections (searchADTrecursion.C:26-101).  The
 // reference walks the tree in preorder (a node's own element, then its left
@@ -1367,16 +1414,17 @@ struct WaveBuf {
       if (lane >= o) pre += v;
     }
     off[lane] = pre - mine;
+    const int reg = blockIdx.x & (NREG - 1);
     int base = 0;
-    if (lane == 0) base = atomicAdd(O.used, n);
+    if (lane == 0) base = atomicAdd(O.used + reg * USED_STRIDE, n);
     base = __shfl(base, 0);
     __syncthreads();  // the lanes' pair writes and offsets before the copy reads them
     for (int j = lane; j < n; j += WAVE) {
       const int kj = key[j];
       const int d = base + off[kj & (WAVE - 1)] + (kj >> 6);
-      if (d < O.cap) {
-        O.cell[d] = val[j];
-        O.query[d] = qid[j];
+      if (d < O.rcap) {
+        O.cell[reg * O.rcap + d] = val[j];
+        O.query[reg * O.rcap + d] = qid[j];
       }
     }
     __syncthreads();  // the copy before the buffer is written again
@@ -1592,8 +1640,7 @@ __device__ __forceinline__ int type_bucket(int nv) { return nv == 4 ? 0 : (nv ==
 __device__ __forceinline__ int bin_of(const CandOut &O, int j, int n)
 {
   if (j >= n) return -1;
-  const int c = O.cell[j];
-  return c == EMPTY_SLOT ? -1 : type_bucket((unsigned)c >> 28);
+  return type_bucket((unsigned)O.cell[j] >> 28);
 }
 
 __global__ void __launch_bounds__(256) k_bin_count(CandOut O, int *__restrict__ bcount)
@@ -1601,10 +1648,10 @@ __global__ void __launch_bounds__(256) k_bin_count(CandOut O, int *__restrict__ 
   __shared__ int loc[4];
   if (threadIdx.x < 4) loc[threadIdx.x] = 0;
   __syncthreads();
-  const int n = min(*O.used, O.cap);
+  const RegionLoop L = region_loop(O);
   const int lane = threadIdx.x & 63;
-  for (int j0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); j0 < n; j0 += gridDim.x * blockDim.x) {
-    const int b = bin_of(O, j0 + lane, n);
+  for (int j0 = L.first + (threadIdx.x & ~63); j0 < L.end; j0 += L.stride) {
+    const int b = bin_of(O, j0 + lane, L.end);
     for (int q = 0; q < 4; q++) {
       const unsigned long long m = __ballot(b == q);
       if (lane == 0 && m) atomicAdd(&loc[q], __popcll(m));
@@ -1620,14 +1667,13 @@ __global__ void __launch_bounds__(256) k_bin_fill(CandOut O, int *__restrict__ b
   __shared__ int loc[4], base[4];
   if (threadIdx.x < 4) loc[threadIdx.x] = 0;
   __syncthreads();
-  const int n = min(*O.used, O.cap);
-  const int stride = gridDim.x * blockDim.x;
+  const RegionLoop L = region_loop(O);
   const int lane = threadIdx.x & 63;
   const unsigned long long lt = (1ull << lane) - 1;
-  const int start0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  const int start0 = L.first + (threadIdx.x & ~63);
   // pass 1: this block's counts; pass 2: its slice of each bucket
-  for (int j0 = start0; j0 < n; j0 += stride) {
-    const int b = bin_of(O, j0 + lane, n);
+  for (int j0 = start0; j0 < L.end; j0 += L.stride) {
+    const int b = bin_of(O, j0 + lane, L.end);
     for (int q = 0; q < 4; q++) {
       const unsigned long long m = __ballot(b == q);
       if (lane == 0 && m) atomicAdd(&loc[q], __popcll(m));
@@ -1641,8 +1687,8 @@ __global__ void __launch_bounds__(256) k_bin_fill(CandOut O, int *__restrict__ b
     loc[threadIdx.x] = 0;
   }
   __syncthreads();
-  for (int j0 = start0; j0 < n; j0 += stride) {
-    const int b = bin_of(O, j0 + lane, n);
+  for (int j0 = start0; j0 < L.end; j0 += L.stride) {
+    const int b = bin_of(O, j0 + lane, L.end);
     for (int q = 0; q < 4; q++) {
       const unsigned long long m = __ballot(b == q);
       if (!m) continue;
@@ -1688,8 +1734,8 @@ __device__ __forceinline__ void record_outcome(const CandOut &O, int j, int st)
 // k_select reads it for)
 __global__ void __launch_bounds__(256) k_last(CandOut O)
 {
-  const int n = min(*O.used, O.cap);
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+  const RegionLoop L = region_loop(O);
+  for (int j = L.first + threadIdx.x; j < L.end; j += L.stride) {
     const int st = O.status[j];
     if (st == OUTSIDE) continue;
     const int g = O.query[j];
@@ -1786,9 +1832,8 @@ __global__ void __launch_bounds__(256) k_status_hist(CandOut O, int *__restrict_
   __shared__ int loc[4];
   if (threadIdx.x < 4) loc[threadIdx.x] = 0;
   __syncthreads();
-  const int n = min(*O.used, O.cap);
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    if (O.cell[j] == EMPTY_SLOT) continue;
+  const RegionLoop L = region_loop(O);
+  for (int j = L.first + threadIdx.x; j < L.end; j += L.stride) {
     const int st = O.status[j];
     if (st >= 0 && st < 4) atomicAdd(&loc[st], 1);
   }
@@ -1800,10 +1845,9 @@ __global__ void __launch_bounds__(256) k_status_hist(CandOut O, int *__restrict_
 // reference's walk stops at that hit and never tests them); a diagnostic
 __global__ void k_after_hit(CandOut O, unsigned long long *__restrict__ out)
 {
-  const int n = min(*O.used, O.cap);
+  const RegionLoop L = region_loop(O);
   unsigned long long v = 0;
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
-    if (O.cell[j] != EMPTY_SLOT) v += (O.cell[j] & 0x0fffffff) > O.best[O.query[j]];
+  for (int j = L.first + threadIdx.x; j < L.end; j += L.stride) v += (O.cell[j] & 0x0fffffff) > O.best[O.query[j]];
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
 }
@@ -2651,7 +2695,7 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     TG_HIP(hipMemcpyAsync(w.sv.p, S, sizeof(SearchView) * ns, hipMemcpyHostToDevice, s));
     TG_HIP(hipMemcpyAsync(w.qb.p, qb.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
   }
-  w.counters.reserve(1);
+  w.counters.reserve(NREG * USED_STRIDE);
   // per query: lowest clean-hit rank, highest tested (rank, outcome), and the
   // tie-band count when a search has a counter
   bool tieband = false;
@@ -2686,30 +2730,33 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
   }
   if (w.cap == 0) w.cap = std::max<size_t>(4 * (size_t)nqt, 1 << 20);
   for (;;) {
-    w.cell.reserve(w.cap);
-    w.query.reserve(w.cap);
-    w.status.reserve(w.cap);
-    CandOut O{w.cell.p, w.query.p, w.status.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
-              w.best.p, w.last.p, tieband ? w.acc.p : nullptr};
-    fill<int>(w.counters.p, 1, 0, s);
+    const int rcap = cand_region_cap(w.cap);
+    const size_t slots = (size_t)NREG * rcap;
+    w.cell.reserve(slots);
+    w.query.reserve(slots);
+    w.status.reserve(slots);
+    CandOut O{w.cell.p, w.query.p, w.status.p, w.counters.p, rcap, w.best.p, w.last.p, tieband ? w.acc.p : nullptr};
+    fill<int>(w.counters.p, NREG * USED_STRIDE, 0, s);
     if (bins) k_find<<<FIND_GRID, WAVE, 0, s>>>(T, O, (int)nwave, w.tune.seg, w.segstart.p);
     if (walk) k_collect<<<(unsigned)nwave, WAVE, 0, s>>>(T, O, bins ? 1 : 0);
     if (check) {
-      int used = 0;
-      TG_HIP(hipMemcpyAsync(&used, w.counters.p, sizeof used, hipMemcpyDeviceToHost, s));
+      std::vector<int> used(NREG * USED_STRIDE);
+      TG_HIP(hipMemcpyAsync(used.data(), w.counters.p, sizeof(int) * used.size(), hipMemcpyDeviceToHost, s));
       TG_HIP(hipStreamSynchronize(s));
-      if ((size_t)used > w.cap) {  // grow the flat list and list again (nothing was tested yet)
-        w.cap = (size_t)used + used / 4;
+      int most = 0;
+      for (int r = 0; r < NREG; r++) most = std::max(most, used[r * USED_STRIDE]);
+      if (most > rcap) {  // grow every region and list again (nothing was tested yet)
+        w.cap = (size_t)NREG * ((size_t)most + most / 4);
         continue;
       }
     }
     if (list_only) return;
     w.bins.reserve(8);
-    w.blist.reserve(w.cap);
+    w.blist.reserve(slots);
     fill<int>(w.bins.p, 8, 0, s);
-    k_bin_count<<<2048, 256, 0, s>>>(O, w.bins.p);
-    k_bin_fill<<<2048, 256, 0, s>>>(O, w.bins.p, w.blist.p);
-    const int lcap = (int)std::min<size_t>(w.cap / 4 + 1024, 0x7fffffff / 4);
+    k_bin_count<<<BIN_GRID, 256, 0, s>>>(O, w.bins.p);
+    k_bin_fill<<<BIN_GRID, 256, 0, s>>>(O, w.bins.p, w.blist.p);
+    const int lcap = (int)std::min<size_t>(slots / 4 + 1024, 0x7fffffff / 4);
     w.lj.reserve(4 * (size_t)lcap);
     w.lst.reserve(4 * (size_t)lcap);
     w.lcount.reserve(4);
@@ -2736,24 +2783,28 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     k_eval_long<8><<<1024, 256, 0, s>>>(T, O, L);
     k_eval<4, 8><<<eg, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
     TG_HIP(hipStreamWaitEvent(s, w.join, 0));
-    k_last<<<4096, 256, 0, s>>>(O);
+    k_last<<<LAST_GRID, 256, 0, s>>>(O);
     k_select<<<(unsigned)std::min<long long>((nwave + 3) / 4, 8192), 256, 0, s>>>(T, O);
     return;
   }
 }
 
 // the listed candidates of a single search: search point index, preorder
-// rank and cell of each (the element-box test is left to the caller)
-__global__ void k_export_cands(int n, SearchSet T, CandOut O, int *__restrict__ qi, int *__restrict__ rank,
+// rank and cell of each (the element-box test is left to the caller), the
+// regions' entries one after the other
+__global__ void k_export_cands(SearchSet T, CandOut O, int *__restrict__ qi, int *__restrict__ rank,
                                int *__restrict__ cell)
 {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  if (j >= cand_slots(O) || !slot_live(O, j)) return;
+  const int r = j / O.rcap;
+  int k = j - r * O.rcap;
+  for (int q = 0; q < r; q++) k += min(O.used[q * USED_STRIDE], O.rcap);
   const SearchView S = view_of(T, 0);
   const int v = O.cell[j];
-  qi[j] = S.perm[O.query[j]];
-  rank[j] = v & 0x0fffffff;
-  cell[j] = cell_of(S, v);
+  qi[k] = S.perm[O.query[j]];
+  rank[k] = v & 0x0fffffff;
+  cell[k] = cell_of(S, v);
 }
 
 // a search's bin grid, when one was built
@@ -4370,28 +4421,29 @@ void Engine::search_stats(long long out[10])
   out[3] = w.last_nqt;
   out[9] = w.last_bins ? 1 : 0;
   if (w.last_nqt <= 0 || w.last_ns <= 0) return;
-  int used = 0, items = 0, h[4] = {0, 0, 0, 0};
+  int items = 0, h[4] = {0, 0, 0, 0};
+  std::vector<int> used(NREG * USED_STRIDE);
   std::vector<int> nin(w.last_ns);
   DBuf<int> hist;
   hist.reserve(4);
   fill<int>(hist.p, 4, 0, s_);
-  CandOut O{w.cell.p, w.query.p, w.status.p, w.counters.p, (int)std::min<size_t>(w.cap, 0x7fffffff),
-            w.best.p, w.last.p, nullptr};
-  k_status_hist<<<1024, 256, 0, s_>>>(O, hist.p);
+  const int rcap = cand_region_cap(w.cap);
+  CandOut O{w.cell.p, w.query.p, w.status.p, w.counters.p, rcap, w.best.p, w.last.p, nullptr};
+  k_status_hist<<<STATS_GRID, 256, 0, s_>>>(O, hist.p);
   DBuf<unsigned long long> ah;
   ah.reserve(1);
   fill<unsigned long long>(ah.p, 1, 0ull, s_);
-  k_after_hit<<<1024, 256, 0, s_>>>(O, ah.p);
+  k_after_hit<<<STATS_GRID, 256, 0, s_>>>(O, ah.p);
   unsigned long long after = 0;
   TG_HIP(hipMemcpyAsync(&after, ah.p, sizeof after, hipMemcpyDeviceToHost, s_));
-  TG_HIP(hipMemcpyAsync(&used, w.counters.p, sizeof used, hipMemcpyDeviceToHost, s_));
+  TG_HIP(hipMemcpyAsync(used.data(), w.counters.p, sizeof(int) * used.size(), hipMemcpyDeviceToHost, s_));
   TG_HIP(hipMemcpyAsync(h, hist.p, sizeof h, hipMemcpyDeviceToHost, s_));
   TG_HIP(hipMemcpyAsync(nin.data(), w.nin.p, sizeof(int) * w.last_ns, hipMemcpyDeviceToHost, s_));
   if (w.last_bins) {
     TG_HIP(hipMemcpyAsync(&items, w.segstart.p + w.last_nwave, sizeof items, hipMemcpyDeviceToHost, s_));
   }
   TG_HIP(hipStreamSynchronize(s_));
-  out[0] = used;
+  out[0] = cand_held(used.data(), rcap);
   long long walking = 0;
   for (int v : nin) walking += v;
   out[1] = w.last_bins ? items : (walking + WAVE - 1) / WAVE;
@@ -4530,18 +4582,19 @@ void Engine::list_candidates(Block &b, std::vector<int> &qi, std::vector<int> &r
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
   run_search(&S, 1, sw_, s_, true, true);
-  int used = 0;
-  TG_HIP(hipMemcpyAsync(&used, sw_.counters.p, sizeof used, hipMemcpyDeviceToHost, s_));
+  const int rcap = cand_region_cap(sw_.cap);
+  std::vector<int> counts(NREG * USED_STRIDE);
+  TG_HIP(hipMemcpyAsync(counts.data(), sw_.counters.p, sizeof(int) * counts.size(), hipMemcpyDeviceToHost, s_));
   sync();
+  const int used = (int)cand_held(counts.data(), rcap);
   if (used <= 0) return;
   DBuf<int> dq, dr, dc;
   dq.reserve(used);
   dr.reserve(used);
   dc.reserve(used);
   const SearchSet T{(const SearchView *)sw_.sv.p, sw_.qb.p, sw_.nin.p, sw_.wstart.p, sw_.qsid.p, 1, b.nsearch};
-  CandOut O{sw_.cell.p, sw_.query.p, sw_.status.p, sw_.counters.p, (int)std::min<size_t>(sw_.cap, 0x7fffffff),
-            sw_.best.p, sw_.last.p, nullptr};
-  k_export_cands<<<nb(used), BS, 0, s_>>>(used, T, O, dq.p, dr.p, dc.p);
+  CandOut O{sw_.cell.p, sw_.query.p, sw_.status.p, sw_.counters.p, rcap, sw_.best.p, sw_.last.p, nullptr};
+  k_export_cands<<<nb((long long)NREG * rcap), BS, 0, s_>>>(T, O, dq.p, dr.p, dc.p);
   qi.resize(used);
   rank.resize(used);
   cell.resize(used);
