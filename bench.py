@@ -1,8 +1,8 @@
 """Donor-search throughput of the MI355X overset-connectivity engine.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config2|config1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hex256_mixed_pair|hex48_pair]
 
-Workload (default ``config2``, the large plain-method configuration): a 256^3
+Workload (default ``hex256_mixed_pair``, the large plain-method configuration): a 256^3
 hex background on [-1,1]^3 with interior nodes jittered by +-10 % of the
 spacing (16.97 M nodes) and a 3.87 M-cell mixed-element shell (40 % tet, 30 %
 prism, 5 % pyramid, 25 % hex) from an ellipsoid wall (semi-axes 0.3, 0.2,
@@ -51,12 +51,12 @@ def log(msg: str) -> None:
 def workload(name: str):
     from tioga_amd import workloads as W
 
-    if name == "config2":
-        return W.config2(), ("config2: 256^3 jittered hex background + 3.87M-cell mixed tet/pyramid/prism/hex "
+    if name == "hex256_mixed_pair":
+        return W.hex256_mixed_pair(), ("hex256_mixed_pair: 256^3 jittered hex background + 3.87M-cell mixed tet/pyramid/prism/hex "
                              "shell (ellipsoid wall 0.3,0.2,0.2 to r=0.6), FP64; all background nodes vs the "
                              "near-body block and all near-body nodes vs the background block")
-    if name == "config1":
-        return W.config1(), ("config1: 48^3 hex background + 64x32x24 lat-long hex shell (r 0.25..0.6, "
+    if name == "hex48_pair":
+        return W.hex48_pair(), ("hex48_pair: 48^3 hex background + 64x32x24 lat-long hex shell (r 0.25..0.6, "
                              "stretch 1.1), FP64; all background nodes vs the near-body block and all "
                              "near-body nodes vs the background block")
     raise SystemExit(f"unknown workload {name}")
@@ -114,7 +114,7 @@ def amr_gather_leg(shell, nvar: int = 8, updates: int = 1000):
 
     t0 = time.time()
     body = W.all_receptors(shell)
-    grid = W.config5_grid()
+    grid = W.amr3_grid()
     pq = grid.patch_fields(nvar, seed=5)
     log(f"AMR workload built in {time.time() - t0:.1f} s: {grid.ngrids} patches, "
         f"{sum(q.size for q in pq) * 8 / 2**30:.2f} GiB of patch fields")
@@ -144,7 +144,7 @@ def amr_gather_leg(shell, nvar: int = 8, updates: int = 1000):
             "ms_per_update": round(ms, 5), "algorithmic_bytes_per_update": algo,
             "GBps": round(gbps, 2) if gbps else None,
             "frac": round(gbps / HBM_PEAK_GBPS, 4) if gbps else None,
-            "traffic": profiled_traffic("amr_gather_config5.md", "traffic_bytes_per_update")}
+            "traffic": profiled_traffic("amr_gather_amr3_mixed_shell.md", "traffic_bytes_per_update")}
 
 
 def host_cpu():
@@ -213,15 +213,15 @@ def batching_leg(steps: int, warmup: int, batched_only: bool = False) -> dict:
     from tioga_amd.tioga import Tioga
 
     t0 = time.time()
-    blocks = W.config3()
-    log(f"config3 generated in {time.time() - t0:.1f} s: {len(blocks)} blocks, "
+    blocks = W.shells64_hex192()
+    log(f"shells64_hex192 generated in {time.time() - t0:.1f} s: {len(blocks)} blocks, "
         f"{sum(b.ncells for b in blocks)} cells, {sum(b.nnodes for b in blocks)} nodes")
     tg = Tioga()
     for b in blocks:
         tg.register_grid(b)
     t0 = time.time()
     tg.preprocess()
-    log(f"config3 preprocess {time.time() - t0:.2f} s")
+    log(f"shells64_hex192 preprocess {time.time() - t0:.2f} s")
 
     def run(batched: bool):
         tg.set_batching(batched)
@@ -234,7 +234,7 @@ def batching_leg(steps: int, warmup: int, batched_only: bool = False) -> dict:
             tg.connect()
             times.append(time.perf_counter() - t1)
             phases.append(tg.phase_times())
-        log(f"config3 connectivity ({'batched' if batched else 'block by block'}): "
+        log(f"shells64_hex192 connectivity ({'batched' if batched else 'block by block'}): "
             f"{', '.join(f'{1000 * t:.1f}' for t in times)} ms")
         ib = [a.copy() for a in tg.iblank]
         rec = [tg.donor_info(b.tag) for b in blocks]
@@ -248,7 +248,7 @@ def batching_leg(steps: int, warmup: int, batched_only: bool = False) -> dict:
     tg.close()
     nsearch = ph_b[-1]["nsearch"]
     ms = 1000 * float(np.median(t_b))
-    out = {"workload": "config3: 64 hex/prism shells (1e4..1e6 cells, log-uniform) + 192^3 background, "
+    out = {"workload": "shells64_hex192: 64 hex/prism shells (1e4..1e6 cells, log-uniform) + 192^3 background, "
                        "performconnectivity with all inputs resident",
            "blocks": len(blocks), "searches_per_step": int(nsearch), "steps": steps,
            "batched": {"ms_per_step": round(ms, 3), "p99_ms": round(1000 * float(np.percentile(t_b, 99)), 3),
@@ -292,15 +292,15 @@ def serving_leg(slots: int, budget_gb: float, rates, seconds: float, max_request
     from tioga_amd.tioga import Tioga
 
     t0 = time.time()
-    shells = W.config4_shells()
+    shells = W.serve16_shells()
     bg = W.cart_box(256, tag=17, jitter=0.1, seed=2)
-    log(f"config4 generated in {time.time() - t0:.1f} s: shell nodes {sorted(s.nnodes for s in shells)}")
+    log(f"serve16_hex256 generated in {time.time() - t0:.1f} s: shell nodes {sorted(s.nnodes for s in shells)}")
     tg = Tioga()
     for b in shells + [bg]:
         tg.register_grid(b)
     t0 = time.time()
     tg.preprocess()
-    log(f"config4 preprocess {time.time() - t0:.2f} s")
+    log(f"serve16_hex256 preprocess {time.time() - t0:.2f} s")
     cen = {s.tag: s.x.mean(axis=0) for s in shells}
     size = {s.tag: s.nnodes for s in shells}
     tg.serve_open(bg.tag, nslots=slots, budget_gb=budget_gb)
@@ -397,7 +397,7 @@ def serving_leg(slots: int, budget_gb: float, rates, seconds: float, max_request
     log(f"plain form: {plain_ms:.1f} ms per request, {plain_sps:.3e} searches/s")
     stable = [r for r in per_rate if r["throughput_req_s"] >= 0.95 * r["rate_req_s"]] or per_rate[:1]
     head = max(stable, key=lambda r: r["rate_req_s"])
-    return {"workload": "config4: 16 hex/prism shells (1e4..2e6 cells) moving rigidly over a resident 256^3 "
+    return {"workload": "serve16_hex256: 16 hex/prism shells (1e4..2e6 cells) moving rigidly over a resident 256^3 "
                         "jittered hex background; Poisson arrivals", "slots": slots, "budget_gb": budget_gb,
             "alone_p50_ms": round(float(np.median(alone)), 3), "alone_max_ms": round(float(max(alone)), 3),
             "alone_searches_per_s": round(alone_s / (sum(alone) / 1000), 1),
@@ -453,7 +453,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="config2")
+    ap.add_argument("--workload", default="hex256_mixed_pair")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU search for the baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
@@ -462,14 +462,14 @@ def main() -> int:
     ap.add_argument("--search-only", action="store_true",
                     help="only the headline query-set passes (profiling runs): no performconnectivity, dataUpdate, "
                          "CPU baseline, reduce_fringes or AMR leg")
-    ap.add_argument("--batched-only", action="store_true", help="config3: skip the block-by-block form")
+    ap.add_argument("--batched-only", action="store_true", help="shells64_hex192: skip the block-by-block form")
     ap.add_argument("--no-batching", action="store_true", help="skip the batching leg (64 shells, both forms)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg (moving shells)")
-    ap.add_argument("--slots", type=int, default=8, help="serving slots (config4)")
-    ap.add_argument("--budget-gb", type=float, default=64.0, help="serving HBM budget (config4)")
-    ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="request rates per second (config4)")
-    ap.add_argument("--seconds", type=float, default=2.0, help="arrival window per rate (config4)")
-    ap.add_argument("--max-requests", type=int, default=300, help="requests per rate at most (config4)")
+    ap.add_argument("--slots", type=int, default=8, help="serving slots (serve16_hex256)")
+    ap.add_argument("--budget-gb", type=float, default=64.0, help="serving HBM budget (serve16_hex256)")
+    ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="request rates per second (serve16_hex256)")
+    ap.add_argument("--seconds", type=float, default=2.0, help="arrival window per rate (serve16_hex256)")
+    ap.add_argument("--max-requests", type=int, default=300, help="requests per rate at most (serve16_hex256)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -493,9 +493,9 @@ def main() -> int:
         return 0
     from tioga_amd.tioga import Tioga
 
-    if args.workload == "config3":
+    if args.workload == "shells64_hex192":
         return batching_bench(args)
-    if args.workload == "config4":
+    if args.workload == "serve16_hex256":
         return serving_bench(args)
     t0 = time.time()
     blocks, desc = workload(args.workload)
@@ -583,9 +583,9 @@ def main() -> int:
     achieved = algo / (kern_ms / 1000.0) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 5),
-            "traffic": profiled_traffic("search_pipeline_config2.md", "traffic_bytes_per_launch")
-            if args.workload == "config2" else None,
-            "traffic_source": "profiles/search_pipeline_config2.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
+            "traffic": profiled_traffic("search_pipeline_hex256_mixed_pair.md", "traffic_bytes_per_launch")
+            if args.workload == "hex256_mixed_pair" else None,
+            "traffic_source": "profiles/search_pipeline_hex256_mixed_pair.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
             "kernel": "search launch (k_find_cost, k_find / k_collect, k_bin_count, k_bin_fill, k_eval<4,5,6,8>, "
                       "k_eval_long<5,6,8>, k_last, k_select)",
             "algorithmic_bytes_per_step": algo, "launches_per_step": 1,
@@ -655,8 +655,8 @@ def main() -> int:
                        "GBps": round(bdu / (du_ms / 1000) / 1e9, 2) if du_ms > 0 else None,
                        "frac": round(bdu / (du_ms / 1000) / 1e9 / HBM_PEAK_GBPS, 4) if du_ms > 0 else None,
                        "kernel": "k_update<0> (fused gather and scatter)",
-                       "traffic": profiled_traffic("dataupdate_config2.md", "traffic_bytes_per_update")
-                       if args.workload == "config2" else None},
+                       "traffic": profiled_traffic("dataupdate_hex256_mixed_pair.md", "traffic_bytes_per_update")
+                       if args.workload == "hex256_mixed_pair" else None},
         "reduce_fringes": rf,
         "amr_gather": amr,
         "batching": batching,

@@ -3,7 +3,7 @@
 Builds the timers variant of the reference (oracle/Makefile target `timers`:
 tioga.C with -DTIOGA_ENABLE_TIMERS, whose myTimer prints "End <phase>, time
 taken=<s>" around each phase, tioga.C:97-151, 715-735), writes a case file
-for a golden case (default: config1, the 48^3 hex background and the
+for a golden case (default: hex48_pair, the 48^3 hex background and the
 64x32x24 lat-long shell) and runs oracle/_ref/timers/ref_driver on it on one
 MPI rank, several times, printing per-phase times (the median over the runs)
 and the driver's own wall times of profile, performConnectivity and
@@ -13,7 +13,7 @@ wall time is the whole call.  Needs the reference sources and this image's
 MPICH (as `make -C oracle ref`); it is a cross-check on this machine's CPU,
 never part of a GPU run.
 
-    python oracle/time_reference.py [--case config1] [--runs 5]
+    python oracle/time_reference.py [--case hex48_pair] [--runs 5]
 """
 import argparse
 import os
@@ -29,7 +29,7 @@ ROOT = os.path.dirname(HERE)
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--case", default="config1")
+    ap.add_argument("--case", default="hex48_pair")
     ap.add_argument("--runs", type=int, default=5)
     args = ap.parse_args()
     r = subprocess.run(["make", "-C", HERE, "timers"], capture_output=True, text=True)

@@ -169,8 +169,8 @@ def build(name: str) -> Case:
         blocks = W.tiny()
         blocks[1].tag = 200
         return Case(name, blocks, nvar=2)
-    if name == "config1":
-        return Case(name, W.config1(), nvar=5)
+    if name == "hex48_pair":
+        return Case(name, W.hex48_pair(), nvar=5)
     # reduce_fringes (tioga.C:737-885) after the connectivity: fringe depths
     # nfringe + 1 = 2, 3 and 4 sweeps, mixed cell types, three blocks, and a
     # moving body reduced at every step
@@ -236,7 +236,7 @@ def build(name: str) -> Case:
 SMALL_CASES = ["tiny", "tiny_isym0", "tiny_isym1", "tiny_isym2", "mixed", "userres", "moving", "three", "bigtag"]
 REDUCE_CASES = ["tiny_rf", "isym0_rf", "mixed_rf", "three_rf", "moving_rf", "nested_rf", "nested3_rf"]
 GID_CASES = ["tiny_gid", "seam_gid", "seam_gid_rf"]
-ALL_CASES = SMALL_CASES + ["config1"] + REDUCE_CASES + GID_CASES
+ALL_CASES = SMALL_CASES + ["hex48_pair"] + REDUCE_CASES + GID_CASES
 # the point path: pinned by the reference directly (the oracle restates the
 # node path only)
 POINT_CASES = ["points", "points_three", "points_cb", "points_cb_jitter"]

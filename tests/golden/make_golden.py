@@ -7,7 +7,7 @@ For each case of cases.py (the AMR gather cases run the driver's AMR mode),
 ``<case>.npz`` holds every array that the driver
 dumps whose size is at most ``FULL_LIMIT`` elements, and ``<case>.json``
 holds the SHA-256 digest, dtype and size of every array (the hole maps and
-config-1 arrays are pinned by digest only).  Run in a container where
+hex48_pair arrays are pinned by digest only).  Run in a container where
 /root/reference exists:
 
     make -C oracle ref && python tests/golden/make_golden.py [case ...]
@@ -99,7 +99,7 @@ def main(names):
             if k.endswith("time_connectivity"):
                 continue
             meta[k] = {"dtype": str(a.dtype), "size": int(a.size), "sha256": digest(a)}
-            if a.size <= FULL_LIMIT and name != "config1":
+            if a.size <= FULL_LIMIT and name != "hex48_pair":
                 full[k] = a
         with open(os.path.join(HERE, f"{name}.json"), "w") as f:
             json.dump(meta, f, indent=0, sort_keys=True)

@@ -132,7 +132,7 @@ def main() -> int:
     from tioga_amd import workloads as W
     from tioga_amd.tioga import Tioga
 
-    blocks = W.config2()
+    blocks = W.hex256_mixed_pair()
     tg = Tioga()
     for b in blocks:
         tg.register_grid(b)
@@ -141,8 +141,8 @@ def main() -> int:
         tg.query_set(btag, xq)
     best, trials = tune(tg, log=lambda m: print(m, flush=True))
     tg.close()
-    save("config2", best, trials, path=args.out)
-    print(json.dumps({"config2": best}), flush=True)
+    save("hex256_mixed_pair", best, trials, path=args.out)
+    print(json.dumps({"hex256_mixed_pair": best}), flush=True)
     return 0
 
 

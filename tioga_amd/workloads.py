@@ -6,38 +6,38 @@ type (codetypes.h:36 ``BASE 1``), 1-based wall / overset-boundary node lists.
 
 Workloads (sizes as stated; all coordinates FP64):
 
-* ``config1``  background 48^3 hex cube [-1,1]^3 (tag 2) + lat-long spherical
+* ``hex48_pair``  background 48^3 hex cube [-1,1]^3 (tag 2) + lat-long spherical
   hex shell, r 0.25..0.6, 64 azimuth x 32 polar x 24 radial cells, radial
   stretching 1.1 (tag 1).  Pole nodes are kept per azimuth (coincident
   copies), which exercises the coincident-query dedupe.
-* ``config2``  background 256^3 hex cube with interior nodes jittered by
+* ``hex256_mixed_pair``  background 256^3 hex cube with interior nodes jittered by
   uniform +-10% of the spacing (tag 2) + a mixed-element shell (about 3.9M
   cells: 40% tet, 30% prism, 5% pyramid, 25% hex by cell count) between an
   ellipsoid wall with semi-axes (0.3, 0.2, 0.2) and a sphere of radius 0.6
   (tag 1).  Poles are merged, so pole rows are prisms.
-* ``tiny``     the config-1 construction at a size the oracle finishes in a
+* ``tiny``     the hex48_pair construction at a size the oracle finishes in a
   fraction of a second (parity cases).
-* ``config3``  batching: 64 hex/prism spherical shells (tags 1..64) whose
+* ``shells64_hex192``  batching: 64 hex/prism spherical shells (tags 1..64) whose
   cell counts are log-uniform in [1e4, 1e6], wall radius uniform in
   [0.02, 0.08], outer radius 2.5 x the wall radius, centres uniform in
   [-0.8, 0.8]^3, uniformly random rotations (Shoemake's quaternion method),
   seed 3; background 192^3 hex cube [-1,1]^3 (tag 65).  Each shell is the
   mixed_shell construction with a spherical wall and only hex and prism
   layers (its polar rows are prisms).
-* ``config4``  serving: the 256^3 jittered background of config2 (tag 17)
+* ``serve16_hex256``  serving: the 256^3 jittered background of hex256_mixed_pair (tag 17)
   and 16 hex/prism spherical shells (tags 1..16) of log-uniform size in
   [1e4, 2e6] cells, wall radius uniform in [0.02, 0.08], centres uniform in
   [-0.6, 0.6]^3, seed 4; requests move one shell rigidly (rotation by an
   angle uniform in [0, 5] degrees about a uniformly random axis, about the
   shell's centroid, then a shift of uniformly random direction and length
   uniform in [0, 0.02]) and arrive as a Poisson stream.
-* ``config5``  AMR Cartesian background: 3 levels (refinement ratio 2) of
+* ``amr3_mixed_shell``  AMR Cartesian background: 3 levels (refinement ratio 2) of
   4096 patches of 16^3 cells with one ghost layer on [-1,1]^3 (level 0: the
   whole cube at h = 1/64, 512 patches; level 1: [-1,1]x[-0.75,0.75]x
   [-0.5,0.5] at h/2, 1536 patches; level 2: [-0.5,0.5]^2x[-0.25,0.25] at
   h/4, 2048 patches), pdegree 0, 8 FP64 fields per patch.  Receptors are the
-  nodes of the mixed-element near-body shell of config2 (the construction
-  named for this workload; config1's shell is all-hex), every one of them
+  nodes of the mixed-element near-body shell of hex256_mixed_pair (the construction
+  named for this workload; hex48_pair's shell is all-hex), every one of them
   marked unresolved by a node resolution of BIGVALUE, so every node takes its
   value from the patches.
 """
@@ -309,7 +309,7 @@ def fields(nnodes: int, nvar: int, seed: int) -> np.ndarray:
     return rng.uniform(-1.0, 1.0, size=nnodes * nvar)
 
 
-def config1() -> List[Block]:
+def hex48_pair() -> List[Block]:
     return [latlong_shell(64, 32, 24, 0.25, 0.6, 1.1, tag=1), cart_box(48, tag=2)]
 
 
@@ -317,7 +317,7 @@ def tiny(nbox: int = 10, nphi: int = 16, ntheta: int = 8, nr: int = 6) -> List[B
     return [latlong_shell(nphi, ntheta, nr, 0.25, 0.6, 1.1, tag=1), cart_box(nbox, tag=2)]
 
 
-def config2(nbox: int = 256, nphi: int = 360, ntheta: int = 180) -> List[Block]:
+def hex256_mixed_pair(nbox: int = 256, nphi: int = 360, ntheta: int = 180) -> List[Block]:
     return [mixed_shell(nphi, ntheta, tag=1), cart_box(nbox, tag=2, jitter=0.1, seed=2)]
 
 
@@ -410,7 +410,7 @@ def all_receptors(b: Block) -> Block:
     return b
 
 
-def config5_grid() -> AmrGrid:
+def amr3_grid() -> AmrGrid:
     """The patch hierarchy of the AMR gather workload."""
     return amr_hierarchy((-1.0, -1.0, -1.0), 2.0 / 128,
                          [((0, 0, 0), (128, 128, 128)),
@@ -418,9 +418,9 @@ def config5_grid() -> AmrGrid:
                           ((128, 128, 192), (384, 384, 320))], patch=16, nf=1)
 
 
-def config5():
+def amr3_mixed_shell():
     """(near-body blocks, AmrGrid) of the AMR gather workload."""
-    return [all_receptors(mixed_shell(360, 180, tag=1))], config5_grid()
+    return [all_receptors(mixed_shell(360, 180, tag=1))], amr3_grid()
 
 
 def shoemake_rotation(u) -> np.ndarray:
@@ -449,7 +449,7 @@ def sphere_shell(ncells: float, r_wall: float, tag: int, center, rot) -> Block:
     return b
 
 
-def config3(nshell: int = 64, nbox: int = 192, seed: int = 3, lo: float = 1e4, hi: float = 1e6) -> List[Block]:
+def shells64_hex192(nshell: int = 64, nbox: int = 192, seed: int = 3, lo: float = 1e4, hi: float = 1e6) -> List[Block]:
     rng = np.random.default_rng(seed)
     blocks = []
     for t in range(1, nshell + 1):
@@ -484,7 +484,7 @@ def axis_angle(axis, angle_deg) -> np.ndarray:
     return np.eye(3) + np.sin(th) * k + (1 - np.cos(th)) * k @ k
 
 
-def config4_shells(nshell: int = 16, seed: int = 4, lo: float = 1e4, hi: float = 2e6) -> List[Block]:
+def serve16_shells(nshell: int = 16, seed: int = 4, lo: float = 1e4, hi: float = 2e6) -> List[Block]:
     rng = np.random.default_rng(seed)
     out = []
     for t in range(1, nshell + 1):
