@@ -1,14 +1,19 @@
 """The donor search's candidate list and selection, restated on the host (no
 GPU).
 
-Listing (k_collect / k_find): each wave gathers its lanes' (rank | nv << 28,
-query) pairs in an LDS buffer of WBUF pairs, one step adding at most one pair
-per lane, and appends the buffer to the flat list with one atomic add of its
-length to `used` whenever more than WBUF - 64 pairs are held, and at the
-end; the copy sorts the pairs by lane, keeping each lane's step order.
-Pairs that land at or past the capacity are dropped; the host then sees used > cap, grows the list and
-lists again before any test runs.  The binning and evaluation kernels read
-every slot below min(used, cap).
+Listing (k_collect / k_find): each wave (one per workgroup) gathers its
+lanes' (rank | nv << 28, query) pairs in an LDS buffer of WBUF pairs, one
+step adding at most one pair per lane, and appends the buffer to region
+b % NREG of the list (b: its workgroup) with one atomic add of its length to
+that region's counter whenever more than WBUF - 64 pairs are held, and at
+the end; the copy sorts the pairs by lane, keeping each lane's step order.
+Region r holds slots r * rcap .. (r + 1) * rcap - 1; pairs that land at or
+past rcap are dropped, and the host then sees a count above rcap, grows
+every region and lists again before any test runs.  The binning and
+evaluation kernels read region r's slots below min(used_r, rcap).  (The
+model runs each wave as one workgroup, as k_collect does; k_find's work
+items of one wave may run in different workgroups, so there a query's pairs
+may span regions, which the order-free selection below does not mind.)
 
 Selection (k_eval's per-query words, k_select): the reference walks the ADT
 in preorder, tests every candidate (an element whose padded box holds the
@@ -35,15 +40,15 @@ WBUF = 512
 OUTSIDE = 3
 
 
-def replay(lists, cap, rng):
-    """lists[q]: the candidate values of query q (queries in wave order).
-    Returns the slots (None = never written), write counts and used."""
+def replay(lists, nreg, rcap, rng):
+    """lists[q]: the candidate values of query q (queries in wave order; wave
+    w runs as workgroup w).  Returns the slots (None = never written), write
+    counts and each region's count."""
     nq = len(lists)
     nwave = (nq + WAVE - 1) // WAVE
-    slots = [None] * cap
-    flush_of = [None] * cap
-    writes = np.zeros(cap, np.int64)
-    used = 0
+    slots = [None] * (nreg * rcap)
+    writes = np.zeros(nreg * rcap, np.int64)
+    used = [0] * nreg
     # each wave: its steps, then its flushes; waves interleave at flushes
     pending = []
     for w in range(nwave):
@@ -60,48 +65,59 @@ def replay(lists, cap, rng):
         if buf:
             flushes.append(buf)
         pending.append(flushes)
-    # the waves' flushes reach the atomic in any interleaving
+    # the waves' flushes reach their region's atomic in any interleaving
     order = [w for w in range(nwave) for _ in pending[w]]
     rng.shuffle(order)
     nxt = [0] * nwave
-    for f, w in enumerate(order):
+    for w in order:
         buf = pending[w][nxt[w]]
         nxt[w] += 1
-        base = used
-        used += len(buf)
+        r = w % nreg
+        base = used[r]
+        used[r] += len(buf)
         buf = sorted(buf, key=lambda qv: qv[0] % WAVE)  # stable: by lane, step order kept
         for j, v in enumerate(buf):
-            if base + j < cap:
-                slots[base + j] = v
-                flush_of[base + j] = f
-                writes[base + j] += 1
-    return slots, writes, used, flush_of
+            if base + j < rcap:
+                slots[r * rcap + base + j] = v
+                writes[r * rcap + base + j] += 1
+    return slots, writes, used
+
+
+def held(slots, used, rcap):
+    """The slots the binning and evaluation kernels read: each region's first
+    min(used_r, rcap)."""
+    return [j for r, u in enumerate(used) for j in range(r * rcap, r * rcap + min(u, rcap))]
 
 
 @pytest.mark.parametrize("seed", range(10))
 def test_every_read_slot_was_written_by_this_launch(seed):
     rng = np.random.default_rng(seed)
-    nq = int(rng.integers(1, 500))
+    nq = int(rng.integers(1, 800))
     counts = rng.choice([0, 1, 2, 3, 5, 8, 30, 90, 700], size=nq,
                         p=[.2, .2, .15, .15, .1, .1, .05, .03, .02])
     lists = [[int(q) * 10000 + k for k in range(int(c))] for q, c in enumerate(counts)]
     total = int(counts.sum())
-    for cap in sorted({1, 7, max(1, total // 5), max(1, total // 2), max(1, total - 1), total, total + 10}):
-        slots, writes, used, flush_of = replay(lists, cap, rng)
-        assert used == total
-        n = min(used, cap)
-        assert all(s is not None for s in slots[:n]), (seed, cap)
-        assert (writes[:n] == 1).all(), (seed, cap)
-        if cap >= total:  # with room for everything: each query's pairs are exactly its candidates
-            got = {}
-            for q, v in slots[:n]:
-                got.setdefault(q, []).append(v)
-            for q, lst in enumerate(lists):
-                assert sorted(got.get(q, [])) == lst, (seed, q)
-            # a query's pairs keep its step order
-            for q in range(len(lists)):
-                mine = [v for (qq, v) in slots[:n] if qq == q]
-                assert mine == lists[q], (seed, q)
+    for nreg in (1, 4, 32):
+        for rcap in sorted({1, 7, max(1, total // (5 * nreg)), max(1, total // nreg), total, total + 10}):
+            slots, writes, used = replay(lists, nreg, rcap, rng)
+            assert sum(used) == total
+            read = held(slots, used, rcap)
+            assert all(slots[j] is not None for j in read), (seed, nreg, rcap)
+            assert (writes[read] == 1).all(), (seed, nreg, rcap)
+            assert writes.max(initial=0) <= 1
+            if max(used) <= rcap:  # room for everything: each query's pairs are exactly its candidates
+                assert len(read) == total
+                got = {}
+                for j in read:
+                    q, v = slots[j]
+                    got.setdefault(q, []).append(v)
+                for q, lst in enumerate(lists):
+                    assert got.get(q, []) == lst, (seed, q)  # a query's pairs keep its step order
+            else:  # the host lists again with every region grown to the largest count
+                grown = max(used) + max(used) // 4
+                slots, writes, used = replay(lists, nreg, grown, rng)
+                assert max(used) <= grown and len(held(slots, used, grown)) == total
+
 
 def sequential(cands):
     """The reference's walk over its candidates in preorder (rank order):
