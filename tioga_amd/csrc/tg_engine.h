@@ -406,12 +406,12 @@ struct SearchScratch {
   // a launch recorded into a HIP graph reads the search tables the previous
   // launch of the same sets uploaded (identical), so it copies none itself
   bool skip_uploads = false;
-  DBuf<int> cell, query, status, counters, bins, blist, lj, lcount, qb, qsid, nin, wstart;
+  DBuf<int> cell, query, status, counters, lj, lcount, qb, qsid, nin, wstart;
   DBuf<int> best, last, acc;     // per query outcome words of a launch
   DBuf<int> nseg, segstart;      // the bin listing's work items per walking wave
   DBuf<unsigned char> sv;  // the table of searches of one launch
   DBuf<double4> lst;
-  size_t cap = 0;  // capacity of the flat candidate list
+  size_t tcap[4] = {0, 0, 0, 0};  // capacity of the candidate list per cell type (0: not sized yet)
   int last_ns = 0, last_nqt = 0;  // searches and queries of the last launch
   bool last_bins = false;         // the last launch listed candidates from bin grids
   long long last_nwave = 0;       // waves the last launch was sized for
@@ -685,12 +685,15 @@ class Engine {
   int query_set_extents(int id, double ext[6]) const;
   // candidate-list capacity (entries) the next donor search starts from; it
   // grows whenever a checked search finds it too small
-  size_t search_capacity() const { return sw_.cap; }
+  size_t search_capacity() const { return sw_.tcap[0] + sw_.tcap[1] + sw_.tcap[2] + sw_.tcap[3]; }
   SearchTuning &search_tuning() { return sw_.tune; }
   // the bin density of the search (0: the tree walk); the query sets' bin
   // grids are built again at a new density, trees built later use it
   void set_search_bins(int bins);
-  void set_search_capacity(size_t cap) { sw_.cap = cap; }
+  void set_search_capacity(size_t cap)
+  {
+    for (size_t &c : sw_.tcap) c = std::max<size_t>(cap / 4, 1);
+  }
 
   // AMR Cartesian background, gather direction (tg_amr.hip):
   // tioga::register_amr_global_data / set_amr_patch_count /

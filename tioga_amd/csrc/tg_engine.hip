@@ -1280,61 +1280,129 @@ static_assert(STACK <= WAVE, "k_collect keeps its stack one entry per lane");
 // highest-ranked candidate inside its element box; -1: none), and with a
 // tie-band counter into acc (candidates accepting the point, plus 2^20 per
 // near-zero-weight acceptance).
-// The list is cut into NREG regions of rcap slots, region r holding slots
-// r * rcap .. (r + 1) * rcap - 1 and appended to through its own counter
-// used[r * USED_STRIDE] (a cache line each): workgroup b of a listing launch
-// appends to region b % NREG, so the appends of all waves do not queue on
-// one atomic counter.  A region's slots past its count are unused; entries
-// appended past rcap are dropped, and the host sees a count above rcap,
-// grows the list and lists again before any test runs.
-constexpr int NREG = 32, USED_STRIDE = 32;
-// workgroups of the launches over the list, multiples of NREG
-constexpr unsigned BIN_GRID = 2048, LAST_GRID = 4096, STATS_GRID = 1024;
-static_assert(BIN_GRID % NREG == 0 && LAST_GRID % NREG == 0 && STATS_GRID % NREG == 0, "grids over the list");
+// The list is cut by cell type (bucket t = 0..3: 4, 5, 6, 8 vertices), so
+// each containment-test kernel reads its own type's entries with no binning
+// pass, and within a type into NREG regions of rcap[t] slots: region (t, r)
+// holds slots tbase[t] + r * rcap[t] .. + rcap[t] - 1 and is appended to
+// through its own counter used[(t * NREG + r) * USED_STRIDE] (a cache line
+// each).  Workgroup b of a listing launch appends to regions b % NREG, so
+// the appends of all waves do not queue on one atomic counter.  A region's
+// slots past its count are unused; entries appended past rcap[t] are
+// dropped, and the host sees a count above it, grows that type's regions
+// and lists again before any test runs.
+constexpr int NTYPE = 4, NREG = 32, USED_STRIDE = 32, NRT = NTYPE * NREG;
+// workgroups of the launches over the whole list
+constexpr unsigned LAST_GRID = 4096, STATS_GRID = 1024;
 struct CandOut {
   int *cell, *query, *status;
-  int *used;  // per region: entries appended so far, at used[r * USED_STRIDE]
-  int rcap;   // capacity of each region
+  int *used;  // per region: entries appended so far
+  int rcap[NTYPE], tbase[NTYPE];  // per type: capacity of each region, first slot
   int *best, *last, *acc;  // per global position (acc: null without a tie-band counter)
 };
 
-// slots per region of a list of about cap entries (at most 2^31 - 1 slots)
-inline int cand_region_cap(size_t cap)
+// Candidate tests by cell type (bucket 0..3 = 4, 5, 6, 8 vertices), so each
+// test kernel runs one type's code with that type's register budget; the
+// outcome goes to the candidate's own slot, so the order is free.
+__host__ __device__ __forceinline__ int type_bucket(int nv) { return nv == 4 ? 0 : (nv == 5 ? 1 : (nv == 6 ? 2 : 3)); }
+
+// the regions' capacities and first slots for per-type list sizes (slots)
+struct CandLayout {
+  int rcap[NTYPE], tbase[NTYPE];
+  size_t slots;
+};
+inline CandLayout cand_layout(const size_t tcap[NTYPE])
 {
-  const size_t r = std::max<size_t>((cap + NREG - 1) / NREG, 1);
-  return (int)std::min<size_t>(r, 0x7fffffff / NREG);
+  CandLayout L;
+  size_t at = 0;
+  for (int t = 0; t < NTYPE; t++) {
+    const size_t r = std::max<size_t>((tcap[t] + NREG - 1) / NREG, 1);
+    L.rcap[t] = (int)std::min<size_t>(r, 0x7fffffff / NRT);
+    L.tbase[t] = (int)at;
+    at += (size_t)NREG * L.rcap[t];
+  }
+  L.slots = at;
+  return L;
 }
 
 // entries held by the regions of a list (counts as the device left them)
-inline long long cand_held(const int *used, int rcap)
+inline long long cand_held(const int *used, const CandLayout &L)
 {
   long long n = 0;
-  for (int r = 0; r < NREG; r++) n += std::min(used[r * USED_STRIDE], rcap);
+  for (int t = 0; t < NTYPE; t++)
+    for (int r = 0; r < NREG; r++) n += std::min(used[(t * NREG + r) * USED_STRIDE], L.rcap[t]);
   return n;
 }
 
-// every slot of the list, the entries region r holds (its first slot is
-// r * rcap), and whether slot j holds a listed candidate
-__device__ __forceinline__ int cand_slots(const CandOut &O) { return NREG * O.rcap; }
-__device__ __forceinline__ int region_held(const CandOut &O, int r) { return min(O.used[r * USED_STRIDE], O.rcap); }
+// the entries region (t, r) holds, and its first slot
+__device__ __forceinline__ int region_held(const CandOut &O, int t, int r)
+{
+  return min(O.used[(t * NREG + r) * USED_STRIDE], O.rcap[t]);
+}
+__device__ __forceinline__ int region_first(const CandOut &O, int t, int r) { return O.tbase[t] + r * O.rcap[t]; }
 
-// A launch over the list (gridDim.x a multiple of NREG) gives region
-// b % NREG to workgroup b and the others of its residue: the workgroup's
-// first slot (its first thread's), the end of the region's entries, and
-// the stride of its loop
+// A launch over the regions of types t0 .. t0 + nt - 1 (gridDim.x a multiple
+// of nt * NREG) gives region q = b % (nt * NREG) to workgroup b and the
+// others of its residue: the workgroup's first slot (its first thread's),
+// the end of the region's entries, and the stride of its loop
 struct RegionLoop {
   int first, end, stride;
 };
-__device__ __forceinline__ RegionLoop region_loop(const CandOut &O)
+__device__ __forceinline__ RegionLoop region_loop(const CandOut &O, int t0, int nt)
 {
-  const int r = (int)blockIdx.x % NREG, lb = (int)blockIdx.x / NREG;
-  const int nb = (int)gridDim.x / NREG, bs = (int)blockDim.x;
-  return RegionLoop{r * O.rcap + lb * bs, r * O.rcap + region_held(O, r), nb * bs};
+  const int nr = nt * NREG, q = (int)blockIdx.x % nr, lb = (int)blockIdx.x / nr;
+  const int t = t0 + q / NREG, r = q % NREG;
+  const int nb = (int)gridDim.x / nr, bs = (int)blockDim.x;
+  const int f = region_first(O, t, r);
+  return RegionLoop{f + lb * bs, f + region_held(O, t, r), nb * bs};
 }
-__device__ __forceinline__ bool slot_live(const CandOut &O, int j)
+
+// A launch over every region's held entries, evenly however the types
+// divide them: the entries are cut into chunks of blockDim.x, numbered
+// region after region (q = t * NREG + r), and workgroup b takes chunks b,
+// b + gridDim.x, ...  chunk_prefix fills pre[q] = chunks before region q
+// (pre[NRT]: all) in the workgroup's LDS (blockDim.x >= 64); chunk_slot
+// gives the slot of thread threadIdx.x in chunk c, or -1 past its region's
+// entries.
+__device__ __forceinline__ void chunk_prefix(const CandOut &O, int *pre)
 {
-  const int r = j / O.rcap;
-  return j - r * O.rcap < O.used[r * USED_STRIDE];
+  static_assert(NRT == 2 * WAVE, "two regions per lane of the first wave");
+  if (threadIdx.x < WAVE) {
+    const int l = threadIdx.x, bs = (int)blockDim.x;
+    const int q0 = 2 * l, q1 = 2 * l + 1;
+    const int c0 = (region_held(O, q0 / NREG, q0 % NREG) + bs - 1) / bs;
+    const int c1 = (region_held(O, q1 / NREG, q1 % NREG) + bs - 1) / bs;
+    int inc = c0 + c1;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const int v = __shfl_up(inc, o);
+      if (l >= o) inc += v;
+    }
+    pre[q0] = inc - c0 - c1;
+    pre[q1] = inc - c1;
+    if (l == WAVE - 1) pre[NRT] = inc;
+  }
+  __syncthreads();
+}
+__device__ __forceinline__ int chunk_slot(const CandOut &O, const int *pre, int c)
+{
+  int lo = 0, hi = NRT - 1;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (pre[m] <= c) lo = m;
+    else hi = m - 1;
+  }
+  const int t = lo / NREG, r = lo % NREG;
+  const int k = (c - pre[lo]) * (int)blockDim.x + (int)threadIdx.x;
+  return k < region_held(O, t, r) ? region_first(O, t, r) + k : -1;
+}
+
+// the region (t, r) of slot j, and whether it holds a listed candidate
+__device__ __forceinline__ bool slot_live(const CandOut &O, int j, int &t, int &r)
+{
+  t = 0;
+  while (t + 1 < NTYPE && O.tbase[t + 1] <= j) t++;
+  r = (j - O.tbase[t]) / O.rcap[t];
+  return j - region_first(O, t, r) < O.used[(t * NREG + r) * USED_STRIDE];
 }
 
 // ADT::searchADT / searchIntersections (searchADTrecursion.C:26-101).  The
@@ -1383,11 +1451,9 @@ constexpr long long BIN_ELEMENTS_PER_QUERY = 2;
 constexpr int WBUF = 512;
 constexpr unsigned FIND_GRID = 16384;  // workgroups of k_find (each takes work items in turn)
 struct WaveBuf {
-  int *val, *qid, *key;  // LDS, WBUF entries each: value, position, lane | index << 6
-  int *off;              // LDS, WAVE: each lane's first slot in the sorted order
+  int *val, *qid;  // LDS, WBUF entries each: value (rank | nv << 28), position
   int lane;
-  int n = 0;     // wave-uniform: pairs held
-  int mine = 0;  // this lane's pairs held
+  int n = 0;  // wave-uniform: pairs held
   // each lane adds at most one pair; called in wave-uniform control flow
   __device__ __forceinline__ void emit(bool hit, int v, int g, const CandOut &O)
   {
@@ -1397,39 +1463,52 @@ struct WaveBuf {
       const int k = n + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
       val[k] = v;
       qid[k] = g;
-      key[k] = lane | (mine << 6);
-      mine++;
     }
     n += __popcll(m);
     if (n > WBUF - WAVE) flush(O);
   }
-  // called in wave-uniform control flow; one wave per workgroup
+  // Appends the held pairs to the workgroup's region of each cell type, in
+  // the order held: one atomic per type present.  Called in wave-uniform
+  // control flow; one wave per workgroup.
   __device__ __forceinline__ void flush(const CandOut &O)
   {
     if (n == 0) return;
-    int pre = mine;
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) {
-      const int v = __shfl_up(pre, o);
-      if (lane >= o) pre += v;
-    }
-    off[lane] = pre - mine;
+    __syncthreads();  // the lanes' pair writes before other lanes read them
     const int reg = blockIdx.x & (NREG - 1);
-    int base = 0;
-    if (lane == 0) base = atomicAdd(O.used + reg * USED_STRIDE, n);
-    base = __shfl(base, 0);
-    __syncthreads();  // the lanes' pair writes and offsets before the copy reads them
-    for (int j = lane; j < n; j += WAVE) {
-      const int kj = key[j];
-      const int d = base + off[kj & (WAVE - 1)] + (kj >> 6);
-      if (d < O.rcap) {
-        O.cell[reg * O.rcap + d] = val[j];
-        O.query[reg * O.rcap + d] = qid[j];
+    const unsigned long long lt = (1ull << lane) - 1;
+    int cnt[NTYPE] = {0, 0, 0, 0};
+    for (int j0 = 0; j0 < n; j0 += WAVE) {
+      const int j = j0 + lane;
+      const int t = j < n ? type_bucket((unsigned)val[j] >> 28) : -1;
+#pragma unroll
+      for (int u = 0; u < NTYPE; u++) cnt[u] += __popcll(__ballot(t == u));
+    }
+    int base[NTYPE];
+#pragma unroll
+    for (int u = 0; u < NTYPE; u++) {
+      int b = 0;
+      if (lane == 0 && cnt[u]) b = atomicAdd(O.used + (u * NREG + reg) * USED_STRIDE, cnt[u]);
+      base[u] = __shfl(b, 0);
+    }
+    for (int j0 = 0; j0 < n; j0 += WAVE) {
+      const int j = j0 + lane;
+      const int t = j < n ? type_bucket((unsigned)val[j] >> 28) : -1;
+#pragma unroll
+      for (int u = 0; u < NTYPE; u++) {
+        const unsigned long long m = __ballot(t == u);
+        if (t == u) {
+          const int d = base[u] + __popcll(m & lt);
+          if (d < O.rcap[u]) {
+            const int k = O.tbase[u] + reg * O.rcap[u] + d;
+            O.cell[k] = val[j];
+            O.query[k] = qid[j];
+          }
+        }
+        base[u] += __popcll(m);
       }
     }
     __syncthreads();  // the copy before the buffer is written again
     n = 0;
-    mine = 0;
   }
 };
 
@@ -1443,7 +1522,7 @@ __device__ __forceinline__ int rank_nv(int node, int cellnv) { return node | (ce
 // loads.
 __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O, int skip_binned)
 {
-  __shared__ int bval[WBUF], bqid[WBUF], bkey[WBUF], boff[WAVE];
+  __shared__ int bval[WBUF], bqid[WBUF];
   const int lane = threadIdx.x;
   // workgroup wv walks walking wave wv; the grid is sized for every query,
   // so the workgroups past the walking waves leave at once
@@ -1468,7 +1547,7 @@ __global__ void __launch_bounds__(WAVE, 8) k_collect(SearchSet T, CandOut O, int
     for (int d = 0; d < 3; d++) q[d] = grid_of(p[d], S.g.lo[d], S.g.inv[d]);
   }
   const int gp = qb + t;  // the query's global position (its outcome words' index)
-  WaveBuf buf{bval, bqid, bkey, boff, lane};
+  WaveBuf buf{bval, bqid, lane};
   // pending right subtrees: entry k (node, lane mask) held by lane k, so a
   // push is a select and a pop a readlane; the left child, when any lane
   // enters it, is visited next without the stack (preorder)
@@ -1589,9 +1668,9 @@ __global__ void __launch_bounds__(256) k_find_cost(SearchSet T, int nwave, int s
 __global__ void __launch_bounds__(WAVE, 8) k_find(SearchSet T, CandOut O, int nwave, int seg,
                                                    const int *__restrict__ segstart)
 {
-  __shared__ int bval[WBUF], bqid[WBUF], bkey[WBUF], boff[WAVE];
+  __shared__ int bval[WBUF], bqid[WBUF];
   const int lane = threadIdx.x;
-  WaveBuf buf{bval, bqid, bkey, boff, lane};
+  WaveBuf buf{bval, bqid, lane};
   const int items = segstart[nwave];
   for (int it = blockIdx.x; it < items; it += gridDim.x) {
     // the walking wave of item it (last w with segstart[w] <= it)
@@ -1630,76 +1709,6 @@ __global__ void __launch_bounds__(WAVE, 8) k_find(SearchSet T, CandOut O, int nw
   buf.flush(O);
 }
 
-// Candidate tests binned by cell type (bucket 0..3 = 4, 5, 6, 8 vertices),
-// so each test kernel runs one type's code with that type's register budget;
-// the outcome goes to the candidate's own slot, so the binning order is free.
-__device__ __forceinline__ int type_bucket(int nv) { return nv == 4 ? 0 : (nv == 5 ? 1 : (nv == 6 ? 2 : 3)); }
-
-// per wave: how many of its lanes hold bucket q (lanes past n hold none),
-// and each lane's rank among them; one LDS atomic per bucket and wave
-__device__ __forceinline__ int bin_of(const CandOut &O, int j, int n)
-{
-  if (j >= n) return -1;
-  return type_bucket((unsigned)O.cell[j] >> 28);
-}
-
-__global__ void __launch_bounds__(256) k_bin_count(CandOut O, int *__restrict__ bcount)
-{
-  __shared__ int loc[4];
-  if (threadIdx.x < 4) loc[threadIdx.x] = 0;
-  __syncthreads();
-  const RegionLoop L = region_loop(O);
-  const int lane = threadIdx.x & 63;
-  for (int j0 = L.first + (threadIdx.x & ~63); j0 < L.end; j0 += L.stride) {
-    const int b = bin_of(O, j0 + lane, L.end);
-    for (int q = 0; q < 4; q++) {
-      const unsigned long long m = __ballot(b == q);
-      if (lane == 0 && m) atomicAdd(&loc[q], __popcll(m));
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < 4 && loc[threadIdx.x]) atomicAdd(&bcount[threadIdx.x], loc[threadIdx.x]);
-}
-
-// bucket b's entries go to blist[bstart[b] ...]; bcount[4..7] are the fill cursors
-__global__ void __launch_bounds__(256) k_bin_fill(CandOut O, int *__restrict__ bcount, int *__restrict__ blist)
-{
-  __shared__ int loc[4], base[4];
-  if (threadIdx.x < 4) loc[threadIdx.x] = 0;
-  __syncthreads();
-  const RegionLoop L = region_loop(O);
-  const int lane = threadIdx.x & 63;
-  const unsigned long long lt = (1ull << lane) - 1;
-  const int start0 = L.first + (threadIdx.x & ~63);
-  // pass 1: this block's counts; pass 2: its slice of each bucket
-  for (int j0 = start0; j0 < L.end; j0 += L.stride) {
-    const int b = bin_of(O, j0 + lane, L.end);
-    for (int q = 0; q < 4; q++) {
-      const unsigned long long m = __ballot(b == q);
-      if (lane == 0 && m) atomicAdd(&loc[q], __popcll(m));
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < 4) {
-    int bstart = 0;
-    for (int q = 0; q < (int)threadIdx.x; q++) bstart += bcount[q];
-    base[threadIdx.x] = bstart + (loc[threadIdx.x] ? atomicAdd(&bcount[4 + threadIdx.x], loc[threadIdx.x]) : 0);
-    loc[threadIdx.x] = 0;
-  }
-  __syncthreads();
-  for (int j0 = start0; j0 < L.end; j0 += L.stride) {
-    const int b = bin_of(O, j0 + lane, L.end);
-    for (int q = 0; q < 4; q++) {
-      const unsigned long long m = __ballot(b == q);
-      if (!m) continue;
-      int w = 0;
-      if (lane == 0) w = atomicAdd(&loc[q], __popcll(m));
-      w = __shfl(w, 0);
-      if (b == q) blist[base[q] + w + __popcll(m & lt)] = j0 + lane;
-    }
-  }
-}
-
 // Outcome of a containment test from the weights (checkContainment.C:62-85):
 // 0 reject, 1 accepted with a near-zero weight in a BIGVALUE cell, 2 clean;
 // OUTSIDE: the point fails the element's padded box test, so the reference
@@ -1734,8 +1743,11 @@ __device__ __forceinline__ void record_outcome(const CandOut &O, int j, int st)
 // k_select reads it for)
 __global__ void __launch_bounds__(256) k_last(CandOut O)
 {
-  const RegionLoop L = region_loop(O);
-  for (int j = L.first + threadIdx.x; j < L.end; j += L.stride) {
+  __shared__ int pre[NRT + 1];
+  chunk_prefix(O, pre);
+  for (int c = blockIdx.x; c < pre[NRT]; c += gridDim.x) {
+    const int j = chunk_slot(O, pre, c);
+    if (j < 0) continue;
     const int st = O.status[j];
     if (st == OUTSIDE) continue;
     const int g = O.query[j];
@@ -1770,15 +1782,11 @@ struct LongQueue {
 
 // containment test of every listed candidate of one cell type
 template <int NV, int WAVES>
-__global__ void __launch_bounds__(256, WAVES) k_eval(SearchSet T, CandOut O, const int *__restrict__ bcount,
-                                                     const int *__restrict__ blist, LongQueue L)
+__global__ void __launch_bounds__(256, WAVES) k_eval(SearchSet T, CandOut O, LongQueue L)
 {
   constexpr int B = NV == 4 ? 0 : (NV == 5 ? 1 : (NV == 6 ? 2 : 3));
-  int b0 = 0;
-  for (int q = 0; q < B; q++) b0 += bcount[q];
-  const int n = bcount[B];
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const int j = blist[b0 + k];
+  const RegionLoop R = region_loop(O, B, 1);
+  for (int j = R.first + threadIdx.x; j < R.end; j += R.stride) {
     const int gp = O.query[j], sid = T.ns == 1 ? 0 : T.qsid[gp];
     const SearchView S = view_of(T, sid);
     const int c = cell_of(S, O.cell[j]);
@@ -1832,8 +1840,11 @@ __global__ void __launch_bounds__(256) k_status_hist(CandOut O, int *__restrict_
   __shared__ int loc[4];
   if (threadIdx.x < 4) loc[threadIdx.x] = 0;
   __syncthreads();
-  const RegionLoop L = region_loop(O);
-  for (int j = L.first + threadIdx.x; j < L.end; j += L.stride) {
+  __shared__ int pre[NRT + 1];
+  chunk_prefix(O, pre);
+  for (int c = blockIdx.x; c < pre[NRT]; c += gridDim.x) {
+    const int j = chunk_slot(O, pre, c);
+    if (j < 0) continue;
     const int st = O.status[j];
     if (st >= 0 && st < 4) atomicAdd(&loc[st], 1);
   }
@@ -1845,9 +1856,13 @@ __global__ void __launch_bounds__(256) k_status_hist(CandOut O, int *__restrict_
 // reference's walk stops at that hit and never tests them); a diagnostic
 __global__ void k_after_hit(CandOut O, unsigned long long *__restrict__ out)
 {
-  const RegionLoop L = region_loop(O);
+  __shared__ int pre[NRT + 1];
+  chunk_prefix(O, pre);
   unsigned long long v = 0;
-  for (int j = L.first + threadIdx.x; j < L.end; j += L.stride) v += (O.cell[j] & 0x0fffffff) > O.best[O.query[j]];
+  for (int c = blockIdx.x; c < pre[NRT]; c += gridDim.x) {
+    const int j = chunk_slot(O, pre, c);
+    if (j >= 0) v += (O.cell[j] & 0x0fffffff) > O.best[O.query[j]];
+  }
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
 }
@@ -2695,7 +2710,7 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     TG_HIP(hipMemcpyAsync(w.sv.p, S, sizeof(SearchView) * ns, hipMemcpyHostToDevice, s));
     TG_HIP(hipMemcpyAsync(w.qb.p, qb.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
   }
-  w.counters.reserve(NREG * USED_STRIDE);
+  w.counters.reserve(NRT * USED_STRIDE);
   // per query: lowest clean-hit rank, highest tested (rank, outcome), and the
   // tie-band count when a search has a counter
   bool tieband = false;
@@ -2728,34 +2743,39 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     fill<int>(w.last.p, nqt, -1, s);
     if (tieband) fill<int>(w.acc.p, nqt, 0, s);
   }
-  if (w.cap == 0) w.cap = std::max<size_t>(4 * (size_t)nqt, 1 << 20);
+  if (w.tcap[0] == 0)
+    for (int t = 0; t < NTYPE; t++) w.tcap[t] = std::max<size_t>((size_t)nqt, 1 << 18);
   for (;;) {
-    const int rcap = cand_region_cap(w.cap);
-    const size_t slots = (size_t)NREG * rcap;
+    const CandLayout Y = cand_layout(w.tcap);
+    if (Y.slots > 0x7fffffff) throw std::runtime_error("tioga_amd: candidate list too large");
+    const size_t slots = Y.slots;
     w.cell.reserve(slots);
     w.query.reserve(slots);
     w.status.reserve(slots);
-    CandOut O{w.cell.p, w.query.p, w.status.p, w.counters.p, rcap, w.best.p, w.last.p, tieband ? w.acc.p : nullptr};
-    fill<int>(w.counters.p, NREG * USED_STRIDE, 0, s);
+    CandOut O{w.cell.p, w.query.p, w.status.p, w.counters.p, {}, {}, w.best.p, w.last.p, tieband ? w.acc.p : nullptr};
+    for (int t = 0; t < NTYPE; t++) {
+      O.rcap[t] = Y.rcap[t];
+      O.tbase[t] = Y.tbase[t];
+    }
+    fill<int>(w.counters.p, NRT * USED_STRIDE, 0, s);
     if (bins) k_find<<<FIND_GRID, WAVE, 0, s>>>(T, O, (int)nwave, w.tune.seg, w.segstart.p);
     if (walk) k_collect<<<(unsigned)nwave, WAVE, 0, s>>>(T, O, bins ? 1 : 0);
     if (check) {
-      std::vector<int> used(NREG * USED_STRIDE);
+      std::vector<int> used(NRT * USED_STRIDE);
       TG_HIP(hipMemcpyAsync(used.data(), w.counters.p, sizeof(int) * used.size(), hipMemcpyDeviceToHost, s));
       TG_HIP(hipStreamSynchronize(s));
-      int most = 0;
-      for (int r = 0; r < NREG; r++) most = std::max(most, used[r * USED_STRIDE]);
-      if (most > rcap) {  // grow every region and list again (nothing was tested yet)
-        w.cap = (size_t)NREG * ((size_t)most + most / 4);
-        continue;
+      bool again = false;
+      for (int t = 0; t < NTYPE; t++) {
+        int most = 0;
+        for (int r = 0; r < NREG; r++) most = std::max(most, used[(t * NREG + r) * USED_STRIDE]);
+        if (most > Y.rcap[t]) {  // grow this type's regions and list again (nothing was tested yet)
+          w.tcap[t] = (size_t)NREG * ((size_t)most + most / 4);
+          again = true;
+        }
       }
+      if (again) continue;
     }
     if (list_only) return;
-    w.bins.reserve(8);
-    w.blist.reserve(slots);
-    fill<int>(w.bins.p, 8, 0, s);
-    k_bin_count<<<BIN_GRID, 256, 0, s>>>(O, w.bins.p);
-    k_bin_fill<<<BIN_GRID, 256, 0, s>>>(O, w.bins.p, w.blist.p);
     const int lcap = (int)std::min<size_t>(slots / 4 + 1024, 0x7fffffff / 4);
     w.lj.reserve(4 * (size_t)lcap);
     w.lst.reserve(4 * (size_t)lcap);
@@ -2766,8 +2786,8 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     // threads, each hundreds of dependent iterations, so bound by latency
     // and not by the device) run on the side stream while the other cell
     // types are tested
-    const unsigned eg = (unsigned)w.tune.eval_grid;
-    k_eval<5, 4><<<eg, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    const unsigned eg = (unsigned)((std::max(w.tune.eval_grid, NREG) + NREG - 1) / NREG * NREG);
+    k_eval<5, 4><<<eg, 256, 0, s>>>(T, O, L);
     if (!w.side) {
       TG_HIP(hipStreamCreateWithFlags(&w.side, hipStreamNonBlocking));
       TG_HIP(hipEventCreateWithFlags(&w.fork, hipEventDisableTiming));
@@ -2777,11 +2797,11 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     TG_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
     k_eval_long<5><<<1024, 256, 0, w.side>>>(T, O, L);
     TG_HIP(hipEventRecord(w.join, w.side));
-    k_eval<6, 4><<<eg, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
-    k_eval<8, 4><<<eg, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    k_eval<6, 4><<<eg, 256, 0, s>>>(T, O, L);
+    k_eval<8, 4><<<eg, 256, 0, s>>>(T, O, L);
     k_eval_long<6><<<1024, 256, 0, s>>>(T, O, L);
     k_eval_long<8><<<1024, 256, 0, s>>>(T, O, L);
-    k_eval<4, 8><<<eg, 256, 0, s>>>(T, O, w.bins.p, w.blist.p, L);
+    k_eval<4, 8><<<eg, 256, 0, s>>>(T, O, L);
     TG_HIP(hipStreamWaitEvent(s, w.join, 0));
     k_last<<<LAST_GRID, 256, 0, s>>>(O);
     k_select<<<(unsigned)std::min<long long>((nwave + 3) / 4, 8192), 256, 0, s>>>(T, O);
@@ -2796,10 +2816,10 @@ __global__ void k_export_cands(SearchSet T, CandOut O, int *__restrict__ qi, int
                                int *__restrict__ cell)
 {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= cand_slots(O) || !slot_live(O, j)) return;
-  const int r = j / O.rcap;
-  int k = j - r * O.rcap;
-  for (int q = 0; q < r; q++) k += min(O.used[q * USED_STRIDE], O.rcap);
+  int t, r;
+  if (j >= O.tbase[NTYPE - 1] + NREG * O.rcap[NTYPE - 1] || !slot_live(O, j, t, r)) return;
+  int k = j - region_first(O, t, r);
+  for (int q = 0; q < t * NREG + r; q++) k += region_held(O, q / NREG, q % NREG);
   const SearchView S = view_of(T, 0);
   const int v = O.cell[j];
   qi[k] = S.perm[O.query[j]];
@@ -4422,13 +4442,17 @@ void Engine::search_stats(long long out[10])
   out[9] = w.last_bins ? 1 : 0;
   if (w.last_nqt <= 0 || w.last_ns <= 0) return;
   int items = 0, h[4] = {0, 0, 0, 0};
-  std::vector<int> used(NREG * USED_STRIDE);
+  std::vector<int> used(NRT * USED_STRIDE);
   std::vector<int> nin(w.last_ns);
   DBuf<int> hist;
   hist.reserve(4);
   fill<int>(hist.p, 4, 0, s_);
-  const int rcap = cand_region_cap(w.cap);
-  CandOut O{w.cell.p, w.query.p, w.status.p, w.counters.p, rcap, w.best.p, w.last.p, nullptr};
+  const CandLayout Y = cand_layout(w.tcap);
+  CandOut O{w.cell.p, w.query.p, w.status.p, w.counters.p, {}, {}, w.best.p, w.last.p, nullptr};
+  for (int t = 0; t < NTYPE; t++) {
+    O.rcap[t] = Y.rcap[t];
+    O.tbase[t] = Y.tbase[t];
+  }
   k_status_hist<<<STATS_GRID, 256, 0, s_>>>(O, hist.p);
   DBuf<unsigned long long> ah;
   ah.reserve(1);
@@ -4443,7 +4467,7 @@ void Engine::search_stats(long long out[10])
     TG_HIP(hipMemcpyAsync(&items, w.segstart.p + w.last_nwave, sizeof items, hipMemcpyDeviceToHost, s_));
   }
   TG_HIP(hipStreamSynchronize(s_));
-  out[0] = cand_held(used.data(), rcap);
+  out[0] = cand_held(used.data(), Y);
   long long walking = 0;
   for (int v : nin) walking += v;
   out[1] = w.last_bins ? items : (walking + WAVE - 1) / WAVE;
@@ -4582,19 +4606,23 @@ void Engine::list_candidates(Block &b, std::vector<int> &qi, std::vector<int> &r
   S.x = b.x.p;
   S.cellRes = b.cellRes.p;
   run_search(&S, 1, sw_, s_, true, true);
-  const int rcap = cand_region_cap(sw_.cap);
-  std::vector<int> counts(NREG * USED_STRIDE);
+  const CandLayout Y = cand_layout(sw_.tcap);
+  std::vector<int> counts(NRT * USED_STRIDE);
   TG_HIP(hipMemcpyAsync(counts.data(), sw_.counters.p, sizeof(int) * counts.size(), hipMemcpyDeviceToHost, s_));
   sync();
-  const int used = (int)cand_held(counts.data(), rcap);
+  const int used = (int)cand_held(counts.data(), Y);
   if (used <= 0) return;
   DBuf<int> dq, dr, dc;
   dq.reserve(used);
   dr.reserve(used);
   dc.reserve(used);
   const SearchSet T{(const SearchView *)sw_.sv.p, sw_.qb.p, sw_.nin.p, sw_.wstart.p, sw_.qsid.p, 1, b.nsearch};
-  CandOut O{sw_.cell.p, sw_.query.p, sw_.status.p, sw_.counters.p, rcap, sw_.best.p, sw_.last.p, nullptr};
-  k_export_cands<<<nb((long long)NREG * rcap), BS, 0, s_>>>(T, O, dq.p, dr.p, dc.p);
+  CandOut O{sw_.cell.p, sw_.query.p, sw_.status.p, sw_.counters.p, {}, {}, sw_.best.p, sw_.last.p, nullptr};
+  for (int t = 0; t < NTYPE; t++) {
+    O.rcap[t] = Y.rcap[t];
+    O.tbase[t] = Y.tbase[t];
+  }
+  k_export_cands<<<nb((long long)Y.slots), BS, 0, s_>>>(T, O, dq.p, dr.p, dc.p);
   qi.resize(used);
   rank.resize(used);
   cell.resize(used);
