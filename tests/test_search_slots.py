@@ -3,17 +3,18 @@ GPU).
 
 Listing (k_collect / k_find): each wave (one per workgroup) gathers its
 lanes' (rank | nv << 28, query) pairs in an LDS buffer of WBUF pairs, one
-step adding at most one pair per lane, and appends the buffer to region
-b % NREG of the list (b: its workgroup) with one atomic add of its length to
-that region's counter whenever more than WBUF - 64 pairs are held, and at
-the end; the copy sorts the pairs by lane, keeping each lane's step order.
-Region r holds slots r * rcap .. (r + 1) * rcap - 1; pairs that land at or
-past rcap are dropped, and the host then sees a count above rcap, grows
-every region and lists again before any test runs.  The binning and
-evaluation kernels read region r's slots below min(used_r, rcap).  (The
-model runs each wave as one workgroup, as k_collect does; k_find's work
-items of one wave may run in different workgroups, so there a query's pairs
-may span regions, which the order-free selection below does not mind.)
+step adding at most one pair per lane, and whenever more than WBUF - 64
+pairs are held, and at the end, appends them by cell type: for each type
+present, one atomic add of its count to the counter of region (type,
+b % NREG) (b: its workgroup), then the type's pairs in the order held.
+Region (t, r) holds slots tbase[t] + r * rcap[t] .. + rcap[t] - 1; pairs that
+land at or past rcap[t] are dropped, and the host then sees a count above
+it, grows that type's regions and lists again before any test runs.  The
+evaluation kernels read region (t, r)'s slots below min(used, rcap[t]).
+(The model runs each wave as one workgroup, as k_collect does; k_find's
+work items of one wave may run in different workgroups, so there a query's
+pairs may span regions, which the order-free selection below does not
+mind.)
 
 Selection (k_eval's per-query words, k_select): the reference walks the ADT
 in preorder, tests every candidate (an element whose padded box holds the
@@ -37,18 +38,26 @@ import pytest
 
 WAVE = 64
 WBUF = 512
+NTYPE = 4
 OUTSIDE = 3
+
+
+def ntype(v):
+    """the cell-type bucket of a listed value (the model's stand-in for nv)"""
+    return v % NTYPE
 
 
 def replay(lists, nreg, rcap, rng):
     """lists[q]: the candidate values of query q (queries in wave order; wave
-    w runs as workgroup w).  Returns the slots (None = never written), write
-    counts and each region's count."""
+    w runs as workgroup w); rcap[t]: the region capacity of type t.  Returns
+    the slots (None = never written), write counts, the regions' counts
+    used[t][r] and the types' first slots."""
     nq = len(lists)
     nwave = (nq + WAVE - 1) // WAVE
-    slots = [None] * (nreg * rcap)
-    writes = np.zeros(nreg * rcap, np.int64)
-    used = [0] * nreg
+    tbase = np.concatenate([[0], np.cumsum([nreg * c for c in rcap])]).astype(int)
+    slots = [None] * int(tbase[-1])
+    writes = np.zeros(int(tbase[-1]), np.int64)
+    used = [[0] * nreg for _ in range(NTYPE)]
     # each wave: its steps, then its flushes; waves interleave at flushes
     pending = []
     for w in range(nwave):
@@ -65,7 +74,7 @@ def replay(lists, nreg, rcap, rng):
         if buf:
             flushes.append(buf)
         pending.append(flushes)
-    # the waves' flushes reach their region's atomic in any interleaving
+    # the waves' flushes reach their regions' atomics in any interleaving
     order = [w for w in range(nwave) for _ in pending[w]]
     rng.shuffle(order)
     nxt = [0] * nwave
@@ -73,20 +82,23 @@ def replay(lists, nreg, rcap, rng):
         buf = pending[w][nxt[w]]
         nxt[w] += 1
         r = w % nreg
-        base = used[r]
-        used[r] += len(buf)
-        buf = sorted(buf, key=lambda qv: qv[0] % WAVE)  # stable: by lane, step order kept
-        for j, v in enumerate(buf):
-            if base + j < rcap:
-                slots[r * rcap + base + j] = v
-                writes[r * rcap + base + j] += 1
-    return slots, writes, used
+        for t in range(NTYPE):
+            mine = [qv for qv in buf if ntype(qv[1]) == t]  # in the order held
+            base = used[t][r]
+            used[t][r] += len(mine)
+            for j, v in enumerate(mine):
+                if base + j < rcap[t]:
+                    k = int(tbase[t]) + r * rcap[t] + base + j
+                    slots[k] = v
+                    writes[k] += 1
+    return slots, writes, used, tbase
 
 
-def held(slots, used, rcap):
-    """The slots the binning and evaluation kernels read: each region's first
-    min(used_r, rcap)."""
-    return [j for r, u in enumerate(used) for j in range(r * rcap, r * rcap + min(u, rcap))]
+def held(used, rcap, tbase):
+    """The slots the evaluation kernels read: each region's first
+    min(used, rcap[t])."""
+    return [int(tbase[t]) + r * rcap[t] + k for t in range(NTYPE) for r, u in enumerate(used[t])
+            for k in range(min(u, rcap[t]))]
 
 
 @pytest.mark.parametrize("seed", range(10))
@@ -98,25 +110,30 @@ def test_every_read_slot_was_written_by_this_launch(seed):
     lists = [[int(q) * 10000 + k for k in range(int(c))] for q, c in enumerate(counts)]
     total = int(counts.sum())
     for nreg in (1, 4, 32):
-        for rcap in sorted({1, 7, max(1, total // (5 * nreg)), max(1, total // nreg), total, total + 10}):
-            slots, writes, used = replay(lists, nreg, rcap, rng)
-            assert sum(used) == total
-            read = held(slots, used, rcap)
-            assert all(slots[j] is not None for j in read), (seed, nreg, rcap)
-            assert (writes[read] == 1).all(), (seed, nreg, rcap)
+        for cap in sorted({1, 7, max(1, total // (5 * nreg)), max(1, total // nreg), total + 10}):
+            rcap = [cap, max(1, cap // 2), cap, 2 * cap]  # uneven type capacities
+            slots, writes, used, tbase = replay(lists, nreg, rcap, rng)
+            assert sum(map(sum, used)) == total
+            read = held(used, rcap, tbase)
+            assert all(slots[j] is not None for j in read), (seed, nreg, cap)
+            assert (writes[read] == 1).all(), (seed, nreg, cap)
             assert writes.max(initial=0) <= 1
-            if max(used) <= rcap:  # room for everything: each query's pairs are exactly its candidates
+            fits = all(max(used[t]) <= rcap[t] for t in range(NTYPE))
+            if fits:  # each query's pairs are exactly its candidates, in step order within a type
                 assert len(read) == total
                 got = {}
                 for j in read:
                     q, v = slots[j]
-                    got.setdefault(q, []).append(v)
+                    got.setdefault((q, ntype(v)), []).append(v)
                 for q, lst in enumerate(lists):
-                    assert got.get(q, []) == lst, (seed, q)  # a query's pairs keep its step order
-            else:  # the host lists again with every region grown to the largest count
-                grown = max(used) + max(used) // 4
-                slots, writes, used = replay(lists, nreg, grown, rng)
-                assert max(used) <= grown and len(held(slots, used, grown)) == total
+                    for t in range(NTYPE):
+                        assert got.get((q, t), []) == [v for v in lst if ntype(v) == t], (seed, q)
+            else:  # the host grows each overflowing type to its largest count and lists again
+                grown = [max(used[t]) + max(used[t]) // 4 if max(used[t]) > rcap[t] else rcap[t]
+                         for t in range(NTYPE)]
+                slots, writes, used, tbase = replay(lists, nreg, grown, rng)
+                assert all(max(used[t]) <= grown[t] for t in range(NTYPE))
+                assert len(held(used, grown, tbase)) == total
 
 
 def sequential(cands):
