@@ -1095,15 +1095,7 @@ __global__ void __launch_bounds__(256) k_walk_mask(int ns, int tot, const Morton
                                                     unsigned long long *__restrict__ mask, int *__restrict__ cnt)
 {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  bool walk = false;
-  if (g < tot) {
-    int lo = 0, hi = ns - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (M[mid].off <= g) lo = mid;
-      else hi = mid - 1;
-    }
-    const MortonSet &m = M[lo];
+  auto test = [&](const MortonSet &m) {
     const int i = g - m.off;
     double p[3];
     load_xyz(m.xs, i, p);
@@ -1112,7 +1104,28 @@ __global__ void __launch_bounds__(256) k_walk_mask(int ns, int tot, const Morton
     for (int d = 0; d < 3; d++) inroot = inroot && (p[d] <= m.ext[2 * d + 1] + TOL);
     const bool dup = m.xtag && m.xtag[i] != i;
     if (!inroot && !dup) m.donorId[i] = -1;
-    walk = inroot && !dup;
+    return inroot && !dup;
+  };
+  // the set of the wave's first query by wave-uniform (scalar) loads; a
+  // wave across a set boundary looks each lane's set up
+  const int g0 = __builtin_amdgcn_readfirstlane(g);
+  int s0 = 0;
+  for (int hi = ns - 1; s0 < hi;) {
+    const int mid = (s0 + hi + 1) >> 1;
+    if (M[mid].off <= g0) s0 = mid;
+    else hi = mid - 1;
+  }
+  bool walk = false;
+  if (s0 == ns - 1 || M[s0 + 1].off > g0 + 63) {
+    if (g < tot) walk = test(M[s0]);
+  } else if (g < tot) {
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (M[mid].off <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    walk = test(M[lo]);
   }
   const unsigned long long b = __ballot(walk);
   if ((threadIdx.x & 63) == 0 && g < tot) {
@@ -1138,14 +1151,26 @@ __global__ void __launch_bounds__(256) k_walk_scatter(int ns, int tot, const Mor
                                                        int *__restrict__ qsid)
 {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  // the set of the wave's first query by wave-uniform (scalar) loads; a
+  // wave across a set boundary looks each lane's set up
+  const int g0 = __builtin_amdgcn_readfirstlane(g);
+  int lo = 0;
+  for (int hi = ns - 1; lo < hi;) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (M[mid].off <= g0) lo = mid;
+    else hi = mid - 1;
+  }
   if (g >= tot) return;
   const unsigned long long mw = mask[g >> 6];
   if (!((mw >> (g & 63)) & 1ull)) return;
-  int lo = 0, hi = ns - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (M[mid].off <= g) lo = mid;
-    else hi = mid - 1;
+  if (lo < ns - 1 && M[lo + 1].off <= g0 + 63) {
+    int hi = ns - 1;
+    lo = 0;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (M[mid].off <= g) lo = mid;
+      else hi = mid - 1;
+    }
   }
   const int off = M[lo].off;
   const int t = walkers_before(g, tot, mask, pre) - walkers_before(off, tot, mask, pre);
@@ -1243,6 +1268,17 @@ __device__ __forceinline__ SearchView view_of(const SearchSet &T, int sid)
   S.boff = as_global(v->boff);
   S.bent = as_global(v->bent);
   S.bshift = v->bshift;
+  return S;
+}
+
+// the search table entry of search sid as every lane of the wave needs it:
+// by wave-uniform (scalar) loads when all lanes have the same search (the
+// usual case), else lane by lane
+__device__ __forceinline__ SearchView view_of_lanes(const SearchSet &T, int sid)
+{
+  const int s0 = __builtin_amdgcn_readfirstlane(sid);
+  SearchView S = view_of(T, s0);
+  if (!__all(sid == s0)) S = view_of(T, sid);
   return S;
 }
 
@@ -1449,7 +1485,7 @@ __device__ __forceinline__ bool slot_live(const CandOut &O, int j, int &t, int &
 constexpr long long BIN_ELEMENTS_PER_QUERY = 2;
 
 constexpr int WBUF = 512;
-constexpr unsigned FIND_GRID = 16384;  // workgroups of k_find (each takes work items in turn)
+constexpr unsigned FIND_GRID = 32768;  // workgroups of k_find (each takes work items in turn)
 struct WaveBuf {
   int *val, *qid;  // LDS, WBUF entries each: value (rank | nv << 28), position
   int lane;
@@ -1665,7 +1701,7 @@ __global__ void __launch_bounds__(256) k_find_cost(SearchSet T, int nwave, int s
   if (lane == 0) nseg[w] = len > seg ? (len + seg - 1) / seg : 1;
 }
 
-__global__ void __launch_bounds__(WAVE, 8) k_find(SearchSet T, CandOut O, int nwave, int seg,
+__global__ void __launch_bounds__(WAVE, 16) k_find(SearchSet T, CandOut O, int nwave, int seg,
                                                    const int *__restrict__ segstart)
 {
   __shared__ int bval[WBUF], bqid[WBUF];
@@ -1673,12 +1709,36 @@ __global__ void __launch_bounds__(WAVE, 8) k_find(SearchSet T, CandOut O, int nw
   WaveBuf buf{bval, bqid, lane};
   const int items = segstart[nwave];
   for (int it = blockIdx.x; it < items; it += gridDim.x) {
-    // the walking wave of item it (last w with segstart[w] <= it)
-    int lo = 0, hi = nwave - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
+    // the walking wave of item it (last w with segstart[w] <= it): most
+    // waves are one item, so the search gallops out from the wave an even
+    // spread would give (segstart[lo] <= it < segstart[hi], segstart[nwave]
+    // = items) and bisects the bracket
+    int lo, hi;
+    {
+      const int g = (int)min((long long)nwave - 1, (long long)it * nwave / items);
+      int step = 1;
+      if (segstart[g] <= it) {
+        lo = g;
+        hi = min(nwave, g + 1);
+        while (hi < nwave && segstart[hi] <= it) {
+          lo = hi;
+          step <<= 1;
+          hi = min(nwave, lo + step);
+        }
+      } else {
+        hi = g;
+        lo = max(0, g - 1);
+        while (lo > 0 && segstart[lo] > it) {
+          hi = lo;
+          step <<= 1;
+          lo = max(0, hi - step);
+        }
+      }
+    }
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
       if (segstart[mid] <= it) lo = mid;
-      else hi = mid - 1;
+      else hi = mid;
     }
     const int w = __builtin_amdgcn_readfirstlane(lo);
     const int k0 = (it - segstart[w]) * seg;
@@ -1788,7 +1848,7 @@ __global__ void __launch_bounds__(256, WAVES) k_eval(SearchSet T, CandOut O, Lon
   const RegionLoop R = region_loop(O, B, 1);
   for (int j = R.first + threadIdx.x; j < R.end; j += R.stride) {
     const int gp = O.query[j], sid = T.ns == 1 ? 0 : T.qsid[gp];
-    const SearchView S = view_of(T, sid);
+    const SearchView S = view_of_lanes(T, sid);
     const int c = cell_of(S, O.cell[j]);
     double p[3], xv[8][3], frac[8];
     load_xyz(S.xs, S.perm[gp - T.qbase[sid]], p);
@@ -1877,7 +1937,7 @@ __global__ void __launch_bounds__(256, 4) k_eval_long(SearchSet T, CandOut O, Lo
     const int j = L.j[B * L.cap + k];
     const double4 st = L.st[B * L.cap + k];
     const int gp = O.query[j], sid = T.ns == 1 ? 0 : T.qsid[gp];
-    const SearchView S = view_of(T, sid);
+    const SearchView S = view_of_lanes(T, sid);
     const int c = cell_of(S, O.cell[j]);
     double p[3], xv[8][3], frac[8], f[8][3];
     load_xyz(S.xs, S.perm[gp - T.qbase[sid]], p);
