@@ -1486,6 +1486,9 @@ constexpr long long BIN_ELEMENTS_PER_QUERY = 2;
 
 constexpr int WBUF = 512;
 constexpr unsigned FIND_GRID = 32768;  // workgroups of k_find (each takes work items in turn)
+// entries of its own list segment each lane of k_find scans itself; the
+// rest of a longer segment is scanned by the whole wave
+constexpr int FIND_OWN = 48;
 struct WaveBuf {
   int *val, *qid;  // LDS, WBUF entries each: value (rank | nv << 28), position
   int lane;
@@ -1751,18 +1754,33 @@ __global__ void __launch_bounds__(WAVE, 16) k_find(SearchSet T, CandOut O, int n
     find_lane(S, i, q, start, len);
     const int gp = T.qbase[sid] + t;
     const int end = min(len, k0 + seg);
-    int lmax = end;
+    // each lane scans the first FIND_OWN entries of its own list segment;
+    // the rest of the longer ones, one list at a time, by the whole wave
+    const int mid = min(end, k0 + FIND_OWN);
+    int lmax = mid;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) lmax = max(lmax, __shfl_xor(lmax, o));
     const uint4 *ent = S.bent;
     for (int k = k0; k < lmax; k += 4) {
       uint4 e[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) e[u] = (k + u < end) ? ent[start + k + u] : make_uint4(0xffffu, 0u, 0u, 0u);
+      for (int u = 0; u < 4; u++) e[u] = (k + u < mid) ? ent[start + k + u] : make_uint4(0xffffu, 0u, 0u, 0u);
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const unsigned wb[3] = {e[u].x, e[u].y, e[u].z};
-        buf.emit((k + u < end) && in_grid(q, wb), (int)e[u].w, gp, O);
+        buf.emit((k + u < mid) && in_grid(q, wb), (int)e[u].w, gp, O);
+      }
+    }
+    for (unsigned long long rest = __ballot(end > mid); rest; rest &= rest - 1) {
+      const int L = __builtin_ctzll(rest);
+      const int st = __shfl(start, L), en = __shfl(end, L), g = __shfl(gp, L), a0 = __shfl(mid, L);
+      const unsigned qq[3] = {(unsigned)__shfl((int)q[0], L), (unsigned)__shfl((int)q[1], L),
+                              (unsigned)__shfl((int)q[2], L)};
+      for (int b = a0; b < en; b += WAVE) {
+        const int k = b + lane;
+        const uint4 e = k < en ? ent[st + k] : make_uint4(0xffffu, 0u, 0u, 0u);
+        const unsigned wb[3] = {e.x, e.y, e.z};
+        buf.emit(k < en && in_grid(qq, wb), (int)e.w, g, O);
       }
     }
   }
