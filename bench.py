@@ -586,7 +586,7 @@ def main() -> int:
             "traffic": profiled_traffic("search_pipeline_hex256_mixed_pair.md", "traffic_bytes_per_launch")
             if args.workload == "hex256_mixed_pair" else None,
             "traffic_source": "profiles/search_pipeline_hex256_mixed_pair.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
-            "kernel": "search launch (k_find_cost, k_find / k_collect, k_bin_count, k_bin_fill, k_eval<4,5,6,8>, "
+            "kernel": "search launch (k_find_cost, k_find / k_collect, k_eval<4,5,6,8>, "
                       "k_eval_long<5,6,8>, k_last, k_select)",
             "algorithmic_bytes_per_step": algo, "launches_per_step": 1,
             "avg_launch_ms": round(kern_ms, 4)}
