@@ -1849,7 +1849,7 @@ __device__ __forceinline__ void cell_vertices(const SearchView &S, int c, double
 // cap) are continued in a separate pass so they do not hold up the lanes of
 // the short ones: the first pass stops every solve after NEWTON_SPLIT
 // iterations and queues the unfinished ones with their state.
-constexpr int NEWTON_SPLIT = 32;
+constexpr int NEWTON_SPLIT = 16;
 
 struct LongQueue {
   int *j;        // candidate slot, per type region [bstart, bstart + cap)
@@ -2860,12 +2860,11 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     w.lcount.reserve(4);
     fill<int>(w.lcount.p, 4, 0, s);
     LongQueue L{w.lj.p, w.lst.p, w.lcount.p, lcap};
-    // the pyramids first: their continued Newton solves (a few thousand
-    // threads, each hundreds of dependent iterations, so bound by latency
-    // and not by the device) run on the side stream while the other cell
-    // types are tested
+    // the pyramids on the side stream: their tests (divergent Newton solves,
+    // a quarter of the lanes busy) and their continued solves (a few
+    // thousand threads, each hundreds of dependent iterations, bound by
+    // latency and not by the device) run beside the other cell types' tests
     const unsigned eg = (unsigned)((std::max(w.tune.eval_grid, NREG) + NREG - 1) / NREG * NREG);
-    k_eval<5, 4><<<eg, 256, 0, s>>>(T, O, L);
     if (!w.side) {
       TG_HIP(hipStreamCreateWithFlags(&w.side, hipStreamNonBlocking));
       TG_HIP(hipEventCreateWithFlags(&w.fork, hipEventDisableTiming));
@@ -2873,6 +2872,7 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
     }
     TG_HIP(hipEventRecord(w.fork, s));
     TG_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
+    k_eval<5, 4><<<eg, 256, 0, w.side>>>(T, O, L);
     k_eval_long<5><<<1024, 256, 0, w.side>>>(T, O, L);
     TG_HIP(hipEventRecord(w.join, w.side));
     k_eval<6, 4><<<eg, 256, 0, s>>>(T, O, L);
