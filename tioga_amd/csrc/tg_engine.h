@@ -406,7 +406,8 @@ struct SearchScratch {
   // a launch recorded into a HIP graph reads the search tables the previous
   // launch of the same sets uploaded (identical), so it copies none itself
   bool skip_uploads = false;
-  DBuf<int> cell, query, status, counters, lj, lcount, qb, qsid, nin, wstart;
+  DBuf<int> cell, query, counters, lj, lcount, qb, qsid, nin, wstart;
+  DBuf<unsigned char> status;    // test outcome per candidate slot
   DBuf<int> best, last, acc;     // per query outcome words of a launch
   DBuf<int> nseg, segstart;      // the bin listing's work items per walking wave
   DBuf<unsigned char> sv;  // the table of searches of one launch

@@ -1330,7 +1330,8 @@ constexpr int NTYPE = 4, NREG = 32, USED_STRIDE = 32, NRT = NTYPE * NREG;
 // workgroups of the launches over the whole list
 constexpr unsigned LAST_GRID = 4096, STATS_GRID = 1024;
 struct CandOut {
-  int *cell, *query, *status;
+  int *cell, *query;
+  unsigned char *status;  // test outcome per slot (0-3)
   int *used;  // per region: entries appended so far
   int rcap[NTYPE], tbase[NTYPE];  // per type: capacity of each region, first slot
   int *best, *last, *acc;  // per global position (acc: null without a tie-band counter)
@@ -1809,7 +1810,7 @@ __device__ __forceinline__ int test_status(const double frac[8], bool big)
 // the queries without a clean hit (k_last)
 __device__ __forceinline__ void record_outcome(const CandOut &O, int j, int st)
 {
-  O.status[j] = st;
+  O.status[j] = (unsigned char)st;
   if (st == OUTSIDE || st == 0) return;
   const int g = O.query[j];
   if (st == 2) atomicMin(&O.best[g], O.cell[j] & 0x0fffffff);
