@@ -1307,18 +1307,18 @@ constexpr int WAVE = 64;
 constexpr int STACK = 32;
 static_assert(STACK <= WAVE, "k_collect keeps its stack one entry per lane");
 
-// Candidates of one search launch: a flat list of entries j = (rank | nv << 28,
+// Candidates of one search launch: a list of entries j = (rank | nv << 28,
 // global position of the query, test outcome), appended in any order.  Per
-// query, by its position (Morton order, so neighbouring candidates' words are
-// neighbours), the
+// query, by its position (processing order, so neighbouring candidates'
+// words are neighbours), the
 // tests fold into best (the lowest rank of a clean hit; INT_MAX: none), then,
 // for the queries without one, into last ((rank << 2) | outcome of the
 // highest-ranked candidate inside its element box; -1: none), and with a
 // tie-band counter into acc (candidates accepting the point, plus 2^20 per
 // near-zero-weight acceptance).
 // The list is cut by cell type (bucket t = 0..3: 4, 5, 6, 8 vertices), so
-// each containment-test kernel reads its own type's entries with no binning
-// pass, and within a type into NREG regions of rcap[t] slots: region (t, r)
+// each containment-test kernel reads its own type's entries with no
+// separate pass that sorts them by type, and within a type into NREG regions of rcap[t] slots: region (t, r)
 // holds slots tbase[t] + r * rcap[t] .. + rcap[t] - 1 and is appended to
 // through its own counter used[(t * NREG + r) * USED_STRIDE] (a cache line
 // each).  Workgroup b of a listing launch appends to regions b % NREG, so
