@@ -1316,16 +1316,16 @@ static_assert(STACK <= WAVE, "k_collect keeps its stack one entry per lane");
 // highest-ranked candidate inside its element box; -1: none), and with a
 // tie-band counter into acc (candidates accepting the point, plus 2^20 per
 // near-zero-weight acceptance).
-// The list is cut by cell type (bucket t = 0..3: 4, 5, 6, 8 vertices), so
-// each containment-test kernel reads its own type's entries with no
-// separate pass that sorts them by type, and within a type into NREG regions of rcap[t] slots: region (t, r)
-// holds slots tbase[t] + r * rcap[t] .. + rcap[t] - 1 and is appended to
-// through its own counter used[(t * NREG + r) * USED_STRIDE] (a cache line
-// each).  Workgroup b of a listing launch appends to regions b % NREG, so
-// the appends of all waves do not queue on one atomic counter.  A region's
-// slots past its count are unused; entries appended past rcap[t] are
-// dropped, and the host sees a count above it, grows that type's regions
-// and lists again before any test runs.
+// The list is cut by cell type (bucket t = 0..3: 4, 5, 6, 8 vertices), so each
+// containment-test kernel reads its own type's entries with no separate pass
+// that sorts them by type, and within a type into NREG regions of rcap[t]
+// slots: region (t, r) holds slots tbase[t] + r * rcap[t] .. + rcap[t] - 1 and
+// is appended to through its own counter used[(t * NREG + r) * USED_STRIDE] (a
+// cache line each).  Workgroup b of a listing launch appends to regions b %
+// NREG, so the appends of all waves do not queue on one atomic counter.  A
+// region's slots past its count are unused; entries appended past rcap[t] are
+// dropped, and the host sees a count above it, grows that type's regions and
+// lists again before any test runs.
 constexpr int NTYPE = 4, NREG = 32, USED_STRIDE = 32, NRT = NTYPE * NREG;
 // workgroups of the launches over the whole list
 constexpr unsigned LAST_GRID = 4096, STATS_GRID = 1024;
