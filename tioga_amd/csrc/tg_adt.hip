@@ -42,7 +42,7 @@ constexpr int PER = 8;         // consecutive positions per thread per chunk
 constexpr int CH = BT * PER;   // positions per chunk of a multi-workgroup round
 constexpr int WAVEMAX = 2048;  // segments at most this long (and above SUB): one wave
 constexpr int BIGMIN = 1 << 14;  // segments longer than this: many workgroups
-constexpr int FINISH = 16384;  // active range a single workgroup finishes a long segment's rounds from
+constexpr int FINISH = 2048;  // active range a single workgroup finishes a long segment's rounds from
 
 // base: offset of the segment's tree in the concatenated element arrays;
 // start: the segment's first position within its tree
