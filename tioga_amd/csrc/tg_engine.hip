@@ -1489,7 +1489,7 @@ constexpr int WBUF = 512;
 constexpr unsigned FIND_GRID = 32768;  // workgroups of k_find (each takes work items in turn)
 // entries of its own list segment each lane of k_find scans itself; the
 // rest of a longer segment is scanned by the whole wave
-constexpr int FIND_OWN = 48;
+constexpr int FIND_OWN = 24;
 struct WaveBuf {
   int *val, *qid;  // LDS, WBUF entries each: value (rank | nv << 28), position
   int lane;
