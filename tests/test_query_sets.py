@@ -195,12 +195,14 @@ def test_query_sets_candidate_list_regrows():
 @pytest.mark.parametrize("tuning", [dict(morton_bits=0, seg=256, eval_grid=1024, bins=0),
                                     dict(morton_bits=30, seg=1024, eval_grid=8192, bins=1),
                                     dict(morton_bits=9, seg=64, eval_grid=64, bins=64),
-                                    dict(morton_bits=21, seg=16, eval_grid=4096, bins=4096)])
+                                    dict(morton_bits=21, seg=16, eval_grid=4096, bins=4096),
+                                    dict(morton_bits=0, seg=1024, eval_grid=8192, bins=4096)])
 def test_query_sets_any_tuning(tuning):
     """The search's parameters (candidates from bin grids of any density or
     from the tree walk, bin lists cut into work items of any length, Morton
     bits, evaluation grid) change no donor; very coarse bins cut into short
-    items make many items per wave."""
+    items make many items per wave, and cut into long ones leave most of
+    every list to the wave-wide scan of the longer lists."""
     print(f"\n[query sets] tuning {tuning}", flush=True)
     blocks = _blocks("mid")
     o = _oracle(blocks)
