@@ -1490,6 +1490,8 @@ constexpr unsigned FIND_GRID = 32768;  // workgroups of k_find (each takes work 
 // entries of its own list segment each lane of k_find scans itself; the
 // rest of a longer segment is scanned by the whole wave
 constexpr int FIND_OWN = 24;
+// workgroups of k_find_cost (each wave takes walking waves in turn)
+constexpr long long FIND_COST_GRID = 2048;
 struct WaveBuf {
   int *val, *qid;  // LDS, WBUF entries each: value (rank | nv << 28), position
   int lane;
@@ -1677,32 +1679,29 @@ __global__ void __launch_bounds__(256) k_find_cost(SearchSet T, int nwave, int s
                                                    int *__restrict__ best, int *__restrict__ last,
                                                    int *__restrict__ acc)
 {
-  const int w = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
-  if (w >= nwave) return;
-  const int nwalk = T.wstart[T.ns];
-  if (w >= nwalk) {
-    if (lane == 0) nseg[w] = 0;
-    return;
+  // nseg is cleared beforehand, so only the walking waves are visited, each
+  // wave of the grid taking them in turn
+  const int lane = threadIdx.x & (WAVE - 1), nwalk = min(T.wstart[T.ns], nwave);
+  const int wstep = (int)(gridDim.x * (blockDim.x / WAVE));
+  for (int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE)); w < nwalk;
+       w += wstep) {
+    const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, w));
+    const SearchView S = view_of(T, sid);
+    const int t = (w - T.wstart[sid]) * WAVE + lane;
+    if (t < T.nin[sid]) {
+      const int gp = T.qbase[sid] + t;
+      best[gp] = 0x7fffffff;
+      last[gp] = -1;
+      if (acc) acc[gp] = 0;
+    }
+    if (!S.boff) continue;  // a tree without bins: its waves walk (k_collect)
+    const int i = t < T.nin[sid] ? S.perm[t] : -1;
+    unsigned q[3] = {0u, 0u, 0u};
+    int start, len;
+    find_lane(S, i, q, start, len);
+    for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o));
+    if (lane == 0) nseg[w] = len > seg ? (len + seg - 1) / seg : 1;
   }
-  const int sid = __builtin_amdgcn_readfirstlane(wave_search(T, w));
-  const SearchView S = view_of(T, sid);
-  const int t = (w - T.wstart[sid]) * WAVE + lane;
-  if (t < T.nin[sid]) {
-    const int gp = T.qbase[sid] + t;
-    best[gp] = 0x7fffffff;
-    last[gp] = -1;
-    if (acc) acc[gp] = 0;
-  }
-  if (!S.boff) {  // a tree without bins: its waves walk (k_collect)
-    if (lane == 0) nseg[w] = 0;
-    return;
-  }
-  const int i = t < T.nin[sid] ? S.perm[t] : -1;
-  unsigned q[3] = {0u, 0u, 0u};
-  int start, len;
-  find_lane(S, i, q, start, len);
-  for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o));
-  if (lane == 0) nseg[w] = len > seg ? (len + seg - 1) / seg : 1;
 }
 
 __global__ void __launch_bounds__(WAVE, 16) k_find(SearchSet T, CandOut O, int nwave, int seg,
@@ -2813,9 +2812,9 @@ void run_search(const SearchView *S, int ns, SearchScratch &w, hipStream_t s, bo
   if (bins) {  // the listing's work items: segments of the walking waves' bin lists
     w.nseg.reserve((size_t)nwave + 1);
     w.segstart.reserve((size_t)nwave + 1);
-    fill<int>(w.nseg.p + nwave, 1, 0, s);
-    k_find_cost<<<nb(nwave * WAVE), BS, 0, s>>>(T, (int)nwave, w.tune.seg, w.nseg.p, w.best.p, w.last.p,
-                                                tieband ? w.acc.p : nullptr);
+    fill<int>(w.nseg.p, (size_t)nwave + 1, 0, s);
+    k_find_cost<<<(unsigned)std::min<long long>(nb(nwave * WAVE), FIND_COST_GRID), BS, 0, s>>>(
+        T, (int)nwave, w.tune.seg, w.nseg.p, w.best.p, w.last.p, tieband ? w.acc.p : nullptr);
     exclusive_sum(w.nseg.p, w.segstart.p, (int)nwave + 1, s);
   } else {
     fill<int>(w.best.p, nqt, 0x7fffffff, s);
