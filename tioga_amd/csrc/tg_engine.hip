@@ -1150,32 +1150,37 @@ __global__ void __launch_bounds__(256) k_walk_scatter(int ns, int tot, const Mor
                                                        const int *__restrict__ pre, int *__restrict__ idx,
                                                        int *__restrict__ qsid)
 {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  // the set of the wave's first query by wave-uniform (scalar) loads; a
-  // wave across a set boundary looks each lane's set up
-  const int g0 = __builtin_amdgcn_readfirstlane(g);
-  int lo = 0;
-  for (int hi = ns - 1; lo < hi;) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (M[mid].off <= g0) lo = mid;
-    else hi = mid - 1;
-  }
-  if (g >= tot) return;
-  const unsigned long long mw = mask[g >> 6];
-  if (!((mw >> (g & 63)) & 1ull)) return;
-  if (lo < ns - 1 && M[lo + 1].off <= g0 + 63) {
-    int hi = ns - 1;
-    lo = 0;
-    while (lo < hi) {
+  // each wave of the grid takes chunks of 64 queries in turn
+  const int lane = threadIdx.x & 63, nw = (tot + 63) >> 6;
+  const int wstep = (int)(gridDim.x * (blockDim.x >> 6));
+  for (int c = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)); c < nw;
+       c += wstep) {
+    const unsigned long long mw = mask[c];
+    if (!mw) continue;
+    const int g0 = c << 6, g = g0 + lane;
+    // the set of the chunk's first query by wave-uniform (scalar) loads; a
+    // chunk across a set boundary looks each lane's set up
+    int lo = 0;
+    for (int hi = ns - 1; lo < hi;) {
       const int mid = (lo + hi + 1) >> 1;
-      if (M[mid].off <= g) lo = mid;
+      if (M[mid].off <= g0) lo = mid;
       else hi = mid - 1;
     }
+    if (lo < ns - 1 && M[lo + 1].off <= g0 + 63) {
+      int hi = ns - 1;
+      lo = 0;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (M[mid].off <= g) lo = mid;
+        else hi = mid - 1;
+      }
+    }
+    if (g >= tot || !((mw >> lane) & 1ull)) continue;
+    const int off = M[lo].off;
+    const int t = walkers_before(g, tot, mask, pre) - walkers_before(off, tot, mask, pre);
+    idx[off + t] = g - off;
+    if (ns > 1) qsid[off + t] = lo;
   }
-  const int off = M[lo].off;
-  const int t = walkers_before(g, tot, mask, pre) - walkers_before(off, tot, mask, pre);
-  idx[off + t] = g - off;
-  if (ns > 1) qsid[off + t] = lo;
 }
 
 // nin and wstart as k_walk_counts forms them, from the compaction's counts
@@ -4408,8 +4413,8 @@ const int *Engine::order_sets(const std::vector<MortonSet> &M, size_t tot)
     fill<int>(wcnt_.p + nw, 1, 0, s_);
     k_walk_mask<<<nb((long long)tot), BS, 0, s_>>>(ns, (int)tot, (const MortonSet *)mset_.p, wmask_.p, wcnt_.p);
     exclusive_sum(wcnt_.p, wpre_.p, (int)nw + 1, s_);
-    k_walk_scatter<<<nb((long long)tot), BS, 0, s_>>>(ns, (int)tot, (const MortonSet *)mset_.p, wmask_.p,
-                                                      wpre_.p, mv1_.p, sw_.qsid.p);
+    k_walk_scatter<<<(unsigned)std::min<long long>(nb((long long)tot), 4096), BS, 0, s_>>>(
+        ns, (int)tot, (const MortonSet *)mset_.p, wmask_.p, wpre_.p, mv1_.p, sw_.qsid.p);
     k_walk_counts_mask<<<1, 256, 0, s_>>>(ns, (int)tot, (const MortonSet *)mset_.p, wmask_.p, wpre_.p,
                                           sw_.nin.p, sw_.wstart.p);
     return mv1_.p;
